@@ -1,0 +1,155 @@
+/*
+ * plenvdb_hip.h -- C ABI of libplenvdb_hip.so, the MI355X (gfx950) library
+ * behind the PlenVDB sparse-VDB radiance-field hot path.
+ *
+ * Conventions
+ *  - Every pointer argument marked "dev" is a device pointer on the current
+ *    HIP device; "host" pointers are read synchronously.  `stream` is a
+ *    hipStream_t (NULL = the legacy default stream).  Calls are asynchronous
+ *    on that stream unless stated otherwise.
+ *  - Every function returns 0 on success or a negative code:
+ *    -1 invalid argument, -2 HIP error, -3 bad handle, -4 out of memory,
+ *    -5 size out of range.  pvdb_last_error() returns the calling thread's
+ *    last message.
+ *  - Payloads are slot-major: value c of slot s is at vals[s*C + c], where a
+ *    slot is leaf*512 + ((x&7)<<6 | (y&7)<<3 | (z&7)) and leaves are numbered
+ *    in NanoVDB depth-first order (the reference's `n`, densityvdb.cu:41-42).
+ *    A 12-channel colour payload keeps channel c of the reference's Vec3f
+ *    grid c/3, component c%3 (colorvdb.cu:59,96-108).
+ *
+ * Reference interfaces cited below are paths in wolfball/PlenVDB.
+ */
+#ifndef PLENVDB_HIP_H
+#define PLENVDB_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PVDB_ABI_VERSION 1
+
+typedef struct pvdb_topology pvdb_topology;
+
+/* Thread-local message of the last failing call (plenvdb/lib/vdb/plenvdb.cu:3-17
+ * prints CUDA errors; this library returns them). */
+const char* pvdb_last_error(void);
+int pvdb_abi_version(void);
+
+/* ---- topology ------------------------------------------------------------
+ * Replaces ScaleGrid/VectorGrid::create (plenvdb/lib/vdb/plenvdb.h:117-125,
+ * 197-210: OpenVDB denseFill -> openToNanoVDB -> deviceUpload).  `mask` is a
+ * host dense [rx][ry][rz] array of active voxels; a leaf exists for every 8^3
+ * block holding at least one active voxel (an all-true mask gives denseFill's
+ * topology).  Synchronous; allocates the device leaf table. */
+int pvdb_topo_create_from_mask(const uint8_t* mask /*host*/, int32_t rx, int32_t ry, int32_t rz,
+                               pvdb_topology** out);
+/* Leaf list (origins multiples of 8, 8 value-mask words per leaf); the
+ * library sorts it into NanoVDB order (openvdb/nanovdb/nanovdb/util/
+ * OpenToNanoVDB.h:509-533).  Synchronous. */
+int pvdb_topo_create_from_leaves(const int32_t* origins /*host [n][3]*/,
+                                 const uint64_t* masks /*host [n][8]*/, int64_t nleaf,
+                                 pvdb_topology** out);
+int pvdb_topo_destroy(pvdb_topology* topo);
+/* tree().nodeCount(0) and active voxel count (plenvdb.h:123: nleafCount). */
+int pvdb_topo_info(const pvdb_topology* topo, int64_t* nleaf, int64_t* nactive,
+                   int32_t* leaf_lo3, int32_t* leaf_dim3);
+/* Leaf origins and value masks in NanoVDB order (tree().getFirstNode<0>()+i). */
+int pvdb_topo_leaves(const pvdb_topology* topo, int32_t* origins /*host*/,
+                     uint64_t* masks /*host*/);
+/* Host slot query: slot of each voxel (or -1 when its leaf does not exist) and
+ * its active flag (LeafNode::CoordToOffset, NanoVDB.h:3893-3900;
+ * ReadAccessor::isActive, NanoVDB.h:4660-4674). */
+int pvdb_topo_slots(const pvdb_topology* topo, const int32_t* ijk /*host [n][3]*/, int64_t n,
+                    int64_t* slots /*host*/, uint8_t* active /*host, may be NULL*/);
+int pvdb_topo_device_arrays(const pvdb_topology* topo, const int32_t** table,
+                            const uint64_t** masks, const int32_t** origins);
+
+/* ---- grid payload kernels ------------------------------------------------
+ * Trilinear forward.  order 0: density rule of density_forward_kernel
+ * (plenvdb/lib/vdb/densityvdb.cu:101-125, replaces DensityVDB::forward
+ * plenvdb.h:441-466 / plenvdb.cpp:11-19); order 1: colour rule of
+ * color_forward_kernel (colorvdb.cu:81-111, ColorVDB::forward plenvdb.h:535-556).
+ * xs/ys/zs are index-space points (grid.py:77-85); out is [n][channels]. */
+int pvdb_trilinear_fwd(const pvdb_topology* topo, const float* vals /*dev*/, int32_t channels,
+                       int32_t order, const float* xs /*dev*/, const float* ys /*dev*/,
+                       const float* zs /*dev*/, int64_t n, float* out /*dev*/, void* stream);
+/* Atomic scatter with the "leaf must exist" rule: density_backward_kernel
+ * (densityvdb.cu:143-181, DensityVDB::backward plenvdb.h:468-485) and
+ * color_backward_kernel (colorvdb.cu:130-175, ColorVDB::backward plenvdb.h:557-573). */
+int pvdb_trilinear_bwd(const pvdb_topology* topo, float* grad /*dev*/, int32_t channels,
+                       int32_t order, const float* xs, const float* ys, const float* zs,
+                       const float* g /*dev [n][channels]*/, int64_t n, void* stream);
+/* copyFromDense (densityvdb.cu:31-61, colorvdb.cu:42-77; plenvdb.cpp:37-39,72-76). */
+int pvdb_copy_from_dense(const pvdb_topology* topo, float* vals, int32_t channels,
+                         const float* dense /*dev [rx][ry][rz][C]*/, int32_t rx, int32_t ry,
+                         int32_t rz, void* stream);
+/* get_dense_grid (plenvdb.h:149-160,242-262; plenvdb.cpp:33). */
+int pvdb_copy_to_dense(const pvdb_topology* topo, const float* vals, int32_t channels,
+                       float* dense /*dev*/, int32_t rx, int32_t ry, int32_t rz, float background,
+                       void* stream);
+/* setValuesOn_bymask (densityvdb.cu:64-97; plenvdb.cpp:7-9). */
+int pvdb_set_values_by_mask(const pvdb_topology* topo, float* vals, const uint8_t* mask /*dev*/,
+                            float value, int32_t rx, int32_t ry, int32_t rz, void* stream);
+/* zero_grad (densityvdb.cu:353-368, colorvdb.cu:355-371; plenvdb.cpp:94,120). */
+int pvdb_zero_active(const pvdb_topology* topo, float* grad, int32_t channels, void* stream);
+/* DensityOpt/ColorOpt::step_optimizer kernels (plenvdb.h:751-789;
+ * densityvdb.cu:185-349, colorvdb.cu:179-351).  group = values per VDB value
+ * (1 density, 3 colour Vec3f); mode 0 plain, 1 skip-zero-grad, 2 per-voxel lr. */
+int pvdb_vdb_adam(const pvdb_topology* topo, float* param, const float* grad, float* exp_avg,
+                  float* exp_avg_sq, const float* per_lr /*dev [slots] or NULL*/,
+                  int32_t channels, int32_t group, int32_t mode, float stepsz, float eps,
+                  float beta0, float beta1, void* stream);
+/* adam_upd / masked_adam_upd / adam_upd_with_perlr
+ * (plenvdb/lib/cuda/adam_upd_kernel.cu:8-132, adam_upd.cpp:79-86). */
+int pvdb_dense_adam(float* param, const float* grad, float* exp_avg, float* exp_avg_sq,
+                    const float* per_lr, int64_t n, int32_t mode, float step_size, float eps,
+                    float beta0, float beta1, void* stream);
+
+/* ---- ray operations (plenvdb/lib/cuda/render_utils_kernel.cu) ------------ */
+/* infer_t_minmax + infer_n_samples (:11-55).  far_per_ray may be NULL. */
+int pvdb_ray_bounds(const float* rays_o, const float* rays_d, int64_t n,
+                    const float* xyz_min /*host[3]*/, const float* xyz_max /*host[3]*/,
+                    float near_, float far_, const float* far_per_ray, float stepdist,
+                    float* t_min, float* t_max, int64_t* n_steps, void* stream);
+/* sample_pts_on_rays (:144-242) given the inclusive prefix sum of n_steps. */
+int pvdb_sample_pts(const float* rays_o, const float* rays_d, const float* t_min,
+                    const int64_t* steps_incl, int64_t n_rays, int64_t total, float stepdist,
+                    const float* xyz_min /*host*/, const float* xyz_max /*host*/, float* pts,
+                    int64_t* ray_id, int64_t* step_id, uint8_t* inbbox, void* stream);
+/* maskcache_lookup (:373-424). */
+int pvdb_mask_lookup(const uint8_t* mask, int32_t sx, int32_t sy, int32_t sz, const float* pts,
+                     int64_t m, const float* scale /*host*/, const float* shift /*host*/,
+                     uint8_t* out, void* stream);
+/* maskcache_lookup's rule evaluated on the VDB value masks (MaskGrid of the
+ * topology's active voxels, plenvdb/lib/grid.py:207-243). */
+int pvdb_active_lookup(const pvdb_topology* topo, const float* pts, int64_t m,
+                       const float* scale /*host*/, const float* shift /*host*/, uint8_t* out,
+                       void* stream);
+/* VDBGrid.wld2idx (plenvdb/lib/grid.py:77-78), SoA output. */
+int pvdb_wld2idx(const float* pts, int64_t m, const float* xyz_min /*host*/,
+                 const float* xyz_max /*host*/, const int32_t* reso /*host*/, float* xs,
+                 float* ys, float* zs, void* stream);
+/* raw2alpha / raw2alpha_backward (:430-443, :506-517). */
+int pvdb_raw2alpha(const float* sigma, int64_t n, float shift, float interval, float* exp_out,
+                   float* alpha_out, void* stream);
+int pvdb_raw2alpha_bwd(const float* exp_d, const float* grad_alpha, int64_t n, float interval,
+                       float* grad_sigma, void* stream);
+/* __set_i_for_segment_start_end (:625-634); i_start/i_end must be zeroed. */
+int pvdb_segment_bounds(const int64_t* ray_id, int64_t n, int64_t* i_start, int64_t* i_end,
+                        void* stream);
+/* alpha2weight / alpha2weight_backward (:576-707). */
+int pvdb_alpha2weight(const float* alpha, int64_t n_rays, float* weight, float* T,
+                      float* alphainv_last, const int64_t* i_start, int64_t* i_end,
+                      void* stream);
+int pvdb_alpha2weight_bwd(const float* alpha, const float* weight, const float* T,
+                          const float* alphainv_last, const int64_t* i_start,
+                          const int64_t* i_end, int64_t n_rays, const float* grad_weights,
+                          const float* grad_last, float* grad_alpha, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PLENVDB_HIP_H */

@@ -1,0 +1,339 @@
+/*
+ * CPU restatement of the PlenVDB sparse-VDB radiance-field hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (plenvdb_amd/) may call,
+ * link or load this code; only tests/, the smoke check and bench.py's CPU
+ * baseline use it, and only as the checker.
+ *
+ * This header is included twice by pvdb_oracle.c: once with REAL=float
+ * (SFX=_f32, the "reference-order" FP32 restatement that follows the
+ * reference's operation order and its float/double promotions) and once with
+ * REAL=double (SFX=_f64, the FP64 oracle).  Every function cites the
+ * reference code it restates (paths relative to the reference repository).
+ *
+ * Grid representation (independent of the product's dense leaf table): the
+ * payload is slot-major (vals[slot*C + c], slot = leaf*512 + in-leaf offset)
+ * and a leaf is found by binary search over its packed origin key (keys[]
+ * sorted ascending, kleaf[] the leaf index of each key).  A voxel in a leaf
+ * that does not exist reads the background value 0 (NanoVDB
+ * ReadAccessor::getValue on a tree without tiles,
+ * openvdb/nanovdb/nanovdb/NanoVDB.h:4626-4640).
+ */
+
+#define CAT2(a, b) a##b
+#define CAT(a, b) CAT2(a, b)
+#define FN(name) CAT(name, SFX)
+
+/* Corner walk order of the reference kernels: 000,001,011,010,110,111,101,100
+ * (plenvdb/lib/vdb/densityvdb.cu:115-122, colorvdb.cu:96-103). */
+static const int FN(k_corner)[8][3] = {
+    {0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0}, {1, 1, 0}, {1, 1, 1}, {1, 0, 1}, {1, 0, 0}};
+
+/* Packed key of the leaf holding voxel (i,j,k): 21 bits per axis. */
+static inline int64_t FN(leaf_key)(int i, int j, int k) {
+    const int64_t a = (int64_t)(i >> 3) + (1 << 20), b = (int64_t)(j >> 3) + (1 << 20),
+                  c = (int64_t)(k >> 3) + (1 << 20);
+    return (a << 42) | (b << 21) | c;
+}
+
+/* Slot of voxel (i,j,k), or -1 when its leaf does not exist. */
+static inline long FN(locate)(const int64_t* keys, const int64_t* kleaf, long nleaf, int i, int j,
+                              int k) {
+    const int64_t key = FN(leaf_key)(i, j, k);
+    long lo = 0, hi = nleaf;
+    while (lo < hi) {
+        long mid = (lo + hi) >> 1;
+        if (keys[mid] < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    if (lo >= nleaf || keys[lo] != key) return -1;
+    return (long)kleaf[lo] * 512 + (((i & 7) << 6) | ((j & 7) << 3) | (k & 7));
+}
+
+/* Trilinear forward.
+ * order 0 = density rule: out = 0; out += ((v*(1-u))*(1-v))*(1-w) per corner
+ *           (plenvdb/lib/vdb/densityvdb.cu:101-125).
+ * order 1 = color rule: s = ((1-u)*(1-v))*(1-w); out[c] += v[c]*s
+ *           (plenvdb/lib/vdb/colorvdb.cu:16-26,81-111).
+ * ijk = floor(x) in float, uvw = x - ijk in float (densityvdb.cu:112-114;
+ * Floor is floorf, NanoVDB.h:711-714).  Output row-major [N][C]. */
+void FN(pvdb_trilinear_fwd)(const REAL* vals, const int64_t* keys, const int64_t* kleaf,
+                            long nleaf, int C, int order, const float* xs,
+                            const float* ys, const float* zs, long N, REAL* out) {
+#pragma omp parallel for schedule(static)
+    for (long n = 0; n < N; ++n) {
+        const float x = xs[n], y = ys[n], z = zs[n];
+        const int i0 = (int)floorf(x), j0 = (int)floorf(y), k0 = (int)floorf(z);
+        const float uf = x - (float)i0, vf = y - (float)j0, wf = z - (float)k0;
+        const REAL u = (REAL)uf, v = (REAL)vf, w = (REAL)wf;
+        REAL* o = out + n * C;
+        for (int c = 0; c < C; ++c) o[c] = 0;
+        for (int q = 0; q < 8; ++q) {
+            const int* cn = FN(k_corner)[q];
+            const REAL a = cn[0] ? u : (REAL)1 - u;
+            const REAL b = cn[1] ? v : (REAL)1 - v;
+            const REAL d = cn[2] ? w : (REAL)1 - w;
+            const long idx = FN(locate)(keys, kleaf, nleaf, i0 + cn[0], j0 + cn[1], k0 + cn[2]);
+            for (int c = 0; c < C; ++c) {
+                const REAL val = idx >= 0 ? vals[idx * C + c] : (REAL)0;
+                if (order == 0)
+                    o[c] = o[c] + ((val * a) * b) * d;
+                else
+                    o[c] = o[c] + val * ((a * b) * d);
+            }
+        }
+    }
+}
+
+/* Trilinear backward: grad[corner] += g * weight for every corner whose leaf
+ * exists (the "leaf must exist" rule: accumulate only when the accessor has the
+ * leaf cached after getValue, densityvdb.cu:16-27; colorvdb.cu:28-37).  Voxels
+ * of existing leaves receive the add whether or not they are active.
+ * order 0: term = ((g*(1-u))*(1-v))*(1-w) (densityvdb.cu:157-164)
+ * order 1: term = g * (((1-u)*(1-v))*(1-w)) (colorvdb.cu:145-152). */
+void FN(pvdb_trilinear_bwd)(REAL* grad, const int64_t* keys, const int64_t* kleaf,
+                            long nleaf, int C, int order, const float* xs,
+                            const float* ys, const float* zs, const REAL* g, long N) {
+#pragma omp parallel for schedule(static)
+    for (long n = 0; n < N; ++n) {
+        const float x = xs[n], y = ys[n], z = zs[n];
+        const int i0 = (int)floorf(x), j0 = (int)floorf(y), k0 = (int)floorf(z);
+        const float uf = x - (float)i0, vf = y - (float)j0, wf = z - (float)k0;
+        const REAL u = (REAL)uf, v = (REAL)vf, w = (REAL)wf;
+        const REAL* gn = g + n * C;
+        for (int q = 0; q < 8; ++q) {
+            const int* cn = FN(k_corner)[q];
+            const REAL a = cn[0] ? u : (REAL)1 - u;
+            const REAL b = cn[1] ? v : (REAL)1 - v;
+            const REAL d = cn[2] ? w : (REAL)1 - w;
+            const long idx = FN(locate)(keys, kleaf, nleaf, i0 + cn[0], j0 + cn[1], k0 + cn[2]);
+            if (idx < 0) continue;
+            for (int c = 0; c < C; ++c) {
+                const REAL t = order == 0 ? ((gn[c] * a) * b) * d : gn[c] * ((a * b) * d);
+#pragma omp atomic
+                grad[idx * C + c] += t;
+            }
+        }
+    }
+}
+
+/* Ray/bbox slab test and sample count (plenvdb/lib/cuda/render_utils_kernel.cu:11-55).
+ * d==0 is replaced by 1e-6; t clamped to [near, far]; n_steps =
+ * max(ceil((tmax-tmin)*|d|/stepdist), 1).  The caller passes far=1e9 to follow
+ * plenvdb/lib/dvgo.py:284; far_per_ray (may be NULL) overrides far per ray. */
+void FN(pvdb_ray_bounds)(const float* rays_o, const float* rays_d, long N, const float* xyz_min,
+                         const float* xyz_max, float near_, float far_,
+                         const float* far_per_ray, float stepdist, REAL* t_min, REAL* t_max,
+                         int64_t* n_steps) {
+#pragma omp parallel for schedule(static)
+    for (long r = 0; r < N; ++r) {
+        const float* o = rays_o + 3 * r;
+        const float* d = rays_d + 3 * r;
+        REAL vx = d[0] == 0 ? (REAL)1e-6f : (REAL)d[0];
+        REAL vy = d[1] == 0 ? (REAL)1e-6f : (REAL)d[1];
+        REAL vz = d[2] == 0 ? (REAL)1e-6f : (REAL)d[2];
+        REAL ax = ((REAL)xyz_max[0] - (REAL)o[0]) / vx;
+        REAL ay = ((REAL)xyz_max[1] - (REAL)o[1]) / vy;
+        REAL az = ((REAL)xyz_max[2] - (REAL)o[2]) / vz;
+        REAL bx = ((REAL)xyz_min[0] - (REAL)o[0]) / vx;
+        REAL by = ((REAL)xyz_min[1] - (REAL)o[1]) / vy;
+        REAL bz = ((REAL)xyz_min[2] - (REAL)o[2]) / vz;
+        REAL lo = fmax(fmax(fmin(ax, bx), fmin(ay, by)), fmin(az, bz));
+        REAL hi = fmin(fmin(fmax(ax, bx), fmax(ay, by)), fmax(az, bz));
+        const REAL fr = (REAL)(far_per_ray ? far_per_ray[r] : far_);
+        REAL tmn = fmax(fmin(lo, fr), (REAL)near_);
+        REAL tmx = fmax(fmin(hi, fr), (REAL)near_);
+        t_min[r] = tmn;
+        t_max[r] = tmx;
+        REAL rn = sqrt((REAL)d[0] * (REAL)d[0] + (REAL)d[1] * (REAL)d[1] + (REAL)d[2] * (REAL)d[2]);
+        REAL q = ceil(((tmx - tmn) * rn) / (REAL)stepdist);
+        n_steps[r] = (int64_t)(q > 1 ? q : 1);
+    }
+}
+
+/* Sample points on rays (render_utils_kernel.cu:57-79,166-194):
+ * start = o + d*tmin, dir = d/|d|, p_i = start + dir*(stepdist*i);
+ * inbbox = !(min>p | p>max) per axis.  offs[r] is the exclusive prefix sum of
+ * n_steps; outputs have length offs[N-1]+n_steps[N-1]. */
+void FN(pvdb_sample_pts)(const float* rays_o, const float* rays_d, const REAL* t_min,
+                         const int64_t* n_steps, const int64_t* offs, long N, float stepdist,
+                         const float* xyz_min, const float* xyz_max, REAL* pts, int64_t* ray_id,
+                         int64_t* step_id, uint8_t* inbbox) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long r = 0; r < N; ++r) {
+        const float* o = rays_o + 3 * r;
+        const float* d = rays_d + 3 * r;
+        REAL rn = sqrt((REAL)d[0] * (REAL)d[0] + (REAL)d[1] * (REAL)d[1] + (REAL)d[2] * (REAL)d[2]);
+        REAL s[3], dir[3];
+        for (int a = 0; a < 3; ++a) {
+            s[a] = (REAL)o[a] + (REAL)d[a] * t_min[r];
+            dir[a] = (REAL)d[a] / rn;
+        }
+        for (int64_t i = 0; i < n_steps[r]; ++i) {
+            const int64_t m = offs[r] + i;
+            const REAL dist = (REAL)stepdist * (REAL)i;
+            REAL p[3];
+            int out = 0;
+            for (int a = 0; a < 3; ++a) {
+                p[a] = s[a] + dir[a] * dist;
+                pts[3 * m + a] = p[a];
+                out |= ((REAL)xyz_min[a] > p[a]) | ((REAL)xyz_max[a] < p[a]);
+            }
+            ray_id[m] = r;
+            step_id[m] = i;
+            inbbox[m] = (uint8_t)!out;
+        }
+    }
+}
+
+/* World -> index space, wld2idx (plenvdb/lib/grid.py:77-78):
+ * idx = (p - min) / (max - min) * (R - 1), one IEEE op at a time. */
+void FN(pvdb_wld2idx)(const REAL* pts, long M, const float* xyz_min, const float* xyz_max,
+                      const int* reso, float* xs, float* ys, float* zs) {
+    float* outs[3] = {xs, ys, zs};
+#pragma omp parallel for schedule(static)
+    for (long m = 0; m < M; ++m)
+        for (int a = 0; a < 3; ++a) {
+            REAL t = (pts[3 * m + a] - (REAL)xyz_min[a]) / ((REAL)xyz_max[a] - (REAL)xyz_min[a]);
+            outs[a][m] = (float)(t * (REAL)(reso[a] - 1));
+        }
+}
+
+/* Dense occupancy lookup (render_utils_kernel.cu:373-390): i = round(p*s + b)
+ * with round half away from zero; a point outside the grid reads false. */
+void FN(pvdb_mask_lookup)(const uint8_t* mask, const int* shape, const REAL* pts, long M,
+                          const float* scale, const float* shift, uint8_t* out) {
+#pragma omp parallel for schedule(static)
+    for (long m = 0; m < M; ++m) {
+        int ii[3];
+        for (int a = 0; a < 3; ++a) {
+            REAL t = pts[3 * m + a] * (REAL)scale[a];
+            t = t + (REAL)shift[a];
+            ii[a] = (int)round(t);
+        }
+        int ok = 1;
+        for (int a = 0; a < 3; ++a) ok &= (ii[a] >= 0) & (ii[a] < shape[a]);
+        out[m] = ok ? mask[((long)ii[0] * shape[1] + ii[1]) * shape[2] + ii[2]] : 0;
+    }
+}
+
+/* raw2alpha (render_utils_kernel.cu:430-443): e = exp(s + shift),
+ * alpha = 1 - (1 + e)^(-interval). */
+void FN(pvdb_raw2alpha)(const REAL* sigma, long N, float shift, float interval, REAL* e_out,
+                        REAL* a_out) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+        REAL e = (REAL)exp(sigma[i] + (REAL)shift);
+        e_out[i] = e;
+        a_out[i] = (REAL)1 - (REAL)pow((REAL)1 + e, -(REAL)interval);
+    }
+}
+
+/* raw2alpha backward (render_utils_kernel.cu:506-517):
+ * g_s = min(e, 1e10) * (1+e)^(-interval-1) * interval * g_a; the reference
+ * promotes the product to double (min(float, double) is double). */
+void FN(pvdb_raw2alpha_bwd)(const REAL* e, const REAL* g_a, long N, float interval, REAL* g_s) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+        double m = (double)e[i] < 1e10 ? (double)e[i] : 1e10;
+        REAL p = (REAL)pow((REAL)1 + e[i], -(REAL)interval - (REAL)1);
+        g_s[i] = (REAL)(m * (double)p * (double)interval * (double)g_a[i]);
+    }
+}
+
+/* Alphas2Weights forward (render_utils_kernel.cu:576-636): per ray, in sample
+ * order, T_i = prod_{j<i}(1-a_j), w_i = T_i a_i; stop after the first sample
+ * with T < 1e-3 (i_end is truncated there).  The reference keeps T_cum in
+ * float and multiplies by the double (1. - a).  Samples past the stop keep
+ * w = 0, T = 1 (the caller initialises w=0, T=1).  seg_start/seg_end give each
+ * ray's [start, end) range in the sample list; seg_end is updated. */
+void FN(pvdb_alpha2weight)(const REAL* alpha, long n_rays, const int64_t* seg_start,
+                           int64_t* seg_end, REAL* w, REAL* T, REAL* t_last) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long r = 0; r < n_rays; ++r) {
+        REAL Tc = 1;
+        int64_t i;
+        for (i = seg_start[r]; i < seg_end[r]; ++i) {
+            T[i] = Tc;
+            w[i] = Tc * alpha[i];
+            Tc = (REAL)((double)Tc * (1.0 - (double)alpha[i]));
+            if ((double)Tc < 1e-3) {
+                i += 1;
+                break;
+            }
+        }
+        seg_end[r] = i;
+        t_last[r] = Tc;
+    }
+}
+
+/* Alphas2Weights backward (render_utils_kernel.cu:653-676):
+ * back = g_last*T_last; for i = end-1 .. start:
+ *   g_a[i] = g_w[i]*T[i] - back/(1 - a[i] + 1e-10); back += g_w[i]*w[i]. */
+void FN(pvdb_alpha2weight_bwd)(const REAL* alpha, const REAL* w, const REAL* T,
+                               const REAL* t_last, const int64_t* seg_start,
+                               const int64_t* seg_end, long n_rays, const REAL* g_w,
+                               const REAL* g_last, REAL* g_a) {
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long r = 0; r < n_rays; ++r) {
+        REAL back = g_last[r] * t_last[r];
+        for (int64_t i = seg_end[r] - 1; i >= seg_start[r]; --i) {
+            double den = (double)((REAL)1 - alpha[i]) + 1e-10;
+            g_a[i] = (REAL)((double)(g_w[i] * T[i]) - (double)back / den);
+            back += g_w[i] * w[i];
+        }
+    }
+}
+
+/* Adam on VDB slots (plenvdb/lib/vdb/densityvdb.cu:185-349,
+ * colorvdb.cu:179-351).  Slots are leaf*512+offset; only active slots are
+ * touched.  group = number of consecutive values that form one grid value
+ * (1 for density, 3 for a Vec3f colour grid): mode 1 skips a slot only when
+ * all values of its group have zero gradient (colorvdb.cu:318).
+ * mode 0 plain, 1 skip-zero-grad, 2 per-voxel lr (per_lr has one value per
+ * slot).  m = b0 m + (1-b0) g; v = b1 v + ((1-b1) g) g;
+ * p -= ([perlr*] stepsz * m) / (eps + sqrt(v)). */
+void FN(pvdb_vdb_adam)(REAL* param, const REAL* grad, REAL* m, REAL* v, const REAL* per_lr,
+                       const uint8_t* active, long nslots, int group, int mode, float stepsz,
+                       float eps, float b0, float b1) {
+#pragma omp parallel for schedule(static)
+    for (long s = 0; s < nslots; ++s) {
+        if (!active[s]) continue;
+        if (mode == 1) {
+            int all0 = 1;
+            for (int c = 0; c < group; ++c) all0 &= grad[s * group + c] == 0;
+            if (all0) continue;
+        }
+        for (int c = 0; c < group; ++c) {
+            const long q = s * group + c;
+            const REAL g = grad[q];
+            const REAL nm = (REAL)b0 * m[q] + ((REAL)1 - (REAL)b0) * g;
+            const REAL nv = (REAL)b1 * v[q] + (((REAL)1 - (REAL)b1) * g) * g;
+            m[q] = nm;
+            v[q] = nv;
+            REAL sz = (REAL)stepsz;
+            if (mode == 2) sz = per_lr[s] * sz;
+            param[q] = param[q] - (sz * nm) / ((REAL)eps + sqrt(nv));
+        }
+    }
+}
+
+/* Dense Adam for the MLP parameters (plenvdb/lib/cuda/adam_upd_kernel.cu:8-58).
+ * mode 0 plain, 1 skip zero grad (masked_adam_upd), 2 per-element lr. */
+void FN(pvdb_dense_adam)(REAL* param, const REAL* grad, REAL* m, REAL* v, const REAL* per_lr,
+                         long n, int mode, float step_size, float eps, float b0, float b1) {
+#pragma omp parallel for schedule(static)
+    for (long q = 0; q < n; ++q) {
+        const REAL g = grad[q];
+        if (mode == 1 && g == 0) continue;
+        m[q] = (REAL)b0 * m[q] + ((REAL)1 - (REAL)b0) * g;
+        v[q] = (REAL)b1 * v[q] + (((REAL)1 - (REAL)b1) * g) * g;
+        REAL sz = (REAL)step_size;
+        if (mode == 2) sz = sz * per_lr[q];
+        param[q] = param[q] - (sz * m[q]) / (sqrt(v[q]) + (REAL)eps);
+    }
+}
+
+#undef FN

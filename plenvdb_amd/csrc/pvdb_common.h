@@ -1,0 +1,108 @@
+// Shared definitions of the PlenVDB MI355X library: the flattened VDB topology
+// as the kernels see it, error plumbing of the C-ABI and small device helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+// ---------------------------------------------------------------------------
+// Flattened topology.  The reference walks a NanoVDB root/upper/lower/leaf
+// tree through a per-thread ReadAccessor (openvdb/nanovdb/nanovdb/NanoVDB.h:
+// 4514-4680).  Here the tree is flattened once on the host into
+//   * a dense leaf table over the bounding box of the leaf grid (one int32 per
+//     8^3 block: the leaf's index in NanoVDB depth-first order, or -1), which
+//     replaces the root scan plus the upper/lower child lookups by one load
+//     that stays L2-resident (40^3 * 4 B = 256 KB at 320^3);
+//   * per-leaf 512-bit value masks (NanoVDB LeafNode::mValueMask);
+//   * per-leaf origins.
+// Leaf indices and the in-leaf offset (x&7)<<6 | (y&7)<<3 | (z&7) equal the
+// reference's (NanoVDB.h:3893-3900), so a payload slot leaf*512+offset is the
+// reference's `n` in densityvdb.cu:41-42.
+// ---------------------------------------------------------------------------
+namespace pvdb {
+
+struct TopoView {
+    const int32_t* table;    // [d0*d1*d2] leaf index or -1
+    const uint64_t* masks;   // [nleaf*8] value masks, bit (offset & 63) of word offset>>6
+    const int32_t* origins;  // [nleaf*3]
+    const int32_t* lower_count;  // [c0*c1*c2] number of leaves per 16^3-leaf block (128^3 voxels)
+    int32_t lo0, lo1, lo2;   // leaf-grid origin, in leaf units (voxel >> 3)
+    int32_t d0, d1, d2;      // leaf-grid extent
+    int32_t c0, c1, c2;      // extent of the lower-node grid (leaf-grid >> 4, relative to lo)
+    int32_t pad_;
+    int64_t nleaf;
+};
+
+__device__ __forceinline__ int voxel_offset(int i, int j, int k) {
+    return ((i & 7) << 6) | ((j & 7) << 3) | (k & 7);
+}
+
+// Leaf index of voxel (i,j,k) or -1 when no leaf holds it.
+__device__ __forceinline__ int32_t leaf_of(const TopoView& t, int i, int j, int k) {
+    const int a = (i >> 3) - t.lo0, b = (j >> 3) - t.lo1, c = (k >> 3) - t.lo2;
+    if ((unsigned)a >= (unsigned)t.d0 || (unsigned)b >= (unsigned)t.d1 ||
+        (unsigned)c >= (unsigned)t.d2)
+        return -1;
+    return t.table[(a * t.d1 + b) * t.d2 + c];
+}
+
+// Slot (leaf*512 + offset) of voxel (i,j,k) or -1 when its leaf does not exist.
+__device__ __forceinline__ int64_t slot_of(const TopoView& t, int i, int j, int k) {
+    const int32_t l = leaf_of(t, i, j, k);
+    return l < 0 ? int64_t(-1) : (int64_t)l * 512 + voxel_offset(i, j, k);
+}
+
+// NanoVDB isActive for a tree without active tiles: the value-mask bit of an
+// existing leaf, false otherwise.
+__device__ __forceinline__ bool voxel_active(const TopoView& t, int i, int j, int k) {
+    const int32_t l = leaf_of(t, i, j, k);
+    if (l < 0) return false;
+    const int off = voxel_offset(i, j, k);
+    return (t.masks[(int64_t)l * 8 + (off >> 6)] >> (off & 63)) & 1ull;
+}
+
+}  // namespace pvdb
+
+// ---------------------------------------------------------------------------
+// C-ABI error plumbing
+// ---------------------------------------------------------------------------
+enum {
+    PVDB_OK = 0,
+    PVDB_EINVAL = -1,
+    PVDB_EHIP = -2,
+    PVDB_EHANDLE = -3,
+    PVDB_ENOMEM = -4,
+    PVDB_ERANGE = -5,
+};
+
+namespace pvdb {
+void set_error(const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+}  // namespace pvdb
+
+#define PVDB_CHECK_ARG(cond, msg)          \
+    do {                                    \
+        if (!(cond)) {                      \
+            pvdb::set_error(msg);           \
+            return PVDB_EINVAL;             \
+        }                                   \
+    } while (0)
+
+#define PVDB_HIP(call)                                         \
+    do {                                                       \
+        hipError_t e_ = (call);                                \
+        if (e_ != hipSuccess) return pvdb::hip_fail(e_, #call); \
+    } while (0)
+
+#define PVDB_LAUNCHED(name)                                            \
+    do {                                                               \
+        hipError_t e_ = hipGetLastError();                             \
+        if (e_ != hipSuccess) return pvdb::hip_fail(e_, "launch " name); \
+    } while (0)
+
+static inline unsigned pvdb_blocks(int64_t n, int threads) {
+    int64_t b = (n + threads - 1) / threads;
+    return (unsigned)(b < 1 ? 1 : b);
+}

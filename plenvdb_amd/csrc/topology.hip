@@ -1,0 +1,313 @@
+// Host-side VDB topology builder and its C-ABI.
+//
+// Replaces the reference's OpenVDB denseFill -> openToNanoVDB -> deviceUpload
+// pipeline (plenvdb/lib/vdb/plenvdb.h:117-125,197-210) by a direct build of
+// the flattened device layout described in pvdb_common.h.  Leaves are ordered
+// exactly as NanoVDB orders them (depth first: upper nodes in lexicographic
+// Coord order of their origins, then lower nodes by InternalNode::CoordToOffset,
+// then leaves by CoordToOffset; openvdb/nanovdb/nanovdb/util/GridBuilder.h:362-376,
+// util/OpenToNanoVDB.h:509-533, NanoVDB.h:932-935,3272-3283).
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <vector>
+
+#include "pvdb_common.h"
+#include "plenvdb_hip.h"
+
+namespace pvdb {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int hip_fail(hipError_t e, const char* what) {
+    g_last_error = std::string(what) + ": " + hipGetErrorString(e);
+    return PVDB_EHIP;
+}
+
+}  // namespace pvdb
+
+struct pvdb_topology {
+    int64_t nleaf = 0;
+    int64_t nactive = 0;
+    int32_t lo[3] = {0, 0, 0};
+    int32_t dim[3] = {0, 0, 0};
+    int32_t cdim[3] = {0, 0, 0};
+    std::vector<int32_t> origins;   // host copies
+    std::vector<uint64_t> masks;
+    std::vector<int32_t> table;
+    int32_t* d_table = nullptr;
+    uint64_t* d_masks = nullptr;
+    int32_t* d_origins = nullptr;
+    int32_t* d_lower = nullptr;
+    int device = 0;
+};
+
+namespace {
+
+struct LeafRec {
+    int32_t o[3];
+    uint64_t m[8];
+};
+
+// NanoVDB depth-first order of leaf origins.
+bool nano_less(const LeafRec& a, const LeafRec& b) {
+    int32_t ua[3], ub[3];
+    for (int d = 0; d < 3; ++d) {
+        ua[d] = a.o[d] & ~4095;
+        ub[d] = b.o[d] & ~4095;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (ua[d] != ub[d]) return ua[d] < ub[d];  // Coord::operator< on upper origins
+    auto lower_off = [](const int32_t* o) {
+        return (((o[0] & 4095) >> 7) << 10) | (((o[1] & 4095) >> 7) << 5) | ((o[2] & 4095) >> 7);
+    };
+    auto leaf_off = [](const int32_t* o) {
+        return (((o[0] & 127) >> 3) << 8) | (((o[1] & 127) >> 3) << 4) | ((o[2] & 127) >> 3);
+    };
+    const int la = lower_off(a.o), lb = lower_off(b.o);
+    if (la != lb) return la < lb;
+    return leaf_off(a.o) < leaf_off(b.o);
+}
+
+int64_t floor_div8(int64_t v) { return v >= 0 ? v / 8 : -((-v + 7) / 8); }
+
+int finalize(std::vector<LeafRec>& leaves, pvdb_topology** out) {
+    std::sort(leaves.begin(), leaves.end(), nano_less);
+    for (size_t i = 1; i < leaves.size(); ++i)
+        if (leaves[i].o[0] == leaves[i - 1].o[0] && leaves[i].o[1] == leaves[i - 1].o[1] &&
+            leaves[i].o[2] == leaves[i - 1].o[2]) {
+            pvdb::set_error("duplicate leaf origin");
+            return PVDB_EINVAL;
+        }
+    pvdb_topology* t = new (std::nothrow) pvdb_topology();
+    if (!t) return PVDB_ENOMEM;
+    t->nleaf = (int64_t)leaves.size();
+    int32_t lmin[3] = {0, 0, 0}, lmax[3] = {-1, -1, -1};
+    if (!leaves.empty()) {
+        for (int d = 0; d < 3; ++d) lmin[d] = lmax[d] = leaves[0].o[d] >> 3;
+        for (auto& L : leaves)
+            for (int d = 0; d < 3; ++d) {
+                lmin[d] = std::min(lmin[d], L.o[d] >> 3);
+                lmax[d] = std::max(lmax[d], L.o[d] >> 3);
+            }
+    }
+    int64_t cells = 1;
+    for (int d = 0; d < 3; ++d) {
+        t->lo[d] = lmin[d];
+        t->dim[d] = lmax[d] - lmin[d] + 1;
+        t->cdim[d] = (t->dim[d] + 15) / 16;
+        cells *= t->dim[d];
+    }
+    if (cells > (int64_t(1) << 28)) {
+        delete t;
+        pvdb::set_error("leaf bounding box too large for a dense leaf table (> 2^28 cells)");
+        return PVDB_ERANGE;
+    }
+    t->table.assign((size_t)std::max<int64_t>(cells, 1), -1);
+    t->origins.resize(3 * leaves.size());
+    t->masks.resize(8 * leaves.size());
+    std::vector<int32_t> lower((size_t)t->cdim[0] * t->cdim[1] * t->cdim[2], 0);
+    int64_t nact = 0;
+    for (size_t i = 0; i < leaves.size(); ++i) {
+        const LeafRec& L = leaves[i];
+        int32_t a = (L.o[0] >> 3) - t->lo[0], b = (L.o[1] >> 3) - t->lo[1], c = (L.o[2] >> 3) - t->lo[2];
+        t->table[((size_t)a * t->dim[1] + b) * t->dim[2] + c] = (int32_t)i;
+        lower[((size_t)(a >> 4) * t->cdim[1] + (b >> 4)) * t->cdim[2] + (c >> 4)] += 1;
+        for (int d = 0; d < 3; ++d) t->origins[3 * i + d] = L.o[d];
+        for (int w = 0; w < 8; ++w) {
+            t->masks[8 * i + w] = L.m[w];
+            nact += __builtin_popcountll(L.m[w]);
+        }
+    }
+    t->nactive = nact;
+    hipError_t e = hipGetDevice(&t->device);
+    size_t tb = t->table.size() * sizeof(int32_t);
+    if (e == hipSuccess) e = hipMalloc(&t->d_table, tb);
+    if (e == hipSuccess) e = hipMalloc(&t->d_masks, std::max<size_t>(t->masks.size(), 8) * 8);
+    if (e == hipSuccess) e = hipMalloc(&t->d_origins, std::max<size_t>(t->origins.size(), 3) * 4);
+    if (e == hipSuccess) e = hipMalloc(&t->d_lower, std::max<size_t>(lower.size(), 1) * 4);
+    if (e == hipSuccess) e = hipMemcpy(t->d_table, t->table.data(), tb, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !t->masks.empty())
+        e = hipMemcpy(t->d_masks, t->masks.data(), t->masks.size() * 8, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !t->origins.empty())
+        e = hipMemcpy(t->d_origins, t->origins.data(), t->origins.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !lower.empty())
+        e = hipMemcpy(t->d_lower, lower.data(), lower.size() * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        int rc = pvdb::hip_fail(e, "topology upload");
+        pvdb_topo_destroy(t);
+        return rc;
+    }
+    *out = t;
+    return PVDB_OK;
+}
+
+}  // namespace
+
+namespace pvdb {
+TopoView view_of(const pvdb_topology* t) {
+    TopoView v;
+    v.table = t->d_table;
+    v.masks = t->d_masks;
+    v.origins = t->d_origins;
+    v.lower_count = t->d_lower;
+    v.lo0 = t->lo[0];
+    v.lo1 = t->lo[1];
+    v.lo2 = t->lo[2];
+    v.d0 = t->dim[0];
+    v.d1 = t->dim[1];
+    v.d2 = t->dim[2];
+    v.c0 = t->cdim[0];
+    v.c1 = t->cdim[1];
+    v.c2 = t->cdim[2];
+    v.pad_ = 0;
+    v.nleaf = t->nleaf;
+    return v;
+}
+}  // namespace pvdb
+
+extern "C" {
+
+const char* pvdb_last_error(void) { return pvdb::g_last_error.c_str(); }
+
+int pvdb_abi_version(void) { return PVDB_ABI_VERSION; }
+
+int pvdb_topo_create_from_mask(const uint8_t* mask, int32_t rx, int32_t ry, int32_t rz,
+                               pvdb_topology** out) {
+    PVDB_CHECK_ARG(out != nullptr, "out is null");
+    PVDB_CHECK_ARG(rx > 0 && ry > 0 && rz > 0, "resolution must be positive");
+    PVDB_CHECK_ARG(mask != nullptr, "mask is null");
+    const int lx = (rx + 7) / 8, ly = (ry + 7) / 8, lz = (rz + 7) / 8;
+    std::vector<LeafRec> leaves;
+    for (int a = 0; a < lx; ++a)
+        for (int b = 0; b < ly; ++b)
+            for (int c = 0; c < lz; ++c) {
+                LeafRec L;
+                std::memset(&L, 0, sizeof(L));
+                L.o[0] = 8 * a;
+                L.o[1] = 8 * b;
+                L.o[2] = 8 * c;
+                bool any = false;
+                for (int i = 0; i < 8; ++i) {
+                    const int x = 8 * a + i;
+                    if (x >= rx) break;
+                    for (int j = 0; j < 8; ++j) {
+                        const int y = 8 * b + j;
+                        if (y >= ry) break;
+                        const uint8_t* row = mask + ((size_t)x * ry + y) * rz;
+                        for (int k = 0; k < 8; ++k) {
+                            const int z = 8 * c + k;
+                            if (z >= rz) break;
+                            if (row[z]) {
+                                const int off = (i << 6) | (j << 3) | k;
+                                L.m[off >> 6] |= 1ull << (off & 63);
+                                any = true;
+                            }
+                        }
+                    }
+                }
+                if (any) leaves.push_back(L);
+            }
+    return finalize(leaves, out);
+}
+
+int pvdb_topo_create_from_leaves(const int32_t* origins, const uint64_t* masks, int64_t nleaf,
+                                 pvdb_topology** out) {
+    PVDB_CHECK_ARG(out != nullptr, "out is null");
+    PVDB_CHECK_ARG(nleaf >= 0, "nleaf must be >= 0");
+    PVDB_CHECK_ARG(nleaf == 0 || (origins && masks), "null leaf arrays");
+    std::vector<LeafRec> leaves((size_t)nleaf);
+    for (int64_t i = 0; i < nleaf; ++i) {
+        for (int d = 0; d < 3; ++d) {
+            PVDB_CHECK_ARG((origins[3 * i + d] & 7) == 0, "leaf origins must be multiples of 8");
+            leaves[i].o[d] = origins[3 * i + d];
+        }
+        for (int w = 0; w < 8; ++w) leaves[i].m[w] = masks[8 * i + w];
+    }
+    return finalize(leaves, out);
+}
+
+int pvdb_topo_destroy(pvdb_topology* t) {
+    if (!t) return PVDB_OK;
+    int dev = 0;
+    hipGetDevice(&dev);
+    if (dev != t->device) hipSetDevice(t->device);
+    if (t->d_table) hipFree(t->d_table);
+    if (t->d_masks) hipFree(t->d_masks);
+    if (t->d_origins) hipFree(t->d_origins);
+    if (t->d_lower) hipFree(t->d_lower);
+    if (dev != t->device) hipSetDevice(dev);
+    delete t;
+    return PVDB_OK;
+}
+
+int pvdb_topo_info(const pvdb_topology* t, int64_t* nleaf, int64_t* nactive, int32_t* lo3,
+                   int32_t* dim3) {
+    if (!t) {
+        pvdb::set_error("null topology");
+        return PVDB_EHANDLE;
+    }
+    if (nleaf) *nleaf = t->nleaf;
+    if (nactive) *nactive = t->nactive;
+    for (int d = 0; d < 3; ++d) {
+        if (lo3) lo3[d] = t->lo[d];
+        if (dim3) dim3[d] = t->dim[d];
+    }
+    return PVDB_OK;
+}
+
+int pvdb_topo_leaves(const pvdb_topology* t, int32_t* origins, uint64_t* masks) {
+    if (!t) {
+        pvdb::set_error("null topology");
+        return PVDB_EHANDLE;
+    }
+    if (origins && !t->origins.empty())
+        std::memcpy(origins, t->origins.data(), t->origins.size() * sizeof(int32_t));
+    if (masks && !t->masks.empty()) std::memcpy(masks, t->masks.data(), t->masks.size() * 8);
+    return PVDB_OK;
+}
+
+int pvdb_topo_slots(const pvdb_topology* t, const int32_t* ijk, int64_t n, int64_t* slots,
+                    uint8_t* active) {
+    if (!t) {
+        pvdb::set_error("null topology");
+        return PVDB_EHANDLE;
+    }
+    PVDB_CHECK_ARG(n >= 0 && (n == 0 || (ijk && slots)), "bad query arrays");
+    for (int64_t q = 0; q < n; ++q) {
+        const int32_t i = ijk[3 * q], j = ijk[3 * q + 1], k = ijk[3 * q + 2];
+        const int64_t a = (int64_t)(i >> 3) - t->lo[0], b = (int64_t)(j >> 3) - t->lo[1],
+                      c = (int64_t)(k >> 3) - t->lo[2];
+        int64_t s = -1;
+        bool act = false;
+        if (a >= 0 && b >= 0 && c >= 0 && a < t->dim[0] && b < t->dim[1] && c < t->dim[2]) {
+            const int32_t l = t->table[((size_t)a * t->dim[1] + b) * t->dim[2] + c];
+            if (l >= 0) {
+                const int off = ((i & 7) << 6) | ((j & 7) << 3) | (k & 7);
+                s = (int64_t)l * 512 + off;
+                act = (t->masks[8 * (size_t)l + (off >> 6)] >> (off & 63)) & 1ull;
+            }
+        }
+        slots[q] = s;
+        if (active) active[q] = act ? 1 : 0;
+    }
+    (void)floor_div8;
+    return PVDB_OK;
+}
+
+int pvdb_topo_device_arrays(const pvdb_topology* t, const int32_t** table, const uint64_t** masks,
+                            const int32_t** origins) {
+    if (!t) {
+        pvdb::set_error("null topology");
+        return PVDB_EHANDLE;
+    }
+    if (table) *table = t->d_table;
+    if (masks) *masks = t->d_masks;
+    if (origins) *origins = t->d_origins;
+    return PVDB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,325 @@
+"""Drop-in replacement of the reference's `plenvdb` extension module.
+
+Same classes, method names, arities and argument meaning as the pybind11
+module of plenvdb/lib/vdb/plenvdb.cpp:3-171 (`DensityVDB`, `ColorVDB`,
+`DensityOpt`, `ColorOpt`, `MGRenderer`).  Each method cites the reference
+method it replaces.
+
+Inputs may be numpy arrays (reference behaviour: host buffers are copied to
+the device, the call completes before returning and numpy arrays come back)
+or torch CUDA tensors (zero-copy and asynchronous on torch's current stream;
+torch tensors come back), so an autograd path can stay on the device.
+
+All device work runs through libplenvdb_hip.so; there is no host fallback.
+"""
+from __future__ import annotations
+
+import math
+import time
+
+import numpy as np
+import torch
+
+from . import lib
+from .topology import Topology
+
+_DEV = "cuda"
+
+
+def _as_dev_f32(a) -> tuple[torch.Tensor, bool]:
+    """(device float32 contiguous tensor, input_was_numpy)."""
+    if isinstance(a, torch.Tensor):
+        lib.require_cuda(a)
+        return a.detach().to(torch.float32).contiguous(), False
+    arr = np.ascontiguousarray(np.asarray(a, dtype=np.float32).reshape(-1))
+    return torch.from_numpy(arr).to(_DEV), True
+
+
+def _as_dev_u8(a) -> torch.Tensor:
+    if isinstance(a, torch.Tensor):
+        lib.require_cuda(a)
+        return a.detach().to(torch.uint8).contiguous()
+    arr = np.ascontiguousarray(np.asarray(a).astype(np.uint8).reshape(-1))
+    return torch.from_numpy(arr).to(_DEV)
+
+
+class _BaseVDB:
+    """Shared part of DensityVDB / ColorVDB (BaseVDB, plenvdb/lib/vdb/plenvdb.h:385-431).
+
+    `grid` holds the data payload and `grad` the gradient payload, both
+    [nslots, ndim] float32 on one topology.
+    """
+
+    _order = 0
+
+    def __init__(self, reso, ndim: int, topology: Topology | None = None, mask=None):
+        self.reso = [int(r) for r in reso]
+        self.ndim = int(ndim)
+        self.num = max(1, self.ndim // 3) if self.ndim > 1 else 1
+        self.timer = 0.0
+        if topology is None:
+            topology = Topology.from_mask(mask) if mask is not None else Topology.dense(self.reso)
+        self.topo = topology
+        self.grid = torch.zeros(self.topo.nslots, self.ndim, dtype=torch.float32, device=_DEV)
+        self.grad = torch.zeros_like(self.grid)
+
+    # --- queries -------------------------------------------------------------
+    def getndim(self) -> int:  # plenvdb.h:398
+        return self.ndim
+
+    def getinfo(self) -> None:  # plenvdb.h:424-428
+        print(f"Resolution: ({self.reso[0]},{self.reso[1]},{self.reso[2]})")
+        print(f"Num: {self.num}")
+        print(f"Dim: {self.ndim}")
+
+    def resetTimer(self) -> None:  # plenvdb.h:396
+        self.timer = 0.0
+
+    def getTimer(self) -> float:  # plenvdb.h:397
+        return self.timer
+
+    def setReso(self, rx: int, ry: int, rz: int) -> None:  # plenvdb.h:429
+        self.reso = [int(rx), int(ry), int(rz)]
+
+    # --- forward / backward ------------------------------------------------------
+    def forward(self, x, y, z):
+        """Trilinear values at index-space points (plenvdb.h:441-466, 535-556).
+
+        Returns [N*ndim] (numpy in -> numpy out, like the reference) or a
+        [N, ndim] device tensor for device inputs."""
+        xs, was_np = _as_dev_f32(x)
+        ys, _ = _as_dev_f32(y)
+        zs, _ = _as_dev_f32(z)
+        n = xs.numel()
+        if ys.numel() != n or zs.numel() != n:
+            raise ValueError("x, y, z must have the same length")
+        out = torch.empty(n, self.ndim, dtype=torch.float32, device=_DEV)
+        lib.call("pvdb_trilinear_fwd", self.topo.handle, lib.ptr(self.grid), self.ndim, self._order,
+                 lib.ptr(xs), lib.ptr(ys), lib.ptr(zs), n, lib.ptr(out), lib.stream())
+        if was_np:
+            return out.reshape(-1).cpu().numpy()
+        return out
+
+    def backward(self, x, y, z, g) -> None:
+        """Accumulate g into the internal gradient payload (plenvdb.h:468-485, 557-573)."""
+        xs, was_np = _as_dev_f32(x)
+        ys, _ = _as_dev_f32(y)
+        zs, _ = _as_dev_f32(z)
+        gs, _ = _as_dev_f32(g)
+        n = xs.numel()
+        if gs.numel() != n * self.ndim:
+            raise ValueError(f"gradient has {gs.numel()} values, expected {n * self.ndim}")
+        lib.call("pvdb_trilinear_bwd", self.topo.handle, lib.ptr(self.grad), self.ndim, self._order,
+                 lib.ptr(xs), lib.ptr(ys), lib.ptr(zs), lib.ptr(gs), n, lib.stream())
+        if was_np:
+            torch.cuda.current_stream().synchronize()
+
+    # --- dense copies ------------------------------------------------------------
+    def copyFromDense(self, arr) -> None:
+        """Active slots <- dense [X][Y][Z][ndim] (plenvdb.h:415; asserts the size)."""
+        a, was_np = _as_dev_f32(arr)
+        expect = self.reso[0] * self.reso[1] * self.reso[2] * self.ndim
+        if a.numel() != expect:
+            raise ValueError(f"copyFromDense: got {a.numel()} values, expected {expect}")
+        lib.call("pvdb_copy_from_dense", self.topo.handle, lib.ptr(self.grid), self.ndim, lib.ptr(a),
+                 *self.reso, lib.stream())
+        if was_np:
+            torch.cuda.current_stream().synchronize()
+
+    def get_dense_grid(self):
+        """Dense [X*Y*Z*ndim] copy of the values (plenvdb.h:403,416-422)."""
+        out = torch.empty(self.reso[0] * self.reso[1] * self.reso[2] * self.ndim,
+                          dtype=torch.float32, device=_DEV)
+        lib.call("pvdb_copy_to_dense", self.topo.handle, lib.ptr(self.grid), self.ndim, lib.ptr(out),
+                 *self.reso, 0.0, lib.stream())
+        return out.cpu().numpy()
+
+    def get_dense_grid_device(self) -> torch.Tensor:
+        out = torch.empty(*self.reso, self.ndim, dtype=torch.float32, device=_DEV)
+        lib.call("pvdb_copy_to_dense", self.topo.handle, lib.ptr(self.grid), self.ndim, lib.ptr(out),
+                 *self.reso, 0.0, lib.stream())
+        return out
+
+    def total_variation_add_grad(self, wx, wy, wz, dense_mode) -> None:
+        """The reference stubs this out (plenvdb.h:497-501,574-578)."""
+        print("Not Supported Now...")
+
+    # --- persistence -------------------------------------------------------------
+    def save_to(self, path: str) -> None:
+        """Writes the grid (plenvdb.h:143-148,269-276) in the container format of vdbio."""
+        from . import vdbio
+
+        vdbio.save_grid(path, self.topo, self.grid, self.reso)
+
+    def load_from(self, path: str) -> None:
+        """Loads a grid written by save_to; reso <- its active voxel extent
+        (plenvdb.h:126-142,211-240; BaseVDB::load_from :404-407)."""
+        from . import vdbio
+
+        topo, vals, reso = vdbio.load_grid(path, self.ndim)
+        self.topo = topo
+        self.grid = vals
+        self.grad = torch.zeros_like(vals)
+        self.reso = list(reso)
+
+
+class DensityVDB(_BaseVDB):
+    """plenvdb.cpp:5-41; plenvdb.h:434-520."""
+
+    _order = 0
+
+    def __init__(self, resolution, n: int = 1, *, topology: Topology | None = None, mask=None):
+        if int(n) != 1:
+            raise ValueError("DensityVDB holds exactly one channel")
+        super().__init__(resolution, 1, topology=topology, mask=mask)
+
+    def setValuesOn_bymask(self, mask, val: float) -> None:
+        """Active slots <- val where the dense [X*Y*Z] mask is set (plenvdb.h:487-495)."""
+        m = _as_dev_u8(mask)
+        if m.numel() != self.reso[0] * self.reso[1] * self.reso[2]:
+            raise ValueError("setValuesOn_bymask: mask size does not match the resolution")
+        lib.call("pvdb_set_values_by_mask", self.topo.handle, lib.ptr(self.grid), lib.ptr(m),
+                 float(val), *self.reso, lib.stream())
+        if not isinstance(mask, torch.Tensor):
+            torch.cuda.current_stream().synchronize()
+
+
+class ColorVDB(_BaseVDB):
+    """plenvdb.cpp:44-78; plenvdb.h:523-597."""
+
+    _order = 1
+
+    def __init__(self, resolution, n: int, *, topology: Topology | None = None, mask=None):
+        if int(n) <= 0 or int(n) % 3 != 0:
+            raise ValueError("ColorVDB needs a positive multiple of 3 channels")
+        super().__init__(resolution, int(n), topology=topology, mask=mask)
+
+
+class _BaseOpt:
+    """BaseOptimizer (plenvdb.h:686-745): Adam moments on the parameter's topology."""
+
+    _group = 1
+
+    def __init__(self, vdb: _BaseVDB, lr: float, eps: float, beta0: float, beta1: float):
+        self.params = vdb  # the reference keeps a reference to the VDB (plenvdb.h:697)
+        self.lr = float(lr)
+        self.eps = float(eps)
+        self.beta0 = float(beta0)
+        self.beta1 = float(beta1)
+        self.step_count = 0
+        self.exp_avg = torch.zeros_like(vdb.grid)
+        self.exp_avg_sq = torch.zeros_like(vdb.grid)
+        self.per_lr = None
+
+    def _stepsz(self) -> float:
+        """stepsz = lr*sqrt(1-beta1^t)/(1-beta0^t) in float (plenvdb.h:753,776)."""
+        f = np.float32
+        t = f(self.step_count)
+        num = np.sqrt(f(1) - np.power(f(self.beta1), t, dtype=np.float32), dtype=np.float32)
+        return float(f(self.lr) * num / (f(1) - np.power(f(self.beta0), t, dtype=np.float32)))
+
+    def step(self, stepmode: int) -> None:
+        """step_optimizer (plenvdb.h:751-767, 774-789)."""
+        mode = int(stepmode)
+        if mode not in (0, 1, 2):
+            raise ValueError("stepmode must be 0, 1 or 2")
+        if mode == 2 and self.per_lr is None:
+            raise RuntimeError("stepmode 2 needs set_pervoxel_lr first")
+        self.step_count += 1
+        p = self.params
+        lib.call("pvdb_vdb_adam", p.topo.handle, lib.ptr(p.grid), lib.ptr(p.grad), lib.ptr(self.exp_avg),
+                 lib.ptr(self.exp_avg_sq), lib.ptr(self.per_lr if mode == 2 else None), p.ndim,
+                 self._group, mode, self._stepsz(), self.eps, self.beta0, self.beta1, lib.stream())
+
+    def zero_grad(self) -> None:
+        """plenvdb.h:750, 773."""
+        p = self.params
+        lib.call("pvdb_zero_active", p.topo.handle, lib.ptr(p.grad), p.ndim, lib.stream())
+
+    def update_lr(self, factor: float) -> None:  # plenvdb.h:712
+        self.lr = float(np.float32(self.lr) * np.float32(factor))
+
+    def set_pervoxel_lr(self, count) -> None:
+        """Per-voxel lr from a dense [X*Y*Z] array (plenvdb.h:713-719); stored on the
+        parameter's own topology."""
+        p = self.params
+        c, was_np = _as_dev_f32(count)
+        if c.numel() != p.reso[0] * p.reso[1] * p.reso[2]:
+            raise ValueError("set_pervoxel_lr: size does not match the resolution")
+        self.per_lr = torch.zeros(p.topo.nslots, dtype=torch.float32, device=_DEV)
+        lib.call("pvdb_copy_from_dense", p.topo.handle, lib.ptr(self.per_lr), 1, lib.ptr(c), *p.reso,
+                 lib.stream())
+        if was_np:
+            torch.cuda.current_stream().synchronize()
+
+    def set_grad(self, arr) -> None:
+        """grad <- dense array on active slots (plenvdb.h:701)."""
+        p = self.params
+        a, was_np = _as_dev_f32(arr)
+        lib.call("pvdb_copy_from_dense", p.topo.handle, lib.ptr(p.grad), p.ndim, lib.ptr(a), *p.reso,
+                 lib.stream())
+        if was_np:
+            torch.cuda.current_stream().synchronize()
+
+    def save_to(self, prefix: str) -> None:
+        """prefix+'exp_avg.vdb', prefix+'exp_avg_sq.vdb' (plenvdb.h:729-736)."""
+        from . import vdbio
+
+        vdbio.save_grid(prefix + "exp_avg.vdb", self.params.topo, self.exp_avg, self.params.reso)
+        vdbio.save_grid(prefix + "exp_avg_sq.vdb", self.params.topo, self.exp_avg_sq, self.params.reso)
+
+    def load_from(self, prefix: str) -> None:
+        """plenvdb.h:720-728; the moments must live on the parameter's topology."""
+        from . import vdbio
+
+        for name in ("exp_avg", "exp_avg_sq"):
+            topo, vals, _ = vdbio.load_grid(prefix + name + ".vdb", self.params.ndim)
+            if topo.nleaf != self.params.topo.nleaf:
+                raise ValueError(f"{name}: topology does not match the parameter grid")
+            setattr(self, name, vals)
+
+    def getinfo(self) -> None:  # plenvdb.h:739-744
+        p = self.params
+        print(f"Resolution: ({p.reso[0]},{p.reso[1]},{p.reso[2]})")
+        print(f"Num: {p.num}")
+        print(f"Dim: {p.ndim}")
+        print(f"Step: {self.step_count}")
+        print(f"lr: {self.lr}")
+        print(f"eps: {self.eps}")
+        print(f"beta: {self.beta0},{self.beta1}")
+
+    # plenvdb.cpp:97-106
+    def getLr(self): return self.lr
+    def getEps(self): return self.eps
+    def getBeta0(self): return self.beta0
+    def getBeta1(self): return self.beta1
+    def getStep(self): return self.step_count
+    def setLr(self, x): self.lr = float(x)
+    def setEps(self, x): self.eps = float(x)
+    def setBeta0(self, x): self.beta0 = float(x)
+    def setBeta1(self, x): self.beta1 = float(x)
+    def setStep(self, x): self.step_count = int(x)
+
+
+class DensityOpt(_BaseOpt):
+    """plenvdb.cpp:81-106; plenvdb.h:747-768."""
+
+    _group = 1
+
+
+class ColorOpt(_BaseOpt):
+    """plenvdb.cpp:108-132; plenvdb.h:770-790 (skip-zero-grad per Vec3 group)."""
+
+    _group = 3
+
+
+def _mgrenderer():
+    from .renderer import MGRenderer
+
+    return MGRenderer
+
+
+def __getattr__(name):
+    if name == "MGRenderer":
+        return _mgrenderer()
+    raise AttributeError(name)

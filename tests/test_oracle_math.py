@@ -1,0 +1,243 @@
+"""Pins the oracle's arithmetic against closed forms, library routines, brute
+force and finite differences, chosen so that a dropped term, a wrong sign or
+index, or a transposed operand fails one of them."""
+import numpy as np
+import pytest
+import torch
+
+
+def dense_block(oracle, n_leaf=2, C=1, seed=0, fill=None):
+    """Fully dense n_leaf^3-leaf block at the origin with random (or given) values."""
+    R = 8 * n_leaf
+    mask = np.ones((R, R, R), bool)
+    origins, vm = oracle.leaves_from_mask(mask)
+    dom = oracle.SlotDomain(origins, vm)
+    coords = dom.coords()
+    if fill is None:
+        rng = np.random.default_rng(seed)
+        vals = rng.normal(size=(dom.nslots, C))
+    else:
+        vals = fill(coords)
+    return dom, vals, coords, R
+
+
+def test_trilinear_reproduces_linear_field(oracle_mod):
+    # NanoVDB TestNanoVDB.cc:4277 pattern: trilinear reproduces a linear field.
+    a = np.array([0.7, -1.3, 2.1])
+    dom, vals, coords, R = dense_block(oracle_mod, fill=lambda c: (c @ a + 0.25)[:, None])
+    rng = np.random.default_rng(1)
+    p = rng.uniform(0, R - 1, size=(2000, 3)).astype(np.float32)
+    for order in (0, 1):
+        got = oracle_mod.trilinear_fwd(dom, vals, p[:, 0], p[:, 1], p[:, 2], order, "f64")[:, 0]
+        np.testing.assert_allclose(got, p.astype(np.float64) @ a + 0.25, rtol=0, atol=1e-10)
+        got32 = oracle_mod.trilinear_fwd(dom, vals.astype(np.float32), p[:, 0], p[:, 1], p[:, 2],
+                                         order, "f32")[:, 0]
+        np.testing.assert_allclose(got32, p.astype(np.float64) @ a + 0.25, rtol=1e-5, atol=1e-4)
+
+
+def test_trilinear_equals_grid_sample(oracle_mod):
+    # On a fully dense block the interpolant equals F.grid_sample(align_corners=True)
+    # (the DenseGrid path, plenvdb/lib/grid.py:151-162).
+    C = 3
+    dom, vals, coords, R = dense_block(oracle_mod, C=C, seed=3)
+    dense = np.zeros((R, R, R, C))
+    dense[coords[:, 0], coords[:, 1], coords[:, 2]] = vals
+    rng = np.random.default_rng(4)
+    p = rng.uniform(0, R - 1, size=(1500, 3)).astype(np.float32)
+    got = oracle_mod.trilinear_fwd(dom, vals, p[:, 0], p[:, 1], p[:, 2], 1, "f64")
+    g = torch.tensor(dense, dtype=torch.float64).permute(3, 0, 1, 2)[None]
+    ind = torch.tensor(p.astype(np.float64) / (R - 1) * 2 - 1).flip(-1).reshape(1, 1, 1, -1, 3)
+    ref = torch.nn.functional.grid_sample(g, ind, mode="bilinear", align_corners=True)
+    ref = ref.reshape(C, -1).T.numpy()
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12)
+
+
+def sparse_partial(oracle, seed=5):
+    """A few partial leaves with gaps, so points straddle existing and missing leaves."""
+    rng = np.random.default_rng(seed)
+    R = 32
+    lm = rng.random((4, 4, 4)) < 0.5
+    vm = rng.random((R, R, R)) < 0.7
+    mask = np.repeat(np.repeat(np.repeat(lm, 8, 0), 8, 1), 8, 2) & vm
+    origins, vmask = oracle.leaves_from_mask(mask)
+    return oracle.SlotDomain(origins, vmask), R
+
+
+def test_scatter_is_adjoint_of_gather(oracle_mod):
+    dom, R = sparse_partial(oracle_mod)
+    rng = np.random.default_rng(6)
+    C = 12
+    G = rng.normal(size=(dom.nslots, C))
+    p = rng.uniform(-1.5, R + 0.5, size=(3000, 3)).astype(np.float32)
+    g = rng.normal(size=(len(p), C))
+    for order in (0, 1):
+        Cc = 1 if order == 0 else C
+        fwd = oracle_mod.trilinear_fwd(dom, G[:, :Cc], p[:, 0], p[:, 1], p[:, 2], order, "f64")
+        bwd = oracle_mod.trilinear_bwd(dom, Cc, p[:, 0], p[:, 1], p[:, 2], g[:, :Cc], order, "f64")
+        lhs = np.sum(fwd * g[:, :Cc])
+        rhs = np.sum(G[:, :Cc] * bwd)
+        assert abs(lhs - rhs) <= 1e-9 * max(1.0, abs(lhs))
+
+
+def test_scatter_leaf_must_exist(oracle_mod):
+    # Leaf at origin exists (all voxels inactive except one), its +x neighbour does not.
+    vm = np.zeros((1, 512), bool)
+    vm[0, 0] = True
+    dom = oracle_mod.SlotDomain(np.array([[0, 0, 0]]), vm)
+    p = np.array([[7.25, 3.5, 2.0]], np.float32)
+    grad = oracle_mod.trilinear_bwd(dom, 1, p[:, 0], p[:, 1], p[:, 2], np.ones((1, 1)), 0, "f64")
+    touched = np.nonzero(grad[:, 0])[0]
+    coords = dom.coords()[touched]
+    # only x=7 corners survive; inactive voxels of the existing leaf receive the add
+    assert set(map(tuple, coords.tolist())) == {(7, 3, 2), (7, 4, 2)}
+    np.testing.assert_allclose(grad[touched, 0], [0.75 * 0.5, 0.75 * 0.5])
+
+
+def test_ray_bounds_hand_cases(oracle_mod):
+    mn, mx = np.array([-1.5] * 3, np.float32), np.array([1.5] * 3, np.float32)
+    o = np.array([[-4, 0, 0], [0, 0, 4], [0.5, 0.2, -4]], np.float32)
+    d = np.array([[1, 0, 0], [0, 0, -1], [0, 0, 2]], np.float32)
+    tmin, tmax, ns = oracle_mod.ray_bounds(o, d, mn, mx, 0.0, 1e9, 0.5, prec="f64")
+    np.testing.assert_allclose(tmin, [2.5, 2.5, 1.25])
+    np.testing.assert_allclose(tmax, [5.5, 5.5, 2.75])
+    np.testing.assert_array_equal(ns, [6, 6, 6])  # ceil((tmax-tmin)*|d|/0.5)
+    # a miss gives tmin > tmax clamped to near; the count is at least 1
+    tmin, tmax, ns = oracle_mod.ray_bounds(np.array([[5, 5, 5]], np.float32),
+                                           np.array([[1, 0, 0]], np.float32), mn, mx, 2.0, 1e9, 0.5)
+    assert ns[0] == 1 and tmin[0] == 2.0 and tmax[0] == 2.0
+
+
+def test_sample_points_and_mask_rounding(oracle_mod):
+    mn, mx = np.array([-1.5] * 3, np.float32), np.array([1.5] * 3, np.float32)
+    o = np.array([[-4, 0.1, 0.2]], np.float32)
+    d = np.array([[2, 0, 0]], np.float32)
+    tmin, tmax, ns = oracle_mod.ray_bounds(o, d, mn, mx, 0.0, 1e9, 0.25, prec="f64")
+    pts, rid, sid, inb = oracle_mod.sample_pts(o, d, tmin, ns, 0.25, mn, mx, prec="f64")
+    np.testing.assert_allclose(pts[:, 0], -1.5 + 0.25 * np.arange(ns[0]))
+    assert inb.all() and len(pts) == 12
+    # round half away from zero: 2.5 -> 3, -0.5 -> -1 (outside)
+    mask = np.zeros((4, 4, 4), bool)
+    mask[3, 0, 0] = True
+    got = oracle_mod.mask_lookup(mask, np.array([[2.5, 0, 0], [2.4999, 0, 0]]), [1, 1, 1], [0, 0, 0])
+    np.testing.assert_array_equal(got, [True, False])
+
+
+def test_raw2alpha_closed_form_and_derivative(oracle_mod):
+    s = np.linspace(-12, 8, 201)
+    shift = float(np.float32(-4.595))  # the kernels take shift as a float
+    e, a = oracle_mod.raw2alpha(s, shift, 0.5, "f64")
+    sp = np.log1p(np.exp(s + shift))
+    np.testing.assert_allclose(a, 1 - np.exp(-sp * 0.5), rtol=1e-12, atol=1e-15)
+    g = oracle_mod.raw2alpha_bwd(e, np.ones_like(s), 0.5, "f64")
+    h = 1e-6
+    fd = (oracle_mod.raw2alpha(s + h, shift, 0.5, "f64")[1]
+          - oracle_mod.raw2alpha(s - h, shift, 0.5, "f64")[1]) / (2 * h)
+    # FD of 1-(1+e)^-k cancels badly for tiny alpha: loose absolute floor
+    np.testing.assert_allclose(g, fd, rtol=1e-6, atol=1e-9)
+    # d/ds [1-(1+e^(s+b))^-k] = k e (1+e)^(-k-1)
+    ee = np.exp(s + shift)
+    np.testing.assert_allclose(g, 0.5 * ee * (1 + ee) ** -1.5, rtol=1e-12)
+
+
+def test_alpha2weight_invariants_and_gradient(oracle_mod):
+    rng = np.random.default_rng(8)
+    n_rays = 40
+    counts = rng.integers(0, 30, size=n_rays)
+    rid = np.repeat(np.arange(n_rays), counts)
+    a = rng.uniform(0, 0.15, size=len(rid))
+    w, T, tl, i_s, i_e = oracle_mod.alpha2weight(a, rid, n_rays, "f64")
+    # no early stop here: sum(w) + T_last = 1 and T is the exclusive product
+    np.testing.assert_array_less(1e-3, tl)
+    for r in range(n_rays):
+        seg = slice(i_s[r], i_e[r])
+        assert abs(w[seg].sum() + tl[r] - 1) < 1e-12
+        ex = np.concatenate([[1.0], np.cumprod(1 - a[seg])[:-1]]) if counts[r] else []
+        np.testing.assert_allclose(T[seg], ex, rtol=1e-12)
+    # gradient of L = <gw, w> + <gl, T_last> against central differences
+    gw = rng.normal(size=len(a))
+    gl = rng.normal(size=n_rays)
+    ga = oracle_mod.alpha2weight_bwd(a, w, T, tl, i_s, i_e, gw, gl, "f64")
+
+    def L(x):
+        w_, _, tl_, _, _ = oracle_mod.alpha2weight(x, rid, n_rays, "f64")
+        return gw @ w_ + gl @ tl_
+
+    h = 1e-7
+    idx = rng.choice(len(a), 40, replace=False)
+    for i in idx:
+        e = np.zeros_like(a)
+        e[i] = h
+        fd = (L(a + e) - L(a - e)) / (2 * h)
+        assert abs(fd - ga[i]) < 1e-6 * max(1, abs(fd))
+
+
+def test_alpha2weight_early_stop(oracle_mod):
+    a = np.array([0.5, 0.9, 0.99, 0.3, 0.2])
+    rid = np.zeros(5, np.int64)
+    w, T, tl, i_s, i_e = oracle_mod.alpha2weight(a, rid, 1, "f64")
+    # T after samples: 0.5, 0.05, 5e-4 (<1e-3: stop after the third sample)
+    assert i_e[0] == 3
+    np.testing.assert_allclose(w[:3], [0.5, 0.45, 0.0495])
+    np.testing.assert_allclose(w[3:], 0)
+    np.testing.assert_allclose(T[3:], 1)
+    np.testing.assert_allclose(tl, [5e-4])
+
+
+def test_vdb_adam_closed_form(oracle_mod):
+    rng = np.random.default_rng(9)
+    n = 64
+    p = rng.normal(size=(n, 3))
+    g = rng.normal(size=(n, 3))
+    g[5] = 0
+    g[7, 0] = 0
+    m = np.zeros_like(p)
+    v = np.zeros_like(p)
+    act = np.ones(n, np.uint8)
+    act[3] = 0
+    p0 = p.copy()
+    sz = oracle_mod.adam_stepsz(0.1, 0.9, 0.99, 1)
+    np.testing.assert_allclose(sz, 0.1 * np.sqrt(1 - 0.99) / (1 - 0.9), rtol=1e-6)
+    oracle_mod.vdb_adam(p, g, m, v, act, 3, 1, sz, 1e-8, 0.9, 0.99, prec="f64")
+    f = lambda x: float(np.float32(x))  # the kernels take the hyper-parameters as floats
+    exp = p0 - f(sz) * ((1 - f(0.9)) * g) / (f(1e-8) + np.sqrt((1 - f(0.99)) * g * g))
+    exp[3] = p0[3]  # inactive
+    exp[5] = p0[5]  # whole Vec3 group has zero grad: skipped
+    np.testing.assert_allclose(p, exp, rtol=1e-12, atol=1e-12)
+    assert m[7, 0] == 0 and m[7, 1] != 0  # the group with one zero component is updated
+
+
+def test_reference_step_gradient_matches_fd(oracle_mod):
+    """Central differences of the composed FP64 reference step w.r.t. density
+    and colour slot values (fast_color_thres 0, no early stop on these rays)."""
+    from plenvdb_amd import scenes
+
+    sc = scenes.make_scene("c2", n_rays=24)
+    origins, vm = oracle_mod.leaves_from_mask(sc["mask"])
+    rng = np.random.default_rng(1)
+    nslots = len(origins) * 512
+    den = rng.normal(0, 2, nslots)
+    col = rng.normal(0, 0.1, (nslots, 12))
+    sc.update(origins=origins, vmask=vm, density=den, color=col, fast_color_thres=0.0)
+    out = oracle_mod.reference_train_step(sc, prec="f64")
+    gd, gc = out["grad_density"], out["grad_color"]
+    cand_d = np.nonzero(np.abs(gd) > 1e-6)[0]
+    cand_c = np.argwhere(np.abs(gc) > 1e-6)
+    assert len(cand_d) > 10 and len(cand_c) > 10
+    h = 1e-5
+    for i in rng.choice(cand_d, 6, replace=False):
+        sc["density"] = den.copy()
+        sc["density"][i] += h
+        lp = oracle_mod.reference_train_step(sc, prec="f64")["loss"]
+        sc["density"][i] -= 2 * h
+        lm = oracle_mod.reference_train_step(sc, prec="f64")["loss"]
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - gd[i]) < 1e-5 * max(abs(fd), 1e-6) + 1e-11
+    sc["density"] = den
+    for s, c in cand_c[rng.choice(len(cand_c), 6, replace=False)]:
+        sc["color"] = col.copy()
+        sc["color"][s, c] += h
+        lp = oracle_mod.reference_train_step(sc, prec="f64")["loss"]
+        sc["color"][s, c] -= 2 * h
+        lm = oracle_mod.reference_train_step(sc, prec="f64")["loss"]
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - gc[s, c]) < 1e-5 * max(abs(fd), 1e-6) + 1e-11
