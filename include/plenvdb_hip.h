@@ -47,14 +47,16 @@ int pvdb_topo_create_from_mask(const uint8_t* mask /*host*/, int32_t rx, int32_t
                                pvdb_topology** out);
 /* Leaf list (origins multiples of 8, 8 value-mask words per leaf); the
  * library sorts it into NanoVDB order (openvdb/nanovdb/nanovdb/util/
- * OpenToNanoVDB.h:509-533).  Synchronous. */
+ * OpenToNanoVDB.h:509-533).  layout: 0 automatic, 1 one dense leaf table,
+ * 2 lower-node table + per-lower-node child tables (for leaves spread over a
+ * large extent).  Synchronous. */
 int pvdb_topo_create_from_leaves(const int32_t* origins /*host [n][3]*/,
                                  const uint64_t* masks /*host [n][8]*/, int64_t nleaf,
-                                 pvdb_topology** out);
+                                 int32_t layout, pvdb_topology** out);
 int pvdb_topo_destroy(pvdb_topology* topo);
 /* tree().nodeCount(0) and active voxel count (plenvdb.h:123: nleafCount). */
 int pvdb_topo_info(const pvdb_topology* topo, int64_t* nleaf, int64_t* nactive,
-                   int32_t* leaf_lo3, int32_t* leaf_dim3);
+                   int32_t* leaf_lo3, int32_t* leaf_dim3, int32_t* two_level);
 /* Leaf origins and value masks in NanoVDB order (tree().getFirstNode<0>()+i). */
 int pvdb_topo_leaves(const pvdb_topology* topo, int32_t* origins /*host*/,
                      uint64_t* masks /*host*/);

@@ -307,8 +307,10 @@ def vdb_adam(param, grad, m, v, active, group, mode, stepsz, eps, b0, b1, per_lr
         assert x.dtype == dt and x.flags.c_contiguous
     act = np.ascontiguousarray(np.asarray(active).astype(np.uint8))
     pl = None if per_lr is None else np.ascontiguousarray(np.asarray(per_lr, dt))
+    nch = param.size // len(act)
     getattr(lib, f"pvdb_vdb_adam_{prec}")(_p(param), _p(grad), _p(m), _p(v), _p(pl), _p(act),
-                                           C.c_long(len(act)), C.c_int(group), C.c_int(mode),
+                                           C.c_long(len(act)), C.c_int(nch), C.c_int(group),
+                                           C.c_int(mode),
                                            C.c_float(stepsz), C.c_float(eps), C.c_float(b0),
                                            C.c_float(b1))
 
@@ -365,21 +367,23 @@ def reference_train_step(scene: dict, prec: str = "f32", with_mlp: bool = True):
     mx = np.asarray(scene["xyz_max"], np.float32)
     N = len(scene["rays_o"])
     stepdist = np.float32(scene["stepsize"]) * np.float32(scene["voxel_size"])
-    # sample_ray (dvgo.py:272-294) with far forced to 1e9
+    # sample_ray (dvgo.py:272-294) with far forced to 1e9.  Sample positions and
+    # the occupancy decision are the reference's FP32 integer work in both
+    # precisions; FP64 applies to the arithmetic on those samples.
     tmin, tmax, ns = ray_bounds(scene["rays_o"], scene["rays_d"], mn, mx, scene["near"], 1e9,
-                                stepdist, far_per_ray=scene.get("far_per_ray"), prec=prec)
+                                stepdist, far_per_ray=scene.get("far_per_ray"), prec="f32")
     pts, rid, sid, inb = sample_pts(scene["rays_o"], scene["rays_d"], tmin, ns, stepdist, mn, mx,
-                                    prec=prec)
+                                    prec="f32")
     pts, rid, sid = pts[inb], rid[inb], sid[inb]
     # mask_cache (dvgo.py:311-316), mask = active voxels of the topology
     occ = dense_active_mask(dom, reso)
     scale = ((reso.astype(np.float32) - np.float32(1)) / (mx - mn)).astype(np.float32)
     shift = (-mn * scale).astype(np.float32)
-    keep = mask_lookup(occ, pts, scale, shift, prec=prec)
+    keep = mask_lookup(occ, pts, scale, shift, prec="f32")
     pts, rid, sid = pts[keep], rid[keep], sid[keep]
     n_mask = len(rid)
     # density + raw2alpha
-    xs, ys, zs = wld2idx(pts, mn, mx, reso, prec=prec)
+    xs, ys, zs = wld2idx(pts, mn, mx, reso, prec="f32")
     sigma = trilinear_fwd(dom, scene["density"], xs, ys, zs, 0, prec)[:, 0]
     interval = np.float32(scene["stepsize"]) * np.float32(scene["voxel_size_ratio"])
     e, alpha = raw2alpha(sigma, scene["act_shift"], interval, prec)

@@ -289,33 +289,38 @@ void FN(pvdb_alpha2weight_bwd)(const REAL* alpha, const REAL* w, const REAL* T,
 
 /* Adam on VDB slots (plenvdb/lib/vdb/densityvdb.cu:185-349,
  * colorvdb.cu:179-351).  Slots are leaf*512+offset; only active slots are
- * touched.  group = number of consecutive values that form one grid value
- * (1 for density, 3 for a Vec3f colour grid): mode 1 skips a slot only when
- * all values of its group have zero gradient (colorvdb.cu:318).
+ * touched.  Payloads are [nslots][C]; group = number of consecutive values
+ * that form one reference grid value (1 for density, 3 for a Vec3f colour
+ * grid): mode 1 skips a group only when all of its values have zero gradient
+ * (colorvdb.cu:318).
  * mode 0 plain, 1 skip-zero-grad, 2 per-voxel lr (per_lr has one value per
  * slot).  m = b0 m + (1-b0) g; v = b1 v + ((1-b1) g) g;
  * p -= ([perlr*] stepsz * m) / (eps + sqrt(v)). */
 void FN(pvdb_vdb_adam)(REAL* param, const REAL* grad, REAL* m, REAL* v, const REAL* per_lr,
-                       const uint8_t* active, long nslots, int group, int mode, float stepsz,
-                       float eps, float b0, float b1) {
+                       const uint8_t* active, long nslots, int C, int group, int mode,
+                       float stepsz, float eps, float b0, float b1) {
+    const int ngroups = C / group;
 #pragma omp parallel for schedule(static)
     for (long s = 0; s < nslots; ++s) {
         if (!active[s]) continue;
-        if (mode == 1) {
-            int all0 = 1;
-            for (int c = 0; c < group; ++c) all0 &= grad[s * group + c] == 0;
-            if (all0) continue;
-        }
-        for (int c = 0; c < group; ++c) {
-            const long q = s * group + c;
-            const REAL g = grad[q];
-            const REAL nm = (REAL)b0 * m[q] + ((REAL)1 - (REAL)b0) * g;
-            const REAL nv = (REAL)b1 * v[q] + (((REAL)1 - (REAL)b1) * g) * g;
-            m[q] = nm;
-            v[q] = nv;
-            REAL sz = (REAL)stepsz;
-            if (mode == 2) sz = per_lr[s] * sz;
-            param[q] = param[q] - (sz * nm) / ((REAL)eps + sqrt(nv));
+        for (int gi = 0; gi < ngroups; ++gi) {
+            const long base = s * C + (long)gi * group;
+            if (mode == 1) {
+                int all0 = 1;
+                for (int c = 0; c < group; ++c) all0 &= grad[base + c] == 0;
+                if (all0) continue;
+            }
+            for (int c = 0; c < group; ++c) {
+                const long q = base + c;
+                const REAL g = grad[q];
+                const REAL nm = (REAL)b0 * m[q] + ((REAL)1 - (REAL)b0) * g;
+                const REAL nv = (REAL)b1 * v[q] + (((REAL)1 - (REAL)b1) * g) * g;
+                m[q] = nm;
+                v[q] = nv;
+                REAL sz = (REAL)stepsz;
+                if (mode == 2) sz = per_lr[s] * sz;
+                param[q] = param[q] - (sz * nm) / ((REAL)eps + (REAL)sqrt(nv));
+            }
         }
     }
 }
@@ -332,7 +337,7 @@ void FN(pvdb_dense_adam)(REAL* param, const REAL* grad, REAL* m, REAL* v, const 
         v[q] = (REAL)b1 * v[q] + (((REAL)1 - (REAL)b1) * g) * g;
         REAL sz = (REAL)step_size;
         if (mode == 2) sz = sz * per_lr[q];
-        param[q] = param[q] - (sz * m[q]) / (sqrt(v[q]) + (REAL)eps);
+        param[q] = param[q] - (sz * m[q]) / ((REAL)sqrt(v[q]) + (REAL)eps);
     }
 }
 

@@ -27,6 +27,7 @@ HIP_FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-ffp-contract=off",
+    "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-munsafe-fp-atomics",
     "-mcode-object-version=5",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

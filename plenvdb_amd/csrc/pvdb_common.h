@@ -15,6 +15,11 @@
 //     8^3 block: the leaf's index in NanoVDB depth-first order, or -1), which
 //     replaces the root scan plus the upper/lower child lookups by one load
 //     that stays L2-resident (40^3 * 4 B = 256 KB at 320^3);
+//   * when that table would exceed 2^24 cells (leaves spread far apart), a
+//     two-level layout instead: a dense table of 128^3 lower nodes and a
+//     16^3 child table per lower node;
+//   * a dense lower-node table in both layouts (the upper node's child mask),
+//     used by the marchers to skip empty 128^3 blocks;
 //   * per-leaf 512-bit value masks (NanoVDB LeafNode::mValueMask);
 //   * per-leaf origins.
 // Leaf indices and the in-leaf offset (x&7)<<6 | (y&7)<<3 | (z&7) equal the
@@ -24,13 +29,16 @@
 namespace pvdb {
 
 struct TopoView {
-    const int32_t* table;    // [d0*d1*d2] leaf index or -1
-    const uint64_t* masks;   // [nleaf*8] value masks, bit (offset & 63) of word offset>>6
-    const int32_t* origins;  // [nleaf*3]
-    const int32_t* lower_count;  // [c0*c1*c2] number of leaves per 16^3-leaf block (128^3 voxels)
-    int32_t lo0, lo1, lo2;   // leaf-grid origin, in leaf units (voxel >> 3)
-    int32_t d0, d1, d2;      // leaf-grid extent
-    int32_t c0, c1, c2;      // extent of the lower-node grid (leaf-grid >> 4, relative to lo)
+    const int32_t* table;        // one-level: [d0*d1*d2] leaf index or -1
+    const uint64_t* masks;       // [nleaf*8] value masks, bit (offset & 63) of word offset>>6
+    const int32_t* origins;      // [nleaf*3]
+    const int32_t* lower_table;  // [e0*e1*e2] lower-node (128^3 block) index or -1
+    const int32_t* lower_leaves; // two-level: [nlower*4096] leaf index per 8^3 block or -1
+    int32_t lo0, lo1, lo2;       // one-level: leaf-grid origin (voxel >> 3)
+    int32_t d0, d1, d2;          // one-level: leaf-grid extent
+    int32_t ulo0, ulo1, ulo2;    // lower-node grid origin (voxel >> 7)
+    int32_t e0, e1, e2;          // lower-node grid extent
+    int32_t two_level;           // 1: look leaves up through lower_table + lower_leaves
     int32_t pad_;
     int64_t nleaf;
 };
@@ -39,13 +47,31 @@ __device__ __forceinline__ int voxel_offset(int i, int j, int k) {
     return ((i & 7) << 6) | ((j & 7) << 3) | (k & 7);
 }
 
-// Leaf index of voxel (i,j,k) or -1 when no leaf holds it.
-__device__ __forceinline__ int32_t leaf_of(const TopoView& t, int i, int j, int k) {
-    const int a = (i >> 3) - t.lo0, b = (j >> 3) - t.lo1, c = (k >> 3) - t.lo2;
-    if ((unsigned)a >= (unsigned)t.d0 || (unsigned)b >= (unsigned)t.d1 ||
-        (unsigned)c >= (unsigned)t.d2)
+// Index of the 128^3 lower node holding voxel (i,j,k), or -1.
+__device__ __forceinline__ int32_t lower_of(const TopoView& t, int i, int j, int k) {
+    const int a = (i >> 7) - t.ulo0, b = (j >> 7) - t.ulo1, c = (k >> 7) - t.ulo2;
+    if ((unsigned)a >= (unsigned)t.e0 || (unsigned)b >= (unsigned)t.e1 ||
+        (unsigned)c >= (unsigned)t.e2)
         return -1;
-    return t.table[(a * t.d1 + b) * t.d2 + c];
+    return t.lower_table[(a * t.e1 + b) * t.e2 + c];
+}
+
+// Leaf index of voxel (i,j,k) or -1 when no leaf holds it.  One-level layout:
+// one load from the dense leaf table.  Two-level layout: the lower node's
+// child table, indexed like NanoVDB's InternalNode::CoordToOffset
+// (openvdb/nanovdb/nanovdb/NanoVDB.h:3272-3283).
+__device__ __forceinline__ int32_t leaf_of(const TopoView& t, int i, int j, int k) {
+    if (!t.two_level) {
+        const int a = (i >> 3) - t.lo0, b = (j >> 3) - t.lo1, c = (k >> 3) - t.lo2;
+        if ((unsigned)a >= (unsigned)t.d0 || (unsigned)b >= (unsigned)t.d1 ||
+            (unsigned)c >= (unsigned)t.d2)
+            return -1;
+        return t.table[(a * t.d1 + b) * t.d2 + c];
+    }
+    const int32_t low = lower_of(t, i, j, k);
+    if (low < 0) return -1;
+    const int off = (((i & 127) >> 3) << 8) | (((j & 127) >> 3) << 4) | ((k & 127) >> 3);
+    return t.lower_leaves[(int64_t)low * 4096 + off];
 }
 
 // Slot (leaf*512 + offset) of voxel (i,j,k) or -1 when its leaf does not exist.

@@ -32,16 +32,22 @@ int hip_fail(hipError_t e, const char* what) {
 struct pvdb_topology {
     int64_t nleaf = 0;
     int64_t nactive = 0;
-    int32_t lo[3] = {0, 0, 0};
+    int64_t nlower = 0;
+    int two_level = 0;
+    int32_t lo[3] = {0, 0, 0};    // leaf grid (voxel >> 3)
     int32_t dim[3] = {0, 0, 0};
-    int32_t cdim[3] = {0, 0, 0};
-    std::vector<int32_t> origins;   // host copies
+    int32_t ulo[3] = {0, 0, 0};   // lower-node grid (voxel >> 7)
+    int32_t udim[3] = {0, 0, 0};
+    std::vector<int32_t> origins;  // host copies
     std::vector<uint64_t> masks;
-    std::vector<int32_t> table;
+    std::vector<int32_t> table;         // one-level leaf table
+    std::vector<int32_t> lower_table;   // lower-node table
+    std::vector<int32_t> lower_leaves;  // two-level child tables
     int32_t* d_table = nullptr;
     uint64_t* d_masks = nullptr;
     int32_t* d_origins = nullptr;
     int32_t* d_lower = nullptr;
+    int32_t* d_lower_leaves = nullptr;
     int device = 0;
 };
 
@@ -72,9 +78,15 @@ bool nano_less(const LeafRec& a, const LeafRec& b) {
     return leaf_off(a.o) < leaf_off(b.o);
 }
 
-int64_t floor_div8(int64_t v) { return v >= 0 ? v / 8 : -((-v + 7) / 8); }
+int upload(const std::vector<int32_t>& h, int32_t** d) {
+    const size_t bytes = std::max<size_t>(h.size(), 1) * sizeof(int32_t);
+    hipError_t e = hipMalloc(d, bytes);
+    if (e == hipSuccess && !h.empty()) e = hipMemcpy(*d, h.data(), h.size() * 4, hipMemcpyHostToDevice);
+    return e == hipSuccess ? PVDB_OK : pvdb::hip_fail(e, "topology upload");
+}
 
-int finalize(std::vector<LeafRec>& leaves, pvdb_topology** out) {
+// layout: 0 automatic, 1 one-level (dense leaf table), 2 two-level.
+int finalize(std::vector<LeafRec>& leaves, int layout, pvdb_topology** out) {
     std::sort(leaves.begin(), leaves.end(), nano_less);
     for (size_t i = 1; i < leaves.size(); ++i)
         if (leaves[i].o[0] == leaves[i - 1].o[0] && leaves[i].o[1] == leaves[i - 1].o[1] &&
@@ -94,28 +106,48 @@ int finalize(std::vector<LeafRec>& leaves, pvdb_topology** out) {
                 lmax[d] = std::max(lmax[d], L.o[d] >> 3);
             }
     }
-    int64_t cells = 1;
+    int64_t cells = 1, ucells = 1;
     for (int d = 0; d < 3; ++d) {
         t->lo[d] = lmin[d];
         t->dim[d] = lmax[d] - lmin[d] + 1;
-        t->cdim[d] = (t->dim[d] + 15) / 16;
+        t->ulo[d] = lmin[d] >> 4;
+        t->udim[d] = (lmax[d] >> 4) - (lmin[d] >> 4) + 1;
         cells *= t->dim[d];
+        ucells *= t->udim[d];
     }
-    if (cells > (int64_t(1) << 28)) {
+    if (ucells > (int64_t(1) << 26)) {
         delete t;
-        pvdb::set_error("leaf bounding box too large for a dense leaf table (> 2^28 cells)");
+        pvdb::set_error("leaf bounding box too large (lower-node table > 2^26 cells)");
         return PVDB_ERANGE;
     }
-    t->table.assign((size_t)std::max<int64_t>(cells, 1), -1);
+    if (layout == 0) layout = cells <= (int64_t(1) << 24) ? 1 : 2;
+    if (layout == 1 && cells > (int64_t(1) << 28)) {
+        delete t;
+        pvdb::set_error("leaf bounding box too large for a one-level leaf table (> 2^28 cells)");
+        return PVDB_ERANGE;
+    }
+    t->two_level = layout == 2;
     t->origins.resize(3 * leaves.size());
     t->masks.resize(8 * leaves.size());
-    std::vector<int32_t> lower((size_t)t->cdim[0] * t->cdim[1] * t->cdim[2], 0);
+    t->lower_table.assign((size_t)ucells, -1);
+    if (!t->two_level) t->table.assign((size_t)std::max<int64_t>(cells, 1), -1);
     int64_t nact = 0;
     for (size_t i = 0; i < leaves.size(); ++i) {
         const LeafRec& L = leaves[i];
-        int32_t a = (L.o[0] >> 3) - t->lo[0], b = (L.o[1] >> 3) - t->lo[1], c = (L.o[2] >> 3) - t->lo[2];
-        t->table[((size_t)a * t->dim[1] + b) * t->dim[2] + c] = (int32_t)i;
-        lower[((size_t)(a >> 4) * t->cdim[1] + (b >> 4)) * t->cdim[2] + (c >> 4)] += 1;
+        const int32_t lx = L.o[0] >> 3, ly = L.o[1] >> 3, lz = L.o[2] >> 3;
+        const size_t u = ((size_t)((lx >> 4) - t->ulo[0]) * t->udim[1] + ((ly >> 4) - t->ulo[1])) *
+                             t->udim[2] + ((lz >> 4) - t->ulo[2]);
+        if (t->lower_table[u] < 0) {
+            t->lower_table[u] = (int32_t)t->nlower++;
+            if (t->two_level) t->lower_leaves.resize((size_t)t->nlower * 4096, -1);
+        }
+        if (t->two_level) {
+            const int off = ((lx & 15) << 8) | ((ly & 15) << 4) | (lz & 15);
+            t->lower_leaves[(size_t)t->lower_table[u] * 4096 + off] = (int32_t)i;
+        } else {
+            t->table[((size_t)(lx - t->lo[0]) * t->dim[1] + (ly - t->lo[1])) * t->dim[2] +
+                     (lz - t->lo[2])] = (int32_t)i;
+        }
         for (int d = 0; d < 3; ++d) t->origins[3 * i + d] = L.o[d];
         for (int w = 0; w < 8; ++w) {
             t->masks[8 * i + w] = L.m[w];
@@ -123,21 +155,18 @@ int finalize(std::vector<LeafRec>& leaves, pvdb_topology** out) {
         }
     }
     t->nactive = nact;
-    hipError_t e = hipGetDevice(&t->device);
-    size_t tb = t->table.size() * sizeof(int32_t);
-    if (e == hipSuccess) e = hipMalloc(&t->d_table, tb);
-    if (e == hipSuccess) e = hipMalloc(&t->d_masks, std::max<size_t>(t->masks.size(), 8) * 8);
-    if (e == hipSuccess) e = hipMalloc(&t->d_origins, std::max<size_t>(t->origins.size(), 3) * 4);
-    if (e == hipSuccess) e = hipMalloc(&t->d_lower, std::max<size_t>(lower.size(), 1) * 4);
-    if (e == hipSuccess) e = hipMemcpy(t->d_table, t->table.data(), tb, hipMemcpyHostToDevice);
-    if (e == hipSuccess && !t->masks.empty())
-        e = hipMemcpy(t->d_masks, t->masks.data(), t->masks.size() * 8, hipMemcpyHostToDevice);
-    if (e == hipSuccess && !t->origins.empty())
-        e = hipMemcpy(t->d_origins, t->origins.data(), t->origins.size() * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess && !lower.empty())
-        e = hipMemcpy(t->d_lower, lower.data(), lower.size() * 4, hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        int rc = pvdb::hip_fail(e, "topology upload");
+    int rc = hipGetDevice(&t->device) == hipSuccess ? PVDB_OK : PVDB_EHIP;
+    if (rc == PVDB_OK) rc = upload(t->table, &t->d_table);
+    if (rc == PVDB_OK) rc = upload(t->origins, &t->d_origins);
+    if (rc == PVDB_OK) rc = upload(t->lower_table, &t->d_lower);
+    if (rc == PVDB_OK) rc = upload(t->lower_leaves, &t->d_lower_leaves);
+    if (rc == PVDB_OK) {
+        hipError_t e = hipMalloc(&t->d_masks, std::max<size_t>(t->masks.size(), 8) * 8);
+        if (e == hipSuccess && !t->masks.empty())
+            e = hipMemcpy(t->d_masks, t->masks.data(), t->masks.size() * 8, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = pvdb::hip_fail(e, "topology upload");
+    }
+    if (rc != PVDB_OK) {
         pvdb_topo_destroy(t);
         return rc;
     }
@@ -153,16 +182,21 @@ TopoView view_of(const pvdb_topology* t) {
     v.table = t->d_table;
     v.masks = t->d_masks;
     v.origins = t->d_origins;
-    v.lower_count = t->d_lower;
+    v.lower_table = t->d_lower;
+    v.lower_leaves = t->d_lower_leaves;
     v.lo0 = t->lo[0];
     v.lo1 = t->lo[1];
     v.lo2 = t->lo[2];
     v.d0 = t->dim[0];
     v.d1 = t->dim[1];
     v.d2 = t->dim[2];
-    v.c0 = t->cdim[0];
-    v.c1 = t->cdim[1];
-    v.c2 = t->cdim[2];
+    v.ulo0 = t->ulo[0];
+    v.ulo1 = t->ulo[1];
+    v.ulo2 = t->ulo[2];
+    v.e0 = t->udim[0];
+    v.e1 = t->udim[1];
+    v.e2 = t->udim[2];
+    v.two_level = t->two_level;
     v.pad_ = 0;
     v.nleaf = t->nleaf;
     return v;
@@ -211,14 +245,15 @@ int pvdb_topo_create_from_mask(const uint8_t* mask, int32_t rx, int32_t ry, int3
                 }
                 if (any) leaves.push_back(L);
             }
-    return finalize(leaves, out);
+    return finalize(leaves, 0, out);
 }
 
 int pvdb_topo_create_from_leaves(const int32_t* origins, const uint64_t* masks, int64_t nleaf,
-                                 pvdb_topology** out) {
+                                 int32_t layout, pvdb_topology** out) {
     PVDB_CHECK_ARG(out != nullptr, "out is null");
     PVDB_CHECK_ARG(nleaf >= 0, "nleaf must be >= 0");
     PVDB_CHECK_ARG(nleaf == 0 || (origins && masks), "null leaf arrays");
+    PVDB_CHECK_ARG(layout >= 0 && layout <= 2, "layout must be 0 (auto), 1 or 2");
     std::vector<LeafRec> leaves((size_t)nleaf);
     for (int64_t i = 0; i < nleaf; ++i) {
         for (int d = 0; d < 3; ++d) {
@@ -227,7 +262,7 @@ int pvdb_topo_create_from_leaves(const int32_t* origins, const uint64_t* masks, 
         }
         for (int w = 0; w < 8; ++w) leaves[i].m[w] = masks[8 * i + w];
     }
-    return finalize(leaves, out);
+    return finalize(leaves, layout, out);
 }
 
 int pvdb_topo_destroy(pvdb_topology* t) {
@@ -239,19 +274,21 @@ int pvdb_topo_destroy(pvdb_topology* t) {
     if (t->d_masks) hipFree(t->d_masks);
     if (t->d_origins) hipFree(t->d_origins);
     if (t->d_lower) hipFree(t->d_lower);
+    if (t->d_lower_leaves) hipFree(t->d_lower_leaves);
     if (dev != t->device) hipSetDevice(dev);
     delete t;
     return PVDB_OK;
 }
 
 int pvdb_topo_info(const pvdb_topology* t, int64_t* nleaf, int64_t* nactive, int32_t* lo3,
-                   int32_t* dim3) {
+                   int32_t* dim3, int32_t* two_level) {
     if (!t) {
         pvdb::set_error("null topology");
         return PVDB_EHANDLE;
     }
     if (nleaf) *nleaf = t->nleaf;
     if (nactive) *nactive = t->nactive;
+    if (two_level) *two_level = t->two_level;
     for (int d = 0; d < 3; ++d) {
         if (lo3) lo3[d] = t->lo[d];
         if (dim3) dim3[d] = t->dim[d];
@@ -279,22 +316,32 @@ int pvdb_topo_slots(const pvdb_topology* t, const int32_t* ijk, int64_t n, int64
     PVDB_CHECK_ARG(n >= 0 && (n == 0 || (ijk && slots)), "bad query arrays");
     for (int64_t q = 0; q < n; ++q) {
         const int32_t i = ijk[3 * q], j = ijk[3 * q + 1], k = ijk[3 * q + 2];
-        const int64_t a = (int64_t)(i >> 3) - t->lo[0], b = (int64_t)(j >> 3) - t->lo[1],
-                      c = (int64_t)(k >> 3) - t->lo[2];
         int64_t s = -1;
         bool act = false;
-        if (a >= 0 && b >= 0 && c >= 0 && a < t->dim[0] && b < t->dim[1] && c < t->dim[2]) {
-            const int32_t l = t->table[((size_t)a * t->dim[1] + b) * t->dim[2] + c];
-            if (l >= 0) {
-                const int off = ((i & 7) << 6) | ((j & 7) << 3) | (k & 7);
-                s = (int64_t)l * 512 + off;
-                act = (t->masks[8 * (size_t)l + (off >> 6)] >> (off & 63)) & 1ull;
+        int32_t l = -1;
+        const int64_t ua = (int64_t)(i >> 7) - t->ulo[0], ub = (int64_t)(j >> 7) - t->ulo[1],
+                      uc = (int64_t)(k >> 7) - t->ulo[2];
+        if (ua >= 0 && ub >= 0 && uc >= 0 && ua < t->udim[0] && ub < t->udim[1] && uc < t->udim[2]) {
+            const int32_t low = t->lower_table[((size_t)ua * t->udim[1] + ub) * t->udim[2] + uc];
+            if (low >= 0) {
+                if (t->two_level) {
+                    const int off = (((i & 127) >> 3) << 8) | (((j & 127) >> 3) << 4) | ((k & 127) >> 3);
+                    l = t->lower_leaves[(size_t)low * 4096 + off];
+                } else {
+                    const int64_t a = (int64_t)(i >> 3) - t->lo[0], b = (int64_t)(j >> 3) - t->lo[1],
+                                  c = (int64_t)(k >> 3) - t->lo[2];
+                    l = t->table[((size_t)a * t->dim[1] + b) * t->dim[2] + c];
+                }
             }
+        }
+        if (l >= 0) {
+            const int off = ((i & 7) << 6) | ((j & 7) << 3) | (k & 7);
+            s = (int64_t)l * 512 + off;
+            act = (t->masks[8 * (size_t)l + (off >> 6)] >> (off & 63)) & 1ull;
         }
         slots[q] = s;
         if (active) active[q] = act ? 1 : 0;
     }
-    (void)floor_div8;
     return PVDB_OK;
 }
 

@@ -26,9 +26,9 @@ F32 = C.c_float
 _SIGS = {
     "pvdb_abi_version": [],
     "pvdb_topo_create_from_mask": [P, I32, I32, I32, C.POINTER(P)],
-    "pvdb_topo_create_from_leaves": [P, P, I64, C.POINTER(P)],
+    "pvdb_topo_create_from_leaves": [P, P, I64, I32, C.POINTER(P)],
     "pvdb_topo_destroy": [P],
-    "pvdb_topo_info": [P, C.POINTER(I64), C.POINTER(I64), P, P],
+    "pvdb_topo_info": [P, C.POINTER(I64), C.POINTER(I64), P, P, P],
     "pvdb_topo_leaves": [P, P, P],
     "pvdb_topo_slots": [P, P, I64, P, P],
     "pvdb_topo_device_arrays": [P, P, P, P],

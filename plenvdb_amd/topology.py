@@ -23,7 +23,9 @@ class Topology:
         self.reso = [int(r) for r in reso]
         nleaf, nact = C.c_int64(), C.c_int64()
         lo, dim = lib.i3([0, 0, 0]), lib.i3([0, 0, 0])
-        lib.call("pvdb_topo_info", self._h, C.byref(nleaf), C.byref(nact), lo, dim)
+        two = (C.c_int32 * 1)()
+        lib.call("pvdb_topo_info", self._h, C.byref(nleaf), C.byref(nact), lo, dim, two)
+        self.two_level = bool(two[0])
         self.nleaf = int(nleaf.value)
         self.nactive = int(nact.value)
         self.leaf_lo = [lo[i] for i in range(3)]
@@ -50,14 +52,16 @@ class Topology:
         return cls.from_mask(np.ones([int(r) for r in reso], dtype=np.uint8))
 
     @classmethod
-    def from_leaves(cls, origins, masks, reso) -> "Topology":
+    def from_leaves(cls, origins, masks, reso, layout: int = 0) -> "Topology":
+        """layout 0 automatic, 1 one dense leaf table, 2 two-level tables."""
         o = np.ascontiguousarray(np.asarray(origins, dtype=np.int32).reshape(-1, 3))
         m = np.ascontiguousarray(np.asarray(masks, dtype=np.uint64).reshape(-1, 8))
         if len(o) != len(m):
             raise ValueError("origins and masks disagree in length")
         lib.load()
         h = C.c_void_p()
-        lib.call("pvdb_topo_create_from_leaves", lib.hptr(o), lib.hptr(m), len(o), C.byref(h))
+        lib.call("pvdb_topo_create_from_leaves", lib.hptr(o), lib.hptr(m), len(o), int(layout),
+                 C.byref(h))
         return cls(h, reso)
 
     # -- queries ----------------------------------------------------------------

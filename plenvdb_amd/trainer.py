@@ -1,0 +1,82 @@
+"""Training-step drivers over a synthetic scene (see scenes.py).
+
+`PlainTrainer` runs the reference's iteration (plenvdb/run.py:541-598) on the
+stage-per-kernel model of dvgo.py: forward, zero_grad, MSE loss, backward, the
+MLP's MaskedAdam step, the VDB Adam step (skip-zero-grad for both grids, as
+the fine stage configures, configs/default.py:74) and the lr decay.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import scenes
+from .dvgo import DirectVoxGO
+from .masked_adam import MaskedAdam, VDBAdam
+from .topology import Topology
+
+
+def build_topology(scene: dict) -> Topology:
+    return Topology.from_mask(scene["mask"])
+
+
+def init_values(scene: dict, topo: Topology):
+    """Seeded per-slot density / colour values for a topology."""
+    den, col = scenes.grid_values(scene["cfg"], topo.nslots)
+    return den, col
+
+
+class PlainTrainer:
+    def __init__(self, scene: dict, topo: Topology | None = None, lr_grid=0.1, lr_mlp=1e-3,
+                 lrate_decay=20, device="cuda"):
+        self.scene = scene
+        cfg = scene["cfg"]
+        self.topo = topo if topo is not None else build_topology(scene)
+        self.model = DirectVoxGO(scene["xyz_min"], scene["xyz_max"], scene["reso"],
+                                 alpha_init=cfg.alpha_init, fast_color_thres=cfg.fast_color_thres,
+                                 rgbnet_dim=cfg.channels if cfg.use_mlp else 0,
+                                 voxel_size=scene["voxel_size"], topology=self.topo,
+                                 mlp_params=scene.get("mlp"), k0_dim=cfg.channels).to(device)
+        den, col = init_values(scene, self.topo)
+        self.model.density.grid.grid.copy_(torch.from_numpy(den).reshape(-1, 1))
+        self.model.k0.grid.grid.copy_(torch.from_numpy(col))
+        self.vdbopt = VDBAdam([
+            {"params": self.model.density, "lr": lr_grid, "skip_zero_grad": True},
+            {"params": self.model.k0, "lr": lr_grid, "skip_zero_grad": True},
+        ])
+        self.opt = None
+        if self.model.rgbnet is not None:
+            self.opt = MaskedAdam([{"params": self.model.rgbnet.parameters(), "lr": lr_mlp,
+                                    "skip_zero_grad": False}])
+        self.decay = 0.1 ** (1 / (lrate_decay * 1000))
+        dev = torch.device(device)
+        self.rays_o = torch.from_numpy(scene["rays_o"]).to(dev)
+        self.rays_d = torch.from_numpy(scene["rays_d"]).to(dev)
+        self.viewdirs = torch.from_numpy(scene["viewdirs"]).to(dev)
+        self.target = torch.from_numpy(scene["target"]).to(dev)
+        fpr = scene.get("far_per_ray")
+        self.far_per_ray = None if fpr is None else torch.from_numpy(fpr).to(dev)
+        self.render_kwargs = dict(near=scene["near"], far=1e9, stepsize=scene["stepsize"],
+                                  bg=scene["bg"], far_per_ray=self.far_per_ray)
+        self.last = None
+
+    def forward_backward(self):
+        out = self.model(self.rays_o, self.rays_d, self.viewdirs, **self.render_kwargs)
+        if self.opt is not None:
+            self.opt.zero_grad(set_to_none=True)
+        self.vdbopt.zero_grad()
+        loss = torch.nn.functional.mse_loss(out["rgb_marched"], self.target)
+        loss.backward()
+        self.last = out
+        return loss, out
+
+    def step(self):
+        loss, out = self.forward_backward()
+        if self.opt is not None:
+            self.opt.step()
+        self.vdbopt.step()
+        if self.opt is not None:
+            for g in self.opt.param_groups:
+                g["lr"] = g["lr"] * self.decay
+        self.vdbopt.update_lr(self.decay)
+        return loss, out
