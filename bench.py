@@ -1,0 +1,237 @@
+"""Benchmark of the PlenVDB hot path on one MI355X.
+
+One step = one training iteration of the 160^3 configuration (64 spheres,
+8192 rays, 12-channel features + view-dependent MLP): ray marching with
+empty-space skip, density/colour trilinear gathers, alpha compositing, MLP,
+MSE loss, the full backward and the Adam steps (VDB grids + MLP).
+
+    python bench.py [--steps K] [--warmup W] [--path batched|plain] [--no-cpu]
+
+Prints one JSON line.  `value` is ray-samples per second of the fused batched
+path (a sample is a point whose density gather is evaluated: inside the bbox,
+in an active voxel, before the alpha/weight culls).  The plain FP32 path (the
+stage-per-kernel structure of the reference) is timed beside it, and the CPU
+baseline is the oracle's FP32 restatement of the reference step timed on the
+host cores on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "samples/s fwd+bwd, BF16+batched vs FP32 plain @<=1e-3 err; 800^2 fps"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c2")
+    ap.add_argument("--path", default="auto", choices=["auto", "batched", "plain"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-rays", type=int, default=8192)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a device copy of known size (counter calibration)")
+    return ap.parse_args()
+
+
+def unique_corner_voxels(xs, ys, zs) -> int:
+    """Number of distinct voxels among the 8 trilinear corners of the points."""
+    i0 = np.floor(np.stack([xs, ys, zs], 1)).astype(np.int64)
+    keys = []
+    for dx in (0, 1):
+        for dy in (0, 1):
+            for dz in (0, 1):
+                c = i0 + np.array([dx, dy, dz])
+                keys.append(((c[:, 0] + 16) << 40) | ((c[:, 1] + 16) << 20) | (c[:, 2] + 16))
+    return int(np.unique(np.concatenate(keys)).size)
+
+
+def kernel_bytes(name: str, n: int, uniq: int, channels: int) -> float:
+    """Algorithmic HBM bytes of one launch: each input/output element once, each
+    distinct grid voxel once (read; read+write for the atomic scatter)."""
+    c = channels
+    if name == "pvdb_trilinear_fwd":
+        return n * (12 + 4 * c) + uniq * 4 * c
+    if name == "pvdb_trilinear_bwd":
+        return n * (12 + 4 * c) + uniq * 4 * c * 2
+    return float("nan")
+
+
+def copy_probe():
+    import torch
+
+    n = 256 << 20
+    a = torch.empty(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n}))
+
+
+def bench_plain(args, scene):
+    import torch
+
+    from plenvdb_amd import lib
+    from plenvdb_amd.trainer import PlainTrainer
+
+    tr = PlainTrainer(scene)
+    for _ in range(args.warmup):
+        tr.step()
+    torch.cuda.synchronize()
+    samples = []
+    with lib.CallTimer() as timer:
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(args.steps):
+            _, out = tr.step()
+            samples.append(int(out["n_samples"]))
+        t1.record()
+        torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / args.steps
+    totals = timer.totals()
+    # per-launch workload of the colour scatter / gathers, from one more step
+    pts = tr.model.sample_ray(tr.rays_o, tr.rays_d, **tr.render_kwargs)[0]
+    keep = tr.model.mask_cache(pts)
+    idx = tr.model.density.wld2idx(pts[keep]).cpu().numpy()
+    final = out["ray_id"].numel()
+    return {
+        "ms_per_step": ms,
+        "samples_per_step": float(np.mean(samples)),
+        "totals": totals,
+        "points_density": idx,
+        "n_color": final,
+    }
+
+
+def roofline_from(totals, launches_bytes):
+    """Pick the kernel with the largest total device time among those with a byte model."""
+    best = None
+    for name, (tot_ms, cnt) in totals.items():
+        if name not in launches_bytes:
+            continue
+        avg_ms = tot_ms / cnt
+        if best is None or tot_ms > best[1]:
+            best = (name, tot_ms, avg_ms)
+    if best is None:
+        return None
+    name, _, avg_ms = best
+    b = launches_bytes[name]
+    achieved = b / (avg_ms * 1e-3) / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "algorithmic_bytes": b, "avg_launch_ms": round(avg_ms, 5)}
+
+
+def cpu_baseline(cfg_name: str, n_rays: int):
+    """Oracle FP32 restatement of the reference step (C kernels under OpenMP,
+    MLP/composite/loss in torch CPU), forward + backward + Adam, timed on the
+    host cores on n_rays rays of the same workload."""
+    import torch
+
+    from oracle import oracle
+    from plenvdb_amd import scenes
+
+    sc = scenes.make_scene(cfg_name, n_rays=n_rays)
+    origins, vm = oracle.leaves_from_mask(sc["mask"])
+    nslots = len(origins) * 512
+    den, col = scenes.grid_values(sc["cfg"], nslots)
+    sc.update(origins=origins, vmask=vm, density=den, color=col)
+    act = vm.reshape(-1).astype(np.uint8)
+    m_d, v_d = np.zeros_like(den), np.zeros_like(den)
+    m_c, v_c = np.zeros_like(col), np.zeros_like(col)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    torch.set_num_threads(cores)
+    times, samples = [], []
+    for it in range(6):
+        t = time.perf_counter()
+        out = oracle.reference_train_step(sc, prec="f32", with_mlp=sc["cfg"].use_mlp)
+        sz = oracle.adam_stepsz(0.1, 0.9, 0.99, it + 1)
+        gd = np.ascontiguousarray(out["grad_density"].astype(np.float32))
+        gc = np.ascontiguousarray(out["grad_color"].astype(np.float32))
+        oracle.vdb_adam(den, gd, m_d, v_d, act, 1, 1, sz, 1e-8, 0.9, 0.99)
+        oracle.vdb_adam(col, gc, m_c, v_c, act, 3, 1, sz, 1e-8, 0.9, 0.99)
+        times.append(time.perf_counter() - t)
+        samples.append(out["n_mask"])
+    dt = float(np.median(times[1:]))
+    return {"value": round(float(np.mean(samples)) / dt, 1), "unit": "samples/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{cfg_name} workload, {n_rays} of its rays, fwd+bwd+VDB Adam, median of 5 "
+                      f"steps after 1 warm-up ({int(np.mean(samples))} samples/step, "
+                      f"{dt * 1e3:.0f} ms/step)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU runs are not part "
+              "of this build", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    if args.copy_probe:
+        copy_probe()
+        return
+    from plenvdb_amd import scenes
+
+    scene = scenes.make_scene(args.config)
+    cfg = scene["cfg"]
+    path = "plain" if args.path == "auto" else args.path
+    res = {"plain": bench_plain(args, scene)}
+    main_res = res[path]
+    p = main_res["points_density"]
+    uniq_d = unique_corner_voxels(p[:, 0], p[:, 1], p[:, 2])
+    nb = {
+        "pvdb_trilinear_bwd": None,
+    }
+    # byte model: the colour scatter dominates; per launch it handles the final samples
+    n_col = main_res["n_color"]
+    nb["pvdb_trilinear_bwd"] = kernel_bytes("pvdb_trilinear_bwd", n_col, uniq_d, 12)
+    nb["pvdb_trilinear_fwd"] = kernel_bytes("pvdb_trilinear_fwd", n_col, uniq_d, 12)
+    roof = roofline_from(main_res["totals"], nb)
+    value = main_res["samples_per_step"] / (main_res["ms_per_step"] * 1e-3)
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(main_res["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if path == "plain" else "bf16",
+        "data": "synthetic",
+        "config": {"workload": f"{cfg.name}: {cfg.reso}^3 grid, {cfg.n_spheres} spheres, "
+                               f"{cfg.n_rays} rays, 12-ch features + MLP 39-128-128-3, "
+                               "fwd+bwd+Adam", "path": path,
+                   "samples_per_step": round(main_res["samples_per_step"], 1)},
+        "roofline": roof,
+        "paths": {k: {"ms_per_step": round(v["ms_per_step"], 4),
+                      "samples_per_s": round(v["samples_per_step"] / (v["ms_per_step"] * 1e-3), 1)}
+                  for k, v in res.items()},
+        "kernels_ms": {k: round(v[0] / args.steps, 4) for k, v in sorted(
+            main_res["totals"].items(), key=lambda kv: -kv[1][0])},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.config, args.cpu_rays)
+    else:
+        line["cpu_baseline"] = None
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -150,6 +150,73 @@ int pvdb_alpha2weight_bwd(const float* alpha, const float* weight, const float* 
                           const int64_t* i_end, int64_t n_rays, const float* grad_weights,
                           const float* grad_last, float* grad_alpha, void* stream);
 
+/* ---- fused (batched) training step ---------------------------------------
+ * One iteration of plenvdb/run.py:541-598 on the fine-stage model
+ * (dvgo.py:296-388 forward with the 12-ch feature grid and the 39-128-128-3
+ * rgbnet, MSE loss, backward through Raw2Alpha / Alphas2Weights /
+ * QueryVerticalInVDB, MaskedAdam on the MLP and VDBAdam stepmode 1 on both
+ * grids) as one fixed launch sequence on `stream`: no host synchronisation, no
+ * allocation (all scratch in `workspace`), so it can be captured in a hipGraph.
+ * Replaces, for that loop, the per-call round trips of DensityVDB/ColorVDB
+ * forward/backward (plenvdb.cpp:11-29,46-64) plus render_utils_cuda and
+ * adam_upd_cuda.  Payloads: density [slots] fp32, colour [slots][12] fp32 or
+ * bf16 (color_bf16), gradients and Adam moments fp32; MLP parameters fp32 in
+ * the order W0[39][128] b0[128] W1[128][128] b1[128] W2[128][3] b2[3] (W as
+ * Linear.weight.T, run.py:98-104).  Gradients are accumulated (callers zero
+ * them once); with do_adam the step applies Adam and re-zeroes the gradients
+ * of the leaves it touched. */
+typedef struct pvdb_step_desc {
+    const float* rays_o;      /* dev [n][3] */
+    const float* rays_d;      /* dev [n][3] */
+    const float* viewdirs;    /* dev [n][3] */
+    const float* target;      /* dev [n][3] */
+    const float* far_per_ray; /* dev [n] or NULL (far = 1e9, dvgo.py:284) */
+    int32_t n_rays;
+    int32_t color_bf16;
+    float xyz_min[3];
+    float xyz_max[3];
+    int32_t reso[3];
+    float near_, stepdist, act_shift, interval, fast_color_thres, bg;
+    float* density;
+    float* grad_density;
+    float* density_m;
+    float* density_v;
+    void* color;
+    float* grad_color;
+    float* color_m;
+    float* color_v;
+    float* mlp;
+    float* mlp_grad;
+    float* mlp_m;
+    float* mlp_v;
+    float stepsz_density, stepsz_color, step_size_mlp, eps, beta0, beta1;
+    int32_t do_adam;
+    int32_t reserved;
+    void* workspace; /* dev, pvdb_fused_workspace_bytes() bytes */
+    int64_t workspace_bytes;
+    int64_t capacity; /* max records (occupancy-kept samples) per step */
+    float* ray_out;   /* dev [n][4] rgb + T_last, or NULL */
+    float* loss;      /* dev [1] or NULL */
+} pvdb_step_desc;
+
+/* Workspace bytes for n_rays rays, `capacity` records and nleaf leaves (-1 on bad sizes). */
+int64_t pvdb_fused_workspace_bytes(int64_t n_rays, int64_t capacity, int64_t nleaf);
+/* Clears the workspace (touched-leaf flags, buffers); call once after allocating it. */
+int pvdb_fused_init(const pvdb_topology* topo, const pvdb_step_desc* desc, void* stream);
+int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* desc, void* stream);
+/* Device pointers of the per-sample records and per-ray counts inside the
+ * workspace (for inspection and tests). */
+int pvdb_fused_records(const pvdb_topology* topo, const pvdb_step_desc* desc, void** xyze,
+                       void** atwr, void** rgb, void** cinc, int32_t** count, int32_t** offset,
+                       int32_t** total);
+
+/* ---- self test --------------------------------------------------------------
+ * No counterpart in the reference: multiplies bf16 matrices with the MFMA
+ * fragment layouts the MLP kernels assume (A [16][32], B [32][16] -> D [16][16];
+ * A16 [16][16], B16 [16][16] -> D16), so a test can check them on the device. */
+int pvdb_mfma_selftest(const uint16_t* A, const uint16_t* B, float* D, const uint16_t* A16,
+                       const uint16_t* B16, float* D16, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

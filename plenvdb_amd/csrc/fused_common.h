@@ -1,0 +1,92 @@
+// Shared definitions of the fused (batched) training step.
+//
+// Pipeline of one step (all on one stream, no host synchronisation):
+//   k_ray_count    wave per ray: slab test + march + occupancy -> samples per ray
+//   k_scan         exclusive scan of the counts -> record offsets, total
+//   k_ray_march    wave per ray: march again, density gather, raw2alpha, alpha cull,
+//                  transmittance product-scan with early stop, weights, weight cull;
+//                  one record per occupancy-kept sample (contiguous per ray)
+//   k_mlp<false>   workgroup per 16-record group: colour gather + MFMA MLP -> rgb
+//   k_ray_composite wave per ray: prefix of w*rgb, ray colour, MSE loss and its
+//                  gradient per ray
+//   k_mlp<true>    workgroup per 16-record group: recompute, MFMA MLP backward,
+//                  alpha/raw2alpha adjoints, scatter into the grid gradients, dW
+//   k_dw_reduce, k_sparse_adam, dense Adam of the MLP, k_pack_mlp
+#pragma once
+
+#include "mfma_util.h"
+#include "pvdb_common.h"
+
+namespace pvdb {
+
+// Per-record flags (bits 30..31 of the packed ray field).
+constexpr uint32_t kAliveA = 1u << 30;  // in the alpha-kept set, before the early stop
+constexpr uint32_t kAliveW = 1u << 31;  // also weight-kept: gets colour, MLP and composite
+constexpr uint32_t kRayMask = (1u << 30) - 1;
+
+struct StepParams {
+    const float* rays_o;
+    const float* rays_d;
+    const float* viewdirs;
+    const float* target;       // [N][3]
+    const float* far_per_ray;  // may be null
+    int32_t n_rays;
+    int32_t color_bf16;        // colour payload stored as bf16
+    float mn[3], mx[3];        // scene bbox
+    float len[3], rm1[3];      // wld2idx: (p - mn) / len * rm1
+    float ms[3], mb[3];        // occupancy lookup: round(p * ms + mb)
+    float near_, far_, stepdist;
+    float act_shift, interval, thres, bg;
+    float loss_scale;          // 1 / (3 N): MSE mean over N rays x 3 channels
+    int64_t capacity;          // record capacity
+};
+
+// Records: structure of arrays, index = record id.
+struct Records {
+    float4* xyze;      // index-space x, y, z; e = exp(sigma + shift)
+    float4* atwr;      // alpha, T, w, ray|flags (as bits)
+    float4* rgb;       // MLP colour (rgb), written by the forward MLP pass
+    float4* cinc;      // inclusive prefix of w*rgb along the ray
+};
+
+struct RayBuffers {
+    int32_t* count;    // occupancy-kept samples per ray
+    int32_t* offset;   // exclusive prefix of count
+    int32_t* total;    // [1] total records
+    float* t_last;     // [N]
+    float4* ray_out;   // [N] rendered rgb + T_last
+    float4* ray_grad;  // [N] dL/drgb (3) + dL/dT_last
+    float4* ray_ctot;  // [N] sum of w*rgb (3)
+    float* loss;       // [1]
+};
+
+// Packed bf16 MLP weights for MFMA (row-major, zero padded):
+//   w0t [128][64]  W0^T (in 0..38 used)       w1t [128][128] W1^T
+//   w2t [16][128]  W2^T (rows 0..2 used)      w2  [128][32]  W2 (cols 0..2 used)
+//   w1  [128][128] W1                          w0  [16][128]  W0 rows 0..15 (0..11 used)
+// and fp32 biases b0[128], b1[128], b2[3].
+struct MlpPacked {
+    const uint16_t* w0t;
+    const uint16_t* w1t;
+    const uint16_t* w2t;
+    const uint16_t* w2;
+    const uint16_t* w1;
+    const uint16_t* w0;
+    const float* b0;
+    const float* b1;
+    const float* b2;
+};
+
+// fp32 gradient layout of the MLP (the layout of the fp32 master parameters):
+//   W0 [39][128] | b0 [128] | W1 [128][128] | b1 [128] | W2 [128][3] | b2 [3]
+constexpr int kW0 = 0;
+constexpr int kB0 = kW0 + 39 * 128;
+constexpr int kW1 = kB0 + 128;
+constexpr int kB1 = kW1 + 128 * 128;
+constexpr int kW2 = kB1 + 128;
+constexpr int kB2 = kW2 + 128 * 3;
+constexpr int kMlpParams = kB2 + 3;  // 22019
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+}  // namespace pvdb

@@ -1,0 +1,271 @@
+// Fused step, ray side: counting march, offsets scan, the forward march that
+// writes per-sample records, and the per-ray composite / loss pass.
+//
+// One wavefront marches one ray, 64 consecutive steps per iteration (lane l
+// takes step base+l), so sample positions are the reference's
+// p_i = (o + d*t_min) + (d/|d|)*(stepdist*i) (render_utils_kernel.cu:57-79,
+// 166-194) and the occupancy decision is the MaskGrid rule round(p*s+b)
+// (render_utils_kernel.cu:373-390) on the VDB value masks.  Live lanes are
+// compacted with ballot + mbcnt.  Transmittance is an exclusive product scan
+// over the wave carried from chunk to chunk; the early stop after the first
+// sample with T < 1e-3 and the alpha / weight culls follow
+// render_utils_kernel.cu:576-603 and dvgo.py:319-335.
+#include "fused_common.h"
+#include "gather.h"
+#include "plenvdb_hip.h"
+
+namespace pvdb {
+TopoView view_of(const pvdb_topology* t);
+}
+
+using namespace pvdb;
+
+namespace {
+
+struct RaySetup {
+    float sx, sy, sz;  // o + d * t_min
+    float ux, uy, uz;  // d / |d|
+    int64_t n_steps;
+    float t_min, t_max;
+};
+
+// Ray bounds with the reference's operation order (render_utils_kernel.cu:11-79).
+__device__ __forceinline__ RaySetup ray_setup(const StepParams& P, int r) {
+    const float ox = P.rays_o[3 * r], oy = P.rays_o[3 * r + 1], oz = P.rays_o[3 * r + 2];
+    const float dx = P.rays_d[3 * r], dy = P.rays_d[3 * r + 1], dz = P.rays_d[3 * r + 2];
+    const float vx = dx == 0.f ? 1e-6f : dx;
+    const float vy = dy == 0.f ? 1e-6f : dy;
+    const float vz = dz == 0.f ? 1e-6f : dz;
+    const float ax = (P.mx[0] - ox) / vx, ay = (P.mx[1] - oy) / vy, az = (P.mx[2] - oz) / vz;
+    const float bx = (P.mn[0] - ox) / vx, by = (P.mn[1] - oy) / vy, bz = (P.mn[2] - oz) / vz;
+    const float fr = P.far_per_ray ? P.far_per_ray[r] : P.far_;
+    const float lo = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));
+    const float hi = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));
+    RaySetup s;
+    s.t_min = fmaxf(fminf(lo, fr), P.near_);
+    s.t_max = fmaxf(fminf(hi, fr), P.near_);
+    const float rn = sqrtf(dx * dx + dy * dy + dz * dz);
+    const float q = ceilf(((s.t_max - s.t_min) * rn) / P.stepdist);
+    s.n_steps = (int64_t)(q > 1.f ? q : 1.f);
+    s.sx = ox + dx * s.t_min;
+    s.sy = oy + dy * s.t_min;
+    s.sz = oz + dz * s.t_min;
+    s.ux = dx / rn;
+    s.uy = dy / rn;
+    s.uz = dz / rn;
+    return s;
+}
+
+struct StepPoint {
+    float px, py, pz;
+    bool keep;  // inside the bbox and in an active voxel
+};
+
+__device__ __forceinline__ StepPoint step_point(const StepParams& P, const TopoView& t,
+                                                const RaySetup& s, int64_t i) {
+    StepPoint sp;
+    const float dist = P.stepdist * (float)i;
+    sp.px = s.sx + s.ux * dist;
+    sp.py = s.sy + s.uy * dist;
+    sp.pz = s.sz + s.uz * dist;
+    const bool out = (P.mn[0] > sp.px) | (P.mn[1] > sp.py) | (P.mn[2] > sp.pz) |
+                     (P.mx[0] < sp.px) | (P.mx[1] < sp.py) | (P.mx[2] < sp.pz);
+    sp.keep = false;
+    if (i < s.n_steps && !out) {
+        const int a = (int)roundf(sp.px * P.ms[0] + P.mb[0]);
+        const int b = (int)roundf(sp.py * P.ms[1] + P.mb[1]);
+        const int c = (int)roundf(sp.pz * P.ms[2] + P.mb[2]);
+        sp.keep = voxel_active(t, a, b, c);
+    }
+    return sp;
+}
+
+__device__ __forceinline__ int mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// Counting march: occupancy-kept samples per ray.
+__global__ void __launch_bounds__(256) k_ray_count(StepParams P, TopoView t, RayBuffers R) {
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= P.n_rays) return;
+    const int lane = lane_id();
+    const RaySetup s = ray_setup(P, r);
+    int count = 0;
+    for (int64_t base = 0; base < s.n_steps; base += 64) {
+        const StepPoint sp = step_point(P, t, s, base + lane);
+        count += __popcll(__ballot(sp.keep));
+    }
+    if (lane == 0) R.count[r] = count;
+}
+
+// Exclusive scan of the per-ray counts by one workgroup of 1024 threads.
+__global__ void __launch_bounds__(1024) k_scan(RayBuffers R, int n) {
+    __shared__ int part[1024];
+    const int tid = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b = tid * per, e = min(n, b + per);
+    int sum = 0;
+    for (int i = b; i < e; ++i) sum += R.count[i];
+    part[tid] = sum;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {
+        const int v = tid >= d ? part[tid - d] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int run = part[tid] - sum;
+    for (int i = b; i < e; ++i) {
+        R.offset[i] = run;
+        run += R.count[i];
+    }
+    if (tid == 1023) R.total[0] = part[1023];
+}
+
+// Forward march: one record per occupancy-kept sample.
+__global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, const float* density,
+                                                   RayBuffers R, Records rec) {
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= P.n_rays) return;
+    const int lane = lane_id();
+    const RaySetup s = ray_setup(P, r);
+    const int64_t off = R.offset[r];
+    double Tc = 1.0;
+    bool stopped = false;
+    int written = 0;
+    for (int64_t base = 0; base < s.n_steps; base += 64) {
+        const StepPoint sp = step_point(P, t, s, base + lane);
+        const uint64_t km = __ballot(sp.keep);
+        if (km == 0) continue;
+        const int idx = written + mbcnt(km);
+        written += __popcll(km);
+        float x = 0.f, y = 0.f, z = 0.f, e = 0.f, alpha = 0.f;
+        const bool live = sp.keep && !stopped;
+        if (live) {
+            x = ((sp.px - P.mn[0]) / P.len[0]) * P.rm1[0];
+            y = ((sp.py - P.mn[1]) / P.len[1]) * P.rm1[1];
+            z = ((sp.pz - P.mn[2]) / P.len[2]) * P.rm1[2];
+            const Cell cell = make_cell(x, y, z);
+            LeafCache cache;
+            const float sigma = gather_density(t, density, cell, cache);
+            e = expf(sigma + P.act_shift);
+            alpha = 1.f - powf(1.f + e, -P.interval);
+        }
+        const bool akeep = live && (P.thres > 0.f ? alpha > P.thres : true);
+        // inclusive product of (1 - alpha) over alpha-kept lanes, in step order
+        const double f = akeep ? 1.0 - (double)alpha : 1.0;
+        double incl = f;
+        for (int d = 1; d < 64; d <<= 1) {
+            const double o = __shfl_up(incl, d, 64);
+            if (lane >= d) incl *= o;
+        }
+        double excl = __shfl_up(incl, 1, 64);
+        if (lane == 0) excl = 1.0;
+        const double Tb = Tc * excl, Ta = Tc * incl;
+        const uint64_t stopm = __ballot(akeep && Ta < 1e-3);
+        const int stop_lane = stopm ? __ffsll((unsigned long long)stopm) - 1 : 64;
+        const bool alive_a = akeep && lane <= stop_lane;
+        const float T = (float)Tb;
+        const float w = T * alpha;
+        const bool alive_w = alive_a && (P.thres > 0.f ? w > P.thres : true);
+        if (sp.keep) {
+            const int64_t q = off + idx;
+            if (q < P.capacity) {
+                uint32_t bits = (uint32_t)r | (alive_a ? kAliveA : 0u) | (alive_w ? kAliveW : 0u);
+                rec.xyze[q] = make_float4(x, y, z, alive_a ? e : 0.f);
+                rec.atwr[q] = make_float4(alive_a ? alpha : 0.f, alive_a ? T : 0.f,
+                                          alive_w ? w : 0.f, __uint_as_float(bits));
+            }
+        }
+        if (!stopped) {
+            if (stop_lane < 64) {
+                Tc = __shfl(Ta, stop_lane, 64);
+                stopped = true;
+            } else {
+                Tc = __shfl(Ta, 63, 64);
+            }
+        }
+    }
+    if (lane == 0) R.t_last[r] = (float)Tc;
+}
+
+// Per-ray composite: inclusive prefix of w*rgb (kept for the alpha adjoint),
+// ray colour = sum + T_last*bg (dvgo.py:365-370), MSE loss and its gradient.
+__global__ void __launch_bounds__(256) k_ray_composite(StepParams P, RayBuffers R, Records rec) {
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= P.n_rays) return;
+    const int lane = lane_id();
+    const int64_t off = R.offset[r];
+    const int cnt = R.count[r];
+    float cr = 0.f, cg = 0.f, cb = 0.f;
+    for (int base = 0; base < cnt; base += 64) {
+        const int j = base + lane;
+        float vr = 0.f, vg = 0.f, vb = 0.f;
+        const int64_t q = off + j;
+        const bool valid = j < cnt && q < P.capacity;
+        if (valid) {
+            const float4 a = rec.atwr[q];
+            if (__float_as_uint(a.w) & kAliveW) {
+                const float4 c = rec.rgb[q];
+                vr = a.z * c.x;
+                vg = a.z * c.y;
+                vb = a.z * c.z;
+            }
+        }
+        for (int d = 1; d < 64; d <<= 1) {
+            const float o0 = __shfl_up(vr, d, 64), o1 = __shfl_up(vg, d, 64),
+                        o2 = __shfl_up(vb, d, 64);
+            if (lane >= d) {
+                vr += o0;
+                vg += o1;
+                vb += o2;
+            }
+        }
+        if (valid) rec.cinc[q] = make_float4(cr + vr, cg + vg, cb + vb, 0.f);
+        cr += __shfl(vr, 63, 64);
+        cg += __shfl(vg, 63, 64);
+        cb += __shfl(vb, 63, 64);
+    }
+    if (lane == 0) {
+        const float tl = R.t_last[r];
+        const float o0 = cr + tl * P.bg, o1 = cg + tl * P.bg, o2 = cb + tl * P.bg;
+        const float d0 = o0 - P.target[3 * r], d1 = o1 - P.target[3 * r + 1],
+                    d2 = o2 - P.target[3 * r + 2];
+        const float g0 = 2.f * d0 * P.loss_scale, g1 = 2.f * d1 * P.loss_scale,
+                    g2 = 2.f * d2 * P.loss_scale;
+        R.ray_out[r] = make_float4(o0, o1, o2, tl);
+        R.ray_grad[r] = make_float4(g0, g1, g2, P.bg * (g0 + g1 + g2));
+        R.ray_ctot[r] = make_float4(cr, cg, cb, 0.f);
+        atomicAdd(R.loss, (d0 * d0 + d1 * d1 + d2 * d2) * P.loss_scale);
+    }
+}
+
+}  // namespace
+
+namespace pvdb {
+
+int launch_ray_count(const StepParams& P, const TopoView& t, const RayBuffers& R, hipStream_t s) {
+    hipLaunchKernelGGL(k_ray_count, dim3(pvdb_blocks(P.n_rays, 4)), dim3(256), 0, s, P, t, R);
+    PVDB_LAUNCHED("k_ray_count");
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, R, P.n_rays);
+    PVDB_LAUNCHED("k_scan");
+    return PVDB_OK;
+}
+
+int launch_ray_march(const StepParams& P, const TopoView& t, const float* density,
+                     const RayBuffers& R, const Records& rec, hipStream_t s) {
+    hipLaunchKernelGGL(k_ray_march, dim3(pvdb_blocks(P.n_rays, 4)), dim3(256), 0, s, P, t, density,
+                       R, rec);
+    PVDB_LAUNCHED("k_ray_march");
+    return PVDB_OK;
+}
+
+int launch_ray_composite(const StepParams& P, const RayBuffers& R, const Records& rec,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_ray_composite, dim3(pvdb_blocks(P.n_rays, 4)), dim3(256), 0, s, P, R, rec);
+    PVDB_LAUNCHED("k_ray_composite");
+    return PVDB_OK;
+}
+
+}  // namespace pvdb

@@ -1,0 +1,593 @@
+// Fused step, sample side: the view-dependent RGB MLP (39-128-128-3, ReLU,
+// sigmoid; plenvdb/lib/dvgo.py:91-109,354-362) on MFMA, forward and backward,
+// over groups of 16 consecutive records.
+//
+// A workgroup of 4 wavefronts takes a group; wave w owns hidden features
+// [32w, 32w+32) of both hidden layers, so each wave's slices of the weights
+// stay in registers for the whole kernel and activations are exchanged through
+// LDS.  Activations are computed transposed (features x samples):
+//   H0^T = relu(W0^T X^T + b0),  H1^T = relu(W1^T H0^T + b1),  L^T = W2^T H1^T + b2
+// with v_mfma_f32_16x16x32_bf16 (samples are the N dimension).  The backward
+// pass recomputes the forward, forms dL/dlogit from the per-ray loss gradient,
+// runs g_H1^T = W2 g_L^T, g_H0^T = W1 g_H1^T (masked by the ReLUs),
+// g_X^T = W0[0:12] g_H0^T, and accumulates dW with v_mfma_f32_16x16x16_bf16
+// over the samples (K = 16) into per-wave register tiles that are written once
+// per workgroup.  The same pass evaluates the Alphas2Weights / raw2alpha
+// adjoints (render_utils_kernel.cu:653-676, 506-517) and scatters the density
+// and colour gradients with the "leaf must exist" rule (densityvdb.cu:16-27,
+// colorvdb.cu:28-37).
+#include "fused_common.h"
+#include "gather.h"
+#include "plenvdb_hip.h"
+
+namespace pvdb {
+TopoView view_of(const pvdb_topology* t);
+}
+
+using namespace pvdb;
+
+namespace {
+
+constexpr int XM = 72;   // sample-major stride of X (64 + 8)
+constexpr int SM = 136;  // sample-major stride of 128-wide activations (128 + 8)
+constexpr int FM = 20;   // feature-major stride (16 samples + 4)
+constexpr int GM = 40;   // sample-major stride of dL/dlogit (32 + 8)
+
+__device__ __forceinline__ bf16x8 ld_frag(const uint16_t* p) {
+    const uint4 v = *reinterpret_cast<const uint4*>(p);
+    return __builtin_bit_cast(bf16x8, v);
+}
+
+__device__ __forceinline__ s16x4 ld_frag4(const uint16_t* p) {
+    const uint2 v = *reinterpret_cast<const uint2*>(p);
+    return __builtin_bit_cast(s16x4, v);
+}
+
+__device__ __forceinline__ void st4(uint16_t* p, float a, float b, float c, float d) {
+    uint2 v;
+    v.x = (uint32_t)f2bf_bits(a) | ((uint32_t)f2bf_bits(b) << 16);
+    v.y = (uint32_t)f2bf_bits(c) | ((uint32_t)f2bf_bits(d) << 16);
+    *reinterpret_cast<uint2*>(p) = v;
+}
+
+struct RecInfo {
+    float x, y, z, e;
+    float alpha, T, w;
+    uint32_t bits;
+};
+
+struct Smem {
+    uint16_t Xs[16 * XM];
+    uint16_t Xf[64 * FM];
+    uint16_t H0s[16 * SM];
+    uint16_t H0f[128 * FM];
+    uint16_t H1s[16 * SM];
+    uint16_t H1f[128 * FM];
+    uint16_t GLs[16 * GM];
+    uint16_t GLf[16 * FM];
+    uint16_t G1s[16 * SM];
+    uint16_t G1f[128 * FM];
+    uint16_t G0s[16 * SM];
+    uint16_t G0f[128 * FM];
+    float GXp[4][16][16];
+    float gsig[16];
+    float rgb[16][4];
+    RecInfo ri[16];
+    float4 rayg[16];
+    float4 ctot[16];
+    float4 cinc[16];
+    float tlast[16];
+};
+
+__device__ __forceinline__ float sigmoidf_(float x) { return 1.f / (1.f + expf(-x)); }
+
+template <bool BWD, bool BF16COL>
+__global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const void* color,
+                                             MlpPacked W, RayBuffers R, Records rec,
+                                             float* __restrict__ grad_density,
+                                             float* __restrict__ grad_color,
+                                             uint8_t* __restrict__ leaf_touched,
+                                             float* __restrict__ dw_partial) {
+    __shared__ Smem sm;
+    const int tid = threadIdx.x;
+    const int wv = tid >> 6, lane = tid & 63, r16 = lane & 15, g = lane >> 4;
+    const int f0 = 32 * wv;  // first hidden feature owned by this wave
+
+    // ---- weight slices -> registers (bf16 fragments) ----
+    bf16x8 A0[2][2], A1[2][4], A2[4];
+    bf16x8 Bw2[2], Bw1[2][4], Bw0;
+    float b0v[2][4], b1v[2][4], b2v[3];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+        const int row = f0 + 16 * mt + r16;
+#pragma unroll
+        for (int kt = 0; kt < 2; ++kt) A0[mt][kt] = ld_frag(W.w0t + row * 64 + 32 * kt + 8 * g);
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) A1[mt][kt] = ld_frag(W.w1t + row * 128 + 32 * kt + 8 * g);
+        if (BWD) {
+            Bw2[mt] = ld_frag(W.w2 + row * 32 + 8 * g);
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt) Bw1[mt][kt] = ld_frag(W.w1 + row * 128 + 32 * kt + 8 * g);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            b0v[mt][i] = W.b0[f0 + 16 * mt + 4 * g + i];
+            b1v[mt][i] = W.b1[f0 + 16 * mt + 4 * g + i];
+        }
+    }
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) A2[kt] = ld_frag(W.w2t + r16 * 128 + 32 * kt + 8 * g);
+    if (BWD) Bw0 = ld_frag(W.w0 + r16 * 128 + f0 + 8 * g);
+    b2v[0] = W.b2[0];
+    b2v[1] = W.b2[1];
+    b2v[2] = W.b2[2];
+
+    // ---- dW accumulators (BWD) ----
+    f32x4 dW0[4][2], dW1[8][2], dW2[2];
+    float db1[2][4], db2[3];
+    if (BWD) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) dW0[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) dW1[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+        dW2[0] = dW2[1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) db1[a][i] = 0.f;
+        db2[0] = db2[1] = db2[2] = 0.f;
+    }
+
+    const int64_t total = min((int64_t)R.total[0], P.capacity);
+    for (int64_t grp = blockIdx.x;; grp += gridDim.x) {
+        const int64_t base = grp * 16;
+        if (base >= total) break;  // uniform over the workgroup
+        // ---- 1. records of the group ----
+        if (tid < 16) {
+            const int64_t q = base + tid;
+            RecInfo ri;
+            ri.bits = 0u;
+            ri.x = ri.y = ri.z = ri.e = ri.alpha = ri.T = ri.w = 0.f;
+            if (q < total) {
+                const float4 a = rec.xyze[q];
+                const float4 b = rec.atwr[q];
+                ri.x = a.x;
+                ri.y = a.y;
+                ri.z = a.z;
+                ri.e = a.w;
+                ri.alpha = b.x;
+                ri.T = b.y;
+                ri.w = b.z;
+                ri.bits = __float_as_uint(b.w);
+            }
+            sm.ri[tid] = ri;
+            if (BWD) {
+                const uint32_t ray = ri.bits & kRayMask;
+                const bool any = (ri.bits & (kAliveA | kAliveW)) != 0u;
+                sm.rayg[tid] = any ? R.ray_grad[ray] : make_float4(0.f, 0.f, 0.f, 0.f);
+                sm.ctot[tid] = any ? R.ray_ctot[ray] : make_float4(0.f, 0.f, 0.f, 0.f);
+                sm.cinc[tid] = any ? rec.cinc[q] : make_float4(0.f, 0.f, 0.f, 0.f);
+                sm.tlast[tid] = any ? R.t_last[ray] : 0.f;
+            }
+        }
+        __syncthreads();
+        // ---- 2. X = [colour features (12) | PE (27) | 1 | 0...] ----
+        if (tid < 128) {
+            const int s = tid >> 3, q = tid & 7;
+            const RecInfo ri = sm.ri[s];
+            float v[12];
+#pragma unroll
+            for (int c = 0; c < 12; ++c) v[c] = 0.f;
+            if (ri.bits & kAliveW) {
+                const Cell cell = make_cell(ri.x, ri.y, ri.z);
+                const int dx = (kCx >> q) & 1, dy = (kCy >> q) & 1, dz = (kCz >> q) & 1;
+                const float a = dx ? cell.u : 1.f - cell.u;
+                const float b = dy ? cell.v : 1.f - cell.v;
+                const float d = dz ? cell.w : 1.f - cell.w;
+                const float sc = (a * b) * d;
+                const int64_t slot = slot_of(topo, cell.i0 + dx, cell.j0 + dy, cell.k0 + dz);
+                if (slot >= 0) {
+                    if (BF16COL) {
+                        const uint16_t* src = reinterpret_cast<const uint16_t*>(color) + slot * 12;
+                        const uint2 p0 = *reinterpret_cast<const uint2*>(src);
+                        const uint2 p1 = *reinterpret_cast<const uint2*>(src + 4);
+                        const uint2 p2 = *reinterpret_cast<const uint2*>(src + 8);
+                        const uint32_t w6[6] = {p0.x, p0.y, p1.x, p1.y, p2.x, p2.y};
+#pragma unroll
+                        for (int c = 0; c < 6; ++c) {
+                            v[2 * c] = bf_bits2f((uint16_t)(w6[c] & 0xffffu)) * sc;
+                            v[2 * c + 1] = bf_bits2f((uint16_t)(w6[c] >> 16)) * sc;
+                        }
+                    } else {
+                        const float4* src = reinterpret_cast<const float4*>(
+                            reinterpret_cast<const float*>(color) + slot * 12);
+                        const float4 p0 = src[0], p1 = src[1], p2 = src[2];
+                        v[0] = p0.x * sc; v[1] = p0.y * sc; v[2] = p0.z * sc; v[3] = p0.w * sc;
+                        v[4] = p1.x * sc; v[5] = p1.y * sc; v[6] = p1.z * sc; v[7] = p1.w * sc;
+                        v[8] = p2.x * sc; v[9] = p2.y * sc; v[10] = p2.z * sc; v[11] = p2.w * sc;
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 12; ++c) {
+                v[c] += __shfl_xor(v[c], 1, 64);
+                v[c] += __shfl_xor(v[c], 2, 64);
+                v[c] += __shfl_xor(v[c], 4, 64);
+            }
+            if (q == 0) {
+#pragma unroll
+                for (int c = 0; c < 12; ++c) {
+                    const uint16_t h = f2bf_bits(v[c]);
+                    sm.Xs[s * XM + c] = h;
+                    sm.Xf[c * FM + s] = h;
+                }
+            }
+        } else {
+            const int tt = tid - 128, s = tt >> 3, part = tt & 7;
+            const RecInfo ri = sm.ri[s];
+            const bool alive = (ri.bits & kAliveW) != 0u;
+            float vd[3] = {0.f, 0.f, 0.f};
+            if (alive) {
+                const uint32_t ray = ri.bits & kRayMask;
+                vd[0] = P.viewdirs[3 * ray];
+                vd[1] = P.viewdirs[3 * ray + 1];
+                vd[2] = P.viewdirs[3 * ray + 2];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int p = part * 4 + k;  // 0..27: PE 0..26, then the bias column
+                if (p < 28) {
+                    float val = 0.f;
+                    if (alive) {
+                        if (p < 3) {
+                            val = vd[p];
+                        } else if (p < 27) {
+                            const int idx = p - 3, axis = (idx % 12) / 4, fr = idx % 4;
+                            const float arg = vd[axis] * (float)(1 << fr);
+                            val = idx < 12 ? sinf(arg) : cosf(arg);
+                        } else {
+                            val = 1.f;
+                        }
+                    }
+                    const uint16_t h = f2bf_bits(val);
+                    sm.Xs[s * XM + 12 + p] = h;
+                    sm.Xf[(12 + p) * FM + s] = h;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const int col = 40 + part * 3 + k;
+                sm.Xs[s * XM + col] = 0;
+                sm.Xf[col * FM + s] = 0;
+            }
+        }
+        __syncthreads();
+        // ---- 3. H0^T slice = relu(W0^T X^T + b0) ----
+        float h0[2][4], h1[2][4];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt)
+                acc = mfma16x16x32(A0[mt][kt], ld_frag(&sm.Xs[r16 * XM + 32 * kt + 8 * g]), acc);
+            const int fr = f0 + 16 * mt + 4 * g;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) h0[mt][i] = fmaxf(acc[i] + b0v[mt][i], 0.f);
+            st4(&sm.H0s[r16 * SM + fr], h0[mt][0], h0[mt][1], h0[mt][2], h0[mt][3]);
+            if (BWD) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) sm.H0f[(fr + i) * FM + r16] = f2bf_bits(h0[mt][i]);
+            }
+        }
+        __syncthreads();
+        // ---- 4. H1^T slice = relu(W1^T H0^T + b1) ----
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt)
+                acc = mfma16x16x32(A1[mt][kt], ld_frag(&sm.H0s[r16 * SM + 32 * kt + 8 * g]), acc);
+            const int fr = f0 + 16 * mt + 4 * g;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) h1[mt][i] = fmaxf(acc[i] + b1v[mt][i], 0.f);
+            st4(&sm.H1s[r16 * SM + fr], h1[mt][0], h1[mt][1], h1[mt][2], h1[mt][3]);
+            if (BWD) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) sm.H1f[(fr + i) * FM + r16] = f2bf_bits(h1[mt][i]);
+            }
+        }
+        __syncthreads();
+        // ---- 5. logits (wave 0): L^T = W2^T H1^T + b2, rgb = sigmoid ----
+        if (wv == 0) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt)
+                acc = mfma16x16x32(A2[kt], ld_frag(&sm.H1s[r16 * SM + 32 * kt + 8 * g]), acc);
+            if (g == 0) {
+                const float c0 = sigmoidf_(acc[0] + b2v[0]);
+                const float c1 = sigmoidf_(acc[1] + b2v[1]);
+                const float c2 = sigmoidf_(acc[2] + b2v[2]);
+                sm.rgb[r16][0] = c0;
+                sm.rgb[r16][1] = c1;
+                sm.rgb[r16][2] = c2;
+                if (!BWD) {
+                    const int64_t q = base + r16;
+                    if (q < total) {
+                        const bool alive = (sm.ri[r16].bits & kAliveW) != 0u;
+                        rec.rgb[q] = alive ? make_float4(c0, c1, c2, 0.f)
+                                           : make_float4(0.f, 0.f, 0.f, 0.f);
+                    }
+                }
+            }
+        }
+        if (!BWD) {
+            __syncthreads();
+            continue;
+        }
+        __syncthreads();
+        // ---- 6. per-sample adjoints: dL/dlogit, alpha, sigma ----
+        if (tid < 16) {
+            const RecInfo ri = sm.ri[tid];
+            const float4 gr = sm.rayg[tid];
+            float gl0 = 0.f, gl1 = 0.f, gl2 = 0.f, gw = 0.f;
+            if (ri.bits & kAliveW) {
+                const float c0 = sm.rgb[tid][0], c1 = sm.rgb[tid][1], c2 = sm.rgb[tid][2];
+                gl0 = ri.w * gr.x * c0 * (1.f - c0);
+                gl1 = ri.w * gr.y * c1 * (1.f - c1);
+                gl2 = ri.w * gr.z * c2 * (1.f - c2);
+                gw = gr.x * c0 + gr.y * c1 + gr.z * c2;
+            }
+            float gs = 0.f;
+            if (ri.bits & kAliveA) {
+                const float4 ct = sm.ctot[tid], ci = sm.cinc[tid];
+                const float back = gr.w * sm.tlast[tid] + gr.x * (ct.x - ci.x) +
+                                   gr.y * (ct.y - ci.y) + gr.z * (ct.z - ci.z);
+                const float ga = gw * ri.T - back / ((1.f - ri.alpha) + 1e-10f);
+                gs = fminf(ri.e, 1e10f) * powf(1.f + ri.e, -P.interval - 1.f) * P.interval * ga;
+            }
+            sm.gsig[tid] = gs;
+            db2[0] += gl0;
+            db2[1] += gl1;
+            db2[2] += gl2;
+#pragma unroll
+            for (int c = 0; c < 32; ++c) {
+                const float v = c == 0 ? gl0 : c == 1 ? gl1 : c == 2 ? gl2 : 0.f;
+                sm.GLs[tid * GM + c] = f2bf_bits(v);
+            }
+#pragma unroll
+            for (int o = 0; o < 16; ++o) {
+                const float v = o == 0 ? gl0 : o == 1 ? gl1 : o == 2 ? gl2 : 0.f;
+                sm.GLf[o * FM + tid] = f2bf_bits(v);
+            }
+        }
+        __syncthreads();
+        // ---- 7. g_H1^T slice = W2[slice] g_L^T, masked; dW2 slice ----
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma16x16x32(Bw2[mt], ld_frag(&sm.GLs[r16 * GM + 8 * g]), acc);
+            const int fr = f0 + 16 * mt + 4 * g;
+            float gv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                gv[i] = h1[mt][i] > 0.f ? acc[i] : 0.f;
+                db1[mt][i] += gv[i];
+            }
+            st4(&sm.G1s[r16 * SM + fr], gv[0], gv[1], gv[2], gv[3]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sm.G1f[(fr + i) * FM + r16] = f2bf_bits(gv[i]);
+            // dW2[in = f0+16mt+row][out] += sum_s H1[s][in] gL[s][out]
+            dW2[mt] = mfma16x16x16(ld_frag4(&sm.H1f[(f0 + 16 * mt + r16) * FM + 4 * g]),
+                                   ld_frag4(&sm.GLf[r16 * FM + 4 * g]), dW2[mt]);
+        }
+        __syncthreads();
+        // ---- 8. g_H0^T slice = W1[slice] g_H1^T, masked; dW1 columns of this wave ----
+        float g0v[2][4];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt)
+                acc = mfma16x16x32(Bw1[mt][kt], ld_frag(&sm.G1s[r16 * SM + 32 * kt + 8 * g]), acc);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) g0v[mt][i] = h0[mt][i] > 0.f ? acc[i] : 0.f;
+            const int fr = f0 + 16 * mt + 4 * g;
+            st4(&sm.G0s[r16 * SM + fr], g0v[mt][0], g0v[mt][1], g0v[mt][2], g0v[mt][3]);
+        }
+#pragma unroll
+        for (int mi = 0; mi < 8; ++mi)
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt)
+                dW1[mi][nt] = mfma16x16x16(ld_frag4(&sm.H0f[(16 * mi + r16) * FM + 4 * g]),
+                                           ld_frag4(&sm.G1f[(f0 + 16 * nt + r16) * FM + 4 * g]),
+                                           dW1[mi][nt]);
+        __syncthreads();
+        // ---- 9. dW0 columns of this wave; partial g_X^T over this wave's K slice ----
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                sm.G0f[(f0 + 16 * mt + 4 * g + i) * FM + r16] = f2bf_bits(g0v[mt][i]);
+        __syncthreads();
+        {
+            const uint16_t* G0f = sm.G0f;
+#pragma unroll
+            for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt)
+                    dW0[mi][nt] = mfma16x16x16(ld_frag4(&sm.Xf[(16 * mi + r16) * FM + 4 * g]),
+                                               ld_frag4(&G0f[(f0 + 16 * nt + r16) * FM + 4 * g]),
+                                               dW0[mi][nt]);
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mfma16x16x32(Bw0, ld_frag(&sm.G0s[r16 * SM + f0 + 8 * g]), acc);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sm.GXp[wv][4 * g + i][r16] = acc[i];
+        }
+        __syncthreads();
+        // ---- 10. scatter into the grid gradients ----
+        if (tid < 128) {
+            const int s = tid >> 3, q = tid & 7;
+            const RecInfo ri = sm.ri[s];
+            if (ri.bits & (kAliveA | kAliveW)) {
+                const Cell cell = make_cell(ri.x, ri.y, ri.z);
+                const int dx = (kCx >> q) & 1, dy = (kCy >> q) & 1, dz = (kCz >> q) & 1;
+                const float a = dx ? cell.u : 1.f - cell.u;
+                const float b = dy ? cell.v : 1.f - cell.v;
+                const float d = dz ? cell.w : 1.f - cell.w;
+                const int32_t leaf = leaf_of(topo, cell.i0 + dx, cell.j0 + dy, cell.k0 + dz);
+                if (leaf >= 0) {
+                    const int64_t slot = (int64_t)leaf * 512 +
+                                         voxel_offset(cell.i0 + dx, cell.j0 + dy, cell.k0 + dz);
+                    leaf_touched[leaf] = 1;
+                    if (ri.bits & kAliveA) atomicAdd(grad_density + slot, ((sm.gsig[s] * a) * b) * d);
+                    if (ri.bits & kAliveW) {
+                        const float sc = (a * b) * d;
+                        float* dst = grad_color + slot * 12;
+#pragma unroll
+                        for (int c = 0; c < 12; ++c) {
+                            const float gx = sm.GXp[0][c][s] + sm.GXp[1][c][s] + sm.GXp[2][c][s] +
+                                             sm.GXp[3][c][s];
+                            atomicAdd(dst + c, gx * sc);
+                        }
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!BWD) return;
+    // ---- flush this workgroup's dW partial ----
+    float* out = dw_partial + (int64_t)blockIdx.x * kMlpParams;
+#pragma unroll
+    for (int mi = 0; mi < 4; ++mi)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int in = 16 * mi + 4 * g + i, o = f0 + 16 * nt + r16;
+                if (in < 39)
+                    out[kW0 + in * 128 + o] = dW0[mi][nt][i];
+                else if (in == 39)
+                    out[kB0 + o] = dW0[mi][nt][i];
+            }
+#pragma unroll
+    for (int mi = 0; mi < 8; ++mi)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                out[kW1 + (16 * mi + 4 * g + i) * 128 + f0 + 16 * nt + r16] = dW1[mi][nt][i];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (r16 < 3) out[kW2 + (f0 + 16 * mt + 4 * g + i) * 3 + r16] = dW2[mt][i];
+            float v = db1[mt][i];
+            v += __shfl_xor(v, 1, 64);
+            v += __shfl_xor(v, 2, 64);
+            v += __shfl_xor(v, 4, 64);
+            v += __shfl_xor(v, 8, 64);
+            if (r16 == 0) out[kB1 + f0 + 16 * mt + 4 * g + i] = v;
+        }
+    if (wv == 0) {
+#pragma unroll
+        for (int o = 0; o < 3; ++o) {
+            float v = tid < 16 ? db2[o] : 0.f;
+            v += __shfl_xor(v, 1, 64);
+            v += __shfl_xor(v, 2, 64);
+            v += __shfl_xor(v, 4, 64);
+            v += __shfl_xor(v, 8, 64);
+            if (tid == 0) out[kB2 + o] = v;
+        }
+    }
+}
+
+// Sum of the per-workgroup dW partials.
+__global__ void k_dw_reduce(const float* __restrict__ part, int nparts, float* __restrict__ dw) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= kMlpParams) return;
+    float s = 0.f;
+    for (int b = 0; b < nparts; ++b) s += part[(int64_t)b * kMlpParams + p];
+    dw[p] = s;
+}
+
+// fp32 master parameters (kMlpParams layout) -> packed bf16 MFMA operands.
+__global__ void k_pack_mlp(const float* __restrict__ prm, uint16_t* __restrict__ w0t,
+                           uint16_t* __restrict__ w1t, uint16_t* __restrict__ w2t,
+                           uint16_t* __restrict__ w2, uint16_t* __restrict__ w1,
+                           uint16_t* __restrict__ w0) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    // w1t / w1: 128 x 128
+    if (t < 128 * 128) {
+        const int a = t >> 7, b = t & 127;
+        w1t[a * 128 + b] = f2bf_bits(prm[kW1 + b * 128 + a]);  // W1^T[out a][in b]
+        w1[a * 128 + b] = f2bf_bits(prm[kW1 + a * 128 + b]);   // W1[in a][out b]
+    }
+    if (t < 128 * 64) {  // w0t [128 out][64 in]
+        const int o = t >> 6, i = t & 63;
+        w0t[o * 64 + i] = i < 39 ? f2bf_bits(prm[kW0 + i * 128 + o]) : (uint16_t)0;
+    }
+    if (t < 16 * 128) {  // w2t [16 out][128 in], w0 [16 in][128 out]
+        const int o = t >> 7, i = t & 127;
+        w2t[o * 128 + i] = o < 3 ? f2bf_bits(prm[kW2 + i * 3 + o]) : (uint16_t)0;
+        w0[o * 128 + i] = o < 12 ? f2bf_bits(prm[kW0 + o * 128 + i]) : (uint16_t)0;
+    }
+    if (t < 128 * 32) {  // w2 [128 in][32 out]
+        const int i = t >> 5, o = t & 31;
+        w2[i * 32 + o] = o < 3 ? f2bf_bits(prm[kW2 + i * 3 + o]) : (uint16_t)0;
+    }
+}
+
+}  // namespace
+
+namespace pvdb {
+
+int mlp_grid_size(int64_t total) {
+    int64_t groups = (total + 15) / 16;
+    int64_t g = groups < 1 ? 1 : groups;
+    return (int)(g < 512 ? g : 512);
+}
+
+int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const TopoView& t,
+               const void* color, const MlpPacked& W, const RayBuffers& R, const Records& rec,
+               float* grad_density, float* grad_color, uint8_t* leaf_touched, float* dw_partial,
+               hipStream_t s) {
+    if (bwd) {
+        if (bf16col)
+            hipLaunchKernelGGL((k_mlp<true, true>), dim3(grid), dim3(256), 0, s, P, t, color, W, R,
+                               rec, grad_density, grad_color, leaf_touched, dw_partial);
+        else
+            hipLaunchKernelGGL((k_mlp<true, false>), dim3(grid), dim3(256), 0, s, P, t, color, W, R,
+                               rec, grad_density, grad_color, leaf_touched, dw_partial);
+    } else {
+        if (bf16col)
+            hipLaunchKernelGGL((k_mlp<false, true>), dim3(grid), dim3(256), 0, s, P, t, color, W, R,
+                               rec, grad_density, grad_color, leaf_touched, dw_partial);
+        else
+            hipLaunchKernelGGL((k_mlp<false, false>), dim3(grid), dim3(256), 0, s, P, t, color, W,
+                               R, rec, grad_density, grad_color, leaf_touched, dw_partial);
+    }
+    PVDB_LAUNCHED("k_mlp");
+    return PVDB_OK;
+}
+
+int launch_dw_reduce(const float* part, int nparts, float* dw, hipStream_t s) {
+    hipLaunchKernelGGL(k_dw_reduce, dim3(pvdb_blocks(kMlpParams, 256)), dim3(256), 0, s, part,
+                       nparts, dw);
+    PVDB_LAUNCHED("k_dw_reduce");
+    return PVDB_OK;
+}
+
+int launch_pack_mlp(const float* prm, uint16_t* w0t, uint16_t* w1t, uint16_t* w2t, uint16_t* w2,
+                    uint16_t* w1, uint16_t* w0, hipStream_t s) {
+    hipLaunchKernelGGL(k_pack_mlp, dim3(pvdb_blocks(128 * 128, 256)), dim3(256), 0, s, prm, w0t,
+                       w1t, w2t, w2, w1, w0);
+    PVDB_LAUNCHED("k_pack_mlp");
+    return PVDB_OK;
+}
+
+}  // namespace pvdb

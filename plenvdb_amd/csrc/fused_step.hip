@@ -1,0 +1,316 @@
+// Fused (batched) training step: C-ABI entry point, workspace layout, the
+// sparse VDB Adam over touched leaves and the per-step fills.
+//
+// pvdb_fused_step runs one iteration of the reference's training loop
+// (plenvdb/run.py:541-598: forward, MSE loss, backward, MaskedAdam on the MLP,
+// VDBAdam on the grids) as a fixed sequence of kernel launches on one stream
+// with no host synchronisation and no allocation, so a caller can capture it
+// in a hipGraph.  All scratch lives in the caller's workspace.
+#include <cmath>
+
+#include "fused_common.h"
+#include "plenvdb_hip.h"
+
+namespace pvdb {
+TopoView view_of(const pvdb_topology* t);
+int launch_ray_count(const StepParams& P, const TopoView& t, const RayBuffers& R, hipStream_t s);
+int launch_ray_march(const StepParams& P, const TopoView& t, const float* density,
+                     const RayBuffers& R, const Records& rec, hipStream_t s);
+int launch_ray_composite(const StepParams& P, const RayBuffers& R, const Records& rec,
+                         hipStream_t s);
+int mlp_grid_size(int64_t total);
+int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const TopoView& t,
+               const void* color, const MlpPacked& W, const RayBuffers& R, const Records& rec,
+               float* grad_density, float* grad_color, uint8_t* leaf_touched, float* dw_partial,
+               hipStream_t s);
+int launch_dw_reduce(const float* part, int nparts, float* dw, hipStream_t s);
+int launch_pack_mlp(const float* prm, uint16_t* w0t, uint16_t* w1t, uint16_t* w2t, uint16_t* w2,
+                    uint16_t* w1, uint16_t* w0, hipStream_t s);
+}  // namespace pvdb
+
+using namespace pvdb;
+
+namespace {
+
+constexpr int kMaxMlpGrid = 512;
+
+struct Layout {
+    size_t count, offset, total, t_last, ray_out, ray_grad, ray_ctot, loss;
+    size_t xyze, atwr, rgb, cinc;
+    size_t touched, dw_part, w0t, w1t, w2t, w2, w1, w0;
+    size_t bytes;
+};
+
+size_t align256(size_t v) { return (v + 255) & ~size_t(255); }
+
+Layout layout_for(int64_t n_rays, int64_t capacity, int64_t nleaf) {
+    Layout L;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = align256(o + bytes);
+        return at;
+    };
+    L.count = take(4 * n_rays);
+    L.offset = take(4 * n_rays);
+    L.total = take(4);
+    L.t_last = take(4 * n_rays);
+    L.ray_out = take(16 * n_rays);
+    L.ray_grad = take(16 * n_rays);
+    L.ray_ctot = take(16 * n_rays);
+    L.loss = take(4);
+    L.xyze = take(16 * capacity);
+    L.atwr = take(16 * capacity);
+    L.rgb = take(16 * capacity);
+    L.cinc = take(16 * capacity);
+    L.touched = take(nleaf > 0 ? nleaf : 1);
+    L.dw_part = take((size_t)4 * kMlpParams * kMaxMlpGrid);
+    L.w0t = take(2 * 128 * 64);
+    L.w1t = take(2 * 128 * 128);
+    L.w2t = take(2 * 16 * 128);
+    L.w2 = take(2 * 128 * 32);
+    L.w1 = take(2 * 128 * 128);
+    L.w0 = take(2 * 16 * 128);
+    L.bytes = o;
+    return L;
+}
+
+__global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_f32(float* p, int64_t n, float v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__device__ __forceinline__ void adam1(float& p, float g, float& m, float& v, float sz, float eps,
+                                      float b0, float b1) {
+    const float nm = b0 * m + (1.f - b0) * g;
+    const float nv = b1 * v + ((1.f - b1) * g) * g;
+    m = nm;
+    v = nv;
+    p = p - (sz * nm) / (eps + sqrtf(nv));
+}
+
+// Adam with skip-zero-grad (stepmode 1, densityvdb.cu:272-349, colorvdb.cu:300-351) on
+// the leaves the backward touched, then zero their gradients and flags.  One
+// workgroup per touched leaf; untouched leaves have all-zero gradients, which
+// stepmode 1 skips, so the result equals the reference's all-slot pass.
+template <bool BF16COL>
+__global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __restrict__ touched,
+                                                     float* __restrict__ den,
+                                                     float* __restrict__ gden,
+                                                     float* __restrict__ mden,
+                                                     float* __restrict__ vden, void* col,
+                                                     float* __restrict__ gcol,
+                                                     float* __restrict__ mcol,
+                                                     float* __restrict__ vcol, float sz_d,
+                                                     float sz_c, float eps, float b0, float b1) {
+    for (int64_t leaf = blockIdx.x; leaf < t.nleaf; leaf += gridDim.x) {
+        if (!touched[leaf]) continue;  // uniform over the workgroup
+        for (int k = threadIdx.x; k < 512; k += blockDim.x) {
+            const int64_t s = leaf * 512 + k;
+            const bool act = (t.masks[leaf * 8 + (k >> 6)] >> (k & 63)) & 1ull;
+            if (act) {
+                const float g = gden[s];
+                if (g != 0.f) adam1(den[s], g, mden[s], vden[s], sz_d, eps, b0, b1);
+#pragma unroll
+                for (int grp = 0; grp < 4; ++grp) {
+                    const int64_t c0 = s * 12 + 3 * grp;
+                    const float g0 = gcol[c0], g1 = gcol[c0 + 1], g2 = gcol[c0 + 2];
+                    if (g0 == 0.f && g1 == 0.f && g2 == 0.f) continue;
+                    const float gg[3] = {g0, g1, g2};
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        if (BF16COL) {
+                            uint16_t* cp = reinterpret_cast<uint16_t*>(col) + c0 + c;
+                            float p = bf_bits2f(*cp);
+                            adam1(p, gg[c], mcol[c0 + c], vcol[c0 + c], sz_c, eps, b0, b1);
+                            *cp = f2bf_bits(p);
+                        } else {
+                            float* cp = reinterpret_cast<float*>(col) + c0 + c;
+                            adam1(*cp, gg[c], mcol[c0 + c], vcol[c0 + c], sz_c, eps, b0, b1);
+                        }
+                    }
+                }
+            }
+            gden[s] = 0.f;
+#pragma unroll
+            for (int c = 0; c < 12; ++c) gcol[s * 12 + c] = 0.f;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) touched[leaf] = 0;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t pvdb_fused_workspace_bytes(int64_t n_rays, int64_t capacity, int64_t nleaf) {
+    if (n_rays < 0 || capacity < 0 || nleaf < 0) return -1;
+    return (int64_t)layout_for(n_rays, capacity, nleaf).bytes;
+}
+
+int pvdb_fused_init(const pvdb_topology* topo, const pvdb_step_desc* d, void* stream) {
+    if (!topo) {
+        pvdb::set_error("null topology");
+        return PVDB_EHANDLE;
+    }
+    PVDB_CHECK_ARG(d && d->workspace, "null descriptor or workspace");
+    const TopoView t = view_of(topo);
+    const Layout L = layout_for(d->n_rays, d->capacity, t.nleaf);
+    PVDB_CHECK_ARG(d->workspace_bytes >= (int64_t)L.bytes, "workspace too small");
+    uint8_t* ws = reinterpret_cast<uint8_t*>(d->workspace);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_fill_u8, dim3(pvdb_blocks(L.bytes, 256)), dim3(256), 0, s, ws,
+                       (int64_t)L.bytes, (uint8_t)0);
+    PVDB_LAUNCHED("k_fill_u8");
+    return PVDB_OK;
+}
+
+int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* stream) {
+    if (!topo) {
+        pvdb::set_error("null topology");
+        return PVDB_EHANDLE;
+    }
+    PVDB_CHECK_ARG(d != nullptr, "null descriptor");
+    PVDB_CHECK_ARG(d->n_rays > 0, "n_rays must be positive");
+    PVDB_CHECK_ARG(d->capacity > 0, "record capacity must be positive");
+    PVDB_CHECK_ARG(d->rays_o && d->rays_d && d->viewdirs && d->target, "null ray arrays");
+    PVDB_CHECK_ARG(d->density && d->grad_density && d->color && d->grad_color, "null grid arrays");
+    PVDB_CHECK_ARG(d->mlp && d->mlp_grad, "null MLP arrays");
+    PVDB_CHECK_ARG(!d->do_adam || (d->density_m && d->density_v && d->color_m && d->color_v &&
+                                   d->mlp_m && d->mlp_v),
+                   "Adam needs moment arrays");
+    PVDB_CHECK_ARG(d->stepdist > 0.f, "stepdist must be positive");
+    const TopoView t = view_of(topo);
+    const Layout L = layout_for(d->n_rays, d->capacity, t.nleaf);
+    PVDB_CHECK_ARG(d->workspace && d->workspace_bytes >= (int64_t)L.bytes, "workspace too small");
+    uint8_t* ws = reinterpret_cast<uint8_t*>(d->workspace);
+    hipStream_t s = (hipStream_t)stream;
+
+    StepParams P;
+    P.rays_o = d->rays_o;
+    P.rays_d = d->rays_d;
+    P.viewdirs = d->viewdirs;
+    P.target = d->target;
+    P.far_per_ray = d->far_per_ray;
+    P.n_rays = d->n_rays;
+    P.color_bf16 = d->color_bf16;
+    for (int a = 0; a < 3; ++a) {
+        P.mn[a] = d->xyz_min[a];
+        P.mx[a] = d->xyz_max[a];
+        P.len[a] = d->xyz_max[a] - d->xyz_min[a];
+        P.rm1[a] = (float)(d->reso[a] - 1);
+        // MaskGrid: scale = (shape-1)/(max-min), shift = -min*scale (grid.py:225-227)
+        P.ms[a] = ((float)d->reso[a] - 1.f) / P.len[a];
+        P.mb[a] = -d->xyz_min[a] * P.ms[a];
+    }
+    P.near_ = d->near_;
+    P.far_ = 1e9f;
+    P.stepdist = d->stepdist;
+    P.act_shift = d->act_shift;
+    P.interval = d->interval;
+    P.thres = d->fast_color_thres;
+    P.bg = d->bg;
+    P.loss_scale = 1.f / (3.f * (float)d->n_rays);
+    P.capacity = d->capacity;
+
+    RayBuffers R;
+    R.count = reinterpret_cast<int32_t*>(ws + L.count);
+    R.offset = reinterpret_cast<int32_t*>(ws + L.offset);
+    R.total = reinterpret_cast<int32_t*>(ws + L.total);
+    R.t_last = reinterpret_cast<float*>(ws + L.t_last);
+    R.ray_out = reinterpret_cast<float4*>(ws + L.ray_out);
+    R.ray_grad = reinterpret_cast<float4*>(ws + L.ray_grad);
+    R.ray_ctot = reinterpret_cast<float4*>(ws + L.ray_ctot);
+    R.loss = d->loss ? d->loss : reinterpret_cast<float*>(ws + L.loss);
+    Records rec;
+    rec.xyze = reinterpret_cast<float4*>(ws + L.xyze);
+    rec.atwr = reinterpret_cast<float4*>(ws + L.atwr);
+    rec.rgb = reinterpret_cast<float4*>(ws + L.rgb);
+    rec.cinc = reinterpret_cast<float4*>(ws + L.cinc);
+    uint8_t* touched = ws + L.touched;
+    float* dw_part = reinterpret_cast<float*>(ws + L.dw_part);
+    uint16_t* w0t = reinterpret_cast<uint16_t*>(ws + L.w0t);
+    uint16_t* w1t = reinterpret_cast<uint16_t*>(ws + L.w1t);
+    uint16_t* w2t = reinterpret_cast<uint16_t*>(ws + L.w2t);
+    uint16_t* w2 = reinterpret_cast<uint16_t*>(ws + L.w2);
+    uint16_t* w1 = reinterpret_cast<uint16_t*>(ws + L.w1);
+    uint16_t* w0 = reinterpret_cast<uint16_t*>(ws + L.w0);
+    MlpPacked W;
+    W.w0t = w0t;
+    W.w1t = w1t;
+    W.w2t = w2t;
+    W.w2 = w2;
+    W.w1 = w1;
+    W.w0 = w0;
+    W.b0 = d->mlp + kB0;
+    W.b1 = d->mlp + kB1;
+    W.b2 = d->mlp + kB2;
+
+    int rc;
+    if ((rc = launch_pack_mlp(d->mlp, w0t, w1t, w2t, w2, w1, w0, s))) return rc;
+    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, R.loss, (int64_t)1, 0.f);
+    PVDB_LAUNCHED("k_fill_f32");
+    if ((rc = launch_ray_count(P, t, R, s))) return rc;
+    if ((rc = launch_ray_march(P, t, d->density, R, rec, s))) return rc;
+    // The sample-side kernels are persistent over the records; their grid is
+    // fixed (the record count stays on the device).
+    const int grid = kMaxMlpGrid;
+    if ((rc = launch_mlp(false, d->color_bf16 != 0, grid, P, t, d->color, W, R, rec, nullptr,
+                         nullptr, touched, nullptr, s)))
+        return rc;
+    if ((rc = launch_ray_composite(P, R, rec, s))) return rc;
+    if ((rc = launch_mlp(true, d->color_bf16 != 0, grid, P, t, d->color, W, R, rec,
+                         d->grad_density, d->grad_color, touched, dw_part, s)))
+        return rc;
+    if ((rc = launch_dw_reduce(dw_part, grid, d->mlp_grad, s))) return rc;
+    if (d->ray_out) {
+        PVDB_HIP(hipMemcpyAsync(d->ray_out, R.ray_out, 16 * (size_t)d->n_rays,
+                                hipMemcpyDeviceToDevice, s));
+    }
+    if (d->do_adam) {
+        const int ag = (int)std::min<int64_t>(t.nleaf > 0 ? t.nleaf : 1, 4096);
+        if (d->color_bf16)
+            hipLaunchKernelGGL(k_sparse_adam<true>, dim3(ag), dim3(256), 0, s, t, touched,
+                               d->density, d->grad_density, d->density_m, d->density_v, d->color,
+                               d->grad_color, d->color_m, d->color_v, d->stepsz_density,
+                               d->stepsz_color, d->eps, d->beta0, d->beta1);
+        else
+            hipLaunchKernelGGL(k_sparse_adam<false>, dim3(ag), dim3(256), 0, s, t, touched,
+                               d->density, d->grad_density, d->density_m, d->density_v, d->color,
+                               d->grad_color, d->color_m, d->color_v, d->stepsz_density,
+                               d->stepsz_color, d->eps, d->beta0, d->beta1);
+        PVDB_LAUNCHED("k_sparse_adam");
+        if ((rc = pvdb_dense_adam(d->mlp, d->mlp_grad, d->mlp_m, d->mlp_v, nullptr, kMlpParams, 0,
+                                  d->step_size_mlp, d->eps, d->beta0, d->beta1, stream)))
+            return rc;
+    }
+    return PVDB_OK;
+}
+
+int pvdb_fused_records(const pvdb_topology* topo, const pvdb_step_desc* d, void** xyze,
+                       void** atwr, void** rgb, void** cinc, int32_t** count, int32_t** offset,
+                       int32_t** total) {
+    if (!topo) {
+        pvdb::set_error("null topology");
+        return PVDB_EHANDLE;
+    }
+    PVDB_CHECK_ARG(d && d->workspace, "null descriptor or workspace");
+    const Layout L = layout_for(d->n_rays, d->capacity, view_of(topo).nleaf);
+    uint8_t* ws = reinterpret_cast<uint8_t*>(d->workspace);
+    if (xyze) *xyze = ws + L.xyze;
+    if (atwr) *atwr = ws + L.atwr;
+    if (rgb) *rgb = ws + L.rgb;
+    if (cinc) *cinc = ws + L.cinc;
+    if (count) *count = reinterpret_cast<int32_t*>(ws + L.count);
+    if (offset) *offset = reinterpret_cast<int32_t*>(ws + L.offset);
+    if (total) *total = reinterpret_cast<int32_t*>(ws + L.total);
+    return PVDB_OK;
+}
+
+}  // extern "C"

@@ -6,6 +6,7 @@
 // launch per call, reference operation order).  The fused training step lives
 // in fused_step.hip.
 #include "pvdb_common.h"
+#include "gather.h"
 #include "plenvdb_hip.h"
 
 namespace pvdb {
@@ -13,46 +14,14 @@ TopoView view_of(const pvdb_topology* t);
 }
 
 using pvdb::TopoView;
+using pvdb::Cell;
+using pvdb::LeafCache;
+using pvdb::kCx;
+using pvdb::kCy;
+using pvdb::kCz;
+using pvdb::make_cell;
 
 namespace {
-
-// Corner walk of the reference: 000,001,011,010,110,111,101,100
-// (plenvdb/lib/vdb/densityvdb.cu:115-122).  Bit q of each mask is the corner's
-// x/y/z offset.
-constexpr unsigned kCx = 0xF0u, kCy = 0x3Cu, kCz = 0x66u;
-
-struct Cell {
-    int i0, j0, k0;
-    float u, v, w;
-};
-
-__device__ __forceinline__ Cell make_cell(float x, float y, float z) {
-    Cell c;
-    c.i0 = (int)floorf(x);
-    c.j0 = (int)floorf(y);
-    c.k0 = (int)floorf(z);
-    c.u = x - (float)c.i0;
-    c.v = y - (float)c.j0;
-    c.w = z - (float)c.k0;
-    return c;
-}
-
-// One-entry leaf cache, the analogue of the accessor's leaf-level cache.
-struct LeafCache {
-    int key0, key1, key2;
-    int32_t leaf;
-    __device__ __forceinline__ LeafCache() : key0(INT32_MIN), key1(0), key2(0), leaf(-1) {}
-    __device__ __forceinline__ int64_t slot(const TopoView& t, int i, int j, int k) {
-        const int a = i >> 3, b = j >> 3, c = k >> 3;
-        if (a != key0 || b != key1 || c != key2) {
-            key0 = a;
-            key1 = b;
-            key2 = c;
-            leaf = pvdb::leaf_of(t, i, j, k);
-        }
-        return leaf < 0 ? int64_t(-1) : (int64_t)leaf * 512 + pvdb::voxel_offset(i, j, k);
-    }
-};
 
 // ORDER 0: out += ((val*a)*b)*c          (densityvdb.cu:115-122)
 // ORDER 1: out[c] += val[c]*((a*b)*c)    (colorvdb.cu:16-26,96-103)

@@ -1,0 +1,247 @@
+"""Fused (batched) training step on the device: `FusedTrainer`.
+
+Holds the grids (density fp32, colour fp32 or bf16), their gradients and Adam
+moments, the MLP master parameters (fp32) and a workspace, and runs one
+iteration of the reference's training loop (plenvdb/run.py:541-598) per call
+through `pvdb_fused_step` -- a fixed sequence of gfx950 kernels with no host
+synchronisation, optionally replayed from a captured hipGraph.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import lib, scenes
+from .topology import Topology
+
+MLP_PARAMS = 39 * 128 + 128 + 128 * 128 + 128 + 128 * 3 + 3
+
+
+class StepDesc(C.Structure):
+    """Mirror of pvdb_step_desc (include/plenvdb_hip.h)."""
+
+    _fields_ = [
+        ("rays_o", C.c_void_p), ("rays_d", C.c_void_p), ("viewdirs", C.c_void_p),
+        ("target", C.c_void_p), ("far_per_ray", C.c_void_p),
+        ("n_rays", C.c_int32), ("color_bf16", C.c_int32),
+        ("xyz_min", C.c_float * 3), ("xyz_max", C.c_float * 3), ("reso", C.c_int32 * 3),
+        ("near_", C.c_float), ("stepdist", C.c_float), ("act_shift", C.c_float),
+        ("interval", C.c_float), ("fast_color_thres", C.c_float), ("bg", C.c_float),
+        ("density", C.c_void_p), ("grad_density", C.c_void_p), ("density_m", C.c_void_p),
+        ("density_v", C.c_void_p),
+        ("color", C.c_void_p), ("grad_color", C.c_void_p), ("color_m", C.c_void_p),
+        ("color_v", C.c_void_p),
+        ("mlp", C.c_void_p), ("mlp_grad", C.c_void_p), ("mlp_m", C.c_void_p), ("mlp_v", C.c_void_p),
+        ("stepsz_density", C.c_float), ("stepsz_color", C.c_float), ("step_size_mlp", C.c_float),
+        ("eps", C.c_float), ("beta0", C.c_float), ("beta1", C.c_float),
+        ("do_adam", C.c_int32), ("reserved", C.c_int32),
+        ("workspace", C.c_void_p), ("workspace_bytes", C.c_int64), ("capacity", C.c_int64),
+        ("ray_out", C.c_void_p), ("loss", C.c_void_p),
+    ]
+
+
+lib.EXTRA_SIGS.update({
+    "pvdb_fused_init": [C.c_void_p, C.POINTER(StepDesc), C.c_void_p],
+    "pvdb_fused_step": [C.c_void_p, C.POINTER(StepDesc), C.c_void_p],
+    "pvdb_fused_records": [C.c_void_p, C.POINTER(StepDesc)] + [C.c_void_p] * 7,
+})
+
+
+def _workspace_bytes(n_rays: int, capacity: int, nleaf: int) -> int:
+    L = lib.load()
+    fn = L.pvdb_fused_workspace_bytes
+    fn.restype = C.c_int64
+    fn.argtypes = [C.c_int64, C.c_int64, C.c_int64]
+    b = int(fn(n_rays, capacity, nleaf))
+    if b < 0:
+        raise RuntimeError("bad workspace sizes")
+    return b
+
+
+def pack_mlp(p: dict) -> np.ndarray:
+    """W0[39][128] b0 W1[128][128] b1 W2[128][3] b2 (W as Linear.weight.T)."""
+    return np.concatenate([np.asarray(p[k], np.float32).ravel()
+                           for k in ("w0", "b0", "w1", "b1", "w2", "b2")])
+
+
+def unpack_mlp(v: np.ndarray) -> dict:
+    v = np.asarray(v)
+    out, o = {}, 0
+    for k, shp in (("w0", (39, 128)), ("b0", (128,)), ("w1", (128, 128)), ("b1", (128,)),
+                   ("w2", (128, 3)), ("b2", (3,))):
+        n = int(np.prod(shp))
+        out[k] = v[o:o + n].reshape(shp)
+        o += n
+    return out
+
+
+def _f32(x) -> float:
+    return float(np.float32(x))
+
+
+class FusedTrainer:
+    """One training iteration per `step()` on the fused kernels.
+
+    scene: scenes.make_scene(...) output (must use the MLP).  color_bf16
+    defaults to the configuration's choice.
+    """
+
+    def __init__(self, scene: dict, topo: Topology | None = None, color_bf16: bool | None = None,
+                 lr_grid=0.1, lr_mlp=1e-3, lrate_decay=20, capacity: int | None = None,
+                 device="cuda"):
+        cfg = scene["cfg"]
+        if not cfg.use_mlp:
+            raise ValueError("the fused path runs the MLP configurations")
+        self.scene = scene
+        dev = torch.device(device)
+        self.topo = topo if topo is not None else Topology.from_mask(scene["mask"])
+        self.color_bf16 = cfg.color_bf16 if color_bf16 is None else bool(color_bf16)
+        ns = self.topo.nslots
+        den, col = scenes.grid_values(cfg, ns)
+        self.density = torch.from_numpy(den).to(dev)
+        colt = torch.from_numpy(col).to(dev)
+        self.color = colt.to(torch.bfloat16).contiguous() if self.color_bf16 else colt
+        self.grad_density = torch.zeros(ns, device=dev)
+        self.grad_color = torch.zeros(ns, 12, device=dev)
+        self.m_density = torch.zeros(ns, device=dev)
+        self.v_density = torch.zeros(ns, device=dev)
+        self.m_color = torch.zeros(ns, 12, device=dev)
+        self.v_color = torch.zeros(ns, 12, device=dev)
+        self.mlp = torch.from_numpy(pack_mlp(scene["mlp"])).to(dev)
+        self.mlp_grad = torch.zeros_like(self.mlp)
+        self.mlp_m = torch.zeros_like(self.mlp)
+        self.mlp_v = torch.zeros_like(self.mlp)
+        self.rays_o = torch.from_numpy(scene["rays_o"]).to(dev).contiguous()
+        self.rays_d = torch.from_numpy(scene["rays_d"]).to(dev).contiguous()
+        self.viewdirs = torch.from_numpy(scene["viewdirs"]).to(dev).contiguous()
+        self.target = torch.from_numpy(scene["target"]).to(dev).contiguous()
+        fpr = scene.get("far_per_ray")
+        self.far_per_ray = None if fpr is None else torch.from_numpy(fpr).to(dev).contiguous()
+        self.n_rays = self.rays_o.shape[0]
+        self.stepdist = _f32(np.float32(scene["stepsize"]) * np.float32(scene["voxel_size"]))
+        if capacity is None:
+            capacity = self._capacity_bound()
+        self.capacity = int(capacity)
+        self.ws_bytes = _workspace_bytes(self.n_rays, self.capacity, self.topo.nleaf)
+        self.workspace = torch.empty(self.ws_bytes, dtype=torch.uint8, device=dev)
+        self.ray_out = torch.zeros(self.n_rays, 4, device=dev)
+        self.loss = torch.zeros(1, device=dev)
+        self.lr_grid = float(lr_grid)
+        self.lr_mlp = float(lr_mlp)
+        self.decay = 0.1 ** (1 / (lrate_decay * 1000))
+        self.step_count = 0
+        self.desc = self._make_desc()
+        lib.call("pvdb_fused_init", self.topo.handle, C.byref(self.desc), lib.stream())
+        self.graph = None
+
+    def _capacity_bound(self) -> int:
+        """Upper bound of the records of a step: the total number of ray steps."""
+        mn, mx = self.scene["xyz_min"], self.scene["xyz_max"]
+        n = self.rays_o.shape[0]
+        t_min = torch.empty(n, device=self.rays_o.device)
+        t_max = torch.empty_like(t_min)
+        n_steps = torch.empty(n, dtype=torch.int64, device=self.rays_o.device)
+        lib.call("pvdb_ray_bounds", lib.ptr(self.rays_o), lib.ptr(self.rays_d), n, lib.f3(mn),
+                 lib.f3(mx), float(self.scene["near"]), 1e9, lib.ptr(self.far_per_ray),
+                 self.stepdist, lib.ptr(t_min), lib.ptr(t_max), lib.ptr(n_steps), lib.stream())
+        return int(n_steps.sum().item()) + 64
+
+    def _make_desc(self) -> StepDesc:
+        sc = self.scene
+        d = StepDesc()
+        d.rays_o, d.rays_d = self.rays_o.data_ptr(), self.rays_d.data_ptr()
+        d.viewdirs, d.target = self.viewdirs.data_ptr(), self.target.data_ptr()
+        d.far_per_ray = self.far_per_ray.data_ptr() if self.far_per_ray is not None else None
+        d.n_rays = self.n_rays
+        d.color_bf16 = int(self.color_bf16)
+        for a in range(3):
+            d.xyz_min[a] = float(sc["xyz_min"][a])
+            d.xyz_max[a] = float(sc["xyz_max"][a])
+            d.reso[a] = int(sc["reso"][a])
+        d.near_ = float(sc["near"])
+        d.stepdist = self.stepdist
+        d.act_shift = float(sc["act_shift"])
+        d.interval = _f32(np.float32(sc["stepsize"]) * np.float32(sc["voxel_size_ratio"]))
+        d.fast_color_thres = float(sc["fast_color_thres"])
+        d.bg = float(sc["bg"])
+        d.density, d.grad_density = self.density.data_ptr(), self.grad_density.data_ptr()
+        d.density_m, d.density_v = self.m_density.data_ptr(), self.v_density.data_ptr()
+        d.color, d.grad_color = self.color.data_ptr(), self.grad_color.data_ptr()
+        d.color_m, d.color_v = self.m_color.data_ptr(), self.v_color.data_ptr()
+        d.mlp, d.mlp_grad = self.mlp.data_ptr(), self.mlp_grad.data_ptr()
+        d.mlp_m, d.mlp_v = self.mlp_m.data_ptr(), self.mlp_v.data_ptr()
+        d.eps, d.beta0, d.beta1 = 1e-8, 0.9, 0.99
+        d.workspace = self.workspace.data_ptr()
+        d.workspace_bytes = self.ws_bytes
+        d.capacity = self.capacity
+        d.ray_out = self.ray_out.data_ptr()
+        d.loss = self.loss.data_ptr()
+        return d
+
+    @staticmethod
+    def _stepsz(lr, t):
+        f = np.float32
+        return float(f(lr) * np.sqrt(f(1) - np.power(f(0.99), f(t))) / (f(1) - np.power(f(0.9), f(t))))
+
+    def forward_backward(self):
+        """Forward + backward only: accumulates gradients, no optimizer step."""
+        self.desc.do_adam = 0
+        lib.call("pvdb_fused_step", self.topo.handle, C.byref(self.desc), lib.stream())
+        return self.loss
+
+    def step(self):
+        """One full iteration: forward, backward, Adam (grids stepmode 1, MLP
+        stepmode 0), gradients re-zeroed, lr decay (run.py:592-598)."""
+        self.step_count += 1
+        t = self.step_count
+        self.desc.do_adam = 1
+        self.desc.stepsz_density = self._stepsz(self.lr_grid, t)
+        self.desc.stepsz_color = self._stepsz(self.lr_grid, t)
+        self.desc.step_size_mlp = self._stepsz(self.lr_mlp, t)
+        lib.call("pvdb_fused_step", self.topo.handle, C.byref(self.desc), lib.stream())
+        self.mlp_grad.zero_()
+        self.lr_grid = _f32(np.float32(self.lr_grid) * np.float32(self.decay))
+        self.lr_mlp = self.lr_mlp * self.decay
+        return self.loss
+
+    def zero_grad(self):
+        self.grad_density.zero_()
+        self.grad_color.zero_()
+        self.mlp_grad.zero_()
+
+    def records(self) -> dict:
+        """Host copies of the per-sample records and per-ray counts of the last step."""
+        ptrs = [C.c_void_p() for _ in range(7)]
+        lib.call("pvdb_fused_records", self.topo.handle, C.byref(self.desc), *[C.byref(p) for p in ptrs])
+        torch.cuda.synchronize()
+        n = self.n_rays
+        total = _read_i32(ptrs[6].value, 1)[0]
+        total = min(total, self.capacity)
+        out = {"total": total, "count": _read_i32(ptrs[4].value, n), "offset": _read_i32(ptrs[5].value, n)}
+        for k, p in zip(("xyze", "atwr", "rgb", "cinc"), ptrs[:4]):
+            out[k] = _read_f32(p.value, 4 * total).reshape(total, 4)
+        return out
+
+
+def _wrap(ptr: int, n: int, typestr: str) -> torch.Tensor:
+    """A CUDA tensor view of n elements at a raw device pointer (no ownership)."""
+
+    class _Holder:
+        __cuda_array_interface__ = {"shape": (n,), "typestr": typestr, "data": (ptr, False),
+                                    "version": 3}
+
+    return torch.as_tensor(_Holder(), device="cuda")
+
+
+def _read_f32(ptr: int, n: int) -> np.ndarray:
+    if n == 0:
+        return np.zeros(0, np.float32)
+    torch.cuda.current_stream().synchronize()
+    return _wrap(ptr, n, "<f4").cpu().numpy()
+
+
+def _read_i32(ptr: int, n: int) -> np.ndarray:
+    torch.cuda.current_stream().synchronize()
+    return _wrap(ptr, n, "<i4").cpu().numpy()

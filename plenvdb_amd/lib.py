@@ -50,6 +50,7 @@ _SIGS = {
     "pvdb_segment_bounds": [P, I64, P, P, P],
     "pvdb_alpha2weight": [P, I64, P, P, P, P, P, P],
     "pvdb_alpha2weight_bwd": [P, P, P, P, P, P, I64, P, P, P, P],
+    "pvdb_mfma_selftest": [P, P, P, P, P, P, P],
 }
 
 # functions with extra declarations added by later modules register here
@@ -83,10 +84,45 @@ def load() -> C.CDLL:
         return lib
 
 
+class CallTimer:
+    """Brackets every library call made while active with HIP events on torch's
+    current stream (the stream the calls launch on); `totals()` gives the
+    device time per entry point in ms after a synchronize."""
+
+    active: "CallTimer | None" = None
+
+    def __init__(self):
+        self.records: list[tuple[str, torch.cuda.Event, torch.cuda.Event]] = []
+
+    def __enter__(self):
+        CallTimer.active = self
+        return self
+
+    def __exit__(self, *exc):
+        CallTimer.active = None
+
+    def totals(self) -> dict[str, tuple[float, int]]:
+        out: dict[str, list] = {}
+        for name, a, b in self.records:
+            t = out.setdefault(name, [0.0, 0])
+            t[0] += a.elapsed_time(b)
+            t[1] += 1
+        return {k: (v[0], v[1]) for k, v in out.items()}
+
+
 def call(name: str, *args) -> None:
     """Calls a library function and raises RuntimeError on a non-zero status."""
     lib = load()
-    rc = getattr(lib, name)(*args)
+    timer = CallTimer.active
+    if timer is not None:
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        rc = getattr(lib, name)(*args)
+        e1.record()
+        timer.records.append((name, e0, e1))
+    else:
+        rc = getattr(lib, name)(*args)
     if rc != 0:
         msg = lib.pvdb_last_error().decode(errors="replace")
         raise RuntimeError(f"{name} failed ({rc}): {msg}")
