@@ -116,6 +116,89 @@ def bench_plain(args, scene):
     }
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+# MLP FLOPs per sample / per ray (PE-factorised first layer): forward 36,608 +
+# 6,912/ray; backward 73,216 + 6,912/ray.
+MLP_FWD_FLOP, MLP_BWD_FLOP, MLP_RAY_FLOP = 36608, 73216, 6912
+
+
+def bench_batched(args, scene):
+    import torch
+
+    from plenvdb_amd import fused
+
+    tr = fused.FusedTrainer(scene)
+    for _ in range(args.warmup):
+        tr.step()
+    torch.cuda.synchronize()
+    fused.stage_timing(1)
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(args.steps):
+        tr.step()
+    t1.record()
+    torch.cuda.synchronize()
+    stages = fused.stage_timing(0)
+    ms = t0.elapsed_time(t1) / args.steps
+    samples = tr.total_records()
+    # workload descriptors of one more (untimed) forward/backward
+    tr.zero_grad()
+    tr.forward_backward()
+    torch.cuda.synchronize()
+    rec = tr.records()
+    flags = rec["atwr"][:, 3].view(np.uint32)
+    alive_w = (flags & np.uint32(1 << 31)) != 0
+    gd = tr.grad_density.cpu().numpy()
+    gc = tr.grad_color.cpu().numpy()
+    touched_leaf = (gd.reshape(-1, 512) != 0).any(1) | (gc.reshape(-1, 512 * 12) != 0).any(1)
+    act = tr.topo.active_slot_mask()
+    v_u = int(((gd != 0) & act).sum())
+    tr.zero_grad()
+    return {
+        "ms_per_step": ms,
+        "samples_per_step": float(samples),
+        "stages": stages,
+        "n_color": int(alive_w.sum()),
+        "points": rec["xyze"][:, :3],
+        "l_t": int(touched_leaf.sum()),
+        "v_u": v_u,
+        "n_rays": tr.n_rays,
+        "color_bytes": 2 if tr.color_bf16 else 4,
+    }
+
+
+def batched_roofline(r, steps):
+    """Roofline of the dominant stage and the survey's whole-step byte model."""
+    st = r["stages"]
+    name = max(st, key=lambda k: st[k][0])
+    tot, calls = st[name]
+    avg_s = tot / max(calls, 1) * 1e-3
+    S, N, sc = r["samples_per_step"], r["n_rays"], r["color_bytes"]
+    uniq = unique_corner_voxels(r["points"][:, 0], r["points"][:, 1], r["points"][:, 2])
+    if name in ("mlp_bwd", "mlp_fwd"):
+        flop = (S * (MLP_FWD_FLOP + MLP_BWD_FLOP) + N * 2 * MLP_RAY_FLOP if name == "mlp_bwd"
+                else S * MLP_FWD_FLOP + N * MLP_RAY_FLOP)
+        achieved = flop / avg_s / 1e12
+        roof = {"kernel": name, "bound": "mfma", "achieved": round(achieved, 3),
+                "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 6), "traffic": None,
+                "algorithmic_flop": flop, "avg_launch_ms": round(avg_s * 1e3, 5)}
+    else:
+        # march: rays in, records out, each distinct density voxel read once
+        b = N * 24 + S * 32 + uniq * 4
+        achieved = b / avg_s / 1e9
+        roof = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "algorithmic_bytes": b, "avg_launch_ms": round(avg_s * 1e3, 5)}
+    L_t, V_u = r["l_t"], r["v_u"]
+    b_step = (L_t * 512 * (4 + 12 * sc) + L_t * 512 * 52 + V_u * (28 + 12 * (2 * sc + 20)) + N * 60)
+    step = {"bytes": b_step, "achieved_GBps": round(b_step / (r["ms_per_step"] * 1e-3) / 1e9, 2),
+            "frac": round(b_step / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+            "leaves_touched": L_t, "voxels_updated": V_u}
+    return roof, step
+
+
 def roofline_from(totals, launches_bytes):
     """Pick the kernel with the largest total device time among those with a byte model."""
     best = None
@@ -188,19 +271,22 @@ def main():
 
     scene = scenes.make_scene(args.config)
     cfg = scene["cfg"]
-    path = "plain" if args.path == "auto" else args.path
-    res = {"plain": bench_plain(args, scene)}
+    path = "batched" if args.path == "auto" else args.path
+    res = {}
+    if path == "batched" or args.path == "auto":
+        res["batched"] = bench_batched(args, scene)
+    res["plain"] = bench_plain(args, scene)
     main_res = res[path]
-    p = main_res["points_density"]
-    uniq_d = unique_corner_voxels(p[:, 0], p[:, 1], p[:, 2])
-    nb = {
-        "pvdb_trilinear_bwd": None,
-    }
-    # byte model: the colour scatter dominates; per launch it handles the final samples
-    n_col = main_res["n_color"]
-    nb["pvdb_trilinear_bwd"] = kernel_bytes("pvdb_trilinear_bwd", n_col, uniq_d, 12)
-    nb["pvdb_trilinear_fwd"] = kernel_bytes("pvdb_trilinear_fwd", n_col, uniq_d, 12)
-    roof = roofline_from(main_res["totals"], nb)
+    step_roof = None
+    if path == "batched":
+        roof, step_roof = batched_roofline(main_res, args.steps)
+    else:
+        p = main_res["points_density"]
+        uniq_d = unique_corner_voxels(p[:, 0], p[:, 1], p[:, 2])
+        n_col = main_res["n_color"]
+        nb = {"pvdb_trilinear_bwd": kernel_bytes("pvdb_trilinear_bwd", n_col, uniq_d, 12),
+              "pvdb_trilinear_fwd": kernel_bytes("pvdb_trilinear_fwd", n_col, uniq_d, 12)}
+        roof = roofline_from(main_res["totals"], nb)
     value = main_res["samples_per_step"] / (main_res["ms_per_step"] * 1e-3)
     line = {
         "metric": METRIC,
@@ -223,9 +309,16 @@ def main():
         "paths": {k: {"ms_per_step": round(v["ms_per_step"], 4),
                       "samples_per_s": round(v["samples_per_step"] / (v["ms_per_step"] * 1e-3), 1)}
                   for k, v in res.items()},
-        "kernels_ms": {k: round(v[0] / args.steps, 4) for k, v in sorted(
-            main_res["totals"].items(), key=lambda kv: -kv[1][0])},
+        "kernels_ms": ({k: round(v[0] / args.steps, 4) for k, v in sorted(
+            main_res["stages"].items(), key=lambda kv: -kv[1][0])} if path == "batched" else
+            {k: round(v[0] / args.steps, 4) for k, v in sorted(
+                main_res["totals"].items(), key=lambda kv: -kv[1][0])}),
+        "step_bytes_model": step_roof,
     }
+    if "batched" in res and "plain" in res:
+        line["speedup_batched_vs_plain"] = round(res["plain"]["ms_per_step"] / res["batched"]["ms_per_step"]
+                                                 * res["batched"]["samples_per_step"]
+                                                 / res["plain"]["samples_per_step"], 3)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.config, args.cpu_rays)
     else:

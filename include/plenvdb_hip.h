@@ -204,6 +204,14 @@ int64_t pvdb_fused_workspace_bytes(int64_t n_rays, int64_t capacity, int64_t nle
 /* Clears the workspace (touched-leaf flags, buffers); call once after allocating it. */
 int pvdb_fused_init(const pvdb_topology* topo, const pvdb_step_desc* desc, void* stream);
 int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* desc, void* stream);
+/* Optional device timing of the fused step's stages (HIP events on its stream).
+ * No counterpart in the reference (which times only the renderer, with CPU
+ * clock(), plenvdb.h:1030-1035).  Collects pending events into per-stage totals
+ * and copies up to n of them to total_ms / calls; enable = 1 starts (and
+ * resets), 0 stops (and resets), -1 only reads.  Must stay off while a caller
+ * captures the step into a graph. */
+int pvdb_fused_timing(int32_t enable, double* total_ms, int32_t* calls, int32_t n);
+const char* pvdb_fused_stage_name(int32_t i);
 /* Device pointers of the per-sample records and per-ray counts inside the
  * workspace (for inspection and tests). */
 int pvdb_fused_records(const pvdb_topology* topo, const pvdb_step_desc* desc, void** xyze,

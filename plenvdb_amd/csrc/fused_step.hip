@@ -7,6 +7,8 @@
 // with no host synchronisation and no allocation, so a caller can capture it
 // in a hipGraph.  All scratch lives in the caller's workspace.
 #include <cmath>
+#include <utility>
+#include <vector>
 
 #include "fused_common.h"
 #include "plenvdb_hip.h"
@@ -145,9 +147,65 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
     }
 }
 
+// Optional per-stage device timing (HIP events on the step's stream).
+constexpr int kStages = 8;
+const char* const kStageNames[kStages] = {"pack_mlp",  "ray_count", "ray_march", "mlp_fwd",
+                                          "composite", "mlp_bwd",   "dw_reduce", "adam"};
+struct StageTimer {
+    bool enabled = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> open;
+    double total_ms[kStages] = {0};
+    int calls[kStages] = {0};
+};
+StageTimer g_timer;
+
+struct StageScope {
+    int stage;
+    hipStream_t s;
+    hipEvent_t a = nullptr, b = nullptr;
+    StageScope(int st, hipStream_t str) : stage(st), s(str) {
+        if (!g_timer.enabled) return;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        hipEventRecord(a, s);
+    }
+    ~StageScope() {
+        if (!g_timer.enabled || !a) return;
+        hipEventRecord(b, s);
+        g_timer.open.push_back({stage, {a, b}});
+    }
+};
+
 }  // namespace
 
 extern "C" {
+
+int pvdb_fused_timing(int32_t enable, double* total_ms, int32_t* calls, int32_t n) {
+    for (auto& e : g_timer.open) {
+        hipEventSynchronize(e.second.second);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e.second.first, e.second.second);
+        g_timer.total_ms[e.first] += ms;
+        g_timer.calls[e.first] += 1;
+        hipEventDestroy(e.second.first);
+        hipEventDestroy(e.second.second);
+    }
+    g_timer.open.clear();
+    for (int i = 0; i < n && i < kStages; ++i) {
+        if (total_ms) total_ms[i] = g_timer.total_ms[i];
+        if (calls) calls[i] = g_timer.calls[i];
+    }
+    if (enable >= 0) {
+        g_timer.enabled = enable != 0;
+        for (int i = 0; i < kStages; ++i) {
+            g_timer.total_ms[i] = 0;
+            g_timer.calls[i] = 0;
+        }
+    }
+    return PVDB_OK;
+}
+
+const char* pvdb_fused_stage_name(int32_t i) { return (i >= 0 && i < kStages) ? kStageNames[i] : ""; }
 
 int64_t pvdb_fused_workspace_bytes(int64_t n_rays, int64_t capacity, int64_t nleaf) {
     if (n_rays < 0 || capacity < 0 || nleaf < 0) return -1;
@@ -253,27 +311,49 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     W.b2 = d->mlp + kB2;
 
     int rc;
-    if ((rc = launch_pack_mlp(d->mlp, w0t, w1t, w2t, w2, w1, w0, s))) return rc;
-    hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, R.loss, (int64_t)1, 0.f);
-    PVDB_LAUNCHED("k_fill_f32");
-    if ((rc = launch_ray_count(P, t, R, s))) return rc;
-    if ((rc = launch_ray_march(P, t, d->density, R, rec, s))) return rc;
+    {
+        StageScope sc(0, s);
+        if ((rc = launch_pack_mlp(d->mlp, w0t, w1t, w2t, w2, w1, w0, s))) return rc;
+        hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, R.loss, (int64_t)1, 0.f);
+        PVDB_LAUNCHED("k_fill_f32");
+    }
+    {
+        StageScope sc(1, s);
+        if ((rc = launch_ray_count(P, t, R, s))) return rc;
+    }
+    {
+        StageScope sc(2, s);
+        if ((rc = launch_ray_march(P, t, d->density, R, rec, s))) return rc;
+    }
     // The sample-side kernels are persistent over the records; their grid is
     // fixed (the record count stays on the device).
     const int grid = kMaxMlpGrid;
-    if ((rc = launch_mlp(false, d->color_bf16 != 0, grid, P, t, d->color, W, R, rec, nullptr,
-                         nullptr, touched, nullptr, s)))
-        return rc;
-    if ((rc = launch_ray_composite(P, R, rec, s))) return rc;
-    if ((rc = launch_mlp(true, d->color_bf16 != 0, grid, P, t, d->color, W, R, rec,
-                         d->grad_density, d->grad_color, touched, dw_part, s)))
-        return rc;
-    if ((rc = launch_dw_reduce(dw_part, grid, d->mlp_grad, s))) return rc;
+    {
+        StageScope sc(3, s);
+        if ((rc = launch_mlp(false, d->color_bf16 != 0, grid, P, t, d->color, W, R, rec, nullptr,
+                             nullptr, touched, nullptr, s)))
+            return rc;
+    }
+    {
+        StageScope sc(4, s);
+        if ((rc = launch_ray_composite(P, R, rec, s))) return rc;
+    }
+    {
+        StageScope sc(5, s);
+        if ((rc = launch_mlp(true, d->color_bf16 != 0, grid, P, t, d->color, W, R, rec,
+                             d->grad_density, d->grad_color, touched, dw_part, s)))
+            return rc;
+    }
+    {
+        StageScope sc(6, s);
+        if ((rc = launch_dw_reduce(dw_part, grid, d->mlp_grad, s))) return rc;
+    }
     if (d->ray_out) {
         PVDB_HIP(hipMemcpyAsync(d->ray_out, R.ray_out, 16 * (size_t)d->n_rays,
                                 hipMemcpyDeviceToDevice, s));
     }
     if (d->do_adam) {
+        StageScope sc(7, s);
         const int ag = (int)std::min<int64_t>(t.nleaf > 0 ? t.nleaf : 1, 4096);
         if (d->color_bf16)
             hipLaunchKernelGGL(k_sparse_adam<true>, dim3(ag), dim3(256), 0, s, t, touched,

@@ -43,6 +43,7 @@ class StepDesc(C.Structure):
 
 
 lib.EXTRA_SIGS.update({
+    "pvdb_fused_timing": [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32],
     "pvdb_fused_init": [C.c_void_p, C.POINTER(StepDesc), C.c_void_p],
     "pvdb_fused_step": [C.c_void_p, C.POINTER(StepDesc), C.c_void_p],
     "pvdb_fused_records": [C.c_void_p, C.POINTER(StepDesc)] + [C.c_void_p] * 7,
@@ -211,6 +212,11 @@ class FusedTrainer:
         self.grad_color.zero_()
         self.mlp_grad.zero_()
 
+    def total_records(self) -> int:
+        ptrs = [C.c_void_p() for _ in range(7)]
+        lib.call("pvdb_fused_records", self.topo.handle, C.byref(self.desc), *[C.byref(p) for p in ptrs])
+        return int(_read_i32(ptrs[6].value, 1)[0])
+
     def records(self) -> dict:
         """Host copies of the per-sample records and per-ray counts of the last step."""
         ptrs = [C.c_void_p() for _ in range(7)]
@@ -223,6 +229,21 @@ class FusedTrainer:
         for k, p in zip(("xyze", "atwr", "rgb", "cinc"), ptrs[:4]):
             out[k] = _read_f32(p.value, 4 * total).reshape(total, 4)
         return out
+
+
+N_STAGES = 8
+
+
+def stage_timing(enable: int) -> dict:
+    """Collects the fused step's per-stage device times (ms totals, calls);
+    enable 1 starts, 0 stops, -1 only reads (see pvdb_fused_timing)."""
+    L = lib.load()
+    L.pvdb_fused_stage_name.restype = C.c_char_p
+    L.pvdb_fused_stage_name.argtypes = [C.c_int32]
+    tot = (C.c_double * N_STAGES)()
+    calls = (C.c_int32 * N_STAGES)()
+    lib.call("pvdb_fused_timing", int(enable), tot, calls, N_STAGES)
+    return {L.pvdb_fused_stage_name(i).decode(): (tot[i], calls[i]) for i in range(N_STAGES)}
 
 
 def _wrap(ptr: int, n: int, typestr: str) -> torch.Tensor:

@@ -58,7 +58,8 @@ EXTRA_SIGS: dict[str, list] = {}
 
 
 def declared_symbols() -> list[str]:
-    return sorted(set(_SIGS) | set(EXTRA_SIGS) | {"pvdb_last_error"})
+    return sorted(set(_SIGS) | set(EXTRA_SIGS) |
+                  {"pvdb_last_error", "pvdb_fused_workspace_bytes", "pvdb_fused_stage_name"})
 
 
 def load() -> C.CDLL:
