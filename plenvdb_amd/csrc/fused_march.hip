@@ -237,8 +237,22 @@ __global__ void __launch_bounds__(256) k_ray_composite(StepParams P, RayBuffers 
         R.ray_out[r] = make_float4(o0, o1, o2, tl);
         R.ray_grad[r] = make_float4(g0, g1, g2, P.bg * (g0 + g1 + g2));
         R.ray_ctot[r] = make_float4(cr, cg, cb, 0.f);
-        atomicAdd(R.loss, (d0 * d0 + d1 * d1 + d2 * d2) * P.loss_scale);
+        R.ray_loss[r] = (d0 * d0 + d1 * d1 + d2 * d2) * P.loss_scale;
     }
+}
+
+// MSE loss = sum of the per-ray terms (one workgroup; the order of the sum is fixed).
+__global__ void __launch_bounds__(1024) k_loss_reduce(RayBuffers R, int n) {
+    __shared__ float part[1024];
+    float s = 0.f;
+    for (int i = threadIdx.x; i < n; i += 1024) s += R.ray_loss[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int d = 512; d > 0; d >>= 1) {
+        if ((int)threadIdx.x < d) part[threadIdx.x] += part[threadIdx.x + d];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) R.loss[0] = part[0];
 }
 
 }  // namespace
@@ -265,6 +279,8 @@ int launch_ray_composite(const StepParams& P, const RayBuffers& R, const Records
                          hipStream_t s) {
     hipLaunchKernelGGL(k_ray_composite, dim3(pvdb_blocks(P.n_rays, 4)), dim3(256), 0, s, P, R, rec);
     PVDB_LAUNCHED("k_ray_composite");
+    hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(1024), 0, s, R, P.n_rays);
+    PVDB_LAUNCHED("k_loss_reduce");
     return PVDB_OK;
 }
 

@@ -13,9 +13,11 @@
 // g_X^T = W0[0:12] g_H0^T, and accumulates dW with v_mfma_f32_16x16x16_bf16
 // over the samples (K = 16) into per-wave register tiles that are written once
 // per workgroup.  The same pass evaluates the Alphas2Weights / raw2alpha
-// adjoints (render_utils_kernel.cu:653-676, 506-517) and scatters the density
-// and colour gradients with the "leaf must exist" rule (densityvdb.cu:16-27,
-// colorvdb.cu:28-37).
+// adjoints (render_utils_kernel.cu:653-676, 506-517) and writes each record's
+// feature and sigma gradients; k_scatter_agg (or k_scatter) then adds them to
+// the grids with the "leaf must exist" rule (densityvdb.cu:16-27,
+// colorvdb.cu:28-37).  The forward pass keeps the gathered BF16 features per
+// record, so the backward reads 32 bytes per record instead of 8 gathers.
 #include "fused_common.h"
 #include "gather.h"
 #include "plenvdb_hip.h"
@@ -84,9 +86,6 @@ __device__ __forceinline__ float sigmoidf_(float x) { return 1.f / (1.f + expf(-
 template <bool BWD, bool BF16COL>
 __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const void* color,
                                              MlpPacked W, RayBuffers R, Records rec,
-                                             float* __restrict__ grad_density,
-                                             float* __restrict__ grad_color,
-                                             uint8_t* __restrict__ leaf_touched,
                                              float* __restrict__ dw_partial) {
     __shared__ Smem sm;
     const int tid = threadIdx.x;
@@ -176,7 +175,25 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
         }
         __syncthreads();
         // ---- 2. X = [colour features (12) | PE (27) | 1 | 0...] ----
-        if (tid < 128) {
+        if (BWD && tid < 16) {
+            // the forward pass kept the gathered features (bf16) per record
+            const int s = tid;
+            const RecInfo ri = sm.ri[s];
+            uint4 f0v = make_uint4(0u, 0u, 0u, 0u), f1v = make_uint4(0u, 0u, 0u, 0u);
+            if (ri.bits & kAliveW) {
+                f0v = rec.feat[2 * (base + s)];
+                f1v = rec.feat[2 * (base + s) + 1];
+            }
+            const uint32_t w6[6] = {f0v.x, f0v.y, f0v.z, f0v.w, f1v.x, f1v.y};
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                const uint16_t lo = (uint16_t)(w6[c] & 0xffffu), hi = (uint16_t)(w6[c] >> 16);
+                sm.Xs[s * XM + 2 * c] = lo;
+                sm.Xs[s * XM + 2 * c + 1] = hi;
+                sm.Xf[(2 * c) * FM + s] = lo;
+                sm.Xf[(2 * c + 1) * FM + s] = hi;
+            }
+        } else if (!BWD && tid < 128) {
             const int s = tid >> 3, q = tid & 7;
             const RecInfo ri = sm.ri[s];
             float v[12];
@@ -219,14 +236,23 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                 v[c] += __shfl_xor(v[c], 4, 64);
             }
             if (q == 0) {
+                uint16_t h[12];
 #pragma unroll
                 for (int c = 0; c < 12; ++c) {
-                    const uint16_t h = f2bf_bits(v[c]);
-                    sm.Xs[s * XM + c] = h;
-                    sm.Xf[c * FM + s] = h;
+                    h[c] = f2bf_bits(v[c]);
+                    sm.Xs[s * XM + c] = h[c];
+                    sm.Xf[c * FM + s] = h[c];
+                }
+                const int64_t qq = base + s;
+                if (qq < total && (ri.bits & kAliveW)) {
+                    uint32_t w6[6];
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) w6[c] = (uint32_t)h[2 * c] | ((uint32_t)h[2 * c + 1] << 16);
+                    rec.feat[2 * qq] = make_uint4(w6[0], w6[1], w6[2], w6[3]);
+                    rec.feat[2 * qq + 1] = make_uint4(w6[4], w6[5], 0u, 0u);
                 }
             }
-        } else {
+        } else if (tid >= 128) {
             const int tt = tid - 128, s = tt >> 3, part = tt & 7;
             const RecInfo ri = sm.ri[s];
             const bool alive = (ri.bits & kAliveW) != 0u;
@@ -428,33 +454,28 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
             for (int i = 0; i < 4; ++i) sm.GXp[wv][4 * g + i][r16] = acc[i];
         }
         __syncthreads();
-        // ---- 10. scatter into the grid gradients ----
-        if (tid < 128) {
-            const int s = tid >> 3, q = tid & 7;
-            const RecInfo ri = sm.ri[s];
-            if (ri.bits & (kAliveA | kAliveW)) {
-                const Cell cell = make_cell(ri.x, ri.y, ri.z);
-                const int dx = (kCx >> q) & 1, dy = (kCy >> q) & 1, dz = (kCz >> q) & 1;
-                const float a = dx ? cell.u : 1.f - cell.u;
-                const float b = dy ? cell.v : 1.f - cell.v;
-                const float d = dz ? cell.w : 1.f - cell.w;
-                const int32_t leaf = leaf_of(topo, cell.i0 + dx, cell.j0 + dy, cell.k0 + dz);
-                if (leaf >= 0) {
-                    const int64_t slot = (int64_t)leaf * 512 +
-                                         voxel_offset(cell.i0 + dx, cell.j0 + dy, cell.k0 + dz);
-                    leaf_touched[leaf] = 1;
-                    if (ri.bits & kAliveA) atomicAdd(grad_density + slot, ((sm.gsig[s] * a) * b) * d);
+        // ---- 10. per-record gradients: dL/dfeature (12) and dL/dsigma ----
+        if (tid < 64) {
+            const int s = tid >> 2, part = tid & 3;  // 4 threads write the 4 float4 of a record
+            const int64_t q = base + s;
+            if (q < total) {
+                const RecInfo ri = sm.ri[s];
+                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (part < 3) {
                     if (ri.bits & kAliveW) {
-                        const float sc = (a * b) * d;
-                        float* dst = grad_color + slot * 12;
+                        float g4[4];
 #pragma unroll
-                        for (int c = 0; c < 12; ++c) {
-                            const float gx = sm.GXp[0][c][s] + sm.GXp[1][c][s] + sm.GXp[2][c][s] +
-                                             sm.GXp[3][c][s];
-                            atomicAdd(dst + c, gx * sc);
+                        for (int k = 0; k < 4; ++k) {
+                            const int c = 4 * part + k;
+                            g4[k] = sm.GXp[0][c][s] + sm.GXp[1][c][s] + sm.GXp[2][c][s] +
+                                    sm.GXp[3][c][s];
                         }
+                        v = make_float4(g4[0], g4[1], g4[2], g4[3]);
                     }
+                } else {
+                    v.x = sm.gsig[s];
                 }
+                rec.grad[4 * q + part] = v;
             }
         }
         __syncthreads();
@@ -506,13 +527,60 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
     }
 }
 
-// Sum of the per-workgroup dW partials.
+// Sum of the per-workgroup dW partials, accumulated into dw: blockIdx.y takes
+// a chunk of kReduceChunk partials, so the reduction runs on enough
+// workgroups to stream the partials at HBM rate.
+constexpr int kReduceChunk = 16;
 __global__ void k_dw_reduce(const float* __restrict__ part, int nparts, float* __restrict__ dw) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= kMlpParams) return;
+    const int b0 = blockIdx.y * kReduceChunk, b1 = min(nparts, b0 + kReduceChunk);
     float s = 0.f;
-    for (int b = 0; b < nparts; ++b) s += part[(int64_t)b * kMlpParams + p];
-    dw[p] = s;
+    for (int b = b0; b < b1; ++b) s += part[(int64_t)b * kMlpParams + p];
+    atomicAdd(dw + p, s);
+}
+
+// Gradients into the grids: one thread per (record, corner), with the
+// "leaf must exist" rule (densityvdb.cu:16-27, colorvdb.cu:28-37); the density
+// term keeps the reference's ((g*a)*b)*c order.  Touched leaves are flagged
+// for the sparse Adam pass.
+__global__ void __launch_bounds__(256) k_scatter(TopoView topo, RayBuffers R, Records rec,
+                                                 int64_t capacity, float* __restrict__ grad_density,
+                                                 float* __restrict__ grad_color,
+                                                 uint8_t* __restrict__ leaf_touched) {
+    const int64_t total = min((int64_t)R.total[0], capacity);
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total * 8;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = t >> 3;
+        const int corner = (int)(t & 7);
+        const uint32_t bits = __float_as_uint(rec.atwr[q].w);
+        if (!(bits & (kAliveA | kAliveW))) continue;
+        const float4 p = rec.xyze[q];
+        const Cell cell = make_cell(p.x, p.y, p.z);
+        const int dx = (kCx >> corner) & 1, dy = (kCy >> corner) & 1, dz = (kCz >> corner) & 1;
+        const int i = cell.i0 + dx, j = cell.j0 + dy, k = cell.k0 + dz;
+        const int32_t leaf = leaf_of(topo, i, j, k);
+        if (leaf < 0) continue;
+        const float a = dx ? cell.u : 1.f - cell.u;
+        const float b = dy ? cell.v : 1.f - cell.v;
+        const float d = dz ? cell.w : 1.f - cell.w;
+        const int64_t slot = (int64_t)leaf * 512 + voxel_offset(i, j, k);
+        leaf_touched[leaf] = 1;
+        const float4* g = rec.grad + 4 * q;
+        if (bits & kAliveA) atomicAdd(grad_density + slot, ((g[3].x * a) * b) * d);
+        if (bits & kAliveW) {
+            const float sc = (a * b) * d;
+            float* dst = grad_color + slot * 12;
+#pragma unroll
+            for (int part = 0; part < 3; ++part) {
+                const float4 v = g[part];
+                atomicAdd(dst + 4 * part + 0, v.x * sc);
+                atomicAdd(dst + 4 * part + 1, v.y * sc);
+                atomicAdd(dst + 4 * part + 2, v.z * sc);
+                atomicAdd(dst + 4 * part + 3, v.w * sc);
+            }
+        }
+    }
 }
 
 // fp32 master parameters (kMlpParams layout) -> packed bf16 MFMA operands.
@@ -554,31 +622,130 @@ int mlp_grid_size(int64_t total) {
 
 int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const TopoView& t,
                const void* color, const MlpPacked& W, const RayBuffers& R, const Records& rec,
-               float* grad_density, float* grad_color, uint8_t* leaf_touched, float* dw_partial,
-               hipStream_t s) {
+               float* dw_partial, hipStream_t s) {
     if (bwd) {
         if (bf16col)
             hipLaunchKernelGGL((k_mlp<true, true>), dim3(grid), dim3(256), 0, s, P, t, color, W, R,
-                               rec, grad_density, grad_color, leaf_touched, dw_partial);
+                               rec, dw_partial);
         else
             hipLaunchKernelGGL((k_mlp<true, false>), dim3(grid), dim3(256), 0, s, P, t, color, W, R,
-                               rec, grad_density, grad_color, leaf_touched, dw_partial);
+                               rec, dw_partial);
     } else {
         if (bf16col)
             hipLaunchKernelGGL((k_mlp<false, true>), dim3(grid), dim3(256), 0, s, P, t, color, W, R,
-                               rec, grad_density, grad_color, leaf_touched, dw_partial);
+                               rec, dw_partial);
         else
             hipLaunchKernelGGL((k_mlp<false, false>), dim3(grid), dim3(256), 0, s, P, t, color, W,
-                               R, rec, grad_density, grad_color, leaf_touched, dw_partial);
+                               R, rec, dw_partial);
     }
     PVDB_LAUNCHED("k_mlp");
     return PVDB_OK;
 }
 
+// Same result as k_scatter with far fewer global atomics: a workgroup takes
+// kAggRecords consecutive records (a few rays; consecutive samples of a ray
+// share most of their corners), sums their (slot, gradient) contributions in
+// an LDS hash table keyed by the 32-bit slot, then adds each distinct slot's
+// 13 sums to the grids once.  The flush gives 16 lanes to a slot, so one
+// atomic instruction covers four slots' contiguous 12-float colour rows.
+constexpr int kAggRecords = 128;
+constexpr int kAggSlots = 8 * kAggRecords;  // every contribution fits: probing ends
+constexpr uint32_t kEmptyKey = 0xffffffffu;
+__global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R, Records rec,
+                                                     int64_t capacity,
+                                                     float* __restrict__ grad_density,
+                                                     float* __restrict__ grad_color,
+                                                     uint8_t* __restrict__ leaf_touched) {
+    __shared__ uint32_t key[kAggSlots];
+    __shared__ float val[13][kAggSlots];
+    const int tid = threadIdx.x;
+    const int64_t total = min((int64_t)R.total[0], capacity);
+    const int64_t nchunk = (total + kAggRecords - 1) / kAggRecords;
+    for (int64_t chunk = blockIdx.x; chunk < nchunk; chunk += gridDim.x) {
+        for (int e = tid; e < kAggSlots; e += 256) {
+            key[e] = kEmptyKey;
+#pragma unroll
+            for (int c = 0; c < 13; ++c) val[c][e] = 0.f;
+        }
+        __syncthreads();
+        for (int p = tid; p < 8 * kAggRecords; p += 256) {
+            const int64_t q = chunk * kAggRecords + (p >> 3);
+            if (q >= total) continue;
+            const int corner = p & 7;
+            const uint32_t bits = __float_as_uint(rec.atwr[q].w);
+            if (!(bits & (kAliveA | kAliveW))) continue;
+            const float4 pt = rec.xyze[q];
+            const Cell cell = make_cell(pt.x, pt.y, pt.z);
+            const int dx = (kCx >> corner) & 1, dy = (kCy >> corner) & 1, dz = (kCz >> corner) & 1;
+            const int i = cell.i0 + dx, j = cell.j0 + dy, k = cell.k0 + dz;
+            const int32_t leaf = leaf_of(topo, i, j, k);
+            if (leaf < 0) continue;
+            const uint32_t slot = (uint32_t)leaf * 512u + (uint32_t)voxel_offset(i, j, k);
+            const float a = dx ? cell.u : 1.f - cell.u;
+            const float b = dy ? cell.v : 1.f - cell.v;
+            const float d = dz ? cell.w : 1.f - cell.w;
+            uint32_t h = (slot * 2654435761u) >> (32 - 10);  // kAggSlots == 1 << 10
+            while (true) {
+                const uint32_t prev = atomicCAS(&key[h], kEmptyKey, slot);
+                if (prev == kEmptyKey || prev == slot) break;
+                h = (h + 1) & (kAggSlots - 1);
+            }
+            const float4* g = rec.grad + 4 * q;
+            if (bits & kAliveA) atomicAdd(&val[12][h], ((g[3].x * a) * b) * d);
+            if (bits & kAliveW) {
+                const float sc = (a * b) * d;
+#pragma unroll
+                for (int part = 0; part < 3; ++part) {
+                    const float4 v = g[part];
+                    atomicAdd(&val[4 * part + 0][h], v.x * sc);
+                    atomicAdd(&val[4 * part + 1][h], v.y * sc);
+                    atomicAdd(&val[4 * part + 2][h], v.z * sc);
+                    atomicAdd(&val[4 * part + 3][h], v.w * sc);
+                }
+            }
+        }
+        __syncthreads();
+        const int c = tid & 15;
+        for (int e = tid >> 4; e < kAggSlots; e += 16) {
+            const uint32_t slot = key[e];
+            if (slot == kEmptyKey) continue;
+            if (c == 13) leaf_touched[slot >> 9] = 1;
+            if (c > 12) continue;
+            const float v = val[c][e];
+            if (v == 0.f) continue;  // adding zero changes nothing
+            if (c < 12)
+                atomicAdd(grad_color + (int64_t)slot * 12 + c, v);
+            else
+                atomicAdd(grad_density + slot, v);
+        }
+        __syncthreads();
+    }
+}
+
 int launch_dw_reduce(const float* part, int nparts, float* dw, hipStream_t s) {
-    hipLaunchKernelGGL(k_dw_reduce, dim3(pvdb_blocks(kMlpParams, 256)), dim3(256), 0, s, part,
-                       nparts, dw);
+    const unsigned chunks = (unsigned)((nparts + kReduceChunk - 1) / kReduceChunk);
+    hipLaunchKernelGGL(k_dw_reduce, dim3(pvdb_blocks(kMlpParams, 256), chunks), dim3(256), 0, s,
+                       part, nparts, dw);
     PVDB_LAUNCHED("k_dw_reduce");
+    return PVDB_OK;
+}
+
+int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, int64_t capacity,
+                   float* grad_density, float* grad_color, uint8_t* leaf_touched, hipStream_t s) {
+    if (t.nleaf < (int64_t(1) << 23) - 1) {  // every slot < 2^32 - 1, the empty key
+        const int64_t chunks = (capacity + kAggRecords - 1) / kAggRecords;
+        const unsigned grid = (unsigned)(chunks < 2048 ? (chunks < 1 ? 1 : chunks) : 2048);
+        hipLaunchKernelGGL(k_scatter_agg, dim3(grid), dim3(256), 0, s, t, R, rec, capacity,
+                           grad_density, grad_color, leaf_touched);
+        PVDB_LAUNCHED("k_scatter_agg");
+        return PVDB_OK;
+    }
+    // enough threads for a few (record, corner) pairs each at the largest sizes
+    const int64_t want = (capacity * 8 + 255) / 256;
+    const unsigned grid = (unsigned)(want < 16384 ? (want < 1 ? 1 : want) : 16384);
+    hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, s, t, R, rec, capacity, grad_density,
+                       grad_color, leaf_touched);
+    PVDB_LAUNCHED("k_scatter");
     return PVDB_OK;
 }
 

@@ -23,9 +23,10 @@ int launch_ray_composite(const StepParams& P, const RayBuffers& R, const Records
 int mlp_grid_size(int64_t total);
 int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const TopoView& t,
                const void* color, const MlpPacked& W, const RayBuffers& R, const Records& rec,
-               float* grad_density, float* grad_color, uint8_t* leaf_touched, float* dw_partial,
-               hipStream_t s);
+               float* dw_partial, hipStream_t s);
 int launch_dw_reduce(const float* part, int nparts, float* dw, hipStream_t s);
+int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, int64_t capacity,
+                   float* grad_density, float* grad_color, uint8_t* leaf_touched, hipStream_t s);
 int launch_pack_mlp(const float* prm, uint16_t* w0t, uint16_t* w1t, uint16_t* w2t, uint16_t* w2,
                     uint16_t* w1, uint16_t* w0, hipStream_t s);
 }  // namespace pvdb
@@ -37,8 +38,8 @@ namespace {
 constexpr int kMaxMlpGrid = 512;
 
 struct Layout {
-    size_t count, offset, total, t_last, ray_out, ray_grad, ray_ctot, loss;
-    size_t xyze, atwr, rgb, cinc;
+    size_t count, offset, total, t_last, ray_out, ray_grad, ray_ctot, ray_loss, loss;
+    size_t xyze, atwr, rgb, cinc, feat, grad;
     size_t touched, dw_part, w0t, w1t, w2t, w2, w1, w0;
     size_t bytes;
 };
@@ -60,11 +61,14 @@ Layout layout_for(int64_t n_rays, int64_t capacity, int64_t nleaf) {
     L.ray_out = take(16 * n_rays);
     L.ray_grad = take(16 * n_rays);
     L.ray_ctot = take(16 * n_rays);
+    L.ray_loss = take(4 * n_rays);
     L.loss = take(4);
     L.xyze = take(16 * capacity);
     L.atwr = take(16 * capacity);
     L.rgb = take(16 * capacity);
     L.cinc = take(16 * capacity);
+    L.feat = take(32 * capacity);
+    L.grad = take(64 * capacity);
     L.touched = take(nleaf > 0 ? nleaf : 1);
     L.dw_part = take((size_t)4 * kMlpParams * kMaxMlpGrid);
     L.w0t = take(2 * 128 * 64);
@@ -148,9 +152,10 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
 }
 
 // Optional per-stage device timing (HIP events on the step's stream).
-constexpr int kStages = 8;
-const char* const kStageNames[kStages] = {"pack_mlp",  "ray_count", "ray_march", "mlp_fwd",
-                                          "composite", "mlp_bwd",   "dw_reduce", "adam"};
+constexpr int kStages = 9;
+const char* const kStageNames[kStages] = {"pack_mlp", "ray_count", "ray_march",
+                                          "mlp_fwd",  "composite", "mlp_bwd",
+                                          "scatter",  "dw_reduce", "adam"};
 struct StageTimer {
     bool enabled = false;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> open;
@@ -285,12 +290,15 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     R.ray_out = reinterpret_cast<float4*>(ws + L.ray_out);
     R.ray_grad = reinterpret_cast<float4*>(ws + L.ray_grad);
     R.ray_ctot = reinterpret_cast<float4*>(ws + L.ray_ctot);
+    R.ray_loss = reinterpret_cast<float*>(ws + L.ray_loss);
     R.loss = d->loss ? d->loss : reinterpret_cast<float*>(ws + L.loss);
     Records rec;
     rec.xyze = reinterpret_cast<float4*>(ws + L.xyze);
     rec.atwr = reinterpret_cast<float4*>(ws + L.atwr);
     rec.rgb = reinterpret_cast<float4*>(ws + L.rgb);
     rec.cinc = reinterpret_cast<float4*>(ws + L.cinc);
+    rec.feat = reinterpret_cast<uint4*>(ws + L.feat);
+    rec.grad = reinterpret_cast<float4*>(ws + L.grad);
     uint8_t* touched = ws + L.touched;
     float* dw_part = reinterpret_cast<float*>(ws + L.dw_part);
     uint16_t* w0t = reinterpret_cast<uint16_t*>(ws + L.w0t);
@@ -314,7 +322,9 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     {
         StageScope sc(0, s);
         if ((rc = launch_pack_mlp(d->mlp, w0t, w1t, w2t, w2, w1, w0, s))) return rc;
-        hipLaunchKernelGGL(k_fill_f32, dim3(1), dim3(64), 0, s, R.loss, (int64_t)1, 0.f);
+        // the MLP gradient is accumulated by k_dw_reduce: it starts at zero
+        hipLaunchKernelGGL(k_fill_f32, dim3(pvdb_blocks(kMlpParams, 256)), dim3(256), 0, s,
+                           d->mlp_grad, (int64_t)kMlpParams, 0.f);
         PVDB_LAUNCHED("k_fill_f32");
     }
     {
@@ -331,7 +341,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     {
         StageScope sc(3, s);
         if ((rc = launch_mlp(false, d->color_bf16 != 0, grid, P, t, d->color, W, R, rec, nullptr,
-                             nullptr, touched, nullptr, s)))
+                             s)))
             return rc;
     }
     {
@@ -340,12 +350,18 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     }
     {
         StageScope sc(5, s);
-        if ((rc = launch_mlp(true, d->color_bf16 != 0, grid, P, t, d->color, W, R, rec,
-                             d->grad_density, d->grad_color, touched, dw_part, s)))
+        if ((rc = launch_mlp(true, d->color_bf16 != 0, grid, P, t, d->color, W, R, rec, dw_part,
+                             s)))
             return rc;
     }
     {
         StageScope sc(6, s);
+        if ((rc = launch_scatter(t, R, rec, d->capacity, d->grad_density, d->grad_color, touched,
+                                 s)))
+            return rc;
+    }
+    {
+        StageScope sc(7, s);
         if ((rc = launch_dw_reduce(dw_part, grid, d->mlp_grad, s))) return rc;
     }
     if (d->ray_out) {
@@ -353,7 +369,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
                                 hipMemcpyDeviceToDevice, s));
     }
     if (d->do_adam) {
-        StageScope sc(7, s);
+        StageScope sc(8, s);
         const int ag = (int)std::min<int64_t>(t.nleaf > 0 ? t.nleaf : 1, 4096);
         if (d->color_bf16)
             hipLaunchKernelGGL(k_sparse_adam<true>, dim3(ag), dim3(256), 0, s, t, touched,

@@ -231,7 +231,7 @@ class FusedTrainer:
         return out
 
 
-N_STAGES = 8
+N_STAGES = 9
 
 
 def stage_timing(enable: int) -> dict:
