@@ -12,7 +12,9 @@ path (a sample is a point whose density gather is evaluated: inside the bbox,
 in an active voxel, before the alpha/weight culls).  The plain FP32 path (the
 stage-per-kernel structure of the reference) is timed beside it, and the CPU
 baseline is the oracle's FP32 restatement of the reference step timed on the
-host cores on a bounded sample of the same workload.
+host cores on a bounded sample of the same workload.  The `render` object is
+the merged-VDB renderer on the 800x800 render-only workload (frames/s and
+ray-samples/s).
 """
 from __future__ import annotations
 
@@ -39,7 +41,11 @@ def parse():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--path", default="auto", choices=["auto", "batched", "plain"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip timing the plain FP32 path beside the batched one")
     ap.add_argument("--cpu-rays", type=int, default=8192)
+    ap.add_argument("--render-frames", type=int, default=100,
+                    help="poses of the merged-VDB render benchmark (0 skips it)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a device copy of known size (counter calibration)")
     return ap.parse_args()
@@ -113,6 +119,7 @@ def bench_plain(args, scene):
         "totals": totals,
         "points_density": idx,
         "n_color": final,
+        "n_rays": int(tr.rays_o.shape[0]),
     }
 
 
@@ -155,12 +162,14 @@ def bench_batched(args, scene):
     act = tr.topo.active_slot_mask()
     v_u = int(((gd != 0) & act).sum())
     tr.zero_grad()
+    pts = rec["xyze"][:, :3]
     return {
         "ms_per_step": ms,
         "samples_per_step": float(samples),
         "stages": stages,
         "n_color": int(alive_w.sum()),
-        "points": rec["xyze"][:, :3],
+        "uniq_all": unique_corner_voxels(pts[:, 0], pts[:, 1], pts[:, 2]),
+        "uniq_w": unique_corner_voxels(pts[alive_w, 0], pts[alive_w, 1], pts[alive_w, 2]),
         "l_t": int(touched_leaf.sum()),
         "v_u": v_u,
         "n_rays": tr.n_rays,
@@ -168,35 +177,67 @@ def bench_batched(args, scene):
     }
 
 
-def batched_roofline(r, steps):
-    """Roofline of the dominant stage and the survey's whole-step byte model."""
-    st = r["stages"]
-    name = max(st, key=lambda k: st[k][0])
-    tot, calls = st[name]
-    avg_s = tot / max(calls, 1) * 1e-3
-    S, N, sc = r["samples_per_step"], r["n_rays"], r["color_bytes"]
-    uniq = unique_corner_voxels(r["points"][:, 0], r["points"][:, 1], r["points"][:, 2])
-    if name in ("mlp_bwd", "mlp_fwd"):
-        flop = (S * (MLP_FWD_FLOP + MLP_BWD_FLOP) + N * 2 * MLP_RAY_FLOP if name == "mlp_bwd"
-                else S * MLP_FWD_FLOP + N * MLP_RAY_FLOP)
-        achieved = flop / avg_s / 1e12
-        roof = {"kernel": name, "bound": "mfma", "achieved": round(achieved, 3),
+def stage_models(r):
+    """Algorithmic work of one launch of each fused stage: FLOPs for the MLP
+    stages (bounded by BF16 MFMA), HBM bytes for the others (each input and
+    output element once, each distinct grid voxel once)."""
+    S, N, sc, nw = r["samples_per_step"], r["n_rays"], r["color_bytes"], r["n_color"]
+    ua, uw, L_t, V_u = r["uniq_all"], r["uniq_w"], r["l_t"], r["v_u"]
+    P = 22019
+    return {
+        "pack_mlp": ("hbm", P * 4 + 2 * (128 * 64 + 2 * 128 * 128 + 2 * 16 * 128 + 128 * 32) + P * 4),
+        # rays in, counts out
+        "ray_count": ("hbm", N * (12 + 12 + 4)),
+        # rays + offsets in, 32-byte records out, distinct density voxels once
+        "ray_march": ("hbm", N * (12 + 12 + 8) + S * 32 + ua * 4),
+        # 36,608 FLOP per MLP sample + 6,912 per ray (PE part of layer 0)
+        "mlp_fwd": ("mfma", nw * MLP_FWD_FLOP + N * MLP_RAY_FLOP),
+        # records + rgb in, prefix out; rays in/out
+        "composite": ("hbm", S * 16 + nw * 32 + N * (12 + 16 + 16 + 16 + 4)),
+        # 73,216 FLOP per MLP sample + 6,912 per ray (dW of the PE block)
+        "mlp_bwd": ("mfma", nw * MLP_BWD_FLOP + N * MLP_RAY_FLOP),
+        # records + per-record gradients in, distinct voxels' 13 gradients read+written
+        "scatter": ("hbm", S * (16 + 16 + 64) + uw * 13 * 4 * 2),
+        # 512 workgroup partials in, MLP gradient read+written
+        "dw_reduce": ("hbm", 512 * P * 4 + 2 * P * 4),
+        # survey B_adam: density p,m,v r/w + g r; colour p,m,v r/w + g r
+        "adam": ("hbm", V_u * (28 + 12 * (2 * sc + 20)) + L_t * 512 * 13 * 4),
+    }
+
+
+def stage_roofline(name, kind, work, avg_s):
+    if kind == "mfma":
+        achieved = work / avg_s / 1e12
+        return {"kernel": name, "bound": "mfma", "achieved": round(achieved, 3),
                 "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 6), "traffic": None,
-                "algorithmic_flop": flop, "avg_launch_ms": round(avg_s * 1e3, 5)}
-    else:
-        # march: rays in, records out, each distinct density voxel read once
-        b = N * 24 + S * 32 + uniq * 4
-        achieved = b / avg_s / 1e9
-        roof = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                "algorithmic_bytes": b, "avg_launch_ms": round(avg_s * 1e3, 5)}
-    L_t, V_u = r["l_t"], r["v_u"]
+                "algorithmic_flop": float(work), "avg_launch_ms": round(avg_s * 1e3, 5)}
+    achieved = work / avg_s / 1e9
+    return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "algorithmic_bytes": float(work), "avg_launch_ms": round(avg_s * 1e3, 5)}
+
+
+def batched_roofline(r, steps):
+    """Roofline of the dominant stage, every stage's fraction, and the whole-step
+    byte model (B_step = B_grid + B_grad + B_adam + B_ray)."""
+    st = r["stages"]
+    models = stage_models(r)
+    table = {}
+    for name, (tot, calls) in st.items():
+        if calls and name in models:
+            kind, work = models[name]
+            table[name] = stage_roofline(name, kind, work, tot / calls * 1e-3)
+    dom = max((k for k in st if k in table), key=lambda k: st[k][0])
+    roof = dict(table[dom])
+    N, sc, L_t, V_u = r["n_rays"], r["color_bytes"], r["l_t"], r["v_u"]
     b_step = (L_t * 512 * (4 + 12 * sc) + L_t * 512 * 52 + V_u * (28 + 12 * (2 * sc + 20)) + N * 60)
     step = {"bytes": b_step, "achieved_GBps": round(b_step / (r["ms_per_step"] * 1e-3) / 1e9, 2),
             "frac": round(b_step / (r["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
             "leaves_touched": L_t, "voxels_updated": V_u}
-    return roof, step
+    stage_fracs = {k: {"unit": v["unit"], "achieved": v["achieved"], "frac": v["frac"]}
+                   for k, v in table.items()}
+    return roof, step, stage_fracs
 
 
 def roofline_from(totals, launches_bytes):
@@ -216,6 +257,66 @@ def roofline_from(totals, launches_bytes):
     return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
             "algorithmic_bytes": b, "avg_launch_ms": round(avg_ms, 5)}
+
+
+def bench_render(frames: int, warmup: int):
+    """Merged-VDB rendering (c5): c4's topology as an index grid with fp16 data,
+    800x800 pinhole camera (focal 1111), orbit poses at radius 4 and -30 deg,
+    step 0.5 voxel, early stop at T < 1e-4.  Frames/s over `frames` poses, timed
+    with HIP events on the render stream; the per-frame sample counts come from
+    an untimed second pass."""
+    import torch
+
+    from plenvdb_amd import scenes, vdbio
+    from plenvdb_amd.renderer import MGRenderer
+
+    ms = scenes.merged_scene("c4", H=800, W=800, focal=1111.0, n_poses=frames)
+    r = MGRenderer(12, 27, 128, 3)
+    p = ms["mlp"]
+    r.load_params(p["w0"], p["b0"], p["w1"], p["b1"], p["w2"], p["b2"])
+    topo, slot_idx = vdbio.index_grid_arrays(ms["idx"])
+    den = ms["den"].astype(np.float32)
+    r.load_data_grid(den, ms["col"].astype(np.float32).reshape(-1), topo, slot_idx, len(den))
+    r.setScene(ms["reso"], ms["K"], ms["xyz_min"], ms["xyz_max"])
+    r.setKwargs(ms["near"], ms["far"], ms["stepdist"], ms["act_shift"], ms["interval"],
+                ms["fast_color_thres"], ms["bg"], ms["inverse_y"], ms["H"], ms["W"])
+    r.set_t_stop(ms["t_stop"])
+    poses = ms["poses"]
+    for i in range(warmup):
+        r.input_a_c2w(poses[i % len(poses)])
+        r.render_async()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for i in range(frames):
+        r.input_a_c2w(poses[i])
+        r.render_async()
+    e1.record()
+    torch.cuda.synchronize()
+    ms_frame = e0.elapsed_time(e1) / frames
+    gathers = kept = 0
+    for i in range(frames):
+        r.input_a_c2w(poses[i])
+        r.render_async()
+        g, k = r.sample_counts()
+        gathers += g
+        kept += k
+    n_px = ms["H"] * ms["W"]
+    flop = (kept * MLP_FWD_FLOP + frames * n_px * MLP_RAY_FLOP) / frames
+    tflops = flop / (ms_frame * 1e-3) / 1e12
+    return {
+        "workload": "c5: c4 topology (320^3, 1959 leaves) as a merged index grid, fp16 data, "
+                    f"800x800, focal 1111, {frames} orbit poses, T<1e-4",
+        "frames_per_s": round(1e3 / ms_frame, 2),
+        "ms_per_frame": round(ms_frame, 4),
+        "samples_per_s": round(gathers / frames / (ms_frame * 1e-3), 1),
+        "samples_per_frame": round(gathers / frames, 1),
+        "kept_per_frame": round(kept / frames, 1),
+        "roofline": {"kernel": "k_render", "bound": "mfma", "achieved": round(tflops, 3),
+                     "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 6),
+                     "algorithmic_flop": flop, "avg_launch_ms": round(ms_frame, 5)},
+    }
 
 
 def cpu_baseline(cfg_name: str, n_rays: int):
@@ -275,11 +376,12 @@ def main():
     res = {}
     if path == "batched" or args.path == "auto":
         res["batched"] = bench_batched(args, scene)
-    res["plain"] = bench_plain(args, scene)
+    if path == "plain" or not args.no_plain:
+        res["plain"] = bench_plain(args, scene)
     main_res = res[path]
-    step_roof = None
+    step_roof = stage_fracs = None
     if path == "batched":
-        roof, step_roof = batched_roofline(main_res, args.steps)
+        roof, step_roof, stage_fracs = batched_roofline(main_res, args.steps)
     else:
         p = main_res["points_density"]
         uniq_d = unique_corner_voxels(p[:, 0], p[:, 1], p[:, 2])
@@ -314,11 +416,17 @@ def main():
             {k: round(v[0] / args.steps, 4) for k, v in sorted(
                 main_res["totals"].items(), key=lambda kv: -kv[1][0])}),
         "step_bytes_model": step_roof,
+        "stage_roofline": stage_fracs,
+        "rays_per_s": round(main_res["n_rays"] / (main_res["ms_per_step"] * 1e-3), 1),
     }
+    if path == "batched":
+        line["adam_voxels_per_s"] = round(main_res["v_u"] / (main_res["ms_per_step"] * 1e-3), 1)
     if "batched" in res and "plain" in res:
         line["speedup_batched_vs_plain"] = round(res["plain"]["ms_per_step"] / res["batched"]["ms_per_step"]
                                                  * res["batched"]["samples_per_step"]
                                                  / res["plain"]["samples_per_step"], 3)
+    if args.render_frames > 0:
+        line["render"] = bench_render(args.render_frames, 3)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.config, args.cpu_rays)
     else:

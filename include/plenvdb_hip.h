@@ -162,9 +162,10 @@ int pvdb_alpha2weight_bwd(const float* alpha, const float* weight, const float* 
  * adam_upd_cuda.  Payloads: density [slots] fp32, colour [slots][12] fp32 or
  * bf16 (color_bf16), gradients and Adam moments fp32; MLP parameters fp32 in
  * the order W0[39][128] b0[128] W1[128][128] b1[128] W2[128][3] b2[3] (W as
- * Linear.weight.T, run.py:98-104).  Gradients are accumulated (callers zero
- * them once); with do_adam the step applies Adam and re-zeroes the gradients
- * of the leaves it touched. */
+ * Linear.weight.T, run.py:98-104).  Grid gradients are accumulated (callers
+ * zero them once); with do_adam the step applies Adam and re-zeroes the
+ * gradients of the leaves it touched.  The MLP gradient is overwritten by
+ * every step. */
 typedef struct pvdb_step_desc {
     const float* rays_o;      /* dev [n][3] */
     const float* rays_d;      /* dev [n][3] */
@@ -218,12 +219,61 @@ int pvdb_fused_records(const pvdb_topology* topo, const pvdb_step_desc* desc, vo
                        void** atwr, void** rgb, void** cinc, int32_t** count, int32_t** offset,
                        int32_t** total);
 
+/* ---- merged-VDB renderer ----------------------------------------------------
+ * MGRenderer (plenvdb/lib/vdb/plenvdb.h:933-1068, plenvdb.cpp:135-171;
+ * render_an_image_cuda renderer.cu:370-424) as one persistent kernel: each
+ * wavefront marches an 8x8 pixel tile with the reference's sample rule and
+ * evaluates the RGB MLP on fp16 MFMA for its kept samples.  The merged index
+ * grid (vdb_compression.py:19-59) is a topology whose active voxels are the
+ * non-zero indices plus `slot_idx` (the index of every slot, 0 when inactive).
+ * Differences from the reference: one march instead of two (the reference's
+ * second march restarts from the first kept t minus one step, so its sample
+ * positions can differ from the first's by rounding, renderer.cu:263), the
+ * early-stop threshold t_stop is a parameter (the reference hard-codes 1e-3,
+ * renderer.cu:265), and the MLP runs in fp16 with fp32 accumulation. */
+typedef struct pvdb_render_desc {
+    float c2w[16];    /* host values: camera-to-world, row-major (input_a_c2w) */
+    float K[9];       /* intrinsics, row-major (setScene) */
+    float xyz_min[3];
+    float xyz_max[3];
+    int32_t reso[3];
+    int32_t H, W, inverse_y;
+    float near_, stepdist, act_shift, interval, fast_color_thres, bg, t_stop;
+    int32_t payload_f32;   /* payload slots hold fp32 (64 B) instead of fp16 (32 B) */
+    const void* payload;   /* dev, pvdb_render_payload_bytes() bytes */
+    const void* weights;   /* dev, pvdb_render_weights_bytes() bytes */
+    float* out;            /* dev [H*W*3] (output_an_image layout) */
+    uint64_t* counters;    /* dev [2] (density gathers, kept samples) or NULL */
+    void* workspace;       /* dev, 256 bytes */
+} pvdb_render_desc;
+
+/* Bytes of the packed MLP operands. */
+int64_t pvdb_render_weights_bytes(void);
+/* MGRenderer::load_params (plenvdb.h:960-990): w* are [in][out] row-major
+ * (Linear.weight.T, run.py:98-104), device fp32.  Only dcol 12, dpe 27,
+ * dhid 128, dout 3 are supported. */
+int pvdb_render_pack_weights(const float* w0, const float* b0, const float* w1, const float* b1,
+                             const float* w2, const float* b2, int32_t dcol, int32_t dpe,
+                             int32_t dhid, int32_t dout, void* packed /*dev*/, void* stream);
+/* Payload bytes for the topology ((nslots + 1) slots; -1 on a null handle). */
+int64_t pvdb_render_payload_bytes(const pvdb_topology* topo, int32_t payload_f32);
+/* MGRenderer::load_data (plenvdb.h:992-1005): den [nidx], col [nidx][12] (dev
+ * fp32, index 0 = the value of inactive voxels) gathered into the slot payload
+ * through slot_idx (dev [nslots], values in [0, nidx)). */
+int pvdb_render_build_payload(const pvdb_topology* topo, const int32_t* slot_idx,
+                              const float* den, const float* col, int64_t nidx,
+                              int32_t payload_f32, void* payload, void* stream);
+/* MGRenderer::render_an_image (plenvdb.h:1029-1035): writes out [H*W*3]. */
+int pvdb_render_image(const pvdb_topology* topo, const pvdb_render_desc* desc, void* stream);
+
 /* ---- self test --------------------------------------------------------------
  * No counterpart in the reference: multiplies bf16 matrices with the MFMA
  * fragment layouts the MLP kernels assume (A [16][32], B [32][16] -> D [16][16];
  * A16 [16][16], B16 [16][16] -> D16), so a test can check them on the device. */
 int pvdb_mfma_selftest(const uint16_t* A, const uint16_t* B, float* D, const uint16_t* A16,
                        const uint16_t* B16, float* D16, void* stream);
+/* The same check for v_mfma_f32_16x16x32_f16 (fp16 A [16][32], B [32][16]). */
+int pvdb_mfma_selftest_f16(const uint16_t* A, const uint16_t* B, float* D, void* stream);
 
 #ifdef __cplusplus
 }

@@ -17,14 +17,16 @@
  * renderer.cu:263) and the different density summation order of the two
  * passes (trigetDensity :191-220 vs trigetDensity2 :271-300).  Pass 2 writes
  * at most as many samples as pass 1 counted (the reference would write past
- * the ray's segment, which is undefined).  The FP64 variant (_f64) runs the
- * same code in double.
+ * the ray's segment, which is undefined).  The FP64 variant (_f64) keeps the
+ * reference's FP32 geometry (ray set-up, t accumulation, sample positions, box
+ * and active tests, corner indices and fractions) and does the rest in double,
+ * so both variants see the same samples.
  */
 
 #define CAT2(a, b) a##b
 #define CAT(a, b) CAT2(a, b)
 
-#define RENDER_IMPL(REAL, SFX)                                                                   \
+#define RENDER_IMPL(REAL, GREAL, SFX)                                                                   \
     static inline int CAT(ridx, SFX)(const int32_t* idx, const int* reso, int i, int j, int k) { \
         if (i < 0 || j < 0 || k < 0 || i >= reso[0] || j >= reso[1] || k >= reso[2]) return 0;   \
         return idx[((long)i * reso[1] + j) * reso[2] + k];                                         \
@@ -33,39 +35,39 @@
     /* 27-d positional encoding of the view direction, and the [0,1]^3 slab test.           */     \
     static void CAT(rsetup, SFX)(long n, const float* c2w, const float* K, const float* mn,        \
                                  const float* mx, int W, int inverse_y, float near_,                \
-                                 float stepdist, REAL* o, REAL* d, REAL* step, REAL* tmin,          \
-                                 REAL* tmax, REAL* pe) {                                            \
+                                 float stepdist, GREAL* o, GREAL* d, GREAL* step, GREAL* tmin,      \
+                                 GREAL* tmax, REAL* pe) {                                           \
         for (int a = 0; a < 3; ++a)                                                                \
-            o[a] = ((REAL)c2w[4 * a + 3] - (REAL)mn[a]) / ((REAL)mx[a] - (REAL)mn[a]);             \
-        REAL pi = (REAL)((float)(n % W) + 0.5f), pj = (REAL)((float)(n / W) + 0.5f);               \
-        REAL dir[3];                                                                               \
-        dir[0] = (pi - (REAL)K[2]) / (REAL)K[0];                                                   \
+            o[a] = ((GREAL)c2w[4 * a + 3] - (GREAL)mn[a]) / ((GREAL)mx[a] - (GREAL)mn[a]);         \
+        GREAL pi = (GREAL)((float)(n % W) + 0.5f), pj = (GREAL)((float)(n / W) + 0.5f);            \
+        GREAL dir[3];                                                                              \
+        dir[0] = (pi - (GREAL)K[2]) / (GREAL)K[0];                                                 \
         if (inverse_y) {                                                                           \
-            dir[1] = (pj - (REAL)K[5]) / (REAL)K[4];                                               \
+            dir[1] = (pj - (GREAL)K[5]) / (GREAL)K[4];                                             \
             dir[2] = 1;                                                                            \
         } else {                                                                                   \
-            dir[1] = -((pj - (REAL)K[5]) / (REAL)K[4]);                                            \
+            dir[1] = -((pj - (GREAL)K[5]) / (GREAL)K[4]);                                          \
             dir[2] = -1;                                                                           \
         }                                                                                          \
-        REAL rd[3];                                                                                \
+        GREAL rd[3];                                                                               \
         for (int a = 0; a < 3; ++a)                                                                \
-            rd[a] = dir[0] * (REAL)c2w[4 * a] + dir[1] * (REAL)c2w[4 * a + 1] +                    \
-                    dir[2] * (REAL)c2w[4 * a + 2];                                                 \
-        REAL len = (REAL)sqrt(rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2]);                      \
-        *step = (REAL)stepdist / len;                                                              \
-        for (int a = 0; a < 3; ++a) d[a] = rd[a] / ((REAL)mx[a] - (REAL)mn[a]);                    \
-        REAL inv = (REAL)1 / len, vd[3];                                                           \
-        for (int a = 0; a < 3; ++a) vd[a] = rd[a] * inv;                                           \
-        REAL v[3], lo[3], hi[3];                                                                   \
+            rd[a] = dir[0] * (GREAL)c2w[4 * a] + dir[1] * (GREAL)c2w[4 * a + 1] +                  \
+                    dir[2] * (GREAL)c2w[4 * a + 2];                                                \
+        GREAL len = (GREAL)sqrt(rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2]);                    \
+        *step = (GREAL)stepdist / len;                                                             \
+        for (int a = 0; a < 3; ++a) d[a] = rd[a] / ((GREAL)mx[a] - (GREAL)mn[a]);                  \
+        REAL inv = (REAL)1 / (REAL)len, vd[3];                                                     \
+        for (int a = 0; a < 3; ++a) vd[a] = (REAL)rd[a] * inv;                                     \
+        GREAL v[3], lo[3], hi[3];                                                                  \
         for (int a = 0; a < 3; ++a) {                                                              \
-            v[a] = d[a] == 0 ? (REAL)1e-6f : d[a];                                                 \
-            REAL A = ((REAL)1 - o[a]) / v[a], B = (-o[a]) / v[a];                                  \
+            v[a] = d[a] == 0 ? (GREAL)1e-6f : d[a];                                                \
+            GREAL A = ((GREAL)1 - o[a]) / v[a], B = (-o[a]) / v[a];                                \
             lo[a] = A < B ? A : B;                                                                 \
             hi[a] = A < B ? B : A;                                                                 \
         }                                                                                          \
-        REAL l = fmax(fmax(lo[0], lo[1]), lo[2]), h = fmin(fmin(hi[0], hi[1]), hi[2]);             \
-        *tmin = fmax(fmin(l, (REAL)1e9f), (REAL)near_);                                            \
-        *tmax = fmax(fmin(h, (REAL)1e9f), (REAL)near_);                                            \
+        GREAL l = fmax(fmax(lo[0], lo[1]), lo[2]), h = fmin(fmin(hi[0], hi[1]), hi[2]);            \
+        *tmin = fmax(fmin(l, (GREAL)1e9f), (GREAL)near_);                                          \
+        *tmax = fmax(fmin(h, (GREAL)1e9f), (GREAL)near_);                                          \
         pe[0] = vd[0];                                                                             \
         pe[1] = vd[1];                                                                             \
         pe[2] = vd[2];                                                                             \
@@ -81,10 +83,10 @@
         }                                                                                          \
     }                                                                                              \
     /* Trilinear weights and the 8 corner indices with int() truncation (renderer.cu:271-300). */ \
-    static void CAT(rcorners, SFX)(const int32_t* idx, const int* reso, const REAL* xyz,           \
+    static void CAT(rcorners, SFX)(const int32_t* idx, const int* reso, const GREAL* xyz,          \
                                    int* link, REAL* sc) {                                          \
         int i = (int)xyz[0], j = (int)xyz[1], k = (int)xyz[2];                                     \
-        REAL u = xyz[0] - (REAL)i, v = xyz[1] - (REAL)j, w = xyz[2] - (REAL)k;                     \
+        REAL u = (REAL)(xyz[0] - (GREAL)i), v = (REAL)(xyz[1] - (GREAL)j), w = (REAL)(xyz[2] - (GREAL)k);                     \
         static const int cn[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0},                   \
                                      {1, 1, 0}, {1, 1, 1}, {1, 0, 1}, {1, 0, 0}};                  \
         for (int q = 0; q < 8; ++q) {                                                              \
@@ -104,16 +106,17 @@
         REAL* tmax_o, REAL* step_o, REAL* pe_o, int64_t* count) {                                 \
         const long HW = (long)H * W;                                                               \
         _Pragma("omp parallel for schedule(dynamic, 256)") for (long n = 0; n < HW; ++n) {         \
-            REAL o[3], d[3], step, tmin, tmax;                                                     \
+            GREAL o[3], d[3], step, tmin, tmax;                                                    \
             CAT(rsetup, SFX)(n, c2w, K, mn, mx, W, inverse_y, near_, stepdist, o, d, &step, &tmin, \
                              &tmax, pe_o + 27 * n);                                                \
-            REAL ws[3] = {(REAL)(reso[0] - 1), (REAL)(reso[1] - 1), (REAL)(reso[2] - 1)};          \
-            REAL T = 1, t = tmin;                                                                  \
+            GREAL ws[3] = {(GREAL)(reso[0] - 1), (GREAL)(reso[1] - 1), (GREAL)(reso[2] - 1)};     \
+            REAL T = 1;                                                                            \
+            GREAL t = tmin;                                                                        \
             int64_t cnt = 0;                                                                       \
             int first = 0;                                                                         \
             while (t < tmax) {                                                                     \
                 t += step;                                                                         \
-                REAL p[3], xyz[3];                                                                 \
+                GREAL p[3], xyz[3];                                                                \
                 int out = 0;                                                                       \
                 for (int a = 0; a < 3; ++a) {                                                      \
                     p[a] = o[a] + t * d[a];                                                        \
@@ -128,7 +131,7 @@
                 REAL sc[8];                                                                        \
                 CAT(rcorners, SFX)(idx, reso, xyz, link, sc);                                      \
                 int i = (int)xyz[0], j = (int)xyz[1], k = (int)xyz[2];                             \
-                REAL u = xyz[0] - (REAL)i, v = xyz[1] - (REAL)j, w = xyz[2] - (REAL)k;             \
+                REAL u = (REAL)(xyz[0] - (GREAL)i), v = (REAL)(xyz[1] - (GREAL)j), w = (REAL)(xyz[2] - (GREAL)k);             \
                 static const int cn[8][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 1, 0},           \
                                              {1, 1, 0}, {1, 1, 1}, {1, 0, 1}, {1, 0, 0}};          \
                 REAL sig = 0;                                                                      \
@@ -170,7 +173,8 @@
         REAL* feat, REAL* wts, int64_t* pix, int64_t* written, REAL* t_last) {                    \
         const long HW = (long)H * W;                                                               \
         _Pragma("omp parallel for schedule(dynamic, 256)") for (long n = 0; n < HW; ++n) {         \
-            REAL o[3], d[3], step, tmin, tmax, pe[27];                                             \
+            GREAL o[3], d[3], step, tmin, tmax;                                                    \
+            REAL pe[27];                                                                           \
             CAT(rsetup, SFX)(n, c2w, K, mn, mx, W, inverse_y, near_, stepdist, o, d, &step, &tmin, \
                              &tmax, pe);                                                           \
             written[n] = 0;                                                                        \
@@ -178,13 +182,14 @@
                 t_last[n] = 1;                                                                     \
                 continue;                                                                          \
             }                                                                                      \
-            REAL ws[3] = {(REAL)(reso[0] - 1), (REAL)(reso[1] - 1), (REAL)(reso[2] - 1)};          \
-            REAL T = 1, t = tmin_i[n];                                                             \
-            tmax = tmax_i[n];                                                                      \
+            GREAL ws[3] = {(GREAL)(reso[0] - 1), (GREAL)(reso[1] - 1), (GREAL)(reso[2] - 1)};     \
+            REAL T = 1;                                                                            \
+            GREAL t = (GREAL)tmin_i[n];                                                            \
+            tmax = (GREAL)tmax_i[n];                                                               \
             int64_t r = offs[n], rend = offs[n] + count[n];                                        \
             while (t < tmax) {                                                                     \
                 t += step;                                                                         \
-                REAL p[3], xyz[3];                                                                 \
+                GREAL p[3], xyz[3];                                                                \
                 int out = 0;                                                                       \
                 for (int a = 0; a < 3; ++a) {                                                      \
                     p[a] = o[a] + t * d[a];                                                        \
@@ -223,5 +228,5 @@
         }                                                                                          \
     }
 
-RENDER_IMPL(float, _f32)
-RENDER_IMPL(double, _f64)
+RENDER_IMPL(float, float, _f32)
+RENDER_IMPL(double, float, _f64)

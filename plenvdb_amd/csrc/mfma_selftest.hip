@@ -35,7 +35,30 @@ __global__ void __launch_bounds__(64) k_mfma_selftest(const uint16_t* __restrict
     for (int i = 0; i < 4; ++i) D16[(4 * g + i) * 16 + r] = c4[i];
 }
 
+__global__ void __launch_bounds__(64) k_mfma_selftest_f16(const uint16_t* __restrict__ A,
+                                                          const uint16_t* __restrict__ B,
+                                                          float* __restrict__ D) {
+    const int l = threadIdx.x;
+    const int r = l & 15, g = l >> 4;
+    pvdb::f16x8 a, b;
+    for (int j = 0; j < 8; ++j) {
+        a[j] = __builtin_bit_cast(_Float16, A[r * 32 + 8 * g + j]);
+        b[j] = __builtin_bit_cast(_Float16, B[(8 * g + j) * 16 + r]);
+    }
+    pvdb::f32x4 c = {0.f, 0.f, 0.f, 0.f};
+    c = pvdb::mfma16x16x32h(a, b, c);
+    for (int i = 0; i < 4; ++i) D[(4 * g + i) * 16 + r] = c[i];
+}
+
 }  // namespace
+
+extern "C" int pvdb_mfma_selftest_f16(const uint16_t* A, const uint16_t* B, float* D,
+                                      void* stream) {
+    PVDB_CHECK_ARG(A && B && D, "null pointer");
+    hipLaunchKernelGGL(k_mfma_selftest_f16, dim3(1), dim3(64), 0, (hipStream_t)stream, A, B, D);
+    PVDB_LAUNCHED("k_mfma_selftest_f16");
+    return PVDB_OK;
+}
 
 extern "C" int pvdb_mfma_selftest(const uint16_t* A, const uint16_t* B, float* D,
                                   const uint16_t* A16, const uint16_t* B16, float* D16,

@@ -111,19 +111,34 @@ class CallTimer:
         return {k: (v[0], v[1]) for k, v in out.items()}
 
 
+def function(name: str):
+    """The library function `name` with its declared argument types (modules that
+    register signatures after the library was loaded get them applied here)."""
+    lib = load()
+    fn = getattr(lib, name)
+    if fn.argtypes is None:
+        sig = _SIGS.get(name, EXTRA_SIGS.get(name))
+        if sig is None:
+            raise RuntimeError(f"{name}: no declared signature")
+        fn.restype = C.c_int32
+        fn.argtypes = sig
+    return fn
+
+
 def call(name: str, *args) -> None:
     """Calls a library function and raises RuntimeError on a non-zero status."""
     lib = load()
+    fn = function(name)
     timer = CallTimer.active
     if timer is not None:
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         e0.record()
-        rc = getattr(lib, name)(*args)
+        rc = fn(*args)
         e1.record()
         timer.records.append((name, e0, e1))
     else:
-        rc = getattr(lib, name)(*args)
+        rc = fn(*args)
     if rc != 0:
         msg = lib.pvdb_last_error().decode(errors="replace")
         raise RuntimeError(f"{name} failed ({rc}): {msg}")

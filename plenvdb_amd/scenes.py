@@ -217,3 +217,29 @@ def make_scene(name: str, n_rays: int | None = None) -> dict:
     if cfg.use_mlp:
         sc["mlp"] = mlp_params(cfg, cfg.channels)
     return sc
+
+
+def merged_scene(name: str = "c4", H: int = 800, W: int = 800, focal: float = 1111.0,
+                 n_poses: int = 100, t_stop: float = 1e-4) -> dict:
+    """Render-only workload (c5 when name is c4): the configuration's topology as a
+    merged index grid (index 1 + C-order rank of each active voxel), fp16 data
+    (density ~N(0,2), colour ~N(0,0.1), zero row 0), the configuration's MLP,
+    and an orbit of camera poses."""
+    cfg = CONFIGS[name]
+    mask = occupancy_mask(cfg)
+    n = int(mask.sum())
+    idx = np.zeros(mask.shape, np.int32)
+    idx[mask] = np.arange(1, n + 1, dtype=np.int32)
+    rng = np.random.default_rng(1)
+    den = np.concatenate([[0.0], rng.normal(0.0, 2.0, n)]).astype(np.float16)
+    col = np.concatenate([np.zeros((1, 12)), rng.normal(0.0, 0.1, (n, 12))]).astype(np.float16)
+    vs = voxel_size(cfg)
+    return {
+        "cfg": cfg, "mask": mask, "idx": idx, "den": den, "col": col,
+        "mlp": mlp_params(cfg, 12),
+        "K": intrinsics(H, W, focal), "poses": orbit_poses(n_poses), "H": H, "W": W,
+        "reso": [cfg.reso] * 3, "xyz_min": XYZ_MIN.copy(), "xyz_max": XYZ_MAX.copy(),
+        "near": cfg.near, "far": cfg.far, "stepdist": float(np.float32(cfg.stepsize * vs)),
+        "act_shift": act_shift(cfg.alpha_init), "interval": cfg.stepsize,
+        "fast_color_thres": 1e-4, "bg": cfg.bg, "inverse_y": False, "t_stop": t_stop,
+    }

@@ -241,3 +241,85 @@ def test_reference_step_gradient_matches_fd(oracle_mod):
         lm = oracle_mod.reference_train_step(sc, prec="f64")["loss"]
         fd = (lp - lm) / (2 * h)
         assert abs(fd - gc[s, c]) < 1e-5 * max(abs(fd), 1e-6) + 1e-11
+
+
+def _pe27(vd):
+    """Per-pixel PE of the renderer in the order get_rays writes it
+    (renderer.cu:153-165): v, then sin(v_x 2^k), sin(v_y 2^k), sin(v_z 2^k),
+    cos(...) for k = 0..3."""
+    out = [vd[:, 0], vd[:, 1], vd[:, 2]]
+    for fn in (np.sin, np.cos):
+        for a in range(3):
+            for k in range(4):
+                out.append(fn(vd[:, a] * 2.0 ** k))
+    return np.stack(out, 1)
+
+
+@pytest.mark.parametrize("sigma,t_stop", [(6.0, 1e-3), (6.0, 1e-4), (5.0, 1e-4), (-1.0, 1e-3)])
+def test_render_constant_field_closed_form(oracle_mod, sigma, t_stop):
+    """All voxels active with one density and one colour: every in-box sample has
+    the same alpha and every kept sample the colour c = sigmoid(MLP) of the
+    constant feature and the pixel's PE.  Along a ray of n in-box samples
+    (t_k = tmin + k*step, k >= 1, from the slab test) a sample is kept when its
+    weight exceeds fast_color_thres, the march stops after a kept sample that
+    leaves T < t_stop, and the pixel is c * sum(kept weights) + T * bg.  The
+    cases cover a stop at 1e-3, a stop at 1e-4 (needs alpha > 0.5), weights that
+    fall below the threshold before T does (no stop: T keeps falling to the box
+    exit) and a thin medium."""
+    from plenvdb_amd import scenes
+
+    R, H, W = 24, 20, 22
+    idx = np.arange(1, R ** 3 + 1, dtype=np.int32).reshape(R, R, R)
+    den = np.full(R ** 3 + 1, sigma, np.float32)
+    den[0] = 0
+    rng = np.random.default_rng(5)
+    cvec = rng.normal(0, 0.5, 12).astype(np.float32)
+    col = np.tile(cvec, (R ** 3 + 1, 1)).astype(np.float32)
+    col[0] = 0
+    mlp = scenes.mlp_params(scenes.CONFIGS["c2"])
+    mlp["b2"] = rng.normal(0, 0.3, 3).astype(np.float32)
+    pose = scenes.orbit_poses(8, radius=4.0)[3]
+    K = scenes.intrinsics(H, W, 14.0)  # wide enough that some rays miss the box
+    mn, mx = np.full(3, -1.5, np.float32), np.full(3, 1.5, np.float32)
+    shift, interval, stepdist, bg, thres = -4.595, 0.5, 3.0 / R * 0.5, 0.25, 1e-4
+    img, info = oracle_mod.reference_render(idx, den, col, mlp, pose, K, mn, mx, H, W, 0.5,
+                                            stepdist, shift, interval, thres, bg, t_stop=t_stop,
+                                            prec="f64")
+    alpha = 1.0 - (1.0 + np.exp(sigma + shift)) ** (-interval)
+    # pixel rays and PE, independently of the oracle
+    i, j = np.meshgrid(np.arange(W) + 0.5, np.arange(H) + 0.5)
+    d = np.stack([(i - K[0, 2]) / K[0, 0], -(j - K[1, 2]) / K[1, 1], -np.ones_like(i)], -1)
+    rd = d.reshape(-1, 3) @ pose[:3, :3].T.astype(np.float64)
+    vd = rd / np.linalg.norm(rd, axis=1, keepdims=True)
+    x = np.concatenate([np.tile(cvec.astype(np.float64), (H * W, 1)), _pe27(vd)], 1)
+    h = np.maximum(x @ mlp["w0"] + mlp["b0"], 0)
+    h = np.maximum(h @ mlp["w1"] + mlp["b1"], 0)
+    c = 1 / (1 + np.exp(-(h @ mlp["w2"] + mlp["b2"])))
+    # in-box sample count from the slab test
+    o = (pose[:3, 3].astype(np.float64) - mn) / (mx - mn)
+    dn = rd / (mx - mn)
+    step = stepdist / np.linalg.norm(rd, axis=1)
+    with np.errstate(divide="ignore"):
+        ta, tb = (1 - o) / dn, -o / dn
+    tmin = np.maximum(np.minimum(ta, tb).max(1), 0.5)
+    tmax = np.maximum(np.maximum(ta, tb).min(1), 0.5)
+    span = (tmax - tmin) / step
+    n_in = np.maximum(np.floor(span + 1e-9), 0).astype(np.int64)
+    clear = np.abs(span - np.round(span)) > 1e-3  # away from a sample on the box face
+    want = np.zeros((H * W, 3))
+    kept = np.zeros(H * W, np.int64)
+    for p in range(H * W):
+        T, S = 1.0, 0.0
+        for _ in range(n_in[p]):
+            w = T * alpha
+            T = T * (1 - alpha)
+            if w <= thres:
+                continue
+            S += w
+            kept[p] += 1
+            if T < t_stop:
+                break
+        want[p] = c[p] * S + T * bg
+    assert (kept[clear] == 0).any() and (kept[clear] > 0).sum() > H * W // 4
+    assert np.array_equal(info["count"][clear], kept[clear])
+    assert np.abs(img.reshape(-1, 3)[clear] - want[clear]).max() < 1e-6
