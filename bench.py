@@ -174,7 +174,19 @@ def bench_batched(args, scene):
         "v_u": v_u,
         "n_rays": tr.n_rays,
         "color_bytes": 2 if tr.color_bf16 else 4,
+        "workload": args.config,
     }
+
+
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
+# the default workloads, scaled by a 256 MiB copy's counts (FETCH x2.0, WRITE
+# x1.0); profiles/pmc_traffic_c2_train_c5_render.md holds the runs.
+TRAFFIC_SOURCE = "profiles/pmc_traffic_c2_train_c5_render.md"
+MEASURED_TRAFFIC_MB = {
+    "c2": {"mlp_bwd": 78.91, "scatter": 70.15, "mlp_fwd": 56.13, "ray_march": 14.07,
+           "dw_reduce": 53.45, "composite": 11.75},
+    "c5": {"render": 112.07},
+}
 
 
 def stage_models(r):
@@ -205,17 +217,28 @@ def stage_models(r):
     }
 
 
-def stage_roofline(name, kind, work, avg_s):
+def measured_traffic(workload: str, stage: str):
+    mb = MEASURED_TRAFFIC_MB.get(workload, {}).get(stage)
+    return None if mb is None else round(mb * 1e6)
+
+
+def stage_roofline(name, kind, work, avg_s, workload="c2"):
+    traffic = measured_traffic(workload, name)
     if kind == "mfma":
         achieved = work / avg_s / 1e12
-        return {"kernel": name, "bound": "mfma", "achieved": round(achieved, 3),
-                "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 6), "traffic": None,
-                "algorithmic_flop": float(work), "avg_launch_ms": round(avg_s * 1e3, 5)}
-    achieved = work / avg_s / 1e9
-    return {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-            "algorithmic_bytes": float(work), "avg_launch_ms": round(avg_s * 1e3, 5)}
+        out = {"kernel": name, "bound": "mfma", "achieved": round(achieved, 3),
+               "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+               "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 6), "traffic": traffic,
+               "algorithmic_flop": float(work), "avg_launch_ms": round(avg_s * 1e3, 5)}
+    else:
+        achieved = work / avg_s / 1e9
+        out = {"kernel": name, "bound": "hbm", "achieved": round(achieved, 2),
+               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+               "traffic": traffic, "algorithmic_bytes": float(work),
+               "avg_launch_ms": round(avg_s * 1e3, 5)}
+    if traffic is not None:
+        out["traffic_source"] = TRAFFIC_SOURCE
+    return out
 
 
 def batched_roofline(r, steps):
@@ -227,7 +250,7 @@ def batched_roofline(r, steps):
     for name, (tot, calls) in st.items():
         if calls and name in models:
             kind, work = models[name]
-            table[name] = stage_roofline(name, kind, work, tot / calls * 1e-3)
+            table[name] = stage_roofline(name, kind, work, tot / calls * 1e-3, r["workload"])
     dom = max((k for k in st if k in table), key=lambda k: st[k][0])
     roof = dict(table[dom])
     N, sc, L_t, V_u = r["n_rays"], r["color_bytes"], r["l_t"], r["v_u"]
@@ -315,6 +338,8 @@ def bench_render(frames: int, warmup: int):
         "roofline": {"kernel": "k_render", "bound": "mfma", "achieved": round(tflops, 3),
                      "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 6),
+                     "traffic": measured_traffic("c5", "render"),
+                     "traffic_source": TRAFFIC_SOURCE,
                      "algorithmic_flop": flop, "avg_launch_ms": round(ms_frame, 5)},
     }
 
