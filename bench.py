@@ -175,6 +175,7 @@ def bench_batched(args, scene):
         "n_rays": tr.n_rays,
         "color_bytes": 2 if tr.color_bf16 else 4,
         "workload": args.config,
+        "n_cu": min(512, torch.cuda.get_device_properties(0).multi_processor_count),
     }
 
 
@@ -210,8 +211,8 @@ def stage_models(r):
         "mlp_bwd": ("mfma", nw * MLP_BWD_FLOP + N * MLP_RAY_FLOP),
         # records + per-record gradients in, distinct voxels' 13 gradients read+written
         "scatter": ("hbm", S * (16 + 16 + 64) + uw * 13 * 4 * 2),
-        # 512 workgroup partials in, MLP gradient read+written
-        "dw_reduce": ("hbm", 512 * P * 4 + 2 * P * 4),
+        # one dW partial per CU in, MLP gradient read+written
+        "dw_reduce": ("hbm", r["n_cu"] * P * 4 + 2 * P * 4),
         # survey B_adam: density p,m,v r/w + g r; colour p,m,v r/w + g r
         "adam": ("hbm", V_u * (28 + 12 * (2 * sc + 20)) + L_t * 512 * 13 * 4),
     }

@@ -1,23 +1,28 @@
 // Fused step, sample side: the view-dependent RGB MLP (39-128-128-3, ReLU,
 // sigmoid; plenvdb/lib/dvgo.py:91-109,354-362) on MFMA, forward and backward,
-// over groups of 16 consecutive records.
+// over groups of 32 consecutive records (two 16-sample MFMA tiles).
 //
 // A workgroup of 4 wavefronts takes a group; wave w owns hidden features
-// [32w, 32w+32) of both hidden layers, so each wave's slices of the weights
-// stay in registers for the whole kernel and activations are exchanged through
-// LDS.  Activations are computed transposed (features x samples):
+// [32w, 32w+32) of both hidden layers and exchanges activations with the
+// other waves through LDS.  Activations are computed transposed (features x
+// samples):
 //   H0^T = relu(W0^T X^T + b0),  H1^T = relu(W1^T H0^T + b1),  L^T = W2^T H1^T + b2
-// with v_mfma_f32_16x16x32_bf16 (samples are the N dimension).  The backward
-// pass recomputes the forward, forms dL/dlogit from the per-ray loss gradient,
-// runs g_H1^T = W2 g_L^T, g_H0^T = W1 g_H1^T (masked by the ReLUs),
-// g_X^T = W0[0:12] g_H0^T, and accumulates dW with v_mfma_f32_16x16x16_bf16
-// over the samples (K = 16) into per-wave register tiles that are written once
-// per workgroup.  The same pass evaluates the Alphas2Weights / raw2alpha
-// adjoints (render_utils_kernel.cu:653-676, 506-517) and writes each record's
-// feature and sigma gradients; k_scatter_agg (or k_scatter) then adds them to
-// the grids with the "leaf must exist" rule (densityvdb.cu:16-27,
+// with v_mfma_f32_16x16x32_bf16 (samples are the N dimension).  The forward
+// weight slices stay in registers; the backward weights sit in LDS.  The
+// backward pass recomputes the forward, forms dL/dlogit from the per-ray loss
+// gradient, runs g_H1^T = W2 g_L^T, g_H0^T = W1 g_H1^T (masked by the ReLUs,
+// kept as bit masks), g_X^T = W0[0:12] g_H0^T, and accumulates dW over the
+// group's 32 samples (K = 32) into per-wave accumulator tiles written once per
+// workgroup.  The same pass evaluates the Alphas2Weights / raw2alpha adjoints
+// (render_utils_kernel.cu:653-676, 506-517) and writes each record's feature
+// and sigma gradients; k_scatter_agg (or k_scatter) then adds them to the
+// grids with the "leaf must exist" rule (densityvdb.cu:16-27,
 // colorvdb.cu:28-37).  The forward pass keeps the gathered BF16 features per
 // record, so the backward reads 32 bytes per record instead of 8 gathers.
+// Each group's records, features and ray data are loaded into registers while
+// the previous group computes.
+#include <type_traits>
+
 #include "fused_common.h"
 #include "gather.h"
 #include "plenvdb_hip.h"
@@ -30,20 +35,19 @@ using namespace pvdb;
 
 namespace {
 
-constexpr int XM = 72;   // sample-major stride of X (64 + 8)
-constexpr int SM = 136;  // sample-major stride of 128-wide activations (128 + 8)
-constexpr int FM = 20;   // feature-major stride (16 samples + 4)
-constexpr int GM = 40;   // sample-major stride of dL/dlogit (32 + 8)
+constexpr int GS = 32;       // records per group
+constexpr int NT = GS / 16;  // 16-sample MFMA tiles per group
+constexpr int XM = 72;       // sample-major stride of X (64 + 8)
+constexpr int SM = 136;      // sample-major stride of 128-wide activations (128 + 8)
+constexpr int FM = 40;       // feature-major stride (32 samples + 8; rows stay 16-byte aligned)
+constexpr int GM = 40;       // sample-major stride of dL/dlogit (32 + 8)
 
 __device__ __forceinline__ bf16x8 ld_frag(const uint16_t* p) {
     const uint4 v = *reinterpret_cast<const uint4*>(p);
     return __builtin_bit_cast(bf16x8, v);
 }
 
-__device__ __forceinline__ s16x4 ld_frag4(const uint16_t* p) {
-    const uint2 v = *reinterpret_cast<const uint2*>(p);
-    return __builtin_bit_cast(s16x4, v);
-}
+__device__ __forceinline__ bf16x8 as_bf8(const uint4& v) { return __builtin_bit_cast(bf16x8, v); }
 
 __device__ __forceinline__ void st4(uint16_t* p, float a, float b, float c, float d) {
     uint2 v;
@@ -58,43 +62,67 @@ struct RecInfo {
     uint32_t bits;
 };
 
-struct Smem {
-    uint16_t Xs[16 * XM];
+struct alignas(16) SmemFwd {
+    uint16_t Xs[GS * XM];
+    uint16_t H0s[GS * SM];
+    uint16_t H1s[GS * SM];
+    RecInfo ri[GS];
+};
+
+// Backward weights as MFMA A fragments, [tile][k-step][lane] x 16 bytes:
+//   W2  [8 row tiles of 16 hidden][1][64]   (K = 3 logits, padded to 32)
+//   W1  [8 row tiles][4][64]                (K = 128 hidden outputs)
+//   W0  [1 (12 feature rows)][4 k-steps of 32 hidden][64]
+constexpr int kBw2Frags = 8 * 64, kBw1Frags = 8 * 4 * 64, kBw0Frags = 4 * 64;
+
+struct alignas(16) SmemBwd {
+    uint4 Bw2[kBw2Frags];
+    uint4 Bw1[kBw1Frags];
+    uint4 Bw0[kBw0Frags];
+    uint16_t Xs[GS * XM];
     uint16_t Xf[64 * FM];
-    uint16_t H0s[16 * SM];
+    uint16_t H0s[GS * SM];
     uint16_t H0f[128 * FM];
-    uint16_t H1s[16 * SM];
+    uint16_t H1s[GS * SM];
     uint16_t H1f[128 * FM];
-    uint16_t GLs[16 * GM];
+    uint16_t GLs[GS * GM];
     uint16_t GLf[16 * FM];
-    uint16_t G1s[16 * SM];
+    uint16_t G1s[GS * SM];
     uint16_t G1f[128 * FM];
-    uint16_t G0s[16 * SM];
+    uint16_t G0s[GS * SM];
     uint16_t G0f[128 * FM];
-    float GXp[4][16][16];
-    float gsig[16];
-    float rgb[16][4];
-    RecInfo ri[16];
-    float4 rayg[16];
-    float4 ctot[16];
-    float4 cinc[16];
-    float tlast[16];
+    float GXp[4][16][GS];
+    float gsig[GS];
+    float rgb[GS][4];
+    RecInfo ri[GS];
+    float4 rayg[GS];
+    float4 ctot[GS];
+    float4 cinc[GS];
+    float tlast[GS];
 };
 
 __device__ __forceinline__ float sigmoidf_(float x) { return 1.f / (1.f + expf(-x)); }
+
+// Register prefetch of one group's per-record inputs (thread = record).
+struct Prefetch {
+    float4 xyze, atwr;          // stage A: the record
+    float4 rayg, ctot, cinc;    // stage B (backward): ray data, prefix
+    float tlast;
+    uint4 f0, f1;               // stage B (backward): kept features
+};
 
 template <bool BWD, bool BF16COL>
 __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const void* color,
                                              MlpPacked W, RayBuffers R, Records rec,
                                              float* __restrict__ dw_partial) {
+    using Smem = std::conditional_t<BWD, SmemBwd, SmemFwd>;
     __shared__ Smem sm;
     const int tid = threadIdx.x;
     const int wv = tid >> 6, lane = tid & 63, r16 = lane & 15, g = lane >> 4;
     const int f0 = 32 * wv;  // first hidden feature owned by this wave
 
-    // ---- weight slices -> registers (bf16 fragments) ----
+    // ---- forward weight slices -> registers (bf16 fragments) ----
     bf16x8 A0[2][2], A1[2][4], A2[4];
-    bf16x8 Bw2[2], Bw1[2][4], Bw0;
     float b0v[2][4], b1v[2][4], b2v[3];
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
@@ -103,11 +131,6 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
         for (int kt = 0; kt < 2; ++kt) A0[mt][kt] = ld_frag(W.w0t + row * 64 + 32 * kt + 8 * g);
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) A1[mt][kt] = ld_frag(W.w1t + row * 128 + 32 * kt + 8 * g);
-        if (BWD) {
-            Bw2[mt] = ld_frag(W.w2 + row * 32 + 8 * g);
-#pragma unroll
-            for (int kt = 0; kt < 4; ++kt) Bw1[mt][kt] = ld_frag(W.w1 + row * 128 + 32 * kt + 8 * g);
-        }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             b0v[mt][i] = W.b0[f0 + 16 * mt + 4 * g + i];
@@ -116,15 +139,31 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
     }
 #pragma unroll
     for (int kt = 0; kt < 4; ++kt) A2[kt] = ld_frag(W.w2t + r16 * 128 + 32 * kt + 8 * g);
-    if (BWD) Bw0 = ld_frag(W.w0 + r16 * 128 + f0 + 8 * g);
     b2v[0] = W.b2[0];
     b2v[1] = W.b2[1];
     b2v[2] = W.b2[2];
 
-    // ---- dW accumulators (BWD) ----
+    // ---- backward weights -> LDS, dW accumulators ----
     f32x4 dW0[4][2], dW1[8][2], dW2[2];
     float db1[2][4], db2[3];
-    if (BWD) {
+    if constexpr (BWD) {
+        for (int f = tid; f < kBw2Frags + kBw1Frags + kBw0Frags; f += 256) {
+            const int ln = f & 63, gg = ln >> 4, rr = ln & 15;
+            uint4 v;
+            if (f < kBw2Frags) {  // W2 [128 in][32 out (3 used)]
+                const int t = f >> 6;
+                v = *reinterpret_cast<const uint4*>(W.w2 + (16 * t + rr) * 32 + 8 * gg);
+                sm.Bw2[f] = v;
+            } else if (f < kBw2Frags + kBw1Frags) {  // W1 [128 in][128 out]
+                const int q = f - kBw2Frags, t = q >> 8, kt = (q >> 6) & 3;
+                v = *reinterpret_cast<const uint4*>(W.w1 + (16 * t + rr) * 128 + 32 * kt + 8 * gg);
+                sm.Bw1[q] = v;
+            } else {  // W0 rows 0..15 [16][128]
+                const int q = f - kBw2Frags - kBw1Frags, kt = q >> 6;
+                v = *reinterpret_cast<const uint4*>(W.w0 + rr * 128 + 32 * kt + 8 * gg);
+                sm.Bw0[q] = v;
+            }
+        }
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -142,120 +181,142 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
     }
 
     const int64_t total = min((int64_t)R.total[0], P.capacity);
-    for (int64_t grp = blockIdx.x;; grp += gridDim.x) {
-        const int64_t base = grp * 16;
-        if (base >= total) break;  // uniform over the workgroup
-        // ---- 1. records of the group ----
-        if (tid < 16) {
-            const int64_t q = base + tid;
-            RecInfo ri;
-            ri.bits = 0u;
-            ri.x = ri.y = ri.z = ri.e = ri.alpha = ri.T = ri.w = 0.f;
-            if (q < total) {
-                const float4 a = rec.xyze[q];
-                const float4 b = rec.atwr[q];
-                ri.x = a.x;
-                ri.y = a.y;
-                ri.z = a.z;
-                ri.e = a.w;
-                ri.alpha = b.x;
-                ri.T = b.y;
-                ri.w = b.z;
-                ri.bits = __float_as_uint(b.w);
+    const int64_t ngroups = (total + GS - 1) / GS;
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+
+    // ---- prefetch of the first group ----
+    Prefetch pf;
+    pf.xyze = pf.atwr = pf.rayg = pf.ctot = pf.cinc = z4;
+    pf.tlast = 0.f;
+    pf.f0 = pf.f1 = make_uint4(0u, 0u, 0u, 0u);
+    auto stage_a = [&](int64_t grp) {
+        const int64_t q = grp * GS + tid;
+        pf.xyze = pf.atwr = z4;
+        if (tid < GS && grp < ngroups && q < total) {
+            pf.xyze = rec.xyze[q];
+            pf.atwr = rec.atwr[q];
+        }
+    };
+    auto stage_b = [&](int64_t grp) {  // backward only; needs stage A's atwr
+        const int64_t q = grp * GS + tid;
+        pf.rayg = pf.ctot = pf.cinc = z4;
+        pf.tlast = 0.f;
+        pf.f0 = pf.f1 = make_uint4(0u, 0u, 0u, 0u);
+        if (tid < GS && grp < ngroups && q < total) {
+            const uint32_t bits = __float_as_uint(pf.atwr.w);
+            if (bits & (kAliveA | kAliveW)) {
+                const uint32_t ray = bits & kRayMask;
+                pf.rayg = R.ray_grad[ray];
+                pf.ctot = R.ray_ctot[ray];
+                pf.tlast = R.t_last[ray];
+                pf.cinc = rec.cinc[q];
             }
-            sm.ri[tid] = ri;
-            if (BWD) {
-                const uint32_t ray = ri.bits & kRayMask;
-                const bool any = (ri.bits & (kAliveA | kAliveW)) != 0u;
-                sm.rayg[tid] = any ? R.ray_grad[ray] : make_float4(0.f, 0.f, 0.f, 0.f);
-                sm.ctot[tid] = any ? R.ray_ctot[ray] : make_float4(0.f, 0.f, 0.f, 0.f);
-                sm.cinc[tid] = any ? rec.cinc[q] : make_float4(0.f, 0.f, 0.f, 0.f);
-                sm.tlast[tid] = any ? R.t_last[ray] : 0.f;
+            if (bits & kAliveW) {
+                pf.f0 = rec.feat[2 * q];
+                pf.f1 = rec.feat[2 * q + 1];
             }
         }
+    };
+    stage_a(blockIdx.x);
+    if constexpr (BWD) stage_b(blockIdx.x);
+
+    for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {  // uniform bound
+        const int64_t base = grp * GS;
+        // ---- 1. commit the prefetched group to LDS; start loading the next one ----
+        if (tid < GS) {
+            RecInfo ri;
+            ri.x = pf.xyze.x;
+            ri.y = pf.xyze.y;
+            ri.z = pf.xyze.z;
+            ri.e = pf.xyze.w;
+            ri.alpha = pf.atwr.x;
+            ri.T = pf.atwr.y;
+            ri.w = pf.atwr.z;
+            ri.bits = __float_as_uint(pf.atwr.w);  // zero for records past the end
+            sm.ri[tid] = ri;
+            if constexpr (BWD) {
+                sm.rayg[tid] = pf.rayg;
+                sm.ctot[tid] = pf.ctot;
+                sm.cinc[tid] = pf.cinc;
+                sm.tlast[tid] = pf.tlast;
+                const uint32_t w6[6] = {pf.f0.x, pf.f0.y, pf.f0.z, pf.f0.w, pf.f1.x, pf.f1.y};
+                uint32_t* xs = reinterpret_cast<uint32_t*>(&sm.Xs[tid * XM]);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    xs[c] = w6[c];
+                    sm.Xf[(2 * c) * FM + tid] = (uint16_t)(w6[c] & 0xffffu);
+                    sm.Xf[(2 * c + 1) * FM + tid] = (uint16_t)(w6[c] >> 16);
+                }
+            }
+        }
+        const int64_t next = grp + gridDim.x;
+        stage_a(next);
         __syncthreads();
         // ---- 2. X = [colour features (12) | PE (27) | 1 | 0...] ----
-        if (BWD && tid < 16) {
-            // the forward pass kept the gathered features (bf16) per record
-            const int s = tid;
+        {
+            const int s = tid >> 3, part = tid & 7;  // 8 threads per record
             const RecInfo ri = sm.ri[s];
-            uint4 f0v = make_uint4(0u, 0u, 0u, 0u), f1v = make_uint4(0u, 0u, 0u, 0u);
-            if (ri.bits & kAliveW) {
-                f0v = rec.feat[2 * (base + s)];
-                f1v = rec.feat[2 * (base + s) + 1];
-            }
-            const uint32_t w6[6] = {f0v.x, f0v.y, f0v.z, f0v.w, f1v.x, f1v.y};
+            const bool alive = (ri.bits & kAliveW) != 0u;
+            if constexpr (!BWD) {
+                // colour gather: thread `part` takes corner `part`
+                float v[12];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                const uint16_t lo = (uint16_t)(w6[c] & 0xffffu), hi = (uint16_t)(w6[c] >> 16);
-                sm.Xs[s * XM + 2 * c] = lo;
-                sm.Xs[s * XM + 2 * c + 1] = hi;
-                sm.Xf[(2 * c) * FM + s] = lo;
-                sm.Xf[(2 * c + 1) * FM + s] = hi;
-            }
-        } else if (!BWD && tid < 128) {
-            const int s = tid >> 3, q = tid & 7;
-            const RecInfo ri = sm.ri[s];
-            float v[12];
+                for (int c = 0; c < 12; ++c) v[c] = 0.f;
+                if (alive) {
+                    const Cell cell = make_cell(ri.x, ri.y, ri.z);
+                    const int dx = (kCx >> part) & 1, dy = (kCy >> part) & 1, dz = (kCz >> part) & 1;
+                    const float a = dx ? cell.u : 1.f - cell.u;
+                    const float b = dy ? cell.v : 1.f - cell.v;
+                    const float d = dz ? cell.w : 1.f - cell.w;
+                    const float sc = (a * b) * d;
+                    const int64_t slot = slot_of(topo, cell.i0 + dx, cell.j0 + dy, cell.k0 + dz);
+                    if (slot >= 0) {
+                        if (BF16COL) {
+                            const uint16_t* src =
+                                reinterpret_cast<const uint16_t*>(color) + slot * 12;
+                            const uint2 p0 = *reinterpret_cast<const uint2*>(src);
+                            const uint2 p1 = *reinterpret_cast<const uint2*>(src + 4);
+                            const uint2 p2 = *reinterpret_cast<const uint2*>(src + 8);
+                            const uint32_t w6[6] = {p0.x, p0.y, p1.x, p1.y, p2.x, p2.y};
 #pragma unroll
-            for (int c = 0; c < 12; ++c) v[c] = 0.f;
-            if (ri.bits & kAliveW) {
-                const Cell cell = make_cell(ri.x, ri.y, ri.z);
-                const int dx = (kCx >> q) & 1, dy = (kCy >> q) & 1, dz = (kCz >> q) & 1;
-                const float a = dx ? cell.u : 1.f - cell.u;
-                const float b = dy ? cell.v : 1.f - cell.v;
-                const float d = dz ? cell.w : 1.f - cell.w;
-                const float sc = (a * b) * d;
-                const int64_t slot = slot_of(topo, cell.i0 + dx, cell.j0 + dy, cell.k0 + dz);
-                if (slot >= 0) {
-                    if (BF16COL) {
-                        const uint16_t* src = reinterpret_cast<const uint16_t*>(color) + slot * 12;
-                        const uint2 p0 = *reinterpret_cast<const uint2*>(src);
-                        const uint2 p1 = *reinterpret_cast<const uint2*>(src + 4);
-                        const uint2 p2 = *reinterpret_cast<const uint2*>(src + 8);
-                        const uint32_t w6[6] = {p0.x, p0.y, p1.x, p1.y, p2.x, p2.y};
-#pragma unroll
-                        for (int c = 0; c < 6; ++c) {
-                            v[2 * c] = bf_bits2f((uint16_t)(w6[c] & 0xffffu)) * sc;
-                            v[2 * c + 1] = bf_bits2f((uint16_t)(w6[c] >> 16)) * sc;
+                            for (int c = 0; c < 6; ++c) {
+                                v[2 * c] = bf_bits2f((uint16_t)(w6[c] & 0xffffu)) * sc;
+                                v[2 * c + 1] = bf_bits2f((uint16_t)(w6[c] >> 16)) * sc;
+                            }
+                        } else {
+                            const float4* src = reinterpret_cast<const float4*>(
+                                reinterpret_cast<const float*>(color) + slot * 12);
+                            const float4 p0 = src[0], p1 = src[1], p2 = src[2];
+                            v[0] = p0.x * sc; v[1] = p0.y * sc; v[2] = p0.z * sc; v[3] = p0.w * sc;
+                            v[4] = p1.x * sc; v[5] = p1.y * sc; v[6] = p1.z * sc; v[7] = p1.w * sc;
+                            v[8] = p2.x * sc; v[9] = p2.y * sc; v[10] = p2.z * sc; v[11] = p2.w * sc;
                         }
-                    } else {
-                        const float4* src = reinterpret_cast<const float4*>(
-                            reinterpret_cast<const float*>(color) + slot * 12);
-                        const float4 p0 = src[0], p1 = src[1], p2 = src[2];
-                        v[0] = p0.x * sc; v[1] = p0.y * sc; v[2] = p0.z * sc; v[3] = p0.w * sc;
-                        v[4] = p1.x * sc; v[5] = p1.y * sc; v[6] = p1.z * sc; v[7] = p1.w * sc;
-                        v[8] = p2.x * sc; v[9] = p2.y * sc; v[10] = p2.z * sc; v[11] = p2.w * sc;
+                    }
+                }
+#pragma unroll
+                for (int c = 0; c < 12; ++c) {
+                    v[c] += __shfl_xor(v[c], 1, 64);
+                    v[c] += __shfl_xor(v[c], 2, 64);
+                    v[c] += __shfl_xor(v[c], 4, 64);
+                }
+                if (part == 0) {
+                    uint32_t w6[6];
+#pragma unroll
+                    for (int c = 0; c < 6; ++c)
+                        w6[c] = (uint32_t)f2bf_bits(v[2 * c]) |
+                                ((uint32_t)f2bf_bits(v[2 * c + 1]) << 16);
+                    uint32_t* xs = reinterpret_cast<uint32_t*>(&sm.Xs[s * XM]);
+#pragma unroll
+                    for (int c = 0; c < 6; ++c) xs[c] = w6[c];
+                    const int64_t qq = base + s;
+                    if (qq < total && alive) {
+                        rec.feat[2 * qq] = make_uint4(w6[0], w6[1], w6[2], w6[3]);
+                        rec.feat[2 * qq + 1] = make_uint4(w6[4], w6[5], 0u, 0u);
                     }
                 }
             }
-#pragma unroll
-            for (int c = 0; c < 12; ++c) {
-                v[c] += __shfl_xor(v[c], 1, 64);
-                v[c] += __shfl_xor(v[c], 2, 64);
-                v[c] += __shfl_xor(v[c], 4, 64);
-            }
-            if (q == 0) {
-                uint16_t h[12];
-#pragma unroll
-                for (int c = 0; c < 12; ++c) {
-                    h[c] = f2bf_bits(v[c]);
-                    sm.Xs[s * XM + c] = h[c];
-                    sm.Xf[c * FM + s] = h[c];
-                }
-                const int64_t qq = base + s;
-                if (qq < total && (ri.bits & kAliveW)) {
-                    uint32_t w6[6];
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) w6[c] = (uint32_t)h[2 * c] | ((uint32_t)h[2 * c + 1] << 16);
-                    rec.feat[2 * qq] = make_uint4(w6[0], w6[1], w6[2], w6[3]);
-                    rec.feat[2 * qq + 1] = make_uint4(w6[4], w6[5], 0u, 0u);
-                }
-            }
-        } else if (tid >= 128) {
-            const int tt = tid - 128, s = tt >> 3, part = tt & 7;
-            const RecInfo ri = sm.ri[s];
-            const bool alive = (ri.bits & kAliveW) != 0u;
+            // positional encoding of the ray's view direction (dvgo.py:354-358) and the
+            // bias input; columns 40..63 are zero
             float vd[3] = {0.f, 0.f, 0.f};
             if (alive) {
                 const uint32_t ray = ri.bits & kRayMask;
@@ -281,205 +342,246 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                     }
                     const uint16_t h = f2bf_bits(val);
                     sm.Xs[s * XM + 12 + p] = h;
-                    sm.Xf[(12 + p) * FM + s] = h;
+                    if constexpr (BWD) sm.Xf[(12 + p) * FM + s] = h;
                 }
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const int col = 40 + part * 3 + k;
                 sm.Xs[s * XM + col] = 0;
-                sm.Xf[col * FM + s] = 0;
+                if constexpr (BWD) sm.Xf[col * FM + s] = 0;
             }
         }
         __syncthreads();
         // ---- 3. H0^T slice = relu(W0^T X^T + b0) ----
-        float h0[2][4], h1[2][4];
+        uint32_t m0 = 0u, m1 = 0u;  // ReLU masks: bit (mt*NT + nt)*4 + i
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int kt = 0; kt < 2; ++kt)
-                acc = mfma16x16x32(A0[mt][kt], ld_frag(&sm.Xs[r16 * XM + 32 * kt + 8 * g]), acc);
             const int fr = f0 + 16 * mt + 4 * g;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) h0[mt][i] = fmaxf(acc[i] + b0v[mt][i], 0.f);
-            st4(&sm.H0s[r16 * SM + fr], h0[mt][0], h0[mt][1], h0[mt][2], h0[mt][3]);
-            if (BWD) {
+            for (int nt = 0; nt < NT; ++nt) {
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int i = 0; i < 4; ++i) sm.H0f[(fr + i) * FM + r16] = f2bf_bits(h0[mt][i]);
+                for (int kt = 0; kt < 2; ++kt)
+                    acc = mfma16x16x32(A0[mt][kt],
+                                       ld_frag(&sm.Xs[(16 * nt + r16) * XM + 32 * kt + 8 * g]), acc);
+                float h[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    h[i] = fmaxf(acc[i] + b0v[mt][i], 0.f);
+                    if (h[i] > 0.f) m0 |= 1u << ((mt * NT + nt) * 4 + i);
+                }
+                st4(&sm.H0s[(16 * nt + r16) * SM + fr], h[0], h[1], h[2], h[3]);
+                if constexpr (BWD) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        sm.H0f[(fr + i) * FM + 16 * nt + r16] = f2bf_bits(h[i]);
+                }
             }
         }
         __syncthreads();
         // ---- 4. H1^T slice = relu(W1^T H0^T + b1) ----
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int kt = 0; kt < 4; ++kt)
-                acc = mfma16x16x32(A1[mt][kt], ld_frag(&sm.H0s[r16 * SM + 32 * kt + 8 * g]), acc);
             const int fr = f0 + 16 * mt + 4 * g;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) h1[mt][i] = fmaxf(acc[i] + b1v[mt][i], 0.f);
-            st4(&sm.H1s[r16 * SM + fr], h1[mt][0], h1[mt][1], h1[mt][2], h1[mt][3]);
-            if (BWD) {
+            for (int nt = 0; nt < NT; ++nt) {
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int i = 0; i < 4; ++i) sm.H1f[(fr + i) * FM + r16] = f2bf_bits(h1[mt][i]);
+                for (int kt = 0; kt < 4; ++kt)
+                    acc = mfma16x16x32(A1[mt][kt],
+                                       ld_frag(&sm.H0s[(16 * nt + r16) * SM + 32 * kt + 8 * g]), acc);
+                float h[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    h[i] = fmaxf(acc[i] + b1v[mt][i], 0.f);
+                    if (h[i] > 0.f) m1 |= 1u << ((mt * NT + nt) * 4 + i);
+                }
+                st4(&sm.H1s[(16 * nt + r16) * SM + fr], h[0], h[1], h[2], h[3]);
+                if constexpr (BWD) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        sm.H1f[(fr + i) * FM + 16 * nt + r16] = f2bf_bits(h[i]);
+                }
             }
         }
         __syncthreads();
-        // ---- 5. logits (wave 0): L^T = W2^T H1^T + b2, rgb = sigmoid ----
-        if (wv == 0) {
+        // ---- 5. logits (wave nt takes tile nt): L^T = W2^T H1^T + b2, rgb = sigmoid ----
+        if (wv < NT) {
+            const int nt = wv;
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int kt = 0; kt < 4; ++kt)
-                acc = mfma16x16x32(A2[kt], ld_frag(&sm.H1s[r16 * SM + 32 * kt + 8 * g]), acc);
+                acc = mfma16x16x32(A2[kt], ld_frag(&sm.H1s[(16 * nt + r16) * SM + 32 * kt + 8 * g]),
+                                   acc);
             if (g == 0) {
+                const int s = 16 * nt + r16;
                 const float c0 = sigmoidf_(acc[0] + b2v[0]);
                 const float c1 = sigmoidf_(acc[1] + b2v[1]);
                 const float c2 = sigmoidf_(acc[2] + b2v[2]);
-                sm.rgb[r16][0] = c0;
-                sm.rgb[r16][1] = c1;
-                sm.rgb[r16][2] = c2;
-                if (!BWD) {
-                    const int64_t q = base + r16;
+                if constexpr (BWD) {
+                    sm.rgb[s][0] = c0;
+                    sm.rgb[s][1] = c1;
+                    sm.rgb[s][2] = c2;
+                } else {
+                    const int64_t q = base + s;
                     if (q < total) {
-                        const bool alive = (sm.ri[r16].bits & kAliveW) != 0u;
-                        rec.rgb[q] = alive ? make_float4(c0, c1, c2, 0.f)
-                                           : make_float4(0.f, 0.f, 0.f, 0.f);
+                        const bool alive = (sm.ri[s].bits & kAliveW) != 0u;
+                        rec.rgb[q] = alive ? make_float4(c0, c1, c2, 0.f) : z4;
                     }
                 }
             }
         }
-        if (!BWD) {
+        if constexpr (!BWD) {
             __syncthreads();
             continue;
-        }
-        __syncthreads();
-        // ---- 6. per-sample adjoints: dL/dlogit, alpha, sigma ----
-        if (tid < 16) {
-            const RecInfo ri = sm.ri[tid];
-            const float4 gr = sm.rayg[tid];
-            float gl0 = 0.f, gl1 = 0.f, gl2 = 0.f, gw = 0.f;
-            if (ri.bits & kAliveW) {
-                const float c0 = sm.rgb[tid][0], c1 = sm.rgb[tid][1], c2 = sm.rgb[tid][2];
-                gl0 = ri.w * gr.x * c0 * (1.f - c0);
-                gl1 = ri.w * gr.y * c1 * (1.f - c1);
-                gl2 = ri.w * gr.z * c2 * (1.f - c2);
-                gw = gr.x * c0 + gr.y * c1 + gr.z * c2;
+        } else {
+            // the next group's stage-A loads have landed by now: start stage B
+            stage_b(next);
+            __syncthreads();
+            // ---- 6. per-sample adjoints: dL/dlogit, alpha, sigma ----
+            if (tid < GS) {
+                const RecInfo ri = sm.ri[tid];
+                const float4 gr = sm.rayg[tid];
+                float gl0 = 0.f, gl1 = 0.f, gl2 = 0.f, gw = 0.f;
+                if (ri.bits & kAliveW) {
+                    const float c0 = sm.rgb[tid][0], c1 = sm.rgb[tid][1], c2 = sm.rgb[tid][2];
+                    gl0 = ri.w * gr.x * c0 * (1.f - c0);
+                    gl1 = ri.w * gr.y * c1 * (1.f - c1);
+                    gl2 = ri.w * gr.z * c2 * (1.f - c2);
+                    gw = gr.x * c0 + gr.y * c1 + gr.z * c2;
+                }
+                float gs = 0.f;
+                if (ri.bits & kAliveA) {
+                    const float4 ct = sm.ctot[tid], ci = sm.cinc[tid];
+                    const float back = gr.w * sm.tlast[tid] + gr.x * (ct.x - ci.x) +
+                                       gr.y * (ct.y - ci.y) + gr.z * (ct.z - ci.z);
+                    const float ga = gw * ri.T - back / ((1.f - ri.alpha) + 1e-10f);
+                    gs = fminf(ri.e, 1e10f) * powf(1.f + ri.e, -P.interval - 1.f) * P.interval * ga;
+                }
+                sm.gsig[tid] = gs;
+                db2[0] += gl0;
+                db2[1] += gl1;
+                db2[2] += gl2;
+                uint32_t* gls = reinterpret_cast<uint32_t*>(&sm.GLs[tid * GM]);
+                gls[0] = (uint32_t)f2bf_bits(gl0) | ((uint32_t)f2bf_bits(gl1) << 16);
+                gls[1] = (uint32_t)f2bf_bits(gl2);
+#pragma unroll
+                for (int c = 2; c < 16; ++c) gls[c] = 0u;
+#pragma unroll
+                for (int o = 0; o < 16; ++o) {
+                    const float v = o == 0 ? gl0 : o == 1 ? gl1 : o == 2 ? gl2 : 0.f;
+                    sm.GLf[o * FM + tid] = f2bf_bits(v);
+                }
             }
-            float gs = 0.f;
-            if (ri.bits & kAliveA) {
-                const float4 ct = sm.ctot[tid], ci = sm.cinc[tid];
-                const float back = gr.w * sm.tlast[tid] + gr.x * (ct.x - ci.x) +
-                                   gr.y * (ct.y - ci.y) + gr.z * (ct.z - ci.z);
-                const float ga = gw * ri.T - back / ((1.f - ri.alpha) + 1e-10f);
-                gs = fminf(ri.e, 1e10f) * powf(1.f + ri.e, -P.interval - 1.f) * P.interval * ga;
+            __syncthreads();
+            // ---- 7. g_H1^T slice = W2[slice] g_L^T, masked; dW2 slice ----
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                const bf16x8 a = as_bf8(sm.Bw2[(wv * 2 + mt) * 64 + lane]);
+                const int fr = f0 + 16 * mt + 4 * g;
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                    acc = mfma16x16x32(a, ld_frag(&sm.GLs[(16 * nt + r16) * GM + 8 * g]), acc);
+                    float gv[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        gv[i] = (m1 >> ((mt * NT + nt) * 4 + i)) & 1u ? acc[i] : 0.f;
+                        db1[mt][i] += gv[i];
+                    }
+                    st4(&sm.G1s[(16 * nt + r16) * SM + fr], gv[0], gv[1], gv[2], gv[3]);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        sm.G1f[(fr + i) * FM + 16 * nt + r16] = f2bf_bits(gv[i]);
+                }
+                // dW2[in = f0+16mt+row][out] += sum over the group's samples H1[s][in] gL[s][out]
+                dW2[mt] = mfma16x16x32(ld_frag(&sm.H1f[(f0 + 16 * mt + r16) * FM + 8 * g]),
+                                       ld_frag(&sm.GLf[r16 * FM + 8 * g]), dW2[mt]);
             }
-            sm.gsig[tid] = gs;
-            db2[0] += gl0;
-            db2[1] += gl1;
-            db2[2] += gl2;
+            __syncthreads();
+            // ---- 8. g_H0^T slice = W1[slice] g_H1^T, masked; dW1 columns of this wave ----
 #pragma unroll
-            for (int c = 0; c < 32; ++c) {
-                const float v = c == 0 ? gl0 : c == 1 ? gl1 : c == 2 ? gl2 : 0.f;
-                sm.GLs[tid * GM + c] = f2bf_bits(v);
+            for (int mt = 0; mt < 2; ++mt) {
+                const int fr = f0 + 16 * mt + 4 * g;
+                f32x4 acc[NT];
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int kt = 0; kt < 4; ++kt) {
+                    const bf16x8 a = as_bf8(sm.Bw1[((wv * 2 + mt) * 4 + kt) * 64 + lane]);
+#pragma unroll
+                    for (int nt = 0; nt < NT; ++nt)
+                        acc[nt] = mfma16x16x32(
+                            a, ld_frag(&sm.G1s[(16 * nt + r16) * SM + 32 * kt + 8 * g]), acc[nt]);
+                }
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    float gv[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        gv[i] = (m0 >> ((mt * NT + nt) * 4 + i)) & 1u ? acc[nt][i] : 0.f;
+                    st4(&sm.G0s[(16 * nt + r16) * SM + fr], gv[0], gv[1], gv[2], gv[3]);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        sm.G0f[(fr + i) * FM + 16 * nt + r16] = f2bf_bits(gv[i]);
+                }
             }
 #pragma unroll
-            for (int o = 0; o < 16; ++o) {
-                const float v = o == 0 ? gl0 : o == 1 ? gl1 : o == 2 ? gl2 : 0.f;
-                sm.GLf[o * FM + tid] = f2bf_bits(v);
-            }
-        }
-        __syncthreads();
-        // ---- 7. g_H1^T slice = W2[slice] g_L^T, masked; dW2 slice ----
+            for (int mi = 0; mi < 8; ++mi)
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            acc = mfma16x16x32(Bw2[mt], ld_frag(&sm.GLs[r16 * GM + 8 * g]), acc);
-            const int fr = f0 + 16 * mt + 4 * g;
-            float gv[4];
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                gv[i] = h1[mt][i] > 0.f ? acc[i] : 0.f;
-                db1[mt][i] += gv[i];
-            }
-            st4(&sm.G1s[r16 * SM + fr], gv[0], gv[1], gv[2], gv[3]);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) sm.G1f[(fr + i) * FM + r16] = f2bf_bits(gv[i]);
-            // dW2[in = f0+16mt+row][out] += sum_s H1[s][in] gL[s][out]
-            dW2[mt] = mfma16x16x16(ld_frag4(&sm.H1f[(f0 + 16 * mt + r16) * FM + 4 * g]),
-                                   ld_frag4(&sm.GLf[r16 * FM + 4 * g]), dW2[mt]);
-        }
-        __syncthreads();
-        // ---- 8. g_H0^T slice = W1[slice] g_H1^T, masked; dW1 columns of this wave ----
-        float g0v[2][4];
-#pragma unroll
-        for (int mt = 0; mt < 2; ++mt) {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int kt = 0; kt < 4; ++kt)
-                acc = mfma16x16x32(Bw1[mt][kt], ld_frag(&sm.G1s[r16 * SM + 32 * kt + 8 * g]), acc);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) g0v[mt][i] = h0[mt][i] > 0.f ? acc[i] : 0.f;
-            const int fr = f0 + 16 * mt + 4 * g;
-            st4(&sm.G0s[r16 * SM + fr], g0v[mt][0], g0v[mt][1], g0v[mt][2], g0v[mt][3]);
-        }
-#pragma unroll
-        for (int mi = 0; mi < 8; ++mi)
-#pragma unroll
-            for (int nt = 0; nt < 2; ++nt)
-                dW1[mi][nt] = mfma16x16x16(ld_frag4(&sm.H0f[(16 * mi + r16) * FM + 4 * g]),
-                                           ld_frag4(&sm.G1f[(f0 + 16 * nt + r16) * FM + 4 * g]),
-                                           dW1[mi][nt]);
-        __syncthreads();
-        // ---- 9. dW0 columns of this wave; partial g_X^T over this wave's K slice ----
-#pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                sm.G0f[(f0 + 16 * mt + 4 * g + i) * FM + r16] = f2bf_bits(g0v[mt][i]);
-        __syncthreads();
-        {
-            const uint16_t* G0f = sm.G0f;
+                for (int nt = 0; nt < 2; ++nt)
+                    dW1[mi][nt] = mfma16x16x32(ld_frag(&sm.H0f[(16 * mi + r16) * FM + 8 * g]),
+                                               ld_frag(&sm.G1f[(f0 + 16 * nt + r16) * FM + 8 * g]),
+                                               dW1[mi][nt]);
+            __syncthreads();
+            // ---- 9. dW0 columns of this wave; partial g_X^T over this wave's K slice ----
 #pragma unroll
             for (int mi = 0; mi < 4; ++mi)
 #pragma unroll
                 for (int nt = 0; nt < 2; ++nt)
-                    dW0[mi][nt] = mfma16x16x16(ld_frag4(&sm.Xf[(16 * mi + r16) * FM + 4 * g]),
-                                               ld_frag4(&G0f[(f0 + 16 * nt + r16) * FM + 4 * g]),
+                    dW0[mi][nt] = mfma16x16x32(ld_frag(&sm.Xf[(16 * mi + r16) * FM + 8 * g]),
+                                               ld_frag(&sm.G0f[(f0 + 16 * nt + r16) * FM + 8 * g]),
                                                dW0[mi][nt]);
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            acc = mfma16x16x32(Bw0, ld_frag(&sm.G0s[r16 * SM + f0 + 8 * g]), acc);
+            {
+                const bf16x8 a = as_bf8(sm.Bw0[wv * 64 + lane]);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) sm.GXp[wv][4 * g + i][r16] = acc[i];
-        }
-        __syncthreads();
-        // ---- 10. per-record gradients: dL/dfeature (12) and dL/dsigma ----
-        if (tid < 64) {
-            const int s = tid >> 2, part = tid & 3;  // 4 threads write the 4 float4 of a record
-            const int64_t q = base + s;
-            if (q < total) {
-                const RecInfo ri = sm.ri[s];
-                float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (part < 3) {
-                    if (ri.bits & kAliveW) {
-                        float g4[4];
+                for (int nt = 0; nt < NT; ++nt) {
+                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                    acc = mfma16x16x32(a, ld_frag(&sm.G0s[(16 * nt + r16) * SM + f0 + 8 * g]), acc);
 #pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            const int c = 4 * part + k;
-                            g4[k] = sm.GXp[0][c][s] + sm.GXp[1][c][s] + sm.GXp[2][c][s] +
-                                    sm.GXp[3][c][s];
-                        }
-                        v = make_float4(g4[0], g4[1], g4[2], g4[3]);
-                    }
-                } else {
-                    v.x = sm.gsig[s];
+                    for (int i = 0; i < 4; ++i) sm.GXp[wv][4 * g + i][16 * nt + r16] = acc[i];
                 }
-                rec.grad[4 * q + part] = v;
             }
+            __syncthreads();
+            // ---- 10. per-record gradients: dL/dfeature (12) and dL/dsigma ----
+            if (tid < 4 * GS) {
+                const int s = tid >> 2, part = tid & 3;  // 4 threads write a record's 4 float4
+                const int64_t q = base + s;
+                if (q < total) {
+                    const RecInfo ri = sm.ri[s];
+                    float4 v = z4;
+                    if (part < 3) {
+                        if (ri.bits & kAliveW) {
+                            float g4[4];
+#pragma unroll
+                            for (int k = 0; k < 4; ++k) {
+                                const int c = 4 * part + k;
+                                g4[k] = sm.GXp[0][c][s] + sm.GXp[1][c][s] + sm.GXp[2][c][s] +
+                                        sm.GXp[3][c][s];
+                            }
+                            v = make_float4(g4[0], g4[1], g4[2], g4[3]);
+                        }
+                    } else {
+                        v.x = sm.gsig[s];
+                    }
+                    rec.grad[4 * q + part] = v;
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
+
     if (!BWD) return;
     // ---- flush this workgroup's dW partial ----
     float* out = dw_partial + (int64_t)blockIdx.x * kMlpParams;
@@ -517,11 +619,12 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
     if (wv == 0) {
 #pragma unroll
         for (int o = 0; o < 3; ++o) {
-            float v = tid < 16 ? db2[o] : 0.f;
+            float v = tid < GS ? db2[o] : 0.f;  // step 6 ran on threads 0..GS-1
             v += __shfl_xor(v, 1, 64);
             v += __shfl_xor(v, 2, 64);
             v += __shfl_xor(v, 4, 64);
             v += __shfl_xor(v, 8, 64);
+            v += __shfl_xor(v, 16, 64);
             if (tid == 0) out[kB2 + o] = v;
         }
     }
@@ -614,11 +717,7 @@ __global__ void k_pack_mlp(const float* __restrict__ prm, uint16_t* __restrict__
 
 namespace pvdb {
 
-int mlp_grid_size(int64_t total) {
-    int64_t groups = (total + 15) / 16;
-    int64_t g = groups < 1 ? 1 : groups;
-    return (int)(g < 512 ? g : 512);
-}
+
 
 int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const TopoView& t,
                const void* color, const MlpPacked& W, const RayBuffers& R, const Records& rec,

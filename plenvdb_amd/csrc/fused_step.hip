@@ -20,7 +20,6 @@ int launch_ray_march(const StepParams& P, const TopoView& t, const float* densit
                      const RayBuffers& R, const Records& rec, hipStream_t s);
 int launch_ray_composite(const StepParams& P, const RayBuffers& R, const Records& rec,
                          hipStream_t s);
-int mlp_grid_size(int64_t total);
 int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const TopoView& t,
                const void* color, const MlpPacked& W, const RayBuffers& R, const Records& rec,
                float* dw_partial, hipStream_t s);
@@ -336,8 +335,17 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
         if ((rc = launch_ray_march(P, t, d->density, R, rec, s))) return rc;
     }
     // The sample-side kernels are persistent over the records; their grid is
-    // fixed (the record count stays on the device).
+    // fixed (the record count stays on the device).  The backward holds one
+    // workgroup per CU (registers), so it runs one per CU and writes that many
+    // dW partials.
     const int grid = kMaxMlpGrid;
+    static int ncu = 0;
+    if (ncu == 0) {
+        int dev = 0;
+        PVDB_HIP(hipGetDevice(&dev));
+        PVDB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    const int grid_bwd = ncu < kMaxMlpGrid ? (ncu < 1 ? 1 : ncu) : kMaxMlpGrid;
     {
         StageScope sc(3, s);
         if ((rc = launch_mlp(false, d->color_bf16 != 0, grid, P, t, d->color, W, R, rec, nullptr,
@@ -350,8 +358,8 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     }
     {
         StageScope sc(5, s);
-        if ((rc = launch_mlp(true, d->color_bf16 != 0, grid, P, t, d->color, W, R, rec, dw_part,
-                             s)))
+        if ((rc = launch_mlp(true, d->color_bf16 != 0, grid_bwd, P, t, d->color, W, R, rec,
+                             dw_part, s)))
             return rc;
     }
     {
@@ -362,7 +370,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     }
     {
         StageScope sc(7, s);
-        if ((rc = launch_dw_reduce(dw_part, grid, d->mlp_grad, s))) return rc;
+        if ((rc = launch_dw_reduce(dw_part, grid_bwd, d->mlp_grad, s))) return rc;
     }
     if (d->ray_out) {
         PVDB_HIP(hipMemcpyAsync(d->ray_out, R.ray_out, 16 * (size_t)d->n_rays,
