@@ -147,7 +147,19 @@ def bench_batched(args, scene):
     t1.record()
     torch.cuda.synchronize()
     stages = fused.stage_timing(0)
-    ms = t0.elapsed_time(t1) / args.steps
+    ms_eager = t0.elapsed_time(t1) / args.steps
+    # the same steps replayed from one captured hipGraph (the production path)
+    tr.capture_graph()
+    for _ in range(args.warmup):
+        tr.step()
+    torch.cuda.synchronize()
+    t0.record()
+    for _ in range(args.steps):
+        tr.step()
+    t1.record()
+    torch.cuda.synchronize()
+    ms_graph = t0.elapsed_time(t1) / args.steps
+    ms = ms_graph
     samples = tr.total_records()
     # workload descriptors of one more (untimed) forward/backward
     tr.zero_grad()
@@ -165,6 +177,7 @@ def bench_batched(args, scene):
     pts = rec["xyze"][:, :3]
     return {
         "ms_per_step": ms,
+        "ms_per_step_eager": ms_eager,
         "samples_per_step": float(samples),
         "stages": stages,
         "n_color": int(alive_w.sum()),
@@ -445,6 +458,9 @@ def main():
         "stage_roofline": stage_fracs,
         "rays_per_s": round(main_res["n_rays"] / (main_res["ms_per_step"] * 1e-3), 1),
     }
+    if "ms_per_step_eager" in main_res:
+        line["ms_per_step_eager"] = round(main_res["ms_per_step_eager"], 4)
+        line["step_launch"] = "hipGraph replay (eager launch timed beside it)"
     if path == "batched":
         line["adam_voxels_per_s"] = round(main_res["v_u"] / (main_res["ms_per_step"] * 1e-3), 1)
     if "batched" in res and "plain" in res:

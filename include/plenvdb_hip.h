@@ -198,6 +198,10 @@ typedef struct pvdb_step_desc {
     int64_t capacity; /* max records (occupancy-kept samples) per step */
     float* ray_out;   /* dev [n][4] rgb + T_last, or NULL */
     float* loss;      /* dev [1] or NULL */
+    /* dev [3] or NULL: stepsz_density, stepsz_color, step_size_mlp read on the
+     * device instead of the three fields above, so one captured graph of the
+     * step can be replayed with each step's values */
+    const float* adam_scalars;
 } pvdb_step_desc;
 
 /* Workspace bytes for n_rays rays, `capacity` records and nleaf leaves (-1 on bad sizes). */

@@ -6,14 +6,15 @@
 //   k_ray_march    wave per ray: march again, density gather, raw2alpha, alpha cull,
 //                  transmittance product-scan with early stop, weights, weight cull;
 //                  one record per occupancy-kept sample (contiguous per ray)
-//   k_mlp<false>   workgroup per 16-record group: colour gather + MFMA MLP -> rgb,
-//                  the BF16 features are kept per record
+//   k_mlp<false>   workgroup per 32-record group: colour gather + MFMA MLP -> rgb,
+//                  the features are kept per record as bf16 (hi, lo) pairs
 //   k_ray_composite wave per ray: prefix of w*rgb, ray colour, MSE loss and its
 //                  gradient per ray (k_loss_reduce sums the per-ray losses)
-//   k_mlp<true>    workgroup per 16-record group: recompute from the kept
+//   k_mlp<true>    workgroup per 32-record group: recompute from the kept
 //                  features, MFMA MLP backward, alpha/raw2alpha adjoints, dW;
 //                  per-record feature and sigma gradients
-//   k_scatter      (record, corner) per thread: gradients into the grids
+//   scatter        records binned by base leaf, summed per leaf in LDS and
+//                  added to the grids without HBM atomics (fused_scatter.hip)
 //   k_dw_reduce, k_sparse_adam, dense Adam of the MLP, k_pack_mlp
 #pragma once
 
@@ -50,7 +51,7 @@ struct Records {
     float4* atwr;      // alpha, T, w, ray|flags (as bits)
     float4* rgb;       // MLP colour (rgb), written by the forward MLP pass
     float4* cinc;      // inclusive prefix of w*rgb along the ray
-    uint4* feat;       // [2 per record] the 12 gathered colour features as bf16
+    uint4* feat;       // [3 per record] the 12 gathered colour features as bf16 hi, then lo
     float4* grad;      // [4 per record] dL/dfeature (12) and dL/dsigma
 };
 

@@ -7,16 +7,21 @@
 // other waves through LDS.  Activations are computed transposed (features x
 // samples):
 //   H0^T = relu(W0^T X^T + b0),  H1^T = relu(W1^T H0^T + b1),  L^T = W2^T H1^T + b2
-// with v_mfma_f32_16x16x32_bf16 (samples are the N dimension).  The forward
-// weight slices stay in registers; the backward weights sit in LDS.  The
+// with v_mfma_f32_16x16x32_bf16 (samples are the N dimension).  Weights are
+// held in bf16; features, activations and gradients travel as (hi, lo) bf16
+// pairs, so each product with a weight is two MFMAs and each dW product
+// (activation x gradient) three, which keeps about 16 significant bits where
+// plain bf16 keeps 8 (plain bf16 inputs put the colour-feature gradients
+// several percent off an FP64 evaluation; the pairs bring them under 1e-3).
+// The forward weight slices stay in registers; the backward weights sit in LDS.  The
 // backward pass recomputes the forward, forms dL/dlogit from the per-ray loss
 // gradient, runs g_H1^T = W2 g_L^T, g_H0^T = W1 g_H1^T (masked by the ReLUs,
 // kept as bit masks), g_X^T = W0[0:12] g_H0^T, and accumulates dW over the
 // group's 32 samples (K = 32) into per-wave accumulator tiles written once per
 // workgroup.  The same pass evaluates the Alphas2Weights / raw2alpha adjoints
 // (render_utils_kernel.cu:653-676, 506-517) and writes each record's feature
-// and sigma gradients; k_scatter_agg (or k_scatter) then adds them to the
-// grids with the "leaf must exist" rule (densityvdb.cu:16-27,
+// and sigma gradients; the leaf-binned scatter (fused_scatter.hip) then adds
+// them to the grids with the "leaf must exist" rule (densityvdb.cu:16-27,
 // colorvdb.cu:28-37).  The forward pass keeps the gathered BF16 features per
 // record, so the backward reads 32 bytes per record instead of 8 gathers.
 // Each group's records, features and ray data are loaded into registers while
@@ -39,22 +44,56 @@ constexpr int GS = 32;       // records per group
 constexpr int NT = GS / 16;  // 16-sample MFMA tiles per group
 constexpr int XM = 72;       // sample-major stride of X (64 + 8)
 constexpr int SM = 136;      // sample-major stride of 128-wide activations (128 + 8)
-constexpr int FM = 40;       // feature-major stride (32 samples + 8; rows stay 16-byte aligned)
-constexpr int GM = 40;       // sample-major stride of dL/dlogit (32 + 8)
+constexpr int GM = 24;       // sample-major stride of dL/dlogit (16 + 8)
+constexpr int FM = 36;       // feature-major stride (32 samples + 4; rows 8-byte aligned)
+constexpr int XF_ROWS = 48;  // feature-major X rows: inputs 0..39 are used
 
-__device__ __forceinline__ bf16x8 ld_frag(const uint16_t* p) {
+__device__ __forceinline__ bf16x8 ld_frag(const uint16_t* p) {  // 16-byte aligned
     const uint4 v = *reinterpret_cast<const uint4*>(p);
     return __builtin_bit_cast(bf16x8, v);
 }
 
+__device__ __forceinline__ bf16x8 ld_frag8(const uint16_t* p) {  // 8-byte aligned
+    const uint2 a = *reinterpret_cast<const uint2*>(p);
+    const uint2 b = *reinterpret_cast<const uint2*>(p + 4);
+    return __builtin_bit_cast(bf16x8, make_uint4(a.x, a.y, b.x, b.y));
+}
+
+__device__ __forceinline__ s16x4 ld_frag4(const uint16_t* p) {
+    const uint2 v = *reinterpret_cast<const uint2*>(p);
+    return __builtin_bit_cast(s16x4, v);
+}
+
 __device__ __forceinline__ bf16x8 as_bf8(const uint4& v) { return __builtin_bit_cast(bf16x8, v); }
 
-__device__ __forceinline__ void st4(uint16_t* p, float a, float b, float c, float d) {
-    uint2 v;
-    v.x = (uint32_t)f2bf_bits(a) | ((uint32_t)f2bf_bits(b) << 16);
-    v.y = (uint32_t)f2bf_bits(c) | ((uint32_t)f2bf_bits(d) << 16);
-    *reinterpret_cast<uint2*>(p) = v;
+// x = hi + lo with hi = bf16(x) and lo = bf16(x - hi): the pair carries about
+// 16 significant bits, so products of a bf16 weight with (hi, lo) keep the
+// activation's precision.
+__device__ __forceinline__ void split_bf(float x, uint16_t& h, uint16_t& l) {
+    h = f2bf_bits(x);
+    l = f2bf_bits(x - bf_bits2f(h));
 }
+
+__device__ __forceinline__ void st4s(uint16_t* ph, uint16_t* pl, const float* v) {
+    uint16_t h[4], l[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) split_bf(v[i], h[i], l[i]);
+    *reinterpret_cast<uint2*>(ph) = make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16),
+                                               (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+    *reinterpret_cast<uint2*>(pl) = make_uint2((uint32_t)l[0] | ((uint32_t)l[1] << 16),
+                                               (uint32_t)l[2] | ((uint32_t)l[3] << 16));
+}
+
+// a (bf16 weight) times a split activation
+__device__ __forceinline__ f32x4 mfma_w(const bf16x8& a, const bf16x8& bh, const bf16x8& bl,
+                                        f32x4 c) {
+    c = mfma16x16x32(a, bh, c);
+    return mfma16x16x32(a, bl, c);
+}
+
+// dW products (activation x gradient) use the hi parts only: the sums over a
+// step's samples average that rounding (a device check keeps every MLP
+// gradient within 5e-4 of FP64), so the gradients' lo parts stay sample-major.
 
 struct RecInfo {
     float x, y, z, e;
@@ -63,35 +102,37 @@ struct RecInfo {
 };
 
 struct alignas(16) SmemFwd {
-    uint16_t Xs[GS * XM];
-    uint16_t H0s[GS * SM];
-    uint16_t H1s[GS * SM];
+    uint16_t Xh[GS * XM], Xl[GS * XM];
+    uint16_t A1h[GS * SM], A1l[GS * SM];  // H0
+    uint16_t A2h[GS * SM], A2l[GS * SM];  // H1
     RecInfo ri[GS];
 };
 
-// Backward weights as MFMA A fragments, [tile][k-step][lane] x 16 bytes:
-//   W2  [8 row tiles of 16 hidden][1][64]   (K = 3 logits, padded to 32)
-//   W1  [8 row tiles][4][64]                (K = 128 hidden outputs)
-//   W0  [1 (12 feature rows)][4 k-steps of 32 hidden][64]
+// Backward weights as MFMA A fragments, [tile][k-step][lane]:
+//   W2  [8 row tiles of 16 hidden][64] x 8 bytes   (K = 16 logits, 3 used)
+//   W1  [8 row tiles][4 k-steps][64] x 16 bytes    (K = 128 hidden outputs)
+//   W0  [4 k-steps of 32 hidden][64] x 16 bytes    (rows: the 12 features)
 constexpr int kBw2Frags = 8 * 64, kBw1Frags = 8 * 4 * 64, kBw0Frags = 4 * 64;
 
+// Sample-major activations are split (hi, lo) bf16 pairs (B operands of the
+// weight products); feature-major copies (dW operands) keep the hi part.  A1
+// holds H0 (steps 3-4) and then g_H1 (7-8); A2 holds H1 (4-5) and then g_H0
+// (8-9); F2 holds H1 feature-major (4-7) and then g_H0 (8-9).  Barriers
+// separate every reuse.
 struct alignas(16) SmemBwd {
-    uint4 Bw2[kBw2Frags];
     uint4 Bw1[kBw1Frags];
     uint4 Bw0[kBw0Frags];
-    uint16_t Xs[GS * XM];
-    uint16_t Xf[64 * FM];
-    uint16_t H0s[GS * SM];
+    uint2 Bw2[kBw2Frags];
+    uint16_t Xh[GS * XM], Xl[GS * XM];
+    uint16_t A1h[GS * SM], A1l[GS * SM];
+    uint16_t A2h[GS * SM], A2l[GS * SM];
+    uint16_t GLh[GS * GM], GLl[GS * GM];
+    uint16_t Xf[XF_ROWS * FM];
     uint16_t H0f[128 * FM];
-    uint16_t H1s[GS * SM];
-    uint16_t H1f[128 * FM];
-    uint16_t GLs[GS * GM];
+    uint16_t F2[128 * FM];
     uint16_t GLf[16 * FM];
-    uint16_t G1s[GS * SM];
     uint16_t G1f[128 * FM];
-    uint16_t G0s[GS * SM];
-    uint16_t G0f[128 * FM];
-    float GXp[4][16][GS];
+    float GX[16][GS];
     float gsig[GS];
     float rgb[GS][4];
     RecInfo ri[GS];
@@ -99,6 +140,12 @@ struct alignas(16) SmemBwd {
     float4 ctot[GS];
     float4 cinc[GS];
     float tlast[GS];
+    // the next group's ray data and kept features, parked after step 7
+    float4 nrayg[GS];
+    float4 nctot[GS];
+    float4 ncinc[GS];
+    uint4 nfeat[GS][3];
+    float ntlast[GS];
 };
 
 __device__ __forceinline__ float sigmoidf_(float x) { return 1.f / (1.f + expf(-x)); }
@@ -108,7 +155,7 @@ struct Prefetch {
     float4 xyze, atwr;          // stage A: the record
     float4 rayg, ctot, cinc;    // stage B (backward): ray data, prefix
     float tlast;
-    uint4 f0, f1;               // stage B (backward): kept features
+    uint4 f0, f1, f2;           // stage B (backward): kept features (hi 12, lo 12)
 };
 
 template <bool BWD, bool BF16COL>
@@ -121,14 +168,14 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
     const int wv = tid >> 6, lane = tid & 63, r16 = lane & 15, g = lane >> 4;
     const int f0 = 32 * wv;  // first hidden feature owned by this wave
 
-    // ---- forward weight slices -> registers (bf16 fragments) ----
-    bf16x8 A0[2][2], A1[2][4], A2[4];
+    // ---- forward weight slices: W1^T in registers; W0^T and W2^T (used by one
+    // step each) are re-read from L2 per group, which keeps the backward's
+    // registers for its dW accumulators ----
+    bf16x8 A1[2][4];
     float b0v[2][4], b1v[2][4], b2v[3];
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
         const int row = f0 + 16 * mt + r16;
-#pragma unroll
-        for (int kt = 0; kt < 2; ++kt) A0[mt][kt] = ld_frag(W.w0t + row * 64 + 32 * kt + 8 * g);
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) A1[mt][kt] = ld_frag(W.w1t + row * 128 + 32 * kt + 8 * g);
 #pragma unroll
@@ -137,35 +184,30 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
             b1v[mt][i] = W.b1[f0 + 16 * mt + 4 * g + i];
         }
     }
-#pragma unroll
-    for (int kt = 0; kt < 4; ++kt) A2[kt] = ld_frag(W.w2t + r16 * 128 + 32 * kt + 8 * g);
     b2v[0] = W.b2[0];
     b2v[1] = W.b2[1];
     b2v[2] = W.b2[2];
 
     // ---- backward weights -> LDS, dW accumulators ----
-    f32x4 dW0[4][2], dW1[8][2], dW2[2];
+    f32x4 dW0[3][2], dW1[8][2], dW2[2];
     float db1[2][4], db2[3];
     if constexpr (BWD) {
         for (int f = tid; f < kBw2Frags + kBw1Frags + kBw0Frags; f += 256) {
             const int ln = f & 63, gg = ln >> 4, rr = ln & 15;
-            uint4 v;
-            if (f < kBw2Frags) {  // W2 [128 in][32 out (3 used)]
+            if (f < kBw2Frags) {  // W2 [128 in][32 out (3 used)], K = 16
                 const int t = f >> 6;
-                v = *reinterpret_cast<const uint4*>(W.w2 + (16 * t + rr) * 32 + 8 * gg);
-                sm.Bw2[f] = v;
+                sm.Bw2[f] = *reinterpret_cast<const uint2*>(W.w2 + (16 * t + rr) * 32 + 4 * gg);
             } else if (f < kBw2Frags + kBw1Frags) {  // W1 [128 in][128 out]
                 const int q = f - kBw2Frags, t = q >> 8, kt = (q >> 6) & 3;
-                v = *reinterpret_cast<const uint4*>(W.w1 + (16 * t + rr) * 128 + 32 * kt + 8 * gg);
-                sm.Bw1[q] = v;
+                sm.Bw1[q] =
+                    *reinterpret_cast<const uint4*>(W.w1 + (16 * t + rr) * 128 + 32 * kt + 8 * gg);
             } else {  // W0 rows 0..15 [16][128]
                 const int q = f - kBw2Frags - kBw1Frags, kt = q >> 6;
-                v = *reinterpret_cast<const uint4*>(W.w0 + rr * 128 + 32 * kt + 8 * gg);
-                sm.Bw0[q] = v;
+                sm.Bw0[q] = *reinterpret_cast<const uint4*>(W.w0 + rr * 128 + 32 * kt + 8 * gg);
             }
         }
 #pragma unroll
-        for (int a = 0; a < 4; ++a)
+        for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int b = 0; b < 2; ++b) dW0[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -183,12 +225,13 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
     const int64_t total = min((int64_t)R.total[0], P.capacity);
     const int64_t ngroups = (total + GS - 1) / GS;
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint4 zu4 = make_uint4(0u, 0u, 0u, 0u);
 
     // ---- prefetch of the first group ----
     Prefetch pf;
     pf.xyze = pf.atwr = pf.rayg = pf.ctot = pf.cinc = z4;
     pf.tlast = 0.f;
-    pf.f0 = pf.f1 = make_uint4(0u, 0u, 0u, 0u);
+    pf.f0 = pf.f1 = pf.f2 = zu4;
     auto stage_a = [&](int64_t grp) {
         const int64_t q = grp * GS + tid;
         pf.xyze = pf.atwr = z4;
@@ -201,7 +244,7 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
         const int64_t q = grp * GS + tid;
         pf.rayg = pf.ctot = pf.cinc = z4;
         pf.tlast = 0.f;
-        pf.f0 = pf.f1 = make_uint4(0u, 0u, 0u, 0u);
+        pf.f0 = pf.f1 = pf.f2 = zu4;
         if (tid < GS && grp < ngroups && q < total) {
             const uint32_t bits = __float_as_uint(pf.atwr.w);
             if (bits & (kAliveA | kAliveW)) {
@@ -212,13 +255,31 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                 pf.cinc = rec.cinc[q];
             }
             if (bits & kAliveW) {
-                pf.f0 = rec.feat[2 * q];
-                pf.f1 = rec.feat[2 * q + 1];
+                pf.f0 = rec.feat[3 * q];
+                pf.f1 = rec.feat[3 * q + 1];
+                pf.f2 = rec.feat[3 * q + 2];
+            }
+        }
+    };
+    auto park = [&]() {  // backward: stage B registers -> LDS next-buffers
+        if constexpr (BWD) {
+            if (tid < GS) {
+                sm.nrayg[tid] = pf.rayg;
+                sm.nctot[tid] = pf.ctot;
+                sm.ncinc[tid] = pf.cinc;
+                sm.ntlast[tid] = pf.tlast;
+                sm.nfeat[tid][0] = pf.f0;
+                sm.nfeat[tid][1] = pf.f1;
+                sm.nfeat[tid][2] = pf.f2;
             }
         }
     };
     stage_a(blockIdx.x);
-    if constexpr (BWD) stage_b(blockIdx.x);
+    if constexpr (BWD) {
+        stage_b(blockIdx.x);
+        park();
+        __syncthreads();
+    }
 
     for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {  // uniform bound
         const int64_t base = grp * GS;
@@ -235,22 +296,33 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
             ri.bits = __float_as_uint(pf.atwr.w);  // zero for records past the end
             sm.ri[tid] = ri;
             if constexpr (BWD) {
-                sm.rayg[tid] = pf.rayg;
-                sm.ctot[tid] = pf.ctot;
-                sm.cinc[tid] = pf.cinc;
-                sm.tlast[tid] = pf.tlast;
-                const uint32_t w6[6] = {pf.f0.x, pf.f0.y, pf.f0.z, pf.f0.w, pf.f1.x, pf.f1.y};
-                uint32_t* xs = reinterpret_cast<uint32_t*>(&sm.Xs[tid * XM]);
+                sm.rayg[tid] = sm.nrayg[tid];
+                sm.ctot[tid] = sm.nctot[tid];
+                sm.cinc[tid] = sm.ncinc[tid];
+                sm.tlast[tid] = sm.ntlast[tid];
+                // kept features: hi halves 0..11 then lo halves 0..11
+                const uint4 f0 = sm.nfeat[tid][0], f1 = sm.nfeat[tid][1], f2 = sm.nfeat[tid][2];
+                const uint32_t w12[12] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y,
+                                          f1.z, f1.w, f2.x, f2.y, f2.z, f2.w};
+                uint32_t* xh = reinterpret_cast<uint32_t*>(&sm.Xh[tid * XM]);
+                uint32_t* xl = reinterpret_cast<uint32_t*>(&sm.Xl[tid * XM]);
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
-                    xs[c] = w6[c];
-                    sm.Xf[(2 * c) * FM + tid] = (uint16_t)(w6[c] & 0xffffu);
-                    sm.Xf[(2 * c + 1) * FM + tid] = (uint16_t)(w6[c] >> 16);
+                    xh[c] = w12[c];
+                    xl[c] = w12[6 + c];
+                    sm.Xf[(2 * c) * FM + tid] = (uint16_t)(w12[c] & 0xffffu);
+                    sm.Xf[(2 * c + 1) * FM + tid] = (uint16_t)(w12[c] >> 16);
                 }
             }
         }
         const int64_t next = grp + gridDim.x;
         stage_a(next);
+        bf16x8 A0[2][2];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt)
+                A0[mt][kt] = ld_frag(W.w0t + (f0 + 16 * mt + r16) * 64 + 32 * kt + 8 * g);
         __syncthreads();
         // ---- 2. X = [colour features (12) | PE (27) | 1 | 0...] ----
         {
@@ -300,18 +372,27 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                     v[c] += __shfl_xor(v[c], 4, 64);
                 }
                 if (part == 0) {
-                    uint32_t w6[6];
+                    uint32_t wh[6], wl[6];
 #pragma unroll
-                    for (int c = 0; c < 6; ++c)
-                        w6[c] = (uint32_t)f2bf_bits(v[2 * c]) |
-                                ((uint32_t)f2bf_bits(v[2 * c + 1]) << 16);
-                    uint32_t* xs = reinterpret_cast<uint32_t*>(&sm.Xs[s * XM]);
+                    for (int c = 0; c < 6; ++c) {
+                        uint16_t h0, l0, h1, l1;
+                        split_bf(v[2 * c], h0, l0);
+                        split_bf(v[2 * c + 1], h1, l1);
+                        wh[c] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+                        wl[c] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+                    }
+                    uint32_t* xh = reinterpret_cast<uint32_t*>(&sm.Xh[s * XM]);
+                    uint32_t* xl = reinterpret_cast<uint32_t*>(&sm.Xl[s * XM]);
 #pragma unroll
-                    for (int c = 0; c < 6; ++c) xs[c] = w6[c];
+                    for (int c = 0; c < 6; ++c) {
+                        xh[c] = wh[c];
+                        xl[c] = wl[c];
+                    }
                     const int64_t qq = base + s;
                     if (qq < total && alive) {
-                        rec.feat[2 * qq] = make_uint4(w6[0], w6[1], w6[2], w6[3]);
-                        rec.feat[2 * qq + 1] = make_uint4(w6[4], w6[5], 0u, 0u);
+                        rec.feat[3 * qq] = make_uint4(wh[0], wh[1], wh[2], wh[3]);
+                        rec.feat[3 * qq + 1] = make_uint4(wh[4], wh[5], wl[0], wl[1]);
+                        rec.feat[3 * qq + 2] = make_uint4(wl[2], wl[3], wl[4], wl[5]);
                     }
                 }
             }
@@ -340,20 +421,25 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                             val = 1.f;
                         }
                     }
-                    const uint16_t h = f2bf_bits(val);
-                    sm.Xs[s * XM + 12 + p] = h;
+                    uint16_t h, l;
+                    split_bf(val, h, l);
+                    sm.Xh[s * XM + 12 + p] = h;
+                    sm.Xl[s * XM + 12 + p] = l;
                     if constexpr (BWD) sm.Xf[(12 + p) * FM + s] = h;
                 }
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const int col = 40 + part * 3 + k;
-                sm.Xs[s * XM + col] = 0;
-                if constexpr (BWD) sm.Xf[col * FM + s] = 0;
+                sm.Xh[s * XM + col] = 0;
+                sm.Xl[s * XM + col] = 0;
+                if constexpr (BWD) {
+                    if (col < XF_ROWS) sm.Xf[col * FM + s] = 0;
+                }
             }
         }
         __syncthreads();
-        // ---- 3. H0^T slice = relu(W0^T X^T + b0) ----
+        // ---- 3. H0^T slice = relu(W0^T X^T + b0) -> A1 (and H0f) ----
         uint32_t m0 = 0u, m1 = 0u;  // ReLU masks: bit (mt*NT + nt)*4 + i
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
@@ -361,47 +447,47 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
                 f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                const int xo = (16 * nt + r16) * XM + 8 * g;
 #pragma unroll
                 for (int kt = 0; kt < 2; ++kt)
-                    acc = mfma16x16x32(A0[mt][kt],
-                                       ld_frag(&sm.Xs[(16 * nt + r16) * XM + 32 * kt + 8 * g]), acc);
+                    acc = mfma_w(A0[mt][kt], ld_frag(&sm.Xh[xo + 32 * kt]),
+                                 ld_frag(&sm.Xl[xo + 32 * kt]), acc);
                 float h[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     h[i] = fmaxf(acc[i] + b0v[mt][i], 0.f);
                     if (h[i] > 0.f) m0 |= 1u << ((mt * NT + nt) * 4 + i);
                 }
-                st4(&sm.H0s[(16 * nt + r16) * SM + fr], h[0], h[1], h[2], h[3]);
+                st4s(&sm.A1h[(16 * nt + r16) * SM + fr], &sm.A1l[(16 * nt + r16) * SM + fr], h);
                 if constexpr (BWD) {
 #pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        sm.H0f[(fr + i) * FM + 16 * nt + r16] = f2bf_bits(h[i]);
+                    for (int i = 0; i < 4; ++i) sm.H0f[(fr + i) * FM + 16 * nt + r16] = f2bf_bits(h[i]);
                 }
             }
         }
         __syncthreads();
-        // ---- 4. H1^T slice = relu(W1^T H0^T + b1) ----
+        // ---- 4. H1^T slice = relu(W1^T H0^T + b1) -> A2 (and H1f in F2) ----
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
             const int fr = f0 + 16 * mt + 4 * g;
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
                 f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                const int ho = (16 * nt + r16) * SM + 8 * g;
 #pragma unroll
                 for (int kt = 0; kt < 4; ++kt)
-                    acc = mfma16x16x32(A1[mt][kt],
-                                       ld_frag(&sm.H0s[(16 * nt + r16) * SM + 32 * kt + 8 * g]), acc);
+                    acc = mfma_w(A1[mt][kt], ld_frag(&sm.A1h[ho + 32 * kt]),
+                                 ld_frag(&sm.A1l[ho + 32 * kt]), acc);
                 float h[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     h[i] = fmaxf(acc[i] + b1v[mt][i], 0.f);
                     if (h[i] > 0.f) m1 |= 1u << ((mt * NT + nt) * 4 + i);
                 }
-                st4(&sm.H1s[(16 * nt + r16) * SM + fr], h[0], h[1], h[2], h[3]);
+                st4s(&sm.A2h[(16 * nt + r16) * SM + fr], &sm.A2l[(16 * nt + r16) * SM + fr], h);
                 if constexpr (BWD) {
 #pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        sm.H1f[(fr + i) * FM + 16 * nt + r16] = f2bf_bits(h[i]);
+                    for (int i = 0; i < 4; ++i) sm.F2[(fr + i) * FM + 16 * nt + r16] = f2bf_bits(h[i]);
                 }
             }
         }
@@ -410,10 +496,11 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
         if (wv < NT) {
             const int nt = wv;
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            const int ho = (16 * nt + r16) * SM + 8 * g;
 #pragma unroll
             for (int kt = 0; kt < 4; ++kt)
-                acc = mfma16x16x32(A2[kt], ld_frag(&sm.H1s[(16 * nt + r16) * SM + 32 * kt + 8 * g]),
-                                   acc);
+                acc = mfma_w(ld_frag(W.w2t + r16 * 128 + 32 * kt + 8 * g),
+                             ld_frag(&sm.A2h[ho + 32 * kt]), ld_frag(&sm.A2l[ho + 32 * kt]), acc);
             if (g == 0) {
                 const int s = 16 * nt + r16;
                 const float c0 = sigmoidf_(acc[0] + b2v[0]);
@@ -440,15 +527,17 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
             stage_b(next);
             __syncthreads();
             // ---- 6. per-sample adjoints: dL/dlogit, alpha, sigma ----
+            for (int e = tid; e < 16 * GS; e += 256) (&sm.GX[0][0])[e] = 0.f;
             if (tid < GS) {
                 const RecInfo ri = sm.ri[tid];
                 const float4 gr = sm.rayg[tid];
-                float gl0 = 0.f, gl1 = 0.f, gl2 = 0.f, gw = 0.f;
+                float gl[3] = {0.f, 0.f, 0.f};
+                float gw = 0.f;
                 if (ri.bits & kAliveW) {
                     const float c0 = sm.rgb[tid][0], c1 = sm.rgb[tid][1], c2 = sm.rgb[tid][2];
-                    gl0 = ri.w * gr.x * c0 * (1.f - c0);
-                    gl1 = ri.w * gr.y * c1 * (1.f - c1);
-                    gl2 = ri.w * gr.z * c2 * (1.f - c2);
+                    gl[0] = ri.w * gr.x * c0 * (1.f - c0);
+                    gl[1] = ri.w * gr.y * c1 * (1.f - c1);
+                    gl[2] = ri.w * gr.z * c2 * (1.f - c2);
                     gw = gr.x * c0 + gr.y * c1 + gr.z * c2;
                 }
                 float gs = 0.f;
@@ -460,47 +549,48 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                     gs = fminf(ri.e, 1e10f) * powf(1.f + ri.e, -P.interval - 1.f) * P.interval * ga;
                 }
                 sm.gsig[tid] = gs;
-                db2[0] += gl0;
-                db2[1] += gl1;
-                db2[2] += gl2;
-                uint32_t* gls = reinterpret_cast<uint32_t*>(&sm.GLs[tid * GM]);
-                gls[0] = (uint32_t)f2bf_bits(gl0) | ((uint32_t)f2bf_bits(gl1) << 16);
-                gls[1] = (uint32_t)f2bf_bits(gl2);
-#pragma unroll
-                for (int c = 2; c < 16; ++c) gls[c] = 0u;
+                db2[0] += gl[0];
+                db2[1] += gl[1];
+                db2[2] += gl[2];
 #pragma unroll
                 for (int o = 0; o < 16; ++o) {
-                    const float v = o == 0 ? gl0 : o == 1 ? gl1 : o == 2 ? gl2 : 0.f;
-                    sm.GLf[o * FM + tid] = f2bf_bits(v);
+                    uint16_t h = 0, l = 0;
+                    if (o < 3) split_bf(gl[o], h, l);
+                    sm.GLh[tid * GM + o] = h;
+                    sm.GLl[tid * GM + o] = l;
+                    sm.GLf[o * FM + tid] = h;
                 }
             }
             __syncthreads();
-            // ---- 7. g_H1^T slice = W2[slice] g_L^T, masked; dW2 slice ----
+            // ---- 7. g_H1^T slice = W2[slice] g_L^T, masked -> A1 (and G1f); dW2 ----
 #pragma unroll
             for (int mt = 0; mt < 2; ++mt) {
-                const bf16x8 a = as_bf8(sm.Bw2[(wv * 2 + mt) * 64 + lane]);
+                const s16x4 a = __builtin_bit_cast(s16x4, sm.Bw2[(wv * 2 + mt) * 64 + lane]);
                 const int fr = f0 + 16 * mt + 4 * g;
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) {
                     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-                    acc = mfma16x16x32(a, ld_frag(&sm.GLs[(16 * nt + r16) * GM + 8 * g]), acc);
+                    const int go = (16 * nt + r16) * GM + 4 * g;
+                    acc = mfma16x16x16(a, ld_frag4(&sm.GLh[go]), acc);
+                    acc = mfma16x16x16(a, ld_frag4(&sm.GLl[go]), acc);
                     float gv[4];
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
                         gv[i] = (m1 >> ((mt * NT + nt) * 4 + i)) & 1u ? acc[i] : 0.f;
                         db1[mt][i] += gv[i];
                     }
-                    st4(&sm.G1s[(16 * nt + r16) * SM + fr], gv[0], gv[1], gv[2], gv[3]);
+                    st4s(&sm.A1h[(16 * nt + r16) * SM + fr], &sm.A1l[(16 * nt + r16) * SM + fr], gv);
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
                         sm.G1f[(fr + i) * FM + 16 * nt + r16] = f2bf_bits(gv[i]);
                 }
                 // dW2[in = f0+16mt+row][out] += sum over the group's samples H1[s][in] gL[s][out]
-                dW2[mt] = mfma16x16x32(ld_frag(&sm.H1f[(f0 + 16 * mt + r16) * FM + 8 * g]),
-                                       ld_frag(&sm.GLf[r16 * FM + 8 * g]), dW2[mt]);
+                const int ho = (f0 + 16 * mt + r16) * FM + 8 * g, lo_ = r16 * FM + 8 * g;
+                dW2[mt] = mfma16x16x32(ld_frag8(&sm.F2[ho]), ld_frag8(&sm.GLf[lo_]), dW2[mt]);
             }
+            park();  // the next group's stage-B loads (issued in step 5) have landed
             __syncthreads();
-            // ---- 8. g_H0^T slice = W1[slice] g_H1^T, masked; dW1 columns of this wave ----
+            // ---- 8. g_H0^T slice = W1[slice] g_H1^T, masked -> A2 (and G0f in F2); dW1 ----
 #pragma unroll
             for (int mt = 0; mt < 2; ++mt) {
                 const int fr = f0 + 16 * mt + 4 * g;
@@ -511,9 +601,11 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                 for (int kt = 0; kt < 4; ++kt) {
                     const bf16x8 a = as_bf8(sm.Bw1[((wv * 2 + mt) * 4 + kt) * 64 + lane]);
 #pragma unroll
-                    for (int nt = 0; nt < NT; ++nt)
-                        acc[nt] = mfma16x16x32(
-                            a, ld_frag(&sm.G1s[(16 * nt + r16) * SM + 32 * kt + 8 * g]), acc[nt]);
+                    for (int nt = 0; nt < NT; ++nt) {
+                        const int go = (16 * nt + r16) * SM + 32 * kt + 8 * g;
+                        acc[nt] = mfma_w(a, ld_frag(&sm.A1h[go]), ld_frag(&sm.A1l[go]), acc[nt]);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
                 }
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) {
@@ -521,36 +613,43 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
                         gv[i] = (m0 >> ((mt * NT + nt) * 4 + i)) & 1u ? acc[nt][i] : 0.f;
-                    st4(&sm.G0s[(16 * nt + r16) * SM + fr], gv[0], gv[1], gv[2], gv[3]);
+                    st4s(&sm.A2h[(16 * nt + r16) * SM + fr], &sm.A2l[(16 * nt + r16) * SM + fr], gv);
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        sm.G0f[(fr + i) * FM + 16 * nt + r16] = f2bf_bits(gv[i]);
+                        sm.F2[(fr + i) * FM + 16 * nt + r16] = f2bf_bits(gv[i]);
                 }
             }
-#pragma unroll
-            for (int mi = 0; mi < 8; ++mi)
-#pragma unroll
-                for (int nt = 0; nt < 2; ++nt)
-                    dW1[mi][nt] = mfma16x16x32(ld_frag(&sm.H0f[(16 * mi + r16) * FM + 8 * g]),
-                                               ld_frag(&sm.G1f[(f0 + 16 * nt + r16) * FM + 8 * g]),
-                                               dW1[mi][nt]);
-            __syncthreads();
-            // ---- 9. dW0 columns of this wave; partial g_X^T over this wave's K slice ----
-#pragma unroll
-            for (int mi = 0; mi < 4; ++mi)
-#pragma unroll
-                for (int nt = 0; nt < 2; ++nt)
-                    dW0[mi][nt] = mfma16x16x32(ld_frag(&sm.Xf[(16 * mi + r16) * FM + 8 * g]),
-                                               ld_frag(&sm.G0f[(f0 + 16 * nt + r16) * FM + 8 * g]),
-                                               dW0[mi][nt]);
             {
+                bf16x8 bh[2];
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt) bh[nt] = ld_frag8(&sm.G1f[(f0 + 16 * nt + r16) * FM + 8 * g]);
+#pragma unroll
+                for (int mi = 0; mi < 8; ++mi) {
+                    const bf16x8 ah = ld_frag8(&sm.H0f[(16 * mi + r16) * FM + 8 * g]);
+#pragma unroll
+                    for (int nt = 0; nt < 2; ++nt) dW1[mi][nt] = mfma16x16x32(ah, bh[nt], dW1[mi][nt]);
+                }
+            }
+            __syncthreads();
+            // ---- 9. dW0 columns of this wave; g_X^T over this wave's K slice ----
+            {
+                bf16x8 bh[2];
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt) bh[nt] = ld_frag8(&sm.F2[(f0 + 16 * nt + r16) * FM + 8 * g]);
+#pragma unroll
+                for (int mi = 0; mi < 3; ++mi) {
+                    const bf16x8 ah = ld_frag8(&sm.Xf[(16 * mi + r16) * FM + 8 * g]);
+#pragma unroll
+                    for (int nt = 0; nt < 2; ++nt) dW0[mi][nt] = mfma16x16x32(ah, bh[nt], dW0[mi][nt]);
+                }
                 const bf16x8 a = as_bf8(sm.Bw0[wv * 64 + lane]);
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) {
+                    const int go = (16 * nt + r16) * SM + f0 + 8 * g;
                     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-                    acc = mfma16x16x32(a, ld_frag(&sm.G0s[(16 * nt + r16) * SM + f0 + 8 * g]), acc);
+                    acc = mfma_w(a, ld_frag(&sm.A2h[go]), ld_frag(&sm.A2l[go]), acc);
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) sm.GXp[wv][4 * g + i][16 * nt + r16] = acc[i];
+                    for (int i = 0; i < 4; ++i) atomicAdd(&sm.GX[4 * g + i][16 * nt + r16], acc[i]);
                 }
             }
             __syncthreads();
@@ -562,16 +661,9 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                     const RecInfo ri = sm.ri[s];
                     float4 v = z4;
                     if (part < 3) {
-                        if (ri.bits & kAliveW) {
-                            float g4[4];
-#pragma unroll
-                            for (int k = 0; k < 4; ++k) {
-                                const int c = 4 * part + k;
-                                g4[k] = sm.GXp[0][c][s] + sm.GXp[1][c][s] + sm.GXp[2][c][s] +
-                                        sm.GXp[3][c][s];
-                            }
-                            v = make_float4(g4[0], g4[1], g4[2], g4[3]);
-                        }
+                        if (ri.bits & kAliveW)
+                            v = make_float4(sm.GX[4 * part][s], sm.GX[4 * part + 1][s],
+                                            sm.GX[4 * part + 2][s], sm.GX[4 * part + 3][s]);
                     } else {
                         v.x = sm.gsig[s];
                     }
@@ -586,7 +678,7 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
     // ---- flush this workgroup's dW partial ----
     float* out = dw_partial + (int64_t)blockIdx.x * kMlpParams;
 #pragma unroll
-    for (int mi = 0; mi < 4; ++mi)
+    for (int mi = 0; mi < 3; ++mi)
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt)
 #pragma unroll
@@ -641,49 +733,6 @@ __global__ void k_dw_reduce(const float* __restrict__ part, int nparts, float* _
     float s = 0.f;
     for (int b = b0; b < b1; ++b) s += part[(int64_t)b * kMlpParams + p];
     atomicAdd(dw + p, s);
-}
-
-// Gradients into the grids: one thread per (record, corner), with the
-// "leaf must exist" rule (densityvdb.cu:16-27, colorvdb.cu:28-37); the density
-// term keeps the reference's ((g*a)*b)*c order.  Touched leaves are flagged
-// for the sparse Adam pass.
-__global__ void __launch_bounds__(256) k_scatter(TopoView topo, RayBuffers R, Records rec,
-                                                 int64_t capacity, float* __restrict__ grad_density,
-                                                 float* __restrict__ grad_color,
-                                                 uint8_t* __restrict__ leaf_touched) {
-    const int64_t total = min((int64_t)R.total[0], capacity);
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total * 8;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t q = t >> 3;
-        const int corner = (int)(t & 7);
-        const uint32_t bits = __float_as_uint(rec.atwr[q].w);
-        if (!(bits & (kAliveA | kAliveW))) continue;
-        const float4 p = rec.xyze[q];
-        const Cell cell = make_cell(p.x, p.y, p.z);
-        const int dx = (kCx >> corner) & 1, dy = (kCy >> corner) & 1, dz = (kCz >> corner) & 1;
-        const int i = cell.i0 + dx, j = cell.j0 + dy, k = cell.k0 + dz;
-        const int32_t leaf = leaf_of(topo, i, j, k);
-        if (leaf < 0) continue;
-        const float a = dx ? cell.u : 1.f - cell.u;
-        const float b = dy ? cell.v : 1.f - cell.v;
-        const float d = dz ? cell.w : 1.f - cell.w;
-        const int64_t slot = (int64_t)leaf * 512 + voxel_offset(i, j, k);
-        leaf_touched[leaf] = 1;
-        const float4* g = rec.grad + 4 * q;
-        if (bits & kAliveA) atomicAdd(grad_density + slot, ((g[3].x * a) * b) * d);
-        if (bits & kAliveW) {
-            const float sc = (a * b) * d;
-            float* dst = grad_color + slot * 12;
-#pragma unroll
-            for (int part = 0; part < 3; ++part) {
-                const float4 v = g[part];
-                atomicAdd(dst + 4 * part + 0, v.x * sc);
-                atomicAdd(dst + 4 * part + 1, v.y * sc);
-                atomicAdd(dst + 4 * part + 2, v.z * sc);
-                atomicAdd(dst + 4 * part + 3, v.w * sc);
-            }
-        }
-    }
 }
 
 // fp32 master parameters (kMlpParams layout) -> packed bf16 MFMA operands.
@@ -741,110 +790,11 @@ int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const Topo
     return PVDB_OK;
 }
 
-// Same result as k_scatter with far fewer global atomics: a workgroup takes
-// kAggRecords consecutive records (a few rays; consecutive samples of a ray
-// share most of their corners), sums their (slot, gradient) contributions in
-// an LDS hash table keyed by the 32-bit slot, then adds each distinct slot's
-// 13 sums to the grids once.  The flush gives 16 lanes to a slot, so one
-// atomic instruction covers four slots' contiguous 12-float colour rows.
-constexpr int kAggRecords = 128;
-constexpr int kAggSlots = 8 * kAggRecords;  // every contribution fits: probing ends
-constexpr uint32_t kEmptyKey = 0xffffffffu;
-__global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R, Records rec,
-                                                     int64_t capacity,
-                                                     float* __restrict__ grad_density,
-                                                     float* __restrict__ grad_color,
-                                                     uint8_t* __restrict__ leaf_touched) {
-    __shared__ uint32_t key[kAggSlots];
-    __shared__ float val[13][kAggSlots];
-    const int tid = threadIdx.x;
-    const int64_t total = min((int64_t)R.total[0], capacity);
-    const int64_t nchunk = (total + kAggRecords - 1) / kAggRecords;
-    for (int64_t chunk = blockIdx.x; chunk < nchunk; chunk += gridDim.x) {
-        for (int e = tid; e < kAggSlots; e += 256) {
-            key[e] = kEmptyKey;
-#pragma unroll
-            for (int c = 0; c < 13; ++c) val[c][e] = 0.f;
-        }
-        __syncthreads();
-        for (int p = tid; p < 8 * kAggRecords; p += 256) {
-            const int64_t q = chunk * kAggRecords + (p >> 3);
-            if (q >= total) continue;
-            const int corner = p & 7;
-            const uint32_t bits = __float_as_uint(rec.atwr[q].w);
-            if (!(bits & (kAliveA | kAliveW))) continue;
-            const float4 pt = rec.xyze[q];
-            const Cell cell = make_cell(pt.x, pt.y, pt.z);
-            const int dx = (kCx >> corner) & 1, dy = (kCy >> corner) & 1, dz = (kCz >> corner) & 1;
-            const int i = cell.i0 + dx, j = cell.j0 + dy, k = cell.k0 + dz;
-            const int32_t leaf = leaf_of(topo, i, j, k);
-            if (leaf < 0) continue;
-            const uint32_t slot = (uint32_t)leaf * 512u + (uint32_t)voxel_offset(i, j, k);
-            const float a = dx ? cell.u : 1.f - cell.u;
-            const float b = dy ? cell.v : 1.f - cell.v;
-            const float d = dz ? cell.w : 1.f - cell.w;
-            uint32_t h = (slot * 2654435761u) >> (32 - 10);  // kAggSlots == 1 << 10
-            while (true) {
-                const uint32_t prev = atomicCAS(&key[h], kEmptyKey, slot);
-                if (prev == kEmptyKey || prev == slot) break;
-                h = (h + 1) & (kAggSlots - 1);
-            }
-            const float4* g = rec.grad + 4 * q;
-            if (bits & kAliveA) atomicAdd(&val[12][h], ((g[3].x * a) * b) * d);
-            if (bits & kAliveW) {
-                const float sc = (a * b) * d;
-#pragma unroll
-                for (int part = 0; part < 3; ++part) {
-                    const float4 v = g[part];
-                    atomicAdd(&val[4 * part + 0][h], v.x * sc);
-                    atomicAdd(&val[4 * part + 1][h], v.y * sc);
-                    atomicAdd(&val[4 * part + 2][h], v.z * sc);
-                    atomicAdd(&val[4 * part + 3][h], v.w * sc);
-                }
-            }
-        }
-        __syncthreads();
-        const int c = tid & 15;
-        for (int e = tid >> 4; e < kAggSlots; e += 16) {
-            const uint32_t slot = key[e];
-            if (slot == kEmptyKey) continue;
-            if (c == 13) leaf_touched[slot >> 9] = 1;
-            if (c > 12) continue;
-            const float v = val[c][e];
-            if (v == 0.f) continue;  // adding zero changes nothing
-            if (c < 12)
-                atomicAdd(grad_color + (int64_t)slot * 12 + c, v);
-            else
-                atomicAdd(grad_density + slot, v);
-        }
-        __syncthreads();
-    }
-}
-
 int launch_dw_reduce(const float* part, int nparts, float* dw, hipStream_t s) {
     const unsigned chunks = (unsigned)((nparts + kReduceChunk - 1) / kReduceChunk);
     hipLaunchKernelGGL(k_dw_reduce, dim3(pvdb_blocks(kMlpParams, 256), chunks), dim3(256), 0, s,
                        part, nparts, dw);
     PVDB_LAUNCHED("k_dw_reduce");
-    return PVDB_OK;
-}
-
-int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, int64_t capacity,
-                   float* grad_density, float* grad_color, uint8_t* leaf_touched, hipStream_t s) {
-    if (t.nleaf < (int64_t(1) << 23) - 1) {  // every slot < 2^32 - 1, the empty key
-        const int64_t chunks = (capacity + kAggRecords - 1) / kAggRecords;
-        const unsigned grid = (unsigned)(chunks < 2048 ? (chunks < 1 ? 1 : chunks) : 2048);
-        hipLaunchKernelGGL(k_scatter_agg, dim3(grid), dim3(256), 0, s, t, R, rec, capacity,
-                           grad_density, grad_color, leaf_touched);
-        PVDB_LAUNCHED("k_scatter_agg");
-        return PVDB_OK;
-    }
-    // enough threads for a few (record, corner) pairs each at the largest sizes
-    const int64_t want = (capacity * 8 + 255) / 256;
-    const unsigned grid = (unsigned)(want < 16384 ? (want < 1 ? 1 : want) : 16384);
-    hipLaunchKernelGGL(k_scatter, dim3(grid), dim3(256), 0, s, t, R, rec, capacity, grad_density,
-                       grad_color, leaf_touched);
-    PVDB_LAUNCHED("k_scatter");
     return PVDB_OK;
 }
 

@@ -24,8 +24,10 @@ int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const Topo
                const void* color, const MlpPacked& W, const RayBuffers& R, const Records& rec,
                float* dw_partial, hipStream_t s);
 int launch_dw_reduce(const float* part, int nparts, float* dw, hipStream_t s);
+int64_t scatter_workspace_bytes(int64_t nleaf, int64_t capacity);
 int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, int64_t capacity,
-                   float* grad_density, float* grad_color, uint8_t* leaf_touched, hipStream_t s);
+                   float* grad_density, float* grad_color, uint8_t* leaf_touched, void* ws,
+                   hipStream_t s);
 int launch_pack_mlp(const float* prm, uint16_t* w0t, uint16_t* w1t, uint16_t* w2t, uint16_t* w2,
                     uint16_t* w1, uint16_t* w0, hipStream_t s);
 }  // namespace pvdb
@@ -39,7 +41,7 @@ constexpr int kMaxMlpGrid = 512;
 struct Layout {
     size_t count, offset, total, t_last, ray_out, ray_grad, ray_ctot, ray_loss, loss;
     size_t xyze, atwr, rgb, cinc, feat, grad;
-    size_t touched, dw_part, w0t, w1t, w2t, w2, w1, w0;
+    size_t touched, dw_part, w0t, w1t, w2t, w2, w1, w0, scatter;
     size_t bytes;
 };
 
@@ -66,7 +68,7 @@ Layout layout_for(int64_t n_rays, int64_t capacity, int64_t nleaf) {
     L.atwr = take(16 * capacity);
     L.rgb = take(16 * capacity);
     L.cinc = take(16 * capacity);
-    L.feat = take(32 * capacity);
+    L.feat = take(48 * capacity);
     L.grad = take(64 * capacity);
     L.touched = take(nleaf > 0 ? nleaf : 1);
     L.dw_part = take((size_t)4 * kMlpParams * kMaxMlpGrid);
@@ -76,6 +78,7 @@ Layout layout_for(int64_t n_rays, int64_t capacity, int64_t nleaf) {
     L.w2 = take(2 * 128 * 32);
     L.w1 = take(2 * 128 * 128);
     L.w0 = take(2 * 16 * 128);
+    L.scatter = take((size_t)scatter_workspace_bytes(nleaf, capacity));
     L.bytes = o;
     return L;
 }
@@ -111,8 +114,12 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
                                                      float* __restrict__ vden, void* col,
                                                      float* __restrict__ gcol,
                                                      float* __restrict__ mcol,
-                                                     float* __restrict__ vcol, float sz_d,
-                                                     float sz_c, float eps, float b0, float b1) {
+                                                     float* __restrict__ vcol, float sz_d_arg,
+                                                     float sz_c_arg,
+                                                     const float* __restrict__ dev_sz, float eps,
+                                                     float b0, float b1) {
+    const float sz_d = dev_sz ? dev_sz[0] : sz_d_arg;
+    const float sz_c = dev_sz ? dev_sz[1] : sz_c_arg;
     for (int64_t leaf = blockIdx.x; leaf < t.nleaf; leaf += gridDim.x) {
         if (!touched[leaf]) continue;  // uniform over the workgroup
         for (int k = threadIdx.x; k < 512; k += blockDim.x) {
@@ -148,6 +155,22 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
         __syncthreads();
         if (threadIdx.x == 0) touched[leaf] = 0;
     }
+}
+
+// Adam of the MLP parameters (stepmode 0, adam_upd_kernel.cu:8-58); the step
+// size comes from the device when the step is replayed from a graph.
+__global__ void k_mlp_adam(float* __restrict__ p, const float* __restrict__ g,
+                           float* __restrict__ m, float* __restrict__ v, int n, float sz_arg,
+                           const float* __restrict__ dev_sz, float eps, float b0, float b1) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const float sz = dev_sz ? dev_sz[2] : sz_arg;
+    const float gg = g[q];
+    const float nm = b0 * m[q] + (1.f - b0) * gg;
+    const float nv = b1 * v[q] + ((1.f - b1) * gg) * gg;
+    m[q] = nm;
+    v[q] = nv;
+    p[q] = p[q] - (sz * nm) / (sqrtf(nv) + eps);
 }
 
 // Optional per-stage device timing (HIP events on the step's stream).
@@ -365,7 +388,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     {
         StageScope sc(6, s);
         if ((rc = launch_scatter(t, R, rec, d->capacity, d->grad_density, d->grad_color, touched,
-                                 s)))
+                                 ws + L.scatter, s)))
             return rc;
     }
     {
@@ -383,16 +406,17 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
             hipLaunchKernelGGL(k_sparse_adam<true>, dim3(ag), dim3(256), 0, s, t, touched,
                                d->density, d->grad_density, d->density_m, d->density_v, d->color,
                                d->grad_color, d->color_m, d->color_v, d->stepsz_density,
-                               d->stepsz_color, d->eps, d->beta0, d->beta1);
+                               d->stepsz_color, d->adam_scalars, d->eps, d->beta0, d->beta1);
         else
             hipLaunchKernelGGL(k_sparse_adam<false>, dim3(ag), dim3(256), 0, s, t, touched,
                                d->density, d->grad_density, d->density_m, d->density_v, d->color,
                                d->grad_color, d->color_m, d->color_v, d->stepsz_density,
-                               d->stepsz_color, d->eps, d->beta0, d->beta1);
+                               d->stepsz_color, d->adam_scalars, d->eps, d->beta0, d->beta1);
         PVDB_LAUNCHED("k_sparse_adam");
-        if ((rc = pvdb_dense_adam(d->mlp, d->mlp_grad, d->mlp_m, d->mlp_v, nullptr, kMlpParams, 0,
-                                  d->step_size_mlp, d->eps, d->beta0, d->beta1, stream)))
-            return rc;
+        hipLaunchKernelGGL(k_mlp_adam, dim3(pvdb_blocks(kMlpParams, 256)), dim3(256), 0, s, d->mlp,
+                           d->mlp_grad, d->mlp_m, d->mlp_v, kMlpParams, d->step_size_mlp,
+                           d->adam_scalars, d->eps, d->beta0, d->beta1);
+        PVDB_LAUNCHED("k_mlp_adam");
     }
     return PVDB_OK;
 }

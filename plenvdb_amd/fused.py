@@ -38,7 +38,7 @@ class StepDesc(C.Structure):
         ("eps", C.c_float), ("beta0", C.c_float), ("beta1", C.c_float),
         ("do_adam", C.c_int32), ("reserved", C.c_int32),
         ("workspace", C.c_void_p), ("workspace_bytes", C.c_int64), ("capacity", C.c_int64),
-        ("ray_out", C.c_void_p), ("loss", C.c_void_p),
+        ("ray_out", C.c_void_p), ("loss", C.c_void_p), ("adam_scalars", C.c_void_p),
     ]
 
 
@@ -133,6 +133,8 @@ class FusedTrainer:
         self.lr_mlp = float(lr_mlp)
         self.decay = 0.1 ** (1 / (lrate_decay * 1000))
         self.step_count = 0
+        # per-step Adam step sizes, read on the device (a captured step replays them)
+        self.adam_scalars = torch.zeros(4, device=dev)
         self.desc = self._make_desc()
         lib.call("pvdb_fused_init", self.topo.handle, C.byref(self.desc), lib.stream())
         self.graph = None
@@ -179,6 +181,7 @@ class FusedTrainer:
         d.capacity = self.capacity
         d.ray_out = self.ray_out.data_ptr()
         d.loss = self.loss.data_ptr()
+        d.adam_scalars = self.adam_scalars.data_ptr()
         return d
 
     @staticmethod
@@ -194,18 +197,37 @@ class FusedTrainer:
 
     def step(self):
         """One full iteration: forward, backward, Adam (grids stepmode 1, MLP
-        stepmode 0), gradients re-zeroed, lr decay (run.py:592-598)."""
+        stepmode 0), gradients re-zeroed, lr decay (run.py:592-598).  Replays
+        the captured graph when `capture_graph` was called."""
         self.step_count += 1
         t = self.step_count
-        self.desc.do_adam = 1
-        self.desc.stepsz_density = self._stepsz(self.lr_grid, t)
-        self.desc.stepsz_color = self._stepsz(self.lr_grid, t)
-        self.desc.step_size_mlp = self._stepsz(self.lr_mlp, t)
-        lib.call("pvdb_fused_step", self.topo.handle, C.byref(self.desc), lib.stream())
-        self.mlp_grad.zero_()
+        sz = [self._stepsz(self.lr_grid, t), self._stepsz(self.lr_grid, t),
+              self._stepsz(self.lr_mlp, t), 0.0]
+        self.adam_scalars.copy_(torch.tensor(sz, dtype=torch.float32))
+        if self.graph is not None:
+            self.graph.replay()
+        else:
+            self.desc.do_adam = 1
+            self.desc.stepsz_density, self.desc.stepsz_color, self.desc.step_size_mlp = sz[:3]
+            lib.call("pvdb_fused_step", self.topo.handle, C.byref(self.desc), lib.stream())
         self.lr_grid = _f32(np.float32(self.lr_grid) * np.float32(self.decay))
         self.lr_mlp = self.lr_mlp * self.decay
         return self.loss
+
+    def capture_graph(self):
+        """Captures one full step (do_adam) into a hipGraph; later `step()` calls
+        replay it.  Run at least one eager step first (library set-up calls must
+        not happen under capture)."""
+        torch.cuda.synchronize()
+        self.desc.do_adam = 1
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.graph(g, stream=s):
+            lib.call("pvdb_fused_step", self.topo.handle, C.byref(self.desc), lib.stream())
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        self.graph = g
 
     def zero_grad(self):
         self.grad_density.zero_()

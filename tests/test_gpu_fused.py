@@ -1,13 +1,14 @@
 """GPU parity of the fused (batched, BF16-MLP) training step against the FP64
 oracle of the reference step.
 
-Tolerances: rendered RGB within 1e-3 max-abs of the FP64 oracle; loss within
-1e-3 relative; gradients compared by relative L2 error: density <= 2e-3, MLP
-<= 2e-2, colour features <= 1e-1.  The colour-feature gradient passes through
-g_X = W0[0:12] g_H0 with BF16 operands, whose cancellation gives ~6e-2 with
-exact BF16 rounding (a CPU emulation of the same rounding reproduces it), so
-that bound is the BF16 precision, not a tolerance for a defect.  Integer work
--- the sample set and per-ray counts -- must match exactly.
+The MLP weights are held in BF16 on the device (their fp32 masters are what
+Adam updates), so the FP64 oracle evaluates the same BF16-valued weights; the
+device carries features, activations and gradients as (hi, lo) BF16 pairs with
+FP32 accumulation.  Tolerances: rendered RGB within 1e-3 max-abs; loss within
+1e-3 relative; gradients (density, colour features, MLP) within 1e-3 relative
+L2 error.  Integer work -- the sample set and per-ray counts -- must match
+exactly.  The error against the oracle with the fp32 master weights (which adds
+the BF16 rounding of the weights themselves) is printed, not asserted.
 """
 import numpy as np
 import pytest
@@ -22,13 +23,24 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def oracle_for(oracle_mod, tr, sc):
+def bf16_weights(mlp):
+    """The weight matrices as the device holds them (BF16); biases stay fp32."""
+    out = dict(mlp)
+    for k in ("w0", "w1", "w2"):
+        out[k] = torch.from_numpy(np.ascontiguousarray(mlp[k], np.float32)).to(
+            torch.bfloat16).double().numpy()
+    return out
+
+
+def oracle_for(oracle_mod, tr, sc, held_weights=True):
     origins, vm = oracle_mod.leaves_from_mask(sc["mask"])
     col = tr.color.float().cpu().numpy().astype(np.float64)
     den = tr.density.cpu().numpy().astype(np.float64)
     from plenvdb_amd.fused import unpack_mlp
 
     mlp = {k: v.astype(np.float64) for k, v in unpack_mlp(tr.mlp.cpu().numpy()).items()}
+    if held_weights:
+        mlp = bf16_weights(mlp)
     osc = dict(sc, origins=origins, vmask=vm, density=den, color=col, mlp=mlp)
     return oracle_mod.reference_train_step(osc, prec="f64", with_mlp=True)
 
@@ -61,10 +73,15 @@ def test_fused_forward_backward_vs_fp64_oracle(oracle_mod, name, n_rays, bf16):
     print(f"[result] rgb max-abs {err_rgb:.2e}  loss rel {err_loss:.2e}  grad density rel-L2 "
           f"{e_den:.2e}  grad color rel-L2 {e_col:.2e}  mlp " +
           " ".join(f"{k} {v:.2e}" for k, v in e_mlp.items()), flush=True)
+    ref32 = oracle_for(oracle_mod, tr, sc, held_weights=False)
+    print(f"[info] against fp32 master weights: rgb max-abs "
+          f"{float(np.abs(rgb - ref32['rgb_marched']).max()):.2e}  grad color rel-L2 "
+          f"{rel_l2(tr.grad_color.cpu().numpy(), ref32['grad_color']):.2e}  grad density rel-L2 "
+          f"{rel_l2(tr.grad_density.cpu().numpy(), ref32['grad_density']):.2e}", flush=True)
     assert err_rgb <= 1e-3
     assert err_loss <= 1e-3
-    assert e_den <= 2e-3 and e_col <= 1e-1
-    assert max(e_mlp.values()) <= 2e-2
+    assert e_den <= 1e-3 and e_col <= 1e-3
+    assert max(e_mlp.values()) <= 1e-3
 
 
 def test_fused_step_applies_adam(oracle_mod):
@@ -94,7 +111,7 @@ def test_fused_step_applies_adam(oracle_mod):
     dc = (tr.color - c0).cpu().numpy()
     e_c = rel_l2(dc, expected(ref["grad_color"]))
     print(f"[result] first-step update rel-L2: density {e_d:.2e} colour {e_c:.2e}", flush=True)
-    assert e_d <= 1e-2 and e_c <= 1e-1
+    assert e_d <= 1e-2 and e_c <= 1e-2
     assert np.all(dd[np.asarray(g) == 0] == 0)
     # gradients are re-zeroed for the next step
     assert float(tr.grad_density.abs().max()) == 0.0
