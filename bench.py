@@ -399,8 +399,8 @@ def cpu_baseline(cfg_name: str, n_rays: int):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}); multi-GPU runs are not part "
-              "of this build", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); runs over several GPUs "
+              "are not implemented", file=sys.stderr)
         sys.exit(2)
     import torch
 

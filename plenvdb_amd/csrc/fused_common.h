@@ -13,8 +13,8 @@
 //   k_mlp<true>    workgroup per 32-record group: recompute from the kept
 //                  features, MFMA MLP backward, alpha/raw2alpha adjoints, dW;
 //                  per-record feature and sigma gradients
-//   scatter        records binned by base leaf, summed per leaf in LDS and
-//                  added to the grids without HBM atomics (fused_scatter.hip)
+//   k_scatter_agg  workgroup per 128 records: corner contributions summed per
+//                  slot in an LDS hash, one device atomic per distinct slot
 //   k_dw_reduce, k_sparse_adam, dense Adam of the MLP, k_pack_mlp
 #pragma once
 

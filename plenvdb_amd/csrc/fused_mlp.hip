@@ -20,7 +20,7 @@
 // group's 32 samples (K = 32) into per-wave accumulator tiles written once per
 // workgroup.  The same pass evaluates the Alphas2Weights / raw2alpha adjoints
 // (render_utils_kernel.cu:653-676, 506-517) and writes each record's feature
-// and sigma gradients; the leaf-binned scatter (fused_scatter.hip) then adds
+// and sigma gradients; the scatter (fused_scatter.hip) then adds
 // them to the grids with the "leaf must exist" rule (densityvdb.cu:16-27,
 // colorvdb.cu:28-37).  The forward pass keeps the gathered BF16 features per
 // record, so the backward reads 32 bytes per record instead of 8 gathers.

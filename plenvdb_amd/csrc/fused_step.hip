@@ -24,10 +24,8 @@ int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const Topo
                const void* color, const MlpPacked& W, const RayBuffers& R, const Records& rec,
                float* dw_partial, hipStream_t s);
 int launch_dw_reduce(const float* part, int nparts, float* dw, hipStream_t s);
-int64_t scatter_workspace_bytes(int64_t nleaf, int64_t capacity);
 int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, int64_t capacity,
-                   float* grad_density, float* grad_color, uint8_t* leaf_touched, void* ws,
-                   hipStream_t s);
+                   float* grad_density, float* grad_color, uint8_t* leaf_touched, hipStream_t s);
 int launch_pack_mlp(const float* prm, uint16_t* w0t, uint16_t* w1t, uint16_t* w2t, uint16_t* w2,
                     uint16_t* w1, uint16_t* w0, hipStream_t s);
 }  // namespace pvdb
@@ -41,7 +39,7 @@ constexpr int kMaxMlpGrid = 512;
 struct Layout {
     size_t count, offset, total, t_last, ray_out, ray_grad, ray_ctot, ray_loss, loss;
     size_t xyze, atwr, rgb, cinc, feat, grad;
-    size_t touched, dw_part, w0t, w1t, w2t, w2, w1, w0, scatter;
+    size_t touched, dw_part, w0t, w1t, w2t, w2, w1, w0;
     size_t bytes;
 };
 
@@ -78,7 +76,6 @@ Layout layout_for(int64_t n_rays, int64_t capacity, int64_t nleaf) {
     L.w2 = take(2 * 128 * 32);
     L.w1 = take(2 * 128 * 128);
     L.w0 = take(2 * 16 * 128);
-    L.scatter = take((size_t)scatter_workspace_bytes(nleaf, capacity));
     L.bytes = o;
     return L;
 }
@@ -388,7 +385,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     {
         StageScope sc(6, s);
         if ((rc = launch_scatter(t, R, rec, d->capacity, d->grad_density, d->grad_color, touched,
-                                 ws + L.scatter, s)))
+                                 s)))
             return rc;
     }
     {
