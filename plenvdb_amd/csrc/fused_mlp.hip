@@ -9,10 +9,11 @@
 //   H0^T = relu(W0^T X^T + b0),  H1^T = relu(W1^T H0^T + b1),  L^T = W2^T H1^T + b2
 // with v_mfma_f32_16x16x32_bf16 (samples are the N dimension).  Weights are
 // held in bf16; features, activations and gradients travel as (hi, lo) bf16
-// pairs, so each product with a weight is two MFMAs and each dW product
-// (activation x gradient) three, which keeps about 16 significant bits where
-// plain bf16 keeps 8 (plain bf16 inputs put the colour-feature gradients
-// several percent off an FP64 evaluation; the pairs bring them under 1e-3).
+// pairs, so each product with a weight is two MFMAs, which keeps about 16
+// significant bits where plain bf16 keeps 8 (plain bf16 inputs put the
+// colour-feature gradients several percent off an FP64 evaluation; the pairs
+// bring them under 1e-3).  dW products (activation x gradient) take the hi
+// parts only.
 // The forward weight slices stay in registers; the backward weights sit in LDS.  The
 // backward pass recomputes the forward, forms dL/dlogit from the per-ray loss
 // gradient, runs g_H1^T = W2 g_L^T, g_H0^T = W1 g_H1^T (masked by the ReLUs,
@@ -22,8 +23,9 @@
 // (render_utils_kernel.cu:653-676, 506-517) and writes each record's feature
 // and sigma gradients; the scatter (fused_scatter.hip) then adds
 // them to the grids with the "leaf must exist" rule (densityvdb.cu:16-27,
-// colorvdb.cu:28-37).  The forward pass keeps the gathered BF16 features per
-// record, so the backward reads 32 bytes per record instead of 8 gathers.
+// colorvdb.cu:28-37).  The forward pass keeps the gathered features per record
+// as (hi, lo) bf16, so the backward reads 48 bytes per record instead of
+// redoing 8 gathers.
 // Each group's records, features and ray data are loaded into registers while
 // the previous group computes.
 #include <type_traits>
@@ -116,9 +118,11 @@ constexpr int kBw2Frags = 8 * 64, kBw1Frags = 8 * 4 * 64, kBw0Frags = 4 * 64;
 
 // Sample-major activations are split (hi, lo) bf16 pairs (B operands of the
 // weight products); feature-major copies (dW operands) keep the hi part.  A1
-// holds H0 (steps 3-4) and then g_H1 (7-8); A2 holds H1 (4-5) and then g_H0
+// holds H0 (steps 3-4) and then g_H1 (7-8); A2 holds H1 (4) and then g_H0
 // (8-9); F2 holds H1 feature-major (4-7) and then g_H0 (8-9).  Barriers
-// separate every reuse.
+// separate every reuse.  dL/dlogit and dL/dsigma are double-buffered: they
+// are formed for the next group (from its prefetched records) while the
+// current group is in step 7.
 struct alignas(16) SmemBwd {
     uint4 Bw1[kBw1Frags];
     uint4 Bw0[kBw0Frags];
@@ -126,26 +130,16 @@ struct alignas(16) SmemBwd {
     uint16_t Xh[GS * XM], Xl[GS * XM];
     uint16_t A1h[GS * SM], A1l[GS * SM];
     uint16_t A2h[GS * SM], A2l[GS * SM];
-    uint16_t GLh[GS * GM], GLl[GS * GM];
+    uint16_t GLh[2][GS * GM], GLl[2][GS * GM];
     uint16_t Xf[XF_ROWS * FM];
     uint16_t H0f[128 * FM];
     uint16_t F2[128 * FM];
-    uint16_t GLf[16 * FM];
+    uint16_t GLf[2][16 * FM];
     uint16_t G1f[128 * FM];
-    float GX[16][GS];
-    float gsig[GS];
-    float rgb[GS][4];
+    float GX[2][16][GS];  // g_X^T partials over the two halves of the hidden layer
+    float gsig[2][GS];
     RecInfo ri[GS];
-    float4 rayg[GS];
-    float4 ctot[GS];
-    float4 cinc[GS];
-    float tlast[GS];
-    // the next group's ray data and kept features, parked after step 7
-    float4 nrayg[GS];
-    float4 nctot[GS];
-    float4 ncinc[GS];
-    uint4 nfeat[GS][3];
-    float ntlast[GS];
+    uint4 nfeat[GS][3];  // the next group's kept features, parked after step 7
 };
 
 __device__ __forceinline__ float sigmoidf_(float x) { return 1.f / (1.f + expf(-x)); }
@@ -154,6 +148,7 @@ __device__ __forceinline__ float sigmoidf_(float x) { return 1.f / (1.f + expf(-
 struct Prefetch {
     float4 xyze, atwr;          // stage A: the record
     float4 rayg, ctot, cinc;    // stage B (backward): ray data, prefix
+    float4 rgb;                 // stage B (backward): the forward pass's colour
     float tlast;
     uint4 f0, f1, f2;           // stage B (backward): kept features (hi 12, lo 12)
 };
@@ -229,7 +224,7 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
 
     // ---- prefetch of the first group ----
     Prefetch pf;
-    pf.xyze = pf.atwr = pf.rayg = pf.ctot = pf.cinc = z4;
+    pf.xyze = pf.atwr = pf.rayg = pf.ctot = pf.cinc = pf.rgb = z4;
     pf.tlast = 0.f;
     pf.f0 = pf.f1 = pf.f2 = zu4;
     auto stage_a = [&](int64_t grp) {
@@ -242,7 +237,7 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
     };
     auto stage_b = [&](int64_t grp) {  // backward only; needs stage A's atwr
         const int64_t q = grp * GS + tid;
-        pf.rayg = pf.ctot = pf.cinc = z4;
+        pf.rayg = pf.ctot = pf.cinc = pf.rgb = z4;
         pf.tlast = 0.f;
         pf.f0 = pf.f1 = pf.f2 = zu4;
         if (tid < GS && grp < ngroups && q < total) {
@@ -255,19 +250,51 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                 pf.cinc = rec.cinc[q];
             }
             if (bits & kAliveW) {
+                pf.rgb = rec.rgb[q];
                 pf.f0 = rec.feat[3 * q];
                 pf.f1 = rec.feat[3 * q + 1];
                 pf.f2 = rec.feat[3 * q + 2];
             }
         }
     };
-    auto park = [&]() {  // backward: stage B registers -> LDS next-buffers
+    // backward: the prefetched group's per-sample adjoints (dL/dlogit from the
+    // per-ray dL/drgb, dL/dsigma through Alphas2Weights and raw2alpha) into LDS
+    // buffer `b`, and its kept features into the next-buffer
+    auto park = [&](int b) {
         if constexpr (BWD) {
             if (tid < GS) {
-                sm.nrayg[tid] = pf.rayg;
-                sm.nctot[tid] = pf.ctot;
-                sm.ncinc[tid] = pf.cinc;
-                sm.ntlast[tid] = pf.tlast;
+                const uint32_t bits = __float_as_uint(pf.atwr.w);
+                const float4 gr = pf.rayg;
+                float gl[3] = {0.f, 0.f, 0.f};
+                float gw = 0.f;
+                if (bits & kAliveW) {
+                    const float c0 = pf.rgb.x, c1 = pf.rgb.y, c2 = pf.rgb.z, w = pf.atwr.z;
+                    gl[0] = w * gr.x * c0 * (1.f - c0);
+                    gl[1] = w * gr.y * c1 * (1.f - c1);
+                    gl[2] = w * gr.z * c2 * (1.f - c2);
+                    gw = gr.x * c0 + gr.y * c1 + gr.z * c2;
+                }
+                float gs = 0.f;
+                if (bits & kAliveA) {
+                    const float4 ct = pf.ctot, ci = pf.cinc;
+                    const float e = pf.xyze.w, alpha = pf.atwr.x, T = pf.atwr.y;
+                    const float back = gr.w * pf.tlast + gr.x * (ct.x - ci.x) +
+                                       gr.y * (ct.y - ci.y) + gr.z * (ct.z - ci.z);
+                    const float ga = gw * T - back / ((1.f - alpha) + 1e-10f);
+                    gs = fminf(e, 1e10f) * powf(1.f + e, -P.interval - 1.f) * P.interval * ga;
+                }
+                sm.gsig[b][tid] = gs;
+                db2[0] += gl[0];
+                db2[1] += gl[1];
+                db2[2] += gl[2];
+#pragma unroll
+                for (int o = 0; o < 3; ++o) {  // logits 3..15 stay zero
+                    uint16_t h, l;
+                    split_bf(gl[o], h, l);
+                    sm.GLh[b][tid * GM + o] = h;
+                    sm.GLl[b][tid * GM + o] = l;
+                    sm.GLf[b][o * FM + tid] = h;
+                }
                 sm.nfeat[tid][0] = pf.f0;
                 sm.nfeat[tid][1] = pf.f1;
                 sm.nfeat[tid][2] = pf.f2;
@@ -276,10 +303,20 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
     };
     stage_a(blockIdx.x);
     if constexpr (BWD) {
+        // the unused logits (3..15) of both dL/dlogit buffers are zero throughout
+        for (int e = tid; e < 2 * GS * GM; e += 256) {
+            if (e % GM >= 3) {
+                (&sm.GLh[0][0])[e] = 0;
+                (&sm.GLl[0][0])[e] = 0;
+            }
+        }
+        for (int e = tid; e < 2 * 16 * FM; e += 256)
+            if ((e % (16 * FM)) >= 3 * FM) (&sm.GLf[0][0])[e] = 0;
         stage_b(blockIdx.x);
-        park();
+        park(0);
         __syncthreads();
     }
+    int cb = 0;  // the current group's adjoint buffer
 
     for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {  // uniform bound
         const int64_t base = grp * GS;
@@ -296,10 +333,6 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
             ri.bits = __float_as_uint(pf.atwr.w);  // zero for records past the end
             sm.ri[tid] = ri;
             if constexpr (BWD) {
-                sm.rayg[tid] = sm.nrayg[tid];
-                sm.ctot[tid] = sm.nctot[tid];
-                sm.cinc[tid] = sm.ncinc[tid];
-                sm.tlast[tid] = sm.ntlast[tid];
                 // kept features: hi halves 0..11 then lo halves 0..11
                 const uint4 f0 = sm.nfeat[tid][0], f1 = sm.nfeat[tid][1], f2 = sm.nfeat[tid][2];
                 const uint32_t w12[12] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y,
@@ -465,6 +498,8 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                 }
             }
         }
+        // the next group's stage-A loads (issued in step 1) have landed: start stage B
+        if constexpr (BWD) stage_b(next);
         __syncthreads();
         // ---- 4. H1^T slice = relu(W1^T H0^T + b1) -> A2 (and H1f in F2) ----
 #pragma unroll
@@ -492,76 +527,35 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
             }
         }
         __syncthreads();
-        // ---- 5. logits (wave nt takes tile nt): L^T = W2^T H1^T + b2, rgb = sigmoid ----
-        if (wv < NT) {
-            const int nt = wv;
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            const int ho = (16 * nt + r16) * SM + 8 * g;
+        if constexpr (!BWD) {
+            // ---- 5. logits (wave nt takes tile nt): L^T = W2^T H1^T + b2, rgb = sigmoid ----
+            if (wv < NT) {
+                const int nt = wv;
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                const int ho = (16 * nt + r16) * SM + 8 * g;
 #pragma unroll
-            for (int kt = 0; kt < 4; ++kt)
-                acc = mfma_w(ld_frag(W.w2t + r16 * 128 + 32 * kt + 8 * g),
-                             ld_frag(&sm.A2h[ho + 32 * kt]), ld_frag(&sm.A2l[ho + 32 * kt]), acc);
-            if (g == 0) {
-                const int s = 16 * nt + r16;
-                const float c0 = sigmoidf_(acc[0] + b2v[0]);
-                const float c1 = sigmoidf_(acc[1] + b2v[1]);
-                const float c2 = sigmoidf_(acc[2] + b2v[2]);
-                if constexpr (BWD) {
-                    sm.rgb[s][0] = c0;
-                    sm.rgb[s][1] = c1;
-                    sm.rgb[s][2] = c2;
-                } else {
+                for (int kt = 0; kt < 4; ++kt)
+                    acc = mfma_w(ld_frag(W.w2t + r16 * 128 + 32 * kt + 8 * g),
+                                 ld_frag(&sm.A2h[ho + 32 * kt]), ld_frag(&sm.A2l[ho + 32 * kt]),
+                                 acc);
+                if (g == 0) {
+                    const int s = 16 * nt + r16;
                     const int64_t q = base + s;
                     if (q < total) {
                         const bool alive = (sm.ri[s].bits & kAliveW) != 0u;
-                        rec.rgb[q] = alive ? make_float4(c0, c1, c2, 0.f) : z4;
+                        rec.rgb[q] = alive ? make_float4(sigmoidf_(acc[0] + b2v[0]),
+                                                         sigmoidf_(acc[1] + b2v[1]),
+                                                         sigmoidf_(acc[2] + b2v[2]), 0.f)
+                                           : z4;
                     }
                 }
             }
-        }
-        if constexpr (!BWD) {
             __syncthreads();
             continue;
         } else {
-            // the next group's stage-A loads have landed by now: start stage B
-            stage_b(next);
-            __syncthreads();
-            // ---- 6. per-sample adjoints: dL/dlogit, alpha, sigma ----
-            for (int e = tid; e < 16 * GS; e += 256) (&sm.GX[0][0])[e] = 0.f;
-            if (tid < GS) {
-                const RecInfo ri = sm.ri[tid];
-                const float4 gr = sm.rayg[tid];
-                float gl[3] = {0.f, 0.f, 0.f};
-                float gw = 0.f;
-                if (ri.bits & kAliveW) {
-                    const float c0 = sm.rgb[tid][0], c1 = sm.rgb[tid][1], c2 = sm.rgb[tid][2];
-                    gl[0] = ri.w * gr.x * c0 * (1.f - c0);
-                    gl[1] = ri.w * gr.y * c1 * (1.f - c1);
-                    gl[2] = ri.w * gr.z * c2 * (1.f - c2);
-                    gw = gr.x * c0 + gr.y * c1 + gr.z * c2;
-                }
-                float gs = 0.f;
-                if (ri.bits & kAliveA) {
-                    const float4 ct = sm.ctot[tid], ci = sm.cinc[tid];
-                    const float back = gr.w * sm.tlast[tid] + gr.x * (ct.x - ci.x) +
-                                       gr.y * (ct.y - ci.y) + gr.z * (ct.z - ci.z);
-                    const float ga = gw * ri.T - back / ((1.f - ri.alpha) + 1e-10f);
-                    gs = fminf(ri.e, 1e10f) * powf(1.f + ri.e, -P.interval - 1.f) * P.interval * ga;
-                }
-                sm.gsig[tid] = gs;
-                db2[0] += gl[0];
-                db2[1] += gl[1];
-                db2[2] += gl[2];
-#pragma unroll
-                for (int o = 0; o < 16; ++o) {
-                    uint16_t h = 0, l = 0;
-                    if (o < 3) split_bf(gl[o], h, l);
-                    sm.GLh[tid * GM + o] = h;
-                    sm.GLl[tid * GM + o] = l;
-                    sm.GLf[o * FM + tid] = h;
-                }
-            }
-            __syncthreads();
+            // no logits here: the per-sample adjoints were formed in park() from
+            // the rgb the forward pass stored (same inputs through the same code,
+            // so the ReLU masks recomputed here are the forward's)
             // ---- 7. g_H1^T slice = W2[slice] g_L^T, masked -> A1 (and G1f); dW2 ----
 #pragma unroll
             for (int mt = 0; mt < 2; ++mt) {
@@ -571,8 +565,8 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                 for (int nt = 0; nt < NT; ++nt) {
                     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
                     const int go = (16 * nt + r16) * GM + 4 * g;
-                    acc = mfma16x16x16(a, ld_frag4(&sm.GLh[go]), acc);
-                    acc = mfma16x16x16(a, ld_frag4(&sm.GLl[go]), acc);
+                    acc = mfma16x16x16(a, ld_frag4(&sm.GLh[cb][go]), acc);
+                    acc = mfma16x16x16(a, ld_frag4(&sm.GLl[cb][go]), acc);
                     float gv[4];
 #pragma unroll
                     for (int i = 0; i < 4; ++i) {
@@ -586,9 +580,9 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                 }
                 // dW2[in = f0+16mt+row][out] += sum over the group's samples H1[s][in] gL[s][out]
                 const int ho = (f0 + 16 * mt + r16) * FM + 8 * g, lo_ = r16 * FM + 8 * g;
-                dW2[mt] = mfma16x16x32(ld_frag8(&sm.F2[ho]), ld_frag8(&sm.GLf[lo_]), dW2[mt]);
+                dW2[mt] = mfma16x16x32(ld_frag8(&sm.F2[ho]), ld_frag8(&sm.GLf[cb][lo_]), dW2[mt]);
             }
-            park();  // the next group's stage-B loads (issued in step 5) have landed
+            park(cb ^ 1);  // the next group's stage-B loads (issued after step 3) have landed
             __syncthreads();
             // ---- 8. g_H0^T slice = W1[slice] g_H1^T, masked -> A2 (and G0f in F2); dW1 ----
 #pragma unroll
@@ -642,15 +636,18 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
 #pragma unroll
                     for (int nt = 0; nt < 2; ++nt) dW0[mi][nt] = mfma16x16x32(ah, bh[nt], dW0[mi][nt]);
                 }
-                const bf16x8 a = as_bf8(sm.Bw0[wv * 64 + lane]);
+                // g_X^T: wave w takes sample tile w & 1 over hidden half w >> 1
+                const int nt = wv & 1, kh = wv >> 1;
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    const int go = (16 * nt + r16) * SM + f0 + 8 * g;
-                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-                    acc = mfma_w(a, ld_frag(&sm.A2h[go]), ld_frag(&sm.A2l[go]), acc);
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) atomicAdd(&sm.GX[4 * g + i][16 * nt + r16], acc[i]);
+                for (int kk = 0; kk < 2; ++kk) {
+                    const int kt = 2 * kh + kk;
+                    const int go = (16 * nt + r16) * SM + 32 * kt + 8 * g;
+                    acc = mfma_w(as_bf8(sm.Bw0[kt * 64 + lane]), ld_frag(&sm.A2h[go]),
+                                 ld_frag(&sm.A2l[go]), acc);
                 }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) sm.GX[kh][4 * g + i][16 * nt + r16] = acc[i];
             }
             __syncthreads();
             // ---- 10. per-record gradients: dL/dfeature (12) and dL/dsigma ----
@@ -661,16 +658,21 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                     const RecInfo ri = sm.ri[s];
                     float4 v = z4;
                     if (part < 3) {
-                        if (ri.bits & kAliveW)
-                            v = make_float4(sm.GX[4 * part][s], sm.GX[4 * part + 1][s],
-                                            sm.GX[4 * part + 2][s], sm.GX[4 * part + 3][s]);
+                        if (ri.bits & kAliveW) {
+                            const int c = 4 * part;
+                            v = make_float4(sm.GX[0][c][s] + sm.GX[1][c][s],
+                                            sm.GX[0][c + 1][s] + sm.GX[1][c + 1][s],
+                                            sm.GX[0][c + 2][s] + sm.GX[1][c + 2][s],
+                                            sm.GX[0][c + 3][s] + sm.GX[1][c + 3][s]);
+                        }
                     } else {
-                        v.x = sm.gsig[s];
+                        v.x = sm.gsig[cb][s];
                     }
                     rec.grad[4 * q + part] = v;
                 }
             }
             __syncthreads();
+            cb ^= 1;
         }
     }
 
