@@ -68,22 +68,66 @@ __device__ __forceinline__ s16x4 ld_frag4(const uint16_t* p) {
 
 __device__ __forceinline__ bf16x8 as_bf8(const uint4& v) { return __builtin_bit_cast(bf16x8, v); }
 
+// Two floats -> two bf16 (round to nearest even) in one dword, element 0 in
+// the low half (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint32_t pk_bf(float a, float b) {
+    typedef __bf16 bf2_t __attribute__((ext_vector_type(2)));
+    typedef float f2_t __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f2_t){a, b}, bf2_t));
+}
+__device__ __forceinline__ float bf_lo(uint32_t p) { return __uint_as_float(p << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t p) { return __uint_as_float(p & 0xffff0000u); }
+
 // x = hi + lo with hi = bf16(x) and lo = bf16(x - hi): the pair carries about
 // 16 significant bits, so products of a bf16 weight with (hi, lo) keep the
-// activation's precision.
+// activation's precision.  Two values per call, packed as above.
+__device__ __forceinline__ void split2(float a, float b, uint32_t& h, uint32_t& l) {
+    h = pk_bf(a, b);
+    l = pk_bf(a - bf_lo(h), b - bf_hi(h));
+}
 __device__ __forceinline__ void split_bf(float x, uint16_t& h, uint16_t& l) {
-    h = f2bf_bits(x);
-    l = f2bf_bits(x - bf_bits2f(h));
+    uint32_t hh, ll;
+    split2(x, 0.f, hh, ll);
+    h = (uint16_t)hh;
+    l = (uint16_t)ll;
 }
 
-__device__ __forceinline__ void st4s(uint16_t* ph, uint16_t* pl, const float* v) {
-    uint16_t h[4], l[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) split_bf(v[i], h[i], l[i]);
-    *reinterpret_cast<uint2*>(ph) = make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16),
-                                               (uint32_t)h[2] | ((uint32_t)h[3] << 16));
-    *reinterpret_cast<uint2*>(pl) = make_uint2((uint32_t)l[0] | ((uint32_t)l[1] << 16),
-                                               (uint32_t)l[2] | ((uint32_t)l[3] << 16));
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {  // quad permutation, all lanes active
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+
+// A lane holds four consecutive features (rows) of one sample; the four lanes of
+// a quad hold four consecutive samples.  Returns row (lane & 3) of the quad's
+// four samples, as four bf16 in two dwords: two quad swaps instead of four
+// 2-byte stores per lane.
+__device__ __forceinline__ uint2 quad_transpose(uint2 h, int q) {
+    const uint32_t v0 = h.x & 0xffffu, v1 = h.x >> 16, v2 = h.y & 0xffffu, v3 = h.y >> 16;
+    // round 1 (lanes q, q^1): even lanes gather rows 0/2, odd lanes rows 1/3, of the pair
+    const uint32_t send = (q & 1) ? (v0 | (v2 << 16)) : (v1 | (v3 << 16));
+    const uint32_t recv = dpp<0xB1>(send);  // quad_perm [1,0,3,2]
+    const uint32_t A = (q & 1) ? ((recv & 0xffffu) | (v1 << 16)) : (v0 | (recv << 16));
+    const uint32_t B = (q & 1) ? ((recv >> 16) | (v3 << 16)) : (v2 | (recv & 0xffff0000u));
+    // round 2 (lanes q, q^2): rows (q & 1) and (q & 1) + 2 of the two pairs
+    const uint32_t send2 = (q & 2) ? A : B;
+    const uint32_t recv2 = dpp<0x4E>(send2);  // quad_perm [2,3,0,1]
+    return (q & 2) ? make_uint2(recv2, B) : make_uint2(A, recv2);
+}
+
+// Four consecutive features fr..fr+3 of sample `col`: sample-major (hi, lo)
+// pairs into sh/sl, and (FMC) the hi parts feature-major into fm.
+template <bool FMC>
+__device__ __forceinline__ void st_act(uint16_t* sh, uint16_t* sl, uint16_t* fm, int col, int fr,
+                                       int lane, const float* v) {
+    uint2 hi, lo;
+    split2(v[0], v[1], hi.x, lo.x);
+    split2(v[2], v[3], hi.y, lo.y);
+    *reinterpret_cast<uint2*>(&sh[col * SM + fr]) = hi;
+    *reinterpret_cast<uint2*>(&sl[col * SM + fr]) = lo;
+    if constexpr (FMC) {
+        const int q = lane & 3;
+        *reinterpret_cast<uint2*>(&fm[(fr + q) * FM + (col & ~3)]) = quad_transpose(hi, q);
+    }
 }
 
 // a (bf16 weight) times a split activation
@@ -138,6 +182,7 @@ struct alignas(16) SmemBwd {
     uint16_t G1f[128 * FM];
     float GX[2][16][GS];  // g_X^T partials over the two halves of the hidden layer
     float gsig[2][GS];
+    float4 vd[2][GS];  // view directions, double-buffered like the adjoints
     RecInfo ri[GS];
     uint4 nfeat[GS][3];  // the next group's kept features, parked after step 7
 };
@@ -149,6 +194,7 @@ struct Prefetch {
     float4 xyze, atwr;          // stage A: the record
     float4 rayg, ctot, cinc;    // stage B (backward): ray data, prefix
     float4 rgb;                 // stage B (backward): the forward pass's colour
+    float4 vd;                  // stage B (backward): the ray's view direction
     float tlast;
     uint4 f0, f1, f2;           // stage B (backward): kept features (hi 12, lo 12)
 };
@@ -162,6 +208,9 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
     const int tid = threadIdx.x;
     const int wv = tid >> 6, lane = tid & 63, r16 = lane & 15, g = lane >> 4;
     const int f0 = 32 * wv;  // first hidden feature owned by this wave
+    // per-record staging (prefetch, adjoints, commit): lanes 0..7 of every wave,
+    // so that the four SIMDs share it
+    const int pr = lane < GS / 4 ? wv * (GS / 4) + lane : -1;
 
     // ---- forward weight slices: W1^T in registers; W0^T and W2^T (used by one
     // step each) are re-read from L2 per group, which keeps the backward's
@@ -224,23 +273,23 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
 
     // ---- prefetch of the first group ----
     Prefetch pf;
-    pf.xyze = pf.atwr = pf.rayg = pf.ctot = pf.cinc = pf.rgb = z4;
+    pf.xyze = pf.atwr = pf.rayg = pf.ctot = pf.cinc = pf.rgb = pf.vd = z4;
     pf.tlast = 0.f;
     pf.f0 = pf.f1 = pf.f2 = zu4;
     auto stage_a = [&](int64_t grp) {
-        const int64_t q = grp * GS + tid;
+        const int64_t q = grp * GS + pr;
         pf.xyze = pf.atwr = z4;
-        if (tid < GS && grp < ngroups && q < total) {
+        if (pr >= 0 && grp < ngroups && q < total) {
             pf.xyze = rec.xyze[q];
             pf.atwr = rec.atwr[q];
         }
     };
     auto stage_b = [&](int64_t grp) {  // backward only; needs stage A's atwr
-        const int64_t q = grp * GS + tid;
-        pf.rayg = pf.ctot = pf.cinc = pf.rgb = z4;
+        const int64_t q = grp * GS + pr;
+        pf.rayg = pf.ctot = pf.cinc = pf.rgb = pf.vd = z4;
         pf.tlast = 0.f;
         pf.f0 = pf.f1 = pf.f2 = zu4;
-        if (tid < GS && grp < ngroups && q < total) {
+        if (pr >= 0 && grp < ngroups && q < total) {
             const uint32_t bits = __float_as_uint(pf.atwr.w);
             if (bits & (kAliveA | kAliveW)) {
                 const uint32_t ray = bits & kRayMask;
@@ -250,6 +299,9 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                 pf.cinc = rec.cinc[q];
             }
             if (bits & kAliveW) {
+                const uint32_t ray = bits & kRayMask;
+                pf.vd = make_float4(P.viewdirs[3 * ray], P.viewdirs[3 * ray + 1],
+                                    P.viewdirs[3 * ray + 2], 0.f);
                 pf.rgb = rec.rgb[q];
                 pf.f0 = rec.feat[3 * q];
                 pf.f1 = rec.feat[3 * q + 1];
@@ -262,7 +314,7 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
     // buffer `b`, and its kept features into the next-buffer
     auto park = [&](int b) {
         if constexpr (BWD) {
-            if (tid < GS) {
+            if (pr >= 0) {
                 const uint32_t bits = __float_as_uint(pf.atwr.w);
                 const float4 gr = pf.rayg;
                 float gl[3] = {0.f, 0.f, 0.f};
@@ -283,7 +335,8 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                     const float ga = gw * T - back / ((1.f - alpha) + 1e-10f);
                     gs = fminf(e, 1e10f) * powf(1.f + e, -P.interval - 1.f) * P.interval * ga;
                 }
-                sm.gsig[b][tid] = gs;
+                sm.gsig[b][pr] = gs;
+                sm.vd[b][pr] = pf.vd;
                 db2[0] += gl[0];
                 db2[1] += gl[1];
                 db2[2] += gl[2];
@@ -291,13 +344,13 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                 for (int o = 0; o < 3; ++o) {  // logits 3..15 stay zero
                     uint16_t h, l;
                     split_bf(gl[o], h, l);
-                    sm.GLh[b][tid * GM + o] = h;
-                    sm.GLl[b][tid * GM + o] = l;
-                    sm.GLf[b][o * FM + tid] = h;
+                    sm.GLh[b][pr * GM + o] = h;
+                    sm.GLl[b][pr * GM + o] = l;
+                    sm.GLf[b][o * FM + pr] = h;
                 }
-                sm.nfeat[tid][0] = pf.f0;
-                sm.nfeat[tid][1] = pf.f1;
-                sm.nfeat[tid][2] = pf.f2;
+                sm.nfeat[pr][0] = pf.f0;
+                sm.nfeat[pr][1] = pf.f1;
+                sm.nfeat[pr][2] = pf.f2;
             }
         }
     };
@@ -321,7 +374,7 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
     for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {  // uniform bound
         const int64_t base = grp * GS;
         // ---- 1. commit the prefetched group to LDS; start loading the next one ----
-        if (tid < GS) {
+        if (pr >= 0) {
             RecInfo ri;
             ri.x = pf.xyze.x;
             ri.y = pf.xyze.y;
@@ -331,20 +384,20 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
             ri.T = pf.atwr.y;
             ri.w = pf.atwr.z;
             ri.bits = __float_as_uint(pf.atwr.w);  // zero for records past the end
-            sm.ri[tid] = ri;
+            sm.ri[pr] = ri;
             if constexpr (BWD) {
                 // kept features: hi halves 0..11 then lo halves 0..11
-                const uint4 f0 = sm.nfeat[tid][0], f1 = sm.nfeat[tid][1], f2 = sm.nfeat[tid][2];
+                const uint4 f0 = sm.nfeat[pr][0], f1 = sm.nfeat[pr][1], f2 = sm.nfeat[pr][2];
                 const uint32_t w12[12] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y,
                                           f1.z, f1.w, f2.x, f2.y, f2.z, f2.w};
-                uint32_t* xh = reinterpret_cast<uint32_t*>(&sm.Xh[tid * XM]);
-                uint32_t* xl = reinterpret_cast<uint32_t*>(&sm.Xl[tid * XM]);
+                uint32_t* xh = reinterpret_cast<uint32_t*>(&sm.Xh[pr * XM]);
+                uint32_t* xl = reinterpret_cast<uint32_t*>(&sm.Xl[pr * XM]);
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
                     xh[c] = w12[c];
                     xl[c] = w12[6 + c];
-                    sm.Xf[(2 * c) * FM + tid] = (uint16_t)(w12[c] & 0xffffu);
-                    sm.Xf[(2 * c + 1) * FM + tid] = (uint16_t)(w12[c] >> 16);
+                    sm.Xf[(2 * c) * FM + pr] = (uint16_t)(w12[c] & 0xffffu);
+                    sm.Xf[(2 * c + 1) * FM + pr] = (uint16_t)(w12[c] >> 16);
                 }
             }
         }
@@ -407,13 +460,7 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                 if (part == 0) {
                     uint32_t wh[6], wl[6];
 #pragma unroll
-                    for (int c = 0; c < 6; ++c) {
-                        uint16_t h0, l0, h1, l1;
-                        split_bf(v[2 * c], h0, l0);
-                        split_bf(v[2 * c + 1], h1, l1);
-                        wh[c] = (uint32_t)h0 | ((uint32_t)h1 << 16);
-                        wl[c] = (uint32_t)l0 | ((uint32_t)l1 << 16);
-                    }
+                    for (int c = 0; c < 6; ++c) split2(v[2 * c], v[2 * c + 1], wh[c], wl[c]);
                     uint32_t* xh = reinterpret_cast<uint32_t*>(&sm.Xh[s * XM]);
                     uint32_t* xl = reinterpret_cast<uint32_t*>(&sm.Xl[s * XM]);
 #pragma unroll
@@ -432,7 +479,12 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
             // positional encoding of the ray's view direction (dvgo.py:354-358) and the
             // bias input; columns 40..63 are zero
             float vd[3] = {0.f, 0.f, 0.f};
-            if (alive) {
+            if constexpr (BWD) {  // prefetched (zero unless alive)
+                const float4 v4 = sm.vd[cb][s];
+                vd[0] = v4.x;
+                vd[1] = v4.y;
+                vd[2] = v4.z;
+            } else if (alive) {
                 const uint32_t ray = ri.bits & kRayMask;
                 vd[0] = P.viewdirs[3 * ray];
                 vd[1] = P.viewdirs[3 * ray + 1];
@@ -491,11 +543,10 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                     h[i] = fmaxf(acc[i] + b0v[mt][i], 0.f);
                     if (h[i] > 0.f) m0 |= 1u << ((mt * NT + nt) * 4 + i);
                 }
-                st4s(&sm.A1h[(16 * nt + r16) * SM + fr], &sm.A1l[(16 * nt + r16) * SM + fr], h);
-                if constexpr (BWD) {
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) sm.H0f[(fr + i) * FM + 16 * nt + r16] = f2bf_bits(h[i]);
-                }
+                if constexpr (BWD)
+                    st_act<true>(sm.A1h, sm.A1l, sm.H0f, 16 * nt + r16, fr, lane, h);
+                else
+                    st_act<false>(sm.A1h, sm.A1l, nullptr, 16 * nt + r16, fr, lane, h);
             }
         }
         // the next group's stage-A loads (issued in step 1) have landed: start stage B
@@ -519,11 +570,10 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                     h[i] = fmaxf(acc[i] + b1v[mt][i], 0.f);
                     if (h[i] > 0.f) m1 |= 1u << ((mt * NT + nt) * 4 + i);
                 }
-                st4s(&sm.A2h[(16 * nt + r16) * SM + fr], &sm.A2l[(16 * nt + r16) * SM + fr], h);
-                if constexpr (BWD) {
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) sm.F2[(fr + i) * FM + 16 * nt + r16] = f2bf_bits(h[i]);
-                }
+                if constexpr (BWD)
+                    st_act<true>(sm.A2h, sm.A2l, sm.F2, 16 * nt + r16, fr, lane, h);
+                else
+                    st_act<false>(sm.A2h, sm.A2l, nullptr, 16 * nt + r16, fr, lane, h);
             }
         }
         __syncthreads();
@@ -573,10 +623,7 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                         gv[i] = (m1 >> ((mt * NT + nt) * 4 + i)) & 1u ? acc[i] : 0.f;
                         db1[mt][i] += gv[i];
                     }
-                    st4s(&sm.A1h[(16 * nt + r16) * SM + fr], &sm.A1l[(16 * nt + r16) * SM + fr], gv);
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        sm.G1f[(fr + i) * FM + 16 * nt + r16] = f2bf_bits(gv[i]);
+                    st_act<true>(sm.A1h, sm.A1l, sm.G1f, 16 * nt + r16, fr, lane, gv);
                 }
                 // dW2[in = f0+16mt+row][out] += sum over the group's samples H1[s][in] gL[s][out]
                 const int ho = (f0 + 16 * mt + r16) * FM + 8 * g, lo_ = r16 * FM + 8 * g;
@@ -607,10 +654,7 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
                         gv[i] = (m0 >> ((mt * NT + nt) * 4 + i)) & 1u ? acc[nt][i] : 0.f;
-                    st4s(&sm.A2h[(16 * nt + r16) * SM + fr], &sm.A2l[(16 * nt + r16) * SM + fr], gv);
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        sm.F2[(fr + i) * FM + 16 * nt + r16] = f2bf_bits(gv[i]);
+                    st_act<true>(sm.A2h, sm.A2l, sm.F2, 16 * nt + r16, fr, lane, gv);
                 }
             }
             {
@@ -676,51 +720,54 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
         }
     }
 
-    if (!BWD) return;
-    // ---- flush this workgroup's dW partial ----
-    float* out = dw_partial + (int64_t)blockIdx.x * kMlpParams;
+    if constexpr (BWD) {
+        // ---- flush this workgroup's dW partial ----
+        float* out = dw_partial + (int64_t)blockIdx.x * kMlpParams;
 #pragma unroll
-    for (int mi = 0; mi < 3; ++mi)
+        for (int mi = 0; mi < 3; ++mi)
 #pragma unroll
-        for (int nt = 0; nt < 2; ++nt)
+            for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int in = 16 * mi + 4 * g + i, o = f0 + 16 * nt + r16;
+                    if (in < 39)
+                        out[kW0 + in * 128 + o] = dW0[mi][nt][i];
+                    else if (in == 39)
+                        out[kB0 + o] = dW0[mi][nt][i];
+                }
+#pragma unroll
+        for (int mi = 0; mi < 8; ++mi)
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    out[kW1 + (16 * mi + 4 * g + i) * 128 + f0 + 16 * nt + r16] = dW1[mi][nt][i];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const int in = 16 * mi + 4 * g + i, o = f0 + 16 * nt + r16;
-                if (in < 39)
-                    out[kW0 + in * 128 + o] = dW0[mi][nt][i];
-                else if (in == 39)
-                    out[kB0 + o] = dW0[mi][nt][i];
+                if (r16 < 3) out[kW2 + (f0 + 16 * mt + 4 * g + i) * 3 + r16] = dW2[mt][i];
+                float v = db1[mt][i];
+                v += __shfl_xor(v, 1, 64);
+                v += __shfl_xor(v, 2, 64);
+                v += __shfl_xor(v, 4, 64);
+                v += __shfl_xor(v, 8, 64);
+                if (r16 == 0) out[kB1 + f0 + 16 * mt + 4 * g + i] = v;
             }
-#pragma unroll
-    for (int mi = 0; mi < 8; ++mi)
-#pragma unroll
-        for (int nt = 0; nt < 2; ++nt)
-#pragma unroll
-            for (int i = 0; i < 4; ++i)
-                out[kW1 + (16 * mi + 4 * g + i) * 128 + f0 + 16 * nt + r16] = dW1[mi][nt][i];
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (r16 < 3) out[kW2 + (f0 + 16 * mt + 4 * g + i) * 3 + r16] = dW2[mt][i];
-            float v = db1[mt][i];
-            v += __shfl_xor(v, 1, 64);
-            v += __shfl_xor(v, 2, 64);
-            v += __shfl_xor(v, 4, 64);
-            v += __shfl_xor(v, 8, 64);
-            if (r16 == 0) out[kB1 + f0 + 16 * mt + 4 * g + i] = v;
-        }
-    if (wv == 0) {
+        // db2: lanes 0..7 of every wave accumulated it (park); the loop ended on a
+        // barrier, so gsig is free for the per-wave sums
 #pragma unroll
         for (int o = 0; o < 3; ++o) {
-            float v = tid < GS ? db2[o] : 0.f;  // step 6 ran on threads 0..GS-1
+            float v = pr >= 0 ? db2[o] : 0.f;
             v += __shfl_xor(v, 1, 64);
             v += __shfl_xor(v, 2, 64);
             v += __shfl_xor(v, 4, 64);
-            v += __shfl_xor(v, 8, 64);
-            v += __shfl_xor(v, 16, 64);
-            if (tid == 0) out[kB2 + o] = v;
+            if (lane == 0) sm.gsig[0][4 * wv + o] = v;
         }
+        __syncthreads();
+        if (tid < 3)
+            out[kB2 + tid] = (sm.gsig[0][tid] + sm.gsig[0][4 + tid]) +
+                             (sm.gsig[0][8 + tid] + sm.gsig[0][12 + tid]);
     }
 }
 
