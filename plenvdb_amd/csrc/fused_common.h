@@ -9,10 +9,12 @@
 //   k_mlp<false>   workgroup per 32-record group: colour gather + MFMA MLP -> rgb,
 //                  the features are kept per record as bf16 (hi, lo) pairs
 //   k_ray_composite wave per ray: prefix of w*rgb, ray colour, MSE loss and its
-//                  gradient per ray (k_loss_reduce sums the per-ray losses)
+//                  gradient per ray (k_loss_reduce sums the per-ray losses); a
+//                  second pass forms each sample's dL/dlogit and, through the
+//                  alpha/raw2alpha adjoints, dL/dsigma
 //   k_mlp<true>    workgroup per 32-record group: recompute from the kept
-//                  features, MFMA MLP backward, alpha/raw2alpha adjoints, dW;
-//                  per-record feature and sigma gradients
+//                  features, MFMA MLP backward, dW; per-record feature and
+//                  sigma gradients
 //   k_scatter_agg  workgroup per 128 records: corner contributions summed per
 //                  slot in an LDS hash, one device atomic per distinct slot
 //   k_dw_reduce, k_sparse_adam, dense Adam of the MLP, k_pack_mlp
@@ -50,7 +52,7 @@ struct Records {
     float4* xyze;      // index-space x, y, z; e = exp(sigma + shift)
     float4* atwr;      // alpha, T, w, ray|flags (as bits)
     float4* rgb;       // MLP colour (rgb), written by the forward MLP pass
-    float4* cinc;      // inclusive prefix of w*rgb along the ray
+    float4* adj;       // per-sample adjoints: dL/dlogit (3), dL/dsigma
     uint4* feat;       // [3 per record] the 12 gathered colour features as bf16 hi, then lo
     float4* grad;      // [4 per record] dL/dfeature (12) and dL/dsigma
 };

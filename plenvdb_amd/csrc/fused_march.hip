@@ -222,22 +222,74 @@ __global__ void __launch_bounds__(256) k_ray_composite(StepParams P, RayBuffers 
                 vb += o2;
             }
         }
-        if (valid) rec.cinc[q] = make_float4(cr + vr, cg + vg, cb + vb, 0.f);
         cr += __shfl(vr, 63, 64);
         cg += __shfl(vg, 63, 64);
         cb += __shfl(vb, 63, 64);
     }
+    // the ray's colour, loss and dL/drgb (every lane holds them)
+    const float tl = R.t_last[r];
+    const float o0 = cr + tl * P.bg, o1 = cg + tl * P.bg, o2 = cb + tl * P.bg;
+    const float d0 = o0 - P.target[3 * r], d1 = o1 - P.target[3 * r + 1],
+                d2 = o2 - P.target[3 * r + 2];
+    const float g0 = 2.f * d0 * P.loss_scale, g1 = 2.f * d1 * P.loss_scale,
+                g2 = 2.f * d2 * P.loss_scale;
+    const float gbg = P.bg * (g0 + g1 + g2);  // dL/dT_last
     if (lane == 0) {
-        const float tl = R.t_last[r];
-        const float o0 = cr + tl * P.bg, o1 = cg + tl * P.bg, o2 = cb + tl * P.bg;
-        const float d0 = o0 - P.target[3 * r], d1 = o1 - P.target[3 * r + 1],
-                    d2 = o2 - P.target[3 * r + 2];
-        const float g0 = 2.f * d0 * P.loss_scale, g1 = 2.f * d1 * P.loss_scale,
-                    g2 = 2.f * d2 * P.loss_scale;
         R.ray_out[r] = make_float4(o0, o1, o2, tl);
-        R.ray_grad[r] = make_float4(g0, g1, g2, P.bg * (g0 + g1 + g2));
+        R.ray_grad[r] = make_float4(g0, g1, g2, gbg);
         R.ray_ctot[r] = make_float4(cr, cg, cb, 0.f);
         R.ray_loss[r] = (d0 * d0 + d1 * d1 + d2 * d2) * P.loss_scale;
+    }
+    // second pass: each sample's adjoints.  dL/dlogit = w * dL/drgb * rgb(1 - rgb)
+    // (sigmoid, dvgo.py:362); dL/dsigma through Alphas2Weights and raw2alpha
+    // (render_utils_kernel.cu:653-676, 506-517), with the exclusive suffix of
+    // w*rgb taken as the ray total minus the inclusive prefix
+    float pr = 0.f, pg = 0.f, pb = 0.f;
+    for (int base = 0; base < cnt; base += 64) {
+        const int j = base + lane;
+        const int64_t q = off + j;
+        const bool valid = j < cnt && q < P.capacity;
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), c = a;
+        float e = 0.f;
+        uint32_t bits = 0u;
+        float vr = 0.f, vg = 0.f, vb = 0.f;
+        if (valid) {
+            a = rec.atwr[q];
+            bits = __float_as_uint(a.w);
+            if (bits & kAliveW) {
+                c = rec.rgb[q];
+                vr = a.z * c.x;
+                vg = a.z * c.y;
+                vb = a.z * c.z;
+            }
+            if (bits & kAliveA) e = rec.xyze[q].w;
+        }
+        for (int d = 1; d < 64; d <<= 1) {  // the same scan as the first pass
+            const float o0_ = __shfl_up(vr, d, 64), o1_ = __shfl_up(vg, d, 64),
+                        o2_ = __shfl_up(vb, d, 64);
+            if (lane >= d) {
+                vr += o0_;
+                vg += o1_;
+                vb += o2_;
+            }
+        }
+        float gl0 = 0.f, gl1 = 0.f, gl2 = 0.f, gw = 0.f, gs = 0.f;
+        if (bits & kAliveW) {
+            gl0 = a.z * g0 * c.x * (1.f - c.x);
+            gl1 = a.z * g1 * c.y * (1.f - c.y);
+            gl2 = a.z * g2 * c.z * (1.f - c.z);
+            gw = g0 * c.x + g1 * c.y + g2 * c.z;
+        }
+        if (bits & kAliveA) {
+            const float back = gbg * tl + g0 * (cr - (pr + vr)) + g1 * (cg - (pg + vg)) +
+                               g2 * (cb - (pb + vb));
+            const float ga = gw * a.y - back / ((1.f - a.x) + 1e-10f);
+            gs = fminf(e, 1e10f) * powf(1.f + e, -P.interval - 1.f) * P.interval * ga;
+        }
+        if (valid) rec.adj[q] = make_float4(gl0, gl1, gl2, gs);
+        pr += __shfl(vr, 63, 64);
+        pg += __shfl(vg, 63, 64);
+        pb += __shfl(vb, 63, 64);
     }
 }
 

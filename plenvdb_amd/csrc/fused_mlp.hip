@@ -15,19 +15,18 @@
 // bring them under 1e-3).  dW products (activation x gradient) take the hi
 // parts only.
 // The forward weight slices stay in registers; the backward weights sit in LDS.  The
-// backward pass recomputes the forward, forms dL/dlogit from the per-ray loss
-// gradient, runs g_H1^T = W2 g_L^T, g_H0^T = W1 g_H1^T (masked by the ReLUs,
-// kept as bit masks), g_X^T = W0[0:12] g_H0^T, and accumulates dW over the
-// group's 32 samples (K = 32) into per-wave accumulator tiles written once per
-// workgroup.  The same pass evaluates the Alphas2Weights / raw2alpha adjoints
-// (render_utils_kernel.cu:653-676, 506-517) and writes each record's feature
-// and sigma gradients; the scatter (fused_scatter.hip) then adds
-// them to the grids with the "leaf must exist" rule (densityvdb.cu:16-27,
-// colorvdb.cu:28-37).  The forward pass keeps the gathered features per record
-// as (hi, lo) bf16, so the backward reads 48 bytes per record instead of
-// redoing 8 gathers.
-// Each group's records, features and ray data are loaded into registers while
-// the previous group computes.
+// backward pass recomputes H0 and H1 (for the ReLU masks and the dW operands),
+// takes each sample's dL/dlogit and dL/dsigma from the composite pass, runs
+// g_H1^T = W2 g_L^T, g_H0^T = W1 g_H1^T (masked by the ReLUs, kept as bit
+// masks), g_X^T = W0[0:12] g_H0^T, and accumulates dW over the group's 32
+// samples (K = 32) into per-wave accumulator tiles written once per workgroup.
+// It writes each record's feature and sigma gradients; the scatter
+// (fused_scatter.hip) then adds them to the grids with the "leaf must exist"
+// rule (densityvdb.cu:16-27, colorvdb.cu:28-37).  The forward pass keeps the
+// gathered features per record as (hi, lo) bf16, so the backward reads 48
+// bytes per record instead of redoing 8 gathers.  Each group's records,
+// features and view directions are loaded into registers while the previous
+// group computes.
 #include <type_traits>
 
 #include "fused_common.h"
@@ -164,9 +163,9 @@ constexpr int kBw2Frags = 8 * 64, kBw1Frags = 8 * 4 * 64, kBw0Frags = 4 * 64;
 // weight products); feature-major copies (dW operands) keep the hi part.  A1
 // holds H0 (steps 3-4) and then g_H1 (7-8); A2 holds H1 (4) and then g_H0
 // (8-9); F2 holds H1 feature-major (4-7) and then g_H0 (8-9).  Barriers
-// separate every reuse.  dL/dlogit and dL/dsigma are double-buffered: they
-// are formed for the next group (from its prefetched records) while the
-// current group is in step 7.
+// separate every reuse.  dL/dlogit, dL/dsigma and the view directions are
+// double-buffered: the next group's are staged from its prefetched records
+// while the current group is in step 7.
 struct alignas(16) SmemBwd {
     uint4 Bw1[kBw1Frags];
     uint4 Bw0[kBw0Frags];
@@ -191,12 +190,11 @@ __device__ __forceinline__ float sigmoidf_(float x) { return 1.f / (1.f + expf(-
 
 // Register prefetch of one group's per-record inputs (thread = record).
 struct Prefetch {
-    float4 xyze, atwr;          // stage A: the record
-    float4 rayg, ctot, cinc;    // stage B (backward): ray data, prefix
-    float4 rgb;                 // stage B (backward): the forward pass's colour
-    float4 vd;                  // stage B (backward): the ray's view direction
-    float tlast;
-    uint4 f0, f1, f2;           // stage B (backward): kept features (hi 12, lo 12)
+    float4 xyze;        // stage A (forward): position, exp(sigma + shift)
+    float4 atwr;        // stage A: alpha, T, w, ray id and flags
+    float4 adj;         // stage A (backward): dL/dlogit (3), dL/dsigma
+    float4 vd;          // stage B (backward): the ray's view direction
+    uint4 f0, f1, f2;   // stage B (backward): kept features (hi 12, lo 12)
 };
 
 template <bool BWD, bool BF16COL>
@@ -273,68 +271,42 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
 
     // ---- prefetch of the first group ----
     Prefetch pf;
-    pf.xyze = pf.atwr = pf.rayg = pf.ctot = pf.cinc = pf.rgb = pf.vd = z4;
-    pf.tlast = 0.f;
+    pf.xyze = pf.atwr = pf.adj = pf.vd = z4;
     pf.f0 = pf.f1 = pf.f2 = zu4;
     auto stage_a = [&](int64_t grp) {
         const int64_t q = grp * GS + pr;
-        pf.xyze = pf.atwr = z4;
+        pf.xyze = pf.atwr = pf.adj = z4;
         if (pr >= 0 && grp < ngroups && q < total) {
-            pf.xyze = rec.xyze[q];
             pf.atwr = rec.atwr[q];
+            if constexpr (BWD)
+                pf.adj = rec.adj[q];
+            else
+                pf.xyze = rec.xyze[q];
         }
     };
     auto stage_b = [&](int64_t grp) {  // backward only; needs stage A's atwr
         const int64_t q = grp * GS + pr;
-        pf.rayg = pf.ctot = pf.cinc = pf.rgb = pf.vd = z4;
-        pf.tlast = 0.f;
+        pf.vd = z4;
         pf.f0 = pf.f1 = pf.f2 = zu4;
         if (pr >= 0 && grp < ngroups && q < total) {
             const uint32_t bits = __float_as_uint(pf.atwr.w);
-            if (bits & (kAliveA | kAliveW)) {
-                const uint32_t ray = bits & kRayMask;
-                pf.rayg = R.ray_grad[ray];
-                pf.ctot = R.ray_ctot[ray];
-                pf.tlast = R.t_last[ray];
-                pf.cinc = rec.cinc[q];
-            }
             if (bits & kAliveW) {
                 const uint32_t ray = bits & kRayMask;
                 pf.vd = make_float4(P.viewdirs[3 * ray], P.viewdirs[3 * ray + 1],
                                     P.viewdirs[3 * ray + 2], 0.f);
-                pf.rgb = rec.rgb[q];
                 pf.f0 = rec.feat[3 * q];
                 pf.f1 = rec.feat[3 * q + 1];
                 pf.f2 = rec.feat[3 * q + 2];
             }
         }
     };
-    // backward: the prefetched group's per-sample adjoints (dL/dlogit from the
-    // per-ray dL/drgb, dL/dsigma through Alphas2Weights and raw2alpha) into LDS
-    // buffer `b`, and its kept features into the next-buffer
+    // backward: the prefetched group's per-sample adjoints (formed by the
+    // composite pass) into LDS buffer `b`, its kept features into the next-buffer
     auto park = [&](int b) {
         if constexpr (BWD) {
             if (pr >= 0) {
-                const uint32_t bits = __float_as_uint(pf.atwr.w);
-                const float4 gr = pf.rayg;
-                float gl[3] = {0.f, 0.f, 0.f};
-                float gw = 0.f;
-                if (bits & kAliveW) {
-                    const float c0 = pf.rgb.x, c1 = pf.rgb.y, c2 = pf.rgb.z, w = pf.atwr.z;
-                    gl[0] = w * gr.x * c0 * (1.f - c0);
-                    gl[1] = w * gr.y * c1 * (1.f - c1);
-                    gl[2] = w * gr.z * c2 * (1.f - c2);
-                    gw = gr.x * c0 + gr.y * c1 + gr.z * c2;
-                }
-                float gs = 0.f;
-                if (bits & kAliveA) {
-                    const float4 ct = pf.ctot, ci = pf.cinc;
-                    const float e = pf.xyze.w, alpha = pf.atwr.x, T = pf.atwr.y;
-                    const float back = gr.w * pf.tlast + gr.x * (ct.x - ci.x) +
-                                       gr.y * (ct.y - ci.y) + gr.z * (ct.z - ci.z);
-                    const float ga = gw * T - back / ((1.f - alpha) + 1e-10f);
-                    gs = fminf(e, 1e10f) * powf(1.f + e, -P.interval - 1.f) * P.interval * ga;
-                }
+                const float gl[3] = {pf.adj.x, pf.adj.y, pf.adj.z};
+                const float gs = pf.adj.w;
                 sm.gsig[b][pr] = gs;
                 sm.vd[b][pr] = pf.vd;
                 db2[0] += gl[0];
@@ -603,9 +575,9 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
             __syncthreads();
             continue;
         } else {
-            // no logits here: the per-sample adjoints were formed in park() from
-            // the rgb the forward pass stored (same inputs through the same code,
-            // so the ReLU masks recomputed here are the forward's)
+            // no logits here: the composite pass formed the per-sample adjoints
+            // from the rgb the forward pass stored (same inputs through the same
+            // code, so the ReLU masks recomputed here are the forward's)
             // ---- 7. g_H1^T slice = W2[slice] g_L^T, masked -> A1 (and G1f); dW2 ----
 #pragma unroll
             for (int mt = 0; mt < 2; ++mt) {

@@ -38,7 +38,7 @@ constexpr int kMaxMlpGrid = 512;
 
 struct Layout {
     size_t count, offset, total, t_last, ray_out, ray_grad, ray_ctot, ray_loss, loss;
-    size_t xyze, atwr, rgb, cinc, feat, grad;
+    size_t xyze, atwr, rgb, adj, feat, grad;
     size_t touched, dw_part, w0t, w1t, w2t, w2, w1, w0;
     size_t bytes;
 };
@@ -65,7 +65,7 @@ Layout layout_for(int64_t n_rays, int64_t capacity, int64_t nleaf) {
     L.xyze = take(16 * capacity);
     L.atwr = take(16 * capacity);
     L.rgb = take(16 * capacity);
-    L.cinc = take(16 * capacity);
+    L.adj = take(16 * capacity);
     L.feat = take(48 * capacity);
     L.grad = take(64 * capacity);
     L.touched = take(nleaf > 0 ? nleaf : 1);
@@ -315,7 +315,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     rec.xyze = reinterpret_cast<float4*>(ws + L.xyze);
     rec.atwr = reinterpret_cast<float4*>(ws + L.atwr);
     rec.rgb = reinterpret_cast<float4*>(ws + L.rgb);
-    rec.cinc = reinterpret_cast<float4*>(ws + L.cinc);
+    rec.adj = reinterpret_cast<float4*>(ws + L.adj);
     rec.feat = reinterpret_cast<uint4*>(ws + L.feat);
     rec.grad = reinterpret_cast<float4*>(ws + L.grad);
     uint8_t* touched = ws + L.touched;
@@ -419,7 +419,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
 }
 
 int pvdb_fused_records(const pvdb_topology* topo, const pvdb_step_desc* d, void** xyze,
-                       void** atwr, void** rgb, void** cinc, int32_t** count, int32_t** offset,
+                       void** atwr, void** rgb, void** adj, int32_t** count, int32_t** offset,
                        int32_t** total) {
     if (!topo) {
         pvdb::set_error("null topology");
@@ -431,7 +431,7 @@ int pvdb_fused_records(const pvdb_topology* topo, const pvdb_step_desc* d, void*
     if (xyze) *xyze = ws + L.xyze;
     if (atwr) *atwr = ws + L.atwr;
     if (rgb) *rgb = ws + L.rgb;
-    if (cinc) *cinc = ws + L.cinc;
+    if (adj) *adj = ws + L.adj;
     if (count) *count = reinterpret_cast<int32_t*>(ws + L.count);
     if (offset) *offset = reinterpret_cast<int32_t*>(ws + L.offset);
     if (total) *total = reinterpret_cast<int32_t*>(ws + L.total);

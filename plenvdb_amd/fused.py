@@ -248,7 +248,7 @@ class FusedTrainer:
         total = _read_i32(ptrs[6].value, 1)[0]
         total = min(total, self.capacity)
         out = {"total": total, "count": _read_i32(ptrs[4].value, n), "offset": _read_i32(ptrs[5].value, n)}
-        for k, p in zip(("xyze", "atwr", "rgb", "cinc"), ptrs[:4]):
+        for k, p in zip(("xyze", "atwr", "rgb", "adj"), ptrs[:4]):
             out[k] = _read_f32(p.value, 4 * total).reshape(total, 4)
         return out
 
