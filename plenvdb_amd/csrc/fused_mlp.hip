@@ -327,7 +327,14 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
         }
     };
     stage_a(blockIdx.x);
+    // X columns 40..63 (and, backward, feature-major rows 40..47) are zero throughout
+    for (int e = tid; e < GS * 24; e += 256) {
+        const int row = e / 24, col = 40 + e % 24;
+        sm.Xh[row * XM + col] = 0;
+        sm.Xl[row * XM + col] = 0;
+    }
     if constexpr (BWD) {
+        for (int e = tid; e < (XF_ROWS - 40) * FM; e += 256) sm.Xf[40 * FM + e] = 0;
         // the unused logits (3..15) of both dL/dlogit buffers are zero throughout
         for (int e = tid; e < 2 * GS * GM; e += 256) {
             if (e % GM >= 3) {
@@ -448,71 +455,92 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                     }
                 }
             }
-            // positional encoding of the ray's view direction (dvgo.py:354-358) and the
-            // bias input; columns 40..63 are zero
+        }
+        // positional encoding of the ray's view direction (dvgo.py:354-358) and the
+        // bias input: thread (sample ps, part pp < 7) writes columns 12 + 4pp .. +3.
+        // Columns 40..63 were zeroed once.  sin/cos on the hardware units: their
+        // error (about 1e-6 absolute for these arguments, |arg| <= 8) is below what
+        // the (hi, lo) pair keeps.
+        {
+            const int ps = tid & (GS - 1), pp = tid >> 5;
+            const bool alive = (sm.ri[ps].bits & kAliveW) != 0u;
             float vd[3] = {0.f, 0.f, 0.f};
             if constexpr (BWD) {  // prefetched (zero unless alive)
-                const float4 v4 = sm.vd[cb][s];
+                const float4 v4 = sm.vd[cb][ps];
                 vd[0] = v4.x;
                 vd[1] = v4.y;
                 vd[2] = v4.z;
             } else if (alive) {
-                const uint32_t ray = ri.bits & kRayMask;
+                const uint32_t ray = sm.ri[ps].bits & kRayMask;
                 vd[0] = P.viewdirs[3 * ray];
                 vd[1] = P.viewdirs[3 * ray + 1];
                 vd[2] = P.viewdirs[3 * ray + 2];
             }
+            float val[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const int p = part * 4 + k;  // 0..27: PE 0..26, then the bias column
-                if (p < 28) {
-                    float val = 0.f;
-                    if (alive) {
-                        if (p < 3) {
-                            val = vd[p];
-                        } else if (p < 27) {
-                            const int idx = p - 3, axis = (idx % 12) / 4, fr = idx % 4;
-                            const float arg = vd[axis] * (float)(1 << fr);
-                            val = idx < 12 ? sinf(arg) : cosf(arg);
-                        } else {
-                            val = 1.f;
-                        }
+                const int p = pp * 4 + k;  // 0..27: PE 0..26, then the bias column
+                float v = 0.f;
+                if (alive) {
+                    if (p < 3) {
+                        v = vd[p];
+                    } else if (p < 27) {
+                        const int idx = p - 3, axis = (idx % 12) / 4, fr = idx % 4;
+                        const float arg = vd[axis] * (float)(1 << fr);
+                        v = idx < 12 ? __sinf(arg) : __cosf(arg);
+                    } else {
+                        v = 1.f;
                     }
-                    uint16_t h, l;
-                    split_bf(val, h, l);
-                    sm.Xh[s * XM + 12 + p] = h;
-                    sm.Xl[s * XM + 12 + p] = l;
-                    if constexpr (BWD) sm.Xf[(12 + p) * FM + s] = h;
                 }
+                val[k] = v;
             }
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const int col = 40 + part * 3 + k;
-                sm.Xh[s * XM + col] = 0;
-                sm.Xl[s * XM + col] = 0;
-                if constexpr (BWD) {
-                    if (col < XF_ROWS) sm.Xf[col * FM + s] = 0;
-                }
+            uint2 hi, lo;
+            split2(val[0], val[1], hi.x, lo.x);
+            split2(val[2], val[3], hi.y, lo.y);
+            if (pp < 7) {
+                *reinterpret_cast<uint2*>(&sm.Xh[ps * XM + 12 + 4 * pp]) = hi;
+                *reinterpret_cast<uint2*>(&sm.Xl[ps * XM + 12 + 4 * pp]) = lo;
+            }
+            if constexpr (BWD) {
+                const uint2 t = quad_transpose(hi, lane & 3);  // the quad: 4 samples, one part
+                if (pp < 7)
+                    *reinterpret_cast<uint2*>(&sm.Xf[(12 + 4 * pp + (lane & 3)) * FM + (ps & ~3)]) = t;
             }
         }
         __syncthreads();
         // ---- 3. H0^T slice = relu(W0^T X^T + b0) -> A1 (and H0f) ----
         uint32_t m0 = 0u, m1 = 0u;  // ReLU masks: bit (mt*NT + nt)*4 + i
+        // the weight products run k-step outer: each B fragment is read once and
+        // the four (mt, nt) accumulator chains interleave
+        f32x4 acc[2][NT];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 2; ++kt) {
+            bf16x8 bh[NT], bl[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int xo = (16 * nt + r16) * XM + 32 * kt + 8 * g;
+                bh[nt] = ld_frag(&sm.Xh[xo]);
+                bl[nt] = ld_frag(&sm.Xl[xo]);
+            }
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mt][nt] = mfma_w(A0[mt][kt], bh[nt], bl[nt], acc[mt][nt]);
+        }
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
             const int fr = f0 + 16 * mt + 4 * g;
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
-                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-                const int xo = (16 * nt + r16) * XM + 8 * g;
-#pragma unroll
-                for (int kt = 0; kt < 2; ++kt)
-                    acc = mfma_w(A0[mt][kt], ld_frag(&sm.Xh[xo + 32 * kt]),
-                                 ld_frag(&sm.Xl[xo + 32 * kt]), acc);
                 float h[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    h[i] = fmaxf(acc[i] + b0v[mt][i], 0.f);
+                    h[i] = fmaxf(acc[mt][nt][i] + b0v[mt][i], 0.f);
                     if (h[i] > 0.f) m0 |= 1u << ((mt * NT + nt) * 4 + i);
                 }
                 if constexpr (BWD)
@@ -526,20 +554,33 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
         __syncthreads();
         // ---- 4. H1^T slice = relu(W1^T H0^T + b1) -> A2 (and H1f in F2) ----
 #pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) {
+            bf16x8 bh[NT], bl[NT];
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int ho = (16 * nt + r16) * SM + 32 * kt + 8 * g;
+                bh[nt] = ld_frag(&sm.A1h[ho]);
+                bl[nt] = ld_frag(&sm.A1l[ho]);
+            }
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt)
+                    acc[mt][nt] = mfma_w(A1[mt][kt], bh[nt], bl[nt], acc[mt][nt]);
+        }
+#pragma unroll
         for (int mt = 0; mt < 2; ++mt) {
             const int fr = f0 + 16 * mt + 4 * g;
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
-                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-                const int ho = (16 * nt + r16) * SM + 8 * g;
-#pragma unroll
-                for (int kt = 0; kt < 4; ++kt)
-                    acc = mfma_w(A1[mt][kt], ld_frag(&sm.A1h[ho + 32 * kt]),
-                                 ld_frag(&sm.A1l[ho + 32 * kt]), acc);
                 float h[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    h[i] = fmaxf(acc[i] + b1v[mt][i], 0.f);
+                    h[i] = fmaxf(acc[mt][nt][i] + b1v[mt][i], 0.f);
                     if (h[i] > 0.f) m1 |= 1u << ((mt * NT + nt) * 4 + i);
                 }
                 if constexpr (BWD)
@@ -605,27 +646,35 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
             __syncthreads();
             // ---- 8. g_H0^T slice = W1[slice] g_H1^T, masked -> A2 (and G0f in F2); dW1 ----
 #pragma unroll
-            for (int mt = 0; mt < 2; ++mt) {
-                const int fr = f0 + 16 * mt + 4 * g;
-                f32x4 acc[NT];
+            for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-                for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+                for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int kt = 0; kt < 4; ++kt) {
+            for (int kt = 0; kt < 4; ++kt) {
+                bf16x8 bh[NT], bl[NT];
+#pragma unroll
+                for (int nt = 0; nt < NT; ++nt) {
+                    const int go = (16 * nt + r16) * SM + 32 * kt + 8 * g;
+                    bh[nt] = ld_frag(&sm.A1h[go]);
+                    bl[nt] = ld_frag(&sm.A1l[go]);
+                }
+#pragma unroll
+                for (int mt = 0; mt < 2; ++mt) {
                     const bf16x8 a = as_bf8(sm.Bw1[((wv * 2 + mt) * 4 + kt) * 64 + lane]);
 #pragma unroll
-                    for (int nt = 0; nt < NT; ++nt) {
-                        const int go = (16 * nt + r16) * SM + 32 * kt + 8 * g;
-                        acc[nt] = mfma_w(a, ld_frag(&sm.A1h[go]), ld_frag(&sm.A1l[go]), acc[nt]);
-                    }
-                    __builtin_amdgcn_sched_barrier(0);
+                    for (int nt = 0; nt < NT; ++nt)
+                        acc[mt][nt] = mfma_w(a, bh[nt], bl[nt], acc[mt][nt]);
                 }
+            }
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                const int fr = f0 + 16 * mt + 4 * g;
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) {
                     float gv[4];
 #pragma unroll
                     for (int i = 0; i < 4; ++i)
-                        gv[i] = (m0 >> ((mt * NT + nt) * 4 + i)) & 1u ? acc[nt][i] : 0.f;
+                        gv[i] = (m0 >> ((mt * NT + nt) * 4 + i)) & 1u ? acc[mt][nt][i] : 0.f;
                     st_act<true>(sm.A2h, sm.A2l, sm.F2, 16 * nt + r16, fr, lane, gv);
                 }
             }
