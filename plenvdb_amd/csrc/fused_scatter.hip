@@ -66,14 +66,20 @@ __global__ void __launch_bounds__(256) k_scatter_direct(TopoView topo, RayBuffer
 }
 
 // Same result as k_scatter_direct with far fewer global atomics: a workgroup takes
-// kAggRecords consecutive records (a few rays; consecutive samples of a ray
-// share most of their corners), sums their (slot, gradient) contributions in
-// an LDS hash table keyed by the 32-bit slot, then adds each distinct slot's
-// 13 sums to the grids once.  The flush gives 16 lanes to a slot, so one
+// kAggRecords consecutive records (two or three rays; consecutive samples of a
+// ray share most of their corners), sums their (slot, gradient) contributions
+// in an LDS hash table keyed by the 32-bit slot, then adds each distinct slot's
+// 13 sums to the grids once.  Every record load is issued before any is used
+// (one memory latency per chunk, not a chain of them); inserted entries are
+// listed, so the flush visits only those, and it clears what it reads, so the
+// table is clean for the next chunk.  The flush gives 16 lanes to a slot: one
 // atomic instruction covers four slots' contiguous 12-float colour rows.
-constexpr int kAggRecords = 128;
+constexpr int kAggRecords = 64;
 constexpr int kAggSlots = 8 * kAggRecords;  // every contribution fits: probing ends
+constexpr int kAggLog2 = 9;                 // kAggSlots == 1 << kAggLog2
+constexpr int kAggPairs = 8 * kAggRecords / 256;  // (record, corner) pairs per thread
 constexpr uint32_t kEmptyKey = 0xffffffffu;
+static_assert(kAggSlots == 1 << kAggLog2, "hash size");
 __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R, Records rec,
                                                      int64_t capacity,
                                                      float* __restrict__ grad_density,
@@ -81,45 +87,77 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
                                                      uint8_t* __restrict__ leaf_touched) {
     __shared__ uint32_t key[kAggSlots];
     __shared__ float val[13][kAggSlots];
+    __shared__ uint16_t listed[kAggSlots];
+    __shared__ int nlisted[2];
     const int tid = threadIdx.x;
+    for (int e = tid; e < kAggSlots; e += 256) {
+        key[e] = kEmptyKey;
+#pragma unroll
+        for (int c = 0; c < 13; ++c) val[c][e] = 0.f;
+    }
+    if (tid < 2) nlisted[tid] = 0;
+    __syncthreads();
     const int64_t total = min((int64_t)R.total[0], capacity);
     const int64_t nchunk = (total + kAggRecords - 1) / kAggRecords;
-    for (int64_t chunk = blockIdx.x; chunk < nchunk; chunk += gridDim.x) {
-        for (int e = tid; e < kAggSlots; e += 256) {
-            key[e] = kEmptyKey;
+    int b = 0;
+    for (int64_t chunk = blockIdx.x; chunk < nchunk; chunk += gridDim.x, b ^= 1) {
+        // ---- this thread's pairs: records loaded first, then the topology ----
+        uint32_t bits[kAggPairs];
+        float4 pt[kAggPairs], gr[kAggPairs][4];
 #pragma unroll
-            for (int c = 0; c < 13; ++c) val[c][e] = 0.f;
+        for (int it = 0; it < kAggPairs; ++it) {
+            const int64_t q = chunk * kAggRecords + ((tid + 256 * it) >> 3);
+            bits[it] = 0u;
+            if (q < total) {
+                bits[it] = __float_as_uint(rec.atwr[q].w);
+                pt[it] = rec.xyze[q];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) gr[it][k] = rec.grad[4 * q + k];
+            }
         }
-        __syncthreads();
-        for (int p = tid; p < 8 * kAggRecords; p += 256) {
-            const int64_t q = chunk * kAggRecords + (p >> 3);
-            if (q >= total) continue;
-            const int corner = p & 7;
-            const uint32_t bits = __float_as_uint(rec.atwr[q].w);
-            if (!(bits & (kAliveA | kAliveW))) continue;
-            const float4 pt = rec.xyze[q];
-            const Cell cell = make_cell(pt.x, pt.y, pt.z);
-            const int dx = (kCx >> corner) & 1, dy = (kCy >> corner) & 1, dz = (kCz >> corner) & 1;
-            const int i = cell.i0 + dx, j = cell.j0 + dy, k = cell.k0 + dz;
-            const int32_t leaf = leaf_of(topo, i, j, k);
-            if (leaf < 0) continue;
-            const uint32_t slot = (uint32_t)leaf * 512u + (uint32_t)voxel_offset(i, j, k);
-            const float a = dx ? cell.u : 1.f - cell.u;
-            const float b = dy ? cell.v : 1.f - cell.v;
-            const float d = dz ? cell.w : 1.f - cell.w;
-            uint32_t h = (slot * 2654435761u) >> (32 - 10);  // kAggSlots == 1 << 10
+        int32_t leaf[kAggPairs];
+        uint32_t slot[kAggPairs];
+        float wgt[kAggPairs][3];
+#pragma unroll
+        for (int it = 0; it < kAggPairs; ++it) {
+            const int corner = (tid + 256 * it) & 7;  // 256 is a multiple of 8
+            leaf[it] = -1;
+            slot[it] = 0u;
+            wgt[it][0] = wgt[it][1] = wgt[it][2] = 0.f;
+            if (bits[it] & (kAliveA | kAliveW)) {
+                const Cell cell = make_cell(pt[it].x, pt[it].y, pt[it].z);
+                const int dx = (kCx >> corner) & 1, dy = (kCy >> corner) & 1,
+                          dz = (kCz >> corner) & 1;
+                const int i = cell.i0 + dx, j = cell.j0 + dy, k = cell.k0 + dz;
+                leaf[it] = leaf_of(topo, i, j, k);
+                slot[it] = (uint32_t)voxel_offset(i, j, k);
+                wgt[it][0] = dx ? cell.u : 1.f - cell.u;
+                wgt[it][1] = dy ? cell.v : 1.f - cell.v;
+                wgt[it][2] = dz ? cell.w : 1.f - cell.w;
+            }
+        }
+        // ---- sums per slot in LDS ----
+#pragma unroll
+        for (int it = 0; it < kAggPairs; ++it) {
+            if (leaf[it] < 0) continue;
+            const uint32_t s = (uint32_t)leaf[it] * 512u + slot[it];
+            uint32_t h = (s * 2654435761u) >> (32 - kAggLog2);
             while (true) {
-                const uint32_t prev = atomicCAS(&key[h], kEmptyKey, slot);
-                if (prev == kEmptyKey || prev == slot) break;
+                const uint32_t prev = atomicCAS(&key[h], kEmptyKey, s);
+                if (prev == kEmptyKey) {
+                    listed[atomicAdd(&nlisted[b], 1)] = (uint16_t)h;
+                    break;
+                }
+                if (prev == s) break;
                 h = (h + 1) & (kAggSlots - 1);
             }
-            const float4* g = rec.grad + 4 * q;
-            if (bits & kAliveA) atomicAdd(&val[12][h], ((g[3].x * a) * b) * d);
-            if (bits & kAliveW) {
-                const float sc = (a * b) * d;
+            const float a = wgt[it][0], bb = wgt[it][1], d = wgt[it][2];
+            if (bits[it] & kAliveA) atomicAdd(&val[12][h], ((gr[it][3].x * a) * bb) * d);
+            if (bits[it] & kAliveW) {
+                const float sc = (a * bb) * d;
 #pragma unroll
                 for (int part = 0; part < 3; ++part) {
-                    const float4 v = g[part];
+                    const float4 v = gr[it][part];
                     atomicAdd(&val[4 * part + 0][h], v.x * sc);
                     atomicAdd(&val[4 * part + 1][h], v.y * sc);
                     atomicAdd(&val[4 * part + 2][h], v.z * sc);
@@ -128,18 +166,26 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
             }
         }
         __syncthreads();
+        // ---- flush the listed slots and clear them; reset the other list counter ----
+        const int n = nlisted[b];
+        if (tid == 0) nlisted[b ^ 1] = 0;
         const int c = tid & 15;
-        for (int e = tid >> 4; e < kAggSlots; e += 16) {
-            const uint32_t slot = key[e];
-            if (slot == kEmptyKey) continue;
-            if (c == 13) leaf_touched[slot >> 9] = 1;
-            if (c > 12) continue;
-            const float v = val[c][e];
-            if (v == 0.f) continue;  // adding zero changes nothing
-            if (c < 12)
-                atomicAdd(grad_color + (int64_t)slot * 12 + c, v);
-            else
-                atomicAdd(grad_density + slot, v);
+        for (int e = tid >> 4; e < n; e += 16) {
+            const int h = listed[e];
+            const uint32_t s = key[h];
+            if (c < 13) {
+                const float v = val[c][h];
+                val[c][h] = 0.f;
+                if (v != 0.f) {  // adding zero changes nothing
+                    if (c < 12)
+                        atomicAdd(grad_color + (int64_t)s * 12 + c, v);
+                    else
+                        atomicAdd(grad_density + s, v);
+                }
+            } else if (c == 13) {
+                leaf_touched[s >> 9] = 1;
+                key[h] = kEmptyKey;  // after this group's lanes read it (one wavefront)
+            }
         }
         __syncthreads();
     }
@@ -153,8 +199,9 @@ int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, i
                    float* grad_density, float* grad_color, uint8_t* leaf_touched, hipStream_t s) {
     if (t.nleaf == 0) return PVDB_OK;
     if (t.nleaf < (int64_t(1) << 23) - 1) {  // every slot < 2^32 - 1, the empty key
+        // five 30 KB workgroups fit a CU; each takes chunks in a grid-stride loop
         const int64_t chunks = (capacity + kAggRecords - 1) / kAggRecords;
-        const unsigned grid = (unsigned)(chunks < 2048 ? (chunks < 1 ? 1 : chunks) : 2048);
+        const unsigned grid = (unsigned)(chunks < 1280 ? (chunks < 1 ? 1 : chunks) : 1280);
         hipLaunchKernelGGL(k_scatter_agg, dim3(grid), dim3(256), 0, s, t, R, rec, capacity,
                            grad_density, grad_color, leaf_touched);
         PVDB_LAUNCHED("k_scatter_agg");
