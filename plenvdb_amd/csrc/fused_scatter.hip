@@ -67,18 +67,23 @@ __global__ void __launch_bounds__(256) k_scatter_direct(TopoView topo, RayBuffer
 
 // Same result as k_scatter_direct with far fewer global atomics: a workgroup takes
 // kAggRecords consecutive records (two or three rays; consecutive samples of a
-// ray share most of their corners), sums their (slot, gradient) contributions
-// in an LDS hash table keyed by the 32-bit slot, then adds each distinct slot's
-// 13 sums to the grids once.  Every record load is issued before any is used
-// (one memory latency per chunk, not a chain of them); inserted entries are
-// listed, so the flush visits only those, and it clears what it reads, so the
-// table is clean for the next chunk.  The flush gives 16 lanes to a slot: one
-// atomic instruction covers four slots' contiguous 12-float colour rows.
+// ray share most of their corners) and adds each distinct slot's 13 sums to the
+// grids once.  Within the chunk, every (record, corner) pair finds its slot's
+// entry in an LDS hash table (compare-and-swap on the 32-bit slot key), stores
+// its 13 contributions with plain stores, and links itself into the entry's
+// list (one exchange); the flush gives each listed entry 16 lanes, which walk
+// the list and sum channel c in lane c.  Two LDS atomics per pair instead of
+// fourteen: LDS float atomics run at well under one lane per clock.  Every
+// record load is issued before any is used (one memory latency per chunk).
+// One atomic instruction of the flush covers four slots' contiguous 12-float
+// colour rows.
 constexpr int kAggRecords = 64;
-constexpr int kAggSlots = 8 * kAggRecords;  // every contribution fits: probing ends
-constexpr int kAggLog2 = 9;                 // kAggSlots == 1 << kAggLog2
-constexpr int kAggPairs = 8 * kAggRecords / 256;  // (record, corner) pairs per thread
+constexpr int kAggPairsAll = 8 * kAggRecords;  // (record, corner) pairs of a chunk
+constexpr int kAggSlots = kAggPairsAll;        // every pair fits: probing ends
+constexpr int kAggLog2 = 9;                    // kAggSlots == 1 << kAggLog2
+constexpr int kAggPairs = kAggPairsAll / 256;  // pairs per thread
 constexpr uint32_t kEmptyKey = 0xffffffffu;
+constexpr uint32_t kEnd = 0xffffu;             // end of a pair list
 static_assert(kAggSlots == 1 << kAggLog2, "hash size");
 __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R, Records rec,
                                                      int64_t capacity,
@@ -86,14 +91,15 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
                                                      float* __restrict__ grad_color,
                                                      uint8_t* __restrict__ leaf_touched) {
     __shared__ uint32_t key[kAggSlots];
-    __shared__ float val[13][kAggSlots];
-    __shared__ uint16_t listed[kAggSlots];
+    __shared__ uint32_t head[kAggSlots];          // first pair of the entry's list
+    __shared__ uint16_t next[kAggPairsAll];       // next pair of the same entry
+    __shared__ float pv[kAggPairsAll * 13];       // each pair's 13 contributions
+    __shared__ uint16_t listed[kAggSlots];        // the entries in use
     __shared__ int nlisted[2];
     const int tid = threadIdx.x;
     for (int e = tid; e < kAggSlots; e += 256) {
         key[e] = kEmptyKey;
-#pragma unroll
-        for (int c = 0; c < 13; ++c) val[c][e] = 0.f;
+        head[e] = kEnd;
     }
     if (tid < 2) nlisted[tid] = 0;
     __syncthreads();
@@ -116,13 +122,13 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
             }
         }
         int32_t leaf[kAggPairs];
-        uint32_t slot[kAggPairs];
+        uint32_t off[kAggPairs];
         float wgt[kAggPairs][3];
 #pragma unroll
         for (int it = 0; it < kAggPairs; ++it) {
             const int corner = (tid + 256 * it) & 7;  // 256 is a multiple of 8
             leaf[it] = -1;
-            slot[it] = 0u;
+            off[it] = 0u;
             wgt[it][0] = wgt[it][1] = wgt[it][2] = 0.f;
             if (bits[it] & (kAliveA | kAliveW)) {
                 const Cell cell = make_cell(pt[it].x, pt[it].y, pt[it].z);
@@ -130,17 +136,31 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
                           dz = (kCz >> corner) & 1;
                 const int i = cell.i0 + dx, j = cell.j0 + dy, k = cell.k0 + dz;
                 leaf[it] = leaf_of(topo, i, j, k);
-                slot[it] = (uint32_t)voxel_offset(i, j, k);
+                off[it] = (uint32_t)voxel_offset(i, j, k);
                 wgt[it][0] = dx ? cell.u : 1.f - cell.u;
                 wgt[it][1] = dy ? cell.v : 1.f - cell.v;
                 wgt[it][2] = dz ? cell.w : 1.f - cell.w;
             }
         }
-        // ---- sums per slot in LDS ----
+        // ---- contributions into LDS, pairs linked per slot ----
 #pragma unroll
         for (int it = 0; it < kAggPairs; ++it) {
             if (leaf[it] < 0) continue;
-            const uint32_t s = (uint32_t)leaf[it] * 512u + slot[it];
+            const int pair = tid + 256 * it;
+            const float a = wgt[it][0], bb = wgt[it][1], d = wgt[it][2];
+            float* v = pv + pair * 13;
+            const float sc = (a * bb) * d;
+            const bool w_ = (bits[it] & kAliveW) != 0u;
+#pragma unroll
+            for (int part = 0; part < 3; ++part) {
+                const float4 g = gr[it][part];
+                v[4 * part + 0] = w_ ? g.x * sc : 0.f;
+                v[4 * part + 1] = w_ ? g.y * sc : 0.f;
+                v[4 * part + 2] = w_ ? g.z * sc : 0.f;
+                v[4 * part + 3] = w_ ? g.w * sc : 0.f;
+            }
+            v[12] = (bits[it] & kAliveA) ? ((gr[it][3].x * a) * bb) * d : 0.f;
+            const uint32_t s = (uint32_t)leaf[it] * 512u + off[it];
             uint32_t h = (s * 2654435761u) >> (32 - kAggLog2);
             while (true) {
                 const uint32_t prev = atomicCAS(&key[h], kEmptyKey, s);
@@ -151,41 +171,36 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
                 if (prev == s) break;
                 h = (h + 1) & (kAggSlots - 1);
             }
-            const float a = wgt[it][0], bb = wgt[it][1], d = wgt[it][2];
-            if (bits[it] & kAliveA) atomicAdd(&val[12][h], ((gr[it][3].x * a) * bb) * d);
-            if (bits[it] & kAliveW) {
-                const float sc = (a * bb) * d;
-#pragma unroll
-                for (int part = 0; part < 3; ++part) {
-                    const float4 v = gr[it][part];
-                    atomicAdd(&val[4 * part + 0][h], v.x * sc);
-                    atomicAdd(&val[4 * part + 1][h], v.y * sc);
-                    atomicAdd(&val[4 * part + 2][h], v.z * sc);
-                    atomicAdd(&val[4 * part + 3][h], v.w * sc);
-                }
-            }
+            next[pair] = (uint16_t)atomicExch(&head[h], (uint32_t)pair);
         }
         __syncthreads();
-        // ---- flush the listed slots and clear them; reset the other list counter ----
+        // ---- flush: walk each listed entry, add its sums, clear it; reset the
+        // other list counter ----
         const int n = nlisted[b];
         if (tid == 0) nlisted[b ^ 1] = 0;
         const int c = tid & 15;
         for (int e = tid >> 4; e < n; e += 16) {
             const int h = listed[e];
             const uint32_t s = key[h];
+            float sum = 0.f;
+            for (uint32_t p = head[h]; p != kEnd; p = next[p])  // same walk in all 16 lanes
+                if (c < 13) sum += pv[p * 13 + c];
             if (c < 13) {
-                const float v = val[c][h];
-                val[c][h] = 0.f;
-                if (v != 0.f) {  // adding zero changes nothing
+                if (sum != 0.f) {  // adding zero changes nothing
                     if (c < 12)
-                        atomicAdd(grad_color + (int64_t)s * 12 + c, v);
+                        atomicAdd(grad_color + (int64_t)s * 12 + c, sum);
                     else
-                        atomicAdd(grad_density + s, v);
+                        atomicAdd(grad_density + s, sum);
                 }
             } else if (c == 13) {
                 leaf_touched[s >> 9] = 1;
-                key[h] = kEmptyKey;  // after this group's lanes read it (one wavefront)
             }
+        }
+        __syncthreads();  // every walk is done: clear the entries
+        for (int e = tid; e < n; e += 256) {
+            const int h = listed[e];
+            key[h] = kEmptyKey;
+            head[h] = kEnd;
         }
         __syncthreads();
     }
@@ -199,9 +214,9 @@ int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, i
                    float* grad_density, float* grad_color, uint8_t* leaf_touched, hipStream_t s) {
     if (t.nleaf == 0) return PVDB_OK;
     if (t.nleaf < (int64_t(1) << 23) - 1) {  // every slot < 2^32 - 1, the empty key
-        // five 30 KB workgroups fit a CU; each takes chunks in a grid-stride loop
+        // four 32 KB workgroups fit a CU; each takes chunks in a grid-stride loop
         const int64_t chunks = (capacity + kAggRecords - 1) / kAggRecords;
-        const unsigned grid = (unsigned)(chunks < 1280 ? (chunks < 1 ? 1 : chunks) : 1280);
+        const unsigned grid = (unsigned)(chunks < 1024 ? (chunks < 1 ? 1 : chunks) : 1024);
         hipLaunchKernelGGL(k_scatter_agg, dim3(grid), dim3(256), 0, s, t, R, rec, capacity,
                            grad_density, grad_color, leaf_touched);
         PVDB_LAUNCHED("k_scatter_agg");
