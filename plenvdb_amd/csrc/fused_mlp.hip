@@ -179,7 +179,7 @@ struct alignas(16) SmemBwd {
     uint16_t F2[128 * FM];
     uint16_t GLf[2][16 * FM];
     uint16_t G1f[128 * FM];
-    float GX[2][16][GS];  // g_X^T partials over the two halves of the hidden layer
+    float GX[4][16][GS];  // g_X^T partials over the four quarters of the hidden layer
     float gsig[2][GS];
     float4 vd[2][GS];  // view directions, double-buffered like the adjoints
     RecInfo ri[GS];
@@ -197,26 +197,35 @@ struct Prefetch {
     uint4 f0, f1, f2;   // stage B (backward): kept features (hi 12, lo 12)
 };
 
+// Wavefronts per workgroup: the forward runs 4 (32 hidden features each, up to
+// three workgroups per CU); the backward, one workgroup per CU for its LDS,
+// runs 8 (16 features each), so that every SIMD holds two wavefronts.
+template <bool BWD>
+constexpr int kWaves = BWD ? 8 : 4;
+
 template <bool BWD, bool BF16COL>
-__global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const void* color,
-                                             MlpPacked W, RayBuffers R, Records rec,
-                                             float* __restrict__ dw_partial) {
+__global__ void __launch_bounds__(64 * kWaves<BWD>) k_mlp(StepParams P, TopoView topo,
+                                                          const void* color, MlpPacked W,
+                                                          RayBuffers R, Records rec,
+                                                          float* __restrict__ dw_partial) {
     using Smem = std::conditional_t<BWD, SmemBwd, SmemFwd>;
+    constexpr int NW = kWaves<BWD>, NTH = 64 * NW;
+    constexpr int MT = 8 / NW;  // 16-feature tiles per wave
     __shared__ Smem sm;
     const int tid = threadIdx.x;
     const int wv = tid >> 6, lane = tid & 63, r16 = lane & 15, g = lane >> 4;
-    const int f0 = 32 * wv;  // first hidden feature owned by this wave
-    // per-record staging (prefetch, adjoints, commit): lanes 0..7 of every wave,
-    // so that the four SIMDs share it
-    const int pr = lane < GS / 4 ? wv * (GS / 4) + lane : -1;
+    const int f0 = 16 * MT * wv;  // first hidden feature owned by this wave
+    // per-record staging (prefetch, adjoints, commit): the first GS / NW lanes of
+    // every wave, so that the SIMDs share it
+    const int pr = lane < GS / NW ? wv * (GS / NW) + lane : -1;
 
     // ---- forward weight slices: W1^T in registers; W0^T and W2^T (used by one
     // step each) are re-read from L2 per group, which keeps the backward's
     // registers for its dW accumulators ----
-    bf16x8 A1[2][4];
-    float b0v[2][4], b1v[2][4], b2v[3];
+    bf16x8 A1[MT][4];
+    float b0v[MT][4], b1v[MT][4], b2v[3];
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
+    for (int mt = 0; mt < MT; ++mt) {
         const int row = f0 + 16 * mt + r16;
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) A1[mt][kt] = ld_frag(W.w1t + row * 128 + 32 * kt + 8 * g);
@@ -231,10 +240,10 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
     b2v[2] = W.b2[2];
 
     // ---- backward weights -> LDS, dW accumulators ----
-    f32x4 dW0[3][2], dW1[8][2], dW2[2];
-    float db1[2][4], db2[3];
+    f32x4 dW0[3][MT], dW1[8][MT], dW2[MT];
+    float db1[MT][4], db2[3];
     if constexpr (BWD) {
-        for (int f = tid; f < kBw2Frags + kBw1Frags + kBw0Frags; f += 256) {
+        for (int f = tid; f < kBw2Frags + kBw1Frags + kBw0Frags; f += NTH) {
             const int ln = f & 63, gg = ln >> 4, rr = ln & 15;
             if (f < kBw2Frags) {  // W2 [128 in][32 out (3 used)], K = 16
                 const int t = f >> 6;
@@ -249,18 +258,15 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
             }
         }
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < MT; ++b) {
 #pragma unroll
-            for (int b = 0; b < 2; ++b) dW0[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int a = 0; a < 3; ++a) dW0[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int a = 0; a < 8; ++a)
+            for (int a = 0; a < 8; ++a) dW1[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+            dW2[b] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int b = 0; b < 2; ++b) dW1[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-        dW2[0] = dW2[1] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int a = 0; a < 2; ++a)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) db1[a][i] = 0.f;
+            for (int i = 0; i < 4; ++i) db1[b][i] = 0.f;
+        }
         db2[0] = db2[1] = db2[2] = 0.f;
     }
 
@@ -328,21 +334,21 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
     };
     stage_a(blockIdx.x);
     // X columns 40..63 (and, backward, feature-major rows 40..47) are zero throughout
-    for (int e = tid; e < GS * 24; e += 256) {
+    for (int e = tid; e < GS * 24; e += NTH) {
         const int row = e / 24, col = 40 + e % 24;
         sm.Xh[row * XM + col] = 0;
         sm.Xl[row * XM + col] = 0;
     }
     if constexpr (BWD) {
-        for (int e = tid; e < (XF_ROWS - 40) * FM; e += 256) sm.Xf[40 * FM + e] = 0;
+        for (int e = tid; e < (XF_ROWS - 40) * FM; e += NTH) sm.Xf[40 * FM + e] = 0;
         // the unused logits (3..15) of both dL/dlogit buffers are zero throughout
-        for (int e = tid; e < 2 * GS * GM; e += 256) {
+        for (int e = tid; e < 2 * GS * GM; e += NTH) {
             if (e % GM >= 3) {
                 (&sm.GLh[0][0])[e] = 0;
                 (&sm.GLl[0][0])[e] = 0;
             }
         }
-        for (int e = tid; e < 2 * 16 * FM; e += 256)
+        for (int e = tid; e < 2 * 16 * FM; e += NTH)
             if ((e % (16 * FM)) >= 3 * FM) (&sm.GLf[0][0])[e] = 0;
         stage_b(blockIdx.x);
         park(0);
@@ -382,9 +388,9 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
         }
         const int64_t next = grp + gridDim.x;
         stage_a(next);
-        bf16x8 A0[2][2];
+        bf16x8 A0[MT][2];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
+        for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
             for (int kt = 0; kt < 2; ++kt)
                 A0[mt][kt] = ld_frag(W.w0t + (f0 + 16 * mt + r16) * 64 + 32 * kt + 8 * g);
@@ -512,9 +518,9 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
         uint32_t m0 = 0u, m1 = 0u;  // ReLU masks: bit (mt*NT + nt)*4 + i
         // the weight products run k-step outer: each B fragment is read once and
         // the four (mt, nt) accumulator chains interleave
-        f32x4 acc[2][NT];
+        f32x4 acc[MT][NT];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
+        for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -527,13 +533,13 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                 bl[nt] = ld_frag(&sm.Xl[xo]);
             }
 #pragma unroll
-            for (int mt = 0; mt < 2; ++mt)
+            for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt)
                     acc[mt][nt] = mfma_w(A0[mt][kt], bh[nt], bl[nt], acc[mt][nt]);
         }
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) {
+        for (int mt = 0; mt < MT; ++mt) {
             const int fr = f0 + 16 * mt + 4 * g;
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
@@ -554,7 +560,7 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
         __syncthreads();
         // ---- 4. H1^T slice = relu(W1^T H0^T + b1) -> A2 (and H1f in F2) ----
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
+        for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -567,13 +573,13 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                 bl[nt] = ld_frag(&sm.A1l[ho]);
             }
 #pragma unroll
-            for (int mt = 0; mt < 2; ++mt)
+            for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt)
                     acc[mt][nt] = mfma_w(A1[mt][kt], bh[nt], bl[nt], acc[mt][nt]);
         }
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt) {
+        for (int mt = 0; mt < MT; ++mt) {
             const int fr = f0 + 16 * mt + 4 * g;
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
@@ -621,8 +627,8 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
             // code, so the ReLU masks recomputed here are the forward's)
             // ---- 7. g_H1^T slice = W2[slice] g_L^T, masked -> A1 (and G1f); dW2 ----
 #pragma unroll
-            for (int mt = 0; mt < 2; ++mt) {
-                const s16x4 a = __builtin_bit_cast(s16x4, sm.Bw2[(wv * 2 + mt) * 64 + lane]);
+            for (int mt = 0; mt < MT; ++mt) {
+                const s16x4 a = __builtin_bit_cast(s16x4, sm.Bw2[(wv * MT + mt) * 64 + lane]);
                 const int fr = f0 + 16 * mt + 4 * g;
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) {
@@ -646,7 +652,7 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
             __syncthreads();
             // ---- 8. g_H0^T slice = W1[slice] g_H1^T, masked -> A2 (and G0f in F2); dW1 ----
 #pragma unroll
-            for (int mt = 0; mt < 2; ++mt)
+            for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -659,15 +665,15 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                     bl[nt] = ld_frag(&sm.A1l[go]);
                 }
 #pragma unroll
-                for (int mt = 0; mt < 2; ++mt) {
-                    const bf16x8 a = as_bf8(sm.Bw1[((wv * 2 + mt) * 4 + kt) * 64 + lane]);
+                for (int mt = 0; mt < MT; ++mt) {
+                    const bf16x8 a = as_bf8(sm.Bw1[((wv * MT + mt) * 4 + kt) * 64 + lane]);
 #pragma unroll
                     for (int nt = 0; nt < NT; ++nt)
                         acc[mt][nt] = mfma_w(a, bh[nt], bl[nt], acc[mt][nt]);
                 }
             }
 #pragma unroll
-            for (int mt = 0; mt < 2; ++mt) {
+            for (int mt = 0; mt < MT; ++mt) {
                 const int fr = f0 + 16 * mt + 4 * g;
 #pragma unroll
                 for (int nt = 0; nt < NT; ++nt) {
@@ -679,34 +685,36 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                 }
             }
             {
-                bf16x8 bh[2];
+                bf16x8 bh[MT];
 #pragma unroll
-                for (int nt = 0; nt < 2; ++nt) bh[nt] = ld_frag8(&sm.G1f[(f0 + 16 * nt + r16) * FM + 8 * g]);
+                for (int nt = 0; nt < MT; ++nt) bh[nt] = ld_frag8(&sm.G1f[(f0 + 16 * nt + r16) * FM + 8 * g]);
 #pragma unroll
                 for (int mi = 0; mi < 8; ++mi) {
                     const bf16x8 ah = ld_frag8(&sm.H0f[(16 * mi + r16) * FM + 8 * g]);
 #pragma unroll
-                    for (int nt = 0; nt < 2; ++nt) dW1[mi][nt] = mfma16x16x32(ah, bh[nt], dW1[mi][nt]);
+                    for (int nt = 0; nt < MT; ++nt) dW1[mi][nt] = mfma16x16x32(ah, bh[nt], dW1[mi][nt]);
                 }
             }
             __syncthreads();
             // ---- 9. dW0 columns of this wave; g_X^T over this wave's K slice ----
             {
-                bf16x8 bh[2];
+                bf16x8 bh[MT];
 #pragma unroll
-                for (int nt = 0; nt < 2; ++nt) bh[nt] = ld_frag8(&sm.F2[(f0 + 16 * nt + r16) * FM + 8 * g]);
+                for (int nt = 0; nt < MT; ++nt) bh[nt] = ld_frag8(&sm.F2[(f0 + 16 * nt + r16) * FM + 8 * g]);
 #pragma unroll
                 for (int mi = 0; mi < 3; ++mi) {
                     const bf16x8 ah = ld_frag8(&sm.Xf[(16 * mi + r16) * FM + 8 * g]);
 #pragma unroll
-                    for (int nt = 0; nt < 2; ++nt) dW0[mi][nt] = mfma16x16x32(ah, bh[nt], dW0[mi][nt]);
+                    for (int nt = 0; nt < MT; ++nt) dW0[mi][nt] = mfma16x16x32(ah, bh[nt], dW0[mi][nt]);
                 }
-                // g_X^T: wave w takes sample tile w & 1 over hidden half w >> 1
+                // g_X^T: wave w takes sample tile w & 1 over the hidden part w >> 1
+                // (NW / 2 parts of 4 / (NW / 2) k-steps each)
+                constexpr int KPW = 8 / NW;  // k-steps per part
                 const int nt = wv & 1, kh = wv >> 1;
                 f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int kk = 0; kk < 2; ++kk) {
-                    const int kt = 2 * kh + kk;
+                for (int kk = 0; kk < KPW; ++kk) {
+                    const int kt = KPW * kh + kk;
                     const int go = (16 * nt + r16) * SM + 32 * kt + 8 * g;
                     acc = mfma_w(as_bf8(sm.Bw0[kt * 64 + lane]), ld_frag(&sm.A2h[go]),
                                  ld_frag(&sm.A2l[go]), acc);
@@ -725,10 +733,12 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                     if (part < 3) {
                         if (ri.bits & kAliveW) {
                             const int c = 4 * part;
-                            v = make_float4(sm.GX[0][c][s] + sm.GX[1][c][s],
-                                            sm.GX[0][c + 1][s] + sm.GX[1][c + 1][s],
-                                            sm.GX[0][c + 2][s] + sm.GX[1][c + 2][s],
-                                            sm.GX[0][c + 3][s] + sm.GX[1][c + 3][s]);
+                            float x[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                            for (int k = 0; k < NW / 2; ++k)
+#pragma unroll
+                                for (int i = 0; i < 4; ++i) x[i] += sm.GX[k][c + i][s];
+                            v = make_float4(x[0], x[1], x[2], x[3]);
                         }
                     } else {
                         v.x = sm.gsig[cb][s];
@@ -747,7 +757,7 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
 #pragma unroll
         for (int mi = 0; mi < 3; ++mi)
 #pragma unroll
-            for (int nt = 0; nt < 2; ++nt)
+            for (int nt = 0; nt < MT; ++nt)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     const int in = 16 * mi + 4 * g + i, o = f0 + 16 * nt + r16;
@@ -759,12 +769,12 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
 #pragma unroll
         for (int mi = 0; mi < 8; ++mi)
 #pragma unroll
-            for (int nt = 0; nt < 2; ++nt)
+            for (int nt = 0; nt < MT; ++nt)
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
                     out[kW1 + (16 * mi + 4 * g + i) * 128 + f0 + 16 * nt + r16] = dW1[mi][nt][i];
 #pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
+        for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 if (r16 < 3) out[kW2 + (f0 + 16 * mt + 4 * g + i) * 3 + r16] = dW2[mt][i];
@@ -775,8 +785,8 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
                 v += __shfl_xor(v, 8, 64);
                 if (r16 == 0) out[kB1 + f0 + 16 * mt + 4 * g + i] = v;
             }
-        // db2: lanes 0..7 of every wave accumulated it (park); the loop ended on a
-        // barrier, so gsig is free for the per-wave sums
+        // db2: the staging lanes of every wave accumulated it (park); the loop
+        // ended on a barrier, so gsig is free for the per-wave sums
 #pragma unroll
         for (int o = 0; o < 3; ++o) {
             float v = pr >= 0 ? db2[o] : 0.f;
@@ -786,9 +796,12 @@ __global__ void __launch_bounds__(256) k_mlp(StepParams P, TopoView topo, const 
             if (lane == 0) sm.gsig[0][4 * wv + o] = v;
         }
         __syncthreads();
-        if (tid < 3)
-            out[kB2 + tid] = (sm.gsig[0][tid] + sm.gsig[0][4 + tid]) +
-                             (sm.gsig[0][8 + tid] + sm.gsig[0][12 + tid]);
+        if (tid < 3) {
+            float v = 0.f;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) v += sm.gsig[0][4 * w + tid];
+            out[kB2 + tid] = v;
+        }
     }
 }
 
@@ -843,17 +856,17 @@ int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const Topo
                float* dw_partial, hipStream_t s) {
     if (bwd) {
         if (bf16col)
-            hipLaunchKernelGGL((k_mlp<true, true>), dim3(grid), dim3(256), 0, s, P, t, color, W, R,
+            hipLaunchKernelGGL((k_mlp<true, true>), dim3(grid), dim3(64 * kWaves<true>), 0, s, P, t, color, W, R,
                                rec, dw_partial);
         else
-            hipLaunchKernelGGL((k_mlp<true, false>), dim3(grid), dim3(256), 0, s, P, t, color, W, R,
+            hipLaunchKernelGGL((k_mlp<true, false>), dim3(grid), dim3(64 * kWaves<true>), 0, s, P, t, color, W, R,
                                rec, dw_partial);
     } else {
         if (bf16col)
-            hipLaunchKernelGGL((k_mlp<false, true>), dim3(grid), dim3(256), 0, s, P, t, color, W, R,
+            hipLaunchKernelGGL((k_mlp<false, true>), dim3(grid), dim3(64 * kWaves<false>), 0, s, P, t, color, W, R,
                                rec, dw_partial);
         else
-            hipLaunchKernelGGL((k_mlp<false, false>), dim3(grid), dim3(256), 0, s, P, t, color, W,
+            hipLaunchKernelGGL((k_mlp<false, false>), dim3(grid), dim3(64 * kWaves<false>), 0, s, P, t, color, W,
                                R, rec, dw_partial);
     }
     PVDB_LAUNCHED("k_mlp");
