@@ -805,6 +805,396 @@ __global__ void __launch_bounds__(64 * kWaves<BWD>) k_mlp(StepParams P, TopoView
     }
 }
 
+// ---- backward ----------------------------------------------------------------
+// One workgroup of 8 wavefronts per CU (two per SIMD); wave w owns hidden
+// features [16w, 16w + 16) of both hidden layers.  Per group of 32 records the
+// phases are (a barrier after each):
+//   3  H0^T = relu(W0^T X^T + b0) (recomputed; ReLU mask m0)   -> A1, H0f
+//   4  H1^T = relu(W1^T H0^T + b1) (ReLU mask m1)                -> A2, F2
+//   7  g_H1^T = W2 g_L^T masked by m1 -> A1, G1f; dW2 += H1^T g_L; the next
+//      group's records are staged (park)
+//   8  g_H0^T = W1 g_H1^T masked by m0 -> A2, F2; dW1 += H0^T g_H1; the next
+//      group's positional encoding
+//   9  dW0 += X^T g_H0; waves 0-1: g_X^T = W0[0:12] g_H0^T over the whole hidden
+//      layer, written from registers as the records' feature gradients
+// X (features, encoding, bias), the per-sample adjoints and the flags are
+// double-buffered, so the next group's inputs are staged while this one runs.
+constexpr int kBwdWaves = 8;
+struct alignas(16) SmemBwd2 {
+    uint4 Bw1[kBw1Frags];
+    uint4 Bw0[kBw0Frags];
+    uint2 Bw2[kBw2Frags];
+    uint16_t Xh[2][GS * XM], Xl[2][GS * XM];
+    uint16_t Xf[2][XF_ROWS * FM];
+    uint16_t A1h[GS * SM], A1l[GS * SM];
+    uint16_t A2h[GS * SM], A2l[GS * SM];
+    uint16_t GLh[2][GS * GM], GLl[2][GS * GM];
+    uint16_t H0f[128 * FM];
+    uint16_t F2[128 * FM];
+    uint16_t G1f[128 * FM];
+    uint16_t GLf[2][16 * FM];
+    float gsig[2][GS];
+    float4 vd[2][GS];
+    uint32_t bits[2][GS];
+};
+
+__global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPacked W,
+                                                            RayBuffers R, Records rec,
+                                                            float* __restrict__ dw_partial) {
+    constexpr int NTH = 64 * kBwdWaves;
+    __shared__ SmemBwd2 sm;
+    const int tid = threadIdx.x;
+    const int wv = tid >> 6, lane = tid & 63, r16 = lane & 15, g = lane >> 4;
+    const int f0 = 16 * wv;
+    const int fr = f0 + 4 * g;  // the 4 features of this lane's D fragment rows
+    const int pr = lane < GS / kBwdWaves ? wv * (GS / kBwdWaves) + lane : -1;  // staging lanes
+
+    // forward weight slices in registers (this wave's rows of W0^T and W1^T)
+    bf16x8 A0[2], A1[4];
+    float b0v[4], b1v[4];
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt) A0[kt] = ld_frag(W.w0t + (f0 + r16) * 64 + 32 * kt + 8 * g);
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt) A1[kt] = ld_frag(W.w1t + (f0 + r16) * 128 + 32 * kt + 8 * g);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        b0v[i] = W.b0[fr + i];
+        b1v[i] = W.b1[fr + i];
+    }
+    // backward weights -> LDS
+    for (int f = tid; f < kBw2Frags + kBw1Frags + kBw0Frags; f += NTH) {
+        const int ln = f & 63, gg = ln >> 4, rr = ln & 15;
+        if (f < kBw2Frags) {  // W2 [128 in][32 out (3 used)], K = 16
+            const int t = f >> 6;
+            sm.Bw2[f] = *reinterpret_cast<const uint2*>(W.w2 + (16 * t + rr) * 32 + 4 * gg);
+        } else if (f < kBw2Frags + kBw1Frags) {  // W1 [128 in][128 out]
+            const int q = f - kBw2Frags, t = q >> 8, kt = (q >> 6) & 3;
+            sm.Bw1[q] =
+                *reinterpret_cast<const uint4*>(W.w1 + (16 * t + rr) * 128 + 32 * kt + 8 * gg);
+        } else {  // W0 rows 0..15 [16][128]
+            const int q = f - kBw2Frags - kBw1Frags, kt = q >> 6;
+            sm.Bw0[q] = *reinterpret_cast<const uint4*>(W.w0 + rr * 128 + 32 * kt + 8 * gg);
+        }
+    }
+    f32x4 dW0[3], dW1[8], dW2;
+    float db1[4] = {0.f, 0.f, 0.f, 0.f}, db2[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dW0[a] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int a = 0; a < 8; ++a) dW1[a] = f32x4{0.f, 0.f, 0.f, 0.f};
+    dW2 = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // constant parts of the double buffers: X columns 40..63, feature-major X
+    // rows 40..47, logits 3..15
+    for (int e = tid; e < 2 * GS * 24; e += NTH) {
+        const int b = e / (GS * 24), row = (e / 24) % GS, col = 40 + e % 24;
+        sm.Xh[b][row * XM + col] = 0;
+        sm.Xl[b][row * XM + col] = 0;
+    }
+    for (int e = tid; e < 2 * (XF_ROWS - 40) * FM; e += NTH) {
+        const int b = e / ((XF_ROWS - 40) * FM);
+        sm.Xf[b][40 * FM + e % ((XF_ROWS - 40) * FM)] = 0;
+    }
+    for (int e = tid; e < 2 * GS * GM; e += NTH) {
+        if (e % GM >= 3) {
+            (&sm.GLh[0][0])[e] = 0;
+            (&sm.GLl[0][0])[e] = 0;
+        }
+    }
+    for (int e = tid; e < 2 * 16 * FM; e += NTH)
+        if ((e % (16 * FM)) >= 3 * FM) (&sm.GLf[0][0])[e] = 0;
+
+    const int64_t total = min((int64_t)R.total[0], P.capacity);
+    const int64_t ngroups = (total + GS - 1) / GS;
+    const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint4 zu4 = make_uint4(0u, 0u, 0u, 0u);
+
+    // register prefetch of a group's records on the staging lanes
+    float4 p_atwr = z4, p_adj = z4, p_vd = z4;
+    uint4 p_f0 = zu4, p_f1 = zu4, p_f2 = zu4;
+    auto stage_a = [&](int64_t grp) {  // the records (independent loads)
+        const int64_t q = grp * GS + pr;
+        p_atwr = p_adj = z4;
+        if (pr >= 0 && grp < ngroups && q < total) {
+            p_atwr = rec.atwr[q];
+            p_adj = rec.adj[q];
+        }
+    };
+    auto stage_b = [&](int64_t grp) {  // view direction and kept features (need the flags)
+        const int64_t q = grp * GS + pr;
+        p_vd = z4;
+        p_f0 = p_f1 = p_f2 = zu4;
+        if (pr >= 0 && grp < ngroups && q < total) {
+            const uint32_t bits = __float_as_uint(p_atwr.w);
+            if (bits & kAliveW) {
+                const uint32_t ray = bits & kRayMask;
+                p_vd = make_float4(P.viewdirs[3 * ray], P.viewdirs[3 * ray + 1],
+                                   P.viewdirs[3 * ray + 2], 0.f);
+                p_f0 = rec.feat[3 * q];
+                p_f1 = rec.feat[3 * q + 1];
+                p_f2 = rec.feat[3 * q + 2];
+            }
+        }
+    };
+    // staged records -> buffer b: flags, adjoints (dL/dlogit as (hi, lo) pairs,
+    // dL/dsigma), view direction, kept features (X columns 0..11)
+    auto park = [&](int b) {
+        if (pr >= 0) {
+            sm.bits[b][pr] = __float_as_uint(p_atwr.w);  // zero for records past the end
+            sm.gsig[b][pr] = p_adj.w;
+            sm.vd[b][pr] = p_vd;
+            const float gl[3] = {p_adj.x, p_adj.y, p_adj.z};
+            db2[0] += gl[0];
+            db2[1] += gl[1];
+            db2[2] += gl[2];
+#pragma unroll
+            for (int o = 0; o < 3; ++o) {
+                uint16_t h, l;
+                split_bf(gl[o], h, l);
+                sm.GLh[b][pr * GM + o] = h;
+                sm.GLl[b][pr * GM + o] = l;
+                sm.GLf[b][o * FM + pr] = h;
+            }
+            const uint32_t w12[12] = {p_f0.x, p_f0.y, p_f0.z, p_f0.w, p_f1.x, p_f1.y,
+                                      p_f1.z, p_f1.w, p_f2.x, p_f2.y, p_f2.z, p_f2.w};
+            uint32_t* xh = reinterpret_cast<uint32_t*>(&sm.Xh[b][pr * XM]);
+            uint32_t* xl = reinterpret_cast<uint32_t*>(&sm.Xl[b][pr * XM]);
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {  // hi halves 0..11, then lo halves 0..11
+                xh[c] = w12[c];
+                xl[c] = w12[6 + c];
+                sm.Xf[b][(2 * c) * FM + pr] = (uint16_t)(w12[c] & 0xffffu);
+                sm.Xf[b][(2 * c + 1) * FM + pr] = (uint16_t)(w12[c] >> 16);
+            }
+        }
+    };
+    // positional encoding (dvgo.py:354-358) and bias column of buffer b's group:
+    // thread (sample ps, part pp < 7) writes X columns 12 + 4pp .. +3; hardware
+    // sin/cos (error about 1e-6 for |arg| <= 8, below what the (hi, lo) pair keeps)
+    auto encode = [&](int b) {
+        const int ps = tid & (GS - 1), pp = tid >> 5;
+        const bool alive = (sm.bits[b][ps] & kAliveW) != 0u;
+        const float4 v4 = sm.vd[b][ps];  // zero unless alive
+        const float vd[3] = {v4.x, v4.y, v4.z};
+        float val[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int p = pp * 4 + k;  // 0..27: PE 0..26, then the bias column
+            float v = 0.f;
+            if (alive) {
+                if (p < 3) {
+                    v = vd[p];
+                } else if (p < 27) {
+                    const int idx = p - 3, axis = (idx % 12) / 4, fq = idx % 4;
+                    const float arg = vd[axis] * (float)(1 << fq);
+                    v = idx < 12 ? __sinf(arg) : __cosf(arg);
+                } else {
+                    v = 1.f;
+                }
+            }
+            val[k] = v;
+        }
+        uint2 hi, lo;
+        split2(val[0], val[1], hi.x, lo.x);
+        split2(val[2], val[3], hi.y, lo.y);
+        const uint2 t = quad_transpose(hi, lane & 3);  // the quad: 4 samples, one part
+        if (pp < 7) {
+            *reinterpret_cast<uint2*>(&sm.Xh[b][ps * XM + 12 + 4 * pp]) = hi;
+            *reinterpret_cast<uint2*>(&sm.Xl[b][ps * XM + 12 + 4 * pp]) = lo;
+            *reinterpret_cast<uint2*>(&sm.Xf[b][(12 + 4 * pp + (lane & 3)) * FM + (ps & ~3)]) = t;
+        }
+    };
+
+    stage_a(blockIdx.x);
+    stage_b(blockIdx.x);
+    park(0);
+    __syncthreads();
+    encode(0);
+    __syncthreads();
+    int cb = 0;  // buffer of the current group
+
+    for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {  // uniform bound
+        const int64_t base = grp * GS, next = grp + gridDim.x;
+        stage_a(next);
+        // ---- 3. H0^T slice = relu(W0^T X^T + b0) -> A1 (and H0f) ----
+        uint32_t m0 = 0u, m1 = 0u;  // ReLU masks: bit nt * 4 + i
+        f32x4 acc[NT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 2; ++kt) {
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int xo = (16 * nt + r16) * XM + 32 * kt + 8 * g;
+                acc[nt] = mfma_w(A0[kt], ld_frag(&sm.Xh[cb][xo]), ld_frag(&sm.Xl[cb][xo]), acc[nt]);
+            }
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            float h[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                h[i] = fmaxf(acc[nt][i] + b0v[i], 0.f);
+                if (h[i] > 0.f) m0 |= 1u << (nt * 4 + i);
+            }
+            st_act<true>(sm.A1h, sm.A1l, sm.H0f, 16 * nt + r16, fr, lane, h);
+        }
+        stage_b(next);  // the stage-A loads have landed by now
+        __syncthreads();
+        // ---- 4. H1^T slice = relu(W1^T H0^T + b1) -> A2 (and F2) ----
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) {
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int ho = (16 * nt + r16) * SM + 32 * kt + 8 * g;
+                acc[nt] = mfma_w(A1[kt], ld_frag(&sm.A1h[ho]), ld_frag(&sm.A1l[ho]), acc[nt]);
+            }
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            float h[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                h[i] = fmaxf(acc[nt][i] + b1v[i], 0.f);
+                if (h[i] > 0.f) m1 |= 1u << (nt * 4 + i);
+            }
+            st_act<true>(sm.A2h, sm.A2l, sm.F2, 16 * nt + r16, fr, lane, h);
+        }
+        __syncthreads();
+        // ---- 7. g_H1^T slice = W2[slice] g_L^T, masked -> A1 (and G1f); dW2; park ----
+        {
+            const s16x4 a = __builtin_bit_cast(s16x4, sm.Bw2[wv * 64 + lane]);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                f32x4 c4 = {0.f, 0.f, 0.f, 0.f};
+                const int go = (16 * nt + r16) * GM + 4 * g;
+                c4 = mfma16x16x16(a, ld_frag4(&sm.GLh[cb][go]), c4);
+                c4 = mfma16x16x16(a, ld_frag4(&sm.GLl[cb][go]), c4);
+                float gv[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    gv[i] = (m1 >> (nt * 4 + i)) & 1u ? c4[i] : 0.f;
+                    db1[i] += gv[i];
+                }
+                st_act<true>(sm.A1h, sm.A1l, sm.G1f, 16 * nt + r16, fr, lane, gv);
+            }
+            // dW2[in = f0 + row][out] += sum over the group's samples H1[s][in] gL[s][out]
+            dW2 = mfma16x16x32(ld_frag8(&sm.F2[(f0 + r16) * FM + 8 * g]),
+                               ld_frag8(&sm.GLf[cb][r16 * FM + 8 * g]), dW2);
+        }
+        park(cb ^ 1);  // the next group's stage-B loads (issued after step 3) have landed
+        __syncthreads();
+        // ---- 8. g_H0^T slice = W1[slice] g_H1^T, masked -> A2 (and F2); dW1; encode ----
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) {
+            const bf16x8 a = as_bf8(sm.Bw1[(wv * 4 + kt) * 64 + lane]);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const int go = (16 * nt + r16) * SM + 32 * kt + 8 * g;
+                acc[nt] = mfma_w(a, ld_frag(&sm.A1h[go]), ld_frag(&sm.A1l[go]), acc[nt]);
+            }
+        }
+        {
+            // dW1[in][out = f0 + col] += sum over samples H0[s][in] g_H1[s][out]
+            const bf16x8 bh = ld_frag8(&sm.G1f[(f0 + r16) * FM + 8 * g]);
+#pragma unroll
+            for (int mi = 0; mi < 8; ++mi)
+                dW1[mi] = mfma16x16x32(ld_frag8(&sm.H0f[(16 * mi + r16) * FM + 8 * g]), bh, dW1[mi]);
+        }
+        // A2 and F2 were last read in step 7 (before its barrier): free for g_H0
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            float gv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) gv[i] = (m0 >> (nt * 4 + i)) & 1u ? acc[nt][i] : 0.f;
+            st_act<true>(sm.A2h, sm.A2l, sm.F2, 16 * nt + r16, fr, lane, gv);
+        }
+        encode(cb ^ 1);
+        __syncthreads();
+        // ---- 9. dW0 columns of this wave; g_X^T (waves 0-1) -> record gradients ----
+        {
+            const bf16x8 bh = ld_frag8(&sm.F2[(f0 + r16) * FM + 8 * g]);
+#pragma unroll
+            for (int mi = 0; mi < 3; ++mi)
+                dW0[mi] = mfma16x16x32(ld_frag8(&sm.Xf[cb][(16 * mi + r16) * FM + 8 * g]), bh, dW0[mi]);
+        }
+        if (wv < NT) {
+            const int nt = wv;
+            f32x4 gx = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt) {
+                const int go = (16 * nt + r16) * SM + 32 * kt + 8 * g;
+                gx = mfma_w(as_bf8(sm.Bw0[kt * 64 + lane]), ld_frag(&sm.A2h[go]),
+                            ld_frag(&sm.A2l[go]), gx);
+            }
+            // lane (g, r16): features 4g..4g+3 of sample 16 nt + r16 = part g of its
+            // record's gradient; part 3 carries dL/dsigma
+            const int sidx = 16 * nt + r16;
+            const int64_t q = base + sidx;
+            if (q < total) {
+                const uint32_t bits = sm.bits[cb][sidx];
+                float4 v = z4;
+                if (g < 3) {
+                    if (bits & kAliveW) v = make_float4(gx[0], gx[1], gx[2], gx[3]);
+                } else {
+                    v.x = sm.gsig[cb][sidx];
+                }
+                rec.grad[4 * q + g] = v;
+            }
+        }
+        __syncthreads();
+        cb ^= 1;
+    }
+
+    // ---- flush this workgroup's dW partial ----
+    float* out = dw_partial + (int64_t)blockIdx.x * kMlpParams;
+#pragma unroll
+    for (int mi = 0; mi < 3; ++mi)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int in = 16 * mi + 4 * g + i, o = f0 + r16;
+            if (in < 39)
+                out[kW0 + in * 128 + o] = dW0[mi][i];
+            else if (in == 39)
+                out[kB0 + o] = dW0[mi][i];
+        }
+#pragma unroll
+    for (int mi = 0; mi < 8; ++mi)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) out[kW1 + (16 * mi + 4 * g + i) * 128 + f0 + r16] = dW1[mi][i];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (r16 < 3) out[kW2 + (fr + i) * 3 + r16] = dW2[i];
+        float v = db1[i];
+        v += __shfl_xor(v, 1, 64);
+        v += __shfl_xor(v, 2, 64);
+        v += __shfl_xor(v, 4, 64);
+        v += __shfl_xor(v, 8, 64);
+        if (r16 == 0) out[kB1 + fr + i] = v;
+    }
+    // db2 was accumulated by the staging lanes (park); the loop ended on a barrier,
+    // so gsig[0] is free for the per-wave sums
+#pragma unroll
+    for (int o = 0; o < 3; ++o) {
+        float v = pr >= 0 ? db2[o] : 0.f;
+        v += __shfl_xor(v, 1, 64);
+        v += __shfl_xor(v, 2, 64);
+        v += __shfl_xor(v, 4, 64);
+        if (lane == 0) sm.gsig[0][4 * wv + o] = v;
+    }
+    __syncthreads();
+    if (tid < 3) {
+        float v = 0.f;
+#pragma unroll
+        for (int w = 0; w < kBwdWaves; ++w) v += sm.gsig[0][4 * w + tid];
+        out[kB2 + tid] = v;
+    }
+}
+
 // Sum of the per-workgroup dW partials, accumulated into dw: blockIdx.y takes
 // a chunk of kReduceChunk partials, so the reduction runs on enough
 // workgroups to stream the partials at HBM rate.
@@ -855,12 +1245,8 @@ int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const Topo
                const void* color, const MlpPacked& W, const RayBuffers& R, const Records& rec,
                float* dw_partial, hipStream_t s) {
     if (bwd) {
-        if (bf16col)
-            hipLaunchKernelGGL((k_mlp<true, true>), dim3(grid), dim3(64 * kWaves<true>), 0, s, P, t, color, W, R,
-                               rec, dw_partial);
-        else
-            hipLaunchKernelGGL((k_mlp<true, false>), dim3(grid), dim3(64 * kWaves<true>), 0, s, P, t, color, W, R,
-                               rec, dw_partial);
+        hipLaunchKernelGGL(k_mlp_bwd, dim3(grid), dim3(64 * kBwdWaves), 0, s, P, W, R, rec,
+                           dw_partial);
     } else {
         if (bf16col)
             hipLaunchKernelGGL((k_mlp<false, true>), dim3(grid), dim3(64 * kWaves<false>), 0, s, P, t, color, W, R,
