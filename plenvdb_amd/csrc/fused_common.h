@@ -6,14 +6,14 @@
 //   k_ray_march    wave per ray: march again, density gather, raw2alpha, alpha cull,
 //                  transmittance product-scan with early stop, weights, weight cull;
 //                  one record per occupancy-kept sample (contiguous per ray)
-//   k_mlp<false>   workgroup per 32-record group: colour gather + MFMA MLP -> rgb,
+//   k_mlp_fwd      workgroup per 32-record group: colour gather + MFMA MLP -> rgb,
 //                  the features are kept per record as bf16 (hi, lo) pairs
 //   k_ray_composite wave per ray: prefix of w*rgb, ray colour, MSE loss and its
 //                  gradient per ray (k_loss_reduce sums the per-ray losses); a
 //                  second pass forms each sample's dL/dlogit and, through the
 //                  alpha/raw2alpha adjoints, dL/dsigma
-//   k_mlp<true>    workgroup per 32-record group: recompute from the kept
-//                  features, MFMA MLP backward, dW; per-record feature and
+//   k_mlp_bwd      workgroup per CU over 32-record groups: recompute from the
+//                  kept features, MFMA MLP backward, dW; per-record feature and
 //                  sigma gradients
 //   k_scatter_agg  workgroup per 128 records: corner contributions summed per
 //                  slot in an LDS hash, one device atomic per distinct slot

@@ -2,32 +2,28 @@
 // sigmoid; plenvdb/lib/dvgo.py:91-109,354-362) on MFMA, forward and backward,
 // over groups of 32 consecutive records (two 16-sample MFMA tiles).
 //
-// A workgroup of 4 wavefronts takes a group; wave w owns hidden features
-// [32w, 32w+32) of both hidden layers and exchanges activations with the
-// other waves through LDS.  Activations are computed transposed (features x
-// samples):
+// Activations are computed transposed (features x samples):
 //   H0^T = relu(W0^T X^T + b0),  H1^T = relu(W1^T H0^T + b1),  L^T = W2^T H1^T + b2
-// with v_mfma_f32_16x16x32_bf16 (samples are the N dimension).  Weights are
-// held in bf16; features, activations and gradients travel as (hi, lo) bf16
-// pairs, so each product with a weight is two MFMAs, which keeps about 16
-// significant bits where plain bf16 keeps 8 (plain bf16 inputs put the
-// colour-feature gradients several percent off an FP64 evaluation; the pairs
-// bring them under 1e-3).  dW products (activation x gradient) take the hi
-// parts only.
-// The forward weight slices stay in registers; the backward weights sit in LDS.  The
-// backward pass recomputes H0 and H1 (for the ReLU masks and the dW operands),
-// takes each sample's dL/dlogit and dL/dsigma from the composite pass, runs
-// g_H1^T = W2 g_L^T, g_H0^T = W1 g_H1^T (masked by the ReLUs, kept as bit
-// masks), g_X^T = W0[0:12] g_H0^T, and accumulates dW over the group's 32
-// samples (K = 32) into per-wave accumulator tiles written once per workgroup.
-// It writes each record's feature and sigma gradients; the scatter
+// with v_mfma_f32_16x16x32_bf16 (samples are the N dimension); each wavefront
+// owns a slice of the hidden features and exchanges activations with the
+// others through LDS.  Weights are held in bf16; features, activations and
+// gradients travel as (hi, lo) bf16 pairs, so each product with a weight is
+// two MFMAs, which keeps about 16 significant bits where plain bf16 keeps 8
+// (plain bf16 inputs put the colour-feature gradients several percent off an
+// FP64 evaluation; the pairs bring them under 1e-3).  dW products (activation
+// x gradient) take the hi parts only.
+//
+// k_mlp_fwd gathers the colour features, keeps them per record as (hi, lo)
+// bf16 (so the backward reads 48 bytes per record instead of redoing 8
+// gathers) and writes rgb.  k_mlp_bwd recomputes H0 and H1 (for the ReLU
+// masks and the dW operands), takes each sample's dL/dlogit and dL/dsigma from
+// the composite pass, runs g_H1^T = W2 g_L^T, g_H0^T = W1 g_H1^T (masked by
+// the ReLUs, kept as bit masks), g_X^T = W0[0:12] g_H0^T, accumulates dW over
+// each group's 32 samples (K = 32) in registers (one partial per workgroup),
+// and writes each record's feature and sigma gradients; the scatter
 // (fused_scatter.hip) then adds them to the grids with the "leaf must exist"
-// rule (densityvdb.cu:16-27, colorvdb.cu:28-37).  The forward pass keeps the
-// gathered features per record as (hi, lo) bf16, so the backward reads 48
-// bytes per record instead of redoing 8 gathers.  Each group's records,
-// features and view directions are loaded into registers while the previous
-// group computes.
-#include <type_traits>
+// rule (densityvdb.cu:16-27, colorvdb.cu:28-37).  Both kernels load a group's
+// records into registers while the previous group computes.
 
 #include "fused_common.h"
 #include "gather.h"
@@ -140,9 +136,15 @@ __device__ __forceinline__ f32x4 mfma_w(const bf16x8& a, const bf16x8& bh, const
 // step's samples average that rounding (a device check keeps every MLP
 // gradient within 5e-4 of FP64), so the gradients' lo parts stay sample-major.
 
+// ---- forward -----------------------------------------------------------------
+// A workgroup of 4 wavefronts takes a group of 32 records; wave w owns hidden
+// features [32w, 32w + 32) of both hidden layers (up to three workgroups per
+// CU).  Phases (a barrier after each): 1 commit the prefetched records, 2 gather
+// the colour features and form X, 3 H0, 4 H1, 5 logits -> rgb.  The arithmetic
+// of X, H0 and H1 is the backward kernel's, so its recomputed ReLU masks are
+// the ones this pass applied.
 struct RecInfo {
-    float x, y, z, e;
-    float alpha, T, w;
+    float x, y, z;
     uint32_t bits;
 };
 
@@ -153,75 +155,19 @@ struct alignas(16) SmemFwd {
     RecInfo ri[GS];
 };
 
-// Backward weights as MFMA A fragments, [tile][k-step][lane]:
-//   W2  [8 row tiles of 16 hidden][64] x 8 bytes   (K = 16 logits, 3 used)
-//   W1  [8 row tiles][4 k-steps][64] x 16 bytes    (K = 128 hidden outputs)
-//   W0  [4 k-steps of 32 hidden][64] x 16 bytes    (rows: the 12 features)
-constexpr int kBw2Frags = 8 * 64, kBw1Frags = 8 * 4 * 64, kBw0Frags = 4 * 64;
-
-// Sample-major activations are split (hi, lo) bf16 pairs (B operands of the
-// weight products); feature-major copies (dW operands) keep the hi part.  A1
-// holds H0 (steps 3-4) and then g_H1 (7-8); A2 holds H1 (4) and then g_H0
-// (8-9); F2 holds H1 feature-major (4-7) and then g_H0 (8-9).  Barriers
-// separate every reuse.  dL/dlogit, dL/dsigma and the view directions are
-// double-buffered: the next group's are staged from its prefetched records
-// while the current group is in step 7.
-struct alignas(16) SmemBwd {
-    uint4 Bw1[kBw1Frags];
-    uint4 Bw0[kBw0Frags];
-    uint2 Bw2[kBw2Frags];
-    uint16_t Xh[GS * XM], Xl[GS * XM];
-    uint16_t A1h[GS * SM], A1l[GS * SM];
-    uint16_t A2h[GS * SM], A2l[GS * SM];
-    uint16_t GLh[2][GS * GM], GLl[2][GS * GM];
-    uint16_t Xf[XF_ROWS * FM];
-    uint16_t H0f[128 * FM];
-    uint16_t F2[128 * FM];
-    uint16_t GLf[2][16 * FM];
-    uint16_t G1f[128 * FM];
-    float GX[4][16][GS];  // g_X^T partials over the four quarters of the hidden layer
-    float gsig[2][GS];
-    float4 vd[2][GS];  // view directions, double-buffered like the adjoints
-    RecInfo ri[GS];
-    uint4 nfeat[GS][3];  // the next group's kept features, parked after step 7
-};
-
 __device__ __forceinline__ float sigmoidf_(float x) { return 1.f / (1.f + expf(-x)); }
 
-// Register prefetch of one group's per-record inputs (thread = record).
-struct Prefetch {
-    float4 xyze;        // stage A (forward): position, exp(sigma + shift)
-    float4 atwr;        // stage A: alpha, T, w, ray id and flags
-    float4 adj;         // stage A (backward): dL/dlogit (3), dL/dsigma
-    float4 vd;          // stage B (backward): the ray's view direction
-    uint4 f0, f1, f2;   // stage B (backward): kept features (hi 12, lo 12)
-};
-
-// Wavefronts per workgroup: the forward runs 4 (32 hidden features each, up to
-// three workgroups per CU); the backward, one workgroup per CU for its LDS,
-// runs 8 (16 features each), so that every SIMD holds two wavefronts.
-template <bool BWD>
-constexpr int kWaves = BWD ? 8 : 4;
-
-template <bool BWD, bool BF16COL>
-__global__ void __launch_bounds__(64 * kWaves<BWD>) k_mlp(StepParams P, TopoView topo,
-                                                          const void* color, MlpPacked W,
-                                                          RayBuffers R, Records rec,
-                                                          float* __restrict__ dw_partial) {
-    using Smem = std::conditional_t<BWD, SmemBwd, SmemFwd>;
-    constexpr int NW = kWaves<BWD>, NTH = 64 * NW;
-    constexpr int MT = 8 / NW;  // 16-feature tiles per wave
-    __shared__ Smem sm;
+template <bool BF16COL>
+__global__ void __launch_bounds__(256) k_mlp_fwd(StepParams P, TopoView topo, const void* color,
+                                                 MlpPacked W, RayBuffers R, Records rec) {
+    constexpr int MT = 2;  // 16-feature tiles per wave
+    __shared__ SmemFwd sm;
     const int tid = threadIdx.x;
     const int wv = tid >> 6, lane = tid & 63, r16 = lane & 15, g = lane >> 4;
-    const int f0 = 16 * MT * wv;  // first hidden feature owned by this wave
-    // per-record staging (prefetch, adjoints, commit): the first GS / NW lanes of
-    // every wave, so that the SIMDs share it
-    const int pr = lane < GS / NW ? wv * (GS / NW) + lane : -1;
+    const int f0 = 32 * wv;
+    const int pr = lane < GS / 4 ? wv * (GS / 4) + lane : -1;  // staging lanes
 
-    // ---- forward weight slices: W1^T in registers; W0^T and W2^T (used by one
-    // step each) are re-read from L2 per group, which keeps the backward's
-    // registers for its dW accumulators ----
+    // W1^T slice in registers; W0^T and W2^T (one phase each) come from L2 per group
     bf16x8 A1[MT][4];
     float b0v[MT][4], b1v[MT][4], b2v[3];
 #pragma unroll
@@ -238,156 +184,39 @@ __global__ void __launch_bounds__(64 * kWaves<BWD>) k_mlp(StepParams P, TopoView
     b2v[0] = W.b2[0];
     b2v[1] = W.b2[1];
     b2v[2] = W.b2[2];
-
-    // ---- backward weights -> LDS, dW accumulators ----
-    f32x4 dW0[3][MT], dW1[8][MT], dW2[MT];
-    float db1[MT][4], db2[3];
-    if constexpr (BWD) {
-        for (int f = tid; f < kBw2Frags + kBw1Frags + kBw0Frags; f += NTH) {
-            const int ln = f & 63, gg = ln >> 4, rr = ln & 15;
-            if (f < kBw2Frags) {  // W2 [128 in][32 out (3 used)], K = 16
-                const int t = f >> 6;
-                sm.Bw2[f] = *reinterpret_cast<const uint2*>(W.w2 + (16 * t + rr) * 32 + 4 * gg);
-            } else if (f < kBw2Frags + kBw1Frags) {  // W1 [128 in][128 out]
-                const int q = f - kBw2Frags, t = q >> 8, kt = (q >> 6) & 3;
-                sm.Bw1[q] =
-                    *reinterpret_cast<const uint4*>(W.w1 + (16 * t + rr) * 128 + 32 * kt + 8 * gg);
-            } else {  // W0 rows 0..15 [16][128]
-                const int q = f - kBw2Frags - kBw1Frags, kt = q >> 6;
-                sm.Bw0[q] = *reinterpret_cast<const uint4*>(W.w0 + rr * 128 + 32 * kt + 8 * gg);
-            }
-        }
-#pragma unroll
-        for (int b = 0; b < MT; ++b) {
-#pragma unroll
-            for (int a = 0; a < 3; ++a) dW0[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int a = 0; a < 8; ++a) dW1[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
-            dW2[b] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int i = 0; i < 4; ++i) db1[b][i] = 0.f;
-        }
-        db2[0] = db2[1] = db2[2] = 0.f;
+    // X columns 40..63 are zero throughout
+    for (int e = tid; e < GS * 24; e += 256) {
+        const int row = e / 24, col = 40 + e % 24;
+        sm.Xh[row * XM + col] = 0;
+        sm.Xl[row * XM + col] = 0;
     }
 
     const int64_t total = min((int64_t)R.total[0], P.capacity);
     const int64_t ngroups = (total + GS - 1) / GS;
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    const uint4 zu4 = make_uint4(0u, 0u, 0u, 0u);
-
-    // ---- prefetch of the first group ----
-    Prefetch pf;
-    pf.xyze = pf.atwr = pf.adj = pf.vd = z4;
-    pf.f0 = pf.f1 = pf.f2 = zu4;
-    auto stage_a = [&](int64_t grp) {
+    float4 p_xyze = z4, p_atwr = z4;  // register prefetch of a group's records
+    auto stage = [&](int64_t grp) {
         const int64_t q = grp * GS + pr;
-        pf.xyze = pf.atwr = pf.adj = z4;
+        p_xyze = p_atwr = z4;
         if (pr >= 0 && grp < ngroups && q < total) {
-            pf.atwr = rec.atwr[q];
-            if constexpr (BWD)
-                pf.adj = rec.adj[q];
-            else
-                pf.xyze = rec.xyze[q];
+            p_xyze = rec.xyze[q];
+            p_atwr = rec.atwr[q];
         }
     };
-    auto stage_b = [&](int64_t grp) {  // backward only; needs stage A's atwr
-        const int64_t q = grp * GS + pr;
-        pf.vd = z4;
-        pf.f0 = pf.f1 = pf.f2 = zu4;
-        if (pr >= 0 && grp < ngroups && q < total) {
-            const uint32_t bits = __float_as_uint(pf.atwr.w);
-            if (bits & kAliveW) {
-                const uint32_t ray = bits & kRayMask;
-                pf.vd = make_float4(P.viewdirs[3 * ray], P.viewdirs[3 * ray + 1],
-                                    P.viewdirs[3 * ray + 2], 0.f);
-                pf.f0 = rec.feat[3 * q];
-                pf.f1 = rec.feat[3 * q + 1];
-                pf.f2 = rec.feat[3 * q + 2];
-            }
-        }
-    };
-    // backward: the prefetched group's per-sample adjoints (formed by the
-    // composite pass) into LDS buffer `b`, its kept features into the next-buffer
-    auto park = [&](int b) {
-        if constexpr (BWD) {
-            if (pr >= 0) {
-                const float gl[3] = {pf.adj.x, pf.adj.y, pf.adj.z};
-                const float gs = pf.adj.w;
-                sm.gsig[b][pr] = gs;
-                sm.vd[b][pr] = pf.vd;
-                db2[0] += gl[0];
-                db2[1] += gl[1];
-                db2[2] += gl[2];
-#pragma unroll
-                for (int o = 0; o < 3; ++o) {  // logits 3..15 stay zero
-                    uint16_t h, l;
-                    split_bf(gl[o], h, l);
-                    sm.GLh[b][pr * GM + o] = h;
-                    sm.GLl[b][pr * GM + o] = l;
-                    sm.GLf[b][o * FM + pr] = h;
-                }
-                sm.nfeat[pr][0] = pf.f0;
-                sm.nfeat[pr][1] = pf.f1;
-                sm.nfeat[pr][2] = pf.f2;
-            }
-        }
-    };
-    stage_a(blockIdx.x);
-    // X columns 40..63 (and, backward, feature-major rows 40..47) are zero throughout
-    for (int e = tid; e < GS * 24; e += NTH) {
-        const int row = e / 24, col = 40 + e % 24;
-        sm.Xh[row * XM + col] = 0;
-        sm.Xl[row * XM + col] = 0;
-    }
-    if constexpr (BWD) {
-        for (int e = tid; e < (XF_ROWS - 40) * FM; e += NTH) sm.Xf[40 * FM + e] = 0;
-        // the unused logits (3..15) of both dL/dlogit buffers are zero throughout
-        for (int e = tid; e < 2 * GS * GM; e += NTH) {
-            if (e % GM >= 3) {
-                (&sm.GLh[0][0])[e] = 0;
-                (&sm.GLl[0][0])[e] = 0;
-            }
-        }
-        for (int e = tid; e < 2 * 16 * FM; e += NTH)
-            if ((e % (16 * FM)) >= 3 * FM) (&sm.GLf[0][0])[e] = 0;
-        stage_b(blockIdx.x);
-        park(0);
-        __syncthreads();
-    }
-    int cb = 0;  // the current group's adjoint buffer
+    stage(blockIdx.x);
 
     for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {  // uniform bound
         const int64_t base = grp * GS;
-        // ---- 1. commit the prefetched group to LDS; start loading the next one ----
+        // ---- 1. commit the prefetched group; start loading the next one ----
         if (pr >= 0) {
             RecInfo ri;
-            ri.x = pf.xyze.x;
-            ri.y = pf.xyze.y;
-            ri.z = pf.xyze.z;
-            ri.e = pf.xyze.w;
-            ri.alpha = pf.atwr.x;
-            ri.T = pf.atwr.y;
-            ri.w = pf.atwr.z;
-            ri.bits = __float_as_uint(pf.atwr.w);  // zero for records past the end
+            ri.x = p_xyze.x;
+            ri.y = p_xyze.y;
+            ri.z = p_xyze.z;
+            ri.bits = __float_as_uint(p_atwr.w);  // zero for records past the end
             sm.ri[pr] = ri;
-            if constexpr (BWD) {
-                // kept features: hi halves 0..11 then lo halves 0..11
-                const uint4 f0 = sm.nfeat[pr][0], f1 = sm.nfeat[pr][1], f2 = sm.nfeat[pr][2];
-                const uint32_t w12[12] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y,
-                                          f1.z, f1.w, f2.x, f2.y, f2.z, f2.w};
-                uint32_t* xh = reinterpret_cast<uint32_t*>(&sm.Xh[pr * XM]);
-                uint32_t* xl = reinterpret_cast<uint32_t*>(&sm.Xl[pr * XM]);
-#pragma unroll
-                for (int c = 0; c < 6; ++c) {
-                    xh[c] = w12[c];
-                    xl[c] = w12[6 + c];
-                    sm.Xf[(2 * c) * FM + pr] = (uint16_t)(w12[c] & 0xffffu);
-                    sm.Xf[(2 * c + 1) * FM + pr] = (uint16_t)(w12[c] >> 16);
-                }
-            }
         }
-        const int64_t next = grp + gridDim.x;
-        stage_a(next);
+        stage(grp + gridDim.x);
         bf16x8 A0[MT][2];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
@@ -397,87 +226,80 @@ __global__ void __launch_bounds__(64 * kWaves<BWD>) k_mlp(StepParams P, TopoView
         __syncthreads();
         // ---- 2. X = [colour features (12) | PE (27) | 1 | 0...] ----
         {
-            const int s = tid >> 3, part = tid & 7;  // 8 threads per record
+            // colour gather: 8 threads per record, thread `part` takes corner `part`
+            const int s = tid >> 3, part = tid & 7;
             const RecInfo ri = sm.ri[s];
             const bool alive = (ri.bits & kAliveW) != 0u;
-            if constexpr (!BWD) {
-                // colour gather: thread `part` takes corner `part`
-                float v[12];
+            float v[12];
 #pragma unroll
-                for (int c = 0; c < 12; ++c) v[c] = 0.f;
-                if (alive) {
-                    const Cell cell = make_cell(ri.x, ri.y, ri.z);
-                    const int dx = (kCx >> part) & 1, dy = (kCy >> part) & 1, dz = (kCz >> part) & 1;
-                    const float a = dx ? cell.u : 1.f - cell.u;
-                    const float b = dy ? cell.v : 1.f - cell.v;
-                    const float d = dz ? cell.w : 1.f - cell.w;
-                    const float sc = (a * b) * d;
-                    const int64_t slot = slot_of(topo, cell.i0 + dx, cell.j0 + dy, cell.k0 + dz);
-                    if (slot >= 0) {
-                        if (BF16COL) {
-                            const uint16_t* src =
-                                reinterpret_cast<const uint16_t*>(color) + slot * 12;
-                            const uint2 p0 = *reinterpret_cast<const uint2*>(src);
-                            const uint2 p1 = *reinterpret_cast<const uint2*>(src + 4);
-                            const uint2 p2 = *reinterpret_cast<const uint2*>(src + 8);
-                            const uint32_t w6[6] = {p0.x, p0.y, p1.x, p1.y, p2.x, p2.y};
+            for (int c = 0; c < 12; ++c) v[c] = 0.f;
+            if (alive) {
+                const Cell cell = make_cell(ri.x, ri.y, ri.z);
+                const int dx = (kCx >> part) & 1, dy = (kCy >> part) & 1, dz = (kCz >> part) & 1;
+                const float a = dx ? cell.u : 1.f - cell.u;
+                const float b = dy ? cell.v : 1.f - cell.v;
+                const float d = dz ? cell.w : 1.f - cell.w;
+                const float sc = (a * b) * d;
+                const int64_t slot = slot_of(topo, cell.i0 + dx, cell.j0 + dy, cell.k0 + dz);
+                if (slot >= 0) {
+                    if (BF16COL) {
+                        const uint16_t* src = reinterpret_cast<const uint16_t*>(color) + slot * 12;
+                        const uint2 p0 = *reinterpret_cast<const uint2*>(src);
+                        const uint2 p1 = *reinterpret_cast<const uint2*>(src + 4);
+                        const uint2 p2 = *reinterpret_cast<const uint2*>(src + 8);
+                        const uint32_t w6[6] = {p0.x, p0.y, p1.x, p1.y, p2.x, p2.y};
 #pragma unroll
-                            for (int c = 0; c < 6; ++c) {
-                                v[2 * c] = bf_bits2f((uint16_t)(w6[c] & 0xffffu)) * sc;
-                                v[2 * c + 1] = bf_bits2f((uint16_t)(w6[c] >> 16)) * sc;
-                            }
-                        } else {
-                            const float4* src = reinterpret_cast<const float4*>(
-                                reinterpret_cast<const float*>(color) + slot * 12);
-                            const float4 p0 = src[0], p1 = src[1], p2 = src[2];
-                            v[0] = p0.x * sc; v[1] = p0.y * sc; v[2] = p0.z * sc; v[3] = p0.w * sc;
-                            v[4] = p1.x * sc; v[5] = p1.y * sc; v[6] = p1.z * sc; v[7] = p1.w * sc;
-                            v[8] = p2.x * sc; v[9] = p2.y * sc; v[10] = p2.z * sc; v[11] = p2.w * sc;
+                        for (int c = 0; c < 6; ++c) {
+                            v[2 * c] = bf_bits2f((uint16_t)(w6[c] & 0xffffu)) * sc;
+                            v[2 * c + 1] = bf_bits2f((uint16_t)(w6[c] >> 16)) * sc;
                         }
-                    }
-                }
-#pragma unroll
-                for (int c = 0; c < 12; ++c) {
-                    v[c] += __shfl_xor(v[c], 1, 64);
-                    v[c] += __shfl_xor(v[c], 2, 64);
-                    v[c] += __shfl_xor(v[c], 4, 64);
-                }
-                if (part == 0) {
-                    uint32_t wh[6], wl[6];
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) split2(v[2 * c], v[2 * c + 1], wh[c], wl[c]);
-                    uint32_t* xh = reinterpret_cast<uint32_t*>(&sm.Xh[s * XM]);
-                    uint32_t* xl = reinterpret_cast<uint32_t*>(&sm.Xl[s * XM]);
-#pragma unroll
-                    for (int c = 0; c < 6; ++c) {
-                        xh[c] = wh[c];
-                        xl[c] = wl[c];
-                    }
-                    const int64_t qq = base + s;
-                    if (qq < total && alive) {
-                        rec.feat[3 * qq] = make_uint4(wh[0], wh[1], wh[2], wh[3]);
-                        rec.feat[3 * qq + 1] = make_uint4(wh[4], wh[5], wl[0], wl[1]);
-                        rec.feat[3 * qq + 2] = make_uint4(wl[2], wl[3], wl[4], wl[5]);
+                    } else {
+                        const float4* src = reinterpret_cast<const float4*>(
+                            reinterpret_cast<const float*>(color) + slot * 12);
+                        const float4 p0 = src[0], p1 = src[1], p2 = src[2];
+                        v[0] = p0.x * sc; v[1] = p0.y * sc; v[2] = p0.z * sc; v[3] = p0.w * sc;
+                        v[4] = p1.x * sc; v[5] = p1.y * sc; v[6] = p1.z * sc; v[7] = p1.w * sc;
+                        v[8] = p2.x * sc; v[9] = p2.y * sc; v[10] = p2.z * sc; v[11] = p2.w * sc;
                     }
                 }
             }
+#pragma unroll
+            for (int c = 0; c < 12; ++c) {
+                v[c] += __shfl_xor(v[c], 1, 64);
+                v[c] += __shfl_xor(v[c], 2, 64);
+                v[c] += __shfl_xor(v[c], 4, 64);
+            }
+            if (part == 0) {
+                uint32_t wh[6], wl[6];
+#pragma unroll
+                for (int c = 0; c < 6; ++c) split2(v[2 * c], v[2 * c + 1], wh[c], wl[c]);
+                uint32_t* xh = reinterpret_cast<uint32_t*>(&sm.Xh[s * XM]);
+                uint32_t* xl = reinterpret_cast<uint32_t*>(&sm.Xl[s * XM]);
+#pragma unroll
+                for (int c = 0; c < 6; ++c) {
+                    xh[c] = wh[c];
+                    xl[c] = wl[c];
+                }
+                // the features kept for the backward: hi halves 0..11, then lo
+                const int64_t qq = base + s;
+                if (qq < total && alive) {
+                    rec.feat[3 * qq] = make_uint4(wh[0], wh[1], wh[2], wh[3]);
+                    rec.feat[3 * qq + 1] = make_uint4(wh[4], wh[5], wl[0], wl[1]);
+                    rec.feat[3 * qq + 2] = make_uint4(wl[2], wl[3], wl[4], wl[5]);
+                }
+            }
         }
-        // positional encoding of the ray's view direction (dvgo.py:354-358) and the
-        // bias input: thread (sample ps, part pp < 7) writes columns 12 + 4pp .. +3.
-        // Columns 40..63 were zeroed once.  sin/cos on the hardware units: their
-        // error (about 1e-6 absolute for these arguments, |arg| <= 8) is below what
-        // the (hi, lo) pair keeps.
         {
+            // positional encoding (dvgo.py:354-358) and the bias column: thread
+            // (sample ps, part pp < 7) writes X columns 12 + 4pp .. +3; hardware
+            // sin/cos (error about 1e-6 for |arg| <= 8, below what the (hi, lo) pair
+            // keeps), as in the backward
             const int ps = tid & (GS - 1), pp = tid >> 5;
-            const bool alive = (sm.ri[ps].bits & kAliveW) != 0u;
+            const uint32_t bits = sm.ri[ps].bits;
+            const bool alive = (bits & kAliveW) != 0u;
             float vd[3] = {0.f, 0.f, 0.f};
-            if constexpr (BWD) {  // prefetched (zero unless alive)
-                const float4 v4 = sm.vd[cb][ps];
-                vd[0] = v4.x;
-                vd[1] = v4.y;
-                vd[2] = v4.z;
-            } else if (alive) {
-                const uint32_t ray = sm.ri[ps].bits & kRayMask;
+            if (alive) {
+                const uint32_t ray = bits & kRayMask;
                 vd[0] = P.viewdirs[3 * ray];
                 vd[1] = P.viewdirs[3 * ray + 1];
                 vd[2] = P.viewdirs[3 * ray + 2];
@@ -491,8 +313,8 @@ __global__ void __launch_bounds__(64 * kWaves<BWD>) k_mlp(StepParams P, TopoView
                     if (p < 3) {
                         v = vd[p];
                     } else if (p < 27) {
-                        const int idx = p - 3, axis = (idx % 12) / 4, fr = idx % 4;
-                        const float arg = vd[axis] * (float)(1 << fr);
+                        const int idx = p - 3, axis = (idx % 12) / 4, fq = idx % 4;
+                        const float arg = vd[axis] * (float)(1 << fq);
                         v = idx < 12 ? __sinf(arg) : __cosf(arg);
                     } else {
                         v = 1.f;
@@ -507,17 +329,10 @@ __global__ void __launch_bounds__(64 * kWaves<BWD>) k_mlp(StepParams P, TopoView
                 *reinterpret_cast<uint2*>(&sm.Xh[ps * XM + 12 + 4 * pp]) = hi;
                 *reinterpret_cast<uint2*>(&sm.Xl[ps * XM + 12 + 4 * pp]) = lo;
             }
-            if constexpr (BWD) {
-                const uint2 t = quad_transpose(hi, lane & 3);  // the quad: 4 samples, one part
-                if (pp < 7)
-                    *reinterpret_cast<uint2*>(&sm.Xf[(12 + 4 * pp + (lane & 3)) * FM + (ps & ~3)]) = t;
-            }
         }
         __syncthreads();
-        // ---- 3. H0^T slice = relu(W0^T X^T + b0) -> A1 (and H0f) ----
-        uint32_t m0 = 0u, m1 = 0u;  // ReLU masks: bit (mt*NT + nt)*4 + i
-        // the weight products run k-step outer: each B fragment is read once and
-        // the four (mt, nt) accumulator chains interleave
+        // ---- 3. H0^T slice = relu(W0^T X^T + b0) -> A1 ----
+        // k-step outer: each B fragment is read once and the (mt, nt) chains interleave
         f32x4 acc[MT][NT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
@@ -539,26 +354,16 @@ __global__ void __launch_bounds__(64 * kWaves<BWD>) k_mlp(StepParams P, TopoView
                     acc[mt][nt] = mfma_w(A0[mt][kt], bh[nt], bl[nt], acc[mt][nt]);
         }
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-            const int fr = f0 + 16 * mt + 4 * g;
+        for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
                 float h[4];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    h[i] = fmaxf(acc[mt][nt][i] + b0v[mt][i], 0.f);
-                    if (h[i] > 0.f) m0 |= 1u << ((mt * NT + nt) * 4 + i);
-                }
-                if constexpr (BWD)
-                    st_act<true>(sm.A1h, sm.A1l, sm.H0f, 16 * nt + r16, fr, lane, h);
-                else
-                    st_act<false>(sm.A1h, sm.A1l, nullptr, 16 * nt + r16, fr, lane, h);
+                for (int i = 0; i < 4; ++i) h[i] = fmaxf(acc[mt][nt][i] + b0v[mt][i], 0.f);
+                st_act<false>(sm.A1h, sm.A1l, nullptr, 16 * nt + r16, f0 + 16 * mt + 4 * g, lane, h);
             }
-        }
-        // the next group's stage-A loads (issued in step 1) have landed: start stage B
-        if constexpr (BWD) stage_b(next);
         __syncthreads();
-        // ---- 4. H1^T slice = relu(W1^T H0^T + b1) -> A2 (and H1f in F2) ----
+        // ---- 4. H1^T slice = relu(W1^T H0^T + b1) -> A2 ----
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -579,231 +384,45 @@ __global__ void __launch_bounds__(64 * kWaves<BWD>) k_mlp(StepParams P, TopoView
                     acc[mt][nt] = mfma_w(A1[mt][kt], bh[nt], bl[nt], acc[mt][nt]);
         }
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-            const int fr = f0 + 16 * mt + 4 * g;
+        for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
                 float h[4];
 #pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    h[i] = fmaxf(acc[mt][nt][i] + b1v[mt][i], 0.f);
-                    if (h[i] > 0.f) m1 |= 1u << ((mt * NT + nt) * 4 + i);
-                }
-                if constexpr (BWD)
-                    st_act<true>(sm.A2h, sm.A2l, sm.F2, 16 * nt + r16, fr, lane, h);
-                else
-                    st_act<false>(sm.A2h, sm.A2l, nullptr, 16 * nt + r16, fr, lane, h);
+                for (int i = 0; i < 4; ++i) h[i] = fmaxf(acc[mt][nt][i] + b1v[mt][i], 0.f);
+                st_act<false>(sm.A2h, sm.A2l, nullptr, 16 * nt + r16, f0 + 16 * mt + 4 * g, lane, h);
             }
-        }
         __syncthreads();
-        if constexpr (!BWD) {
-            // ---- 5. logits (wave nt takes tile nt): L^T = W2^T H1^T + b2, rgb = sigmoid ----
-            if (wv < NT) {
-                const int nt = wv;
-                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-                const int ho = (16 * nt + r16) * SM + 8 * g;
+        // ---- 5. logits (wave nt takes tile nt): L^T = W2^T H1^T + b2, rgb = sigmoid ----
+        if (wv < NT) {
+            const int nt = wv;
+            f32x4 a5 = {0.f, 0.f, 0.f, 0.f};
+            const int ho = (16 * nt + r16) * SM + 8 * g;
 #pragma unroll
-                for (int kt = 0; kt < 4; ++kt)
-                    acc = mfma_w(ld_frag(W.w2t + r16 * 128 + 32 * kt + 8 * g),
-                                 ld_frag(&sm.A2h[ho + 32 * kt]), ld_frag(&sm.A2l[ho + 32 * kt]),
-                                 acc);
-                if (g == 0) {
-                    const int s = 16 * nt + r16;
-                    const int64_t q = base + s;
-                    if (q < total) {
-                        const bool alive = (sm.ri[s].bits & kAliveW) != 0u;
-                        rec.rgb[q] = alive ? make_float4(sigmoidf_(acc[0] + b2v[0]),
-                                                         sigmoidf_(acc[1] + b2v[1]),
-                                                         sigmoidf_(acc[2] + b2v[2]), 0.f)
-                                           : z4;
-                    }
-                }
-            }
-            __syncthreads();
-            continue;
-        } else {
-            // no logits here: the composite pass formed the per-sample adjoints
-            // from the rgb the forward pass stored (same inputs through the same
-            // code, so the ReLU masks recomputed here are the forward's)
-            // ---- 7. g_H1^T slice = W2[slice] g_L^T, masked -> A1 (and G1f); dW2 ----
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-                const s16x4 a = __builtin_bit_cast(s16x4, sm.Bw2[(wv * MT + mt) * 64 + lane]);
-                const int fr = f0 + 16 * mt + 4 * g;
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-                    const int go = (16 * nt + r16) * GM + 4 * g;
-                    acc = mfma16x16x16(a, ld_frag4(&sm.GLh[cb][go]), acc);
-                    acc = mfma16x16x16(a, ld_frag4(&sm.GLl[cb][go]), acc);
-                    float gv[4];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        gv[i] = (m1 >> ((mt * NT + nt) * 4 + i)) & 1u ? acc[i] : 0.f;
-                        db1[mt][i] += gv[i];
-                    }
-                    st_act<true>(sm.A1h, sm.A1l, sm.G1f, 16 * nt + r16, fr, lane, gv);
-                }
-                // dW2[in = f0+16mt+row][out] += sum over the group's samples H1[s][in] gL[s][out]
-                const int ho = (f0 + 16 * mt + r16) * FM + 8 * g, lo_ = r16 * FM + 8 * g;
-                dW2[mt] = mfma16x16x32(ld_frag8(&sm.F2[ho]), ld_frag8(&sm.GLf[cb][lo_]), dW2[mt]);
-            }
-            park(cb ^ 1);  // the next group's stage-B loads (issued after step 3) have landed
-            __syncthreads();
-            // ---- 8. g_H0^T slice = W1[slice] g_H1^T, masked -> A2 (and G0f in F2); dW1 ----
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int kt = 0; kt < 4; ++kt) {
-                bf16x8 bh[NT], bl[NT];
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    const int go = (16 * nt + r16) * SM + 32 * kt + 8 * g;
-                    bh[nt] = ld_frag(&sm.A1h[go]);
-                    bl[nt] = ld_frag(&sm.A1l[go]);
-                }
-#pragma unroll
-                for (int mt = 0; mt < MT; ++mt) {
-                    const bf16x8 a = as_bf8(sm.Bw1[((wv * MT + mt) * 4 + kt) * 64 + lane]);
-#pragma unroll
-                    for (int nt = 0; nt < NT; ++nt)
-                        acc[mt][nt] = mfma_w(a, bh[nt], bl[nt], acc[mt][nt]);
-                }
-            }
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt) {
-                const int fr = f0 + 16 * mt + 4 * g;
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt) {
-                    float gv[4];
-#pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        gv[i] = (m0 >> ((mt * NT + nt) * 4 + i)) & 1u ? acc[mt][nt][i] : 0.f;
-                    st_act<true>(sm.A2h, sm.A2l, sm.F2, 16 * nt + r16, fr, lane, gv);
-                }
-            }
-            {
-                bf16x8 bh[MT];
-#pragma unroll
-                for (int nt = 0; nt < MT; ++nt) bh[nt] = ld_frag8(&sm.G1f[(f0 + 16 * nt + r16) * FM + 8 * g]);
-#pragma unroll
-                for (int mi = 0; mi < 8; ++mi) {
-                    const bf16x8 ah = ld_frag8(&sm.H0f[(16 * mi + r16) * FM + 8 * g]);
-#pragma unroll
-                    for (int nt = 0; nt < MT; ++nt) dW1[mi][nt] = mfma16x16x32(ah, bh[nt], dW1[mi][nt]);
-                }
-            }
-            __syncthreads();
-            // ---- 9. dW0 columns of this wave; g_X^T over this wave's K slice ----
-            {
-                bf16x8 bh[MT];
-#pragma unroll
-                for (int nt = 0; nt < MT; ++nt) bh[nt] = ld_frag8(&sm.F2[(f0 + 16 * nt + r16) * FM + 8 * g]);
-#pragma unroll
-                for (int mi = 0; mi < 3; ++mi) {
-                    const bf16x8 ah = ld_frag8(&sm.Xf[(16 * mi + r16) * FM + 8 * g]);
-#pragma unroll
-                    for (int nt = 0; nt < MT; ++nt) dW0[mi][nt] = mfma16x16x32(ah, bh[nt], dW0[mi][nt]);
-                }
-                // g_X^T: wave w takes sample tile w & 1 over the hidden part w >> 1
-                // (NW / 2 parts of 4 / (NW / 2) k-steps each)
-                constexpr int KPW = 8 / NW;  // k-steps per part
-                const int nt = wv & 1, kh = wv >> 1;
-                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                for (int kk = 0; kk < KPW; ++kk) {
-                    const int kt = KPW * kh + kk;
-                    const int go = (16 * nt + r16) * SM + 32 * kt + 8 * g;
-                    acc = mfma_w(as_bf8(sm.Bw0[kt * 64 + lane]), ld_frag(&sm.A2h[go]),
-                                 ld_frag(&sm.A2l[go]), acc);
-                }
-#pragma unroll
-                for (int i = 0; i < 4; ++i) sm.GX[kh][4 * g + i][16 * nt + r16] = acc[i];
-            }
-            __syncthreads();
-            // ---- 10. per-record gradients: dL/dfeature (12) and dL/dsigma ----
-            if (tid < 4 * GS) {
-                const int s = tid >> 2, part = tid & 3;  // 4 threads write a record's 4 float4
-                const int64_t q = base + s;
+            for (int kt = 0; kt < 4; ++kt)
+                a5 = mfma_w(ld_frag(W.w2t + r16 * 128 + 32 * kt + 8 * g),
+                            ld_frag(&sm.A2h[ho + 32 * kt]), ld_frag(&sm.A2l[ho + 32 * kt]), a5);
+            if (g == 0) {
+                const int sidx = 16 * nt + r16;
+                const int64_t q = base + sidx;
                 if (q < total) {
-                    const RecInfo ri = sm.ri[s];
-                    float4 v = z4;
-                    if (part < 3) {
-                        if (ri.bits & kAliveW) {
-                            const int c = 4 * part;
-                            float x[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                            for (int k = 0; k < NW / 2; ++k)
-#pragma unroll
-                                for (int i = 0; i < 4; ++i) x[i] += sm.GX[k][c + i][s];
-                            v = make_float4(x[0], x[1], x[2], x[3]);
-                        }
-                    } else {
-                        v.x = sm.gsig[cb][s];
-                    }
-                    rec.grad[4 * q + part] = v;
+                    const bool alive = (sm.ri[sidx].bits & kAliveW) != 0u;
+                    rec.rgb[q] = alive ? make_float4(sigmoidf_(a5[0] + b2v[0]),
+                                                     sigmoidf_(a5[1] + b2v[1]),
+                                                     sigmoidf_(a5[2] + b2v[2]), 0.f)
+                                       : z4;
                 }
             }
-            __syncthreads();
-            cb ^= 1;
-        }
-    }
-
-    if constexpr (BWD) {
-        // ---- flush this workgroup's dW partial ----
-        float* out = dw_partial + (int64_t)blockIdx.x * kMlpParams;
-#pragma unroll
-        for (int mi = 0; mi < 3; ++mi)
-#pragma unroll
-            for (int nt = 0; nt < MT; ++nt)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const int in = 16 * mi + 4 * g + i, o = f0 + 16 * nt + r16;
-                    if (in < 39)
-                        out[kW0 + in * 128 + o] = dW0[mi][nt][i];
-                    else if (in == 39)
-                        out[kB0 + o] = dW0[mi][nt][i];
-                }
-#pragma unroll
-        for (int mi = 0; mi < 8; ++mi)
-#pragma unroll
-            for (int nt = 0; nt < MT; ++nt)
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    out[kW1 + (16 * mi + 4 * g + i) * 128 + f0 + 16 * nt + r16] = dW1[mi][nt][i];
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                if (r16 < 3) out[kW2 + (f0 + 16 * mt + 4 * g + i) * 3 + r16] = dW2[mt][i];
-                float v = db1[mt][i];
-                v += __shfl_xor(v, 1, 64);
-                v += __shfl_xor(v, 2, 64);
-                v += __shfl_xor(v, 4, 64);
-                v += __shfl_xor(v, 8, 64);
-                if (r16 == 0) out[kB1 + f0 + 16 * mt + 4 * g + i] = v;
-            }
-        // db2: the staging lanes of every wave accumulated it (park); the loop
-        // ended on a barrier, so gsig is free for the per-wave sums
-#pragma unroll
-        for (int o = 0; o < 3; ++o) {
-            float v = pr >= 0 ? db2[o] : 0.f;
-            v += __shfl_xor(v, 1, 64);
-            v += __shfl_xor(v, 2, 64);
-            v += __shfl_xor(v, 4, 64);
-            if (lane == 0) sm.gsig[0][4 * wv + o] = v;
         }
         __syncthreads();
-        if (tid < 3) {
-            float v = 0.f;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) v += sm.gsig[0][4 * w + tid];
-            out[kB2 + tid] = v;
-        }
     }
 }
+
+// Backward weights as MFMA A fragments, [tile][k-step][lane]:
+//   W2  [8 row tiles of 16 hidden][64] x 8 bytes   (K = 16 logits, 3 used)
+//   W1  [8 row tiles][4 k-steps][64] x 16 bytes    (K = 128 hidden outputs)
+//   W0  [4 k-steps of 32 hidden][64] x 16 bytes    (rows: the 12 features)
+constexpr int kBw2Frags = 8 * 64, kBw1Frags = 8 * 4 * 64, kBw0Frags = 4 * 64;
 
 // ---- backward ----------------------------------------------------------------
 // One workgroup of 8 wavefronts per CU (two per SIMD); wave w owns hidden
@@ -820,7 +439,7 @@ __global__ void __launch_bounds__(64 * kWaves<BWD>) k_mlp(StepParams P, TopoView
 // X (features, encoding, bias), the per-sample adjoints and the flags are
 // double-buffered, so the next group's inputs are staged while this one runs.
 constexpr int kBwdWaves = 8;
-struct alignas(16) SmemBwd2 {
+struct alignas(16) SmemBwd {
     uint4 Bw1[kBw1Frags];
     uint4 Bw0[kBw0Frags];
     uint2 Bw2[kBw2Frags];
@@ -842,7 +461,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
                                                             RayBuffers R, Records rec,
                                                             float* __restrict__ dw_partial) {
     constexpr int NTH = 64 * kBwdWaves;
-    __shared__ SmemBwd2 sm;
+    __shared__ SmemBwd sm;
     const int tid = threadIdx.x;
     const int wv = tid >> 6, lane = tid & 63, r16 = lane & 15, g = lane >> 4;
     const int f0 = 16 * wv;
@@ -1247,15 +866,14 @@ int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const Topo
     if (bwd) {
         hipLaunchKernelGGL(k_mlp_bwd, dim3(grid), dim3(64 * kBwdWaves), 0, s, P, W, R, rec,
                            dw_partial);
+        PVDB_LAUNCHED("k_mlp_bwd");
     } else {
         if (bf16col)
-            hipLaunchKernelGGL((k_mlp<false, true>), dim3(grid), dim3(64 * kWaves<false>), 0, s, P, t, color, W, R,
-                               rec, dw_partial);
+            hipLaunchKernelGGL((k_mlp_fwd<true>), dim3(grid), dim3(256), 0, s, P, t, color, W, R, rec);
         else
-            hipLaunchKernelGGL((k_mlp<false, false>), dim3(grid), dim3(64 * kWaves<false>), 0, s, P, t, color, W,
-                               R, rec, dw_partial);
+            hipLaunchKernelGGL((k_mlp_fwd<false>), dim3(grid), dim3(256), 0, s, P, t, color, W, R, rec);
+        PVDB_LAUNCHED("k_mlp_fwd");
     }
-    PVDB_LAUNCHED("k_mlp");
     return PVDB_OK;
 }
 
