@@ -6,12 +6,12 @@
 // and gathers them, :312-367), a host read-back of the sample total, four
 // SGEMMs over the materialised sample list (cuda_rgbnet, :83-109) and an
 // atomic composite (final_render, :112-119).  Here each wavefront takes an
-// 8x8 pixel tile, one pixel per lane (a lane whose ray ends takes the next
-// pixel), and marches its 64 rays at once with the reference's sample rule (first sample at tmin+step, the [0,1]^3 box test,
+// 8x8 pixel tile, one pixel per lane, and marches all 64 rays at once with the
+// reference's sample rule (first sample at tmin+step, the [0,1]^3 box test,
 // isActive(Round(xyz)), raw2alpha, the two fast_color_thres culls, stop after a
-// kept sample that leaves T < t_stop).  When half the wave (or every live
-// lane) holds its next kept sample, the wave evaluates the RGB MLP for those
-// samples on v_mfma_f32_16x16x32_f16 and composites them in registers.  No sample list,
+// kept sample that leaves T < t_stop).  When every live lane holds its next
+// kept sample, the wave evaluates the RGB MLP for those (up to 64) samples on
+// v_mfma_f32_16x16x32_f16 and composites them in registers.  No sample list,
 // no host synchronisation, no allocation.
 //
 // Data layout.  The merged index grid (vdb_compression.py:19-59: index 1 +
@@ -258,62 +258,6 @@ struct BlockCache {
     }
 };
 
-// Per-pixel ray (get_rays, renderer.cu:122-167; get_tminmax :50-64): normalised
-// origin and direction, world step, slab bounds with far forced to 1e9
-// (plenvdb.h:1008), and the 27-d PE of the view direction as fp16 pairs (the
-// last pair carries the bias input 1).
-struct PixelRay {
-    float o[3], d[3], step, tmin, tmax;
-    uint32_t pe[14];
-};
-
-__device__ __forceinline__ void pixel_ray(const RenderArgs& A, int px, int py, PixelRay& R) {
-    const float pi = (float)px + 0.5f, pj = (float)py + 0.5f;
-    const float dx = (pi - A.cx) / A.fx;
-    const float dy0 = (pj - A.cy) / A.fy;
-    const float dy = A.inverse_y ? dy0 : -dy0;
-    const float dz = A.inverse_y ? 1.f : -1.f;
-    float rd[3];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        R.o[a] = (A.c2w[4 * a + 3] - A.mn[a]) / A.ext[a];
-        rd[a] = dx * A.c2w[4 * a] + dy * A.c2w[4 * a + 1] + dz * A.c2w[4 * a + 2];
-    }
-    const float len = sqrtf(rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2]);
-    R.step = A.stepdist / len;
-    float lo = -INFINITY, hi = INFINITY;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        R.d[a] = rd[a] / A.ext[a];
-        const float v = R.d[a] == 0.f ? 1e-6f : R.d[a];
-        const float ta = (1.f - R.o[a]) / v, tb = (-R.o[a]) / v;
-        lo = a == 0 ? fminf(ta, tb) : fmaxf(lo, fminf(ta, tb));
-        hi = a == 0 ? fmaxf(ta, tb) : fminf(hi, fmaxf(ta, tb));
-    }
-    R.tmin = fmaxf(fminf(lo, 1e9f), A.near_);
-    R.tmax = fmaxf(fminf(hi, 1e9f), A.near_);
-    const float inv = 1.f / len;
-    const float vd[3] = {rd[0] * inv, rd[1] * inv, rd[2] * inv};
-    float p27[27];
-    p27[0] = vd[0];
-    p27[1] = vd[1];
-    p27[2] = vd[2];
-    float base = 1.f;
-#pragma unroll
-    for (int q = 3; q < 7; ++q) {
-        p27[q] = sinf(vd[0] * base);
-        p27[q + 4] = sinf(vd[1] * base);
-        p27[q + 8] = sinf(vd[2] * base);
-        p27[q + 12] = cosf(vd[0] * base);
-        p27[q + 16] = cosf(vd[1] * base);
-        p27[q + 20] = cosf(vd[2] * base);
-        base *= 2.f;
-    }
-#pragma unroll
-    for (int q = 0; q < 13; ++q) R.pe[q] = pack2(p27[2 * q], p27[2 * q + 1]);
-    R.pe[13] = pack2(p27[26], 1.f);
-}
-
 template <bool F32PAY>
 __global__ void __launch_bounds__(64 * kWaves) k_render(RenderArgs A) {
     extern __shared__ uint4 smem[];
@@ -335,157 +279,171 @@ __global__ void __launch_bounds__(64 * kWaves) k_render(RenderArgs A) {
 
     const TopoView& topo = A.topo;
     const int64_t miss = A.nslots;  // payload slot of index 0
-    const uint64_t below = (1ull << lane) - 1ull;
     unsigned long long n_gather = 0, n_kept = 0;
-
-    // Lanes take pixels from the wave's current 8x8 tile and refill from the
-    // next tile when their ray ends, so the wave stays full until the image
-    // runs out.  The MLP runs when half the wave holds a kept sample, or when
-    // every live lane does.
-    int cur_tile = 0, cur_pos = 64;  // wave-uniform tile pool
-    bool pool_done = false;
-    bool active = false, have = false, stop_after = false;
-    int px = 0, py = 0, iter = 0, max_iter = 0;
-    float t = 0.f, T = 1.f, acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, wt_keep = 0.f;
-    PixelRay R;
-    BlockCache bc;
-    auto finish = [&]() {  // final_render: T*bg plus the kept samples (renderer.cu:112-119,363-366)
-        const float last = T * A.bg;
-        float* dst = A.out + ((int64_t)py * A.W + px) * 3;
-        dst[0] = last + acc0;
-        dst[1] = last + acc1;
-        dst[2] = last + acc2;
-        active = false;
-    };
     while (true) {
-        // ---- refill idle lanes ----
-        uint64_t need = __ballot(!active);
-        while (need && !pool_done) {  // uniform
-            if (cur_pos >= 64) {
-                int nt = 0;
-                if (lane == 0) nt = atomicAdd(A.tile_counter, 1);
-                cur_tile = __shfl(nt, 0, 64);
-                cur_pos = 0;
-                if (cur_tile >= A.ntiles) {
-                    pool_done = true;
+        int tile = 0;
+        if (lane == 0) tile = atomicAdd(A.tile_counter, 1);
+        tile = __shfl(tile, 0, 64);
+        if (tile >= A.ntiles) break;  // uniform over the wave
+        const int px = (tile % A.ntx) * 8 + (lane & 7), py = (tile / A.ntx) * 8 + (lane >> 3);
+        const bool valid = px < A.W && py < A.H;
+
+        // ---- ray of the pixel (get_rays, renderer.cu:122-167; get_tminmax :50-64) ----
+        float o[3], d[3], step = 0.f, tmin = 0.f, tmax = 0.f;
+        uint32_t pe[14];
+        {
+            const float pi = (float)px + 0.5f, pj = (float)py + 0.5f;
+            const float dx = (pi - A.cx) / A.fx;
+            const float dy0 = (pj - A.cy) / A.fy;
+            const float dy = A.inverse_y ? dy0 : -dy0;
+            const float dz = A.inverse_y ? 1.f : -1.f;
+            float rd[3];
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                o[a] = (A.c2w[4 * a + 3] - A.mn[a]) / A.ext[a];
+                rd[a] = dx * A.c2w[4 * a] + dy * A.c2w[4 * a + 1] + dz * A.c2w[4 * a + 2];
+            }
+            const float len = sqrtf(rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2]);
+            step = A.stepdist / len;
+            float lo = -INFINITY, hi = INFINITY;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                d[a] = rd[a] / A.ext[a];
+                const float v = d[a] == 0.f ? 1e-6f : d[a];
+                const float ta = (1.f - o[a]) / v, tb = (-o[a]) / v;
+                lo = a == 0 ? fminf(ta, tb) : fmaxf(lo, fminf(ta, tb));
+                hi = a == 0 ? fmaxf(ta, tb) : fminf(hi, fmaxf(ta, tb));
+            }
+            tmin = fmaxf(fminf(lo, 1e9f), A.near_);  // far is forced to 1e9 (plenvdb.h:1008)
+            tmax = fmaxf(fminf(hi, 1e9f), A.near_);
+            const float inv = 1.f / len;
+            const float vd[3] = {rd[0] * inv, rd[1] * inv, rd[2] * inv};
+            float p27[27];
+            p27[0] = vd[0];
+            p27[1] = vd[1];
+            p27[2] = vd[2];
+            float base = 1.f;
+#pragma unroll
+            for (int q = 3; q < 7; ++q) {
+                p27[q] = sinf(vd[0] * base);
+                p27[q + 4] = sinf(vd[1] * base);
+                p27[q + 8] = sinf(vd[2] * base);
+                p27[q + 12] = cosf(vd[0] * base);
+                p27[q + 16] = cosf(vd[1] * base);
+                p27[q + 20] = cosf(vd[2] * base);
+                base *= 2.f;
+            }
+#pragma unroll
+            for (int q = 0; q < 13; ++q) pe[q] = pack2(p27[2 * q], p27[2 * q + 1]);
+            pe[13] = pack2(p27[26], 1.f);  // PE 26 and the bias input
+        }
+        // march bound: the reference loop ends when t reaches tmax; the cap only
+        // guards steps too small to advance t
+        const float span = (tmax - tmin) / step;
+        const int max_iter = (span >= 0.f) ? (int)fminf(span * 1.0625f + 16.f, 16777216.f) : 0;
+
+        float t = tmin, T = 1.f, acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, wt_keep = 0.f;
+        bool done = !valid || !(t < tmax), have = false, stop_after = false;
+        int iter = 0;
+        BlockCache bc;
+        while (true) {
+            while (!done && !have) {
+                if (!(t < tmax) || iter >= max_iter) {
+                    done = true;
                     break;
                 }
-            }
-            const int rank = __popcll(need & below);
-            const int navail = 64 - cur_pos, nneed = __popcll(need);
-            const int ntake = nneed < navail ? nneed : navail;
-            if (!active && rank < ntake) {
-                const int li = cur_pos + rank;
-                px = (cur_tile % A.ntx) * 8 + (li & 7);
-                py = (cur_tile / A.ntx) * 8 + (li >> 3);
-                if (px < A.W && py < A.H) {
-                    pixel_ray(A, px, py, R);
-                    t = R.tmin;
-                    T = 1.f;
-                    acc0 = acc1 = acc2 = 0.f;
-                    have = stop_after = false;
-                    iter = 0;
-                    // the cap only guards steps too small to advance t
-                    const float span = (R.tmax - R.tmin) / R.step;
-                    max_iter = (span >= 0.f) ? (int)fminf(span * 1.0625f + 16.f, 16777216.f) : 0;
-                    active = true;
-                    if (!(t < R.tmax) || max_iter == 0) finish();  // no sample on this ray
+                t += step;
+                ++iter;
+                const float p0 = o[0] + t * d[0], p1 = o[1] + t * d[1], p2 = o[2] + t * d[2];
+                if (!(p0 >= 0.f && p1 >= 0.f && p2 >= 0.f && p0 <= 1.f && p1 <= 1.f && p2 <= 1.f))
+                    continue;
+                const float x = p0 * A.ws[0], y = p1 * A.ws[1], z = p2 * A.ws[2];
+                // isActive(Round(xyz)): rintf, half to even (NanoVDB.h:774-779)
+                {
+                    const int vi = (int)rintf(x), vj = (int)rintf(y), vk = (int)rintf(z);
+                    const int32_t l = bc.get(topo, vi, vj, vk);
+                    if (l < 0) continue;
+                    const int off = voxel_offset(vi, vj, vk);
+                    if (!((topo.masks[(int64_t)l * 8 + (off >> 6)] >> (off & 63)) & 1ull)) continue;
                 }
+                // corners with int() truncation (renderer.cu:271-300), x >= 0 here
+                const int i0 = (int)x, j0 = (int)y, k0 = (int)z;
+                const float u = x - (float)i0, v = y - (float)j0, w = z - (float)k0;
+                int64_t slot[8];
+                float sc[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int cx = (kCx >> q) & 1, cy = (kCy >> q) & 1, cz = (kCz >> q) & 1;
+                    const int32_t l = bc.get(topo, i0 + cx, j0 + cy, k0 + cz);
+                    slot[q] = l < 0 ? miss
+                                    : (int64_t)l * 512 + voxel_offset(i0 + cx, j0 + cy, k0 + cz);
+                    const float a = cx ? u : 1.f - u, b = cy ? v : 1.f - v, c = cz ? w : 1.f - w;
+                    sc[q] = (a * b) * c;
+                }
+                // trigetDensity2 order: sum of den*scale, left to right
+                float sig = load_den<F32PAY>(A.payload, slot[0]) * sc[0];
+#pragma unroll
+                for (int q = 1; q < 8; ++q) sig = sig + load_den<F32PAY>(A.payload, slot[q]) * sc[q];
+                ++n_gather;
+                const float e = expf(sig + A.act_shift);
+                const float alpha = 1.f - powf(1.f + e, -A.interval);
+                if (alpha <= A.thres) continue;
+                const float wt = T * alpha;
+                T = T * (1.f - alpha);
+                if (wt <= A.thres) continue;
+                // kept: colour features (trigetColor order) staged as this lane's input row
+                float f[12], cq[12];
+                load_col<F32PAY>(A.payload, slot[0], cq);
+#pragma unroll
+                for (int c = 0; c < 12; ++c) f[c] = cq[c] * sc[0];
+#pragma unroll
+                for (int q = 1; q < 8; ++q) {
+                    load_col<F32PAY>(A.payload, slot[q], cq);
+#pragma unroll
+                    for (int c = 0; c < 12; ++c) f[c] = f[c] + cq[c] * sc[q];
+                }
+                uint4* row = reinterpret_cast<uint4*>(sX + lane * XS);
+                row[0] = make_uint4(pack2(f[0], f[1]), pack2(f[2], f[3]), pack2(f[4], f[5]),
+                                    pack2(f[6], f[7]));
+                row[1] = make_uint4(pack2(f[8], f[9]), pack2(f[10], f[11]), pe[0], pe[1]);
+                row[2] = make_uint4(pe[2], pe[3], pe[4], pe[5]);
+                row[3] = make_uint4(pe[6], pe[7], pe[8], pe[9]);
+                row[4] = make_uint4(pe[10], pe[11], pe[12], pe[13]);
+                wt_keep = wt;
+                have = true;
+                ++n_kept;
+                // first_look_merged stops after a kept sample with T below the threshold
+                if (T < A.t_stop) stop_after = true;
             }
-            cur_pos += ntake;
-            need = __ballot(!active);
+            const unsigned long long hm = __ballot(have);
+            if (hm == 0ull) break;  // every lane is done
+            unsigned tmask = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if ((hm >> (16 * q)) & 0xffffull) tmask |= 1u << q;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            wave_mlp(sW, sB, sX, sL, tmask, lane);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            if (have) {
+                // final_render: weight / (1 + exp(-raw)) (renderer.cu:112-119)
+                const float* lg = sL + lane * 4;
+                acc0 = acc0 + wt_keep / (1.f + expf(-lg[0]));
+                acc1 = acc1 + wt_keep / (1.f + expf(-lg[1]));
+                acc2 = acc2 + wt_keep / (1.f + expf(-lg[2]));
+                have = false;
+                if (stop_after) done = true;
+            }
+            __builtin_amdgcn_wave_barrier();
         }
-        if (__ballot(active) == 0ull) break;  // the image is done for this wave
-        // ---- march: live lanes without a pending sample advance, at most 64 steps ----
-        for (int budget = 0; budget < 64 && active && !have; ++budget) {
-            if (!(t < R.tmax) || iter >= max_iter) {
-                finish();
-                break;
-            }
-            t += R.step;
-            ++iter;
-            const float p0 = R.o[0] + t * R.d[0], p1 = R.o[1] + t * R.d[1], p2 = R.o[2] + t * R.d[2];
-            if (!(p0 >= 0.f && p1 >= 0.f && p2 >= 0.f && p0 <= 1.f && p1 <= 1.f && p2 <= 1.f))
-                continue;
-            const float x = p0 * A.ws[0], y = p1 * A.ws[1], z = p2 * A.ws[2];
-            // isActive(Round(xyz)): rintf, half to even (NanoVDB.h:774-779)
-            {
-                const int vi = (int)rintf(x), vj = (int)rintf(y), vk = (int)rintf(z);
-                const int32_t l = bc.get(topo, vi, vj, vk);
-                if (l < 0) continue;
-                const int off = voxel_offset(vi, vj, vk);
-                if (!((topo.masks[(int64_t)l * 8 + (off >> 6)] >> (off & 63)) & 1ull)) continue;
-            }
-            // corners with int() truncation (renderer.cu:271-300), x >= 0 here
-            const int i0 = (int)x, j0 = (int)y, k0 = (int)z;
-            const float u = x - (float)i0, v = y - (float)j0, w = z - (float)k0;
-            int64_t slot[8];
-            float sc[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const int cx = (kCx >> q) & 1, cy = (kCy >> q) & 1, cz = (kCz >> q) & 1;
-                const int32_t l = bc.get(topo, i0 + cx, j0 + cy, k0 + cz);
-                slot[q] = l < 0 ? miss : (int64_t)l * 512 + voxel_offset(i0 + cx, j0 + cy, k0 + cz);
-                const float a = cx ? u : 1.f - u, b = cy ? v : 1.f - v, c = cz ? w : 1.f - w;
-                sc[q] = (a * b) * c;
-            }
-            // trigetDensity2 order: sum of den*scale, left to right
-            float sig = load_den<F32PAY>(A.payload, slot[0]) * sc[0];
-#pragma unroll
-            for (int q = 1; q < 8; ++q) sig = sig + load_den<F32PAY>(A.payload, slot[q]) * sc[q];
-            ++n_gather;
-            const float e = expf(sig + A.act_shift);
-            const float alpha = 1.f - powf(1.f + e, -A.interval);
-            if (alpha <= A.thres) continue;
-            const float wt = T * alpha;
-            T = T * (1.f - alpha);
-            if (wt <= A.thres) continue;
-            // kept: colour features (trigetColor order) staged as this lane's input row
-            float f[12], cq[12];
-            load_col<F32PAY>(A.payload, slot[0], cq);
-#pragma unroll
-            for (int c = 0; c < 12; ++c) f[c] = cq[c] * sc[0];
-#pragma unroll
-            for (int q = 1; q < 8; ++q) {
-                load_col<F32PAY>(A.payload, slot[q], cq);
-#pragma unroll
-                for (int c = 0; c < 12; ++c) f[c] = f[c] + cq[c] * sc[q];
-            }
-            uint4* row = reinterpret_cast<uint4*>(sX + lane * XS);
-            row[0] = make_uint4(pack2(f[0], f[1]), pack2(f[2], f[3]), pack2(f[4], f[5]),
-                                pack2(f[6], f[7]));
-            row[1] = make_uint4(pack2(f[8], f[9]), pack2(f[10], f[11]), R.pe[0], R.pe[1]);
-            row[2] = make_uint4(R.pe[2], R.pe[3], R.pe[4], R.pe[5]);
-            row[3] = make_uint4(R.pe[6], R.pe[7], R.pe[8], R.pe[9]);
-            row[4] = make_uint4(R.pe[10], R.pe[11], R.pe[12], R.pe[13]);
-            wt_keep = wt;
-            have = true;
-            ++n_kept;
-            // first_look_merged stops after a kept sample with T below the threshold
-            if (T < A.t_stop) stop_after = true;
+        if (valid) {
+            const float last = T * A.bg;
+            float* dst = A.out + ((int64_t)py * A.W + px) * 3;
+            dst[0] = last + acc0;
+            dst[1] = last + acc1;
+            dst[2] = last + acc2;
         }
-        const unsigned long long hm = __ballot(have);
-        const unsigned long long searching = __ballot(active && !have);
-        if (hm == 0ull || (__popcll(hm) < 32 && searching != 0ull)) continue;  // uniform
-        unsigned tmask = 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if ((hm >> (16 * q)) & 0xffffull) tmask |= 1u << q;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        wave_mlp(sW, sB, sX, sL, tmask, lane);
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        if (have) {
-            // final_render: weight / (1 + exp(-raw)) (renderer.cu:112-119)
-            const float* lg = sL + lane * 4;
-            acc0 = acc0 + wt_keep / (1.f + expf(-lg[0]));
-            acc1 = acc1 + wt_keep / (1.f + expf(-lg[1]));
-            acc2 = acc2 + wt_keep / (1.f + expf(-lg[2]));
-            have = false;
-            if (stop_after) finish();
-        }
-        __builtin_amdgcn_wave_barrier();
     }
     if (A.counters) {
         // wave sums, one atomic per wave
