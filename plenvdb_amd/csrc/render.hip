@@ -362,7 +362,29 @@ __global__ void __launch_bounds__(64 * kWaves) k_render(RenderArgs A) {
                 {
                     const int vi = (int)rintf(x), vj = (int)rintf(y), vk = (int)rintf(z);
                     const int32_t l = bc.get(topo, vi, vj, vk);
-                    if (l < 0) continue;
+                    if (l < 0) {
+                        // Empty space: the rounded position is in a missing leaf, so
+                        // every step until the ray leaves that leaf's rounding box
+                        // [8L - 0.5, 8L + 7.5) is skipped by this same test.  Take
+                        // those steps with the accumulation alone (t stays the
+                        // reference's running sum) and stop two steps short of the
+                        // computed exit, which covers its rounding.
+                        const int L[3] = {vi >> 3, vj >> 3, vk >> 3};
+                        float te = INFINITY;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a) {
+                            if (d[a] == 0.f) continue;
+                            const float xb = d[a] > 0.f ? 8.f * (float)L[a] + 7.5f
+                                                        : 8.f * (float)L[a] - 0.5f;
+                            te = fminf(te, (xb / A.ws[a] - o[a]) / d[a]);
+                        }
+                        const float tsafe = te - 2.f * step;
+                        while (t + step < tsafe && t < tmax && iter < max_iter) {
+                            t += step;
+                            ++iter;
+                        }
+                        continue;
+                    }
                     const int off = voxel_offset(vi, vj, vk);
                     if (!((topo.masks[(int64_t)l * 8 + (off >> 6)] >> (off & 63)) & 1ull)) continue;
                 }
