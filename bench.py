@@ -197,9 +197,9 @@ def bench_batched(args, scene):
 # x1.0); profiles/pmc_traffic_c2_train_c5_render.md holds the runs.
 TRAFFIC_SOURCE = "profiles/pmc_traffic_c2_train_c5_render.md"
 MEASURED_TRAFFIC_MB = {
-    "c2": {"mlp_bwd": 78.91, "scatter": 70.15, "mlp_fwd": 56.13, "ray_march": 14.07,
-           "dw_reduce": 53.45, "composite": 11.75},
-    "c5": {"render": 112.07},
+    "c2": {"mlp_bwd": 53.24, "scatter": 64.05, "mlp_fwd": 57.27, "ray_march": 14.07,
+           "dw_reduce": 26.74, "composite": 15.23, "adam": 150.26},
+    "c5": {"render": 112.01},
 }
 
 
