@@ -99,28 +99,61 @@ __global__ void __launch_bounds__(256) k_ray_count(StepParams P, TopoView t, Ray
     if (lane == 0) R.count[r] = count;
 }
 
-// Exclusive scan of the per-ray counts by one workgroup of 1024 threads.
+// Exclusive scan of the per-ray counts by one workgroup of 1024 threads, in
+// chunks of 8192: each thread takes 8 consecutive counts (two 16-byte loads),
+// wavefronts scan their thread sums with shuffles, one LDS pass combines the
+// 16 wavefronts, and a carry runs from chunk to chunk.
 __global__ void __launch_bounds__(1024) k_scan(RayBuffers R, int n) {
-    __shared__ int part[1024];
-    const int tid = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int b = tid * per, e = min(n, b + per);
-    int sum = 0;
-    for (int i = b; i < e; ++i) sum += R.count[i];
-    part[tid] = sum;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {
-        const int v = tid >= d ? part[tid - d] : 0;
+    __shared__ int wsum[16];
+    __shared__ int carry_s;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    int carry = 0;
+    for (int c0 = 0; c0 < n; c0 += 8192) {
+        const int b = c0 + 8 * tid;
+        int v[8];
+        if (b + 8 <= n) {  // 32-byte aligned (the workspace arrays are 256-byte aligned)
+            const int4 a0 = *reinterpret_cast<const int4*>(R.count + b);
+            const int4 a1 = *reinterpret_cast<const int4*>(R.count + b + 4);
+            v[0] = a0.x; v[1] = a0.y; v[2] = a0.z; v[3] = a0.w;
+            v[4] = a1.x; v[5] = a1.y; v[6] = a1.z; v[7] = a1.w;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = b + k < n ? R.count[b + k] : 0;
+        }
+        int sum = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) sum += v[k];
+        int inc = sum;  // inclusive scan of the thread sums within the wavefront
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int o = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += o;
+        }
+        if (lane == 63) wsum[wv] = inc;
         __syncthreads();
-        part[tid] += v;
+        int before = carry;
+        for (int w = 0; w < wv; ++w) before += wsum[w];
+        if (tid == 1023) carry_s = before + inc;
+        int run = before + inc - sum;
+        int off[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            off[k] = run;
+            run += v[k];
+        }
+        if (b + 8 <= n) {
+            *reinterpret_cast<int4*>(R.offset + b) = make_int4(off[0], off[1], off[2], off[3]);
+            *reinterpret_cast<int4*>(R.offset + b + 4) = make_int4(off[4], off[5], off[6], off[7]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (b + k < n) R.offset[b + k] = off[k];
+        }
         __syncthreads();
+        carry = carry_s;
+        __syncthreads();  // wsum and carry_s are reused by the next chunk
     }
-    int run = part[tid] - sum;
-    for (int i = b; i < e; ++i) {
-        R.offset[i] = run;
-        run += R.count[i];
-    }
-    if (tid == 1023) R.total[0] = part[1023];
+    if (tid == 0) R.total[0] = carry;
 }
 
 // Forward march: one record per occupancy-kept sample.

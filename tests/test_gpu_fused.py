@@ -120,3 +120,22 @@ def test_fused_step_applies_adam(oracle_mod):
         tr.step()
     torch.cuda.synchronize()
     assert np.isfinite(float(tr.loss.item()))
+
+
+@pytest.mark.parametrize("n_rays", [8192, 20001])
+def test_fused_record_offsets_are_the_exclusive_scan(n_rays):
+    """Per-ray record offsets are the exclusive prefix sum of the per-ray counts
+    (integer work: exact), across the scan's 8192-ray chunks and a ragged tail."""
+    from plenvdb_amd import scenes
+    from plenvdb_amd.fused import FusedTrainer
+
+    print("[case] record offsets c2", n_rays, flush=True)
+    sc = scenes.make_scene("c2", n_rays=n_rays)
+    tr = FusedTrainer(sc)
+    tr.forward_backward()
+    torch.cuda.synchronize()
+    recs = tr.records()
+    cnt = recs["count"].astype(np.int64)
+    exp = np.concatenate([[0], np.cumsum(cnt)[:-1]])
+    np.testing.assert_array_equal(recs["offset"].astype(np.int64), exp)
+    assert int(cnt.sum()) == int(tr.total_records())
