@@ -25,6 +25,8 @@
 // rule (densityvdb.cu:16-27, colorvdb.cu:28-37).  Both kernels load a group's
 // records into registers while the previous group computes.
 
+#include <type_traits>
+
 #include "fused_common.h"
 #include "gather.h"
 #include "plenvdb_hip.h"
@@ -139,20 +141,26 @@ __device__ __forceinline__ f32x4 mfma_w(const bf16x8& a, const bf16x8& bh, const
 // ---- forward -----------------------------------------------------------------
 // A workgroup of 4 wavefronts takes a group of 32 records; wave w owns hidden
 // features [32w, 32w + 32) of both hidden layers (up to three workgroups per
-// CU).  Phases (a barrier after each): 1 commit the prefetched records, 2 gather
-// the colour features and form X, 3 H0, 4 H1, 5 logits -> rgb.  The arithmetic
-// of X, H0 and H1 is the backward kernel's, so its recomputed ReLU masks are
-// the ones this pass applied.
+// CU).  Per group, four phases with a barrier after each:
+//   A  X = [colour features | PE | 1] from the gather and view directions
+//      loaded during the previous group; the features are kept per record
+//   B  H0, then the next group's topology lookups and view-direction loads
+//   C  H1, then the next group's colour loads
+//   D  logits -> rgb
+// The records are staged two groups ahead (flags and positions double-buffered
+// in LDS); W0^T and W1^T slices stay in registers, W2^T in LDS.  The arithmetic of X, H0 and H1 is the backward kernel's, so its
+// recomputed ReLU masks are the ones this pass applied.
 struct RecInfo {
     float x, y, z;
     uint32_t bits;
 };
 
 struct alignas(16) SmemFwd {
+    uint16_t W2t[16 * 128];  // W2^T [16 logits (3 used)][128 hidden]
     uint16_t Xh[GS * XM], Xl[GS * XM];
     uint16_t A1h[GS * SM], A1l[GS * SM];  // H0
     uint16_t A2h[GS * SM], A2l[GS * SM];  // H1
-    RecInfo ri[GS];
+    RecInfo ri[2][GS];
 };
 
 __device__ __forceinline__ float sigmoidf_(float x) { return 1.f / (1.f + expf(-x)); }
@@ -161,18 +169,23 @@ template <bool BF16COL>
 __global__ void __launch_bounds__(256) k_mlp_fwd(StepParams P, TopoView topo, const void* color,
                                                  MlpPacked W, RayBuffers R, Records rec) {
     constexpr int MT = 2;  // 16-feature tiles per wave
+    using Raw = std::conditional_t<BF16COL, uint2, float4>;  // one third of a colour row
     __shared__ SmemFwd sm;
     const int tid = threadIdx.x;
     const int wv = tid >> 6, lane = tid & 63, r16 = lane & 15, g = lane >> 4;
     const int f0 = 32 * wv;
     const int pr = lane < GS / 4 ? wv * (GS / 4) + lane : -1;  // staging lanes
+    const int gs = tid >> 3, gpart = tid & 7;                  // gather: record, corner
+    const int ps = tid & (GS - 1), pp = tid >> 5;              // encoding: sample, part
 
-    // W1^T slice in registers; W0^T and W2^T (one phase each) come from L2 per group
-    bf16x8 A1[MT][4];
+    // this wave's W0^T and W1^T slices in registers; W2^T in LDS
+    bf16x8 A0[MT][2], A1[MT][4];
     float b0v[MT][4], b1v[MT][4], b2v[3];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
         const int row = f0 + 16 * mt + r16;
+#pragma unroll
+        for (int kt = 0; kt < 2; ++kt) A0[mt][kt] = ld_frag(W.w0t + row * 64 + 32 * kt + 8 * g);
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) A1[mt][kt] = ld_frag(W.w1t + row * 128 + 32 * kt + 8 * g);
 #pragma unroll
@@ -184,6 +197,8 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(StepParams P, TopoView topo, co
     b2v[0] = W.b2[0];
     b2v[1] = W.b2[1];
     b2v[2] = W.b2[2];
+    for (int e = tid; e < 16 * 128 / 8; e += 256)
+        reinterpret_cast<uint4*>(sm.W2t)[e] = reinterpret_cast<const uint4*>(W.w2t)[e];
     // X columns 40..63 are zero throughout
     for (int e = tid; e < GS * 24; e += 256) {
         const int row = e / 24, col = 40 + e % 24;
@@ -194,7 +209,8 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(StepParams P, TopoView topo, co
     const int64_t total = min((int64_t)R.total[0], P.capacity);
     const int64_t ngroups = (total + GS - 1) / GS;
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
-    float4 p_xyze = z4, p_atwr = z4;  // register prefetch of a group's records
+    // staging registers: a group's records (two groups ahead of the one computed)
+    float4 p_xyze = z4, p_atwr = z4;
     auto stage = [&](int64_t grp) {
         const int64_t q = grp * GS + pr;
         p_xyze = p_atwr = z4;
@@ -203,64 +219,95 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(StepParams P, TopoView topo, co
             p_atwr = rec.atwr[q];
         }
     };
-    stage(blockIdx.x);
-
-    for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {  // uniform bound
-        const int64_t base = grp * GS;
-        // ---- 1. commit the prefetched group; start loading the next one ----
+    auto commit = [&](int b) {
         if (pr >= 0) {
             RecInfo ri;
             ri.x = p_xyze.x;
             ri.y = p_xyze.y;
             ri.z = p_xyze.z;
             ri.bits = __float_as_uint(p_atwr.w);  // zero for records past the end
-            sm.ri[pr] = ri;
+            sm.ri[b][pr] = ri;
         }
-        stage(grp + gridDim.x);
-        bf16x8 A0[MT][2];
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-            for (int kt = 0; kt < 2; ++kt)
-                A0[mt][kt] = ld_frag(W.w0t + (f0 + 16 * mt + r16) * 64 + 32 * kt + 8 * g);
-        __syncthreads();
-        // ---- 2. X = [colour features (12) | PE (27) | 1 | 0...] ----
+    };
+    // gather prefetch of the next group: this thread's corner of record gs
+    int64_t c_slot = -1;
+    float c_sc = 0.f;
+    Raw c_raw[3];
+    float vdp[3] = {0.f, 0.f, 0.f};  // view direction of sample ps
+    auto lookup = [&](int b) {  // topology and view direction (after the records' barrier)
+        const RecInfo ri = sm.ri[b][gs];
+        c_slot = -1;
+        c_sc = 0.f;
+        if (ri.bits & kAliveW) {
+            const Cell cell = make_cell(ri.x, ri.y, ri.z);
+            const int dx = (kCx >> gpart) & 1, dy = (kCy >> gpart) & 1, dz = (kCz >> gpart) & 1;
+            const float a = dx ? cell.u : 1.f - cell.u;
+            const float bb = dy ? cell.v : 1.f - cell.v;
+            const float d = dz ? cell.w : 1.f - cell.w;
+            c_sc = (a * bb) * d;
+            c_slot = slot_of(topo, cell.i0 + dx, cell.j0 + dy, cell.k0 + dz);
+        }
+        const uint32_t pb = sm.ri[b][ps].bits;
+        vdp[0] = vdp[1] = vdp[2] = 0.f;
+        if (pb & kAliveW) {
+            const uint32_t ray = pb & kRayMask;
+            vdp[0] = P.viewdirs[3 * ray];
+            vdp[1] = P.viewdirs[3 * ray + 1];
+            vdp[2] = P.viewdirs[3 * ray + 2];
+        }
+    };
+    auto fetch = [&]() {  // the colour row of the looked-up corner
+        if constexpr (BF16COL) {
+            c_raw[0] = c_raw[1] = c_raw[2] = make_uint2(0u, 0u);
+            if (c_slot >= 0) {
+                const uint2* src = reinterpret_cast<const uint2*>(
+                    reinterpret_cast<const uint16_t*>(color) + c_slot * 12);
+                c_raw[0] = src[0];
+                c_raw[1] = src[1];
+                c_raw[2] = src[2];
+            }
+        } else {
+            c_raw[0] = c_raw[1] = c_raw[2] = z4;
+            if (c_slot >= 0) {
+                const float4* src = reinterpret_cast<const float4*>(
+                    reinterpret_cast<const float*>(color) + c_slot * 12);
+                c_raw[0] = src[0];
+                c_raw[1] = src[1];
+                c_raw[2] = src[2];
+            }
+        }
+    };
+
+    stage(blockIdx.x);
+    commit(0);
+    stage((int64_t)blockIdx.x + gridDim.x);
+    __syncthreads();
+    lookup(0);
+    fetch();
+    int cb = 0;  // buffer of the current group's records
+
+    for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {  // uniform bound
+        const int64_t base = grp * GS;
+        // ---- A. X = [colour features (12) | PE (27) | 1 | 0...]; stage the records ----
         {
-            // colour gather: 8 threads per record, thread `part` takes corner `part`
-            const int s = tid >> 3, part = tid & 7;
-            const RecInfo ri = sm.ri[s];
-            const bool alive = (ri.bits & kAliveW) != 0u;
+            // trilinear colour: corner gpart's row times its weight, summed over
+            // the record's 8 threads (zero where the record or corner is absent)
             float v[12];
+            if constexpr (BF16COL) {
 #pragma unroll
-            for (int c = 0; c < 12; ++c) v[c] = 0.f;
-            if (alive) {
-                const Cell cell = make_cell(ri.x, ri.y, ri.z);
-                const int dx = (kCx >> part) & 1, dy = (kCy >> part) & 1, dz = (kCz >> part) & 1;
-                const float a = dx ? cell.u : 1.f - cell.u;
-                const float b = dy ? cell.v : 1.f - cell.v;
-                const float d = dz ? cell.w : 1.f - cell.w;
-                const float sc = (a * b) * d;
-                const int64_t slot = slot_of(topo, cell.i0 + dx, cell.j0 + dy, cell.k0 + dz);
-                if (slot >= 0) {
-                    if (BF16COL) {
-                        const uint16_t* src = reinterpret_cast<const uint16_t*>(color) + slot * 12;
-                        const uint2 p0 = *reinterpret_cast<const uint2*>(src);
-                        const uint2 p1 = *reinterpret_cast<const uint2*>(src + 4);
-                        const uint2 p2 = *reinterpret_cast<const uint2*>(src + 8);
-                        const uint32_t w6[6] = {p0.x, p0.y, p1.x, p1.y, p2.x, p2.y};
+                for (int k = 0; k < 3; ++k) {
+                    v[4 * k + 0] = bf_lo(c_raw[k].x) * c_sc;
+                    v[4 * k + 1] = bf_hi(c_raw[k].x) * c_sc;
+                    v[4 * k + 2] = bf_lo(c_raw[k].y) * c_sc;
+                    v[4 * k + 3] = bf_hi(c_raw[k].y) * c_sc;
+                }
+            } else {
 #pragma unroll
-                        for (int c = 0; c < 6; ++c) {
-                            v[2 * c] = bf_bits2f((uint16_t)(w6[c] & 0xffffu)) * sc;
-                            v[2 * c + 1] = bf_bits2f((uint16_t)(w6[c] >> 16)) * sc;
-                        }
-                    } else {
-                        const float4* src = reinterpret_cast<const float4*>(
-                            reinterpret_cast<const float*>(color) + slot * 12);
-                        const float4 p0 = src[0], p1 = src[1], p2 = src[2];
-                        v[0] = p0.x * sc; v[1] = p0.y * sc; v[2] = p0.z * sc; v[3] = p0.w * sc;
-                        v[4] = p1.x * sc; v[5] = p1.y * sc; v[6] = p1.z * sc; v[7] = p1.w * sc;
-                        v[8] = p2.x * sc; v[9] = p2.y * sc; v[10] = p2.z * sc; v[11] = p2.w * sc;
-                    }
+                for (int k = 0; k < 3; ++k) {
+                    v[4 * k + 0] = c_raw[k].x * c_sc;
+                    v[4 * k + 1] = c_raw[k].y * c_sc;
+                    v[4 * k + 2] = c_raw[k].z * c_sc;
+                    v[4 * k + 3] = c_raw[k].w * c_sc;
                 }
             }
 #pragma unroll
@@ -269,20 +316,20 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(StepParams P, TopoView topo, co
                 v[c] += __shfl_xor(v[c], 2, 64);
                 v[c] += __shfl_xor(v[c], 4, 64);
             }
-            if (part == 0) {
+            if (gpart == 0) {
                 uint32_t wh[6], wl[6];
 #pragma unroll
                 for (int c = 0; c < 6; ++c) split2(v[2 * c], v[2 * c + 1], wh[c], wl[c]);
-                uint32_t* xh = reinterpret_cast<uint32_t*>(&sm.Xh[s * XM]);
-                uint32_t* xl = reinterpret_cast<uint32_t*>(&sm.Xl[s * XM]);
+                uint32_t* xh = reinterpret_cast<uint32_t*>(&sm.Xh[gs * XM]);
+                uint32_t* xl = reinterpret_cast<uint32_t*>(&sm.Xl[gs * XM]);
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
                     xh[c] = wh[c];
                     xl[c] = wl[c];
                 }
                 // the features kept for the backward: hi halves 0..11, then lo
-                const int64_t qq = base + s;
-                if (qq < total && alive) {
+                const int64_t qq = base + gs;
+                if (qq < total && (sm.ri[cb][gs].bits & kAliveW)) {
                     rec.feat[3 * qq] = make_uint4(wh[0], wh[1], wh[2], wh[3]);
                     rec.feat[3 * qq + 1] = make_uint4(wh[4], wh[5], wl[0], wl[1]);
                     rec.feat[3 * qq + 2] = make_uint4(wl[2], wl[3], wl[4], wl[5]);
@@ -294,16 +341,7 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(StepParams P, TopoView topo, co
             // (sample ps, part pp < 7) writes X columns 12 + 4pp .. +3; hardware
             // sin/cos (error about 1e-6 for |arg| <= 8, below what the (hi, lo) pair
             // keeps), as in the backward
-            const int ps = tid & (GS - 1), pp = tid >> 5;
-            const uint32_t bits = sm.ri[ps].bits;
-            const bool alive = (bits & kAliveW) != 0u;
-            float vd[3] = {0.f, 0.f, 0.f};
-            if (alive) {
-                const uint32_t ray = bits & kRayMask;
-                vd[0] = P.viewdirs[3 * ray];
-                vd[1] = P.viewdirs[3 * ray + 1];
-                vd[2] = P.viewdirs[3 * ray + 2];
-            }
+            const bool alive = (sm.ri[cb][ps].bits & kAliveW) != 0u;
             float val[4];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
@@ -311,10 +349,10 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(StepParams P, TopoView topo, co
                 float v = 0.f;
                 if (alive) {
                     if (p < 3) {
-                        v = vd[p];
+                        v = vdp[p];
                     } else if (p < 27) {
                         const int idx = p - 3, axis = (idx % 12) / 4, fq = idx % 4;
-                        const float arg = vd[axis] * (float)(1 << fq);
+                        const float arg = vdp[axis] * (float)(1 << fq);
                         v = idx < 12 ? __sinf(arg) : __cosf(arg);
                     } else {
                         v = 1.f;
@@ -330,8 +368,10 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(StepParams P, TopoView topo, co
                 *reinterpret_cast<uint2*>(&sm.Xl[ps * XM + 12 + 4 * pp]) = lo;
             }
         }
+        commit(cb ^ 1);                 // the next group's records
+        stage(grp + 2 * (int64_t)gridDim.x);  // and the one after's loads
         __syncthreads();
-        // ---- 3. H0^T slice = relu(W0^T X^T + b0) -> A1 ----
+        // ---- B. H0^T slice = relu(W0^T X^T + b0) -> A1; next group's lookups ----
         // k-step outer: each B fragment is read once and the (mt, nt) chains interleave
         f32x4 acc[MT][NT];
 #pragma unroll
@@ -362,8 +402,9 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(StepParams P, TopoView topo, co
                 for (int i = 0; i < 4; ++i) h[i] = fmaxf(acc[mt][nt][i] + b0v[mt][i], 0.f);
                 st_act<false>(sm.A1h, sm.A1l, nullptr, 16 * nt + r16, f0 + 16 * mt + 4 * g, lane, h);
             }
+        lookup(cb ^ 1);
         __syncthreads();
-        // ---- 4. H1^T slice = relu(W1^T H0^T + b1) -> A2 ----
+        // ---- C. H1^T slice = relu(W1^T H0^T + b1) -> A2; next group's colour loads ----
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -392,21 +433,22 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(StepParams P, TopoView topo, co
                 for (int i = 0; i < 4; ++i) h[i] = fmaxf(acc[mt][nt][i] + b1v[mt][i], 0.f);
                 st_act<false>(sm.A2h, sm.A2l, nullptr, 16 * nt + r16, f0 + 16 * mt + 4 * g, lane, h);
             }
+        fetch();
         __syncthreads();
-        // ---- 5. logits (wave nt takes tile nt): L^T = W2^T H1^T + b2, rgb = sigmoid ----
+        // ---- D. logits (wave nt takes tile nt): L^T = W2^T H1^T + b2, rgb = sigmoid ----
         if (wv < NT) {
             const int nt = wv;
             f32x4 a5 = {0.f, 0.f, 0.f, 0.f};
             const int ho = (16 * nt + r16) * SM + 8 * g;
 #pragma unroll
             for (int kt = 0; kt < 4; ++kt)
-                a5 = mfma_w(ld_frag(W.w2t + r16 * 128 + 32 * kt + 8 * g),
+                a5 = mfma_w(ld_frag(&sm.W2t[r16 * 128 + 32 * kt + 8 * g]),
                             ld_frag(&sm.A2h[ho + 32 * kt]), ld_frag(&sm.A2l[ho + 32 * kt]), a5);
             if (g == 0) {
                 const int sidx = 16 * nt + r16;
                 const int64_t q = base + sidx;
                 if (q < total) {
-                    const bool alive = (sm.ri[sidx].bits & kAliveW) != 0u;
+                    const bool alive = (sm.ri[cb][sidx].bits & kAliveW) != 0u;
                     rec.rgb[q] = alive ? make_float4(sigmoidf_(a5[0] + b2v[0]),
                                                      sigmoidf_(a5[1] + b2v[1]),
                                                      sigmoidf_(a5[2] + b2v[2]), 0.f)
@@ -415,6 +457,7 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(StepParams P, TopoView topo, co
             }
         }
         __syncthreads();
+        cb ^= 1;
     }
 }
 
