@@ -41,6 +41,9 @@ def parse():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--path", default="auto", choices=["auto", "batched", "plain"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--configs", default="c3,c4",
+                    help="other workloads whose fused step is timed beside the headline one "
+                         "(comma list, empty for none)")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip timing the plain FP32 path beside the batched one")
     ap.add_argument("--cpu-rays", type=int, default=8192)
@@ -277,6 +280,42 @@ def batched_roofline(r, steps):
     return roof, step, stage_fracs
 
 
+def bench_config(name: str, steps: int, warmup: int):
+    """The fused step (hipGraph replay) on another workload of the set: its
+    default rays, timed with HIP events over `steps` replays."""
+    import torch
+
+    from plenvdb_amd import fused, scenes
+
+    sc = scenes.make_scene(name)
+    tr = fused.FusedTrainer(sc)
+    tr.capture_graph()
+    for _ in range(warmup):
+        tr.step()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(steps):
+        tr.step()
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / steps
+    samples = tr.total_records()
+    cfg = sc["cfg"]
+    out = {"workload": f"{cfg.name}: {cfg.reso}^3 grid, {cfg.n_spheres} spheres"
+                       f"{' + shell' if cfg.shell else ''}, {cfg.n_rays} rays"
+                       f"{' (far in [%g, %g])' % (cfg.far_lo, cfg.far_hi) if cfg.far_lo else ''}, "
+                       f"{'bf16' if tr.color_bf16 else 'fp32'} colour, fwd+bwd+Adam",
+           "ms_per_step": round(ms, 4), "samples_per_step": int(samples),
+           "samples_per_s": round(samples / (ms * 1e-3), 1)}
+    if samples > tr.capacity:
+        out["note"] = f"records capped at capacity {tr.capacity}"
+    del tr
+    torch.cuda.empty_cache()
+    return out
+
+
 def roofline_from(totals, launches_bytes):
     """Pick the kernel with the largest total device time among those with a byte model."""
     best = None
@@ -469,6 +508,9 @@ def main():
                                                  / res["plain"]["samples_per_step"], 3)
     if args.render_frames > 0:
         line["render"] = bench_render(args.render_frames, 3)
+    others = [c for c in args.configs.split(",") if c and c != args.config]
+    if others:
+        line["configs"] = {c: bench_config(c, 5, 2) for c in others}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.config, args.cpu_rays)
     else:

@@ -46,7 +46,7 @@ def oracle_for(oracle_mod, tr, sc, held_weights=True):
 
 
 @pytest.mark.parametrize("name,n_rays,bf16", [("c2", 512, False), ("c3", 512, False),
-                                               ("c2", 384, True)])
+                                               ("c2", 384, True), ("c4", 256, True)])
 def test_fused_forward_backward_vs_fp64_oracle(oracle_mod, name, n_rays, bf16):
     from plenvdb_amd import scenes
     from plenvdb_amd.fused import FusedTrainer, unpack_mlp
