@@ -151,6 +151,14 @@ def bench_batched(args, scene):
     torch.cuda.synchronize()
     stages = fused.stage_timing(0)
     ms_eager = t0.elapsed_time(t1) / args.steps
+    # forward + backward alone (no optimizer step), eager launches
+    t0.record()
+    for _ in range(args.steps):
+        tr.forward_backward()
+    t1.record()
+    torch.cuda.synchronize()
+    ms_fwd_bwd = t0.elapsed_time(t1) / args.steps
+    tr.zero_grad()
     # the same steps replayed from one captured hipGraph (the production path)
     tr.capture_graph()
     for _ in range(args.warmup):
@@ -181,6 +189,7 @@ def bench_batched(args, scene):
     return {
         "ms_per_step": ms,
         "ms_per_step_eager": ms_eager,
+        "ms_fwd_bwd_eager": ms_fwd_bwd,
         "samples_per_step": float(samples),
         "stages": stages,
         "n_color": int(alive_w.sum()),
@@ -397,10 +406,10 @@ def bench_render(frames: int, warmup: int):
     }
 
 
-def cpu_baseline(cfg_name: str, n_rays: int):
+def cpu_baseline(cfg_name: str, n_rays: int, threads: int | None = None):
     """Oracle FP32 restatement of the reference step (C kernels under OpenMP,
     MLP/composite/loss in torch CPU), forward + backward + Adam, timed on the
-    host cores on n_rays rays of the same workload."""
+    host cores (all of them, or `threads`) on n_rays rays of the same workload."""
     import torch
 
     from oracle import oracle
@@ -414,8 +423,8 @@ def cpu_baseline(cfg_name: str, n_rays: int):
     act = vm.reshape(-1).astype(np.uint8)
     m_d, v_d = np.zeros_like(den), np.zeros_like(den)
     m_c, v_c = np.zeros_like(col), np.zeros_like(col)
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    torch.set_num_threads(cores)
+    cores = threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = oracle.set_threads(cores)
     times, samples = [], []
     for it in range(6):
         t = time.perf_counter()
@@ -499,6 +508,7 @@ def main():
     }
     if "ms_per_step_eager" in main_res:
         line["ms_per_step_eager"] = round(main_res["ms_per_step_eager"], 4)
+        line["ms_fwd_bwd_eager"] = round(main_res["ms_fwd_bwd_eager"], 4)
         line["step_launch"] = "hipGraph replay (eager launch timed beside it)"
     if path == "batched":
         line["adam_voxels_per_s"] = round(main_res["v_u"] / (main_res["ms_per_step"] * 1e-3), 1)
@@ -513,6 +523,8 @@ def main():
         line["configs"] = {c: bench_config(c, 5, 2) for c in others}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.config, args.cpu_rays)
+        # and one host thread, on an eighth of the rays
+        line["cpu_baseline_1thread"] = cpu_baseline(args.config, max(args.cpu_rays // 8, 64), 1)
     else:
         line["cpu_baseline"] = None
     print(json.dumps(line), flush=True)

@@ -55,6 +55,18 @@ def load() -> C.CDLL:
     return _lib
 
 
+def set_threads(n: int) -> int:
+    """Threads of the oracle's OpenMP loops (and of torch's CPU ops) from here
+    on; returns the OpenMP maximum in effect."""
+    import torch
+
+    torch.set_num_threads(int(n))
+    fn = load().pvdb_oracle_set_threads
+    fn.argtypes = [C.c_int]
+    fn.restype = C.c_int
+    return int(fn(int(n)))
+
+
 def _p(a):
     return C.c_void_p(None) if a is None else C.c_void_p(a.ctypes.data)
 

@@ -29,3 +29,17 @@
 #include "pvdb_oracle_render.h"
 
 int pvdb_oracle_abi_version(void) { return 1; }
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+/* Threads of the OpenMP loops from here on (the CPU baseline's 1-thread run). */
+int pvdb_oracle_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
