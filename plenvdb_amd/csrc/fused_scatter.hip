@@ -74,9 +74,8 @@ __global__ void __launch_bounds__(256) k_scatter_direct(TopoView topo, RayBuffer
 // list (one exchange); the flush gives each listed entry 16 lanes, which walk
 // the list and sum channel c in lane c.  Two LDS atomics per pair instead of
 // fourteen: LDS float atomics run at well under one lane per clock.  Every
-// record load is issued before any is used (one memory latency per chunk).
-// One atomic instruction of the flush covers four slots' contiguous 12-float
-// colour rows.
+// record load is issued before any is used, and a chunk's records are loaded
+// during the previous chunk's flush.
 constexpr int kAggRecords = 64;
 constexpr int kAggPairsAll = 8 * kAggRecords;  // (record, corner) pairs of a chunk
 constexpr int kAggSlots = kAggPairsAll;        // every pair fits: probing ends
@@ -105,22 +104,27 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
     __syncthreads();
     const int64_t total = min((int64_t)R.total[0], capacity);
     const int64_t nchunk = (total + kAggRecords - 1) / kAggRecords;
-    int b = 0;
-    for (int64_t chunk = blockIdx.x; chunk < nchunk; chunk += gridDim.x, b ^= 1) {
-        // ---- this thread's pairs: records loaded first, then the topology ----
-        uint32_t bits[kAggPairs];
-        float4 pt[kAggPairs], gr[kAggPairs][4];
+    // this thread's pairs of a chunk: the records are loaded one chunk ahead
+    // (issued before the previous chunk's flush)
+    uint32_t bits[kAggPairs];
+    float4 pt[kAggPairs], gr[kAggPairs][4];
+    auto load = [&](int64_t chunk) {
 #pragma unroll
         for (int it = 0; it < kAggPairs; ++it) {
             const int64_t q = chunk * kAggRecords + ((tid + 256 * it) >> 3);
             bits[it] = 0u;
-            if (q < total) {
+            if (chunk < nchunk && q < total) {
                 bits[it] = __float_as_uint(rec.atwr[q].w);
                 pt[it] = rec.xyze[q];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) gr[it][k] = rec.grad[4 * q + k];
             }
         }
+    };
+    load(blockIdx.x);
+    int b = 0;
+    for (int64_t chunk = blockIdx.x; chunk < nchunk; chunk += gridDim.x, b ^= 1) {
+        // ---- topology of this thread's pairs ----
         int32_t leaf[kAggPairs];
         uint32_t off[kAggPairs];
         float wgt[kAggPairs][3];
@@ -174,8 +178,11 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
             next[pair] = (uint16_t)atomicExch(&head[h], (uint32_t)pair);
         }
         __syncthreads();
+        load(chunk + gridDim.x);  // the next chunk's records, in flight during the flush
         // ---- flush: walk each listed entry, add its sums, clear it; reset the
-        // other list counter ----
+        // other list counter.  16 lanes per entry, so one atomic instruction covers
+        // four slots' contiguous 12-float colour rows (fewer cache lines than
+        // narrower groups, measured) ----
         const int n = nlisted[b];
         if (tid == 0) nlisted[b ^ 1] = 0;
         const int c = tid & 15;
