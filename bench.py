@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--config", default="c2")
     ap.add_argument("--path", default="auto", choices=["auto", "batched", "plain"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="built-in launch shapes instead of the autotuner's choice")
     ap.add_argument("--configs", default="c3,c4",
                     help="other workloads whose fused step is timed beside the headline one "
                          "(comma list, empty for none)")
@@ -138,6 +140,7 @@ def bench_batched(args, scene):
     from plenvdb_amd import fused
 
     tr = fused.FusedTrainer(scene)
+    tuned = tr.autotune() if not args.no_tune else {"launch": None, "source": "built-in defaults"}
     for _ in range(args.warmup):
         tr.step()
     torch.cuda.synchronize()
@@ -190,6 +193,7 @@ def bench_batched(args, scene):
         "ms_per_step": ms,
         "ms_per_step_eager": ms_eager,
         "ms_fwd_bwd_eager": ms_fwd_bwd,
+        "launch": tuned,
         "samples_per_step": float(samples),
         "stages": stages,
         "n_color": int(alive_w.sum()),
@@ -298,6 +302,7 @@ def bench_config(name: str, steps: int, warmup: int):
 
     sc = scenes.make_scene(name)
     tr = fused.FusedTrainer(sc)
+    tuned = tr.autotune()
     tr.capture_graph()
     for _ in range(warmup):
         tr.step()
@@ -317,7 +322,7 @@ def bench_config(name: str, steps: int, warmup: int):
                        f"{' (far in [%g, %g])' % (cfg.far_lo, cfg.far_hi) if cfg.far_lo else ''}, "
                        f"{'bf16' if tr.color_bf16 else 'fp32'} colour, fwd+bwd+Adam",
            "ms_per_step": round(ms, 4), "samples_per_step": int(samples),
-           "samples_per_s": round(samples / (ms * 1e-3), 1)}
+           "samples_per_s": round(samples / (ms * 1e-3), 1), "launch": tuned["launch"]}
     if samples > tr.capacity:
         out["note"] = f"records capped at capacity {tr.capacity}"
     del tr
@@ -509,6 +514,7 @@ def main():
     if "ms_per_step_eager" in main_res:
         line["ms_per_step_eager"] = round(main_res["ms_per_step_eager"], 4)
         line["ms_fwd_bwd_eager"] = round(main_res["ms_fwd_bwd_eager"], 4)
+        line["launch"] = main_res["launch"]
         line["step_launch"] = "hipGraph replay (eager launch timed beside it)"
     if path == "batched":
         line["adam_voxels_per_s"] = round(main_res["v_u"] / (main_res["ms_per_step"] * 1e-3), 1)

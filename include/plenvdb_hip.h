@@ -202,8 +202,15 @@ typedef struct pvdb_step_desc {
      * device instead of the three fields above, so one captured graph of the
      * step can be replayed with each step's values */
     const float* adam_scalars;
+    /* launch shapes, 0 = built-in default (results do not depend on them):
+     * [0] forward-MLP workgroups, [1] scatter workgroups, [2] scatter records
+     * per chunk (32 or 64), [3] unused; pvdb_fused_launch_defaults() fills the
+     * defaults for this device */
+    int32_t launch[4];
 } pvdb_step_desc;
 
+/* The default launch shapes (pvdb_step_desc.launch) on the current device. */
+int pvdb_fused_launch_defaults(int32_t* launch4);
 /* Workspace bytes for n_rays rays, `capacity` records and nleaf leaves (-1 on bad sizes). */
 int64_t pvdb_fused_workspace_bytes(int64_t n_rays, int64_t capacity, int64_t nleaf);
 /* Clears the workspace (touched-leaf flags, buffers); call once after allocating it. */

@@ -76,19 +76,20 @@ __global__ void __launch_bounds__(256) k_scatter_direct(TopoView topo, RayBuffer
 // fourteen: LDS float atomics run at well under one lane per clock.  Every
 // record load is issued before any is used, and a chunk's records are loaded
 // during the previous chunk's flush.
-constexpr int kAggRecords = 64;
-constexpr int kAggPairsAll = 8 * kAggRecords;  // (record, corner) pairs of a chunk
-constexpr int kAggSlots = kAggPairsAll;        // every pair fits: probing ends
-constexpr int kAggLog2 = 9;                    // kAggSlots == 1 << kAggLog2
-constexpr int kAggPairs = kAggPairsAll / 256;  // pairs per thread
+// kAggRecords (32 or 64, a launch choice) records per chunk.
 constexpr uint32_t kEmptyKey = 0xffffffffu;
-constexpr uint32_t kEnd = 0xffffu;             // end of a pair list
-static_assert(kAggSlots == 1 << kAggLog2, "hash size");
+constexpr uint32_t kEnd = 0xffffu;  // end of a pair list
+template <int kAggRecords>
 __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R, Records rec,
                                                      int64_t capacity,
                                                      float* __restrict__ grad_density,
                                                      float* __restrict__ grad_color,
                                                      uint8_t* __restrict__ leaf_touched) {
+    constexpr int kAggPairsAll = 8 * kAggRecords;  // (record, corner) pairs of a chunk
+    constexpr int kAggSlots = kAggPairsAll;        // every pair fits: probing ends
+    constexpr int kAggLog2 = kAggRecords == 64 ? 9 : 8;  // kAggSlots == 1 << kAggLog2
+    constexpr int kAggPairs = kAggPairsAll / 256;  // pairs per thread
+    static_assert(kAggSlots == 1 << kAggLog2 && kAggPairs >= 1, "chunk size");
     __shared__ uint32_t key[kAggSlots];
     __shared__ uint32_t head[kAggSlots];          // first pair of the entry's list
     __shared__ uint16_t next[kAggPairsAll];       // next pair of the same entry
@@ -218,14 +219,23 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
 namespace pvdb {
 
 int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, int64_t capacity,
-                   float* grad_density, float* grad_color, uint8_t* leaf_touched, hipStream_t s) {
+                   float* grad_density, float* grad_color, uint8_t* leaf_touched, int grid_req,
+                   int chunk_records, hipStream_t s) {
     if (t.nleaf == 0) return PVDB_OK;
     if (t.nleaf < (int64_t(1) << 23) - 1) {  // every slot < 2^32 - 1, the empty key
-        // four 32 KB workgroups fit a CU; each takes chunks in a grid-stride loop
-        const int64_t chunks = (capacity + kAggRecords - 1) / kAggRecords;
-        const unsigned grid = (unsigned)(chunks < 1024 ? (chunks < 1 ? 1 : chunks) : 1024);
-        hipLaunchKernelGGL(k_scatter_agg, dim3(grid), dim3(256), 0, s, t, R, rec, capacity,
-                           grad_density, grad_color, leaf_touched);
+        PVDB_CHECK_ARG(chunk_records == 32 || chunk_records == 64,
+                       "launch[2]: scatter chunk must be 32 or 64 records");
+        const int64_t chunks = (capacity + chunk_records - 1) / chunk_records;
+        // each workgroup takes chunks in a grid-stride loop; four (64-record) or
+        // eight (32-record) workgroups fit a CU by LDS
+        const int64_t want = grid_req > 0 ? grid_req : 1;
+        const unsigned grid = (unsigned)(chunks < want ? (chunks < 1 ? 1 : chunks) : want);
+        if (chunk_records == 64)
+            hipLaunchKernelGGL(k_scatter_agg<64>, dim3(grid), dim3(256), 0, s, t, R, rec, capacity,
+                               grad_density, grad_color, leaf_touched);
+        else
+            hipLaunchKernelGGL(k_scatter_agg<32>, dim3(grid), dim3(256), 0, s, t, R, rec, capacity,
+                               grad_density, grad_color, leaf_touched);
         PVDB_LAUNCHED("k_scatter_agg");
         return PVDB_OK;
     }

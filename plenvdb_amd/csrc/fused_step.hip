@@ -25,7 +25,8 @@ int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const Topo
                float* dw_partial, hipStream_t s);
 int launch_dw_reduce(const float* part, int nparts, float* dw, hipStream_t s);
 int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, int64_t capacity,
-                   float* grad_density, float* grad_color, uint8_t* leaf_touched, hipStream_t s);
+                   float* grad_density, float* grad_color, uint8_t* leaf_touched, int grid_req,
+                   int chunk_records, hipStream_t s);
 int launch_pack_mlp(const float* prm, uint16_t* w0t, uint16_t* w1t, uint16_t* w2t, uint16_t* w2,
                     uint16_t* w1, uint16_t* w0, hipStream_t s);
 }  // namespace pvdb
@@ -34,7 +35,10 @@ using namespace pvdb;
 
 namespace {
 
-constexpr int kMaxMlpGrid = 512;
+constexpr int kMaxMlpGrid = 512;   // workgroups of the MLP kernels at most (dW partials)
+constexpr int kFwdGrid = 512;      // default forward-MLP workgroups
+constexpr int kScatterChunk = 64;  // default scatter records per chunk
+constexpr int kScatterGrid = 1024; // default scatter workgroups
 
 struct Layout {
     size_t count, offset, total, t_last, ray_out, ray_grad, ray_ctot, ray_loss, loss;
@@ -204,6 +208,15 @@ struct StageScope {
 
 extern "C" {
 
+int pvdb_fused_launch_defaults(int32_t* launch4) {
+    PVDB_CHECK_ARG(launch4 != nullptr, "launch4 is NULL");
+    launch4[0] = kFwdGrid;
+    launch4[1] = kScatterGrid;
+    launch4[2] = kScatterChunk;
+    launch4[3] = 0;
+    return PVDB_OK;
+}
+
 int pvdb_fused_timing(int32_t enable, double* total_ms, int32_t* calls, int32_t n) {
     for (auto& e : g_timer.open) {
         hipEventSynchronize(e.second.second);
@@ -268,6 +281,11 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
                                    d->mlp_m && d->mlp_v),
                    "Adam needs moment arrays");
     PVDB_CHECK_ARG(d->stepdist > 0.f, "stepdist must be positive");
+    PVDB_CHECK_ARG(d->launch[0] >= 0 && d->launch[0] <= 4096 && d->launch[1] >= 0 &&
+                       d->launch[1] <= 65536,
+                   "launch[0] must be in [0, 4096] and launch[1] in [0, 65536]");
+    PVDB_CHECK_ARG(d->launch[2] == 0 || d->launch[2] == 32 || d->launch[2] == 64,
+                   "launch[2]: scatter chunk must be 32 or 64 records");
     const TopoView t = view_of(topo);
     const Layout L = layout_for(d->n_rays, d->capacity, t.nleaf);
     PVDB_CHECK_ARG(d->workspace && d->workspace_bytes >= (int64_t)L.bytes, "workspace too small");
@@ -358,7 +376,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     // fixed (the record count stays on the device).  The backward holds one
     // workgroup per CU (registers), so it runs one per CU and writes that many
     // dW partials.
-    const int grid = kMaxMlpGrid;
+    const int grid = d->launch[0] > 0 ? d->launch[0] : kFwdGrid;
     static int ncu = 0;
     if (ncu == 0) {
         int dev = 0;
@@ -385,6 +403,8 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     {
         StageScope sc(6, s);
         if ((rc = launch_scatter(t, R, rec, d->capacity, d->grad_density, d->grad_color, touched,
+                                 d->launch[1] > 0 ? d->launch[1] : kScatterGrid,
+                                 d->launch[2] > 0 ? d->launch[2] : kScatterChunk,
                                  s)))
             return rc;
     }

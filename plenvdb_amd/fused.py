@@ -39,6 +39,7 @@ class StepDesc(C.Structure):
         ("do_adam", C.c_int32), ("reserved", C.c_int32),
         ("workspace", C.c_void_p), ("workspace_bytes", C.c_int64), ("capacity", C.c_int64),
         ("ray_out", C.c_void_p), ("loss", C.c_void_p), ("adam_scalars", C.c_void_p),
+        ("launch", C.c_int32 * 4),
     ]
 
 
@@ -46,6 +47,7 @@ lib.EXTRA_SIGS.update({
     "pvdb_fused_timing": [C.c_int32, C.c_void_p, C.c_void_p, C.c_int32],
     "pvdb_fused_init": [C.c_void_p, C.POINTER(StepDesc), C.c_void_p],
     "pvdb_fused_step": [C.c_void_p, C.POINTER(StepDesc), C.c_void_p],
+    "pvdb_fused_launch_defaults": [C.c_void_p],
     "pvdb_fused_records": [C.c_void_p, C.POINTER(StepDesc)] + [C.c_void_p] * 7,
 })
 
@@ -228,6 +230,16 @@ class FusedTrainer:
         torch.cuda.current_stream().wait_stream(s)
         torch.cuda.synchronize()
         self.graph = g
+
+    def autotune(self, reps: int = 3, force: bool = False) -> dict:
+        """Chooses the step's launch shapes for this device and workload (see
+        plenvdb_amd.autotune); a captured graph is captured again with them."""
+        from . import autotune
+
+        out = autotune.tune(self, reps=reps, force=force)
+        if self.graph is not None:
+            self.capture_graph()
+        return out
 
     def zero_grad(self):
         self.grad_density.zero_()
