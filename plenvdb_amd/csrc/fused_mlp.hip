@@ -874,8 +874,10 @@ __global__ void k_dw_reduce(const float* __restrict__ part, int nparts, float* _
 __global__ void k_pack_mlp(const float* __restrict__ prm, uint16_t* __restrict__ w0t,
                            uint16_t* __restrict__ w1t, uint16_t* __restrict__ w2t,
                            uint16_t* __restrict__ w2, uint16_t* __restrict__ w1,
-                           uint16_t* __restrict__ w0) {
+                           uint16_t* __restrict__ w0, float* __restrict__ grad) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    // the MLP gradient, accumulated by k_dw_reduce later in the step, starts at zero
+    if (grad != nullptr && t < kMlpParams) grad[t] = 0.f;
     // w1t / w1: 128 x 128
     if (t < 128 * 128) {
         const int a = t >> 7, b = t & 127;
@@ -929,9 +931,10 @@ int launch_dw_reduce(const float* part, int nparts, float* dw, hipStream_t s) {
 }
 
 int launch_pack_mlp(const float* prm, uint16_t* w0t, uint16_t* w1t, uint16_t* w2t, uint16_t* w2,
-                    uint16_t* w1, uint16_t* w0, hipStream_t s) {
-    hipLaunchKernelGGL(k_pack_mlp, dim3(pvdb_blocks(128 * 128, 256)), dim3(256), 0, s, prm, w0t,
-                       w1t, w2t, w2, w1, w0);
+                    uint16_t* w1, uint16_t* w0, float* grad, hipStream_t s) {
+    const int n = kMlpParams > 128 * 128 ? kMlpParams : 128 * 128;
+    hipLaunchKernelGGL(k_pack_mlp, dim3(pvdb_blocks(n, 256)), dim3(256), 0, s, prm, w0t, w1t, w2t,
+                       w2, w1, w0, grad);
     PVDB_LAUNCHED("k_pack_mlp");
     return PVDB_OK;
 }

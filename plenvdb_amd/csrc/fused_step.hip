@@ -28,7 +28,7 @@ int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, i
                    float* grad_density, float* grad_color, uint8_t* leaf_touched, int grid_req,
                    int chunk_records, hipStream_t s);
 int launch_pack_mlp(const float* prm, uint16_t* w0t, uint16_t* w1t, uint16_t* w2t, uint16_t* w2,
-                    uint16_t* w1, uint16_t* w0, hipStream_t s);
+                    uint16_t* w1, uint16_t* w0, float* grad, hipStream_t s);
 }  // namespace pvdb
 
 using namespace pvdb;
@@ -85,11 +85,6 @@ Layout layout_for(int64_t n_rays, int64_t capacity, int64_t nleaf) {
 }
 
 __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-__global__ void k_fill_f32(float* p, int64_t n, float v) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
@@ -358,11 +353,9 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     int rc;
     {
         StageScope sc(0, s);
-        if ((rc = launch_pack_mlp(d->mlp, w0t, w1t, w2t, w2, w1, w0, s))) return rc;
-        // the MLP gradient is accumulated by k_dw_reduce: it starts at zero
-        hipLaunchKernelGGL(k_fill_f32, dim3(pvdb_blocks(kMlpParams, 256)), dim3(256), 0, s,
-                           d->mlp_grad, (int64_t)kMlpParams, 0.f);
-        PVDB_LAUNCHED("k_fill_f32");
+        // packs the bf16 operands and zeroes the MLP gradient (accumulated by
+        // k_dw_reduce later in the step)
+        if ((rc = launch_pack_mlp(d->mlp, w0t, w1t, w2t, w2, w1, w0, d->mlp_grad, s))) return rc;
     }
     {
         StageScope sc(1, s);
