@@ -124,29 +124,79 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
             if (act) {
                 const float g = gden[s];
                 if (g != 0.f) adam1(den[s], g, mden[s], vden[s], sz_d, eps, b0, b1);
+                // the voxel's 12 channels as three 16-byte vectors per array
+                float4* g4 = reinterpret_cast<float4*>(gcol) + s * 3;
+                float4* m4 = reinterpret_cast<float4*>(mcol) + s * 3;
+                float4* v4 = reinterpret_cast<float4*>(vcol) + s * 3;
+                float gg[12], mm[12], vv[12], pp[12];
+                bool any = false;
 #pragma unroll
-                for (int grp = 0; grp < 4; ++grp) {
-                    const int64_t c0 = s * 12 + 3 * grp;
-                    const float g0 = gcol[c0], g1 = gcol[c0 + 1], g2 = gcol[c0 + 2];
-                    if (g0 == 0.f && g1 == 0.f && g2 == 0.f) continue;
-                    const float gg[3] = {g0, g1, g2};
+                for (int k4 = 0; k4 < 3; ++k4) {
+                    const float4 q = g4[k4];
+                    gg[4 * k4] = q.x; gg[4 * k4 + 1] = q.y; gg[4 * k4 + 2] = q.z; gg[4 * k4 + 3] = q.w;
+                }
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        if (BF16COL) {
-                            uint16_t* cp = reinterpret_cast<uint16_t*>(col) + c0 + c;
-                            float p = bf_bits2f(*cp);
-                            adam1(p, gg[c], mcol[c0 + c], vcol[c0 + c], sz_c, eps, b0, b1);
-                            *cp = f2bf_bits(p);
-                        } else {
-                            float* cp = reinterpret_cast<float*>(col) + c0 + c;
-                            adam1(*cp, gg[c], mcol[c0 + c], vcol[c0 + c], sz_c, eps, b0, b1);
+                for (int c = 0; c < 12; ++c) any = any || gg[c] != 0.f;
+                if (any) {
+#pragma unroll
+                    for (int k4 = 0; k4 < 3; ++k4) {
+                        const float4 qm = m4[k4], qv = v4[k4];
+                        mm[4 * k4] = qm.x; mm[4 * k4 + 1] = qm.y; mm[4 * k4 + 2] = qm.z; mm[4 * k4 + 3] = qm.w;
+                        vv[4 * k4] = qv.x; vv[4 * k4 + 1] = qv.y; vv[4 * k4 + 2] = qv.z; vv[4 * k4 + 3] = qv.w;
+                    }
+                    if (BF16COL) {
+                        const uint2* c2 = reinterpret_cast<const uint2*>(col) + s * 3;
+#pragma unroll
+                        for (int k4 = 0; k4 < 3; ++k4) {
+                            const uint2 q = c2[k4];
+                            pp[4 * k4] = bf_bits2f((uint16_t)(q.x & 0xffffu));
+                            pp[4 * k4 + 1] = bf_bits2f((uint16_t)(q.x >> 16));
+                            pp[4 * k4 + 2] = bf_bits2f((uint16_t)(q.y & 0xffffu));
+                            pp[4 * k4 + 3] = bf_bits2f((uint16_t)(q.y >> 16));
                         }
+                    } else {
+                        const float4* c4 = reinterpret_cast<const float4*>(col) + s * 3;
+#pragma unroll
+                        for (int k4 = 0; k4 < 3; ++k4) {
+                            const float4 q = c4[k4];
+                            pp[4 * k4] = q.x; pp[4 * k4 + 1] = q.y; pp[4 * k4 + 2] = q.z; pp[4 * k4 + 3] = q.w;
+                        }
+                    }
+                    // stepmode 1 skips a Vec3 group only when all three components are zero
+#pragma unroll
+                    for (int grp = 0; grp < 4; ++grp) {
+                        const int c0 = 3 * grp;
+                        if (gg[c0] == 0.f && gg[c0 + 1] == 0.f && gg[c0 + 2] == 0.f) continue;
+#pragma unroll
+                        for (int c = c0; c < c0 + 3; ++c) adam1(pp[c], gg[c], mm[c], vv[c], sz_c, eps, b0, b1);
+                    }
+#pragma unroll
+                    for (int k4 = 0; k4 < 3; ++k4) {
+                        m4[k4] = make_float4(mm[4 * k4], mm[4 * k4 + 1], mm[4 * k4 + 2], mm[4 * k4 + 3]);
+                        v4[k4] = make_float4(vv[4 * k4], vv[4 * k4 + 1], vv[4 * k4 + 2], vv[4 * k4 + 3]);
+                    }
+                    if (BF16COL) {
+                        uint2* c2 = reinterpret_cast<uint2*>(col) + s * 3;
+#pragma unroll
+                        for (int k4 = 0; k4 < 3; ++k4)
+                            c2[k4] = make_uint2((uint32_t)f2bf_bits(pp[4 * k4]) | ((uint32_t)f2bf_bits(pp[4 * k4 + 1]) << 16),
+                                                (uint32_t)f2bf_bits(pp[4 * k4 + 2]) | ((uint32_t)f2bf_bits(pp[4 * k4 + 3]) << 16));
+                    } else {
+                        float4* c4 = reinterpret_cast<float4*>(col) + s * 3;
+#pragma unroll
+                        for (int k4 = 0; k4 < 3; ++k4)
+                            c4[k4] = make_float4(pp[4 * k4], pp[4 * k4 + 1], pp[4 * k4 + 2], pp[4 * k4 + 3]);
                     }
                 }
             }
             gden[s] = 0.f;
-#pragma unroll
-            for (int c = 0; c < 12; ++c) gcol[s * 12 + c] = 0.f;
+            {
+                float4* g4 = reinterpret_cast<float4*>(gcol) + s * 3;
+                const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+                g4[0] = z;
+                g4[1] = z;
+                g4[2] = z;
+            }
         }
         __syncthreads();
         if (threadIdx.x == 0) touched[leaf] = 0;
