@@ -127,6 +127,16 @@ __device__ __forceinline__ void st_act(uint16_t* sh, uint16_t* sl, uint16_t* fm,
     }
 }
 
+// The hi parts of four consecutive features fr..fr+3 of sample `col`,
+// feature-major into fm (the feature-major half of st_act<true>).
+__device__ __forceinline__ void st_fm(uint16_t* fm, int col, int fr, int lane, const float* v) {
+    uint2 hi, lo;
+    split2(v[0], v[1], hi.x, lo.x);
+    split2(v[2], v[3], hi.y, lo.y);
+    const int q = lane & 3;
+    *reinterpret_cast<uint2*>(&fm[(fr + q) * FM + (col & ~3)]) = quad_transpose(hi, q);
+}
+
 // a (bf16 weight) times a split activation
 __device__ __forceinline__ f32x4 mfma_w(const bf16x8& a, const bf16x8& bh, const bf16x8& bl,
                                         f32x4 c) {
@@ -472,7 +482,7 @@ constexpr int kBw2Frags = 8 * 64, kBw1Frags = 8 * 4 * 64, kBw0Frags = 4 * 64;
 // features [16w, 16w + 16) of both hidden layers.  Per group of 32 records the
 // phases are (a barrier after each):
 //   3  H0^T = relu(W0^T X^T + b0) (recomputed; ReLU mask m0)   -> A1, H0f
-//   4  H1^T = relu(W1^T H0^T + b1) (ReLU mask m1)                -> A2, F2
+//   4  H1^T = relu(W1^T H0^T + b1) (ReLU mask m1)                -> F2
 //   7  g_H1^T = W2 g_L^T masked by m1 -> A1, G1f; dW2 += H1^T g_L; the next
 //      group's records are staged (park)
 //   8  g_H0^T = W1 g_H1^T masked by m0 -> A2, F2; dW1 += H0^T g_H1; the next
@@ -633,8 +643,9 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
     // positional encoding (dvgo.py:354-358) and bias column of buffer b's group:
     // thread (sample ps, part pp < 7) writes X columns 12 + 4pp .. +3; hardware
     // sin/cos (error about 1e-6 for |arg| <= 8, below what the (hi, lo) pair keeps)
-    auto encode = [&](int b) {
-        const int ps = tid & (GS - 1), pp = tid >> 5;
+    auto encode = [&](int b) {  // waves 2..5
+        const int te = tid - 128;
+        const int ps = te & (GS - 1), pp = te >> 5;
         const bool alive = (sm.bits[b][ps] & kAliveW) != 0u;
         const float4 v4 = sm.vd[b][ps];  // zero unless alive
         const float vd[3] = {v4.x, v4.y, v4.z};
@@ -671,13 +682,13 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
     stage_b(blockIdx.x);
     park(0);
     __syncthreads();
-    encode(0);
+    if (wv >= 2 && wv < 6) encode(0);
     __syncthreads();
+    stage_a(blockIdx.x + gridDim.x);
     int cb = 0;  // buffer of the current group
 
     for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {  // uniform bound
         const int64_t base = grp * GS, next = grp + gridDim.x;
-        stage_a(next);
         // ---- 3. H0^T slice = relu(W0^T X^T + b0) -> A1 (and H0f) ----
         uint32_t m0 = 0u, m1 = 0u;  // ReLU masks: bit nt * 4 + i
         f32x4 acc[NT];
@@ -701,9 +712,8 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
             }
             st_act<true>(sm.A1h, sm.A1l, sm.H0f, 16 * nt + r16, fr, lane, h);
         }
-        stage_b(next);  // the stage-A loads have landed by now
         __syncthreads();
-        // ---- 4. H1^T slice = relu(W1^T H0^T + b1) -> A2 (and F2) ----
+        // ---- 4. H1^T slice = relu(W1^T H0^T + b1) -> F2 (only dW2 reads H1) ----
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -722,8 +732,9 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
                 h[i] = fmaxf(acc[nt][i] + b1v[i], 0.f);
                 if (h[i] > 0.f) m1 |= 1u << (nt * 4 + i);
             }
-            st_act<true>(sm.A2h, sm.A2l, sm.F2, 16 * nt + r16, fr, lane, h);
+            st_fm(sm.F2, 16 * nt + r16, fr, lane, h);
         }
+        stage_b(next);  // the next group's records were loaded during the last group
         __syncthreads();
         // ---- 7. g_H1^T slice = W2[slice] g_L^T, masked -> A1 (and G1f); dW2; park ----
         {
@@ -746,7 +757,6 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
             dW2 = mfma16x16x32(ld_frag8(&sm.F2[(f0 + r16) * FM + 8 * g]),
                                ld_frag8(&sm.GLf[cb][r16 * FM + 8 * g]), dW2);
         }
-        park(cb ^ 1);  // the next group's stage-B loads (issued after step 3) have landed
         __syncthreads();
         // ---- 8. g_H0^T slice = W1[slice] g_H1^T, masked -> A2 (and F2); dW1; encode ----
 #pragma unroll
@@ -767,7 +777,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
             for (int mi = 0; mi < 8; ++mi)
                 dW1[mi] = mfma16x16x32(ld_frag8(&sm.H0f[(16 * mi + r16) * FM + 8 * g]), bh, dW1[mi]);
         }
-        // A2 and F2 were last read in step 7 (before its barrier): free for g_H0
+        // A2 is free and F2 was last read in step 7 (before its barrier): both take g_H0
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
             float gv[4];
@@ -775,7 +785,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
             for (int i = 0; i < 4; ++i) gv[i] = (m0 >> (nt * 4 + i)) & 1u ? acc[nt][i] : 0.f;
             st_act<true>(sm.A2h, sm.A2l, sm.F2, 16 * nt + r16, fr, lane, gv);
         }
-        encode(cb ^ 1);
+        park(cb ^ 1);  // the next group's stage-B loads (issued after step 4) have landed
         __syncthreads();
         // ---- 9. dW0 columns of this wave; g_X^T (waves 0-1) -> record gradients ----
         {
@@ -807,7 +817,10 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
                 }
                 rec.grad[4 * q + g] = v;
             }
+        } else if (wv < 6) {
+            encode(cb ^ 1);
         }
+        stage_a(next + gridDim.x);  // the records of the group after next
         __syncthreads();
         cb ^= 1;
     }
