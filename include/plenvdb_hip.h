@@ -207,6 +207,12 @@ typedef struct pvdb_step_desc {
      * per chunk (32 or 64), [3] unused; pvdb_fused_launch_defaults() fills the
      * defaults for this device */
     int32_t launch[4];
+    /* loss weights of plenvdb/run.py:551-574 (cfg_train.weight_*): the MSE
+     * term (1.0 in every configuration), the background entropy of T_last
+     * (clamped to [1e-6, 1 - 1e-6], mean over rays) and the per-sample rgb
+     * loss sum(w.detach() * |rgb_i - target|^2) / N; 0 switches a term off */
+    float weight_main, weight_entropy_last, weight_rgbper;
+    int32_t reserved2;
 } pvdb_step_desc;
 
 /* The default launch shapes (pvdb_step_desc.launch) on the current device. */

@@ -363,7 +363,13 @@ def reference_train_step(scene: dict, prec: str = "f32", with_mlp: bool = True):
     scene keys: origins, vmask (topology), density [nslots], color [nslots,C],
     reso, xyz_min, xyz_max, rays_o, rays_d, viewdirs, target, near, stepsize,
     voxel_size, voxel_size_ratio, act_shift, fast_color_thres, bg, mlp (dict of
-    w0,b0,w1,b1,w2,b2 with w as [in,out]) and optional far_per_ray.
+    w0,b0,w1,b1,w2,b2 with w as [in,out]), optional far_per_ray and optional
+    loss weights weight_main (1), weight_entropy_last (0), weight_rgbper (0).
+
+    The loss is run.py:551-574: weight_main * MSE, plus weight_entropy_last *
+    the mean over rays of -(p log p + (1-p) log(1-p)) with p = alphainv_last
+    clamped to [1e-6, 1-1e-6], plus weight_rgbper * sum over the weight-kept
+    samples of weights.detach() * |raw_rgb - target[ray_id]|^2, divided by N.
 
     The occupancy mask is the topology's active-voxel set, looked up with the
     MaskGrid rule (grid.py:234-243).  Returns a dict with the loss, rendered
@@ -434,7 +440,15 @@ def reference_train_step(scene: dict, prec: str = "f32", with_mlp: bool = True):
     acc = torch.zeros(N, nch, dtype=tdt).index_add_(0, rid_s, w_t[:, None] * rgb)
     rgb_marched = acc + tl_t[:, None] * float(scene["bg"])
     target = torch.tensor(np.asarray(scene["target"], dt), dtype=tdt)
-    loss = torch.mean((rgb_marched - target) ** 2)
+    loss = float(scene.get("weight_main", 1.0)) * torch.mean((rgb_marched - target) ** 2)
+    w_ent = float(scene.get("weight_entropy_last", 0.0))
+    if w_ent > 0:  # run.py:553-556
+        pout = tl_t.clamp(1e-6, 1 - 1e-6)
+        loss = loss + w_ent * (-(pout * torch.log(pout) + (1 - pout) * torch.log(1 - pout))).mean()
+    w_rp = float(scene.get("weight_rgbper", 0.0))
+    if w_rp > 0:  # run.py:570-573
+        rgbper = (rgb - target[rid_s]).pow(2).sum(-1)
+        loss = loss + w_rp * (rgbper * w_t.detach()).sum() / N
     loss.backward()
     g_k0 = k0_t.grad.numpy()
     g_w = np.zeros(len(a_c), dt)

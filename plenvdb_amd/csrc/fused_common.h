@@ -43,7 +43,9 @@ struct StepParams {
     float ms[3], mb[3];        // occupancy lookup: round(p * ms + mb)
     float near_, far_, stepdist;
     float act_shift, interval, thres, bg;
-    float loss_scale;          // 1 / (3 N): MSE mean over N rays x 3 channels
+    float loss_scale;          // weight_main / (3 N): MSE mean over N rays x 3 channels
+    float ent_scale;           // weight_entropy_last / N: entropy of T_last, mean over rays
+    float rgbper_scale;        // weight_rgbper / N: sum of w * |rgb_i - target|^2 over samples
     int64_t capacity;          // record capacity
 };
 
@@ -65,7 +67,7 @@ struct RayBuffers {
     float4* ray_out;   // [N] rendered rgb + T_last
     float4* ray_grad;  // [N] dL/drgb (3) + dL/dT_last
     float4* ray_ctot;  // [N] sum of w*rgb (3)
-    float* ray_loss;   // [N] per-ray squared error * loss_scale
+    float* ray_loss;   // [N] the ray's share of the loss (all weighted terms)
     float* loss;       // [1]
 };
 

@@ -224,14 +224,19 @@ __global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, con
 }
 
 // Per-ray composite: inclusive prefix of w*rgb (kept for the alpha adjoint),
-// ray colour = sum + T_last*bg (dvgo.py:365-370), MSE loss and its gradient.
+// ray colour = sum + T_last*bg (dvgo.py:365-370), the loss of run.py:551-574
+// (MSE; background entropy of T_last; per-sample rgb loss with the weights
+// detached) and its gradient.
 __global__ void __launch_bounds__(256) k_ray_composite(StepParams P, RayBuffers R, Records rec) {
     const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= P.n_rays) return;
     const int lane = lane_id();
     const int64_t off = R.offset[r];
     const int cnt = R.count[r];
+    const float t0 = P.target[3 * r], t1 = P.target[3 * r + 1], t2 = P.target[3 * r + 2];
+    const bool rgbper = P.rgbper_scale > 0.f;
     float cr = 0.f, cg = 0.f, cb = 0.f;
+    float sq = 0.f;  // this lane's sum of w * |rgb - target|^2 (rgbper term)
     for (int base = 0; base < cnt; base += 64) {
         const int j = base + lane;
         float vr = 0.f, vg = 0.f, vb = 0.f;
@@ -244,6 +249,10 @@ __global__ void __launch_bounds__(256) k_ray_composite(StepParams P, RayBuffers 
                 vr = a.z * c.x;
                 vg = a.z * c.y;
                 vb = a.z * c.z;
+                if (rgbper) {
+                    const float e0 = c.x - t0, e1 = c.y - t1, e2 = c.z - t2;
+                    sq += a.z * ((e0 * e0 + e1 * e1) + e2 * e2);
+                }
             }
         }
         for (int d = 1; d < 64; d <<= 1) {
@@ -262,19 +271,35 @@ __global__ void __launch_bounds__(256) k_ray_composite(StepParams P, RayBuffers 
     // the ray's colour, loss and dL/drgb (every lane holds them)
     const float tl = R.t_last[r];
     const float o0 = cr + tl * P.bg, o1 = cg + tl * P.bg, o2 = cb + tl * P.bg;
-    const float d0 = o0 - P.target[3 * r], d1 = o1 - P.target[3 * r + 1],
-                d2 = o2 - P.target[3 * r + 2];
+    const float d0 = o0 - t0, d1 = o1 - t1, d2 = o2 - t2;
     const float g0 = 2.f * d0 * P.loss_scale, g1 = 2.f * d1 * P.loss_scale,
                 g2 = 2.f * d2 * P.loss_scale;
-    const float gbg = P.bg * (g0 + g1 + g2);  // dL/dT_last
+    float loss = (d0 * d0 + d1 * d1 + d2 * d2) * P.loss_scale;
+    float gent = 0.f;  // dL/dT_last of the entropy term
+    if (P.ent_scale > 0.f) {
+        // pout = clamp(T_last, 1e-6, 1 - 1e-6); -(p log p + (1-p) log(1-p)), whose
+        // derivative log(1-p) - log(p) passes the clamp only inside its bounds
+        const float p = fminf(fmaxf(tl, 1e-6f), 0.999999f);
+        const float lp = logf(p), lq = logf(1.f - p);
+        loss += -(p * lp + (1.f - p) * lq) * P.ent_scale;
+        if (tl >= 1e-6f && tl <= 0.999999f) gent = (lq - lp) * P.ent_scale;
+    }
+    if (rgbper) {
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) sq += __shfl_xor(sq, d, 64);
+        loss += sq * P.rgbper_scale;
+    }
+    const float gbg = P.bg * (g0 + g1 + g2) + gent;  // dL/dT_last
+    const float grp = 2.f * P.rgbper_scale;          // dL/drgb_i = w_i (g + grp (rgb_i - target))
     if (lane == 0) {
         R.ray_out[r] = make_float4(o0, o1, o2, tl);
         R.ray_grad[r] = make_float4(g0, g1, g2, gbg);
         R.ray_ctot[r] = make_float4(cr, cg, cb, 0.f);
-        R.ray_loss[r] = (d0 * d0 + d1 * d1 + d2 * d2) * P.loss_scale;
+        R.ray_loss[r] = loss;
     }
-    // second pass: each sample's adjoints.  dL/dlogit = w * dL/drgb * rgb(1 - rgb)
-    // (sigmoid, dvgo.py:362); dL/dsigma through Alphas2Weights and raw2alpha
+    // second pass: each sample's adjoints.  dL/dlogit = dL/drgb_i * rgb(1 - rgb)
+    // (sigmoid, dvgo.py:362), dL/drgb_i = w (g + grp (rgb_i - target)) with the
+    // weight detached in the rgbper term; dL/dsigma through Alphas2Weights and raw2alpha
     // (render_utils_kernel.cu:653-676, 506-517), with the exclusive suffix of
     // w*rgb taken as the ray total minus the inclusive prefix
     float pr = 0.f, pg = 0.f, pb = 0.f;
@@ -308,9 +333,9 @@ __global__ void __launch_bounds__(256) k_ray_composite(StepParams P, RayBuffers 
         }
         float gl0 = 0.f, gl1 = 0.f, gl2 = 0.f, gw = 0.f, gs = 0.f;
         if (bits & kAliveW) {
-            gl0 = a.z * g0 * c.x * (1.f - c.x);
-            gl1 = a.z * g1 * c.y * (1.f - c.y);
-            gl2 = a.z * g2 * c.z * (1.f - c.z);
+            gl0 = a.z * (g0 + grp * (c.x - t0)) * c.x * (1.f - c.x);
+            gl1 = a.z * (g1 + grp * (c.y - t1)) * c.y * (1.f - c.y);
+            gl2 = a.z * (g2 + grp * (c.z - t2)) * c.z * (1.f - c.z);
             gw = g0 * c.x + g1 * c.y + g2 * c.z;
         }
         if (bits & kAliveA) {
@@ -326,7 +351,7 @@ __global__ void __launch_bounds__(256) k_ray_composite(StepParams P, RayBuffers 
     }
 }
 
-// MSE loss = sum of the per-ray terms (one workgroup; the order of the sum is fixed).
+// Loss = sum of the per-ray terms (one workgroup; the order of the sum is fixed).
 __global__ void __launch_bounds__(1024) k_loss_reduce(RayBuffers R, int n) {
     __shared__ float part[1024];
     float s = 0.f;

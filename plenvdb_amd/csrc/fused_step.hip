@@ -331,6 +331,10 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
                    "launch[0] must be in [0, 4096] and launch[1] in [0, 65536]");
     PVDB_CHECK_ARG(d->launch[2] == 0 || d->launch[2] == 32 || d->launch[2] == 64,
                    "launch[2]: scatter chunk must be 32 or 64 records");
+    PVDB_CHECK_ARG(d->weight_main >= 0.f && d->weight_entropy_last >= 0.f &&
+                       d->weight_rgbper >= 0.f && d->weight_main < INFINITY &&
+                       d->weight_entropy_last < INFINITY && d->weight_rgbper < INFINITY,
+                   "loss weights must be finite and >= 0");
     const TopoView t = view_of(topo);
     const Layout L = layout_for(d->n_rays, d->capacity, t.nleaf);
     PVDB_CHECK_ARG(d->workspace && d->workspace_bytes >= (int64_t)L.bytes, "workspace too small");
@@ -361,7 +365,9 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     P.interval = d->interval;
     P.thres = d->fast_color_thres;
     P.bg = d->bg;
-    P.loss_scale = 1.f / (3.f * (float)d->n_rays);
+    P.loss_scale = d->weight_main / (3.f * (float)d->n_rays);
+    P.ent_scale = d->weight_entropy_last / (float)d->n_rays;
+    P.rgbper_scale = d->weight_rgbper / (float)d->n_rays;
     P.capacity = d->capacity;
 
     RayBuffers R;

@@ -40,6 +40,8 @@ class StepDesc(C.Structure):
         ("workspace", C.c_void_p), ("workspace_bytes", C.c_int64), ("capacity", C.c_int64),
         ("ray_out", C.c_void_p), ("loss", C.c_void_p), ("adam_scalars", C.c_void_p),
         ("launch", C.c_int32 * 4),
+        ("weight_main", C.c_float), ("weight_entropy_last", C.c_float),
+        ("weight_rgbper", C.c_float), ("reserved2", C.c_int32),
     ]
 
 
@@ -88,12 +90,15 @@ class FusedTrainer:
     """One training iteration per `step()` on the fused kernels.
 
     scene: scenes.make_scene(...) output (must use the MLP).  color_bf16
-    defaults to the configuration's choice.
+    defaults to the configuration's choice.  weight_main, weight_entropy_last
+    and weight_rgbper are the loss weights of run.py:551-574 (the fine stage of
+    configs/default.py:73-74 uses 1, 0.001 and 0.01; the defaults here are MSE only).
     """
 
     def __init__(self, scene: dict, topo: Topology | None = None, color_bf16: bool | None = None,
                  lr_grid=0.1, lr_mlp=1e-3, lrate_decay=20, capacity: int | None = None,
-                 device="cuda"):
+                 weight_main: float = 1.0, weight_entropy_last: float = 0.0,
+                 weight_rgbper: float = 0.0, device="cuda"):
         cfg = scene["cfg"]
         if not cfg.use_mlp:
             raise ValueError("the fused path runs the MLP configurations")
@@ -137,6 +142,7 @@ class FusedTrainer:
         self.step_count = 0
         # per-step Adam step sizes, read on the device (a captured step replays them)
         self.adam_scalars = torch.zeros(4, device=dev)
+        self.weights = (float(weight_main), float(weight_entropy_last), float(weight_rgbper))
         self.desc = self._make_desc()
         lib.call("pvdb_fused_init", self.topo.handle, C.byref(self.desc), lib.stream())
         self.graph = None
@@ -184,6 +190,7 @@ class FusedTrainer:
         d.ray_out = self.ray_out.data_ptr()
         d.loss = self.loss.data_ptr()
         d.adam_scalars = self.adam_scalars.data_ptr()
+        d.weight_main, d.weight_entropy_last, d.weight_rgbper = self.weights
         return d
 
     @staticmethod

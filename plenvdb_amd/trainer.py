@@ -1,7 +1,7 @@
 """Training-step drivers over a synthetic scene (see scenes.py).
 
 `PlainTrainer` runs the reference's iteration (plenvdb/run.py:541-598) on the
-stage-per-kernel model of dvgo.py: forward, zero_grad, MSE loss, backward, the
+stage-per-kernel model of dvgo.py: forward, zero_grad, the loss (`step_loss`), backward, the
 MLP's MaskedAdam step, the VDB Adam step (skip-zero-grad for both grids, as
 the fine stage configures, configs/default.py:74) and the lr decay.
 """
@@ -26,10 +26,28 @@ def init_values(scene: dict, topo: Topology):
     return den, col
 
 
+def step_loss(out: dict, target: torch.Tensor, weight_main: float = 1.0,
+              weight_entropy_last: float = 0.0, weight_rgbper: float = 0.0) -> torch.Tensor:
+    """The training loss of run.py:551-574 from a forward's outputs: weighted
+    MSE, background entropy of alphainv_last and the per-sample rgb loss with
+    the weights detached (nearclip and distortion are off in every configuration)."""
+    loss = weight_main * torch.nn.functional.mse_loss(out["rgb_marched"], target)
+    if weight_entropy_last > 0:
+        pout = out["alphainv_last"].clamp(1e-6, 1 - 1e-6)
+        ent = -(pout * torch.log(pout) + (1 - pout) * torch.log(1 - pout)).mean()
+        loss = loss + weight_entropy_last * ent
+    if weight_rgbper > 0:
+        rgbper = (out["raw_rgb"] - target[out["ray_id"]]).pow(2).sum(-1)
+        loss = loss + weight_rgbper * (rgbper * out["weights"].detach()).sum() / len(target)
+    return loss
+
+
 class PlainTrainer:
     def __init__(self, scene: dict, topo: Topology | None = None, lr_grid=0.1, lr_mlp=1e-3,
-                 lrate_decay=20, device="cuda"):
+                 lrate_decay=20, weight_main: float = 1.0, weight_entropy_last: float = 0.0,
+                 weight_rgbper: float = 0.0, device="cuda"):
         self.scene = scene
+        self.weights = (float(weight_main), float(weight_entropy_last), float(weight_rgbper))
         cfg = scene["cfg"]
         self.topo = topo if topo is not None else build_topology(scene)
         self.model = DirectVoxGO(scene["xyz_min"], scene["xyz_max"], scene["reso"],
@@ -65,7 +83,7 @@ class PlainTrainer:
         if self.opt is not None:
             self.opt.zero_grad(set_to_none=True)
         self.vdbopt.zero_grad()
-        loss = torch.nn.functional.mse_loss(out["rgb_marched"], self.target)
+        loss = step_loss(out, self.target, *self.weights)
         loss.backward()
         self.last = out
         return loss, out

@@ -41,19 +41,29 @@ def oracle_for(oracle_mod, tr, sc, held_weights=True):
     mlp = {k: v.astype(np.float64) for k, v in unpack_mlp(tr.mlp.cpu().numpy()).items()}
     if held_weights:
         mlp = bf16_weights(mlp)
-    osc = dict(sc, origins=origins, vmask=vm, density=den, color=col, mlp=mlp)
+    wm, we, wr = tr.weights
+    osc = dict(sc, origins=origins, vmask=vm, density=den, color=col, mlp=mlp, weight_main=wm,
+               weight_entropy_last=we, weight_rgbper=wr)
     return oracle_mod.reference_train_step(osc, prec="f64", with_mlp=True)
 
 
-@pytest.mark.parametrize("name,n_rays,bf16", [("c2", 512, False), ("c3", 512, False),
-                                               ("c2", 384, True), ("c4", 256, True)])
-def test_fused_forward_backward_vs_fp64_oracle(oracle_mod, name, n_rays, bf16):
+MSE = (1.0, 0.0, 0.0)
+# the fine stage's loss weights (configs/default.py:73-74), the only stage with the MLP
+FINE = (1.0, 0.001, 0.01)
+
+
+@pytest.mark.parametrize("name,n_rays,bf16,weights", [
+    ("c2", 512, False, MSE), ("c3", 512, False, MSE), ("c2", 384, True, MSE),
+    ("c4", 256, True, MSE), ("c2", 384, False, FINE), ("c3", 256, True, FINE)])
+def test_fused_forward_backward_vs_fp64_oracle(oracle_mod, name, n_rays, bf16, weights):
     from plenvdb_amd import scenes
     from plenvdb_amd.fused import FusedTrainer, unpack_mlp
 
-    print("[case] fused fwd/bwd", name, n_rays, "bf16" if bf16 else "fp32", flush=True)
+    print("[case] fused fwd/bwd", name, n_rays, "bf16" if bf16 else "fp32", "loss weights",
+          weights, flush=True)
     sc = scenes.make_scene(name, n_rays=n_rays)
-    tr = FusedTrainer(sc, color_bf16=bf16)
+    tr = FusedTrainer(sc, color_bf16=bf16, weight_main=weights[0],
+                      weight_entropy_last=weights[1], weight_rgbper=weights[2])
     tr.forward_backward()
     torch.cuda.synchronize()
     ref = oracle_for(oracle_mod, tr, sc)

@@ -14,18 +14,25 @@ def _oracle_scene(oracle_mod, sc, den, col):
     return out
 
 
-@pytest.mark.parametrize("name,n_rays", [("c1", 512), ("c2", 512), ("c3", 384)])
-def test_plain_step_matches_oracle(oracle_mod, name, n_rays):
+@pytest.mark.parametrize("name,n_rays,weights", [
+    ("c1", 512, (1.0, 0.0, 0.0)), ("c2", 512, (1.0, 0.0, 0.0)), ("c3", 384, (1.0, 0.0, 0.0)),
+    ("c2", 384, (1.0, 0.001, 0.01)), ("c1", 384, (1.0, 0.01, 0.1)), ("c1", 256, (0.5, 0.3, 0.5))])
+def test_plain_step_matches_oracle(oracle_mod, name, n_rays, weights):
+    """weights: (main, entropy_last, rgbper) loss weights of run.py:551-574: MSE
+    only, the fine stage's (configs/default.py:73-74), the coarse stage's
+    (:52-55, no MLP as in c1) and a stress setting."""
     from plenvdb_amd import scenes
     from plenvdb_amd.trainer import PlainTrainer, init_values
 
-    print("[case] plain step", name, n_rays, flush=True)
+    print("[case] plain step", name, n_rays, "loss weights", weights, flush=True)
     sc = scenes.make_scene(name, n_rays=n_rays)
-    tr = PlainTrainer(sc)
+    tr = PlainTrainer(sc, weight_main=weights[0], weight_entropy_last=weights[1],
+                      weight_rgbper=weights[2])
     den, col = init_values(sc, tr.topo)
     loss, out = tr.forward_backward()
     torch.cuda.synchronize()
     osc = _oracle_scene(oracle_mod, sc, den, col)
+    osc.update(weight_main=weights[0], weight_entropy_last=weights[1], weight_rgbper=weights[2])
     ref = oracle_mod.reference_train_step(osc, prec="f32", with_mlp=sc["cfg"].use_mlp)
     ref64 = oracle_mod.reference_train_step(
         dict(osc, density=den.astype(np.float64), color=col.astype(np.float64)), prec="f64",

@@ -244,6 +244,58 @@ def test_reference_step_gradient_matches_fd(oracle_mod):
         assert abs(fd - gc[s, c]) < 1e-5 * max(abs(fd), 1e-6) + 1e-11
 
 
+def test_reference_step_loss_extras(oracle_mod):
+    """The fine stage's extra loss terms (run.py:553-573) in the FP64 reference
+    step: the entropy term equals scipy's binary entropy of the clamped
+    alphainv_last; the rgbper term is positive and, its weights being detached,
+    leaves the density gradient unchanged; central differences match the slot
+    gradients (64 density entries with the entropy term, whose gradient is the
+    loss's true derivative, and 64 colour entries with both terms: the colour
+    does not move the weights, so there the detach is exact)."""
+    from scipy.special import entr
+
+    from plenvdb_amd import scenes
+
+    sc = scenes.make_scene("c2", n_rays=24)
+    origins, vm = oracle_mod.leaves_from_mask(sc["mask"])
+    rng = np.random.default_rng(2)
+    nslots = len(origins) * 512
+    den = rng.normal(0, 2, nslots)
+    col = rng.normal(0, 0.1, (nslots, 12))
+    sc.update(origins=origins, vmask=vm, density=den, color=col, fast_color_thres=0.0)
+    base = oracle_mod.reference_train_step(sc, prec="f64")
+    ent = oracle_mod.reference_train_step(dict(sc, weight_entropy_last=0.3), prec="f64")
+    p = np.clip(base["alphainv_last"], 1e-6, 1 - 1e-6)
+    np.testing.assert_allclose(ent["loss"] - base["loss"], 0.3 * np.mean(entr(p) + entr(1 - p)),
+                               rtol=1e-10)
+    assert np.abs(ent["grad_density"] - base["grad_density"]).max() > 1e-6
+    rp = oracle_mod.reference_train_step(dict(sc, weight_rgbper=0.5), prec="f64")
+    assert rp["loss"] > base["loss"]
+    np.testing.assert_array_equal(rp["grad_density"], base["grad_density"])
+    assert np.abs(rp["grad_color"] - base["grad_color"]).max() > 1e-6
+    h = 1e-5
+
+    def fd_check(scene, key, idx, g):
+        v0 = scene[key]
+        vp, vm_ = v0.copy(), v0.copy()
+        vp[idx] += h
+        vm_[idx] -= h
+        lp = oracle_mod.reference_train_step(dict(scene, **{key: vp}), prec="f64")["loss"]
+        lm = oracle_mod.reference_train_step(dict(scene, **{key: vm_}), prec="f64")["loss"]
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g) < 1e-5 * max(abs(fd), 1e-6) + 1e-11, (key, idx, fd, g)
+
+    sc_e = dict(sc, weight_entropy_last=0.3)
+    gd = ent["grad_density"]
+    for i in rng.choice(np.nonzero(np.abs(gd) > 1e-6)[0], 64, replace=False):
+        fd_check(sc_e, "density", i, gd[i])
+    sc_b = dict(sc, weight_entropy_last=0.3, weight_rgbper=0.5)
+    gc = oracle_mod.reference_train_step(sc_b, prec="f64")["grad_color"]
+    cand_c = np.argwhere(np.abs(gc) > 1e-6)
+    for s_, c in cand_c[rng.choice(len(cand_c), 64, replace=False)]:
+        fd_check(sc_b, "color", (s_, c), gc[s_, c])
+
+
 def _pe27(vd):
     """Per-pixel PE of the renderer in the order get_rays writes it
     (renderer.cu:153-165): v, then sin(v_x 2^k), sin(v_y 2^k), sin(v_z 2^k),
