@@ -110,6 +110,15 @@ int pvdb_dense_adam(float* param, const float* grad, float* exp_avg, float* exp_
                     float beta0, float beta1, void* stream);
 
 /* ---- ray operations (plenvdb/lib/cuda/render_utils_kernel.cu) ------------ */
+/* get_rays_of_a_view (plenvdb/lib/dvgo.py:522-527) with get_rays (:471-499) and
+ * ndc_rays (:502-519, near = 1, focal = K[0][0]): rays_o, rays_d and viewdirs
+ * [H][W][3] of one camera.  K: host 3x3 row-major (double, as the loaders hand
+ * it over; rounded to float32 where torch rounds it); c2w: host 3x4 row-major
+ * float32.  mode 0 = 'center', 1 = 'lefttop' ('random' is not supported). */
+int pvdb_rays_of_view(int32_t H, int32_t W, const double* K /*host[9]*/,
+                      const float* c2w /*host[12]*/, int32_t ndc, int32_t inverse_y,
+                      int32_t flip_x, int32_t flip_y, int32_t mode, float* rays_o,
+                      float* rays_d, float* viewdirs, void* stream);
 /* infer_t_minmax + infer_n_samples (:11-55).  far_per_ray may be NULL. */
 int pvdb_ray_bounds(const float* rays_o, const float* rays_d, int64_t n,
                     const float* xyz_min /*host[3]*/, const float* xyz_max /*host[3]*/,

@@ -184,6 +184,53 @@ def trilinear_bwd(dom: SlotDomain, C_: int, xs, ys, zs, g, order: int, prec: str
     return grad
 
 
+def rays_of_view(H, W, K, c2w, ndc=False, inverse_y=False, flip_x=False, flip_y=False,
+                 mode="center"):
+    """get_rays_of_a_view (dvgo.py:522-527) with get_rays (:471-499) and ndc_rays
+    (:502-519, near 1, focal K[0][0]) in float32, one numpy operation per torch
+    operation: K's entries and the NDC constants are rounded to float32 where
+    torch takes them as scalars.  Returns rays_o, rays_d, viewdirs [H, W, 3]."""
+    f = np.float32
+    K = np.asarray(K, np.float64)
+    c2w = np.asarray(c2w, np.float32)
+    i, j = np.meshgrid(np.arange(W, dtype=f), np.arange(H, dtype=f))  # [H, W]: i = column
+    if mode == "center":
+        i, j = i + f(0.5), j + f(0.5)
+    elif mode != "lefttop":
+        raise NotImplementedError(mode)
+    if flip_x:
+        i = i[:, ::-1]
+    if flip_y:
+        j = j[::-1, :]
+    d0 = (i - f(K[0][2])) / f(K[0][0])
+    if inverse_y:
+        d1 = (j - f(K[1][2])) / f(K[1][1])
+        d2 = np.ones_like(i)
+    else:
+        d1 = -((j - f(K[1][2])) / f(K[1][1]))
+        d2 = -np.ones_like(i)
+    R = c2w[:3, :3]
+    rd = np.stack([(d0 * R[k, 0] + d1 * R[k, 1]) + d2 * R[k, 2] for k in range(3)], -1).astype(f)
+    nrm = np.sqrt((rd[..., 0] * rd[..., 0] + rd[..., 1] * rd[..., 1]) + rd[..., 2] * rd[..., 2])
+    vd = rd / nrm[..., None]
+    ro = np.broadcast_to(c2w[:3, 3], rd.shape).astype(f)
+    if ndc:
+        near = 1.0
+        focal = float(K[0][0])
+        t = -(f(near) + ro[..., 2]) / rd[..., 2]
+        ro = ro + t[..., None] * rd
+        ax, ay = f(-1. / (W / (2. * focal))), f(-1. / (H / (2. * focal)))
+        o0 = (ax * ro[..., 0]) / ro[..., 2]
+        o1 = (ay * ro[..., 1]) / ro[..., 2]
+        o2 = f(1.) + f(2. * near) / ro[..., 2]
+        e0 = ax * (rd[..., 0] / rd[..., 2] - ro[..., 0] / ro[..., 2])
+        e1 = ay * (rd[..., 1] / rd[..., 2] - ro[..., 1] / ro[..., 2])
+        e2 = f(-2. * near) / ro[..., 2]
+        ro = np.stack([o0, o1, o2], -1)
+        rd = np.stack([e0, e1, e2], -1)
+    return ro.astype(f), rd.astype(f), vd.astype(f)
+
+
 def ray_bounds(rays_o, rays_d, xyz_min, xyz_max, near, far, stepdist, far_per_ray=None,
                prec="f32"):
     lib = load()

@@ -190,9 +190,114 @@ __global__ void k_alpha2weight_bwd(const float* __restrict__ alpha, const float*
     }
 }
 
+// get_rays + ndc_rays + get_rays_of_a_view (dvgo.py:471-527), one pixel per
+// thread.  The scalars that torch rounds to float32 (K's entries, the NDC
+// constants formed in double on the host) arrive rounded; every tensor
+// operation is one float32 operation in the reference's order.
+struct ViewParams {
+    int H, W;
+    float fx, fy, cx, cy;  // K[0][0], K[1][1], K[0][2], K[1][2]
+    float R[9];            // c2w[:3,:3], row-major
+    float o[3];            // c2w[:3,3]
+    int ndc, inverse_y, flip_x, flip_y;
+    float off;           // 0.5 ('center') or 0 ('lefttop')
+    float near_;         // NDC near plane (1.0)
+    float ax, ay;        // -1./(W/(2.*focal)), -1./(H/(2.*focal))
+    float two_near;      // 2. * near
+    float neg_two_near;  // -2. * near
+};
+
+__global__ void k_rays_of_view(ViewParams V, float* __restrict__ ro, float* __restrict__ rd,
+                               float* __restrict__ vd) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)V.H * V.W) return;
+    const int h = (int)(p / V.W), w = (int)(p % V.W);
+    // linspace(0, n-1, n) holds the exact integers; flips reverse an axis
+    const float i = (float)(V.flip_x ? V.W - 1 - w : w) + V.off;
+    const float j = (float)(V.flip_y ? V.H - 1 - h : h) + V.off;
+    const float d0 = (i - V.cx) / V.fx;
+    float d1, d2;
+    if (V.inverse_y) {
+        d1 = (j - V.cy) / V.fy;
+        d2 = 1.f;
+    } else {
+        d1 = -((j - V.cy) / V.fy);
+        d2 = -1.f;
+    }
+    // rays_d = sum(dirs[..., None, :] * c2w[:3,:3], -1)
+    float dx = (d0 * V.R[0] + d1 * V.R[1]) + d2 * V.R[2];
+    float dy = (d0 * V.R[3] + d1 * V.R[4]) + d2 * V.R[5];
+    float dz = (d0 * V.R[6] + d1 * V.R[7]) + d2 * V.R[8];
+    const float nrm = sqrtf((dx * dx + dy * dy) + dz * dz);
+    vd[3 * p] = dx / nrm;
+    vd[3 * p + 1] = dy / nrm;
+    vd[3 * p + 2] = dz / nrm;
+    float ox = V.o[0], oy = V.o[1], oz = V.o[2];
+    if (V.ndc) {
+        const float t = -(V.near_ + oz) / dz;
+        ox = ox + t * dx;
+        oy = oy + t * dy;
+        oz = oz + t * dz;
+        const float o0 = (V.ax * ox) / oz, o1 = (V.ay * oy) / oz;
+        const float o2 = 1.f + V.two_near / oz;
+        const float e0 = V.ax * (dx / dz - ox / oz), e1 = V.ay * (dy / dz - oy / oz);
+        const float e2 = V.neg_two_near / oz;
+        ox = o0;
+        oy = o1;
+        oz = o2;
+        dx = e0;
+        dy = e1;
+        dz = e2;
+    }
+    ro[3 * p] = ox;
+    ro[3 * p + 1] = oy;
+    ro[3 * p + 2] = oz;
+    rd[3 * p] = dx;
+    rd[3 * p + 1] = dy;
+    rd[3 * p + 2] = dz;
+}
+
 }  // namespace
 
 extern "C" {
+
+int pvdb_rays_of_view(int32_t H, int32_t W, const double* K, const float* c2w, int32_t ndc,
+                      int32_t inverse_y, int32_t flip_x, int32_t flip_y, int32_t mode,
+                      float* rays_o, float* rays_d, float* viewdirs, void* stream) {
+    PVDB_CHECK_ARG(H >= 0 && W >= 0 && K && c2w, "bad arguments");
+    PVDB_CHECK_ARG(mode == 0 || mode == 1,
+                   "mode must be 0 ('center') or 1 ('lefttop'); 'random' draws from torch's RNG");
+    if ((int64_t)H * W == 0) return PVDB_OK;
+    PVDB_CHECK_ARG(rays_o && rays_d && viewdirs, "null pointer");
+    PVDB_CHECK_ARG(K[0] != 0.0 && K[4] != 0.0, "focal lengths must be non-zero");
+    ViewParams V;
+    V.H = H;
+    V.W = W;
+    V.fx = (float)K[0];
+    V.fy = (float)K[4];
+    V.cx = (float)K[2];
+    V.cy = (float)K[5];
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) V.R[3 * r + c] = c2w[4 * r + c];
+        V.o[r] = c2w[4 * r + 3];
+    }
+    V.ndc = ndc != 0;
+    V.inverse_y = inverse_y != 0;
+    V.flip_x = flip_x != 0;
+    V.flip_y = flip_y != 0;
+    V.off = mode == 0 ? 0.5f : 0.f;
+    const double focal = K[0], near_ = 1.0;  // ndc_rays(H, W, K[0][0], 1., ...)
+    V.near_ = (float)near_;
+    V.ax = (float)(-1. / ((double)W / (2. * focal)));
+    V.ay = (float)(-1. / ((double)H / (2. * focal)));
+    V.two_near = (float)(2. * near_);
+    V.neg_two_near = (float)(-2. * near_);
+    const int64_t n = (int64_t)H * W;
+    hipLaunchKernelGGL(k_rays_of_view, dim3(pvdb_blocks(n, 256)), dim3(256), 0,
+                       (hipStream_t)stream, V, rays_o, rays_d, viewdirs);
+    PVDB_LAUNCHED("k_rays_of_view");
+    return PVDB_OK;
+}
 
 int pvdb_ray_bounds(const float* rays_o, const float* rays_d, int64_t n, const float* xyz_min,
                     const float* xyz_max, float near_, float far_, const float* far_per_ray,

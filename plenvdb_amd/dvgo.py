@@ -13,6 +13,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 import torch.nn as nn
+import torch.nn.functional as F
 
 from . import render_utils
 from .grid import MaskGrid, VDBGrid
@@ -132,6 +133,88 @@ class DirectVoxGO(nn.Module):
             far_per_ray=far_per_ray)
         inb = ~mask_outbbox
         return ray_pts[inb], ray_id[inb], step_id[inb]
+
+    def _grid_xyz(self, shape):
+        """World positions of a grid of `shape` points spanning the bbox
+        (torch.meshgrid of three linspaces, dvgo.py:121-125)."""
+        ls = [torch.linspace(float(self.xyz_min[a]), float(self.xyz_max[a]), int(shape[a]),
+                             device=self.xyz_min.device) for a in range(3)]
+        return torch.stack(torch.meshgrid(*ls, indexing="ij"), -1)
+
+    def _dense_mask(self) -> torch.Tensor:
+        """The occupancy as a dense bool grid (a topology MaskGrid's active voxels)."""
+        mc = self.mask_cache
+        if mc.topology is None:
+            return mc.mask
+        return torch.from_numpy(mc.topology.dense_active(self.world_size.tolist())).to(
+            self.xyz_min.device)
+
+    @torch.no_grad()
+    def hit_coarse_geo(self, rays_o, rays_d, near, far, stepsize, **render_kwargs):
+        """Whether each ray has a sample inside the bbox that the occupancy keeps
+        (dvgo.py:258-270; far is forced to 1e9 as there)."""
+        shape = rays_o.shape[:-1]
+        rays_o = rays_o.reshape(-1, 3).contiguous()
+        rays_d = rays_d.reshape(-1, 3).contiguous()
+        stepdist = stepsize * self.voxel_size
+        ray_pts, mask_outbbox, ray_id, *_ = render_utils.sample_pts_on_rays(
+            rays_o, rays_d, self.xyz_min, self.xyz_max, near, 1e9, stepdist)
+        inb = ~mask_outbbox
+        hit = torch.zeros(len(rays_o), dtype=torch.bool, device=rays_o.device)
+        hit[ray_id[inb][self.mask_cache(ray_pts[inb])]] = True
+        return hit.reshape(shape)
+
+    def _set_grid_resolution(self, num_voxels, num_voxels_base=None):
+        """dvgo.py:133-142: voxel size from the voxel budget, world size and the
+        step-interval ratio against the base voxel size."""
+        self.num_voxels = int(num_voxels)
+        vol = (self.xyz_max - self.xyz_min).prod()
+        self.voxel_size = float((vol / num_voxels).pow(1 / 3))
+        self.world_size = ((self.xyz_max - self.xyz_min) / self.voxel_size).long().cpu()
+        if num_voxels_base is not None:
+            self.voxel_size_base = float((vol / num_voxels_base).pow(1 / 3))
+        base = getattr(self, "voxel_size_base", self.voxel_size)
+        self.voxel_size_ratio = self.voxel_size / base
+
+    @torch.no_grad()
+    def maskout_near_cam_vox(self, cam_o, near_clip):
+        """Density of the grid points within near_clip of any camera centre <- -100
+        (dvgo.py:164-175)."""
+        xyz = self._grid_xyz(self.world_size.tolist())
+        cam_o = torch.as_tensor(cam_o, dtype=torch.float32, device=xyz.device)
+        nearest = torch.stack([
+            (xyz.unsqueeze(-2) - co).pow(2).sum(-1).sqrt().amin(-1) for co in cam_o.split(100)
+        ]).amin(0)
+        self.density.setValuesOn_bymask(nearest <= near_clip, -100)
+
+    @torch.no_grad()
+    def update_occupancy_cache(self):
+        """mask &= maxpool3(alpha(density at the cache grid points)) > fast_color_thres
+        (dvgo.py:201-210).  A topology-backed cache becomes the dense mask of its
+        active voxels first, as the reference keeps it."""
+        mask = self._dense_mask()
+        xyz = self._grid_xyz(mask.shape)
+        alpha = self.activate_density(self.density(xyz))[None, None]
+        alpha = F.max_pool3d(alpha, kernel_size=3, padding=1, stride=1)[0, 0]
+        self.mask_cache = MaskGrid(mask=mask & (alpha > self.fast_color_thres),
+                                   xyz_min=self.xyz_min, xyz_max=self.xyz_max)
+
+    @torch.no_grad()
+    def scale_volume_grid(self, num_voxels):
+        """dvgo.py:178-198: new resolution, both grids resampled (VDBGrid
+        scale_volume_grid), and for grids up to 256^3 a dense occupancy cache
+        from the old cache and the max-pooled alpha."""
+        self._set_grid_resolution(num_voxels)
+        old_mask = self.mask_cache
+        self.density.scale_volume_grid(self.world_size)
+        self.k0.scale_volume_grid(self.world_size)
+        if int(torch.prod(self.world_size)) <= 256 ** 3:
+            xyz = self._grid_xyz(self.world_size.tolist())
+            alpha = F.max_pool3d(self.activate_density(self.density.get_dense_grid()),
+                                 kernel_size=3, padding=1, stride=1)[0, 0]
+            self.mask_cache = MaskGrid(mask=old_mask(xyz) & (alpha > self.fast_color_thres),
+                                       xyz_min=self.xyz_min, xyz_max=self.xyz_max)
+        self.topology = None
 
     def activate_density(self, density, interval=None):
         interval = interval if interval is not None else self.voxel_size_ratio

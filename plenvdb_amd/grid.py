@@ -120,8 +120,8 @@ class MaskGrid(nn.Module):
 
     def __init__(self, mask=None, xyz_min=None, xyz_max=None, topology: Topology | None = None):
         super().__init__()
-        xyz_min = torch.as_tensor(xyz_min, dtype=torch.float32)
-        xyz_max = torch.as_tensor(xyz_max, dtype=torch.float32)
+        xyz_min = torch.as_tensor(xyz_min, dtype=torch.float32).cpu()
+        xyz_max = torch.as_tensor(xyz_max, dtype=torch.float32).cpu()
         self.topology = topology
         if topology is not None:
             shape = torch.tensor(topology.reso, dtype=torch.float32)

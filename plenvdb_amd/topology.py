@@ -94,6 +94,21 @@ class Topology:
         bits = np.unpackbits(m.view(np.uint8).reshape(-1, 64), axis=1, bitorder="little")
         return bits.reshape(-1).astype(bool)
 
+    def dense_active(self, reso=None) -> np.ndarray:
+        """Dense bool [X, Y, Z] grid of the active voxels inside [0, reso)."""
+        reso = [int(r) for r in (reso if reso is not None else self.reso)]
+        o, _ = self.leaves()
+        act = self.active_slot_mask().reshape(-1, 512)
+        v = np.arange(512)
+        local = np.stack([v >> 6, (v >> 3) & 7, v & 7], 1)
+        out = np.zeros(reso, dtype=bool)
+        li, vi = np.nonzero(act)
+        c = o[li] + local[vi]
+        ok = np.all((c >= 0) & (c < np.asarray(reso)), axis=1)
+        c = c[ok]
+        out[c[:, 0], c[:, 1], c[:, 2]] = True
+        return out
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

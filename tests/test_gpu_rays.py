@@ -1,0 +1,201 @@
+"""GPU parity of the ray front end and the occupancy maintenance of
+plenvdb/lib/dvgo.py: camera rays (get_rays_of_a_view, HIP kernel) bit-exact
+against the oracle's float32 restatement; hit_coarse_geo and the in-maskcache
+training-ray builder exactly (integer work); update_occupancy_cache and
+maskout_near_cam_vox against numpy recomputations, exact away from their
+thresholds (the grid positions come from torch.linspace, so a value within
+1e-5 of a threshold may fall either way)."""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _scene_model(oracle_mod, name="c2"):
+    from plenvdb_amd import scenes
+    from plenvdb_amd.trainer import PlainTrainer, init_values
+
+    sc = scenes.make_scene(name, n_rays=64)
+    tr = PlainTrainer(sc)
+    den, col = init_values(sc, tr.topo)
+    origins, vm = oracle_mod.leaves_from_mask(sc["mask"])
+    dom = oracle_mod.SlotDomain(origins, vm)
+    return sc, tr.model, den, dom
+
+
+@pytest.mark.parametrize("ndc,inverse_y,flip_x,flip_y,mode", [
+    (False, False, False, False, "center"), (False, True, False, False, "center"),
+    (False, False, True, True, "lefttop"), (True, False, False, False, "center"),
+    (True, True, True, False, "lefttop")])
+def test_rays_of_view_bit_exact(oracle_mod, ndc, inverse_y, flip_x, flip_y, mode):
+    from plenvdb_amd import rays, scenes
+
+    print("[case] rays_of_view", ndc, inverse_y, flip_x, flip_y, mode, flush=True)
+    H, W = 37, 61
+    K = np.array([[71.3, 0, 30.2], [0, 69.9, 18.6], [0, 0, 1]])
+    for c2w in scenes.orbit_poses(3):
+        ro, rd, vd = rays.get_rays_of_a_view(H, W, K, torch.from_numpy(c2w), ndc, inverse_y,
+                                             flip_x, flip_y, mode=mode)
+        eo, ed, ev = oracle_mod.rays_of_view(H, W, K, c2w, ndc, inverse_y, flip_x, flip_y, mode)
+        np.testing.assert_array_equal(ro.cpu().numpy(), eo)
+        np.testing.assert_array_equal(rd.cpu().numpy(), ed)
+        np.testing.assert_array_equal(vd.cpu().numpy(), ev)
+
+
+def test_rays_of_view_edges():
+    from plenvdb_amd import rays
+
+    K = np.array([[50.0, 0, 4], [0, 50.0, 4], [0, 0, 1]])
+    ro, rd, vd = rays.get_rays_of_a_view(0, 8, K, np.eye(4, dtype=np.float32), False, False,
+                                         False, False)
+    assert ro.shape == (0, 8, 3)
+    with pytest.raises(NotImplementedError):
+        rays.get_rays_of_a_view(4, 4, K, np.eye(4), False, False, False, False, mode="random")
+
+
+def _oracle_hits(oracle_mod, sc, dom, ro, rd):
+    mn = np.asarray(sc["xyz_min"], np.float32)
+    mx = np.asarray(sc["xyz_max"], np.float32)
+    reso = np.asarray(sc["reso"], np.int32)
+    stepdist = np.float32(sc["stepsize"]) * np.float32(sc["voxel_size"])
+    tmin, _, ns = oracle_mod.ray_bounds(ro, rd, mn, mx, sc["near"], 1e9, stepdist, prec="f32")
+    pts, rid, _, inb = oracle_mod.sample_pts(ro, rd, tmin, ns, stepdist, mn, mx, prec="f32")
+    occ = oracle_mod.dense_active_mask(dom, reso)
+    scale = ((reso.astype(np.float32) - np.float32(1)) / (mx - mn)).astype(np.float32)
+    shift = (-mn * scale).astype(np.float32)
+    keep = oracle_mod.mask_lookup(occ, pts[inb], scale, shift, prec="f32")
+    hit = np.zeros(len(ro), bool)
+    hit[rid[inb][keep]] = True
+    return hit
+
+
+def test_hit_coarse_geo_and_maskcache_rays(oracle_mod):
+    from plenvdb_amd import rays, scenes
+
+    print("[case] hit_coarse_geo c2", flush=True)
+    sc, model, _, dom = _scene_model(oracle_mod)
+    H, W = 48, 40
+    K = scenes.intrinsics(H, W, 60.0)
+    poses = scenes.orbit_poses(2)
+    kw = dict(near=sc["near"], far=6.0, stepsize=sc["stepsize"])
+    imgs = [torch.rand(H, W, 3, device="cuda") for _ in poses]
+    rgb, ro_t, rd_t, vd_t, imsz = rays.get_training_rays_in_maskcache_sampling(
+        imgs, torch.from_numpy(poses), [(H, W)] * 2, [K] * 2, False, False, False, False,
+        model, kw)
+    top = 0
+    for v, c2w in enumerate(poses):
+        eo, ed, ev = oracle_mod.rays_of_view(H, W, K, c2w)
+        hit = _oracle_hits(oracle_mod, sc, dom, eo.reshape(-1, 3), ed.reshape(-1, 3))
+        got = model.hit_coarse_geo(torch.from_numpy(eo).cuda(), torch.from_numpy(ed).cuda(), **kw)
+        np.testing.assert_array_equal(got.cpu().numpy().reshape(-1), hit)
+        n = int(hit.sum())
+        print(f"[result] view {v}: {n} of {H * W} rays hit", flush=True)
+        assert 0 < n < H * W and imsz[v] == n
+        np.testing.assert_array_equal(rd_t[top:top + n].cpu().numpy(), ed.reshape(-1, 3)[hit])
+        np.testing.assert_array_equal(vd_t[top:top + n].cpu().numpy(), ev.reshape(-1, 3)[hit])
+        np.testing.assert_array_equal(rgb[top:top + n].cpu().numpy(),
+                                      imgs[v].cpu().numpy().reshape(-1, 3)[hit])
+        top += n
+    assert rgb.shape[0] == top
+    flat = rays.get_training_rays_flatten(imgs, poses, [(H, W)] * 2, [K] * 2, False, False,
+                                          False, False)
+    assert flat[1].shape == (2 * H * W, 3) and flat[4] == [H * W] * 2
+
+
+def _grid_xyz(sc, shape):
+    ls = [np.linspace(sc["xyz_min"][a], sc["xyz_max"][a], shape[a]).astype(np.float32)
+          for a in range(3)]
+    return np.stack(np.meshgrid(*ls, indexing="ij"), -1)
+
+
+def test_update_occupancy_cache(oracle_mod):
+    from scipy.ndimage import maximum_filter
+
+    print("[case] update_occupancy_cache c2", flush=True)
+    sc, model, den, dom = _scene_model(oracle_mod)
+    reso = sc["reso"]
+    xyz = _grid_xyz(sc, reso).reshape(-1, 3)
+    mn, mx = np.asarray(sc["xyz_min"], np.float64), np.asarray(sc["xyz_max"], np.float64)
+    idx = (xyz - mn) / (mx - mn) * (np.asarray(reso) - 1)
+    sigma = oracle_mod.trilinear_fwd(dom, den.astype(np.float64), idx[:, 0], idx[:, 1], idx[:, 2],
+                                     0, "f64")[:, 0]
+    _, alpha = oracle_mod.raw2alpha(sigma, model.act_shift, model.voxel_size_ratio, "f64")
+    pooled = maximum_filter(alpha.reshape(reso), size=3, mode="constant", cval=-np.inf)
+    active = oracle_mod.dense_active_mask(dom, reso)
+    thres = float(np.median(pooled[active]))
+    model.fast_color_thres = thres
+    model.update_occupancy_cache()
+    got = model.mask_cache.mask.cpu().numpy()
+    want = active & (pooled > thres)
+    near = np.abs(pooled - thres) < 1e-5
+    print(f"[result] kept {int(got.sum())} of {int(active.sum())}; "
+          f"{int((got != want).sum())} differ, {int(near.sum())} within 1e-5", flush=True)
+    assert np.all(got[~near] == want[~near])
+    assert 0 < got.sum() < active.sum()
+
+
+def test_maskout_near_cam_vox(oracle_mod):
+    from plenvdb_amd import scenes
+
+    print("[case] maskout_near_cam_vox c2", flush=True)
+    sc, model, den, dom = _scene_model(oracle_mod)
+    reso = sc["reso"]
+    cams = scenes.orbit_poses(6)[:, :3, 3]
+    clip = 2.9
+    model.maskout_near_cam_vox(cams, clip)
+    xyz = _grid_xyz(sc, reso).astype(np.float64)
+    d = np.min(np.stack([np.linalg.norm(xyz - c, axis=-1) for c in cams.astype(np.float64)]), 0)
+    coords = dom.coords()
+    ok = np.all((coords >= 0) & (coords < np.asarray(reso)), axis=1)
+    inside = np.zeros(len(coords), bool)
+    inside[ok] = d[coords[ok, 0], coords[ok, 1], coords[ok, 2]] <= clip
+    act = dom.active()
+    want = np.where(act & inside, -100.0, den).astype(np.float32)
+    got = model.density.grid.grid.detach().cpu().numpy()[:, 0]
+    dist = np.full(len(coords), np.inf)
+    dist[ok] = d[coords[ok, 0], coords[ok, 1], coords[ok, 2]]
+    clear = np.abs(dist - clip) > 1e-5
+    print(f"[result] {int((got == -100).sum())} voxels masked", flush=True)
+    assert (got == -100).sum() > 0
+    np.testing.assert_array_equal(got[clear], want[clear])
+
+
+def test_scale_volume_grid(oracle_mod):
+    """dvgo.py:178-198: the resampled density equals the trilinear
+    (align_corners=True) resize of the old dense grid, computed on the CPU; the
+    new occupancy cache is old-cache(points) & maxpool(alpha) > thres away from
+    the threshold; a forward pass runs at the new resolution."""
+    import torch.nn.functional as F
+
+    print("[case] scale_volume_grid c2 -> 96^3 voxels", flush=True)
+    sc, model, den, dom = _scene_model(oracle_mod)
+    model.fast_color_thres = 1e-3
+    old_dense = model.density.get_dense_grid().cpu()
+    old_mask = oracle_mod.dense_active_mask(dom, sc["reso"])
+    model.scale_volume_grid(96 ** 3)
+    ws = model.world_size.tolist()
+    vs = float(np.prod(np.asarray(sc["xyz_max"]) - np.asarray(sc["xyz_min"])) / 96 ** 3) ** (1 / 3)
+    print(f"[result] world size {ws}, voxel size {model.voxel_size:.6f}", flush=True)
+    assert abs(model.voxel_size - vs) < 1e-6 * vs and all(abs(w - 96) <= 1 for w in ws)
+    want = F.interpolate(old_dense, size=tuple(ws), mode="trilinear", align_corners=True)
+    got = model.density.get_dense_grid().cpu()
+    np.testing.assert_allclose(got.numpy(), want.numpy(), rtol=1e-5, atol=1e-5)
+    alpha = model.activate_density(want.cuda()).cpu().numpy()[0, 0]
+    from scipy.ndimage import maximum_filter
+
+    pooled = maximum_filter(alpha, size=3, mode="constant", cval=-np.inf)
+    xyz = _grid_xyz(sc, ws).reshape(-1, 3)
+    reso = np.asarray(sc["reso"], np.float32)
+    mn, mx = np.asarray(sc["xyz_min"], np.float32), np.asarray(sc["xyz_max"], np.float32)
+    scale = ((reso - np.float32(1)) / (mx - mn)).astype(np.float32)
+    shift = (-mn * scale).astype(np.float32)
+    inside = oracle_mod.mask_lookup(old_mask, xyz, scale, shift, prec="f32").reshape(ws)
+    exp = inside & (pooled > 1e-3)
+    m = model.mask_cache.mask.cpu().numpy()
+    clear = np.abs(pooled - 1e-3) > 1e-5
+    assert np.all(m[clear] == exp[clear]) and 0 < m.sum() < m.size
+    out = model(torch.from_numpy(sc["rays_o"]).cuda(), torch.from_numpy(sc["rays_d"]).cuda(),
+                torch.from_numpy(sc["viewdirs"]).cuda(), near=sc["near"], far=1e9,
+                stepsize=sc["stepsize"], bg=sc["bg"])
+    assert torch.isfinite(out["rgb_marched"]).all()

@@ -296,6 +296,61 @@ def test_reference_step_loss_extras(oracle_mod):
         fd_check(sc_b, "color", (s_, c), gc[s_, c])
 
 
+def _rotation(seed):
+    q, _ = np.linalg.qr(np.random.default_rng(seed).normal(size=(3, 3)))
+    return q * np.sign(np.linalg.det(q))
+
+
+def test_rays_of_view_closed_forms(oracle_mod):
+    """get_rays / ndc_rays (dvgo.py:471-527) in the oracle: the identity pose gives
+    the pinhole directions; a rotation pose rotates them (FP64 product); flips
+    reverse an axis; inverse_y negates y and z; viewdirs are unit vectors; NDC
+    origins lie on z = -1, origin + direction on z = 1, and the NDC image of any
+    point of the camera ray lies on the NDC ray."""
+    H, W = 37, 53
+    K = np.array([[120.5, 0, 26.25], [0, 118.0, 17.75], [0, 0, 1]])
+    eye = np.eye(4)[:3]
+    ro, rd, vd = oracle_mod.rays_of_view(H, W, K, eye, mode="center")
+    jj, ii = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    np.testing.assert_allclose(rd[..., 0], (ii + 0.5 - 26.25) / 120.5, rtol=1e-6)
+    np.testing.assert_allclose(rd[..., 1], -(jj + 0.5 - 17.75) / 118.0, rtol=1e-6, atol=1e-7)
+    assert np.all(rd[..., 2] == -1) and np.all(ro == 0)
+    lt = oracle_mod.rays_of_view(H, W, K, eye, mode="lefttop")[1]
+    np.testing.assert_allclose(lt[..., 0], (ii - 26.25) / 120.5, rtol=1e-6)
+    fx = oracle_mod.rays_of_view(H, W, K, eye, flip_x=True)[1]
+    np.testing.assert_array_equal(fx[..., 0], rd[:, ::-1, 0])
+    fy = oracle_mod.rays_of_view(H, W, K, eye, flip_y=True)[1]
+    np.testing.assert_array_equal(fy[..., 1], rd[::-1, :, 1])
+    iy = oracle_mod.rays_of_view(H, W, K, eye, inverse_y=True)[1]
+    np.testing.assert_array_equal(iy[..., 1], -rd[..., 1])
+    assert np.all(iy[..., 2] == 1)
+    R = _rotation(3)
+    c2w = np.concatenate([R, [[0.3], [-1.2], [2.5]]], 1)
+    ro, rdR, vdR = oracle_mod.rays_of_view(H, W, K, c2w)
+    np.testing.assert_allclose(rdR, np.einsum("km,hwm->hwk", R, rd.astype(np.float64)),
+                               rtol=0, atol=1e-6)
+    np.testing.assert_allclose(np.linalg.norm(vdR, axis=-1), 1.0, atol=1e-6)
+    np.testing.assert_array_equal(ro, np.broadcast_to(np.float32(c2w[:, 3]), ro.shape))
+    # NDC: camera looking down -z from z = 3
+    c2w = np.concatenate([np.eye(3), [[0.1], [0.2], [3.0]]], 1)
+    rd_w = oracle_mod.rays_of_view(H, W, K, c2w)[1].astype(np.float64)
+    o_n, d_n, _ = oracle_mod.rays_of_view(H, W, K, c2w, ndc=True)
+    np.testing.assert_allclose(o_n[..., 2], -1.0, atol=1e-6)
+    np.testing.assert_allclose(o_n[..., 2] + d_n[..., 2], 1.0, atol=1e-6)
+    f = K[0][0]
+
+    def proj(p):
+        return np.stack([-(2 * f / W) * p[..., 0] / p[..., 2], -(2 * f / H) * p[..., 1] / p[..., 2],
+                         1 + 2 * 1.0 / p[..., 2]], -1)
+
+    o_w = np.broadcast_to(c2w[:, 3], rd_w.shape)
+    t = -(1.0 + o_w[..., 2]) / rd_w[..., 2]
+    for s_ in (0.5, 4.0):
+        q = proj(o_w + (t + s_)[..., None] * rd_w)  # a point beyond the near plane
+        lam = (q[..., 2] - o_n[..., 2]) / d_n[..., 2]
+        np.testing.assert_allclose(q, o_n + lam[..., None] * d_n, rtol=0, atol=1e-5)
+
+
 def _pe27(vd):
     """Per-pixel PE of the renderer in the order get_rays writes it
     (renderer.cu:153-165): v, then sin(v_x 2^k), sin(v_y 2^k), sin(v_z 2^k),
