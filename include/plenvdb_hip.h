@@ -109,6 +109,31 @@ int pvdb_dense_adam(float* param, const float* grad, float* exp_avg, float* exp_
                     const float* per_lr, int64_t n, int32_t mode, float step_size, float eps,
                     float beta0, float beta1, void* stream);
 
+/* ---- FP64 twins of the hand-written adjoint pairs ------------------------
+ * Same formulas as pvdb_trilinear_fwd/_bwd (densityvdb.cu:101-181,
+ * colorvdb.cu:81-175; channels 1, 3 or 12, slot-major), pvdb_raw2alpha/_bwd
+ * (render_utils_kernel.cu:430-443,506-517) and pvdb_alpha2weight/_bwd
+ * (:576-603,653-676) with every quantity in double, for central
+ * finite-difference checks of the adjoints on the device.  No counterpart in
+ * the reference, which checks none of its adjoints. */
+int pvdb_trilinear_fwd_f64(const pvdb_topology* topo, const double* vals, int32_t channels,
+                           const double* xs, const double* ys, const double* zs, int64_t n,
+                           double* out, void* stream);
+int pvdb_trilinear_bwd_f64(const pvdb_topology* topo, double* grad, int32_t channels,
+                           const double* xs, const double* ys, const double* zs, const double* g,
+                           int64_t n, void* stream);
+int pvdb_raw2alpha_f64(const double* sigma, int64_t n, double shift, double interval,
+                       double* exp_out, double* alpha_out, void* stream);
+int pvdb_raw2alpha_bwd_f64(const double* exp_d, const double* grad_alpha, int64_t n,
+                           double interval, double* grad_sigma, void* stream);
+int pvdb_alpha2weight_f64(const double* alpha, int64_t n_rays, double* weight, double* T,
+                          double* alphainv_last, const int64_t* i_start, int64_t* i_end,
+                          void* stream);
+int pvdb_alpha2weight_bwd_f64(const double* alpha, const double* weight, const double* T,
+                              const double* alphainv_last, const int64_t* i_start,
+                              const int64_t* i_end, int64_t n_rays, const double* grad_weights,
+                              const double* grad_last, double* grad_alpha, void* stream);
+
 /* ---- ray operations (plenvdb/lib/cuda/render_utils_kernel.cu) ------------ */
 /* get_rays_of_a_view (plenvdb/lib/dvgo.py:522-527) with get_rays (:471-499) and
  * ndc_rays (:502-519, near = 1, focal = K[0][0]): rays_o, rays_d and viewdirs

@@ -21,6 +21,7 @@ P = C.c_void_p
 I32 = C.c_int32
 I64 = C.c_int64
 F32 = C.c_float
+F64 = C.c_double
 
 # name -> argtypes (every function returns int32 status)
 _SIGS = {
@@ -52,6 +53,12 @@ _SIGS = {
     "pvdb_alpha2weight": [P, I64, P, P, P, P, P, P],
     "pvdb_alpha2weight_bwd": [P, P, P, P, P, P, I64, P, P, P, P],
     "pvdb_mfma_selftest": [P, P, P, P, P, P, P],
+    "pvdb_trilinear_fwd_f64": [P, P, I32, P, P, P, I64, P, P],
+    "pvdb_trilinear_bwd_f64": [P, P, I32, P, P, P, P, I64, P],
+    "pvdb_raw2alpha_f64": [P, I64, F64, F64, P, P, P],
+    "pvdb_raw2alpha_bwd_f64": [P, P, I64, F64, P, P],
+    "pvdb_alpha2weight_f64": [P, I64, P, P, P, P, P, P],
+    "pvdb_alpha2weight_bwd_f64": [P, P, P, P, P, P, I64, P, P, P, P],
 }
 
 # functions with extra declarations added by later modules register here
