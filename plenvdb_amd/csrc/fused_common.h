@@ -56,6 +56,8 @@ struct Records {
     float4* rgb;       // MLP colour (rgb), written by the forward MLP pass
     float4* adj;       // per-sample adjoints: dL/dlogit (3), dL/dsigma
     uint4* feat;       // [3 per record] the 12 gathered colour features as bf16 hi, then lo
+    uint32_t* relu;    // [8 per record] forward ReLU masks: word 4l + j = layer l's
+                       // hidden features 32j..32j+31, bit f%32
     float4* grad;      // [4 per record] dL/dfeature (12) and dL/dsigma
 };
 

@@ -127,6 +127,18 @@ __device__ __forceinline__ void st_act(uint16_t* sh, uint16_t* sl, uint16_t* fm,
     }
 }
 
+// The hi parts of four consecutive features fr..fr+3 of sample `col`:
+// sample-major into sh and feature-major into fm.
+__device__ __forceinline__ void st_hi_fm(uint16_t* sh, uint16_t* fm, int col, int fr, int lane,
+                                         const float* v) {
+    uint2 hi, lo;
+    split2(v[0], v[1], hi.x, lo.x);
+    split2(v[2], v[3], hi.y, lo.y);
+    *reinterpret_cast<uint2*>(&sh[col * SM + fr]) = hi;
+    const int q = lane & 3;
+    *reinterpret_cast<uint2*>(&fm[(fr + q) * FM + (col & ~3)]) = quad_transpose(hi, q);
+}
+
 // The hi parts of four consecutive features fr..fr+3 of sample `col`,
 // feature-major into fm (the feature-major half of st_act<true>).
 __device__ __forceinline__ void st_fm(uint16_t* fm, int col, int fr, int lane, const float* v) {
@@ -137,6 +149,16 @@ __device__ __forceinline__ void st_fm(uint16_t* fm, int col, int fr, int lane, c
     *reinterpret_cast<uint2*>(&fm[(fr + q) * FM + (col & ~3)]) = quad_transpose(hi, q);
 }
 
+// A wave's 32-bit slice of one sample's ReLU mask from the D fragments of its
+// two 16-feature tiles: lane (g, n) holds bit i of nib[mt] for feature
+// 16 mt + 4g + i; the four lanes of column n are OR-ed together.
+__device__ __forceinline__ uint32_t relu_word(uint32_t nib0, uint32_t nib1, int g) {
+    uint32_t w = (nib0 << (4 * g)) | (nib1 << (16 + 4 * g));
+    w |= (uint32_t)__shfl_xor((int)w, 16, 64);
+    w |= (uint32_t)__shfl_xor((int)w, 32, 64);
+    return w;
+}
+
 // a (bf16 weight) times a split activation
 __device__ __forceinline__ f32x4 mfma_w(const bf16x8& a, const bf16x8& bh, const bf16x8& bl,
                                         f32x4 c) {
@@ -144,9 +166,10 @@ __device__ __forceinline__ f32x4 mfma_w(const bf16x8& a, const bf16x8& bh, const
     return mfma16x16x32(a, bl, c);
 }
 
-// dW products (activation x gradient) use the hi parts only: the sums over a
-// step's samples average that rounding (a device check keeps every MLP
-// gradient within 5e-4 of FP64), so the gradients' lo parts stay sample-major.
+// dW products (activation x gradient) use the hi parts only, and the backward
+// recomputes the activations they take from hi parts: the sums over a step's
+// samples average that rounding (a device check keeps every MLP gradient
+// within 6e-4 of FP64), so the gradients' lo parts stay sample-major.
 
 // ---- forward -----------------------------------------------------------------
 // A workgroup of 4 wavefronts takes a group of 32 records; wave w owns hidden
@@ -158,8 +181,8 @@ __device__ __forceinline__ f32x4 mfma_w(const bf16x8& a, const bf16x8& bh, const
 //   C  H1, then the next group's colour loads
 //   D  logits -> rgb
 // The records are staged two groups ahead (flags and positions double-buffered
-// in LDS); W0^T and W1^T slices stay in registers, W2^T in LDS.  The arithmetic of X, H0 and H1 is the backward kernel's, so its
-// recomputed ReLU masks are the ones this pass applied.
+// in LDS); W0^T and W1^T slices stay in registers, W2^T in LDS.  The ReLU
+// masks of both hidden layers are stored per record for the backward.
 struct RecInfo {
     float x, y, z;
     uint32_t bits;
@@ -176,7 +199,7 @@ struct alignas(16) SmemFwd {
 __device__ __forceinline__ float sigmoidf_(float x) { return 1.f / (1.f + expf(-x)); }
 
 template <bool BF16COL>
-__global__ void __launch_bounds__(256) k_mlp_fwd(StepParams P, TopoView topo, const void* color,
+__global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo, const void* color,
                                                  MlpPacked W, RayBuffers R, Records rec) {
     constexpr int MT = 2;  // 16-feature tiles per wave
     using Raw = std::conditional_t<BF16COL, uint2, float4>;  // one third of a colour row
@@ -403,15 +426,26 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(StepParams P, TopoView topo, co
                 for (int nt = 0; nt < NT; ++nt)
                     acc[mt][nt] = mfma_w(A0[mt][kt], bh[nt], bl[nt], acc[mt][nt]);
         }
+        uint32_t nib[MT][NT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
                 float h[4];
+                nib[mt][nt] = 0u;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) h[i] = fmaxf(acc[mt][nt][i] + b0v[mt][i], 0.f);
+                for (int i = 0; i < 4; ++i) {
+                    h[i] = fmaxf(acc[mt][nt][i] + b0v[mt][i], 0.f);
+                    nib[mt][nt] |= (h[i] > 0.f ? 1u : 0u) << i;
+                }
                 st_act<false>(sm.A1h, sm.A1l, nullptr, 16 * nt + r16, f0 + 16 * mt + 4 * g, lane, h);
             }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const uint32_t w = relu_word(nib[0][nt], nib[1][nt], g);
+            const int64_t q = base + 16 * nt + r16;
+            if (g == 0 && q < total) rec.relu[8 * q + wv] = w;
+        }
         lookup(cb ^ 1);
         __syncthreads();
         // ---- C. H1^T slice = relu(W1^T H0^T + b1) -> A2; next group's colour loads ----
@@ -439,10 +473,20 @@ __global__ void __launch_bounds__(256) k_mlp_fwd(StepParams P, TopoView topo, co
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
                 float h[4];
+                nib[mt][nt] = 0u;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) h[i] = fmaxf(acc[mt][nt][i] + b1v[mt][i], 0.f);
+                for (int i = 0; i < 4; ++i) {
+                    h[i] = fmaxf(acc[mt][nt][i] + b1v[mt][i], 0.f);
+                    nib[mt][nt] |= (h[i] > 0.f ? 1u : 0u) << i;
+                }
                 st_act<false>(sm.A2h, sm.A2l, nullptr, 16 * nt + r16, f0 + 16 * mt + 4 * g, lane, h);
             }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const uint32_t w = relu_word(nib[0][nt], nib[1][nt], g);
+            const int64_t q = base + 16 * nt + r16;
+            if (g == 0 && q < total) rec.relu[8 * q + 4 + wv] = w;
+        }
         fetch();
         __syncthreads();
         // ---- D. logits (wave nt takes tile nt): L^T = W2^T H1^T + b2, rgb = sigmoid ----
@@ -481,8 +525,11 @@ constexpr int kBw2Frags = 8 * 64, kBw1Frags = 8 * 4 * 64, kBw0Frags = 4 * 64;
 // One workgroup of 8 wavefronts per CU (two per SIMD); wave w owns hidden
 // features [16w, 16w + 16) of both hidden layers.  Per group of 32 records the
 // phases are (a barrier after each):
-//   3  H0^T = relu(W0^T X^T + b0) (recomputed; ReLU mask m0)   -> A1, H0f
-//   4  H1^T = relu(W1^T H0^T + b1) (ReLU mask m1)                -> F2
+//   3  H0^T = W0^T X^T + b0 under the forward's ReLU mask m0    -> A1h, H0f
+//   4  H1^T = W1^T H0^T + b1 under the forward's mask m1         -> F2
+// H0 and H1 are recomputed for the dW operands only (from hi parts, one MFMA
+// per product): the masks come from the forward pass (per-record words), so the
+// backward applies exactly the ReLU pattern the forward applied.
 //   7  g_H1^T = W2 g_L^T masked by m1 -> A1, G1f; dW2 += H1^T g_L; the next
 //      group's records are staged (park)
 //   8  g_H0^T = W1 g_H1^T masked by m0 -> A2, F2; dW1 += H0^T g_H1; the next
@@ -496,7 +543,8 @@ struct alignas(16) SmemBwd {
     uint4 Bw1[kBw1Frags];
     uint4 Bw0[kBw0Frags];
     uint2 Bw2[kBw2Frags];
-    uint16_t Xh[2][GS * XM], Xl[2][GS * XM];
+    uint32_t relu[2][GS * 8];  // the forward's ReLU mask words of each record
+    uint16_t Xh[2][GS * XM];
     uint16_t Xf[2][XF_ROWS * FM];
     uint16_t A1h[GS * SM], A1l[GS * SM];
     uint16_t A2h[GS * SM], A2l[GS * SM];
@@ -561,7 +609,6 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
     for (int e = tid; e < 2 * GS * 24; e += NTH) {
         const int b = e / (GS * 24), row = (e / 24) % GS, col = 40 + e % 24;
         sm.Xh[b][row * XM + col] = 0;
-        sm.Xl[b][row * XM + col] = 0;
     }
     for (int e = tid; e < 2 * (XF_ROWS - 40) * FM; e += NTH) {
         const int b = e / ((XF_ROWS - 40) * FM);
@@ -583,7 +630,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
 
     // register prefetch of a group's records on the staging lanes
     float4 p_atwr = z4, p_adj = z4, p_vd = z4;
-    uint4 p_f0 = zu4, p_f1 = zu4, p_f2 = zu4;
+    uint4 p_f0 = zu4, p_f1 = zu4, p_r0 = zu4, p_r1 = zu4;
     auto stage_a = [&](int64_t grp) {  // the records (independent loads)
         const int64_t q = grp * GS + pr;
         p_atwr = p_adj = z4;
@@ -595,26 +642,30 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
     auto stage_b = [&](int64_t grp) {  // view direction and kept features (need the flags)
         const int64_t q = grp * GS + pr;
         p_vd = z4;
-        p_f0 = p_f1 = p_f2 = zu4;
+        p_f0 = p_f1 = p_r0 = p_r1 = zu4;
         if (pr >= 0 && grp < ngroups && q < total) {
             const uint32_t bits = __float_as_uint(p_atwr.w);
             if (bits & kAliveW) {
                 const uint32_t ray = bits & kRayMask;
                 p_vd = make_float4(P.viewdirs[3 * ray], P.viewdirs[3 * ray + 1],
                                    P.viewdirs[3 * ray + 2], 0.f);
-                p_f0 = rec.feat[3 * q];
-                p_f1 = rec.feat[3 * q + 1];
-                p_f2 = rec.feat[3 * q + 2];
+                p_f0 = rec.feat[3 * q];  // hi halves 0..7
+                p_f1 = rec.feat[3 * q + 1];  // hi halves 8..11 (.x, .y)
+                p_r0 = reinterpret_cast<const uint4*>(rec.relu)[2 * q];  // ReLU masks
+                p_r1 = reinterpret_cast<const uint4*>(rec.relu)[2 * q + 1];
             }
         }
     };
     // staged records -> buffer b: flags, adjoints (dL/dlogit as (hi, lo) pairs,
-    // dL/dsigma), view direction, kept features (X columns 0..11)
+    // dL/dsigma), view direction, ReLU masks, the kept features' hi parts (X
+    // columns 0..11)
     auto park = [&](int b) {
         if (pr >= 0) {
             sm.bits[b][pr] = __float_as_uint(p_atwr.w);  // zero for records past the end
             sm.gsig[b][pr] = p_adj.w;
             sm.vd[b][pr] = p_vd;
+            reinterpret_cast<uint4*>(&sm.relu[b][8 * pr])[0] = p_r0;
+            reinterpret_cast<uint4*>(&sm.relu[b][8 * pr])[1] = p_r1;
             const float gl[3] = {p_adj.x, p_adj.y, p_adj.z};
             db2[0] += gl[0];
             db2[1] += gl[1];
@@ -627,16 +678,13 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
                 sm.GLl[b][pr * GM + o] = l;
                 sm.GLf[b][o * FM + pr] = h;
             }
-            const uint32_t w12[12] = {p_f0.x, p_f0.y, p_f0.z, p_f0.w, p_f1.x, p_f1.y,
-                                      p_f1.z, p_f1.w, p_f2.x, p_f2.y, p_f2.z, p_f2.w};
+            const uint32_t w6[6] = {p_f0.x, p_f0.y, p_f0.z, p_f0.w, p_f1.x, p_f1.y};
             uint32_t* xh = reinterpret_cast<uint32_t*>(&sm.Xh[b][pr * XM]);
-            uint32_t* xl = reinterpret_cast<uint32_t*>(&sm.Xl[b][pr * XM]);
 #pragma unroll
-            for (int c = 0; c < 6; ++c) {  // hi halves 0..11, then lo halves 0..11
-                xh[c] = w12[c];
-                xl[c] = w12[6 + c];
-                sm.Xf[b][(2 * c) * FM + pr] = (uint16_t)(w12[c] & 0xffffu);
-                sm.Xf[b][(2 * c + 1) * FM + pr] = (uint16_t)(w12[c] >> 16);
+            for (int c = 0; c < 6; ++c) {  // hi halves of features 2c, 2c + 1
+                xh[c] = w6[c];
+                sm.Xf[b][(2 * c) * FM + pr] = (uint16_t)(w6[c] & 0xffffu);
+                sm.Xf[b][(2 * c + 1) * FM + pr] = (uint16_t)(w6[c] >> 16);
             }
         }
     };
@@ -673,7 +721,6 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
         const uint2 t = quad_transpose(hi, lane & 3);  // the quad: 4 samples, one part
         if (pp < 7) {
             *reinterpret_cast<uint2*>(&sm.Xh[b][ps * XM + 12 + 4 * pp]) = hi;
-            *reinterpret_cast<uint2*>(&sm.Xl[b][ps * XM + 12 + 4 * pp]) = lo;
             *reinterpret_cast<uint2*>(&sm.Xf[b][(12 + 4 * pp + (lane & 3)) * FM + (ps & ~3)]) = t;
         }
     };
@@ -689,8 +736,17 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
 
     for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {  // uniform bound
         const int64_t base = grp * GS, next = grp + gridDim.x;
-        // ---- 3. H0^T slice = relu(W0^T X^T + b0) -> A1 (and H0f) ----
-        uint32_t m0 = 0u, m1 = 0u;  // ReLU masks: bit nt * 4 + i
+        // ---- 3. H0^T slice = W0^T X^T + b0 under the forward's mask -> A1h, H0f ----
+        uint32_t m0 = 0u, m1 = 0u;  // the forward's ReLU masks: bit nt * 4 + i
+        {
+            const int sh = ((wv & 1) << 4) + 4 * g;  // features fr..fr+3 in word wv/2
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const uint32_t* mw = &sm.relu[cb][8 * (16 * nt + r16)];
+                m0 |= ((mw[wv >> 1] >> sh) & 0xFu) << (nt * 4);
+                m1 |= ((mw[4 + (wv >> 1)] >> sh) & 0xFu) << (nt * 4);
+            }
+        }
         f32x4 acc[NT];
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -699,21 +755,18 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
                 const int xo = (16 * nt + r16) * XM + 32 * kt + 8 * g;
-                acc[nt] = mfma_w(A0[kt], ld_frag(&sm.Xh[cb][xo]), ld_frag(&sm.Xl[cb][xo]), acc[nt]);
+                acc[nt] = mfma16x16x32(A0[kt], ld_frag(&sm.Xh[cb][xo]), acc[nt]);
             }
         }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
             float h[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                h[i] = fmaxf(acc[nt][i] + b0v[i], 0.f);
-                if (h[i] > 0.f) m0 |= 1u << (nt * 4 + i);
-            }
-            st_act<true>(sm.A1h, sm.A1l, sm.H0f, 16 * nt + r16, fr, lane, h);
+            for (int i = 0; i < 4; ++i) h[i] = (m0 >> (nt * 4 + i)) & 1u ? acc[nt][i] + b0v[i] : 0.f;
+            st_hi_fm(sm.A1h, sm.H0f, 16 * nt + r16, fr, lane, h);
         }
         __syncthreads();
-        // ---- 4. H1^T slice = relu(W1^T H0^T + b1) -> F2 (only dW2 reads H1) ----
+        // ---- 4. H1^T slice = W1^T H0^T + b1 under the forward's mask -> F2 (dW2 only) ----
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -721,17 +774,14 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
                 const int ho = (16 * nt + r16) * SM + 32 * kt + 8 * g;
-                acc[nt] = mfma_w(A1[kt], ld_frag(&sm.A1h[ho]), ld_frag(&sm.A1l[ho]), acc[nt]);
+                acc[nt] = mfma16x16x32(A1[kt], ld_frag(&sm.A1h[ho]), acc[nt]);
             }
         }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
             float h[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                h[i] = fmaxf(acc[nt][i] + b1v[i], 0.f);
-                if (h[i] > 0.f) m1 |= 1u << (nt * 4 + i);
-            }
+            for (int i = 0; i < 4; ++i) h[i] = (m1 >> (nt * 4 + i)) & 1u ? acc[nt][i] + b1v[i] : 0.f;
             st_fm(sm.F2, 16 * nt + r16, fr, lane, h);
         }
         stage_b(next);  // the next group's records were loaded during the last group

@@ -42,7 +42,7 @@ constexpr int kScatterGrid = 1024; // default scatter workgroups
 
 struct Layout {
     size_t count, offset, total, t_last, ray_out, ray_grad, ray_ctot, ray_loss, loss;
-    size_t xyze, atwr, rgb, adj, feat, grad;
+    size_t xyze, atwr, rgb, adj, feat, relu, grad;
     size_t touched, dw_part, w0t, w1t, w2t, w2, w1, w0;
     size_t bytes;
 };
@@ -71,6 +71,7 @@ Layout layout_for(int64_t n_rays, int64_t capacity, int64_t nleaf) {
     L.rgb = take(16 * capacity);
     L.adj = take(16 * capacity);
     L.feat = take(48 * capacity);
+    L.relu = take(32 * capacity);
     L.grad = take(64 * capacity);
     L.touched = take(nleaf > 0 ? nleaf : 1);
     L.dw_part = take((size_t)4 * kMlpParams * kMaxMlpGrid);
@@ -386,6 +387,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     rec.rgb = reinterpret_cast<float4*>(ws + L.rgb);
     rec.adj = reinterpret_cast<float4*>(ws + L.adj);
     rec.feat = reinterpret_cast<uint4*>(ws + L.feat);
+    rec.relu = reinterpret_cast<uint32_t*>(ws + L.relu);
     rec.grad = reinterpret_cast<float4*>(ws + L.grad);
     uint8_t* touched = ws + L.touched;
     float* dw_part = reinterpret_cast<float*>(ws + L.dw_part);
