@@ -229,7 +229,12 @@ typedef struct pvdb_step_desc {
     int32_t reserved;
     void* workspace; /* dev, pvdb_fused_workspace_bytes() bytes */
     int64_t workspace_bytes;
-    int64_t capacity; /* max records (occupancy-kept samples) per step */
+    /* max records (occupancy-kept samples) per step.  Records past it are
+     * dropped: the step then covers the first `capacity` records in ray order
+     * (no memory is touched beyond it) and pvdb_fused_records' total, the full
+     * count, exceeds it.  The sum of the rays' step counts (pvdb_ray_bounds)
+     * bounds every step. */
+    int64_t capacity;
     float* ray_out;   /* dev [n][4] rgb + T_last, or NULL */
     float* loss;      /* dev [1] or NULL */
     /* dev [3] or NULL: stepsz_density, stepsz_color, step_size_mlp read on the

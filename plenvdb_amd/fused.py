@@ -253,6 +253,11 @@ class FusedTrainer:
         self.grad_color.zero_()
         self.mlp_grad.zero_()
 
+    def overflowed(self) -> bool:
+        """Whether the last step had more records than the capacity (the step
+        then covered the first `capacity` of them; reads the count back)."""
+        return self.total_records() > self.capacity
+
     def total_records(self) -> int:
         ptrs = [C.c_void_p() for _ in range(7)]
         lib.call("pvdb_fused_records", self.topo.handle, C.byref(self.desc), *[C.byref(p) for p in ptrs])

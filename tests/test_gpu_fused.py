@@ -149,3 +149,26 @@ def test_fused_record_offsets_are_the_exclusive_scan(n_rays):
     exp = np.concatenate([[0], np.cumsum(cnt)[:-1]])
     np.testing.assert_array_equal(recs["offset"].astype(np.int64), exp)
     assert int(cnt.sum()) == int(tr.total_records())
+
+
+def test_fused_step_with_too_small_capacity():
+    """A capacity below the step's record count: the step stays in bounds, covers
+    the first `capacity` records, reports the overflow, and yields finite values."""
+    from plenvdb_amd import scenes
+    from plenvdb_amd.fused import FusedTrainer
+
+    print("[case] fused step c2/256 rays with capacity 1000", flush=True)
+    sc = scenes.make_scene("c2", n_rays=256)
+    full = FusedTrainer(sc)
+    full.forward_backward()
+    torch.cuda.synchronize()
+    n = full.total_records()
+    assert n > 1000 and not full.overflowed()
+    tr = FusedTrainer(sc, capacity=1000)
+    tr.step()
+    torch.cuda.synchronize()
+    print(f"[result] {n} records, capacity 1000, overflowed {tr.overflowed()}", flush=True)
+    assert tr.total_records() == n and tr.overflowed()
+    assert torch.isfinite(tr.loss).all() and torch.isfinite(tr.ray_out).all()
+    assert torch.isfinite(tr.density).all() and torch.isfinite(tr.color.float()).all()
+    assert torch.isfinite(tr.mlp).all()
