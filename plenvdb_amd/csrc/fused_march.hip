@@ -61,8 +61,11 @@ struct StepPoint {
     bool keep;  // inside the bbox and in an active voxel
 };
 
+// Step i of n (I = int32_t when n < 2^24, where (float)i is the same as from
+// int64, so the common case needs no 64-bit index arithmetic).
+template <typename I>
 __device__ __forceinline__ StepPoint step_point(const StepParams& P, const TopoView& t,
-                                                const RaySetup& s, int64_t i) {
+                                                const RaySetup& s, I i, I n) {
     StepPoint sp;
     const float dist = P.stepdist * (float)i;
     sp.px = s.sx + s.ux * dist;
@@ -71,13 +74,25 @@ __device__ __forceinline__ StepPoint step_point(const StepParams& P, const TopoV
     const bool out = (P.mn[0] > sp.px) | (P.mn[1] > sp.py) | (P.mn[2] > sp.pz) |
                      (P.mx[0] < sp.px) | (P.mx[1] < sp.py) | (P.mx[2] < sp.pz);
     sp.keep = false;
-    if (i < s.n_steps && !out) {
+    if (i < n && !out) {
         const int a = (int)roundf(sp.px * P.ms[0] + P.mb[0]);
         const int b = (int)roundf(sp.py * P.ms[1] + P.mb[1]);
         const int c = (int)roundf(sp.pz * P.ms[2] + P.mb[2]);
         sp.keep = voxel_active(t, a, b, c);
     }
     return sp;
+}
+
+// Runs window(base, n) for base = 0, 64, ... < n_steps, with 32-bit indices
+// when the ray's step count allows.
+template <typename F>
+__device__ __forceinline__ void for_windows(int64_t n_steps, F&& window) {
+    if (n_steps < (int64_t(1) << 24)) {
+        const int32_t n = (int32_t)n_steps;
+        for (int32_t base = 0; base < n; base += 64) window(base, n);
+    } else {
+        for (int64_t base = 0; base < n_steps; base += 64) window(base, n_steps);
+    }
 }
 
 __device__ __forceinline__ int mbcnt(uint64_t m) {
@@ -92,10 +107,10 @@ __global__ void __launch_bounds__(256) k_ray_count(StepParams P, TopoView t, Ray
     const int lane = lane_id();
     const RaySetup s = ray_setup(P, r);
     int count = 0;
-    for (int64_t base = 0; base < s.n_steps; base += 64) {
-        const StepPoint sp = step_point(P, t, s, base + lane);
+    for_windows(s.n_steps, [&](auto base, auto n) {
+        const StepPoint sp = step_point(P, t, s, base + lane, n);
         count += __popcll(__ballot(sp.keep));
-    }
+    });
     if (lane == 0) R.count[r] = count;
 }
 
@@ -167,10 +182,10 @@ __global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, con
     double Tc = 1.0;
     bool stopped = false;
     int written = 0;
-    for (int64_t base = 0; base < s.n_steps; base += 64) {
-        const StepPoint sp = step_point(P, t, s, base + lane);
+    for_windows(s.n_steps, [&](auto base, auto n) {
+        const StepPoint sp = step_point(P, t, s, base + lane, n);
         const uint64_t km = __ballot(sp.keep);
-        if (km == 0) continue;
+        if (km == 0) return;
         const int idx = written + mbcnt(km);
         written += __popcll(km);
         float x = 0.f, y = 0.f, z = 0.f, e = 0.f, alpha = 0.f;
@@ -219,7 +234,7 @@ __global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, con
                 Tc = __shfl(Ta, 63, 64);
             }
         }
-    }
+    });
     if (lane == 0) R.t_last[r] = (float)Tc;
 }
 
