@@ -99,6 +99,7 @@ constexpr int kB1 = kW1 + 128 * 128;
 constexpr int kW2 = kB1 + 128;
 constexpr int kB2 = kW2 + 128 * 3;
 constexpr int kMlpParams = kB2 + 3;  // 22019
+constexpr int kPartStride = (kMlpParams + 3) & ~3;  // dW partial rows padded to 16 bytes
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 

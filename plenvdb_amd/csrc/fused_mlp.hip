@@ -876,7 +876,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
     }
 
     // ---- flush this workgroup's dW partial ----
-    float* out = dw_partial + (int64_t)blockIdx.x * kMlpParams;
+    float* out = dw_partial + (int64_t)blockIdx.x * kPartStride;
 #pragma unroll
     for (int mi = 0; mi < 3; ++mi)
 #pragma unroll
@@ -920,17 +920,54 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
     }
 }
 
-// Sum of the per-workgroup dW partials, accumulated into dw: blockIdx.y takes
-// a chunk of kReduceChunk partials, so the reduction runs on enough
-// workgroups to stream the partials at HBM rate.
-constexpr int kReduceChunk = 16;
-__global__ void k_dw_reduce(const float* __restrict__ part, int nparts, float* __restrict__ dw) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= kMlpParams) return;
-    const int b0 = blockIdx.y * kReduceChunk, b1 = min(nparts, b0 + kReduceChunk);
-    float s = 0.f;
-    for (int b = b0; b < b1; ++b) s += part[(int64_t)b * kMlpParams + p];
-    atomicAdd(dw + p, s);
+// Sum of the per-workgroup dW partials, added into dw without atomics: a
+// workgroup owns kReduceVec four-parameter vectors and splits the partials
+// among kReduceGrp lane groups (16-byte loads, eight in flight per lane); the
+// groups' sums are combined through LDS in a fixed order, so the result is
+// reproducible.
+constexpr int kReduceVec = 16, kReduceGrp = 16;
+static_assert(kReduceVec * kReduceGrp == 256, "one thread per (vector, group)");
+__global__ void __launch_bounds__(256) k_dw_reduce(const float* __restrict__ part, int nparts,
+                                                   float* __restrict__ dw) {
+    __shared__ float4 red[kReduceGrp][kReduceVec];
+    const int lv = threadIdx.x % kReduceVec, grp = threadIdx.x / kReduceVec;
+    const int v = blockIdx.x * kReduceVec + lv;  // float4 index; rows are padded to 16 B
+    const int q = (nparts + kReduceGrp - 1) / kReduceGrp, b0 = grp * q, b1 = min(nparts, b0 + q);
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (4 * v < kMlpParams) {
+        for (int b = b0; b < b1; b += 8) {
+            float4 x[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                x[k] = b + k < b1
+                           ? reinterpret_cast<const float4*>(part + (int64_t)(b + k) * kPartStride)[v]
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                s.x += x[k].x;
+                s.y += x[k].y;
+                s.z += x[k].z;
+                s.w += x[k].w;
+            }
+        }
+    }
+    red[grp][lv] = s;
+    __syncthreads();
+    if (grp == 0 && 4 * v < kMlpParams) {
+        float4 t = red[0][lv];
+#pragma unroll
+        for (int g2 = 1; g2 < kReduceGrp; ++g2) {
+            const float4 u = red[g2][lv];
+            t.x += u.x;
+            t.y += u.y;
+            t.z += u.z;
+            t.w += u.w;
+        }
+        const float tv[4] = {t.x, t.y, t.z, t.w};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)  // the padding element of the last vector is skipped
+            if (4 * v + k < kMlpParams) dw[4 * v + k] += tv[k];
+    }
 }
 
 // fp32 master parameters (kMlpParams layout) -> packed bf16 MFMA operands.
@@ -986,9 +1023,8 @@ int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const Topo
 }
 
 int launch_dw_reduce(const float* part, int nparts, float* dw, hipStream_t s) {
-    const unsigned chunks = (unsigned)((nparts + kReduceChunk - 1) / kReduceChunk);
-    hipLaunchKernelGGL(k_dw_reduce, dim3(pvdb_blocks(kMlpParams, 256), chunks), dim3(256), 0, s,
-                       part, nparts, dw);
+    hipLaunchKernelGGL(k_dw_reduce, dim3((kPartStride / 4 + kReduceVec - 1) / kReduceVec), dim3(256),
+                       0, s, part, nparts, dw);
     PVDB_LAUNCHED("k_dw_reduce");
     return PVDB_OK;
 }

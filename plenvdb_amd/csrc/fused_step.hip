@@ -74,7 +74,7 @@ Layout layout_for(int64_t n_rays, int64_t capacity, int64_t nleaf) {
     L.relu = take(32 * capacity);
     L.grad = take(64 * capacity);
     L.touched = take(nleaf > 0 ? nleaf : 1);
-    L.dw_part = take((size_t)4 * kMlpParams * kMaxMlpGrid);
+    L.dw_part = take((size_t)4 * kPartStride * kMaxMlpGrid);
     L.w0t = take(2 * 128 * 64);
     L.w1t = take(2 * 128 * 128);
     L.w2t = take(2 * 16 * 128);
