@@ -229,10 +229,10 @@ def bench_batched(args, scene):
 # x1.0); profiles/pmc_traffic_c2_train_c5_render.md holds the runs.
 TRAFFIC_SOURCE = "profiles/pmc_traffic_c2_train_c5_render.md"
 MEASURED_TRAFFIC_MB = {
-    "c2": {"mlp_bwd": 53.25, "scatter": 64.87, "mlp_fwd": 57.23, "ray_march": 14.07,
-           "dw_reduce": 26.73, "composite": 15.24, "adam": 107.76, "ray_count": 1.39,
+    "c2": {"mlp_bwd": 59.69, "scatter": 64.87, "mlp_fwd": 64.29, "ray_march": 14.08,
+           "dw_reduce": 32.62, "composite": 15.28, "adam": 107.76, "ray_count": 1.42,
            "pack_mlp": 0.91},
-    "c5": {"render": 111.69},
+    "c5": {"render": 111.76},
 }
 
 
