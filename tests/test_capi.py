@@ -47,3 +47,37 @@ def test_argument_errors_are_reported():
         lib.call("pvdb_vdb_adam", None, None, None, None, None, None, 12, 3, 0, 0.1, 1e-8, 0.9,
                  0.99, None)
     assert L.pvdb_abi_version() == 1
+
+
+def test_new_entry_points_check_arguments():
+    """Host argument checks of the camera-ray, FP64 adjoint and fused-step entry
+    points (all before any device work)."""
+    import numpy as np
+
+    from plenvdb_amd import lib
+
+    K = np.array([[50.0, 0, 4], [0, 50.0, 4], [0, 0, 1]])
+    c2w = np.eye(4, dtype=np.float32)[:3].copy()
+    kp, cp = K.ctypes.data_as(C.c_void_p), c2w.ctypes.data_as(C.c_void_p)
+    with pytest.raises(RuntimeError, match="mode must be"):
+        lib.call("pvdb_rays_of_view", 4, 4, kp, cp, 0, 0, 0, 0, 2, None, None, None, None)
+    with pytest.raises(RuntimeError, match="null pointer"):
+        lib.call("pvdb_rays_of_view", 4, 4, kp, cp, 0, 0, 0, 0, 0, None, None, None, None)
+    Kz = np.zeros((3, 3))
+    with pytest.raises(RuntimeError, match="focal"):
+        lib.call("pvdb_rays_of_view", 4, 4, Kz.ctypes.data_as(C.c_void_p), cp, 0, 0, 0, 0, 0,
+                 1, 1, 1, None)
+    lib.call("pvdb_rays_of_view", 0, 4, kp, cp, 0, 0, 0, 0, 0, None, None, None, None)  # empty
+    with pytest.raises(RuntimeError, match="null topology"):
+        lib.call("pvdb_trilinear_fwd_f64", None, None, 1, None, None, None, 4, None, None)
+    with pytest.raises(RuntimeError, match="n must be"):
+        lib.call("pvdb_raw2alpha_f64", None, -1, 0.0, 0.5, None, None, None)
+    with pytest.raises(RuntimeError, match="null pointer"):
+        lib.call("pvdb_alpha2weight_bwd_f64", None, None, None, None, None, None, 3, None, None,
+                 None, None)
+    from plenvdb_amd.fused import StepDesc
+
+    d = StepDesc()
+    with pytest.raises(RuntimeError, match="null topology"):
+        lib.call("pvdb_fused_step", None, C.byref(d), None)
+    assert lib.load().pvdb_fused_workspace_bytes(-1, 1, 1) == -1
