@@ -101,6 +101,17 @@ constexpr int kB2 = kW2 + 128 * 3;
 constexpr int kMlpParams = kB2 + 3;  // 22019
 constexpr int kPartStride = (kMlpParams + 3) & ~3;  // dW partial rows padded to 16 bytes
 
+// Adam of the MLP parameters (stepmode 0, adam_upd_kernel.cu:8-58), applied by
+// the dW reduction to each parameter once its gradient is final.  p == nullptr
+// skips it; the step size comes from the device (dev_sz[2]) when given.
+struct MlpAdam {
+    float* p;
+    float* m;
+    float* v;
+    const float* dev_sz;
+    float sz, eps, b0, b1;
+};
+
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
 }  // namespace pvdb

@@ -924,11 +924,12 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
 // workgroup owns kReduceVec four-parameter vectors and splits the partials
 // among kReduceGrp lane groups (16-byte loads, eight in flight per lane); the
 // groups' sums are combined through LDS in a fixed order, so the result is
-// reproducible.
+// reproducible.  With Adam on, the same thread then updates the parameters
+// (adam_upd_kernel.cu:8-58, stepmode 0, on the final gradient).
 constexpr int kReduceVec = 16, kReduceGrp = 16;
 static_assert(kReduceVec * kReduceGrp == 256, "one thread per (vector, group)");
 __global__ void __launch_bounds__(256) k_dw_reduce(const float* __restrict__ part, int nparts,
-                                                   float* __restrict__ dw) {
+                                                   float* __restrict__ dw, MlpAdam A) {
     __shared__ float4 red[kReduceGrp][kReduceVec];
     const int lv = threadIdx.x % kReduceVec, grp = threadIdx.x / kReduceVec;
     const int v = blockIdx.x * kReduceVec + lv;  // float4 index; rows are padded to 16 B
@@ -964,9 +965,21 @@ __global__ void __launch_bounds__(256) k_dw_reduce(const float* __restrict__ par
             t.w += u.w;
         }
         const float tv[4] = {t.x, t.y, t.z, t.w};
+        const float sz = A.dev_sz ? A.dev_sz[2] : A.sz;
 #pragma unroll
-        for (int k = 0; k < 4; ++k)  // the padding element of the last vector is skipped
-            if (4 * v + k < kMlpParams) dw[4 * v + k] += tv[k];
+        for (int k = 0; k < 4; ++k) {  // the padding element of the last vector is skipped
+            const int q = 4 * v + k;
+            if (q >= kMlpParams) continue;
+            const float gg = dw[q] + tv[k];
+            dw[q] = gg;
+            if (A.p) {
+                const float nm = A.b0 * A.m[q] + (1.f - A.b0) * gg;
+                const float nv = A.b1 * A.v[q] + ((1.f - A.b1) * gg) * gg;
+                A.m[q] = nm;
+                A.v[q] = nv;
+                A.p[q] = A.p[q] - (sz * nm) / (sqrtf(nv) + A.eps);
+            }
+        }
     }
 }
 
@@ -1022,9 +1035,10 @@ int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const Topo
     return PVDB_OK;
 }
 
-int launch_dw_reduce(const float* part, int nparts, float* dw, hipStream_t s) {
+int launch_dw_reduce(const float* part, int nparts, float* dw, const MlpAdam& adam,
+                     hipStream_t s) {
     hipLaunchKernelGGL(k_dw_reduce, dim3((kPartStride / 4 + kReduceVec - 1) / kReduceVec), dim3(256),
-                       0, s, part, nparts, dw);
+                       0, s, part, nparts, dw, adam);
     PVDB_LAUNCHED("k_dw_reduce");
     return PVDB_OK;
 }

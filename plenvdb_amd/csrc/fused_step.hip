@@ -23,7 +23,8 @@ int launch_ray_composite(const StepParams& P, const RayBuffers& R, const Records
 int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const TopoView& t,
                const void* color, const MlpPacked& W, const RayBuffers& R, const Records& rec,
                float* dw_partial, hipStream_t s);
-int launch_dw_reduce(const float* part, int nparts, float* dw, hipStream_t s);
+int launch_dw_reduce(const float* part, int nparts, float* dw, const MlpAdam& adam,
+                     hipStream_t s);
 int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, int64_t capacity,
                    float* grad_density, float* grad_color, uint8_t* leaf_touched, int grid_req,
                    int chunk_records, hipStream_t s);
@@ -204,21 +205,6 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
     }
 }
 
-// Adam of the MLP parameters (stepmode 0, adam_upd_kernel.cu:8-58); the step
-// size comes from the device when the step is replayed from a graph.
-__global__ void k_mlp_adam(float* __restrict__ p, const float* __restrict__ g,
-                           float* __restrict__ m, float* __restrict__ v, int n, float sz_arg,
-                           const float* __restrict__ dev_sz, float eps, float b0, float b1) {
-    const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    const float sz = dev_sz ? dev_sz[2] : sz_arg;
-    const float gg = g[q];
-    const float nm = b0 * m[q] + (1.f - b0) * gg;
-    const float nv = b1 * v[q] + ((1.f - b1) * gg) * gg;
-    m[q] = nm;
-    v[q] = nv;
-    p[q] = p[q] - (sz * nm) / (sqrtf(nv) + eps);
-}
 
 // Optional per-stage device timing (HIP events on the step's stream).
 constexpr int kStages = 9;
@@ -461,7 +447,12 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     }
     {
         StageScope sc(7, s);
-        if ((rc = launch_dw_reduce(dw_part, grid_bwd, d->mlp_grad, s))) return rc;
+        // the MLP Adam rides on the reduction (each parameter once its gradient is final)
+        MlpAdam adam{nullptr, nullptr, nullptr, nullptr, 0.f, 0.f, 0.f, 0.f};
+        if (d->do_adam)
+            adam = MlpAdam{d->mlp,           d->mlp_m, d->mlp_v, d->adam_scalars,
+                           d->step_size_mlp, d->eps,   d->beta0, d->beta1};
+        if ((rc = launch_dw_reduce(dw_part, grid_bwd, d->mlp_grad, adam, s))) return rc;
     }
     if (d->ray_out) {
         PVDB_HIP(hipMemcpyAsync(d->ray_out, R.ray_out, 16 * (size_t)d->n_rays,
@@ -481,10 +472,6 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
                                d->grad_color, d->color_m, d->color_v, d->stepsz_density,
                                d->stepsz_color, d->adam_scalars, d->eps, d->beta0, d->beta1);
         PVDB_LAUNCHED("k_sparse_adam");
-        hipLaunchKernelGGL(k_mlp_adam, dim3(pvdb_blocks(kMlpParams, 256)), dim3(256), 0, s, d->mlp,
-                           d->mlp_grad, d->mlp_m, d->mlp_v, kMlpParams, d->step_size_mlp,
-                           d->adam_scalars, d->eps, d->beta0, d->beta1);
-        PVDB_LAUNCHED("k_mlp_adam");
     }
     return PVDB_OK;
 }

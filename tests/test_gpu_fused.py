@@ -172,3 +172,34 @@ def test_fused_step_with_too_small_capacity():
     assert torch.isfinite(tr.loss).all() and torch.isfinite(tr.ray_out).all()
     assert torch.isfinite(tr.density).all() and torch.isfinite(tr.color.float()).all()
     assert torch.isfinite(tr.mlp).all()
+
+
+def test_fused_step_mlp_adam_first_step():
+    """The MLP Adam of the fused step (stepmode 0, adam_upd_kernel.cu:8-58): from
+    m = v = 0 the first step moves every parameter by
+    -sz * (0.1 g) / (eps + sqrt(0.01 g^2)), g the step's MLP gradient (the
+    reduction is deterministic, so a forward+backward of a twin trainer gives
+    the same g)."""
+    from plenvdb_amd import scenes
+    from plenvdb_amd.fused import FusedTrainer
+
+    print("[case] fused step MLP adam c2", flush=True)
+    sc = scenes.make_scene("c2", n_rays=256)
+    twin = FusedTrainer(sc)
+    twin.forward_backward()
+    torch.cuda.synchronize()
+    g = twin.mlp_grad.cpu().numpy().astype(np.float64)
+    tr = FusedTrainer(sc)
+    p0 = tr.mlp.cpu().numpy().astype(np.float64)
+    tr.step()
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(tr.mlp_grad.cpu().numpy(), twin.mlp_grad.cpu().numpy())
+    f = lambda x: float(np.float32(x))  # noqa: E731
+    sz = f(tr._stepsz(1e-3, 1))
+    exp = p0 - sz * ((1 - f(0.9)) * g) / (f(1e-8) + np.sqrt((1 - f(0.99)) * g * g))
+    got = tr.mlp.cpu().numpy().astype(np.float64)
+    err = rel_l2(got - p0, exp - p0)
+    print(f"[result] MLP first-step update rel-L2 {err:.2e}", flush=True)
+    assert (g != 0).sum() > 1000
+    assert err <= 1e-5
+    assert np.all(got[g == 0] == p0[g == 0])
