@@ -230,7 +230,7 @@ def bench_batched(args, scene):
 TRAFFIC_SOURCE = "profiles/pmc_traffic_c2_train_c5_render.md"
 MEASURED_TRAFFIC_MB = {
     "c2": {"mlp_bwd": 59.69, "scatter": 64.87, "mlp_fwd": 64.29, "ray_march": 14.08,
-           "dw_reduce": 32.62, "composite": 15.28, "adam": 107.76, "ray_count": 1.42,
+           "dw_reduce": 33.26, "composite": 15.28, "adam": 107.12, "ray_count": 1.42,
            "pack_mlp": 0.91},
     "c5": {"render": 111.76},
 }
@@ -257,8 +257,9 @@ def stage_models(r):
         "mlp_bwd": ("mfma", nw * MLP_BWD_FLOP + N * MLP_RAY_FLOP),
         # records + per-record gradients in, distinct voxels' 13 gradients read+written
         "scatter": ("hbm", S * (16 + 16 + 64) + uw * 13 * 4 * 2),
-        # one dW partial per CU in, MLP gradient read+written
-        "dw_reduce": ("hbm", r["n_cu"] * P * 4 + 2 * P * 4),
+        # one dW partial per CU in, MLP gradient read+written; with the MLP
+        # Adam: parameters and both moments read+written
+        "dw_reduce": ("hbm", r["n_cu"] * P * 4 + 2 * P * 4 + 6 * P * 4),
         # survey B_adam: density p,m,v r/w + g r; colour p,m,v r/w + g r
         "adam": ("hbm", V_u * (28 + 12 * (2 * sc + 20)) + L_t * 512 * 13 * 4),
     }
