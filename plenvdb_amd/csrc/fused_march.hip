@@ -366,18 +366,29 @@ __global__ void __launch_bounds__(256) k_ray_composite(StepParams P, RayBuffers 
     }
 }
 
-// Loss = sum of the per-ray terms (one workgroup; the order of the sum is fixed).
+// Loss = sum of the per-ray terms (one workgroup, a fixed order): 16-byte loads
+// (the per-ray array is 256-byte aligned), wavefront shuffles, then the 16
+// wavefront sums in order.
 __global__ void __launch_bounds__(1024) k_loss_reduce(RayBuffers R, int n) {
-    __shared__ float part[1024];
+    __shared__ float wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int n4 = n >> 2;
     float s = 0.f;
-    for (int i = threadIdx.x; i < n; i += 1024) s += R.ray_loss[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int d = 512; d > 0; d >>= 1) {
-        if ((int)threadIdx.x < d) part[threadIdx.x] += part[threadIdx.x + d];
-        __syncthreads();
+    for (int i = tid; i < n4; i += 1024) {
+        const float4 v = reinterpret_cast<const float4*>(R.ray_loss)[i];
+        s += ((v.x + v.y) + v.z) + v.w;
     }
-    if (threadIdx.x == 0) R.loss[0] = part[0];
+    if (tid < (n & 3)) s += R.ray_loss[4 * n4 + tid];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (lane == 0) wsum[wv] = s;
+    __syncthreads();
+    if (tid == 0) {
+        float t = 0.f;
+#pragma unroll
+        for (int w = 0; w < 16; ++w) t += wsum[w];
+        R.loss[0] = t;
+    }
 }
 
 }  // namespace
