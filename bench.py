@@ -230,8 +230,7 @@ def bench_batched(args, scene):
 TRAFFIC_SOURCE = "profiles/pmc_traffic_c2_train_c5_render.md"
 MEASURED_TRAFFIC_MB = {
     "c2": {"mlp_bwd": 59.69, "scatter": 64.87, "mlp_fwd": 64.29, "ray_march": 14.08,
-           "dw_reduce": 33.26, "composite": 15.28, "adam": 107.12, "ray_count": 1.42,
-           "pack_mlp": 0.91},
+           "dw_reduce": 33.26, "composite": 15.28, "adam": 107.12, "ray_count": 2.33},
     "c5": {"render": 111.76},
 }
 
@@ -244,9 +243,10 @@ def stage_models(r):
     ua, uw, L_t, V_u = r["uniq_all"], r["uniq_w"], r["l_t"], r["v_u"]
     P = 22019
     return {
-        "pack_mlp": ("hbm", P * 4 + 2 * (128 * 64 + 2 * 128 * 128 + 2 * 16 * 128 + 128 * 32) + P * 4),
-        # rays in, counts out
-        "ray_count": ("hbm", N * (12 + 12 + 4)),
+        # rays in, counts out; the MLP parameters packed to bf16 and the MLP
+        # gradient zeroed by the same launch
+        "ray_count": ("hbm", N * (12 + 12 + 4)
+                      + P * 4 + 2 * (128 * 64 + 2 * 128 * 128 + 2 * 16 * 128 + 128 * 32) + P * 4),
         # rays + offsets in, 32-byte records out, distinct density voxels once
         "ray_march": ("hbm", N * (12 + 12 + 8) + S * 32 + ua * 4),
         # 36,608 FLOP per MLP sample + 6,912 per ray (PE part of layer 0)
@@ -522,7 +522,8 @@ def main():
                       "samples_per_s": round(v["samples_per_step"] / (v["ms_per_step"] * 1e-3), 1)}
                   for k, v in res.items()},
         "kernels_ms": ({k: round(v[0] / args.steps, 4) for k, v in sorted(
-            main_res["stages"].items(), key=lambda kv: -kv[1][0])} if path == "batched" else
+            main_res["stages"].items(), key=lambda kv: -kv[1][0]) if v[1] > 0}
+            if path == "batched" else
             {k: round(v[0] / args.steps, 4) for k, v in sorted(
                 main_res["totals"].items(), key=lambda kv: -kv[1][0])}),
         "step_bytes_model": step_roof,

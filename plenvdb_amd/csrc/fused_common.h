@@ -104,6 +104,38 @@ constexpr int kPartStride = (kMlpParams + 3) & ~3;  // dW partial rows padded to
 // Adam of the MLP parameters (stepmode 0, adam_upd_kernel.cu:8-58), applied by
 // the dW reduction to each parameter once its gradient is final.  p == nullptr
 // skips it; the step size comes from the device (dev_sz[2]) when given.
+// The fp32 master parameters (kMlpParams layout) packed into the bf16 MFMA
+// operands of the step, and the MLP gradient zeroed (k_dw_reduce accumulates
+// it later in the step): item t of kPackItems.
+struct PackArgs {
+    const float* prm;
+    uint16_t *w0t, *w1t, *w2t, *w2, *w1, *w0;
+    float* grad;
+};
+constexpr int kPackItems = kMlpParams > 128 * 128 ? kMlpParams : 128 * 128;
+
+__device__ __forceinline__ void pack_mlp_item(int t, const PackArgs& A) {
+    if (t < kMlpParams) A.grad[t] = 0.f;
+    if (t < 128 * 128) {  // w1t / w1: 128 x 128
+        const int a = t >> 7, b = t & 127;
+        A.w1t[a * 128 + b] = f2bf_bits(A.prm[kW1 + b * 128 + a]);  // W1^T[out a][in b]
+        A.w1[a * 128 + b] = f2bf_bits(A.prm[kW1 + a * 128 + b]);   // W1[in a][out b]
+    }
+    if (t < 128 * 64) {  // w0t [128 out][64 in]
+        const int o = t >> 6, i = t & 63;
+        A.w0t[o * 64 + i] = i < 39 ? f2bf_bits(A.prm[kW0 + i * 128 + o]) : (uint16_t)0;
+    }
+    if (t < 16 * 128) {  // w2t [16 out][128 in], w0 [16 in][128 out]
+        const int o = t >> 7, i = t & 127;
+        A.w2t[o * 128 + i] = o < 3 ? f2bf_bits(A.prm[kW2 + i * 3 + o]) : (uint16_t)0;
+        A.w0[o * 128 + i] = o < 12 ? f2bf_bits(A.prm[kW0 + o * 128 + i]) : (uint16_t)0;
+    }
+    if (t < 128 * 32) {  // w2 [128 in][32 out]
+        const int i = t >> 5, o = t & 31;
+        A.w2[i * 32 + o] = o < 3 ? f2bf_bits(A.prm[kW2 + i * 3 + o]) : (uint16_t)0;
+    }
+}
+
 struct MlpAdam {
     float* p;
     float* m;

@@ -100,8 +100,15 @@ __device__ __forceinline__ int mbcnt(uint64_t m) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Counting march: occupancy-kept samples per ray.
-__global__ void __launch_bounds__(256) k_ray_count(StepParams P, TopoView t, RayBuffers R) {
+// Counting march: occupancy-kept samples per ray.  The workgroups past the
+// rays' pack the step's MLP operands (independent work, one launch fewer).
+__global__ void __launch_bounds__(256) k_ray_count(StepParams P, TopoView t, RayBuffers R,
+                                                   PackArgs pk, int ray_blocks) {
+    if ((int)blockIdx.x >= ray_blocks) {  // uniform per workgroup
+        const int item = ((int)blockIdx.x - ray_blocks) * 256 + (int)threadIdx.x;
+        if (item < kPackItems) pack_mlp_item(item, pk);
+        return;
+    }
     const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (r >= P.n_rays) return;
     const int lane = lane_id();
@@ -395,8 +402,11 @@ __global__ void __launch_bounds__(1024) k_loss_reduce(RayBuffers R, int n) {
 
 namespace pvdb {
 
-int launch_ray_count(const StepParams& P, const TopoView& t, const RayBuffers& R, hipStream_t s) {
-    hipLaunchKernelGGL(k_ray_count, dim3(pvdb_blocks(P.n_rays, 4)), dim3(256), 0, s, P, t, R);
+int launch_ray_count(const StepParams& P, const TopoView& t, const RayBuffers& R,
+                     const PackArgs& pk, hipStream_t s) {
+    const int ray_blocks = (int)pvdb_blocks(P.n_rays, 4);
+    const int grid = ray_blocks + (int)pvdb_blocks(kPackItems, 256);
+    hipLaunchKernelGGL(k_ray_count, dim3(grid), dim3(256), 0, s, P, t, R, pk, ray_blocks);
     PVDB_LAUNCHED("k_ray_count");
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, R, P.n_rays);
     PVDB_LAUNCHED("k_scan");

@@ -983,34 +983,6 @@ __global__ void __launch_bounds__(256) k_dw_reduce(const float* __restrict__ par
     }
 }
 
-// fp32 master parameters (kMlpParams layout) -> packed bf16 MFMA operands.
-__global__ void k_pack_mlp(const float* __restrict__ prm, uint16_t* __restrict__ w0t,
-                           uint16_t* __restrict__ w1t, uint16_t* __restrict__ w2t,
-                           uint16_t* __restrict__ w2, uint16_t* __restrict__ w1,
-                           uint16_t* __restrict__ w0, float* __restrict__ grad) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    // the MLP gradient, accumulated by k_dw_reduce later in the step, starts at zero
-    if (grad != nullptr && t < kMlpParams) grad[t] = 0.f;
-    // w1t / w1: 128 x 128
-    if (t < 128 * 128) {
-        const int a = t >> 7, b = t & 127;
-        w1t[a * 128 + b] = f2bf_bits(prm[kW1 + b * 128 + a]);  // W1^T[out a][in b]
-        w1[a * 128 + b] = f2bf_bits(prm[kW1 + a * 128 + b]);   // W1[in a][out b]
-    }
-    if (t < 128 * 64) {  // w0t [128 out][64 in]
-        const int o = t >> 6, i = t & 63;
-        w0t[o * 64 + i] = i < 39 ? f2bf_bits(prm[kW0 + i * 128 + o]) : (uint16_t)0;
-    }
-    if (t < 16 * 128) {  // w2t [16 out][128 in], w0 [16 in][128 out]
-        const int o = t >> 7, i = t & 127;
-        w2t[o * 128 + i] = o < 3 ? f2bf_bits(prm[kW2 + i * 3 + o]) : (uint16_t)0;
-        w0[o * 128 + i] = o < 12 ? f2bf_bits(prm[kW0 + o * 128 + i]) : (uint16_t)0;
-    }
-    if (t < 128 * 32) {  // w2 [128 in][32 out]
-        const int i = t >> 5, o = t & 31;
-        w2[i * 32 + o] = o < 3 ? f2bf_bits(prm[kW2 + i * 3 + o]) : (uint16_t)0;
-    }
-}
 
 }  // namespace
 
@@ -1043,13 +1015,5 @@ int launch_dw_reduce(const float* part, int nparts, float* dw, const MlpAdam& ad
     return PVDB_OK;
 }
 
-int launch_pack_mlp(const float* prm, uint16_t* w0t, uint16_t* w1t, uint16_t* w2t, uint16_t* w2,
-                    uint16_t* w1, uint16_t* w0, float* grad, hipStream_t s) {
-    const int n = kMlpParams > 128 * 128 ? kMlpParams : 128 * 128;
-    hipLaunchKernelGGL(k_pack_mlp, dim3(pvdb_blocks(n, 256)), dim3(256), 0, s, prm, w0t, w1t, w2t,
-                       w2, w1, w0, grad);
-    PVDB_LAUNCHED("k_pack_mlp");
-    return PVDB_OK;
-}
 
 }  // namespace pvdb

@@ -15,7 +15,8 @@
 
 namespace pvdb {
 TopoView view_of(const pvdb_topology* t);
-int launch_ray_count(const StepParams& P, const TopoView& t, const RayBuffers& R, hipStream_t s);
+int launch_ray_count(const StepParams& P, const TopoView& t, const RayBuffers& R,
+                     const PackArgs& pk, hipStream_t s);
 int launch_ray_march(const StepParams& P, const TopoView& t, const float* density,
                      const RayBuffers& R, const Records& rec, hipStream_t s);
 int launch_ray_composite(const StepParams& P, const RayBuffers& R, const Records& rec,
@@ -28,8 +29,6 @@ int launch_dw_reduce(const float* part, int nparts, float* dw, const MlpAdam& ad
 int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, int64_t capacity,
                    float* grad_density, float* grad_color, uint8_t* leaf_touched, int grid_req,
                    int chunk_records, hipStream_t s);
-int launch_pack_mlp(const float* prm, uint16_t* w0t, uint16_t* w1t, uint16_t* w2t, uint16_t* w2,
-                    uint16_t* w1, uint16_t* w0, float* grad, hipStream_t s);
 }  // namespace pvdb
 
 using namespace pvdb;
@@ -396,14 +395,11 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
 
     int rc;
     {
-        StageScope sc(0, s);
-        // packs the bf16 operands and zeroes the MLP gradient (accumulated by
-        // k_dw_reduce later in the step)
-        if ((rc = launch_pack_mlp(d->mlp, w0t, w1t, w2t, w2, w1, w0, d->mlp_grad, s))) return rc;
-    }
-    {
+        // the counting march also packs the bf16 MLP operands and zeroes the
+        // MLP gradient (accumulated by k_dw_reduce later in the step)
         StageScope sc(1, s);
-        if ((rc = launch_ray_count(P, t, R, s))) return rc;
+        const PackArgs pk{d->mlp, w0t, w1t, w2t, w2, w1, w0, d->mlp_grad};
+        if ((rc = launch_ray_count(P, t, R, pk, s))) return rc;
     }
     {
         StageScope sc(2, s);
