@@ -1,23 +1,26 @@
 // Shared definitions of the fused (batched) training step.
 //
 // Pipeline of one step (all on one stream, no host synchronisation):
-//   k_ray_count    wave per ray: slab test + march + occupancy -> samples per ray
+//   k_ray_count    wave per ray: slab test + march + occupancy -> samples per ray;
+//                  its extra workgroups pack the bf16 MLP operands and zero the
+//                  MLP gradient
 //   k_scan         exclusive scan of the counts -> record offsets, total
 //   k_ray_march    wave per ray: march again, density gather, raw2alpha, alpha cull,
 //                  transmittance product-scan with early stop, weights, weight cull;
 //                  one record per occupancy-kept sample (contiguous per ray)
-//   k_mlp_fwd      workgroup per 32-record group: colour gather + MFMA MLP -> rgb,
-//                  the features are kept per record as bf16 (hi, lo) pairs
-//   k_ray_composite wave per ray: prefix of w*rgb, ray colour, MSE loss and its
-//                  gradient per ray (k_loss_reduce sums the per-ray losses); a
-//                  second pass forms each sample's dL/dlogit and, through the
-//                  alpha/raw2alpha adjoints, dL/dsigma
+//   k_mlp_fwd      workgroup per 32-record group: colour gather + MFMA MLP -> rgb;
+//                  the features (bf16 hi, lo) and the ReLU masks kept per record
+//   k_ray_composite wave per ray: prefix of w*rgb, ray colour, the loss terms and
+//                  their gradient per ray; a second pass forms each sample's
+//                  dL/dlogit and, through the alpha/raw2alpha adjoints, dL/dsigma
+//   k_loss_reduce  the per-ray loss terms summed in a fixed order
 //   k_mlp_bwd      workgroup per CU over 32-record groups: recompute from the
-//                  kept features, MFMA MLP backward, dW; per-record feature and
-//                  sigma gradients
-//   k_scatter_agg  workgroup per 128 records: corner contributions summed per
-//                  slot in an LDS hash, one device atomic per distinct slot
-//   k_dw_reduce, k_sparse_adam, dense Adam of the MLP, k_pack_mlp
+//                  kept features under the kept masks, MFMA MLP backward, dW
+//                  partials; per-record feature and sigma gradients
+//   k_scatter_agg  workgroups over 32- or 64-record chunks: corner contributions
+//                  summed per slot in an LDS hash, device atomics per distinct slot
+//   k_dw_reduce    the dW partials summed in a fixed order (+ the MLP Adam)
+//   k_sparse_adam  VDB Adam over the touched leaves, gradients re-zeroed
 #pragma once
 
 #include "mfma_util.h"
