@@ -515,11 +515,11 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
     }
 }
 
-// Backward weights as MFMA A fragments, [tile][k-step][lane]:
+// Backward weights as MFMA A fragments in LDS, [tile][k-step][lane] (each
+// wave's W1 rows, 16 x 128, sit in its registers):
 //   W2  [8 row tiles of 16 hidden][64] x 8 bytes   (K = 16 logits, 3 used)
-//   W1  [8 row tiles][4 k-steps][64] x 16 bytes    (K = 128 hidden outputs)
 //   W0  [4 k-steps of 32 hidden][64] x 16 bytes    (rows: the 12 features)
-constexpr int kBw2Frags = 8 * 64, kBw1Frags = 8 * 4 * 64, kBw0Frags = 4 * 64;
+constexpr int kBw2Frags = 8 * 64, kBw0Frags = 4 * 64;
 
 // ---- backward ----------------------------------------------------------------
 // One workgroup of 8 wavefronts per CU (two per SIMD); wave w owns hidden
@@ -540,7 +540,6 @@ constexpr int kBw2Frags = 8 * 64, kBw1Frags = 8 * 4 * 64, kBw0Frags = 4 * 64;
 // double-buffered, so the next group's inputs are staged while this one runs.
 constexpr int kBwdWaves = 8;
 struct alignas(16) SmemBwd {
-    uint4 Bw1[kBw1Frags];
     uint4 Bw0[kBw0Frags];
     uint2 Bw2[kBw2Frags];
     uint32_t relu[2][GS * 8];  // the forward's ReLU mask words of each record
@@ -581,18 +580,19 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
         b0v[i] = W.b0[fr + i];
         b1v[i] = W.b1[fr + i];
     }
-    // backward weights -> LDS
-    for (int f = tid; f < kBw2Frags + kBw1Frags + kBw0Frags; f += NTH) {
+    // this wave's W1 rows [16 in][128 out] as MFMA A fragments, in registers
+    bf16x8 Bw1r[4];
+#pragma unroll
+    for (int kt = 0; kt < 4; ++kt)
+        Bw1r[kt] = ld_frag(W.w1 + (16 * wv + r16) * 128 + 32 * kt + 8 * g);
+    // the other backward weights -> LDS
+    for (int f = tid; f < kBw2Frags + kBw0Frags; f += NTH) {
         const int ln = f & 63, gg = ln >> 4, rr = ln & 15;
         if (f < kBw2Frags) {  // W2 [128 in][32 out (3 used)], K = 16
             const int t = f >> 6;
             sm.Bw2[f] = *reinterpret_cast<const uint2*>(W.w2 + (16 * t + rr) * 32 + 4 * gg);
-        } else if (f < kBw2Frags + kBw1Frags) {  // W1 [128 in][128 out]
-            const int q = f - kBw2Frags, t = q >> 8, kt = (q >> 6) & 3;
-            sm.Bw1[q] =
-                *reinterpret_cast<const uint4*>(W.w1 + (16 * t + rr) * 128 + 32 * kt + 8 * gg);
         } else {  // W0 rows 0..15 [16][128]
-            const int q = f - kBw2Frags - kBw1Frags, kt = q >> 6;
+            const int q = f - kBw2Frags, kt = q >> 6;
             sm.Bw0[q] = *reinterpret_cast<const uint4*>(W.w0 + rr * 128 + 32 * kt + 8 * gg);
         }
     }
@@ -813,7 +813,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
         for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) {
-            const bf16x8 a = as_bf8(sm.Bw1[(wv * 4 + kt) * 64 + lane]);
+            const bf16x8 a = Bw1r[kt];
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
                 const int go = (16 * nt + r16) * SM + 32 * kt + 8 * g;
