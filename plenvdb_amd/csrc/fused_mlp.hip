@@ -821,11 +821,18 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
             }
         }
         {
-            // dW1[in][out = f0 + col] += sum over samples H0[s][in] g_H1[s][out]
-            const bf16x8 bh = ld_frag8(&sm.G1f[(f0 + r16) * FM + 8 * g]);
+            // dW1[in][out] += sum over samples H0[s][in] g_H1[s][out]; this wave's
+            // block: inputs 64 (wv & 1) .. +63, outputs 32 (wv >> 1) .. +31 (fewer
+            // LDS reads than a full column of 128 inputs x 16 outputs)
+            const int i0 = 64 * (wv & 1), o0 = 32 * (wv >> 1);
+            const bf16x8 b0 = ld_frag8(&sm.G1f[(o0 + r16) * FM + 8 * g]);
+            const bf16x8 b1 = ld_frag8(&sm.G1f[(o0 + 16 + r16) * FM + 8 * g]);
 #pragma unroll
-            for (int mi = 0; mi < 8; ++mi)
-                dW1[mi] = mfma16x16x32(ld_frag8(&sm.H0f[(16 * mi + r16) * FM + 8 * g]), bh, dW1[mi]);
+            for (int mi = 0; mi < 4; ++mi) {
+                const bf16x8 a = ld_frag8(&sm.H0f[(i0 + 16 * mi + r16) * FM + 8 * g]);
+                dW1[2 * mi] = mfma16x16x32(a, b0, dW1[2 * mi]);
+                dW1[2 * mi + 1] = mfma16x16x32(a, b1, dW1[2 * mi + 1]);
+            }
         }
         // A2 is free and F2 was last read in step 7 (before its barrier): both take g_H0
 #pragma unroll
@@ -888,9 +895,13 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
                 out[kB0 + o] = dW0[mi][i];
         }
 #pragma unroll
-    for (int mi = 0; mi < 8; ++mi)
+    for (int t = 0; t < 8; ++t)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) out[kW1 + (16 * mi + 4 * g + i) * 128 + f0 + r16] = dW1[mi][i];
+        for (int i = 0; i < 4; ++i) {
+            const int in = 64 * (wv & 1) + 16 * (t >> 1) + 4 * g + i;
+            const int o = 32 * (wv >> 1) + 16 * (t & 1) + r16;
+            out[kW1 + in * 128 + o] = dW1[t][i];
+        }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         if (r16 < 3) out[kW2 + (fr + i) * 3 + r16] = dW2[i];
