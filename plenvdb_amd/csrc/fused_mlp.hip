@@ -523,7 +523,9 @@ constexpr int kBw2Frags = 8 * 64, kBw0Frags = 4 * 64;
 
 // ---- backward ----------------------------------------------------------------
 // One workgroup of 8 wavefronts per CU (two per SIMD); wave w owns hidden
-// features [16w, 16w + 16) of both hidden layers.  Per group of 32 records the
+// features [16w, 16w + 16) of both hidden layers, except in the g_H0 product
+// (phase 8), where it owns features [32(w/2), +32) of samples [16(w%2), +16).
+// Per group of 32 records the
 // phases are (a barrier after each):
 //   3  H0^T = W0^T X^T + b0 under the forward's ReLU mask m0    -> A1h, H0f
 //   4  H1^T = W1^T H0^T + b1 under the forward's mask m1         -> F2
@@ -580,11 +582,16 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
         b0v[i] = W.b0[fr + i];
         b1v[i] = W.b1[fr + i];
     }
-    // this wave's W1 rows [16 in][128 out] as MFMA A fragments, in registers
-    bf16x8 Bw1r[4];
+    // the g_H0 product's tile of this wave: hidden features 32 fb .. +31 (two
+    // row tiles) of samples 16 sh .. +15; W1 rows [32 in][128 out] as MFMA A
+    // fragments, in registers
+    const int fb = wv >> 1, sh = wv & 1;
+    bf16x8 Bw1r[2][4];
 #pragma unroll
-    for (int kt = 0; kt < 4; ++kt)
-        Bw1r[kt] = ld_frag(W.w1 + (16 * wv + r16) * 128 + 32 * kt + 8 * g);
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt)
+            Bw1r[mt][kt] = ld_frag(W.w1 + (32 * fb + 16 * mt + r16) * 128 + 32 * kt + 8 * g);
     // the other backward weights -> LDS
     for (int f = tid; f < kBw2Frags + kBw0Frags; f += NTH) {
         const int ln = f & 63, gg = ln >> 4, rr = ln & 15;
@@ -808,17 +815,16 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
                                ld_frag8(&sm.GLf[cb][r16 * FM + 8 * g]), dW2);
         }
         __syncthreads();
-        // ---- 8. g_H0^T slice = W1[slice] g_H1^T, masked -> A2 (and F2); dW1; encode ----
+        // ---- 8. g_H0^T tile = W1[rows] g_H1^T, masked -> A2 (and F2); dW1; encode ----
+        // (32 features x 16 samples per wave: half the g_H1 reads of 16 x 32)
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int mt = 0; mt < 2; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) {
-            const bf16x8 a = Bw1r[kt];
+            const int go = (16 * sh + r16) * SM + 32 * kt + 8 * g;
+            const bf16x8 bh = ld_frag(&sm.A1h[go]), bl = ld_frag(&sm.A1l[go]);
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                const int go = (16 * nt + r16) * SM + 32 * kt + 8 * g;
-                acc[nt] = mfma_w(a, ld_frag(&sm.A1h[go]), ld_frag(&sm.A1l[go]), acc[nt]);
-            }
+            for (int mt = 0; mt < 2; ++mt) acc[mt] = mfma_w(Bw1r[mt][kt], bh, bl, acc[mt]);
         }
         {
             // dW1[in][out] += sum over samples H0[s][in] g_H1[s][out]; this wave's
@@ -834,13 +840,18 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
                 dW1[2 * mi + 1] = mfma16x16x32(a, b1, dW1[2 * mi + 1]);
             }
         }
-        // A2 is free and F2 was last read in step 7 (before its barrier): both take g_H0
+        // A2 is free and F2 was last read in step 7 (before its barrier): both take
+        // g_H0, masked by the layer-0 word of features 32 fb .. +31
+        {
+            const uint32_t mw = sm.relu[cb][8 * (16 * sh + r16) + fb];
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            float gv[4];
+            for (int mt = 0; mt < 2; ++mt) {
+                float gv[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) gv[i] = (m0 >> (nt * 4 + i)) & 1u ? acc[nt][i] : 0.f;
-            st_act<true>(sm.A2h, sm.A2l, sm.F2, 16 * nt + r16, fr, lane, gv);
+                for (int i = 0; i < 4; ++i)
+                    gv[i] = (mw >> (16 * mt + 4 * g + i)) & 1u ? acc[mt][i] : 0.f;
+                st_act<true>(sm.A2h, sm.A2l, sm.F2, 16 * sh + r16, 32 * fb + 16 * mt + 4 * g, lane, gv);
+            }
         }
         park(cb ^ 1);  // the next group's stage-B loads (issued after step 4) have landed
         __syncthreads();
