@@ -44,18 +44,12 @@ constexpr int NT = GS / 16;  // 16-sample MFMA tiles per group
 constexpr int XM = 72;       // sample-major stride of X (64 + 8)
 constexpr int SM = 136;      // sample-major stride of 128-wide activations (128 + 8)
 constexpr int GM = 24;       // sample-major stride of dL/dlogit (16 + 8)
-constexpr int FM = 36;       // feature-major stride (32 samples + 4; rows 8-byte aligned)
+constexpr int FM = 40;       // feature-major stride (32 samples + 8; rows 16-byte aligned)
 constexpr int XF_ROWS = 48;  // feature-major X rows: inputs 0..39 are used
 
 __device__ __forceinline__ bf16x8 ld_frag(const uint16_t* p) {  // 16-byte aligned
     const uint4 v = *reinterpret_cast<const uint4*>(p);
     return __builtin_bit_cast(bf16x8, v);
-}
-
-__device__ __forceinline__ bf16x8 ld_frag8(const uint16_t* p) {  // 8-byte aligned
-    const uint2 a = *reinterpret_cast<const uint2*>(p);
-    const uint2 b = *reinterpret_cast<const uint2*>(p + 4);
-    return __builtin_bit_cast(bf16x8, make_uint4(a.x, a.y, b.x, b.y));
 }
 
 __device__ __forceinline__ s16x4 ld_frag4(const uint16_t* p) {
@@ -811,8 +805,8 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
                 st_act<true>(sm.A1h, sm.A1l, sm.G1f, 16 * nt + r16, fr, lane, gv);
             }
             // dW2[in = f0 + row][out] += sum over the group's samples H1[s][in] gL[s][out]
-            dW2 = mfma16x16x32(ld_frag8(&sm.F2[(f0 + r16) * FM + 8 * g]),
-                               ld_frag8(&sm.GLf[cb][r16 * FM + 8 * g]), dW2);
+            dW2 = mfma16x16x32(ld_frag(&sm.F2[(f0 + r16) * FM + 8 * g]),
+                               ld_frag(&sm.GLf[cb][r16 * FM + 8 * g]), dW2);
         }
         __syncthreads();
         // ---- 8. g_H0^T tile = W1[rows] g_H1^T, masked -> A2 (and F2); dW1; encode ----
@@ -831,11 +825,11 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
             // block: inputs 64 (wv & 1) .. +63, outputs 32 (wv >> 1) .. +31 (fewer
             // LDS reads than a full column of 128 inputs x 16 outputs)
             const int i0 = 64 * (wv & 1), o0 = 32 * (wv >> 1);
-            const bf16x8 b0 = ld_frag8(&sm.G1f[(o0 + r16) * FM + 8 * g]);
-            const bf16x8 b1 = ld_frag8(&sm.G1f[(o0 + 16 + r16) * FM + 8 * g]);
+            const bf16x8 b0 = ld_frag(&sm.G1f[(o0 + r16) * FM + 8 * g]);
+            const bf16x8 b1 = ld_frag(&sm.G1f[(o0 + 16 + r16) * FM + 8 * g]);
 #pragma unroll
             for (int mi = 0; mi < 4; ++mi) {
-                const bf16x8 a = ld_frag8(&sm.H0f[(i0 + 16 * mi + r16) * FM + 8 * g]);
+                const bf16x8 a = ld_frag(&sm.H0f[(i0 + 16 * mi + r16) * FM + 8 * g]);
                 dW1[2 * mi] = mfma16x16x32(a, b0, dW1[2 * mi]);
                 dW1[2 * mi + 1] = mfma16x16x32(a, b1, dW1[2 * mi + 1]);
             }
@@ -857,10 +851,10 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
         __syncthreads();
         // ---- 9. dW0 columns of this wave; g_X^T (waves 0-1) -> record gradients ----
         {
-            const bf16x8 bh = ld_frag8(&sm.F2[(f0 + r16) * FM + 8 * g]);
+            const bf16x8 bh = ld_frag(&sm.F2[(f0 + r16) * FM + 8 * g]);
 #pragma unroll
             for (int mi = 0; mi < 3; ++mi)
-                dW0[mi] = mfma16x16x32(ld_frag8(&sm.Xf[cb][(16 * mi + r16) * FM + 8 * g]), bh, dW0[mi]);
+                dW0[mi] = mfma16x16x32(ld_frag(&sm.Xf[cb][(16 * mi + r16) * FM + 8 * g]), bh, dW0[mi]);
         }
         if (wv < NT) {
             const int nt = wv;
