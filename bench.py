@@ -229,9 +229,9 @@ def bench_batched(args, scene):
 # x1.0); profiles/pmc_traffic_c2_train_c5_render.md holds the runs.
 TRAFFIC_SOURCE = "profiles/pmc_traffic_c2_train_c5_render.md"
 MEASURED_TRAFFIC_MB = {
-    "c2": {"mlp_bwd": 59.67, "scatter": 64.87, "mlp_fwd": 64.28, "ray_march": 14.09,
-           "dw_reduce": 33.00, "composite": 15.28, "adam": 107.13, "ray_count": 2.30},
-    "c5": {"render": 112.07},
+    "c2": {"mlp_bwd": 59.67, "scatter": 64.86, "mlp_fwd": 64.29, "ray_march": 14.09,
+           "dw_reduce": 33.00, "composite": 15.24, "adam": 107.12, "ray_count": 2.23},
+    "c5": {"render": 111.83},
 }
 
 
