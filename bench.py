@@ -260,7 +260,7 @@ def stage_models(r):
         # one dW partial per CU in, MLP gradient read+written; with the MLP
         # Adam: parameters and both moments read+written
         "dw_reduce": ("hbm", r["n_cu"] * P * 4 + 2 * P * 4 + 6 * P * 4),
-        # survey B_adam: density p,m,v r/w + g r; colour p,m,v r/w + g r
+        # B_adam: density p,m,v r/w + g r; colour p,m,v r/w + g r
         "adam": ("hbm", V_u * (28 + 12 * (2 * sc + 20)) + L_t * 512 * 13 * 4),
     }
 

@@ -94,12 +94,12 @@ __device__ __forceinline__ uint32_t dpp(uint32_t v) {  // quad permutation, all 
 // 2-byte stores per lane.
 __device__ __forceinline__ uint2 quad_transpose(uint2 h, int q) {
     const uint32_t v0 = h.x & 0xffffu, v1 = h.x >> 16, v2 = h.y & 0xffffu, v3 = h.y >> 16;
-    // round 1 (lanes q, q^1): even lanes gather rows 0/2, odd lanes rows 1/3, of the pair
+    // exchange 1 (lanes q, q^1): even lanes gather rows 0/2, odd lanes rows 1/3, of the pair
     const uint32_t send = (q & 1) ? (v0 | (v2 << 16)) : (v1 | (v3 << 16));
     const uint32_t recv = dpp<0xB1>(send);  // quad_perm [1,0,3,2]
     const uint32_t A = (q & 1) ? ((recv & 0xffffu) | (v1 << 16)) : (v0 | (recv << 16));
     const uint32_t B = (q & 1) ? ((recv >> 16) | (v3 << 16)) : (v2 | (recv & 0xffff0000u));
-    // round 2 (lanes q, q^2): rows (q & 1) and (q & 1) + 2 of the two pairs
+    // exchange 2 (lanes q, q^2): rows (q & 1) and (q & 1) + 2 of the two pairs
     const uint32_t send2 = (q & 2) ? A : B;
     const uint32_t recv2 = dpp<0x4E>(send2);  // quad_perm [2,3,0,1]
     return (q & 2) ? make_uint2(recv2, B) : make_uint2(A, recv2);
