@@ -510,7 +510,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
 }
 
 // Backward weights as MFMA A fragments in LDS, [tile][k-step][lane] (each
-// wave's W1 rows, 16 x 128, sit in its registers):
+// wave's W1 rows for the g_H0 product, 32 x 128, sit in its registers):
 //   W2  [8 row tiles of 16 hidden][64] x 8 bytes   (K = 16 logits, 3 used)
 //   W0  [4 k-steps of 32 hidden][64] x 16 bytes    (rows: the 12 features)
 constexpr int kBw2Frags = 8 * 64, kBw0Frags = 4 * 64;
