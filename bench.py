@@ -429,6 +429,44 @@ def bench_render(frames: int, warmup: int):
     }
 
 
+def bench_tv(cfg_name: str, reps: int = 20):
+    """VDB total-variation gradient (pvdb_tv_add_grad, dense mode) on the
+    workload's density (1 channel) and colour (12 channel) grids: average launch
+    time over `reps` launches with HIP events on the launch stream.  Algorithmic
+    bytes per slot in the box: its values read, its gradient read and written
+    (12 B per channel); neighbour values are counted once, as own values."""
+    import torch
+
+    from plenvdb_amd import scenes
+    from plenvdb_amd.plenvdb import ColorVDB, DensityVDB
+    from plenvdb_amd.topology import Topology
+
+    cfg = scenes.CONFIGS[cfg_name]
+    topo = Topology.from_mask(scenes.occupancy_mask(cfg))
+    reso = [cfg.reso] * 3
+    out = {"workload": f"{cfg_name}: {cfg.reso}^3 grid, {topo.nslots // 512} leaves, dense mode"}
+    for name, vdb in (("density", DensityVDB(reso, 1, topology=topo)),
+                      ("color", ColorVDB(reso, 12, topology=topo))):
+        g = torch.Generator(device="cuda").manual_seed(0)
+        vdb.grid.copy_(torch.randn(vdb.grid.shape, device="cuda", generator=g) * 0.5)
+        vdb.grad.copy_(torch.randn(vdb.grad.shape, device="cuda", generator=g))
+        for _ in range(3):
+            vdb.tv_add_grad(1.0, 1.0, 1.0, True)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            vdb.tv_add_grad(1.0, 1.0, 1.0, True)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        nbytes = topo.nslots * 12 * vdb.ndim
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out[name] = {"avg_launch_ms": round(ms, 5), "bytes": nbytes, "achieved": round(gbs, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5)}
+    return out
+
+
 def cpu_baseline(cfg_name: str, n_rays: int, threads: int | None = None):
     """Oracle FP32 restatement of the reference step (C kernels under OpenMP,
     MLP/composite/loss in torch CPU), forward + backward + Adam, timed on the
@@ -543,6 +581,7 @@ def main():
                                                  / res["plain"]["samples_per_step"], 3)
     if args.render_frames > 0:
         line["render"] = bench_render(args.render_frames, 3)
+    line["tv_add_grad"] = bench_tv(args.config)
     others = [c for c in args.configs.split(",") if c and c != args.config]
     if others:
         line["configs"] = {c: bench_config(c, 5, 2) for c in others}

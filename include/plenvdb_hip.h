@@ -94,6 +94,15 @@ int pvdb_copy_to_dense(const pvdb_topology* topo, const float* vals, int32_t cha
 /* setValuesOn_bymask (densityvdb.cu:64-97; plenvdb.cpp:7-9). */
 int pvdb_set_values_by_mask(const pvdb_topology* topo, float* vals, const uint8_t* mask /*dev*/,
                             float value, int32_t rx, int32_t ry, int32_t rz, void* stream);
+/* Total-variation gradient added into grad [nslots][channels]: the dense
+ * kernel total_variation_add_grad_cuda_kernel (total_variation_kernel.cu:13-35,
+ * weights divided by 6 as in :45-47; the x axis takes wz there and here) on the
+ * grid's dense view over [0,rx)x[0,ry)x[0,rz), for every slot in that box.
+ * Replaces the VDB entry the reference stubs (DensityVDB/ColorVDB::
+ * total_variation_add_grad, plenvdb.h:497-501,574-578). */
+int pvdb_tv_add_grad(const pvdb_topology* topo, const float* vals /*dev*/, float* grad /*dev*/,
+                     int32_t channels, int32_t rx, int32_t ry, int32_t rz, float wx, float wy,
+                     float wz, int32_t dense_mode, void* stream);
 /* zero_grad (densityvdb.cu:353-368, colorvdb.cu:355-371; plenvdb.cpp:94,120). */
 int pvdb_zero_active(const pvdb_topology* topo, float* grad, int32_t channels, void* stream);
 /* DensityOpt/ColorOpt::step_optimizer kernels (plenvdb.h:751-789;

@@ -394,6 +394,61 @@ def adam_stepsz(lr, beta0, beta1, step):
 
 
 # ----------------------------------------------------------------------------
+# Total-variation gradient
+# ----------------------------------------------------------------------------
+def tv_add_grad_dense(param, grad, wx, wy, wz, dense_mode):
+    """Dense TV gradient (total_variation_kernel.cu:13-35, host scaling :45-47)
+    in FP32: param/grad are [C, X, Y, Z]; returns the new grad.  Each element
+    whose grad is nonzero (or every element in dense mode) gains
+    sum over its six neighbours of w * clamp(p - p_nb, -1, 1), in the order
+    z-1, z+1, y-1, y+1, x-1, x+1, with w = weight / 6 and no term past the
+    border.  The reference weighs the x axis with wz, not wx (lines 31-32);
+    that is kept, so wx has no effect."""
+    f = np.float32
+    p = np.asarray(param, dtype=f)
+    g = np.array(grad, dtype=f)
+    wy6, wz6 = f(wy) / f(6), f(wz) / f(6)
+    gta = np.zeros_like(p)
+    for w, axis in ((wz6, 3), (wy6, 2), (wz6, 1)):
+        n = p.shape[axis]
+        lo = [slice(None)] * 4
+        hi = [slice(None)] * 4
+        lo[axis], hi[axis] = slice(0, n - 1), slice(1, n)
+        lo, hi = tuple(lo), tuple(hi)
+        prev = np.zeros_like(p)  # p - p[index - 1] where that neighbour exists
+        prev[hi] = p[hi] - p[lo]
+        nxt = np.zeros_like(p)  # p - p[index + 1]
+        nxt[lo] = p[lo] - p[hi]
+        gta = gta + w * np.clip(prev, f(-1), f(1))
+        gta = gta + w * np.clip(nxt, f(-1), f(1))
+    upd = np.ones_like(g, dtype=bool) if dense_mode else g != 0
+    return np.where(upd, g + gta, g).astype(f)
+
+
+def tv_add_grad_vdb(dom: SlotDomain, vals, grad, reso, wx, wy, wz, dense_mode):
+    """The dense rule above on a VDB payload: every slot whose voxel lies in
+    [0, reso) takes the dense kernel's update on the grid's dense view
+    (stored value in an existing leaf, 0 elsewhere, as get_dense_grid).
+    vals/grad are [nslots, C]; returns the new grad."""
+    f = np.float32
+    vals = np.asarray(vals, dtype=f)
+    grad = np.asarray(grad, dtype=f)
+    C_ = vals.shape[1]
+    R = [int(r) for r in reso]
+    xyz = dom.coords()
+    inside = np.all((xyz >= 0) & (xyz < np.array(R)), axis=1)
+    x, y, z = xyz[inside].T
+    dense = np.zeros((C_, *R), dtype=f)
+    gd = np.zeros((C_, *R), dtype=f)
+    dense[:, x, y, z] = vals[inside].T
+    gd[:, x, y, z] = grad[inside].T
+    new = tv_add_grad_dense(dense, gd, wx, wy, wz, dense_mode)
+    out = grad.copy()
+    out[inside] = new[:, x, y, z].T
+    return out
+
+
+# ----------------------------------------------------------------------------
 # Composed reference training step
 # ----------------------------------------------------------------------------
 def positional_encoding(viewdirs, n_freq=4):

@@ -5,6 +5,8 @@
 // These are the kernels behind the reference-compatible plenvdb API (one
 // launch per call, reference operation order).  The fused training step lives
 // in fused_step.hip.
+#include <type_traits>
+
 #include "pvdb_common.h"
 #include "gather.h"
 #include "plenvdb_hip.h"
@@ -171,6 +173,78 @@ __global__ void k_zero_active(TopoView t, float* __restrict__ grad, int C) {
     for (int c = 0; c < C; ++c) grad[s * C + c] = 0.f;
 }
 
+// Total-variation gradient of a VDB payload: the dense kernel's rule
+// (total_variation_kernel.cu:13-35) on the grid's dense view over
+// [0,rx)x[0,ry)x[0,rz) (stored value in an existing leaf, 0 elsewhere), applied
+// to every slot inside that box.  Terms in the dense kernel's order z-1, z+1,
+// y-1, y+1, x-1, x+1; the x axis takes wz, as there (lines 31-32).  One thread
+// per (slot, V consecutive channels), so neighbouring threads read neighbouring
+// 4V-byte pieces; a thread reads its own and its neighbours' values and writes
+// only its own gradient, so no two threads touch one element.
+template <int V>
+__global__ void __launch_bounds__(256) k_tv_add_grad(TopoView t, const float* __restrict__ vals,
+                                                     float* __restrict__ grad, int C, int rx,
+                                                     int ry, int rz, float wy6, float wz6,
+                                                     int dense_mode) {
+    const int G = C / V;  // channel groups per slot
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tid >= t.nleaf * 512 * G) return;
+    const int64_t s = tid / G;
+    const int q = (int)(tid - s * G);
+    int x, y, z;
+    slot_coord(t, s, x, y, z);
+    if (x < 0 || y < 0 || z < 0 || x >= rx || y >= ry || z >= rz) return;
+    using Vec = std::conditional_t<V == 4, float4, float>;
+    auto ld = [&](const float* p, float* o) {
+        const Vec w = *reinterpret_cast<const Vec*>(p);
+        if constexpr (V == 4) {
+            o[0] = w.x; o[1] = w.y; o[2] = w.z; o[3] = w.w;
+        } else {
+            o[0] = w;
+        }
+    };
+    const int64_t e = s * C + (int64_t)q * V;
+    float g0[V];
+    ld(grad + e, g0);
+    bool any = dense_mode != 0;
+#pragma unroll
+    for (int c = 0; c < V; ++c) any = any || g0[c] != 0.f;
+    if (!any) return;
+    // neighbour slots: -2 past the box (no term), -1 in a missing leaf (value 0)
+    int64_t nb[6];
+    nb[0] = z > 0 ? pvdb::slot_of(t, x, y, z - 1) : -2;
+    nb[1] = z < rz - 1 ? pvdb::slot_of(t, x, y, z + 1) : -2;
+    nb[2] = y > 0 ? pvdb::slot_of(t, x, y - 1, z) : -2;
+    nb[3] = y < ry - 1 ? pvdb::slot_of(t, x, y + 1, z) : -2;
+    nb[4] = x > 0 ? pvdb::slot_of(t, x - 1, y, z) : -2;
+    nb[5] = x < rx - 1 ? pvdb::slot_of(t, x + 1, y, z) : -2;
+    const float w[6] = {wz6, wz6, wy6, wy6, wz6, wz6};
+    float v[V], acc[V];
+    ld(vals + e, v);
+#pragma unroll
+    for (int c = 0; c < V; ++c) acc[c] = 0.f;
+#pragma unroll
+    for (int n = 0; n < 6; ++n) {
+        if (nb[n] == -2) continue;
+        float u[V];
+        if (nb[n] >= 0) {
+            ld(vals + nb[n] * C + (int64_t)q * V, u);
+        } else {
+#pragma unroll
+            for (int c = 0; c < V; ++c) u[c] = 0.f;
+        }
+#pragma unroll
+        for (int c = 0; c < V; ++c) acc[c] += w[n] * fminf(fmaxf(v[c] - u[c], -1.f), 1.f);
+    }
+    float r[V];
+#pragma unroll
+    for (int c = 0; c < V; ++c) r[c] = (dense_mode || g0[c] != 0.f) ? g0[c] + acc[c] : g0[c];
+    if constexpr (V == 4)
+        *reinterpret_cast<float4*>(grad + e) = make_float4(r[0], r[1], r[2], r[3]);
+    else
+        grad[e] = r[0];
+}
+
 // VDB Adam over all slots, one thread per (slot, group of `group` channels)
 // (densityvdb.cu:185-349, colorvdb.cu:179-351).
 __global__ void k_vdb_adam(TopoView t, float* __restrict__ param, const float* __restrict__ grad,
@@ -333,6 +407,32 @@ int pvdb_set_values_by_mask(const pvdb_topology* topo, float* vals, const uint8_
     hipLaunchKernelGGL(k_set_by_mask, dim3(pvdb_blocks(t.nleaf * 512, 256)), dim3(256), 0,
                        (hipStream_t)stream, t, vals, mask, value, rx, ry, rz);
     PVDB_LAUNCHED("k_set_by_mask");
+    return PVDB_OK;
+}
+
+int pvdb_tv_add_grad(const pvdb_topology* topo, const float* vals, float* grad,
+                     int32_t channels, int32_t rx, int32_t ry, int32_t rz, float wx, float wy,
+                     float wz, int32_t dense_mode, void* stream) {
+    if (int rc = check_topo(topo)) return rc;
+    PVDB_CHECK_ARG(channels >= 1 && rx > 0 && ry > 0 && rz > 0, "bad sizes");
+    PVDB_CHECK_ARG(vals && grad, "null pointer");
+    (void)wx;  // the dense kernel weighs the x axis with wz
+    const TopoView t = pvdb::view_of(topo);
+    if (t.nleaf == 0) return PVDB_OK;
+    // weights scaled on the host as total_variation_kernel.cu:45-47 does; four
+    // channels per thread when the row splits into 16-byte pieces
+    const bool vec4 = channels % 4 == 0 && reinterpret_cast<uintptr_t>(vals) % 16 == 0 &&
+                      reinterpret_cast<uintptr_t>(grad) % 16 == 0;
+    const int64_t nthreads = t.nleaf * 512 * (vec4 ? channels / 4 : channels);
+    if (vec4)
+        hipLaunchKernelGGL(k_tv_add_grad<4>, dim3(pvdb_blocks(nthreads, 256)), dim3(256), 0,
+                           (hipStream_t)stream, t, vals, grad, channels, rx, ry, rz, wy / 6.f,
+                           wz / 6.f, dense_mode != 0 ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_tv_add_grad<1>, dim3(pvdb_blocks(nthreads, 256)), dim3(256), 0,
+                           (hipStream_t)stream, t, vals, grad, channels, rx, ry, rz, wy / 6.f,
+                           wz / 6.f, dense_mode != 0 ? 1 : 0);
+    PVDB_LAUNCHED("k_tv_add_grad");
     return PVDB_OK;
 }
 

@@ -39,6 +39,7 @@ _SIGS = {
     "pvdb_copy_to_dense": [P, P, I32, P, I32, I32, I32, F32, P],
     "pvdb_set_values_by_mask": [P, P, P, F32, I32, I32, I32, P],
     "pvdb_zero_active": [P, P, I32, P],
+    "pvdb_tv_add_grad": [P, P, P, I32, I32, I32, I32, F32, F32, F32, I32, P],
     "pvdb_vdb_adam": [P, P, P, P, P, P, I32, I32, I32, F32, F32, F32, F32, P],
     "pvdb_dense_adam": [P, P, P, P, P, I64, I32, F32, F32, F32, F32, P],
     "pvdb_rays_of_view": [I32, I32, P, P, I32, I32, I32, I32, I32, P, P, P, P],

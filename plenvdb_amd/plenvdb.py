@@ -144,6 +144,16 @@ class _BaseVDB:
         """The reference stubs this out (plenvdb.h:497-501,574-578)."""
         print("Not Supported Now...")
 
+    def tv_add_grad(self, wx, wy, wz, dense_mode) -> None:
+        """A working VDB total-variation gradient, added into `grad`: the dense
+        kernel's rule (total_variation_kernel.cu:13-35, weights / 6 as :45-47,
+        the x axis weighed by wz as there) on the grid's dense view, for every
+        voxel with storage inside `reso`.  Opt-in: the reference's
+        total_variation_add_grad above stays the no-op it is there."""
+        lib.call("pvdb_tv_add_grad", self.topo.handle, lib.ptr(self.grid), lib.ptr(self.grad),
+                 self.ndim, *self.reso, float(wx), float(wy), float(wz), int(bool(dense_mode)),
+                 lib.stream())
+
     # --- persistence -------------------------------------------------------------
     def save_to(self, path: str) -> None:
         """Writes the grid (plenvdb.h:143-148,269-276) in the container format of vdbio."""

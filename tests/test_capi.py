@@ -75,6 +75,8 @@ def test_new_entry_points_check_arguments():
     with pytest.raises(RuntimeError, match="null pointer"):
         lib.call("pvdb_alpha2weight_bwd_f64", None, None, None, None, None, None, 3, None, None,
                  None, None)
+    with pytest.raises(RuntimeError, match="null topology"):
+        lib.call("pvdb_tv_add_grad", None, None, None, 1, 4, 4, 4, 1.0, 1.0, 1.0, 1, None)
     from plenvdb_amd.fused import StepDesc
 
     d = StepDesc()

@@ -431,3 +431,96 @@ def test_render_constant_field_closed_form(oracle_mod, sigma, t_stop):
     assert (kept[clear] == 0).any() and (kept[clear] > 0).sum() > H * W // 4
     assert np.array_equal(info["count"][clear], kept[clear])
     assert np.abs(img.reshape(-1, 3)[clear] - want[clear]).max() < 1e-6
+
+
+# ---- total-variation gradient: closed forms pin the dense restatement ----
+def _tv(p, g, w, dense=True):
+    from oracle import oracle
+
+    return oracle.tv_add_grad_dense(p, g, *w, dense)
+
+
+def test_tv_constant_grid_adds_nothing():
+    p = np.full((2, 5, 6, 7), 0.37, np.float32)
+    g = np.random.default_rng(0).normal(size=p.shape).astype(np.float32)
+    np.testing.assert_array_equal(_tv(p, g, (1.0, 2.0, 3.0)), g)
+
+
+@pytest.mark.parametrize("h", [0.3, -0.25, 5.0])
+def test_tv_spike_closed_form(h):
+    """One interior spike of height h on zeros: the spike gains 2 (wz + wy +
+    wz)/6 clamp(h) and each face neighbour -w/6 clamp(h) with w = wz on the
+    z and x axes (the reference's x-axis weight) and wy on y."""
+    wx, wy, wz = 0.7, 1.1, 1.3
+    p = np.zeros((1, 5, 5, 5), np.float32)
+    p[0, 2, 2, 2] = h
+    out = _tv(p, np.zeros_like(p), (wx, wy, wz))
+    c = float(np.clip(h, -1, 1))
+    exp = np.zeros_like(p)
+    exp[0, 2, 2, 2] = (4 * wz + 2 * wy) / 6 * c
+    for (dx, dy, dz), w in (((1, 0, 0), wz), ((0, 1, 0), wy), ((0, 0, 1), wz)):
+        exp[0, 2 + dx, 2 + dy, 2 + dz] = -w / 6 * c
+        exp[0, 2 - dx, 2 - dy, 2 - dz] = -w / 6 * c
+    np.testing.assert_allclose(out, exp, rtol=1e-6, atol=1e-7)
+    assert _tv(p, np.zeros_like(p), (99.0, wy, wz))[0, 3, 2, 2] == out[0, 3, 2, 2]  # wx unused
+
+
+def test_tv_corner_spike_and_gate():
+    """A corner spike has three neighbours; without dense mode only elements
+    with a nonzero gradient change."""
+    p = np.zeros((1, 3, 3, 3), np.float32)
+    p[0, 0, 0, 0] = 0.5
+    out = _tv(p, np.zeros_like(p), (1.0, 1.0, 1.0))
+    assert abs(out[0, 0, 0, 0] - 3 * 0.5 / 6) < 1e-7
+    assert abs(out[0, 1, 0, 0] + 0.5 / 6) < 1e-7 and out[0, 1, 1, 0] == 0
+    g = np.zeros_like(p)
+    g[0, 1, 0, 0] = 2.0
+    gated = _tv(p, g, (1.0, 1.0, 1.0), dense=False)
+    assert gated[0, 0, 0, 0] == 0 and abs(gated[0, 1, 0, 0] - (2.0 - 0.5 / 6)) < 1e-6
+    assert np.count_nonzero(gated) == 1
+
+
+def test_tv_brute_force_small():
+    """Element loop over a small random grid (border, clamp and gate all hit)."""
+    rng = np.random.default_rng(3)
+    p = (rng.normal(size=(2, 3, 4, 5)) * 0.8).astype(np.float32)
+    g = rng.normal(size=p.shape).astype(np.float32) * (rng.random(p.shape) < 0.5)
+    w = (0.3, 0.6, 0.9)
+    out = _tv(p, g, w, dense=False)
+    exp = g.astype(np.float64).copy()
+    for c, i, j, k in np.ndindex(*p.shape):
+        if g[c, i, j, k] == 0:
+            continue
+        acc = 0.0
+        for (di, dj, dk), ww in (((0, 0, 1), w[2]), ((0, 1, 0), w[1]), ((1, 0, 0), w[2])):
+            for sgn in (-1, 1):
+                ii, jj, kk = i + sgn * di, j + sgn * dj, k + sgn * dk
+                if 0 <= ii < p.shape[1] and 0 <= jj < p.shape[2] and 0 <= kk < p.shape[3]:
+                    acc += ww / 6 * np.clip(float(p[c, i, j, k]) - float(p[c, ii, jj, kk]), -1, 1)
+        exp[c, i, j, k] += acc
+    np.testing.assert_allclose(out, exp, rtol=1e-6, atol=1e-6)
+
+
+def test_tv_vdb_view_matches_dense_on_full_cover():
+    """A topology covering the box with leaves: the VDB form equals the dense
+    form slot by slot (pins the slot <-> voxel mapping of the VDB form)."""
+    from oracle import oracle
+
+    R = (16, 8, 12)
+    mask = np.ones(R, bool)
+    origins, vmask = oracle.leaves_from_mask(mask)
+    dom = oracle.SlotDomain(origins, vmask)
+    rng = np.random.default_rng(5)
+    vals = (rng.normal(size=(dom.nslots, 3)) * 0.7).astype(np.float32)
+    grad = rng.normal(size=(dom.nslots, 3)).astype(np.float32)
+    got = oracle.tv_add_grad_vdb(dom, vals, grad, R, 1.0, 0.5, 2.0, True)
+    xyz = dom.coords()
+    inside = np.all(xyz < np.array(R), axis=1)
+    dense = np.zeros((3, *R), np.float32)
+    gd = np.zeros((3, *R), np.float32)
+    x, y, z = xyz[inside].T
+    dense[:, x, y, z] = vals[inside].T
+    gd[:, x, y, z] = grad[inside].T
+    ref = oracle.tv_add_grad_dense(dense, gd, 1.0, 0.5, 2.0, True)
+    np.testing.assert_array_equal(got[inside], ref[:, x, y, z].T)
+    np.testing.assert_array_equal(got[~inside], grad[~inside])
