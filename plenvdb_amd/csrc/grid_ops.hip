@@ -210,14 +210,17 @@ __global__ void __launch_bounds__(256) k_tv_add_grad(TopoView t, const float* __
 #pragma unroll
     for (int c = 0; c < V; ++c) any = any || g0[c] != 0.f;
     if (!any) return;
-    // neighbour slots: -2 past the box (no term), -1 in a missing leaf (value 0)
+    // neighbour slots: -2 past the box (no term), -1 in a missing leaf (value 0);
+    // inside this leaf the neighbour is slot +-1, +-8, +-64 (offset x<<6|y<<3|z),
+    // only a leaf face goes through the leaf table
+    const int lx = x & 7, ly = y & 7, lz = z & 7;
     int64_t nb[6];
-    nb[0] = z > 0 ? pvdb::slot_of(t, x, y, z - 1) : -2;
-    nb[1] = z < rz - 1 ? pvdb::slot_of(t, x, y, z + 1) : -2;
-    nb[2] = y > 0 ? pvdb::slot_of(t, x, y - 1, z) : -2;
-    nb[3] = y < ry - 1 ? pvdb::slot_of(t, x, y + 1, z) : -2;
-    nb[4] = x > 0 ? pvdb::slot_of(t, x - 1, y, z) : -2;
-    nb[5] = x < rx - 1 ? pvdb::slot_of(t, x + 1, y, z) : -2;
+    nb[0] = z > 0 ? (lz != 0 ? s - 1 : pvdb::slot_of(t, x, y, z - 1)) : -2;
+    nb[1] = z < rz - 1 ? (lz != 7 ? s + 1 : pvdb::slot_of(t, x, y, z + 1)) : -2;
+    nb[2] = y > 0 ? (ly != 0 ? s - 8 : pvdb::slot_of(t, x, y - 1, z)) : -2;
+    nb[3] = y < ry - 1 ? (ly != 7 ? s + 8 : pvdb::slot_of(t, x, y + 1, z)) : -2;
+    nb[4] = x > 0 ? (lx != 0 ? s - 64 : pvdb::slot_of(t, x - 1, y, z)) : -2;
+    nb[5] = x < rx - 1 ? (lx != 7 ? s + 64 : pvdb::slot_of(t, x + 1, y, z)) : -2;
     const float w[6] = {wz6, wz6, wy6, wy6, wz6, wz6};
     float v[V], acc[V];
     ld(vals + e, v);
