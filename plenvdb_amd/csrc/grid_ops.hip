@@ -187,7 +187,13 @@ __global__ void __launch_bounds__(256) k_tv_add_grad(TopoView t, const float* __
                                                      int ry, int rz, float wy6, float wz6,
                                                      int dense_mode) {
     const int G = C / V;  // channel groups per slot
-    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // XCD-aware block order: workgroups are dealt to the 8 XCDs in turn, so
+    // block b of the first 8*(n/8) is given the run (b % 8)*(n/8) + b/8 and each
+    // XCD's L2 holds a contiguous range of slots with their in-leaf neighbours
+    // (a bijection; the last n % 8 blocks keep their index)
+    const unsigned nblk = gridDim.x, per = nblk / 8, bid = blockIdx.x;
+    const unsigned lblk = bid < 8 * per ? (bid % 8) * per + bid / 8 : bid;
+    const int64_t tid = (int64_t)lblk * blockDim.x + threadIdx.x;
     if (tid >= t.nleaf * 512 * G) return;
     const int64_t s = tid / G;
     const int q = (int)(tid - s * G);
