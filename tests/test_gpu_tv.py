@@ -44,6 +44,19 @@ def test_tv_add_grad_vs_oracle_c1(C, dense, oracle_mod):
     _case(topo, (128, 128, 128), C, dense, 11 + C, oracle_mod)
 
 
+def test_tv_add_grad_two_level_layout(oracle_mod):
+    """c1's leaves behind the two-level leaf tables: the leaf-face lookups go
+    through the lower-node table."""
+    from plenvdb_amd import scenes
+    from plenvdb_amd.topology import Topology
+
+    one = Topology.from_mask(scenes.occupancy_mask(scenes.CONFIGS["c1"]))
+    o, m = one.leaves()
+    topo = Topology.from_leaves(o, m, [128, 128, 128], layout=2)
+    assert topo.two_level
+    _case(topo, (128, 128, 128), 12, False, 21, oracle_mod)
+
+
 def test_tv_add_grad_box_not_multiple_of_8(oracle_mod):
     """Leaves reach past the box: their outside voxels keep their gradient and
     the box faces contribute no terms."""
