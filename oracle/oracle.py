@@ -627,6 +627,7 @@ def reference_render(idx_grid, den, col, mlp, c2w, K, xyz_min, xyz_max, H, W, ne
     M = int(cnt.sum())
     feat = np.zeros((M, dcol), dt)
     wts = np.zeros(M, dt)
+    aux = np.zeros((M, 4), dt)
     pix = np.zeros(M, np.int64)
     written = np.zeros(HW, np.int64)
     tl = np.ones(HW, dt)
@@ -634,11 +635,11 @@ def reference_render(idx_grid, den, col, mlp, c2w, K, xyz_min, xyz_max, H, W, ne
         _p(idx), _p(reso), _p(den), _p(col), C.c_int(dcol), _p(c2w), _p(K), _p(mn), _p(mx),
         C.c_int(H), C.c_int(W), C.c_int(int(inverse_y)), C.c_float(near), C.c_float(stepdist),
         C.c_float(act_shift), C.c_float(interval), C.c_float(fast_color_thres), _p(tmin), _p(tmax),
-        _p(cnt), _p(offs), _p(feat), _p(wts), _p(pix), _p(written), _p(tl))
+        _p(cnt), _p(offs), _p(feat), _p(wts), _p(aux), _p(pix), _p(written), _p(tl))
     valid = np.zeros(M, bool)
     for n in np.nonzero(written)[0]:
         valid[offs[n]:offs[n] + written[n]] = True
-    feat, wts, pix = feat[valid], wts[valid], pix[valid]
+    feat, wts, pix, aux = feat[valid], wts[valid], pix[valid], aux[valid]
     w0 = np.asarray(mlp["w0"], dt)
     h = feat @ w0[:dcol] + (pe @ w0[dcol:])[pix] + np.asarray(mlp["b0"], dt)
     h = np.maximum(h, 0)
@@ -647,4 +648,6 @@ def reference_render(idx_grid, den, col, mlp, c2w, K, xyz_min, xyz_max, H, W, ne
     rgb = 1 / (1 + np.exp(-raw))
     img = np.repeat((tl * dt(bg))[:, None], 3, axis=1)
     np.add.at(img, pix, wts[:, None] * rgb)
-    return img.reshape(-1), {"count": cnt, "written": written, "t_last": tl}
+    return img.reshape(-1), {"count": cnt, "written": written, "t_last": tl, "feat": feat,
+                             "weight": wts, "pix": pix, "xyz": aux[:, :3], "alpha": aux[:, 3],
+                             "tmin": tmin, "tmax": tmax, "step": step}

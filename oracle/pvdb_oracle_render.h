@@ -165,12 +165,13 @@
     }                                                                                              \
     /* Pass 2 (ray_marching_merged, renderer.cu:312-367): re-march from the rewritten tmin   */    \
     /* to tmax, gather 12-ch features and weights of kept samples; T_last is the final T.    */    \
+    /* aux (optional, 4 per sample): the sample's index-space position and its alpha.        */    \
     void CAT(pvdb_render_fill, SFX)(                                                               \
         const int32_t* idx, const int* reso, const float* den, const float* col, int dcol,        \
         const float* c2w, const float* K, const float* mn, const float* mx, int H, int W,         \
         int inverse_y, float near_, float stepdist, float act_shift, float interval, float thres, \
         const REAL* tmin_i, const REAL* tmax_i, const int64_t* count, const int64_t* offs,        \
-        REAL* feat, REAL* wts, int64_t* pix, int64_t* written, REAL* t_last) {                    \
+        REAL* feat, REAL* wts, REAL* aux, int64_t* pix, int64_t* written, REAL* t_last) {         \
         const long HW = (long)H * W;                                                               \
         _Pragma("omp parallel for schedule(dynamic, 256)") for (long n = 0; n < HW; ++n) {         \
             GREAL o[3], d[3], step, tmin, tmax;                                                    \
@@ -219,6 +220,10 @@
                         feat[r * dcol + c] = f;                                                    \
                     }                                                                              \
                     wts[r] = wt;                                                                   \
+                    if (aux) {                                                                     \
+                        for (int a = 0; a < 3; ++a) aux[4 * r + a] = (REAL)xyz[a];                 \
+                        aux[4 * r + 3] = alpha;                                                    \
+                    }                                                                              \
                     pix[r] = n;                                                                    \
                     r++;                                                                           \
                 }                                                                                  \
