@@ -208,7 +208,8 @@ int pvdb_alpha2weight_bwd(const float* alpha, const float* weight, const float* 
  * Linear.weight.T, run.py:98-104).  Grid gradients are accumulated (callers
  * zero them once); with do_adam the step applies Adam and re-zeroes the
  * gradients of the leaves it touched.  The MLP gradient is overwritten by
- * every step. */
+ * every step.  Alignment: color 16 bytes (8 for a bf16 payload), grad_color,
+ * color_m and color_v 16 bytes, workspace 256 bytes (PVDB_EINVAL otherwise). */
 typedef struct pvdb_step_desc {
     const float* rays_o;      /* dev [n][3] */
     const float* rays_d;      /* dev [n][3] */
@@ -236,13 +237,13 @@ typedef struct pvdb_step_desc {
     float stepsz_density, stepsz_color, step_size_mlp, eps, beta0, beta1;
     int32_t do_adam;
     int32_t reserved;
-    void* workspace; /* dev, pvdb_fused_workspace_bytes() bytes */
+    void* workspace; /* dev, pvdb_fused_workspace_bytes() bytes, 256-byte aligned */
     int64_t workspace_bytes;
-    /* max records (occupancy-kept samples) per step.  Records past it are
-     * dropped: the step then covers the first `capacity` records in ray order
-     * (no memory is touched beyond it) and pvdb_fused_records' total, the full
-     * count, exceeds it.  The sum of the rays' step counts (pvdb_ray_bounds)
-     * bounds every step. */
+    /* max records (occupancy-kept samples) per step, in [1, 2^31 - 2].  Records
+     * past it are dropped: the step then covers the first `capacity` records in
+     * ray order (no memory is touched beyond it) and pvdb_fused_records' total,
+     * the full count saturated at 2^31 - 1, exceeds it.  The sum of the rays'
+     * step counts (pvdb_ray_bounds) bounds every step. */
     int64_t capacity;
     float* ray_out;   /* dev [n][4] rgb + T_last, or NULL */
     float* loss;      /* dev [1] or NULL */

@@ -124,12 +124,16 @@ __global__ void __launch_bounds__(256) k_ray_count(StepParams P, TopoView t, Ray
 // Exclusive scan of the per-ray counts by one workgroup of 1024 threads, in
 // chunks of 8192: each thread takes 8 consecutive counts (two 16-byte loads),
 // wavefronts scan their thread sums with shuffles, one LDS pass combines the
-// 16 wavefronts, and a carry runs from chunk to chunk.
+// 16 wavefronts, and a carry runs from chunk to chunk.  The sums run in 64 bits
+// and every stored offset (and the total) saturates at INT32_MAX: a batch with
+// more samples than that keeps offsets >= the record capacity (which the step
+// caps below INT32_MAX), so its excess records are dropped, never wrapped.
 __global__ void __launch_bounds__(1024) k_scan(RayBuffers R, int n) {
-    __shared__ int wsum[16];
-    __shared__ int carry_s;
+    __shared__ long long wsum[16];
+    __shared__ long long carry_s;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    int carry = 0;
+    const long long kSat = 0x7fffffffll;
+    long long carry = 0;
     for (int c0 = 0; c0 < n; c0 += 8192) {
         const int b = c0 + 8 * tid;
         int v[8];
@@ -142,25 +146,25 @@ __global__ void __launch_bounds__(1024) k_scan(RayBuffers R, int n) {
 #pragma unroll
             for (int k = 0; k < 8; ++k) v[k] = b + k < n ? R.count[b + k] : 0;
         }
-        int sum = 0;
+        long long sum = 0;
 #pragma unroll
         for (int k = 0; k < 8; ++k) sum += v[k];
-        int inc = sum;  // inclusive scan of the thread sums within the wavefront
+        long long inc = sum;  // inclusive scan of the thread sums within the wavefront
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
-            const int o = __shfl_up(inc, d, 64);
+            const long long o = __shfl_up(inc, d, 64);
             if (lane >= d) inc += o;
         }
         if (lane == 63) wsum[wv] = inc;
         __syncthreads();
-        int before = carry;
+        long long before = carry;
         for (int w = 0; w < wv; ++w) before += wsum[w];
         if (tid == 1023) carry_s = before + inc;
-        int run = before + inc - sum;
+        long long run = before + inc - sum;
         int off[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            off[k] = run;
+            off[k] = (int)(run < kSat ? run : kSat);
             run += v[k];
         }
         if (b + 8 <= n) {
@@ -175,7 +179,7 @@ __global__ void __launch_bounds__(1024) k_scan(RayBuffers R, int n) {
         carry = carry_s;
         __syncthreads();  // wsum and carry_s are reused by the next chunk
     }
-    if (tid == 0) R.total[0] = carry;
+    if (tid == 0) R.total[0] = (int)(carry < kSat ? carry : kSat);
 }
 
 // Forward march: one record per occupancy-kept sample.
@@ -226,7 +230,7 @@ __global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, con
         const bool alive_w = alive_a && (P.thres > 0.f ? w > P.thres : true);
         if (sp.keep) {
             const int64_t q = off + idx;
-            if (q < P.capacity) {
+            if (q >= 0 && q < P.capacity) {
                 uint32_t bits = (uint32_t)r | (alive_a ? kAliveA : 0u) | (alive_w ? kAliveW : 0u);
                 rec.xyze[q] = make_float4(x, y, z, alive_a ? e : 0.f);
                 rec.atwr[q] = make_float4(alive_a ? alpha : 0.f, alive_a ? T : 0.f,
@@ -263,7 +267,7 @@ __global__ void __launch_bounds__(256) k_ray_composite(StepParams P, RayBuffers 
         const int j = base + lane;
         float vr = 0.f, vg = 0.f, vb = 0.f;
         const int64_t q = off + j;
-        const bool valid = j < cnt && q < P.capacity;
+        const bool valid = j < cnt && q >= 0 && q < P.capacity;
         if (valid) {
             const float4 a = rec.atwr[q];
             if (__float_as_uint(a.w) & kAliveW) {
@@ -328,7 +332,7 @@ __global__ void __launch_bounds__(256) k_ray_composite(StepParams P, RayBuffers 
     for (int base = 0; base < cnt; base += 64) {
         const int j = base + lane;
         const int64_t q = off + j;
-        const bool valid = j < cnt && q < P.capacity;
+        const bool valid = j < cnt && q >= 0 && q < P.capacity;
         float4 a = make_float4(0.f, 0.f, 0.f, 0.f), c = a;
         float e = 0.f;
         uint32_t bits = 0u;

@@ -39,6 +39,12 @@ constexpr int kMaxMlpGrid = 512;   // workgroups of the MLP kernels at most (dW 
 constexpr int kFwdGrid = 512;      // default forward-MLP workgroups
 constexpr int kScatterChunk = 64;  // default scatter records per chunk
 constexpr int kScatterGrid = 1024; // default scatter workgroups
+// Records are indexed through int32 offsets (saturating at INT32_MAX), so a
+// capacity below that bound keeps every stored offset of a dropped record
+// outside the record arrays.
+constexpr int64_t kMaxCapacity = 0x7fffffffll;
+
+inline bool aligned(const void* p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
 
 struct Layout {
     size_t count, offset, total, t_last, ray_out, ray_grad, ray_ctot, ray_loss, loss;
@@ -286,6 +292,7 @@ int pvdb_fused_init(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
         return PVDB_EHANDLE;
     }
     PVDB_CHECK_ARG(d && d->workspace, "null descriptor or workspace");
+    PVDB_CHECK_ARG(aligned(d->workspace, 256), "workspace must be 256-byte aligned");
     const TopoView t = view_of(topo);
     const Layout L = layout_for(d->n_rays, d->capacity, t.nleaf);
     PVDB_CHECK_ARG(d->workspace_bytes >= (int64_t)L.bytes, "workspace too small");
@@ -304,7 +311,8 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     }
     PVDB_CHECK_ARG(d != nullptr, "null descriptor");
     PVDB_CHECK_ARG(d->n_rays > 0, "n_rays must be positive");
-    PVDB_CHECK_ARG(d->capacity > 0, "record capacity must be positive");
+    PVDB_CHECK_ARG(d->capacity > 0 && d->capacity < kMaxCapacity,
+                   "record capacity must be in [1, 2^31 - 2]");
     PVDB_CHECK_ARG(d->rays_o && d->rays_d && d->viewdirs && d->target, "null ray arrays");
     PVDB_CHECK_ARG(d->density && d->grad_density && d->color && d->grad_color, "null grid arrays");
     PVDB_CHECK_ARG(d->mlp && d->mlp_grad, "null MLP arrays");
@@ -321,6 +329,13 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
                        d->weight_rgbper >= 0.f && d->weight_main < INFINITY &&
                        d->weight_entropy_last < INFINITY && d->weight_rgbper < INFINITY,
                    "loss weights must be finite and >= 0");
+    // the colour payload, its gradient and moments are moved as 16-byte vectors
+    // (bf16 colour as 8-byte ones), the workspace's sub-arrays assume a
+    // 256-byte aligned base
+    PVDB_CHECK_ARG(aligned(d->grad_color, 16) && aligned(d->color_m, 16) &&
+                       aligned(d->color_v, 16) && aligned(d->color, d->color_bf16 ? 8 : 16),
+                   "colour arrays must be 16-byte aligned (8-byte for a bf16 payload)");
+    PVDB_CHECK_ARG(aligned(d->workspace, 256), "workspace must be 256-byte aligned");
     const TopoView t = view_of(topo);
     const Layout L = layout_for(d->n_rays, d->capacity, t.nleaf);
     PVDB_CHECK_ARG(d->workspace && d->workspace_bytes >= (int64_t)L.bytes, "workspace too small");

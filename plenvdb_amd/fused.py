@@ -157,7 +157,8 @@ class FusedTrainer:
         lib.call("pvdb_ray_bounds", lib.ptr(self.rays_o), lib.ptr(self.rays_d), n, lib.f3(mn),
                  lib.f3(mx), float(self.scene["near"]), 1e9, lib.ptr(self.far_per_ray),
                  self.stepdist, lib.ptr(t_min), lib.ptr(t_max), lib.ptr(n_steps), lib.stream())
-        return int(n_steps.sum().item()) + 64
+        # record offsets are int32 on the device; the step takes at most 2^31 - 2
+        return min(int(n_steps.sum().item()) + 64, 2 ** 31 - 2)
 
     def _make_desc(self) -> StepDesc:
         sc = self.scene

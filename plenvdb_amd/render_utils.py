@@ -132,7 +132,7 @@ def alpha2weight_backward(alpha, weight, T, alphainv_last, i_start, i_end, n_ray
     return grad
 
 
-def _adam_step_size(lr, beta1, beta2, step):
+def adam_step_size(lr, beta1, beta2, step):
     """lr*sqrt(1-beta2^t)/(1-beta1^t) in float (adam_upd_kernel.cu:72)."""
     import numpy as np
 
@@ -146,7 +146,7 @@ def _dense_adam(mode, param, grad, exp_avg, exp_avg_sq, perlr, step, beta1, beta
         if t.dtype != torch.float32 or not t.is_contiguous():
             raise ValueError("adam tensors must be contiguous float32")
     lib.call("pvdb_dense_adam", lib.ptr(param), lib.ptr(grad), lib.ptr(exp_avg), lib.ptr(exp_avg_sq),
-             lib.ptr(perlr), param.numel(), mode, _adam_step_size(lr, beta1, beta2, step), float(eps),
+             lib.ptr(perlr), param.numel(), mode, adam_step_size(lr, beta1, beta2, step), float(eps),
              float(beta1), float(beta2), lib.stream())
 
 

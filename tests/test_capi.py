@@ -83,3 +83,46 @@ def test_new_entry_points_check_arguments():
     with pytest.raises(RuntimeError, match="null topology"):
         lib.call("pvdb_fused_step", None, C.byref(d), None)
     assert lib.load().pvdb_fused_workspace_bytes(-1, 1, 1) == -1
+
+
+def test_fused_step_rejects_capacity_and_misaligned_buffers():
+    """pvdb_fused_step checks the record capacity (offsets are int32, so at most
+    2^31 - 2) and the alignment of the buffers it moves as 16-byte vectors
+    before it reads the topology or launches anything.  The topology argument
+    is a zero-filled stand-in buffer: none of these checks reads it."""
+    from plenvdb_amd import lib
+    from plenvdb_amd.fused import StepDesc
+
+    fake_topo = C.create_string_buffer(4096)
+    base = 1 << 20  # a 256-byte aligned fake device address, never dereferenced
+
+    def desc(**kw):
+        d = StepDesc()
+        for i, f in enumerate(("rays_o", "rays_d", "viewdirs", "target", "density",
+                               "grad_density", "color", "grad_color", "mlp", "mlp_grad",
+                               "density_m", "density_v", "color_m", "color_v", "mlp_m",
+                               "mlp_v")):
+            setattr(d, f, base + 4096 * (i + 1))
+        d.n_rays, d.capacity, d.stepdist = 16, 1024, 0.01
+        d.workspace, d.workspace_bytes = base, 1 << 40
+        d.weight_main = 1.0
+        for k, v in kw.items():
+            setattr(d, k, v)
+        return d
+
+    cases = [
+        (dict(capacity=2 ** 31 - 1), "capacity"),
+        (dict(capacity=2 ** 40), "capacity"),
+        (dict(capacity=0), "capacity"),
+        (dict(grad_color=base + 4100), "16-byte"),
+        (dict(color_m=base + 8), "16-byte"),
+        (dict(color=base + 8), "16-byte"),
+        (dict(color=base + 4, color_bf16=1), "8-byte"),
+        (dict(workspace=base + 64), "256-byte"),
+    ]
+    for kw, msg in cases:
+        with pytest.raises(RuntimeError, match=msg):
+            lib.call("pvdb_fused_step", C.addressof(fake_topo), C.byref(desc(**kw)), None)
+    with pytest.raises(RuntimeError, match="256-byte"):
+        lib.call("pvdb_fused_init", C.addressof(fake_topo), C.byref(desc(workspace=base + 16)),
+                 None)

@@ -26,9 +26,9 @@ def _case(topo, reso, C, dense, seed, oracle_mod):
     torch.cuda.synchronize()
     got = vdb.grad.cpu().numpy()
     ref = oracle_mod.tv_add_grad_vdb(dom, vals, grad, reso, *w, dense)
-    # same FP32 operations in the same order (no contraction): equal up to the
-    # sign of a zero sum
-    np.testing.assert_allclose(got, ref, rtol=1e-6, atol=1e-7)
+    # same FP32 operations in the same order (no contraction): bit-exact
+    # (-0.0 == 0.0 holds, so a zero sum's sign does not matter)
+    np.testing.assert_array_equal(got, ref)
     changed = got != grad
     assert changed.any()
     if not dense:
