@@ -142,6 +142,17 @@ def bench_batched(args, scene):
 
     from plenvdb_amd import fused
 
+    # the first forward+backward of a fresh trainer, kept for the accuracy check
+    # that runs beside the CPU baseline
+    twin = fused.FusedTrainer(scene)
+    twin.forward_backward()
+    torch.cuda.synchronize()
+    first = {"rgb": twin.ray_out[:, :3].cpu().numpy(), "loss": float(twin.loss.item()),
+             "grad_density": twin.grad_density.cpu().numpy(),
+             "grad_color": twin.grad_color.cpu().numpy(),
+             "grad_mlp": twin.mlp_grad.cpu().numpy(), "mlp": twin.mlp.cpu().numpy(),
+             "density": twin.density.cpu().numpy(), "color": twin.color.float().cpu().numpy()}
+    del twin
     tr = fused.FusedTrainer(scene)
     if args.launch:
         from plenvdb_amd import autotune
@@ -205,7 +216,8 @@ def bench_batched(args, scene):
     v_u = int(((gd != 0) & act).sum())
     tr.zero_grad()
     pts = rec["xyze"][:, :3]
-    return {
+    return {"first_step": first,
+            **{
         "ms_per_step": ms,
         "ms_per_step_eager": ms_eager,
         "ms_fwd_bwd_eager": ms_fwd_bwd,
@@ -221,7 +233,7 @@ def bench_batched(args, scene):
         "color_bytes": 2 if tr.color_bf16 else 4,
         "workload": args.config,
         "n_cu": min(512, torch.cuda.get_device_properties(0).multi_processor_count),
-    }
+    }}
 
 
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
@@ -390,6 +402,10 @@ def bench_render(frames: int, warmup: int):
                 ms["fast_color_thres"], ms["bg"], ms["inverse_y"], ms["H"], ms["W"])
     r.set_t_stop(ms["t_stop"])
     poses = ms["poses"]
+    # one synchronous render per pose: the record capacity grows to the largest frame
+    for i in range(frames):
+        r.input_a_c2w(poses[i])
+        r.render_an_image()
     for i in range(warmup):
         r.input_a_c2w(poses[i % len(poses)])
         r.render_async()
@@ -403,15 +419,25 @@ def bench_render(frames: int, warmup: int):
     torch.cuda.synchronize()
     ms_frame = e0.elapsed_time(e1) / frames
     gathers = kept = 0
+    stage_ms = np.zeros(3)
     for i in range(frames):
         r.input_a_c2w(poses[i])
         r.render_async()
         g, k = r.sample_counts()
         gathers += g
         kept += k
+    # per-kernel times (HIP events around the launches), on a second pass
+    r.stage_timing(True)
+    for i in range(frames):
+        r.input_a_c2w(poses[i])
+        r.render_async()
+    torch.cuda.synchronize()
+    stage_ms = r.stage_timing(False) / frames
     n_px = ms["H"] * ms["W"]
     flop = (kept * MLP_FWD_FLOP + frames * n_px * MLP_RAY_FLOP) / frames
     tflops = flop / (ms_frame * 1e-3) / 1e12
+    mlp_ms = float(stage_ms[1])
+    mlp_tflops = flop / (mlp_ms * 1e-3) / 1e12 if mlp_ms > 0 else 0.0
     return {
         "workload": "c5: c4 topology (320^3, 1959 leaves) as a merged index grid, fp16 data, "
                     f"800x800, focal 1111, {frames} orbit poses, T<1e-4",
@@ -420,12 +446,15 @@ def bench_render(frames: int, warmup: int):
         "samples_per_s": round(gathers / frames / (ms_frame * 1e-3), 1),
         "samples_per_frame": round(gathers / frames, 1),
         "kept_per_frame": round(kept / frames, 1),
-        "roofline": {"kernel": "k_render", "bound": "mfma", "achieved": round(tflops, 3),
+        "kernels_ms": {"march": round(float(stage_ms[0]), 4), "mlp": round(mlp_ms, 4),
+                       "composite": round(float(stage_ms[2]), 4)},
+        "frame_tflops": round(tflops, 3),
+        "roofline": {"kernel": "k_render_mlp", "bound": "mfma", "achieved": round(mlp_tflops, 3),
                      "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(tflops / MFMA_BF16_PEAK_TFLOPS, 6),
-                     "traffic": measured_traffic("c5", "render"),
+                     "frac": round(mlp_tflops / MFMA_BF16_PEAK_TFLOPS, 6),
+                     "traffic": measured_traffic("c5", "render_mlp"),
                      "traffic_source": TRAFFIC_SOURCE,
-                     "algorithmic_flop": flop, "avg_launch_ms": round(ms_frame, 5)},
+                     "algorithmic_flop": flop, "avg_launch_ms": round(mlp_ms, 5)},
     }
 
 
@@ -467,6 +496,41 @@ def bench_tv(cfg_name: str, reps: int = 20):
     return out
 
 
+def fused_accuracy(scene: dict, first: dict) -> dict:
+    """Error of the fused step's first forward+backward (rgb, loss, gradients)
+    against the oracle's FP64 reference step, evaluated once with the bf16
+    weights the step holds and once with the fp32 master weights."""
+    import torch
+
+    from oracle import oracle
+    from plenvdb_amd.fused import unpack_mlp
+
+    def rel(a, b):
+        a, b = np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel()
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+    origins, vm = oracle.leaves_from_mask(scene["mask"])
+    mlp32 = {k: v.astype(np.float64) for k, v in unpack_mlp(first["mlp"]).items()}
+    held = dict(mlp32)
+    for k in ("w0", "w1", "w2"):
+        held[k] = torch.from_numpy(np.ascontiguousarray(mlp32[k], np.float32)).to(
+            torch.bfloat16).double().numpy()
+    out = {}
+    for name, mlp in (("vs_fp64_held_bf16_weights", held), ("vs_fp64_fp32_weights", mlp32)):
+        ref = oracle.reference_train_step(
+            dict(scene, origins=origins, vmask=vm, mlp=mlp,
+                 density=first["density"].astype(np.float64),
+                 color=first["color"].astype(np.float64)), prec="f64")
+        gm = unpack_mlp(first["grad_mlp"])
+        out[name] = {
+            "rgb_max_abs": float(np.abs(first["rgb"] - ref["rgb_marched"]).max()),
+            "loss_rel": abs(first["loss"] - ref["loss"]) / abs(ref["loss"]),
+            "grad_density_rel_l2": rel(first["grad_density"], ref["grad_density"]),
+            "grad_color_rel_l2": rel(first["grad_color"], ref["grad_color"]),
+            "grad_mlp_rel_l2_max": max(rel(gm[k], ref["grad_mlp"][k]) for k in gm)}
+    return out
+
+
 def cpu_baseline(cfg_name: str, n_rays: int, threads: int | None = None):
     """Oracle FP32 restatement of the reference step (C kernels under OpenMP,
     MLP/composite/loss in torch CPU), forward + backward + Adam, timed on the
@@ -487,7 +551,8 @@ def cpu_baseline(cfg_name: str, n_rays: int, threads: int | None = None):
     cores = threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     cores = oracle.set_threads(cores)
     times, samples = [], []
-    for it in range(6):
+    warm, timed = 3, 20
+    for it in range(warm + timed):
         t = time.perf_counter()
         out = oracle.reference_train_step(sc, prec="f32", with_mlp=sc["cfg"].use_mlp)
         sz = oracle.adam_stepsz(0.1, 0.9, 0.99, it + 1)
@@ -497,12 +562,24 @@ def cpu_baseline(cfg_name: str, n_rays: int, threads: int | None = None):
         oracle.vdb_adam(col, gc, m_c, v_c, act, 3, 1, sz, 1e-8, 0.9, 0.99)
         times.append(time.perf_counter() - t)
         samples.append(out["n_mask"])
-    dt = float(np.median(times[1:]))
+    dt = float(np.median(times[warm:]))
     return {"value": round(float(np.mean(samples)) / dt, 1), "unit": "samples/s", "cores": cores,
-            "kind": "port",
-            "sample": f"{cfg_name} workload, {n_rays} of its rays, fwd+bwd+VDB Adam, median of 5 "
-                      f"steps after 1 warm-up ({int(np.mean(samples))} samples/step, "
+            "kind": "port", "cpu_model": cpu_model(),
+            "sample": f"{cfg_name} workload, {n_rays} of its rays, fwd+bwd+VDB Adam, median of "
+                      f"{timed} steps after {warm} warm-ups ({int(np.mean(samples))} samples/step, "
                       f"{dt * 1e3:.0f} ms/step)"}
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def main():
@@ -586,6 +663,9 @@ def main():
     if others:
         line["configs"] = {c: bench_config(c, 5, 2) for c in others}
     if not args.no_cpu:
+        if "batched" in res:
+            # the oracle as the checker of the measured step's numbers
+            line["accuracy"] = fused_accuracy(scene, res["batched"]["first_step"])
         line["cpu_baseline"] = cpu_baseline(args.config, args.cpu_rays)
         # and one host thread, on an eighth of the rays
         line["cpu_baseline_1thread"] = cpu_baseline(args.config, max(args.cpu_rays // 8, 64), 1)

@@ -148,11 +148,14 @@ int pvdb_alpha2weight_bwd_f64(const double* alpha, const double* weight, const d
  * ndc_rays (:502-519, near = 1, focal = K[0][0]): rays_o, rays_d and viewdirs
  * [H][W][3] of one camera.  K: host 3x3 row-major (double, as the loaders hand
  * it over; rounded to float32 where torch rounds it); c2w: host 3x4 row-major
- * float32.  mode 0 = 'center', 1 = 'lefttop' ('random' is not supported). */
+ * float32.  mode 0 = 'center', 1 = 'lefttop', 2 = 'random': pixel (h, w)
+ * of the unflipped grids sits at (w + jitter[0][h][w], h + jitter[1][h][w]),
+ * jitter being dev [2][H][W] float32 drawn by the caller (the reference draws
+ * it with torch.rand_like, dvgo.py:482-484); NULL for modes 0 and 1. */
 int pvdb_rays_of_view(int32_t H, int32_t W, const double* K /*host[9]*/,
                       const float* c2w /*host[12]*/, int32_t ndc, int32_t inverse_y,
                       int32_t flip_x, int32_t flip_y, int32_t mode, float* rays_o,
-                      float* rays_d, float* viewdirs, void* stream);
+                      float* rays_d, float* viewdirs, const float* jitter, void* stream);
 /* infer_t_minmax + infer_n_samples (:11-55).  far_per_ray may be NULL. */
 int pvdb_ray_bounds(const float* rays_o, const float* rays_d, int64_t n,
                     const float* xyz_min /*host[3]*/, const float* xyz_max /*host[3]*/,
@@ -235,6 +238,10 @@ typedef struct pvdb_step_desc {
     float* mlp_m;
     float* mlp_v;
     float stepsz_density, stepsz_color, step_size_mlp, eps, beta0, beta1;
+    /* 0: forward + backward (gradients accumulate); 1: forward + backward +
+     * Adam; 2: Adam alone on the gradients accumulated by earlier calls (the
+     * reference's separate VDBAdam.step / MaskedAdam.step after a backward,
+     * masked_adam.py:70-80,127-158), over the leaves those calls touched */
     int32_t do_adam;
     int32_t reserved;
     void* workspace; /* dev, pvdb_fused_workspace_bytes() bytes, 256-byte aligned */
@@ -262,6 +269,11 @@ typedef struct pvdb_step_desc {
      * loss sum(w.detach() * |rgb_i - target|^2) / N; 0 switches a term off */
     float weight_main, weight_entropy_last, weight_rgbper;
     int32_t reserved2;
+    /* dev [slots][12] fp32 or NULL: with a bf16 colour payload, the fp32
+     * master the colour Adam updates; the payload then receives its
+     * round-to-nearest bf16 value after each update (NULL: the bf16 payload is
+     * updated in place).  Ignored for an fp32 payload. */
+    float* color_master;
 } pvdb_step_desc;
 
 /* The default launch shapes (pvdb_step_desc.launch) on the current device. */
@@ -289,16 +301,21 @@ int pvdb_fused_records(const pvdb_topology* topo, const pvdb_step_desc* desc, vo
 
 /* ---- merged-VDB renderer ----------------------------------------------------
  * MGRenderer (plenvdb/lib/vdb/plenvdb.h:933-1068, plenvdb.cpp:135-171;
- * render_an_image_cuda renderer.cu:370-424) as one persistent kernel: each
- * wavefront marches an 8x8 pixel tile with the reference's sample rule and
- * evaluates the RGB MLP on fp16 MFMA for its kept samples.  The merged index
- * grid (vdb_compression.py:19-59) is a topology whose active voxels are the
- * non-zero indices plus `slot_idx` (the index of every slot, 0 when inactive).
- * Differences from the reference: one march instead of two (the reference's
- * second march restarts from the first kept t minus one step, so its sample
- * positions can differ from the first's by rounding, renderer.cu:263), the
+ * render_an_image_cuda renderer.cu:370-424) as three kernels on `stream`,
+ * with no host synchronisation and no allocation: a march kernel runs both of
+ * the reference's marches per pixel (first_look_merged's count and t_min /
+ * t_max rewrite, then ray_marching_merged from the rewritten t_min, each with
+ * its own summation order, renderer.cu:222-367) and writes one record per
+ * kept sample; an MLP kernel evaluates the RGB MLP on fp16 MFMA over the
+ * records; a composite kernel adds each pixel's contributions to T * bg in
+ * sample order.  The merged index grid (vdb_compression.py:19-59) is a
+ * topology whose active voxels are the non-zero indices plus `slot_idx` (the
+ * index of every slot, 0 when inactive).  Differences from the reference: the
  * early-stop threshold t_stop is a parameter (the reference hard-codes 1e-3,
- * renderer.cu:265), and the MLP runs in fp16 with fp32 accumulation. */
+ * renderer.cu:265); the MLP runs in fp16 with fp32 accumulation; a pixel whose
+ * second march keeps more samples than its first counted contributes the first
+ * `count` of them (the reference writes past its segment there, renderer.cu:
+ * 363 comments the check out). */
 typedef struct pvdb_render_desc {
     float c2w[16];    /* host values: camera-to-world, row-major (input_a_c2w) */
     float K[9];       /* intrinsics, row-major (setScene) */
@@ -311,8 +328,16 @@ typedef struct pvdb_render_desc {
     const void* payload;   /* dev, pvdb_render_payload_bytes() bytes */
     const void* weights;   /* dev, pvdb_render_weights_bytes() bytes */
     float* out;            /* dev [H*W*3] (output_an_image layout) */
-    uint64_t* counters;    /* dev [2] (density gathers, kept samples) or NULL */
-    void* workspace;       /* dev, 256 bytes */
+    /* dev [3] or NULL: first-march density gathers, written samples, records the
+     * frame reserved (above `capacity` when the frame did not fit) */
+    uint64_t* counters;
+    /* dev, pvdb_render_workspace_bytes(H, W, capacity) bytes, 256-byte aligned */
+    void* workspace;
+    int64_t workspace_bytes;
+    /* records (kept samples) per frame, in [0, 2^31 - 2]; a frame with more
+     * renders the records that fit and pvdb_render_frame_info's total, the
+     * full count, exceeds it */
+    int64_t capacity;
 } pvdb_render_desc;
 
 /* Bytes of the packed MLP operands. */
@@ -331,8 +356,22 @@ int64_t pvdb_render_payload_bytes(const pvdb_topology* topo, int32_t payload_f32
 int pvdb_render_build_payload(const pvdb_topology* topo, const int32_t* slot_idx,
                               const float* den, const float* col, int64_t nidx,
                               int32_t payload_f32, void* payload, void* stream);
+/* Workspace bytes of an H x W frame with `capacity` records (-1 on bad sizes). */
+int64_t pvdb_render_workspace_bytes(int32_t H, int32_t W, int64_t capacity);
 /* MGRenderer::render_an_image (plenvdb.h:1029-1035): writes out [H*W*3]. */
 int pvdb_render_image(const pvdb_topology* topo, const pvdb_render_desc* desc, void* stream);
+/* The last frame's record total (synchronous copy to the host; may exceed the
+ * capacity) and device pointers of its per-pixel first-march counts, written
+ * samples and first record (the reference keeps the counts in n_samples,
+ * renderer.cu:236-262).  No counterpart in the reference for the rest. */
+int pvdb_render_frame_info(const pvdb_render_desc* desc, uint64_t* total, int32_t** pix_count,
+                           int32_t** pix_written, int32_t** pix_start);
+/* Optional device timing of the renderer's march, MLP and composite kernels
+ * (HIP events around each launch; no counterpart in the reference, which sums
+ * CPU clock() time, plenvdb.h:1030-1035).  Collects pending events and copies
+ * the three totals (ms) to total_ms3; enable = 1 starts (and resets), 0 stops
+ * (and resets), -1 only reads.  Must stay off under graph capture. */
+int pvdb_render_timing(int32_t enable, double* total_ms3);
 
 /* ---- self test --------------------------------------------------------------
  * No counterpart in the reference: multiplies bf16 matrices with the MFMA

@@ -185,7 +185,7 @@ def trilinear_bwd(dom: SlotDomain, C_: int, xs, ys, zs, g, order: int, prec: str
 
 
 def rays_of_view(H, W, K, c2w, ndc=False, inverse_y=False, flip_x=False, flip_y=False,
-                 mode="center"):
+                 mode="center", jitter=None):
     """get_rays_of_a_view (dvgo.py:522-527) with get_rays (:471-499) and ndc_rays
     (:502-519, near 1, focal K[0][0]) in float32, one numpy operation per torch
     operation: K's entries and the NDC constants are rounded to float32 where
@@ -196,6 +196,9 @@ def rays_of_view(H, W, K, c2w, ndc=False, inverse_y=False, flip_x=False, flip_y=
     i, j = np.meshgrid(np.arange(W, dtype=f), np.arange(H, dtype=f))  # [H, W]: i = column
     if mode == "center":
         i, j = i + f(0.5), j + f(0.5)
+    elif mode == "random":  # i + rand_like(i), j + rand_like(j) (dvgo.py:482-484)
+        jit = np.asarray(jitter, f).reshape(2, H, W)
+        i, j = i + jit[0], j + jit[1]
     elif mode != "lefttop":
         raise NotImplementedError(mode)
     if flip_x:

@@ -83,15 +83,65 @@ __device__ __forceinline__ StepPoint step_point(const StepParams& P, const TopoV
     return sp;
 }
 
-// Runs window(base, n) for base = 0, 64, ... < n_steps, with 32-bit indices
-// when the ray's step count allows.
+// Empty-space skip: the first step at or after i whose occupancy voxel can lie
+// in an existing leaf.  While step i's voxel round(p*ms + mb) falls in a
+// 128^3 block without a lower node, or in an 8^3 block without a leaf, every
+// step until the ray leaves that block's rounding box [lo - 0.5, hi + 0.5)
+// is rejected by the occupancy test, so the march jumps to two steps before
+// the exit computed from p_i = s + u*(stepdist*i) (positions come from i, not
+// from a running sum, so a jump changes no sample).  The two steps (a voxel
+// at the configurations' step of half a voxel) cover the rounding of the
+// exit.  Every lane computes the same value.
+template <typename I>
+__device__ __forceinline__ I skip_empty(const StepParams& P, const TopoView& t, const RaySetup& s,
+                                        I i, I n) {
+    const float su[3] = {s.ux, s.uy, s.uz}, ss[3] = {s.sx, s.sy, s.sz};
+#pragma unroll 1
+    for (int hop = 0; hop < 16 && i < n; ++hop) {
+        const float dist = P.stepdist * (float)i;
+        int v[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[a] = (int)roundf((ss[a] + su[a] * dist) * P.ms[a] + P.mb[a]);
+        int shift;
+        if (lower_of(t, v[0], v[1], v[2]) < 0)
+            shift = 7;
+        else if (leaf_of(t, v[0], v[1], v[2]) < 0)
+            shift = 3;
+        else
+            return i;
+        float dmin = INFINITY;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float g = su[a] * P.ms[a];
+            if (g == 0.f) continue;
+            const float lo = (float)((v[a] >> shift) << shift);
+            const float bound = g > 0.f ? lo + (float)(1 << shift) - 0.5f : lo - 0.5f;
+            dmin = fminf(dmin, (bound - (ss[a] * P.ms[a] + P.mb[a])) / g);
+        }
+        const float xe = dmin / P.stepdist;
+        if (!(xe < 1.0e9f)) return n;  // no exit before the ray's end
+        const I ie = (I)floorf(xe) - 1;
+        if (ie <= i) return i;
+        i = ie < n ? ie : n;
+    }
+    return i;
+}
+
+// Runs window(base, n) over the ray's steps in windows of 64, base advancing
+// past empty space (skip_empty), with 32-bit indices when the ray's step count
+// allows.
 template <typename F>
-__device__ __forceinline__ void for_windows(int64_t n_steps, F&& window) {
-    if (n_steps < (int64_t(1) << 24)) {
-        const int32_t n = (int32_t)n_steps;
-        for (int32_t base = 0; base < n; base += 64) window(base, n);
+__device__ __forceinline__ void for_windows(const StepParams& P, const TopoView& t,
+                                            const RaySetup& s, F&& window) {
+    if (s.n_steps < (int64_t(1) << 24)) {
+        const int32_t n = (int32_t)s.n_steps;
+        for (int32_t base = skip_empty(P, t, s, 0, n); base < n;
+             base = skip_empty(P, t, s, base + 64, n))
+            window(base, n);
     } else {
-        for (int64_t base = 0; base < n_steps; base += 64) window(base, n_steps);
+        for (int64_t base = skip_empty(P, t, s, int64_t(0), s.n_steps); base < s.n_steps;
+             base = skip_empty(P, t, s, base + 64, s.n_steps))
+            window(base, s.n_steps);
     }
 }
 
@@ -114,7 +164,7 @@ __global__ void __launch_bounds__(256) k_ray_count(StepParams P, TopoView t, Ray
     const int lane = lane_id();
     const RaySetup s = ray_setup(P, r);
     int count = 0;
-    for_windows(s.n_steps, [&](auto base, auto n) {
+    for_windows(P, t, s, [&](auto base, auto n) {
         const StepPoint sp = step_point(P, t, s, base + lane, n);
         count += __popcll(__ballot(sp.keep));
     });
@@ -182,29 +232,36 @@ __global__ void __launch_bounds__(1024) k_scan(RayBuffers R, int n) {
     if (tid == 0) R.total[0] = (int)(carry < kSat ? carry : kSat);
 }
 
-// Forward march: one record per occupancy-kept sample.
+// Forward march: one record per occupancy-kept sample.  The occupancy test
+// runs over windows of 64 consecutive steps; the kept step indices are queued
+// in LDS (in step order) and the sample work -- density gather, raw2alpha,
+// the transmittance scan, records -- runs on 64 queued samples at a time, so
+// lanes are not spent on rejected steps.
 __global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, const float* density,
                                                    RayBuffers R, Records rec) {
+    __shared__ int64_t queue_all[4][128];
     const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (r >= P.n_rays) return;
+    if (r >= P.n_rays) return;  // uniform over the wave
     const int lane = lane_id();
+    int64_t* queue = queue_all[threadIdx.x >> 6];
     const RaySetup s = ray_setup(P, r);
     const int64_t off = R.offset[r];
-    double Tc = 1.0;
+    const int cnt = R.count[r];  // the counting march's total bounds this ray's records
+    float Tc = 1.f;  // transmittance carried from batch to batch (FP32, as the reference's T_cum)
     bool stopped = false;
-    int written = 0;
-    for_windows(s.n_steps, [&](auto base, auto n) {
-        const StepPoint sp = step_point(P, t, s, base + lane, n);
-        const uint64_t km = __ballot(sp.keep);
-        if (km == 0) return;
-        const int idx = written + mbcnt(km);
-        written += __popcll(km);
+    int written = 0, queued = 0;
+    // sample work on queue[0 .. k-1] (k <= 64), in step order
+    auto flush = [&](int k) {
+        const bool mine = lane < k;
         float x = 0.f, y = 0.f, z = 0.f, e = 0.f, alpha = 0.f;
-        const bool live = sp.keep && !stopped;
+        const bool live = mine && !stopped;
         if (live) {
-            x = ((sp.px - P.mn[0]) / P.len[0]) * P.rm1[0];
-            y = ((sp.py - P.mn[1]) / P.len[1]) * P.rm1[1];
-            z = ((sp.pz - P.mn[2]) / P.len[2]) * P.rm1[2];
+            const int64_t i = queue[lane];
+            const float dist = P.stepdist * (float)i;
+            const float px = s.sx + s.ux * dist, py = s.sy + s.uy * dist, pz = s.sz + s.uz * dist;
+            x = ((px - P.mn[0]) / P.len[0]) * P.rm1[0];
+            y = ((py - P.mn[1]) / P.len[1]) * P.rm1[1];
+            z = ((pz - P.mn[2]) / P.len[2]) * P.rm1[2];
             const Cell cell = make_cell(x, y, z);
             LeafCache cache;
             const float sigma = gather_density(t, density, cell, cache);
@@ -213,22 +270,24 @@ __global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, con
         }
         const bool akeep = live && (P.thres > 0.f ? alpha > P.thres : true);
         // inclusive product of (1 - alpha) over alpha-kept lanes, in step order
-        const double f = akeep ? 1.0 - (double)alpha : 1.0;
-        double incl = f;
+        // (a log-depth scan: each T is a product of the same factors as the
+        // reference's running product, grouped differently)
+        const float f = akeep ? 1.f - alpha : 1.f;
+        float incl = f;
         for (int d = 1; d < 64; d <<= 1) {
-            const double o = __shfl_up(incl, d, 64);
+            const float o = __shfl_up(incl, d, 64);
             if (lane >= d) incl *= o;
         }
-        double excl = __shfl_up(incl, 1, 64);
-        if (lane == 0) excl = 1.0;
-        const double Tb = Tc * excl, Ta = Tc * incl;
-        const uint64_t stopm = __ballot(akeep && Ta < 1e-3);
+        float excl = __shfl_up(incl, 1, 64);
+        if (lane == 0) excl = 1.f;
+        const float T = Tc * excl, Ta = Tc * incl;
+        const uint64_t stopm = __ballot(akeep && Ta < 1e-3f);
         const int stop_lane = stopm ? __ffsll((unsigned long long)stopm) - 1 : 64;
         const bool alive_a = akeep && lane <= stop_lane;
-        const float T = (float)Tb;
         const float w = T * alpha;
         const bool alive_w = alive_a && (P.thres > 0.f ? w > P.thres : true);
-        if (sp.keep) {
+        const int idx = written + lane;
+        if (mine && idx < cnt) {
             const int64_t q = off + idx;
             if (q >= 0 && q < P.capacity) {
                 uint32_t bits = (uint32_t)r | (alive_a ? kAliveA : 0u) | (alive_w ? kAliveW : 0u);
@@ -237,6 +296,7 @@ __global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, con
                                           alive_w ? w : 0.f, __uint_as_float(bits));
             }
         }
+        written += k;
         if (!stopped) {
             if (stop_lane < 64) {
                 Tc = __shfl(Ta, stop_lane, 64);
@@ -245,8 +305,32 @@ __global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, con
                 Tc = __shfl(Ta, 63, 64);
             }
         }
+    };
+    for_windows(P, t, s, [&](auto base, auto n) {
+        const StepPoint sp = step_point(P, t, s, base + lane, n);
+        const uint64_t km = __ballot(sp.keep);
+        if (km == 0) return;
+        if (sp.keep) queue[queued + mbcnt(km)] = (int64_t)(base + lane);
+        queued += __popcll(km);
+        if (queued >= 64) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            flush(64);
+            // the overflow (fewer than 64 entries) moves to the queue's front
+            const int64_t rest = lane + 64 < queued ? queue[lane + 64] : 0;
+            __builtin_amdgcn_wave_barrier();
+            if (lane + 64 < queued) queue[lane] = rest;
+            queued -= 64;
+        }
     });
-    if (lane == 0) R.t_last[r] = (float)Tc;
+    if (queued > 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        flush(queued);
+    }
+    if (lane == 0) R.t_last[r] = Tc;
 }
 
 // Per-ray composite: inclusive prefix of w*rgb (kept for the alpha adjoint),

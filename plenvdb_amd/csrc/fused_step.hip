@@ -109,12 +109,17 @@ __device__ __forceinline__ void adam1(float& p, float g, float& m, float& v, flo
 // the leaves the backward touched, then zero their gradients and flags.  One
 // workgroup per touched leaf; untouched leaves have all-zero gradients, which
 // stepmode 1 skips, so the result equals the reference's all-slot pass.
+// With a bf16 colour payload and an fp32 master copy (cmaster != nullptr) the
+// update runs on the master and the payload receives its round-to-nearest
+// bf16 value, so updates below half a bf16 step accumulate instead of being
+// lost; without one the bf16 payload is updated in place.
 template <bool BF16COL>
 __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __restrict__ touched,
                                                      float* __restrict__ den,
                                                      float* __restrict__ gden,
                                                      float* __restrict__ mden,
                                                      float* __restrict__ vden, void* col,
+                                                     float* __restrict__ cmaster,
                                                      float* __restrict__ gcol,
                                                      float* __restrict__ mcol,
                                                      float* __restrict__ vcol, float sz_d_arg,
@@ -151,7 +156,7 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
                         mm[4 * k4] = qm.x; mm[4 * k4 + 1] = qm.y; mm[4 * k4 + 2] = qm.z; mm[4 * k4 + 3] = qm.w;
                         vv[4 * k4] = qv.x; vv[4 * k4 + 1] = qv.y; vv[4 * k4 + 2] = qv.z; vv[4 * k4 + 3] = qv.w;
                     }
-                    if (BF16COL) {
+                    if (BF16COL && !cmaster) {
                         const uint2* c2 = reinterpret_cast<const uint2*>(col) + s * 3;
 #pragma unroll
                         for (int k4 = 0; k4 < 3; ++k4) {
@@ -162,7 +167,8 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
                             pp[4 * k4 + 3] = bf_bits2f((uint16_t)(q.y >> 16));
                         }
                     } else {
-                        const float4* c4 = reinterpret_cast<const float4*>(col) + s * 3;
+                        const float4* c4 =
+                            reinterpret_cast<const float4*>(BF16COL ? cmaster : col) + s * 3;
 #pragma unroll
                         for (int k4 = 0; k4 < 3; ++k4) {
                             const float4 q = c4[k4];
@@ -188,6 +194,13 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
                         for (int k4 = 0; k4 < 3; ++k4)
                             c2[k4] = make_uint2((uint32_t)f2bf_bits(pp[4 * k4]) | ((uint32_t)f2bf_bits(pp[4 * k4 + 1]) << 16),
                                                 (uint32_t)f2bf_bits(pp[4 * k4 + 2]) | ((uint32_t)f2bf_bits(pp[4 * k4 + 3]) << 16));
+                        if (cmaster) {
+                            float4* c4 = reinterpret_cast<float4*>(cmaster) + s * 3;
+#pragma unroll
+                            for (int k4 = 0; k4 < 3; ++k4)
+                                c4[k4] = make_float4(pp[4 * k4], pp[4 * k4 + 1], pp[4 * k4 + 2],
+                                                     pp[4 * k4 + 3]);
+                        }
                     } else {
                         float4* c4 = reinterpret_cast<float4*>(col) + s * 3;
 #pragma unroll
@@ -240,6 +253,24 @@ struct StageScope {
         g_timer.open.push_back({stage, {a, b}});
     }
 };
+
+// VDB Adam over the touched leaves (stepmode 1 on both grids).
+int launch_grid_adam(const pvdb_step_desc* d, const TopoView& t, uint8_t* touched, hipStream_t s) {
+    const int ag = (int)std::min<int64_t>(t.nleaf > 0 ? t.nleaf : 1, 4096);
+    float* master = d->color_bf16 ? d->color_master : nullptr;
+    if (d->color_bf16)
+        hipLaunchKernelGGL(k_sparse_adam<true>, dim3(ag), dim3(256), 0, s, t, touched, d->density,
+                           d->grad_density, d->density_m, d->density_v, d->color, master,
+                           d->grad_color, d->color_m, d->color_v, d->stepsz_density,
+                           d->stepsz_color, d->adam_scalars, d->eps, d->beta0, d->beta1);
+    else
+        hipLaunchKernelGGL(k_sparse_adam<false>, dim3(ag), dim3(256), 0, s, t, touched,
+                           d->density, d->grad_density, d->density_m, d->density_v, d->color,
+                           master, d->grad_color, d->color_m, d->color_v, d->stepsz_density,
+                           d->stepsz_color, d->adam_scalars, d->eps, d->beta0, d->beta1);
+    PVDB_LAUNCHED("k_sparse_adam");
+    return PVDB_OK;
+}
 
 }  // namespace
 
@@ -316,6 +347,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     PVDB_CHECK_ARG(d->rays_o && d->rays_d && d->viewdirs && d->target, "null ray arrays");
     PVDB_CHECK_ARG(d->density && d->grad_density && d->color && d->grad_color, "null grid arrays");
     PVDB_CHECK_ARG(d->mlp && d->mlp_grad, "null MLP arrays");
+    PVDB_CHECK_ARG(d->do_adam >= 0 && d->do_adam <= 2, "do_adam must be 0, 1 or 2");
     PVDB_CHECK_ARG(!d->do_adam || (d->density_m && d->density_v && d->color_m && d->color_v &&
                                    d->mlp_m && d->mlp_v),
                    "Adam needs moment arrays");
@@ -333,7 +365,8 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     // (bf16 colour as 8-byte ones), the workspace's sub-arrays assume a
     // 256-byte aligned base
     PVDB_CHECK_ARG(aligned(d->grad_color, 16) && aligned(d->color_m, 16) &&
-                       aligned(d->color_v, 16) && aligned(d->color, d->color_bf16 ? 8 : 16),
+                       aligned(d->color_v, 16) && aligned(d->color, d->color_bf16 ? 8 : 16) &&
+                       aligned(d->color_master, 16),
                    "colour arrays must be 16-byte aligned (8-byte for a bf16 payload)");
     PVDB_CHECK_ARG(aligned(d->workspace, 256), "workspace must be 256-byte aligned");
     const TopoView t = view_of(topo);
@@ -409,6 +442,16 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     W.b2 = d->mlp + kB2;
 
     int rc;
+    if (d->do_adam == 2) {  // the optimizer alone, on the gradients already accumulated
+        {
+            StageScope sc(7, s);
+            const MlpAdam adam{d->mlp,           d->mlp_m, d->mlp_v, d->adam_scalars,
+                               d->step_size_mlp, d->eps,   d->beta0, d->beta1};
+            if ((rc = launch_dw_reduce(dw_part, 0, d->mlp_grad, adam, s))) return rc;
+        }
+        StageScope sc(8, s);
+        return launch_grid_adam(d, t, touched, s);
+    }
     {
         // the counting march also packs the bf16 MLP operands and zeroes the
         // MLP gradient (accumulated by k_dw_reduce later in the step)
@@ -471,18 +514,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     }
     if (d->do_adam) {
         StageScope sc(8, s);
-        const int ag = (int)std::min<int64_t>(t.nleaf > 0 ? t.nleaf : 1, 4096);
-        if (d->color_bf16)
-            hipLaunchKernelGGL(k_sparse_adam<true>, dim3(ag), dim3(256), 0, s, t, touched,
-                               d->density, d->grad_density, d->density_m, d->density_v, d->color,
-                               d->grad_color, d->color_m, d->color_v, d->stepsz_density,
-                               d->stepsz_color, d->adam_scalars, d->eps, d->beta0, d->beta1);
-        else
-            hipLaunchKernelGGL(k_sparse_adam<false>, dim3(ag), dim3(256), 0, s, t, touched,
-                               d->density, d->grad_density, d->density_m, d->density_v, d->color,
-                               d->grad_color, d->color_m, d->color_v, d->stepsz_density,
-                               d->stepsz_color, d->adam_scalars, d->eps, d->beta0, d->beta1);
-        PVDB_LAUNCHED("k_sparse_adam");
+        if ((rc = launch_grid_adam(d, t, touched, s))) return rc;
     }
     return PVDB_OK;
 }

@@ -207,14 +207,19 @@ struct ViewParams {
     float neg_two_near;  // -2. * near
 };
 
-__global__ void k_rays_of_view(ViewParams V, float* __restrict__ ro, float* __restrict__ rd,
+__global__ void k_rays_of_view(ViewParams V, const float* __restrict__ jitter,
+                               float* __restrict__ ro, float* __restrict__ rd,
                                float* __restrict__ vd) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (int64_t)V.H * V.W) return;
     const int h = (int)(p / V.W), w = (int)(p % V.W);
-    // linspace(0, n-1, n) holds the exact integers; flips reverse an axis
-    const float i = (float)(V.flip_x ? V.W - 1 - w : w) + V.off;
-    const float j = (float)(V.flip_y ? V.H - 1 - h : h) + V.off;
+    // linspace(0, n-1, n) holds the exact integers; flips reverse an axis of
+    // the (jittered) grids, so a flipped pixel takes its mirror's jitter
+    const int wi = V.flip_x ? V.W - 1 - w : w, hj = V.flip_y ? V.H - 1 - h : h;
+    const float oi = jitter ? jitter[(int64_t)h * V.W + wi] : V.off;
+    const float oj = jitter ? jitter[(int64_t)V.H * V.W + (int64_t)hj * V.W + w] : V.off;
+    const float i = (float)wi + oi;
+    const float j = (float)hj + oj;
     const float d0 = (i - V.cx) / V.fx;
     float d1, d2;
     if (V.inverse_y) {
@@ -263,12 +268,14 @@ extern "C" {
 
 int pvdb_rays_of_view(int32_t H, int32_t W, const double* K, const float* c2w, int32_t ndc,
                       int32_t inverse_y, int32_t flip_x, int32_t flip_y, int32_t mode,
-                      float* rays_o, float* rays_d, float* viewdirs, void* stream) {
+                      float* rays_o, float* rays_d, float* viewdirs, const float* jitter,
+                      void* stream) {
     PVDB_CHECK_ARG(H >= 0 && W >= 0 && K && c2w, "bad arguments");
-    PVDB_CHECK_ARG(mode == 0 || mode == 1,
-                   "mode must be 0 ('center') or 1 ('lefttop'); 'random' draws from torch's RNG");
+    PVDB_CHECK_ARG(mode == 0 || mode == 1 || mode == 2,
+                   "mode must be 0 ('center'), 1 ('lefttop') or 2 ('random', with jitter)");
     if ((int64_t)H * W == 0) return PVDB_OK;
     PVDB_CHECK_ARG(rays_o && rays_d && viewdirs, "null pointer");
+    PVDB_CHECK_ARG((mode == 2) == (jitter != nullptr), "mode 2 ('random') takes the jitter array");
     PVDB_CHECK_ARG(K[0] != 0.0 && K[4] != 0.0, "focal lengths must be non-zero");
     ViewParams V;
     V.H = H;
@@ -294,7 +301,7 @@ int pvdb_rays_of_view(int32_t H, int32_t W, const double* K, const float* c2w, i
     V.neg_two_near = (float)(-2. * near_);
     const int64_t n = (int64_t)H * W;
     hipLaunchKernelGGL(k_rays_of_view, dim3(pvdb_blocks(n, 256)), dim3(256), 0,
-                       (hipStream_t)stream, V, rays_o, rays_d, viewdirs);
+                       (hipStream_t)stream, V, jitter, rays_o, rays_d, viewdirs);
     PVDB_LAUNCHED("k_rays_of_view");
     return PVDB_OK;
 }

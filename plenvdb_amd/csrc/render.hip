@@ -1,34 +1,51 @@
 // Merged-VDB renderer: MGRenderer::render_an_image (plenvdb/lib/vdb/plenvdb.h:
-// 1029-1035 -> renderer.cu:370-424) as one persistent kernel.
+// 1029-1035 -> renderer.cu:370-424) in three kernels and no host round trip.
 //
 // The reference renders in two full marches per pixel (first_look_merged
-// counts the kept samples, renderer.cu:222-268; ray_marching_merged re-marches
-// and gathers them, :312-367), a host read-back of the sample total, four
-// SGEMMs over the materialised sample list (cuda_rgbnet, :83-109) and an
-// atomic composite (final_render, :112-119).  Here each wavefront takes an
-// 8x8 pixel tile, one pixel per lane, and marches all 64 rays at once with the
-// reference's sample rule (first sample at tmin+step, the [0,1]^3 box test,
-// isActive(Round(xyz)), raw2alpha, the two fast_color_thres culls, stop after a
-// kept sample that leaves T < t_stop).  When every live lane holds its next
-// kept sample, the wave evaluates the RGB MLP for those (up to 64) samples on
-// v_mfma_f32_16x16x32_f16 and composites them in registers.  No sample list,
-// no host synchronisation, no allocation.
+// counts the kept samples and rewrites t_min / t_max, renderer.cu:222-268;
+// ray_marching_merged re-marches from the rewritten t_min and gathers them,
+// :312-367), a host read-back of the sample total, four SGEMMs over the
+// materialised sample list (cuda_rgbnet, :83-109) and an atomic composite
+// (final_render, :112-119).  Here:
+//
+//   k_render_march  a wavefront takes an 8x8 pixel tile (one pixel per lane,
+//                   tiles handed out by a device counter) and runs both of the
+//                   reference's marches with their own arithmetic: pass 1 with
+//                   trigetDensity's summation order counts the kept samples,
+//                   records the first kept t minus one step and the early-stop
+//                   t; the wave then reserves its tile's records with one
+//                   atomic and scans the counts; pass 2 restarts from the
+//                   rewritten t_min with trigetDensity2's order and writes one
+//                   32-byte record per kept sample (12 colour features in
+//                   fp16, the weight, the pixel), at most pass 1's count of
+//                   them, contiguous per pixel.  So the sample set is the
+//                   reference's, including the pass-2 restart (renderer.cu:263).
+//   k_render_mlp    persistent waves over 64-record chunks: the RGB MLP on
+//                   v_mfma_f32_16x16x32_f16 (weights in LDS, shared by the
+//                   workgroup's 8 waves) and each record's contribution
+//                   w / (1 + exp(-raw)) (final_render).
+//   k_render_composite  per pixel: T_last * bg plus its records' contributions
+//                   in sample order (a fixed order, so frames are reproducible).
 //
 // Data layout.  The merged index grid (vdb_compression.py:19-59: index 1 +
-// rank of an active voxel, 0 elsewhere) is turned into a topology whose value
-// masks are the non-zero indices, plus a slot-major payload holding, per
-// slot, density and the 12 colour channels of the voxel's index (index 0's
-// values for inactive voxels; slot `nslots` holds index 0's values for
-// corners whose leaf does not exist).  A slot is 32 bytes (fp16: exact for
-// the fp16 data of mergeddata.npz) or 64 bytes (fp32).
+// rank of an active voxel, 0 elsewhere) is a topology whose value masks are
+// the non-zero indices, plus a slot-major payload holding, per slot, density
+// and the 12 colour channels of the voxel's index (index 0's values for
+// inactive voxels; slot `nslots` holds index 0's values for corners whose leaf
+// does not exist).  A slot is 32 bytes (fp16: exact for the fp16 data of
+// mergeddata.npz) or 64 bytes (fp32).
 //
 // MLP.  Layer outputs are computed transposed (features x samples) and stay in
 // registers: the D fragment of one layer (lane (g,n): features 16m+4g+i of
 // sample n) is the B fragment of the next once the next layer's weights are
 // stored with their K index permuted accordingly (see kperm).  Weights are
-// fp16 in LDS, read as 16-byte fragments; the first layer takes
-// [feature(12) | PE(27) | 1] so its bias is a weight column.
+// fp16 in LDS, read as 16-byte fragments.  The first layer's input row is
+// [feature 0..11 | 0 x4 | PE 0..26 | 1 | 0 ...] (64 halves), so its bias is a
+// weight column and a record's 16-byte halves and the pixel's PE row land on
+// 16-byte boundaries.
 #include <cmath>
+#include <utility>
+#include <vector>
 
 #include "gather.h"
 #include "mfma_util.h"
@@ -44,14 +61,17 @@ using namespace pvdb;
 namespace {
 
 constexpr int kHid = 128;
-constexpr int kWaves = 8;   // waves per workgroup: they share the LDS weights
+constexpr int kWaves = 8;   // MLP waves per workgroup: they share the LDS weights
 constexpr int XS = 72;      // halves per staged input row (64 + 8)
+constexpr int kPeIn = 16;   // first-layer input of PE component 0
 constexpr int kW0Frag = 8 * 2 * 64 * 8;  // [m-tile 8][k-step 2][lane][8]
 constexpr int kW1Frag = 8 * 4 * 64 * 8;  // [m-tile 8][k-step 4][lane][8]
 constexpr int kW2Frag = 4 * 64 * 8;      // [k-step 4][lane][8]
 constexpr int kPackedHalves = kW0Frag + kW1Frag + kW2Frag;
 constexpr int kBiasFloats = 128 + 4;  // b1, b2 (padded)
 constexpr int kPackedBytes = kPackedHalves * 2 + kBiasFloats * 4;
+constexpr int kMarchThreads = 256;    // 4 waves per march workgroup
+constexpr int64_t kMaxRecords = 0x7fffffffll;  // record indices are int32 in the pixel table
 
 // Position j (0..7) of k-step s in lane group g holds feature kperm(s, g, j):
 // the features lane group g of the previous layer's D fragments carries.
@@ -65,15 +85,16 @@ __global__ void k_render_pack(const float* __restrict__ w0, const float* __restr
                               int dcol, int dpe, int dout, _Float16* __restrict__ outh,
                               float* __restrict__ outb) {
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    const int nin = dcol + dpe;  // the bias column follows the inputs
     if (idx < kW0Frag) {
         const int j = idx & 7, lane = (idx >> 3) & 63, ms = idx >> 9;
         const int s = ms & 1, m = ms >> 1, g = lane >> 4, r = lane & 15;
         const int o = 16 * m + r, in = 32 * s + 8 * g + j;
         float v = 0.f;
-        if (in < nin)
+        if (in < dcol)
             v = w0[in * kHid + o];
-        else if (in == nin)
+        else if (in >= kPeIn && in < kPeIn + dpe)
+            v = w0[(dcol + in - kPeIn) * kHid + o];
+        else if (in == kPeIn + dpe)
             v = b0[o];
         outh[idx] = (_Float16)v;
     } else if (idx < kW0Frag + kW1Frag) {
@@ -108,14 +129,29 @@ __global__ void k_render_payload(const int32_t* __restrict__ slot_idx, int64_t n
     p[13] = p[14] = p[15] = (PT)0.f;
 }
 
+// Workspace of one frame (pvdb_render_workspace_bytes), 256-byte aligned parts.
+struct FrameBuffers {
+    uint32_t* tile_counter;          // [1]
+    unsigned long long* total;       // [1] records reserved (may exceed the capacity)
+    int32_t* pix_start;              // [HW] first record of the pixel (saturated)
+    int32_t* pix_count;              // [HW] pass-1 kept count (records reserved)
+    int32_t* pix_written;            // [HW] records pass 2 filled (<= count)
+    float* t_last;                   // [HW] transmittance after pass 2
+    uint4* pe;                       // [HW][4] PE row: 27 halves, 1, 0 x4
+    uint4* rec;                      // [capacity][2] records
+    float4* contrib;                 // [capacity] w * sigmoid(raw), per channel
+    int64_t capacity;
+};
+
 struct RenderArgs {
     TopoView topo;
     const void* payload;
     int64_t nslots;
     const uint4* packed;
     float* out;
-    unsigned long long* counters;  // [2]: density gathers, kept samples (may be null)
-    int32_t* tile_counter;
+    unsigned long long* counters;  // [3]: pass-1 density gathers, written samples, records
+                                   // reserved (may be null)
+    FrameBuffers fb;
     float c2w[12];
     float fx, cx, fy, cy;
     float mn[3], ext[3], ws[3];
@@ -258,8 +294,297 @@ struct BlockCache {
     }
 };
 
+// A pixel's ray in the renderer's normalised box coordinates (get_rays,
+// renderer.cu:122-167; get_tminmax :50-64).
+struct PixelRay {
+    float o[3], d[3], step, tmin, tmax;
+};
+
+__device__ __forceinline__ void pixel_dir(const RenderArgs& A, int px, int py, float* rd) {
+    const float pi = (float)px + 0.5f, pj = (float)py + 0.5f;
+    const float dx = (pi - A.cx) / A.fx;
+    const float dy0 = (pj - A.cy) / A.fy;
+    const float dy = A.inverse_y ? dy0 : -dy0;
+    const float dz = A.inverse_y ? 1.f : -1.f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        rd[a] = dx * A.c2w[4 * a] + dy * A.c2w[4 * a + 1] + dz * A.c2w[4 * a + 2];
+}
+
+// The pixel's PE row (27 components of the view direction, dvgo.py:91-109
+// order, then the bias input 1 and four zeros) as 16 packed fp16 pairs.
+__device__ __forceinline__ void pixel_pe(const RenderArgs& A, int px, int py, uint32_t* pe) {
+    float rd[3];
+    pixel_dir(A, px, py, rd);
+    const float len = sqrtf(rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2]);
+    const float inv = 1.f / len;
+    const float vd[3] = {rd[0] * inv, rd[1] * inv, rd[2] * inv};
+    float p27[28];
+    p27[0] = vd[0];
+    p27[1] = vd[1];
+    p27[2] = vd[2];
+    float base = 1.f;
+#pragma unroll
+    for (int q = 3; q < 7; ++q) {
+        p27[q] = sinf(vd[0] * base);
+        p27[q + 4] = sinf(vd[1] * base);
+        p27[q + 8] = sinf(vd[2] * base);
+        p27[q + 12] = cosf(vd[0] * base);
+        p27[q + 16] = cosf(vd[1] * base);
+        p27[q + 20] = cosf(vd[2] * base);
+        base *= 2.f;
+    }
+    p27[27] = 1.f;  // the bias input
+#pragma unroll
+    for (int q = 0; q < 14; ++q) pe[q] = pack2(p27[2 * q], p27[2 * q + 1]);
+    pe[14] = pe[15] = 0u;
+}
+
+__device__ __forceinline__ PixelRay pixel_ray(const RenderArgs& A, int px, int py) {
+    PixelRay R;
+    float rd[3];
+    pixel_dir(A, px, py, rd);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) R.o[a] = (A.c2w[4 * a + 3] - A.mn[a]) / A.ext[a];
+    const float len = sqrtf(rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2]);
+    R.step = A.stepdist / len;
+    float lo = -INFINITY, hi = INFINITY;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        R.d[a] = rd[a] / A.ext[a];
+        const float v = R.d[a] == 0.f ? 1e-6f : R.d[a];
+        const float ta = (1.f - R.o[a]) / v, tb = (-R.o[a]) / v;
+        lo = a == 0 ? fminf(ta, tb) : fmaxf(lo, fminf(ta, tb));
+        hi = a == 0 ? fmaxf(ta, tb) : fminf(hi, fmaxf(ta, tb));
+    }
+    R.tmin = fmaxf(fminf(lo, 1e9f), A.near_);  // far is forced to 1e9 (plenvdb.h:1008)
+    R.tmax = fmaxf(fminf(hi, 1e9f), A.near_);
+    return R;
+}
+
+// One step of a march: advances t, applies the box and isActive(Round(xyz))
+// tests and, for a sample that passes, forms its 8 corner slots (int()
+// truncation, renderer.cu:271-300) and trilinear weights.  Returns false for a
+// rejected step.  Missing leaves are stepped over with the t += step
+// accumulation alone (see the comment inside), so t stays the reference's
+// running sum and the set of tested steps is unchanged.
+struct Corners {
+    int32_t slot[8];  // the host keeps nslots + 1 below 2^31
+    float u, v, w;
+};
+
+__device__ __forceinline__ bool march_step(const RenderArgs& A, const PixelRay& R, float& t,
+                                           float tend, int& iter, int max_iter, BlockCache& bc,
+                                           Corners& c) {
+    t += R.step;
+    ++iter;
+    const float p0 = R.o[0] + t * R.d[0], p1 = R.o[1] + t * R.d[1], p2 = R.o[2] + t * R.d[2];
+    if (!(p0 >= 0.f && p1 >= 0.f && p2 >= 0.f && p0 <= 1.f && p1 <= 1.f && p2 <= 1.f))
+        return false;
+    const float x = p0 * A.ws[0], y = p1 * A.ws[1], z = p2 * A.ws[2];
+    // isActive(Round(xyz)): rintf, half to even (NanoVDB.h:774-779)
+    const int vi = (int)rintf(x), vj = (int)rintf(y), vk = (int)rintf(z);
+    const int32_t l = bc.get(A.topo, vi, vj, vk);
+    if (l < 0) {
+        // Empty space: the rounded position is in a missing leaf, so every step
+        // until the ray leaves that leaf's rounding box [8L - 0.5, 8L + 7.5) is
+        // rejected by this same test.  Take those steps with the accumulation
+        // alone and stop two steps short of the computed exit, which covers its
+        // rounding.
+        const int L[3] = {vi >> 3, vj >> 3, vk >> 3};
+        float te = INFINITY;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            if (R.d[a] == 0.f) continue;
+            const float xb = R.d[a] > 0.f ? 8.f * (float)L[a] + 7.5f : 8.f * (float)L[a] - 0.5f;
+            te = fminf(te, (xb / A.ws[a] - R.o[a]) / R.d[a]);
+        }
+        const float tsafe = te - 2.f * R.step;
+        while (t + R.step < tsafe && t < tend && iter < max_iter) {
+            t += R.step;
+            ++iter;
+        }
+        return false;
+    }
+    const int off = voxel_offset(vi, vj, vk);
+    if (!((A.topo.masks[(int64_t)l * 8 + (off >> 6)] >> (off & 63)) & 1ull)) return false;
+    const int i0 = (int)x, j0 = (int)y, k0 = (int)z;  // x >= 0 here: int() == floor
+    c.u = x - (float)i0;
+    c.v = y - (float)j0;
+    c.w = z - (float)k0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int cx = (kCx >> q) & 1, cy = (kCy >> q) & 1, cz = (kCz >> q) & 1;
+        const int32_t lq = bc.get(A.topo, i0 + cx, j0 + cy, k0 + cz);
+        c.slot[q] = lq < 0 ? (int32_t)A.nslots : lq * 512 + voxel_offset(i0 + cx, j0 + cy, k0 + cz);
+    }
+    return true;
+}
+
+__device__ __forceinline__ int march_cap(const PixelRay& R, float t0, float t1) {
+    // the reference loop ends when t reaches its bound; the cap only guards
+    // steps too small to advance t
+    const float span = (t1 - t0) / R.step;
+    return (span >= 0.f) ? (int)fminf(span * 1.0625f + 16.f, 16777216.f) : 0;
+}
+
 template <bool F32PAY>
-__global__ void __launch_bounds__(64 * kWaves) k_render(RenderArgs A) {
+__global__ void __launch_bounds__(kMarchThreads) k_render_march(RenderArgs A) {
+    const int lane = threadIdx.x & 63;
+    const FrameBuffers& F = A.fb;
+    unsigned long long n_gather = 0, n_written = 0;
+    while (true) {
+        uint32_t tile = 0;
+        if (lane == 0) tile = atomicAdd(F.tile_counter, 1u);
+        tile = __shfl(tile, 0, 64);
+        if (tile >= (uint32_t)A.ntiles) break;  // uniform over the wave
+        const int px = ((int)tile % A.ntx) * 8 + (lane & 7), py = ((int)tile / A.ntx) * 8 + (lane >> 3);
+        const bool valid = px < A.W && py < A.H;
+        const int64_t pix = (int64_t)py * A.W + px;
+        const PixelRay R = pixel_ray(A, px, py);
+        BlockCache bc;
+
+        // ---- pass 1: first_look_merged (renderer.cu:222-268) ----
+        float t = R.tmin, T = 1.f, tmin2 = R.tmin, tmax2 = R.tmax;
+        int cnt = 0, iter = 0;
+        const int cap1 = march_cap(R, R.tmin, R.tmax);
+        bool first = false;
+        if (valid) {
+            while (t < R.tmax && iter < cap1) {
+                Corners c;
+                if (!march_step(A, R, t, R.tmax, iter, cap1, bc, c)) continue;
+                // trigetDensity order (:191-220): res += den * (1-u) * (1-v) * (1-w), ...
+                float sig = 0.f;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const float a = ((kCx >> q) & 1) ? c.u : 1.f - c.u;
+                    const float b = ((kCy >> q) & 1) ? c.v : 1.f - c.v;
+                    const float e = ((kCz >> q) & 1) ? c.w : 1.f - c.w;
+                    sig = sig + ((load_den<F32PAY>(A.payload, c.slot[q]) * a) * b) * e;
+                }
+                ++n_gather;
+                const float alpha = 1.f - powf(1.f + expf(sig + A.act_shift), -A.interval);
+                if (alpha <= A.thres) continue;
+                const float wt = T * alpha;
+                T = T * (1.f - alpha);
+                if (wt <= A.thres) continue;
+                ++cnt;
+                if (!first) {
+                    tmin2 = t - R.step;
+                    first = true;
+                }
+                if (T < A.t_stop) {
+                    tmax2 = t;
+                    break;
+                }
+            }
+        }
+
+        // ---- the tile's records: one reservation, counts scanned across the wave ----
+        int incl = cnt;
+#pragma unroll
+        for (int dd = 1; dd < 64; dd <<= 1) {
+            const int o = __shfl_up(incl, dd, 64);
+            if (lane >= dd) incl += o;
+        }
+        const int tile_total = __shfl(incl, 63, 64);
+        unsigned long long base = 0;
+        if (lane == 0 && tile_total > 0) base = atomicAdd(F.total, (unsigned long long)tile_total);
+        base = __shfl(base, 0, 64);
+        const int64_t start = (int64_t)base + (incl - cnt);
+
+        // ---- pass 2: ray_marching_merged (renderer.cu:312-367) ----
+        int r = 0;
+        T = 1.f;
+        if (valid && cnt > 0) {
+            t = tmin2;
+            iter = 0;
+            const int cap2 = march_cap(R, tmin2, tmax2);
+            while (t < tmax2 && iter < cap2) {
+                Corners c;
+                if (!march_step(A, R, t, tmax2, iter, cap2, bc, c)) continue;
+                float sc[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const float a = ((kCx >> q) & 1) ? c.u : 1.f - c.u;
+                    const float b = ((kCy >> q) & 1) ? c.v : 1.f - c.v;
+                    const float e = ((kCz >> q) & 1) ? c.w : 1.f - c.w;
+                    sc[q] = (a * b) * e;
+                }
+                // trigetDensity2 order (:271-300): sum of den * scale, left to right
+                float sig = load_den<F32PAY>(A.payload, c.slot[0]) * sc[0];
+#pragma unroll
+                for (int q = 1; q < 8; ++q) sig = sig + load_den<F32PAY>(A.payload, c.slot[q]) * sc[q];
+                const float alpha = 1.f - powf(1.f + expf(sig + A.act_shift), -A.interval);
+                if (alpha <= A.thres) continue;
+                const float wt = T * alpha;
+                T = T * (1.f - alpha);
+                if (wt <= A.thres) continue;
+                if (r < cnt) {
+                    // trigetColor order (:303-310), fp16 features
+                    float f[12], cq[12];
+                    load_col<F32PAY>(A.payload, c.slot[0], cq);
+#pragma unroll
+                    for (int ch = 0; ch < 12; ++ch) f[ch] = cq[ch] * sc[0];
+#pragma unroll 1
+                    for (int q = 1; q < 8; ++q) {
+                        load_col<F32PAY>(A.payload, c.slot[q], cq);
+#pragma unroll
+                        for (int ch = 0; ch < 12; ++ch) f[ch] = f[ch] + cq[ch] * sc[q];
+                    }
+                    const int64_t q = start + r;
+                    if (q < F.capacity) {
+                        F.rec[2 * q] = make_uint4(pack2(f[0], f[1]), pack2(f[2], f[3]),
+                                                  pack2(f[4], f[5]), pack2(f[6], f[7]));
+                        F.rec[2 * q + 1] = make_uint4(pack2(f[8], f[9]), pack2(f[10], f[11]),
+                                                      __float_as_uint(wt), (uint32_t)pix);
+                    }
+                }
+                ++r;  // pass 2 may keep more than pass 1 counted; those are not written
+            }
+        }
+        const int written = r < cnt ? r : cnt;
+        // reserved records pass 2 did not fill: zero weight, zero features
+        for (int k = written; k < cnt; ++k) {
+            const int64_t q = start + k;
+            if (q < F.capacity) {
+                F.rec[2 * q] = make_uint4(0u, 0u, 0u, 0u);
+                F.rec[2 * q + 1] = make_uint4(0u, 0u, 0u, (uint32_t)pix);
+            }
+        }
+        if (valid) {
+            F.pix_start[pix] = (int32_t)(start < kMaxRecords ? start : kMaxRecords);
+            F.pix_count[pix] = cnt;
+            F.pix_written[pix] = written;
+            F.t_last[pix] = cnt > 0 ? T : 1.f;
+            if (cnt > 0) {
+                uint32_t pe[16];
+                pixel_pe(A, px, py, pe);
+                uint4* row = F.pe + 4 * pix;
+                row[0] = make_uint4(pe[0], pe[1], pe[2], pe[3]);
+                row[1] = make_uint4(pe[4], pe[5], pe[6], pe[7]);
+                row[2] = make_uint4(pe[8], pe[9], pe[10], pe[11]);
+                row[3] = make_uint4(pe[12], pe[13], pe[14], pe[15]);
+            }
+        }
+        n_written += (unsigned long long)written;
+    }
+    if (A.counters) {
+        // wave sums, one atomic per wave
+        for (int off = 32; off > 0; off >>= 1) {
+            n_gather += __shfl_xor(n_gather, off, 64);
+            n_written += __shfl_xor(n_written, off, 64);
+        }
+        if (lane == 0) {
+            atomicAdd(A.counters, n_gather);
+            atomicAdd(A.counters + 1, n_written);
+        }
+    }
+}
+
+// The MLP over the records, 64 per wave-chunk; each record's contribution
+// w / (1 + exp(-raw)) per channel (final_render, renderer.cu:112-119).
+__global__ void __launch_bounds__(64 * kWaves) k_render_mlp(RenderArgs A) {
     extern __shared__ uint4 smem[];
     uint4* sW = smem;  // weights + biases
     float* sB = reinterpret_cast<float*>(reinterpret_cast<uint8_t*>(smem) + kPackedHalves * 2);
@@ -270,223 +595,157 @@ __global__ void __launch_bounds__(64 * kWaves) k_render(RenderArgs A) {
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     _Float16* sX = sXall + wv * 64 * XS;
     float* sL = sLall + wv * 64 * 4;
-    {
-        // input columns 40..63 stay zero
-        uint4* row = reinterpret_cast<uint4*>(sX + lane * XS + 40);
-        row[0] = row[1] = row[2] = make_uint4(0u, 0u, 0u, 0u);
-    }
+    uint4* row = reinterpret_cast<uint4*>(sX + lane * XS);
+    row[6] = row[7] = make_uint4(0u, 0u, 0u, 0u);  // inputs 48..63 stay zero
     __syncthreads();
 
-    const TopoView& topo = A.topo;
-    const int64_t miss = A.nslots;  // payload slot of index 0
-    unsigned long long n_gather = 0, n_kept = 0;
-    while (true) {
-        int tile = 0;
-        if (lane == 0) tile = atomicAdd(A.tile_counter, 1);
-        tile = __shfl(tile, 0, 64);
-        if (tile >= A.ntiles) break;  // uniform over the wave
-        const int px = (tile % A.ntx) * 8 + (lane & 7), py = (tile / A.ntx) * 8 + (lane >> 3);
-        const bool valid = px < A.W && py < A.H;
-
-        // ---- ray of the pixel (get_rays, renderer.cu:122-167; get_tminmax :50-64) ----
-        float o[3], d[3], step = 0.f, tmin = 0.f, tmax = 0.f;
-        uint32_t pe[14];
-        {
-            const float pi = (float)px + 0.5f, pj = (float)py + 0.5f;
-            const float dx = (pi - A.cx) / A.fx;
-            const float dy0 = (pj - A.cy) / A.fy;
-            const float dy = A.inverse_y ? dy0 : -dy0;
-            const float dz = A.inverse_y ? 1.f : -1.f;
-            float rd[3];
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                o[a] = (A.c2w[4 * a + 3] - A.mn[a]) / A.ext[a];
-                rd[a] = dx * A.c2w[4 * a] + dy * A.c2w[4 * a + 1] + dz * A.c2w[4 * a + 2];
-            }
-            const float len = sqrtf(rd[0] * rd[0] + rd[1] * rd[1] + rd[2] * rd[2]);
-            step = A.stepdist / len;
-            float lo = -INFINITY, hi = INFINITY;
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                d[a] = rd[a] / A.ext[a];
-                const float v = d[a] == 0.f ? 1e-6f : d[a];
-                const float ta = (1.f - o[a]) / v, tb = (-o[a]) / v;
-                lo = a == 0 ? fminf(ta, tb) : fmaxf(lo, fminf(ta, tb));
-                hi = a == 0 ? fmaxf(ta, tb) : fminf(hi, fmaxf(ta, tb));
-            }
-            tmin = fmaxf(fminf(lo, 1e9f), A.near_);  // far is forced to 1e9 (plenvdb.h:1008)
-            tmax = fmaxf(fminf(hi, 1e9f), A.near_);
-            const float inv = 1.f / len;
-            const float vd[3] = {rd[0] * inv, rd[1] * inv, rd[2] * inv};
-            float p27[27];
-            p27[0] = vd[0];
-            p27[1] = vd[1];
-            p27[2] = vd[2];
-            float base = 1.f;
-#pragma unroll
-            for (int q = 3; q < 7; ++q) {
-                p27[q] = sinf(vd[0] * base);
-                p27[q + 4] = sinf(vd[1] * base);
-                p27[q + 8] = sinf(vd[2] * base);
-                p27[q + 12] = cosf(vd[0] * base);
-                p27[q + 16] = cosf(vd[1] * base);
-                p27[q + 20] = cosf(vd[2] * base);
-                base *= 2.f;
-            }
-#pragma unroll
-            for (int q = 0; q < 13; ++q) pe[q] = pack2(p27[2 * q], p27[2 * q + 1]);
-            pe[13] = pack2(p27[26], 1.f);  // PE 26 and the bias input
+    const FrameBuffers& F = A.fb;
+    const unsigned long long tot = *F.total;
+    if (A.counters && blockIdx.x == 0 && threadIdx.x == 0) A.counters[2] = tot;
+    const int64_t n = (int64_t)(tot < (unsigned long long)F.capacity ? tot : F.capacity);
+    const int64_t nchunks = (n + 63) / 64;
+    for (int64_t ch = (int64_t)blockIdx.x * kWaves + wv; ch < nchunks;
+         ch += (int64_t)gridDim.x * kWaves) {
+        const int64_t q = ch * 64 + lane;
+        const bool live = q < n;
+        uint4 a = make_uint4(0u, 0u, 0u, 0u), b = a;
+        if (live) {
+            a = F.rec[2 * q];
+            b = F.rec[2 * q + 1];
+            const uint4* pr = F.pe + 4 * (int64_t)b.w;
+            row[2] = pr[0];
+            row[3] = pr[1];
+            row[4] = pr[2];
+            row[5] = pr[3];
+        } else {
+            row[2] = row[3] = row[4] = row[5] = make_uint4(0u, 0u, 0u, 0u);
         }
-        // march bound: the reference loop ends when t reaches tmax; the cap only
-        // guards steps too small to advance t
-        const float span = (tmax - tmin) / step;
-        const int max_iter = (span >= 0.f) ? (int)fminf(span * 1.0625f + 16.f, 16777216.f) : 0;
-
-        float t = tmin, T = 1.f, acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, wt_keep = 0.f;
-        bool done = !valid || !(t < tmax), have = false, stop_after = false;
-        int iter = 0;
-        BlockCache bc;
-        while (true) {
-            while (!done && !have) {
-                if (!(t < tmax) || iter >= max_iter) {
-                    done = true;
-                    break;
-                }
-                t += step;
-                ++iter;
-                const float p0 = o[0] + t * d[0], p1 = o[1] + t * d[1], p2 = o[2] + t * d[2];
-                if (!(p0 >= 0.f && p1 >= 0.f && p2 >= 0.f && p0 <= 1.f && p1 <= 1.f && p2 <= 1.f))
-                    continue;
-                const float x = p0 * A.ws[0], y = p1 * A.ws[1], z = p2 * A.ws[2];
-                // isActive(Round(xyz)): rintf, half to even (NanoVDB.h:774-779)
-                {
-                    const int vi = (int)rintf(x), vj = (int)rintf(y), vk = (int)rintf(z);
-                    const int32_t l = bc.get(topo, vi, vj, vk);
-                    if (l < 0) {
-                        // Empty space: the rounded position is in a missing leaf, so
-                        // every step until the ray leaves that leaf's rounding box
-                        // [8L - 0.5, 8L + 7.5) is skipped by this same test.  Take
-                        // those steps with the accumulation alone (t stays the
-                        // reference's running sum) and stop two steps short of the
-                        // computed exit, which covers its rounding.
-                        const int L[3] = {vi >> 3, vj >> 3, vk >> 3};
-                        float te = INFINITY;
+        row[0] = a;
+        row[1] = make_uint4(b.x, b.y, 0u, 0u);
+        const unsigned long long lm = __ballot(live);
+        unsigned tmask = 0;
 #pragma unroll
-                        for (int a = 0; a < 3; ++a) {
-                            if (d[a] == 0.f) continue;
-                            const float xb = d[a] > 0.f ? 8.f * (float)L[a] + 7.5f
-                                                        : 8.f * (float)L[a] - 0.5f;
-                            te = fminf(te, (xb / A.ws[a] - o[a]) / d[a]);
-                        }
-                        const float tsafe = te - 2.f * step;
-                        while (t + step < tsafe && t < tmax && iter < max_iter) {
-                            t += step;
-                            ++iter;
-                        }
-                        continue;
-                    }
-                    const int off = voxel_offset(vi, vj, vk);
-                    if (!((topo.masks[(int64_t)l * 8 + (off >> 6)] >> (off & 63)) & 1ull)) continue;
-                }
-                // corners with int() truncation (renderer.cu:271-300), x >= 0 here
-                const int i0 = (int)x, j0 = (int)y, k0 = (int)z;
-                const float u = x - (float)i0, v = y - (float)j0, w = z - (float)k0;
-                int64_t slot[8];
-                float sc[8];
-#pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int cx = (kCx >> q) & 1, cy = (kCy >> q) & 1, cz = (kCz >> q) & 1;
-                    const int32_t l = bc.get(topo, i0 + cx, j0 + cy, k0 + cz);
-                    slot[q] = l < 0 ? miss
-                                    : (int64_t)l * 512 + voxel_offset(i0 + cx, j0 + cy, k0 + cz);
-                    const float a = cx ? u : 1.f - u, b = cy ? v : 1.f - v, c = cz ? w : 1.f - w;
-                    sc[q] = (a * b) * c;
-                }
-                // trigetDensity2 order: sum of den*scale, left to right
-                float sig = load_den<F32PAY>(A.payload, slot[0]) * sc[0];
-#pragma unroll
-                for (int q = 1; q < 8; ++q) sig = sig + load_den<F32PAY>(A.payload, slot[q]) * sc[q];
-                ++n_gather;
-                const float e = expf(sig + A.act_shift);
-                const float alpha = 1.f - powf(1.f + e, -A.interval);
-                if (alpha <= A.thres) continue;
-                const float wt = T * alpha;
-                T = T * (1.f - alpha);
-                if (wt <= A.thres) continue;
-                // kept: colour features (trigetColor order) staged as this lane's input row
-                float f[12], cq[12];
-                load_col<F32PAY>(A.payload, slot[0], cq);
-#pragma unroll
-                for (int c = 0; c < 12; ++c) f[c] = cq[c] * sc[0];
-#pragma unroll
-                for (int q = 1; q < 8; ++q) {
-                    load_col<F32PAY>(A.payload, slot[q], cq);
-#pragma unroll
-                    for (int c = 0; c < 12; ++c) f[c] = f[c] + cq[c] * sc[q];
-                }
-                uint4* row = reinterpret_cast<uint4*>(sX + lane * XS);
-                row[0] = make_uint4(pack2(f[0], f[1]), pack2(f[2], f[3]), pack2(f[4], f[5]),
-                                    pack2(f[6], f[7]));
-                row[1] = make_uint4(pack2(f[8], f[9]), pack2(f[10], f[11]), pe[0], pe[1]);
-                row[2] = make_uint4(pe[2], pe[3], pe[4], pe[5]);
-                row[3] = make_uint4(pe[6], pe[7], pe[8], pe[9]);
-                row[4] = make_uint4(pe[10], pe[11], pe[12], pe[13]);
-                wt_keep = wt;
-                have = true;
-                ++n_kept;
-                // first_look_merged stops after a kept sample with T below the threshold
-                if (T < A.t_stop) stop_after = true;
-            }
-            const unsigned long long hm = __ballot(have);
-            if (hm == 0ull) break;  // every lane is done
-            unsigned tmask = 0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if ((hm >> (16 * q)) & 0xffffull) tmask |= 1u << q;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            wave_mlp(sW, sB, sX, sL, tmask, lane);
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            if (have) {
-                // final_render: weight / (1 + exp(-raw)) (renderer.cu:112-119)
-                const float* lg = sL + lane * 4;
-                acc0 = acc0 + wt_keep / (1.f + expf(-lg[0]));
-                acc1 = acc1 + wt_keep / (1.f + expf(-lg[1]));
-                acc2 = acc2 + wt_keep / (1.f + expf(-lg[2]));
-                have = false;
-                if (stop_after) done = true;
-            }
-            __builtin_amdgcn_wave_barrier();
+        for (int k = 0; k < 4; ++k)
+            if ((lm >> (16 * k)) & 0xffffull) tmask |= 1u << k;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        wave_mlp(sW, sB, sX, sL, tmask, lane);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        if (live) {
+            const float w = __uint_as_float(b.z);
+            const float* lg = sL + lane * 4;
+            F.contrib[q] = make_float4(w / (1.f + expf(-lg[0])), w / (1.f + expf(-lg[1])),
+                                       w / (1.f + expf(-lg[2])), 0.f);
         }
-        if (valid) {
-            const float last = T * A.bg;
-            float* dst = A.out + ((int64_t)py * A.W + px) * 3;
-            dst[0] = last + acc0;
-            dst[1] = last + acc1;
-            dst[2] = last + acc2;
-        }
-    }
-    if (A.counters) {
-        // wave sums, one atomic per wave
-        for (int off = 32; off > 0; off >>= 1) {
-            n_gather += __shfl_xor(n_gather, off, 64);
-            n_kept += __shfl_xor(n_kept, off, 64);
-        }
-        if (lane == 0) {
-            atomicAdd(A.counters, n_gather);
-            atomicAdd(A.counters + 1, n_kept);
-        }
+        __builtin_amdgcn_wave_barrier();  // sL and the rows are rewritten by the next chunk
     }
 }
 
-__global__ void k_render_reset(int32_t* tile_counter, unsigned long long* counters) {
-    if (threadIdx.x == 0) *tile_counter = 0;
-    if (counters && threadIdx.x < 2) counters[threadIdx.x] = 0ull;
+// Pixel colour: T_last * bg + the pixel's contributions in sample order.
+__global__ void __launch_bounds__(256) k_render_composite(RenderArgs A) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (int64_t)A.H * A.W) return;
+    const FrameBuffers& F = A.fb;
+    const unsigned long long tot = *F.total;
+    const int64_t n = (int64_t)(tot < (unsigned long long)F.capacity ? tot : F.capacity);
+    const int64_t s = F.pix_start[p];
+    const int wn = F.pix_written[p];
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    for (int k = 0; k < wn; ++k) {
+        const int64_t q = s + k;
+        if (q >= n) break;
+        const float4 c = F.contrib[q];
+        a0 = a0 + c.x;
+        a1 = a1 + c.y;
+        a2 = a2 + c.z;
+    }
+    const float last = F.t_last[p] * A.bg;
+    A.out[3 * p] = last + a0;
+    A.out[3 * p + 1] = last + a1;
+    A.out[3 * p + 2] = last + a2;
+}
+
+__global__ void k_render_reset(uint32_t* tile_counter, unsigned long long* total,
+                               unsigned long long* counters) {
+    if (threadIdx.x == 0) {
+        *tile_counter = 0u;
+        *total = 0ull;
+    }
+    if (counters && threadIdx.x < 3) counters[threadIdx.x] = 0ull;
 }
 
 constexpr size_t kRenderSmem =
     (size_t)kPackedBytes + (size_t)kWaves * 64 * XS * 2 + (size_t)kWaves * 64 * 4 * 4;
+
+inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Optional device timing of the three kernels (HIP events around each launch).
+struct RenderTimer {
+    bool enabled = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> open;
+    double total_ms[3] = {0, 0, 0};
+};
+RenderTimer g_rtimer;
+
+struct RenderStage {
+    int stage;
+    hipStream_t s;
+    hipEvent_t a = nullptr, b = nullptr;
+    RenderStage(int st, hipStream_t str) : stage(st), s(str) {
+        if (!g_rtimer.enabled) return;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        hipEventRecord(a, s);
+    }
+    ~RenderStage() {
+        if (!g_rtimer.enabled || !a) return;
+        hipEventRecord(b, s);
+        g_rtimer.open.push_back({stage, {a, b}});
+    }
+};
+
+struct FrameLayout {
+    size_t tile, total, start, count, written, tlast, pe, rec, contrib, bytes;
+};
+
+FrameLayout frame_layout(int64_t hw, int64_t capacity) {
+    FrameLayout L;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = align256(o + bytes);
+        return at;
+    };
+    L.tile = take(4);
+    L.total = take(8);
+    L.start = take(4 * (size_t)hw);
+    L.count = take(4 * (size_t)hw);
+    L.written = take(4 * (size_t)hw);
+    L.tlast = take(4 * (size_t)hw);
+    L.pe = take(64 * (size_t)hw);
+    L.rec = take(32 * (size_t)capacity);
+    L.contrib = take(16 * (size_t)capacity);
+    L.bytes = o;
+    return L;
+}
+
+FrameBuffers frame_buffers(void* ws, int64_t hw, int64_t capacity) {
+    const FrameLayout L = frame_layout(hw, capacity);
+    uint8_t* b = reinterpret_cast<uint8_t*>(ws);
+    FrameBuffers F;
+    F.tile_counter = reinterpret_cast<uint32_t*>(b + L.tile);
+    F.total = reinterpret_cast<unsigned long long*>(b + L.total);
+    F.pix_start = reinterpret_cast<int32_t*>(b + L.start);
+    F.pix_count = reinterpret_cast<int32_t*>(b + L.count);
+    F.pix_written = reinterpret_cast<int32_t*>(b + L.written);
+    F.t_last = reinterpret_cast<float*>(b + L.tlast);
+    F.pe = reinterpret_cast<uint4*>(b + L.pe);
+    F.rec = reinterpret_cast<uint4*>(b + L.rec);
+    F.contrib = reinterpret_cast<float4*>(b + L.contrib);
+    F.capacity = capacity;
+    return F;
+}
 
 }  // namespace
 
@@ -537,6 +796,26 @@ int pvdb_render_build_payload(const pvdb_topology* topo, const int32_t* slot_idx
     return PVDB_OK;
 }
 
+int64_t pvdb_render_workspace_bytes(int32_t H, int32_t W, int64_t capacity) {
+    if (H <= 0 || W <= 0 || capacity < 0 || capacity >= kMaxRecords) return -1;
+    return (int64_t)frame_layout((int64_t)H * W, capacity).bytes;
+}
+
+int pvdb_render_frame_info(const pvdb_render_desc* d, uint64_t* total, int32_t** pix_count,
+                           int32_t** pix_written, int32_t** pix_start) {
+    PVDB_CHECK_ARG(d && d->workspace, "null descriptor or workspace");
+    const FrameBuffers F = frame_buffers(d->workspace, (int64_t)d->H * d->W, d->capacity);
+    if (total) {
+        unsigned long long v = 0;
+        PVDB_HIP(hipMemcpy(&v, F.total, sizeof(v), hipMemcpyDeviceToHost));
+        *total = v;
+    }
+    if (pix_count) *pix_count = F.pix_count;
+    if (pix_written) *pix_written = F.pix_written;
+    if (pix_start) *pix_start = F.pix_start;
+    return PVDB_OK;
+}
+
 int pvdb_render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void* stream) {
     if (!topo) {
         pvdb::set_error("null topology");
@@ -546,11 +825,18 @@ int pvdb_render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void
     PVDB_CHECK_ARG(d->H > 0 && d->W > 0 && (int64_t)d->H * d->W < (int64_t(1) << 31),
                    "bad image size");
     PVDB_CHECK_ARG(d->stepdist > 0.f && std::isfinite(d->stepdist), "stepdist must be positive");
+    PVDB_CHECK_ARG(d->capacity >= 0 && d->capacity < kMaxRecords,
+                   "record capacity must be in [0, 2^31 - 2]");
+    PVDB_CHECK_ARG(reinterpret_cast<uintptr_t>(d->workspace) % 256 == 0,
+                   "workspace must be 256-byte aligned");
+    PVDB_CHECK_ARG(d->workspace_bytes >= pvdb_render_workspace_bytes(d->H, d->W, d->capacity),
+                   "workspace too small");
     for (int a = 0; a < 3; ++a) {
         PVDB_CHECK_ARG(d->reso[a] >= 2, "reso must be at least 2");
         PVDB_CHECK_ARG(d->xyz_max[a] > d->xyz_min[a], "empty bounding box");
     }
     const TopoView t = view_of(topo);
+    PVDB_CHECK_ARG(t.nleaf * 512 + 1 < kMaxRecords, "too many leaves for the renderer's slot indices");
     RenderArgs A;
     A.topo = t;
     A.payload = d->payload;
@@ -558,7 +844,7 @@ int pvdb_render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void
     A.packed = reinterpret_cast<const uint4*>(d->weights);
     A.out = d->out;
     A.counters = reinterpret_cast<unsigned long long*>(d->counters);
-    A.tile_counter = reinterpret_cast<int32_t*>(d->workspace);
+    A.fb = frame_buffers(d->workspace, (int64_t)d->H * d->W, d->capacity);
     for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 4; ++c) A.c2w[4 * r + c] = d->c2w[4 * r + c];
     A.fx = d->K[0];
@@ -583,23 +869,62 @@ int pvdb_render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void
     A.bg = d->bg;
     A.t_stop = d->t_stop;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_render_reset, dim3(1), dim3(64), 0, s, A.tile_counter, A.counters);
+    hipLaunchKernelGGL(k_render_reset, dim3(1), dim3(64), 0, s, A.fb.tile_counter, A.fb.total,
+                       A.counters);
     PVDB_LAUNCHED("k_render_reset");
-    int dev = 0, ncu = 0;
-    PVDB_HIP(hipGetDevice(&dev));
-    PVDB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const int want = (A.ntiles + kWaves - 1) / kWaves;
-    const int grid = want < ncu ? (want < 1 ? 1 : want) : ncu;
-    if (d->payload_f32) {
-        PVDB_HIP(hipFuncSetAttribute((const void*)k_render<true>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRenderSmem));
-        hipLaunchKernelGGL(k_render<true>, dim3(grid), dim3(64 * kWaves), kRenderSmem, s, A);
-    } else {
-        PVDB_HIP(hipFuncSetAttribute((const void*)k_render<false>,
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRenderSmem));
-        hipLaunchKernelGGL(k_render<false>, dim3(grid), dim3(64 * kWaves), kRenderSmem, s, A);
+    static int ncu = 0;
+    if (ncu == 0) {
+        int dev = 0;
+        PVDB_HIP(hipGetDevice(&dev));
+        PVDB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     }
-    PVDB_LAUNCHED("k_render");
+    // marching: persistent waves, 8 four-wave workgroups per CU, at most one wave per tile
+    const int waves = kMarchThreads / 64;
+    const int want = (A.ntiles + waves - 1) / waves;
+    const int mgrid = want < 8 * ncu ? want : 8 * ncu;
+    {
+        RenderStage st(0, s);
+        if (d->payload_f32)
+            hipLaunchKernelGGL(k_render_march<true>, dim3(mgrid), dim3(kMarchThreads), 0, s, A);
+        else
+            hipLaunchKernelGGL(k_render_march<false>, dim3(mgrid), dim3(kMarchThreads), 0, s, A);
+        PVDB_LAUNCHED("k_render_march");
+    }
+    static bool smem_set = false;
+    if (!smem_set) {
+        PVDB_HIP(hipFuncSetAttribute((const void*)k_render_mlp,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)kRenderSmem));
+        smem_set = true;
+    }
+    {
+        RenderStage st(1, s);
+        hipLaunchKernelGGL(k_render_mlp, dim3(ncu), dim3(64 * kWaves), kRenderSmem, s, A);
+        PVDB_LAUNCHED("k_render_mlp");
+    }
+    {
+        RenderStage st(2, s);
+        hipLaunchKernelGGL(k_render_composite, dim3(pvdb_blocks((int64_t)A.H * A.W, 256)),
+                           dim3(256), 0, s, A);
+        PVDB_LAUNCHED("k_render_composite");
+    }
+    return PVDB_OK;
+}
+
+int pvdb_render_timing(int32_t enable, double* total_ms3) {
+    for (auto& e : g_rtimer.open) {
+        hipEventSynchronize(e.second.second);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e.second.first, e.second.second);
+        g_rtimer.total_ms[e.first] += ms;
+        hipEventDestroy(e.second.first);
+        hipEventDestroy(e.second.second);
+    }
+    g_rtimer.open.clear();
+    for (int i = 0; i < 3; ++i) {
+        if (total_ms3) total_ms3[i] = g_rtimer.total_ms[i];
+        if (enable >= 0) g_rtimer.total_ms[i] = 0;
+    }
+    if (enable >= 0) g_rtimer.enabled = enable != 0;
     return PVDB_OK;
 }
 

@@ -65,6 +65,18 @@ def positional_encoding(viewdirs: torch.Tensor, viewfreq: torch.Tensor) -> torch
     return torch.cat([viewdirs, emb.sin(), emb.cos()], -1)
 
 
+def build_rgbnet(dim0: int, width: int = 128, depth: int = 3) -> nn.Module:
+    """The rgbnet of dvgo.py:91-109: Linear-ReLU, (depth-2) x [Linear-ReLU], Linear
+    to 3, last bias zero."""
+    layers = [nn.Linear(dim0, width), nn.ReLU(inplace=True)]
+    for _ in range(depth - 2):
+        layers += [nn.Sequential(nn.Linear(width, width), nn.ReLU(inplace=True))]
+    layers += [nn.Linear(width, 3)]
+    net = nn.Sequential(*layers)
+    nn.init.constant_(net[-1].bias, 0)
+    return net
+
+
 class DirectVoxGO(nn.Module):
     """Fine-stage DVGO on VDB grids (dvgo.py:24-388), VDBGrid density + k0.
 
@@ -100,12 +112,7 @@ class DirectVoxGO(nn.Module):
         if rgbnet_dim > 0:
             self.register_buffer("viewfreq", torch.tensor([2.0 ** i for i in range(viewbase_pe)]))
             dim0 = 3 + 3 * viewbase_pe * 2 + self.k0_dim
-            layers = [nn.Linear(dim0, rgbnet_width), nn.ReLU(inplace=True)]
-            for _ in range(rgbnet_depth - 2):
-                layers += [nn.Sequential(nn.Linear(rgbnet_width, rgbnet_width), nn.ReLU(inplace=True))]
-            layers += [nn.Linear(rgbnet_width, 3)]
-            self.rgbnet = nn.Sequential(*layers)
-            nn.init.constant_(self.rgbnet[-1].bias, 0)
+            self.rgbnet = build_rgbnet(dim0, rgbnet_width, rgbnet_depth)
             if mlp_params is not None:
                 self.load_mlp(mlp_params)
         else:

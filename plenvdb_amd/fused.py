@@ -42,6 +42,7 @@ class StepDesc(C.Structure):
         ("launch", C.c_int32 * 4),
         ("weight_main", C.c_float), ("weight_entropy_last", C.c_float),
         ("weight_rgbper", C.c_float), ("reserved2", C.c_int32),
+        ("color_master", C.c_void_p),
     ]
 
 
@@ -98,7 +99,7 @@ class FusedTrainer:
     def __init__(self, scene: dict, topo: Topology | None = None, color_bf16: bool | None = None,
                  lr_grid=0.1, lr_mlp=1e-3, lrate_decay=20, capacity: int | None = None,
                  weight_main: float = 1.0, weight_entropy_last: float = 0.0,
-                 weight_rgbper: float = 0.0, device="cuda"):
+                 weight_rgbper: float = 0.0, color_master: bool = True, device="cuda"):
         cfg = scene["cfg"]
         if not cfg.use_mlp:
             raise ValueError("the fused path runs the MLP configurations")
@@ -111,6 +112,8 @@ class FusedTrainer:
         self.density = torch.from_numpy(den).to(dev)
         colt = torch.from_numpy(col).to(dev)
         self.color = colt.to(torch.bfloat16).contiguous() if self.color_bf16 else colt
+        # the bf16 payload's fp32 master, which the colour Adam updates
+        self.color_master = colt if (self.color_bf16 and color_master) else None
         self.grad_density = torch.zeros(ns, device=dev)
         self.grad_color = torch.zeros(ns, 12, device=dev)
         self.m_density = torch.zeros(ns, device=dev)
@@ -192,6 +195,7 @@ class FusedTrainer:
         d.loss = self.loss.data_ptr()
         d.adam_scalars = self.adam_scalars.data_ptr()
         d.weight_main, d.weight_entropy_last, d.weight_rgbper = self.weights
+        d.color_master = None if self.color_master is None else self.color_master.data_ptr()
         return d
 
     @staticmethod
@@ -205,24 +209,44 @@ class FusedTrainer:
         lib.call("pvdb_fused_step", self.topo.handle, C.byref(self.desc), lib.stream())
         return self.loss
 
-    def step(self):
-        """One full iteration: forward, backward, Adam (grids stepmode 1, MLP
-        stepmode 0), gradients re-zeroed, lr decay (run.py:592-598).  Replays
-        the captured graph when `capture_graph` was called."""
+    def _next_step_sizes(self) -> list:
+        """Advances the step counter; the Adam step sizes of this step (grid,
+        grid, MLP), also written to the device buffer a captured step reads."""
         self.step_count += 1
         t = self.step_count
         sz = [self._stepsz(self.lr_grid, t), self._stepsz(self.lr_grid, t),
               self._stepsz(self.lr_mlp, t), 0.0]
         self.adam_scalars.copy_(torch.tensor(sz, dtype=torch.float32))
+        self.desc.stepsz_density, self.desc.stepsz_color, self.desc.step_size_mlp = sz[:3]
+        return sz
+
+    def _decay_lr(self):
+        # lr decay of run.py:592-598 (the VDB optimizers multiply in float)
+        self.lr_grid = _f32(np.float32(self.lr_grid) * np.float32(self.decay))
+        self.lr_mlp = self.lr_mlp * self.decay
+
+    def step(self):
+        """One full iteration: forward, backward, Adam (grids stepmode 1, MLP
+        stepmode 0), gradients re-zeroed, lr decay (run.py:592-598).  Replays
+        the captured graph when `capture_graph` was called."""
+        self._next_step_sizes()
         if self.graph is not None:
             self.graph.replay()
         else:
             self.desc.do_adam = 1
-            self.desc.stepsz_density, self.desc.stepsz_color, self.desc.step_size_mlp = sz[:3]
             lib.call("pvdb_fused_step", self.topo.handle, C.byref(self.desc), lib.stream())
-        self.lr_grid = _f32(np.float32(self.lr_grid) * np.float32(self.decay))
-        self.lr_mlp = self.lr_mlp * self.decay
+        self._decay_lr()
         return self.loss
+
+    def adam_only(self):
+        """The optimizer step alone on the gradients accumulated by earlier
+        `forward_backward` calls (VDBAdam.step + MaskedAdam.step after a
+        separate backward): grid Adam over the touched leaves with their
+        gradients re-zeroed, MLP Adam on `mlp_grad`, then the lr decay."""
+        self._next_step_sizes()
+        self.desc.do_adam = 2
+        lib.call("pvdb_fused_step", self.topo.handle, C.byref(self.desc), lib.stream())
+        self._decay_lr()
 
     def capture_graph(self):
         """Captures one full step (do_adam) into a hipGraph; later `step()` calls
@@ -248,6 +272,93 @@ class FusedTrainer:
         if self.graph is not None:
             self.capture_graph()
         return out
+
+    # -- checkpoints: the plain path's file set (PlainTrainer.save_checkpoint) --
+    def _rgbnet(self):
+        """The MLP as DirectVoxGO's rgbnet module (its state_dict layout)."""
+        from .dvgo import build_rgbnet
+
+        net = build_rgbnet(39, 128, 3)
+        p = unpack_mlp(self.mlp.detach().cpu().numpy())
+        lin = [m for m in net.modules() if isinstance(m, torch.nn.Linear)]
+        with torch.no_grad():
+            for i, l in enumerate(lin):
+                l.weight.copy_(torch.from_numpy(np.ascontiguousarray(p[f"w{i}"].T)))
+                l.bias.copy_(torch.from_numpy(p[f"b{i}"]))
+        return net, lin
+
+    def save_to(self, prefix: str, global_step: int | None = None) -> None:
+        """Writes density.vdb, color.vdb (the fp32 master of a bf16 grid), the
+        grid moments and vdbopt.tar (masked_adam.py:83-100 keys) and last.tar
+        with the MLP as rgbnet_state_dict plus its Adam state in torch's
+        optimizer format, so PlainTrainer.load_checkpoint reads the same files."""
+        from . import vdbio
+
+        t, reso = self.topo, self.scene["reso"]
+        col = self.color_master if self.color_master is not None else self.color.float()
+        for name, val in (("density.vdb", self.density), ("color.vdb", col),
+                          ("density_exp_avg.vdb", self.m_density),
+                          ("density_exp_avg_sq.vdb", self.v_density),
+                          ("color_exp_avg.vdb", self.m_color), ("color_exp_avg_sq.vdb", self.v_color)):
+            vdbio.save_grid(prefix + name, t, val, reso)
+        sc = {"lr": self.lr_grid, "eps": 1e-8, "beta0": 0.9, "beta1": 0.99, "step": self.step_count}
+        torch.save({k + n: v for k in ("d", "c") for n, v in sc.items()}, prefix + "vdbopt.tar")
+        net, lin = self._rgbnet()
+        pm = unpack_mlp(self.mlp_m.detach().cpu().numpy())
+        pv = unpack_mlp(self.mlp_v.detach().cpu().numpy())
+        state = {}
+        for i in range(3):
+            for j, (kw, tr) in enumerate(((f"w{i}", True), (f"b{i}", False))):
+                conv = (lambda a: np.ascontiguousarray(a.T)) if tr else np.ascontiguousarray
+                state[2 * i + j] = {"step": self.step_count,
+                                    "exp_avg": torch.from_numpy(conv(pm[kw])),
+                                    "exp_avg_sq": torch.from_numpy(conv(pv[kw]))}
+        opt = {"state": state, "param_groups": [{"lr": self.lr_mlp, "betas": (0.9, 0.99),
+                                                 "eps": 1e-8, "skip_zero_grad": False,
+                                                 "params": list(range(6))}]}
+        torch.save({"global_step": int(self.step_count if global_step is None else global_step),
+                    "rgbnet_state_dict": net.state_dict(), "optimizer_state_dict": opt},
+                   prefix + "last.tar")
+
+    def load_from(self, prefix: str) -> int:
+        """Reads the files save_to (or PlainTrainer.save_checkpoint) writes; the
+        grids are pruned on load like the reference's, the moments follow them
+        by leaf origin.  Returns the saved global step."""
+        from . import vdbio
+
+        topo, den, _ = vdbio.load_grid(prefix + "density.vdb", 1)
+        ctopo, col, _ = vdbio.load_grid(prefix + "color.vdb", 12)
+        if ctopo.nleaf != topo.nleaf:  # a colour leaf pruned alone: keep the density's set
+            col = vdbio.load_onto(prefix + "color.vdb", topo, 12)
+        if topo.nleaf != self.topo.nleaf:
+            raise ValueError("the checkpoint's leaf set differs from this trainer's topology")
+        self.density.copy_(den)
+        if self.color_master is not None:
+            self.color_master.copy_(col)
+        self.color.copy_(col.to(self.color.dtype))
+        for name, dst in (("density_exp_avg", self.m_density), ("density_exp_avg_sq", self.v_density),
+                          ("color_exp_avg", self.m_color), ("color_exp_avg_sq", self.v_color)):
+            dst.copy_(vdbio.load_onto(prefix + name + ".vdb", self.topo, dst.numel() // self.topo.nslots))
+        st = torch.load(prefix + "vdbopt.tar", weights_only=True)
+        self.lr_grid = _f32(st["dlr"])
+        self.step_count = int(st["dstep"])
+        ck = torch.load(prefix + "last.tar", weights_only=True)
+        net, lin = self._rgbnet()
+        net.load_state_dict(ck["rgbnet_state_dict"])
+        vec = lambda ts: np.concatenate([  # noqa: E731
+            np.ascontiguousarray(t.detach().cpu().numpy().T if t.dim() == 2 else t.detach().cpu().numpy()
+                                 ).ravel() for t in ts]).astype(np.float32)
+        self.mlp.copy_(torch.from_numpy(vec([x for l in lin for x in (l.weight, l.bias)])))
+        opt = ck["optimizer_state_dict"]
+        if opt is not None:
+            order = sorted(opt["state"])
+            self.mlp_m.copy_(torch.from_numpy(vec([opt["state"][k]["exp_avg"] for k in order])))
+            self.mlp_v.copy_(torch.from_numpy(vec([opt["state"][k]["exp_avg_sq"] for k in order])))
+            self.lr_mlp = float(opt["param_groups"][0]["lr"])
+        self.grad_density.zero_()
+        self.grad_color.zero_()
+        lib.call("pvdb_fused_init", self.topo.handle, C.byref(self.desc), lib.stream())
+        return int(ck["global_step"])
 
     def zero_grad(self):
         self.grad_density.zero_()

@@ -42,7 +42,7 @@ _SIGS = {
     "pvdb_tv_add_grad": [P, P, P, I32, I32, I32, I32, F32, F32, F32, I32, P],
     "pvdb_vdb_adam": [P, P, P, P, P, P, I32, I32, I32, F32, F32, F32, F32, P],
     "pvdb_dense_adam": [P, P, P, P, P, I64, I32, F32, F32, F32, F32, P],
-    "pvdb_rays_of_view": [I32, I32, P, P, I32, I32, I32, I32, I32, P, P, P, P],
+    "pvdb_rays_of_view": [I32, I32, P, P, I32, I32, I32, I32, I32, P, P, P, P, P],
     "pvdb_ray_bounds": [P, P, I64, P, P, F32, F32, P, F32, P, P, P, P],
     "pvdb_sample_pts": [P, P, P, P, I64, I64, F32, P, P, P, P, P, P, P],
     "pvdb_mask_lookup": [P, I32, I32, I32, P, I64, P, P, P, P],

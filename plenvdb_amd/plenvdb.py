@@ -279,14 +279,15 @@ class _BaseOpt:
         vdbio.save_grid(prefix + "exp_avg_sq.vdb", self.params.topo, self.exp_avg_sq, self.params.reso)
 
     def load_from(self, prefix: str) -> None:
-        """plenvdb.h:720-728; the moments must live on the parameter's topology."""
+        """prefix+'exp_avg.vdb', prefix+'exp_avg_sq.vdb' (plenvdb.h:720-728).  The
+        moments are placed on the parameter's topology by leaf origin, so they
+        stay paired with their voxels when the parameter's leaf set was pruned
+        on load."""
         from . import vdbio
 
         for name in ("exp_avg", "exp_avg_sq"):
-            topo, vals, _ = vdbio.load_grid(prefix + name + ".vdb", self.params.ndim)
-            if topo.nleaf != self.params.topo.nleaf:
-                raise ValueError(f"{name}: topology does not match the parameter grid")
-            setattr(self, name, vals)
+            setattr(self, name, vdbio.load_onto(prefix + name + ".vdb", self.params.topo,
+                                                self.params.ndim))
 
     def getinfo(self) -> None:  # plenvdb.h:739-744
         p = self.params

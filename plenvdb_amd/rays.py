@@ -17,25 +17,39 @@ import torch
 
 from . import lib
 
-_MODES = {"center": 0, "lefttop": 1}
+_MODES = {"center": 0, "lefttop": 1, "random": 2}
+
+
+def random_jitter(H: int, W: int, device) -> torch.Tensor:
+    """The 'random' mode's pixel offsets [2, H, W], drawn from torch's generator
+    with the reference's calls (dvgo.py:472-484: rand_like of the transposed
+    meshgrid for i, then for j), so a seeded run draws the same numbers."""
+    i, j = torch.meshgrid(torch.linspace(0, W - 1, W, device=device),
+                          torch.linspace(0, H - 1, H, device=device), indexing="ij")
+    i = i.t().float()
+    j = j.t().float()
+    ri = torch.rand_like(i)
+    rj = torch.rand_like(j)
+    return torch.stack([ri, rj]).contiguous()
 
 
 def get_rays_of_a_view(H, W, K, c2w, ndc, inverse_y, flip_x, flip_y, mode="center"):
     """dvgo.py:522-527: rays_o, rays_d, viewdirs [H, W, 3] (float32, device)."""
     if mode not in _MODES:
-        raise NotImplementedError(f"get_rays mode {mode!r}: only 'center' and 'lefttop'")
+        raise NotImplementedError(f"get_rays mode {mode!r}")
     H, W = int(H), int(W)
     k = np.ascontiguousarray(np.asarray(K, np.float64).reshape(3, 3))
     c = c2w.detach().cpu().numpy() if isinstance(c2w, torch.Tensor) else np.asarray(c2w)
     c = np.ascontiguousarray(np.asarray(c, np.float32)[:3, :4])
     dev = torch.device("cuda")
+    jit = random_jitter(H, W, dev) if mode == "random" else None
     ro = torch.empty(H, W, 3, dtype=torch.float32, device=dev)
     rd = torch.empty_like(ro)
     vd = torch.empty_like(ro)
     lib.call("pvdb_rays_of_view", H, W, k.ctypes.data_as(C.c_void_p),
              c.ctypes.data_as(C.c_void_p), int(bool(ndc)), int(bool(inverse_y)),
              int(bool(flip_x)), int(bool(flip_y)), _MODES[mode], lib.ptr(ro), lib.ptr(rd),
-             lib.ptr(vd), lib.stream())
+             lib.ptr(vd), lib.ptr(jit), lib.stream())
     return ro, rd, vd
 
 

@@ -15,6 +15,8 @@ Differences: the index grid is read from this package's grid container
 (`set_t_stop`, default the reference's 1e-3); the MLP runs in fp16 with fp32
 accumulation; the timer counts wall time around the render and its
 synchronisation (the reference sums CPU clock() time, plenvdb.h:1030-1035).
+The kept samples of a frame live in a device workspace sized in records; a
+synchronous render that finds it too small grows it and renders again.
 """
 from __future__ import annotations
 
@@ -38,7 +40,8 @@ class RenderDesc(C.Structure):
         ("interval", C.c_float), ("fast_color_thres", C.c_float), ("bg", C.c_float),
         ("t_stop", C.c_float), ("payload_f32", C.c_int32),
         ("payload", C.c_void_p), ("weights", C.c_void_p), ("out", C.c_void_p),
-        ("counters", C.c_void_p), ("workspace", C.c_void_p),
+        ("counters", C.c_void_p), ("workspace", C.c_void_p), ("workspace_bytes", C.c_int64),
+        ("capacity", C.c_int64),
     ]
 
 
@@ -47,6 +50,9 @@ lib.EXTRA_SIGS.update({
     "pvdb_render_build_payload": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                   C.c_int32, C.c_void_p, C.c_void_p],
     "pvdb_render_image": [C.c_void_p, C.POINTER(RenderDesc), C.c_void_p],
+    "pvdb_render_frame_info": [C.POINTER(RenderDesc), C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_void_p],
+    "pvdb_render_timing": [C.c_int32, C.c_void_p],
     "pvdb_mfma_selftest_f16": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
 })
 
@@ -88,8 +94,10 @@ class MGRenderer:
         self.payload = None
         self.topo = None
         self.image = None
-        self.counters = torch.zeros(2, dtype=torch.int64, device="cuda")
-        self.workspace = torch.zeros(64, dtype=torch.int32, device="cuda")
+        self.counters = torch.zeros(3, dtype=torch.int64, device="cuda")
+        self.workspace = None
+        self.capacity = 0
+        self._ws_hw = None
 
     # -- setters (plenvdb.h:959-1027) ------------------------------------------------
     def load_params(self, w0, b0, w1, b1, w2, b2) -> None:
@@ -150,6 +158,9 @@ class MGRenderer:
         d.inverse_y = int(bool(inverse_y))
         d.H, d.W = int(H), int(W)
         self.image = torch.zeros(int(H) * int(W) * 3, dtype=torch.float32, device="cuda")
+        # records per frame: start at 16 per pixel; a frame that needs more is
+        # rendered again with room for all of them (render_an_image)
+        self._reserve(16 * int(H) * int(W))
         self.flags[3] = True
 
     def input_a_c2w(self, c2w) -> None:
@@ -165,6 +176,41 @@ class MGRenderer:
         """Early-stop transmittance (the reference hard-codes 1e-3)."""
         self.t_stop = float(t)
 
+    def _reserve(self, capacity: int) -> None:
+        """Workspace for frames of up to `capacity` records (kept samples)."""
+        capacity = min(int(capacity), 2 ** 31 - 2)
+        d = self.desc
+        if self.workspace is not None and capacity <= self.capacity and \
+                self._ws_hw == (d.H, d.W):
+            return
+        capacity = max(capacity, self.capacity if self._ws_hw == (d.H, d.W) else 0)
+        nbytes = int(_i64_fn("pvdb_render_workspace_bytes", [C.c_int32, C.c_int32, C.c_int64])(
+            d.H, d.W, capacity))
+        if nbytes < 0:
+            raise ValueError("bad image size or record capacity")
+        self.workspace = None
+        self.workspace = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        self.capacity = capacity
+        self._ws_hw = (d.H, d.W)
+
+    def frame_total(self) -> int:
+        """Records (kept samples) the last frame needed (synchronises)."""
+        v = C.c_uint64()
+        lib.call("pvdb_render_frame_info", C.byref(self.desc), C.byref(v), None, None, None)
+        return int(v.value)
+
+    def pixel_counts(self) -> tuple[np.ndarray, np.ndarray]:
+        """Per pixel: the first march's kept count and the samples the second
+        march wrote (host copies of the last frame's)."""
+        pc, pw = C.c_void_p(), C.c_void_p()
+        lib.call("pvdb_render_frame_info", C.byref(self.desc), None, C.byref(pc), C.byref(pw),
+                 None)
+        n = self.desc.H * self.desc.W
+        torch.cuda.current_stream().synchronize()
+        from .fused import _read_i32
+
+        return _read_i32(pc.value, n), _read_i32(pw.value, n)
+
     # -- rendering (plenvdb.h:1029-1049) ----------------------------------------------
     def _launch(self) -> None:
         d = self.desc
@@ -175,6 +221,8 @@ class MGRenderer:
         d.out = self.image.data_ptr()
         d.counters = self.counters.data_ptr()
         d.workspace = self.workspace.data_ptr()
+        d.workspace_bytes = self.workspace.numel()
+        d.capacity = self.capacity
         lib.call("pvdb_render_image", self.topo.handle, C.byref(d), lib.stream())
 
     def render_an_image(self) -> None:
@@ -182,12 +230,19 @@ class MGRenderer:
             return
         t0 = time.perf_counter()
         self._launch()
-        torch.cuda.current_stream().synchronize()
+        need = self.frame_total()  # synchronises
+        if need > self.capacity:  # the frame kept more samples than there was room for
+            self._reserve(need + need // 4)
+            self._launch()
+            torch.cuda.current_stream().synchronize()
         self.timer += time.perf_counter() - t0
 
     def render_async(self) -> torch.Tensor:
         """Launches a render on the current stream without waiting; returns the
-        device image [H*W*3] (overwritten by the next render)."""
+        device image [H*W*3] (overwritten by the next render).  A frame with more
+        kept samples than the record capacity renders only those that fit:
+        render the poses once with render_an_image (which grows the capacity)
+        or call reserve() first."""
         if not all(self.flags):
             raise RuntimeError("MGRenderer: load_data, load_params, setScene, setKwargs and "
                                "input_a_c2w must all be called before rendering")
@@ -199,8 +254,20 @@ class MGRenderer:
             return None
         return self.image.cpu().numpy()
 
+    @staticmethod
+    def stage_timing(enable: bool) -> np.ndarray:
+        """Starts (True) or stops (False) the device timing of the march, MLP
+        and composite kernels; returns the totals (ms) collected until now."""
+        out = (C.c_double * 3)()
+        lib.call("pvdb_render_timing", 1 if enable else 0, out)
+        return np.array(out[:])
+
+    def reserve(self, capacity: int) -> None:
+        """Room for frames of up to `capacity` kept samples."""
+        self._reserve(capacity)
+
     def sample_counts(self) -> tuple[int, int]:
-        """(density gathers, kept samples) of the last render."""
+        """(first-march density gathers, written samples) of the last render."""
         c = self.counters.cpu()
         return int(c[0]), int(c[1])
 

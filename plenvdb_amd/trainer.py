@@ -88,6 +88,37 @@ class PlainTrainer:
         self.last = out
         return loss, out
 
+    # -- checkpoints (run.py:404-424 resume, :611-626 save at the end of a stage) --
+    def save_checkpoint(self, prefix: str, global_step: int) -> None:
+        """The reference's end-of-stage save: model.save_to and vdbopt.save_to
+        under `prefix` (density.vdb, color.vdb, the moments and vdbopt.tar), and
+        `prefix + 'last.tar'` with the step, the MLP state and its optimizer
+        state (run.py:611-626; fine_last.tar there)."""
+        self.model.save_to(prefix)
+        self.vdbopt.save_to(prefix)
+        mlp = None if self.model.rgbnet is None else self.model.rgbnet.state_dict()
+        opt = None if self.opt is None else self.opt.state_dict()
+        torch.save({"global_step": int(global_step), "rgbnet_state_dict": mlp,
+                    "optimizer_state_dict": opt}, prefix + "last.tar")
+
+    def load_checkpoint(self, prefix: str) -> int:
+        """Resume (run.py:404-424 with load_checkpoint): the grids and their
+        optimizer from `prefix`, the MLP and its optimizer from last.tar.
+        Returns the saved global step."""
+        self.model.load_from(prefix)
+        # the grids' optimizers follow the (possibly pruned) loaded topologies
+        lr_d, lr_c = self.vdbopt.densityOpt.getLr(), self.vdbopt.colorOpt.getLr()
+        self.vdbopt = VDBAdam([
+            {"params": self.model.density, "lr": lr_d, "skip_zero_grad": True},
+            {"params": self.model.k0, "lr": lr_c, "skip_zero_grad": True},
+        ])
+        self.vdbopt.load_from(prefix)
+        ck = torch.load(prefix + "last.tar", weights_only=True)
+        if self.model.rgbnet is not None:
+            self.model.rgbnet.load_state_dict(ck["rgbnet_state_dict"])
+            self.opt.load_state_dict(ck["optimizer_state_dict"])
+        return int(ck["global_step"])
+
     def step(self):
         loss, out = self.forward_backward()
         if self.opt is not None:

@@ -60,21 +60,67 @@ def active_voxel_dim(origins: np.ndarray, masks: np.ndarray) -> list[int]:
     return [int(d) for d in (xyz.max(0) - xyz.min(0) + 1)]
 
 
-def load_grid(path: str, ndim: int, device="cuda"):
+def prune_leaves(origins: np.ndarray, masks: np.ndarray, values: np.ndarray):
+    """The leaves pruneGrid keeps (plenvdb.h:135, 227).  OpenVDB's prune
+    replaces a node whose values are all equal and whose active states are all
+    equal by a tile; a leaf with no active voxel whose values are all the
+    background 0 becomes background, which is no leaf at all, so it is dropped
+    here.  A uniform leaf with active voxels (or with another value) would
+    become a tile that this layout does not hold; it stays a leaf with the same
+    values and activity.  Returns (origins, masks, values) of the kept leaves
+    in their original (NanoVDB) order."""
+    nl = len(origins)
+    if nl == 0:
+        return origins, masks, values
+    per_leaf = values.reshape(nl, 512, -1)
+    empty = ~masks.reshape(nl, 8).any(1) & ~per_leaf.reshape(nl, -1).any(1)
+    keep = ~empty
+    return origins[keep], masks[keep], per_leaf[keep].reshape(-1, values.shape[1])
+
+
+def load_grid(path: str, ndim: int, device="cuda", prune: bool = True):
     """Returns (topology, values [slots] or [slots, ndim] on the device, reso) where
     reso is the active voxels' extent (plenvdb.h:126-142 sets reso from
-    evalActiveVoxelDim)."""
+    evalActiveVoxelDim).  With prune (as the reference's load_from) the leaves
+    pruneGrid turns into background are dropped (prune_leaves)."""
     z = _read(path)
     if str(z["kind"]) != "float":
         raise ValueError(f"{path}: expected a float grid")
     vals = z["values"]
     if vals.shape[1] != ndim:
         raise ValueError(f"{path}: grid has {vals.shape[1]} channels, expected {ndim}")
+    if prune:
+        z["leaf_origins"], z["leaf_masks"], vals = prune_leaves(z["leaf_origins"], z["leaf_masks"],
+                                                               vals)
     topo = Topology.from_leaves(z["leaf_origins"], z["leaf_masks"], z["reso"])
     v = torch.from_numpy(np.ascontiguousarray(vals, np.float32)).to(device)
     if ndim == 1:
         v = v.reshape(-1)
     return topo, v.contiguous(), active_voxel_dim(z["leaf_origins"], z["leaf_masks"])
+
+
+def load_onto(path: str, topo: Topology, ndim: int, device="cuda") -> torch.Tensor:
+    """Values of a saved grid placed on another topology by leaf origin: slots
+    of leaves the file does not hold read 0.  Adam moments saved with one leaf
+    set thus land on the right voxels of a parameter whose leaf set was pruned
+    on load (the reference loads both with their own leaf counts and pairs
+    them by index, which no longer matches then)."""
+    z = _read(path)
+    if str(z["kind"]) != "float":
+        raise ValueError(f"{path}: expected a float grid")
+    vals = z["values"]
+    if vals.shape[1] != ndim:
+        raise ValueError(f"{path}: grid has {vals.shape[1]} channels, expected {ndim}")
+    src = {tuple(o): i for i, o in enumerate(np.asarray(z["leaf_origins"]).tolist())}
+    dst, _ = topo.leaves()
+    out = np.zeros((topo.nslots, ndim), np.float32)
+    per_leaf = vals.reshape(-1, 512, ndim)
+    for j, o in enumerate(np.asarray(dst).tolist()):
+        i = src.get(tuple(o))
+        if i is not None:
+            out[512 * j:512 * (j + 1)] = per_leaf[i]
+    v = torch.from_numpy(out).to(device)
+    return (v.reshape(-1) if ndim == 1 else v).contiguous()
 
 
 def index_grid_arrays(idx_grid: np.ndarray):

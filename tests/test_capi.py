@@ -60,14 +60,16 @@ def test_new_entry_points_check_arguments():
     c2w = np.eye(4, dtype=np.float32)[:3].copy()
     kp, cp = K.ctypes.data_as(C.c_void_p), c2w.ctypes.data_as(C.c_void_p)
     with pytest.raises(RuntimeError, match="mode must be"):
-        lib.call("pvdb_rays_of_view", 4, 4, kp, cp, 0, 0, 0, 0, 2, None, None, None, None)
+        lib.call("pvdb_rays_of_view", 4, 4, kp, cp, 0, 0, 0, 0, 3, None, None, None, None, None)
+    with pytest.raises(RuntimeError, match="jitter"):
+        lib.call("pvdb_rays_of_view", 4, 4, kp, cp, 0, 0, 0, 0, 2, 1, 1, 1, None, None)
     with pytest.raises(RuntimeError, match="null pointer"):
-        lib.call("pvdb_rays_of_view", 4, 4, kp, cp, 0, 0, 0, 0, 0, None, None, None, None)
+        lib.call("pvdb_rays_of_view", 4, 4, kp, cp, 0, 0, 0, 0, 0, None, None, None, None, None)
     Kz = np.zeros((3, 3))
     with pytest.raises(RuntimeError, match="focal"):
         lib.call("pvdb_rays_of_view", 4, 4, Kz.ctypes.data_as(C.c_void_p), cp, 0, 0, 0, 0, 0,
-                 1, 1, 1, None)
-    lib.call("pvdb_rays_of_view", 0, 4, kp, cp, 0, 0, 0, 0, 0, None, None, None, None)  # empty
+                 1, 1, 1, None, None)
+    lib.call("pvdb_rays_of_view", 0, 4, kp, cp, 0, 0, 0, 0, 0, None, None, None, None, None)
     with pytest.raises(RuntimeError, match="null topology"):
         lib.call("pvdb_trilinear_fwd_f64", None, None, 1, None, None, None, 4, None, None)
     with pytest.raises(RuntimeError, match="n must be"):

@@ -51,7 +51,36 @@ def test_rays_of_view_edges():
                                          False, False)
     assert ro.shape == (0, 8, 3)
     with pytest.raises(NotImplementedError):
-        rays.get_rays_of_a_view(4, 4, K, np.eye(4), False, False, False, False, mode="random")
+        rays.get_rays_of_a_view(4, 4, K, np.eye(4), False, False, False, False, mode="jitter")
+
+
+@pytest.mark.parametrize("ndc,flip_x,flip_y", [(False, False, False), (False, True, True),
+                                               (True, True, False)])
+def test_rays_of_view_random_mode(oracle_mod, ndc, flip_x, flip_y):
+    """'random' (dvgo.py:482-484): the offsets come from torch's generator in the
+    reference's call order (a seeded run draws the same ones), lie in [0, 1),
+    and the rays equal the oracle's for those offsets bit for bit."""
+    from plenvdb_amd import rays, scenes
+
+    print("[case] rays_of_view random", ndc, flip_x, flip_y, flush=True)
+    H, W = 29, 43
+    K = np.array([[61.0, 0, 21.7], [0, 60.5, 14.2], [0, 0, 1]])
+    c2w = scenes.orbit_poses(5)[2]
+    torch.manual_seed(123)
+    ro, rd, vd = rays.get_rays_of_a_view(H, W, K, torch.from_numpy(c2w), ndc, False, flip_x,
+                                         flip_y, mode="random")
+    torch.manual_seed(123)
+    jit = rays.random_jitter(H, W, torch.device("cuda")).cpu().numpy()
+    assert jit.shape == (2, H, W) and jit.min() >= 0 and jit.max() < 1
+    assert np.unique(jit).size > H * W  # fresh draws, not a constant offset
+    eo, ed, ev = oracle_mod.rays_of_view(H, W, K, c2w, ndc, False, flip_x, flip_y, "random",
+                                         jitter=jit)
+    np.testing.assert_array_equal(ro.cpu().numpy(), eo)
+    np.testing.assert_array_equal(rd.cpu().numpy(), ed)
+    np.testing.assert_array_equal(vd.cpu().numpy(), ev)
+    # the jitter moves the rays off the pixel centres
+    _, ec, _ = oracle_mod.rays_of_view(H, W, K, c2w, ndc, False, flip_x, flip_y, "center")
+    assert np.abs(rd.cpu().numpy() - ec).max() > 0
 
 
 def _oracle_hits(oracle_mod, sc, dom, ro, rd):
