@@ -6,12 +6,15 @@
 //   H0^T = relu(W0^T X^T + b0),  H1^T = relu(W1^T H0^T + b1),  L^T = W2^T H1^T + b2
 // with v_mfma_f32_16x16x32_bf16 (samples are the N dimension); each wavefront
 // owns a slice of the hidden features and exchanges activations with the
-// others through LDS.  Weights are held in bf16; features, activations and
-// gradients travel as (hi, lo) bf16 pairs, so each product with a weight is
+// others through LDS.  Weights are held in bf16; gradients and the last
+// hidden layer travel as (hi, lo) bf16 pairs, so each product with a weight is
 // two MFMAs, which keeps about 16 significant bits where plain bf16 keeps 8
 // (plain bf16 inputs put the colour-feature gradients several percent off an
-// FP64 evaluation; the pairs bring them under 1e-3).  dW products (activation
-// x gradient) take the hi parts only.
+// FP64 evaluation).  The forward's first two products take (hi, mid, lo)
+// triples (three MFMAs, fp32's 24 bits): their pre-activations decide the ReLU
+// masks the backward applies, and a mask bit that rounds the other way moves a
+// sample's feature gradient by a whole term.  dW products (activation x
+// gradient) take the hi parts only.
 //
 // k_mlp_fwd gathers the colour features, keeps them per record as (hi, lo)
 // bf16 (so the backward reads 48 bytes per record instead of redoing 8
@@ -75,6 +78,17 @@ __device__ __forceinline__ float bf_hi(uint32_t p) { return __uint_as_float(p & 
 __device__ __forceinline__ void split2(float a, float b, uint32_t& h, uint32_t& l) {
     h = pk_bf(a, b);
     l = pk_bf(a - bf_lo(h), b - bf_hi(h));
+}
+// x = hi + mid + lo, each bf16: about 24 significant bits, as many as fp32
+// holds (x - hi and mid's remainder are exact in fp32).  The forward pass feeds
+// its products with triples: with pairs, about one pre-activation in a million
+// rounds to the other side of zero at the benchmarked sizes, and each such
+// ReLU flip moves its sample's feature gradient by a whole term.
+__device__ __forceinline__ void split3(float a, float b, uint32_t& h, uint32_t& m, uint32_t& l) {
+    h = pk_bf(a, b);
+    const float ra = a - bf_lo(h), rb = b - bf_hi(h);
+    m = pk_bf(ra, rb);
+    l = pk_bf(ra - bf_lo(m), rb - bf_hi(m));
 }
 __device__ __forceinline__ void split_bf(float x, uint16_t& h, uint16_t& l) {
     uint32_t hh, ll;
@@ -160,6 +174,18 @@ __device__ __forceinline__ f32x4 mfma_w(const bf16x8& a, const bf16x8& bh, const
     return mfma16x16x32(a, bl, c);
 }
 
+// Four consecutive features fr..fr+3 of sample `col` as (hi, mid, lo) triples,
+// sample-major.
+__device__ __forceinline__ void st_act3(uint16_t* sh, uint16_t* sm_, uint16_t* sl, int col,
+                                        int fr, const float* v) {
+    uint2 hi, mi, lo;
+    split3(v[0], v[1], hi.x, mi.x, lo.x);
+    split3(v[2], v[3], hi.y, mi.y, lo.y);
+    *reinterpret_cast<uint2*>(&sh[col * SM + fr]) = hi;
+    *reinterpret_cast<uint2*>(&sm_[col * SM + fr]) = mi;
+    *reinterpret_cast<uint2*>(&sl[col * SM + fr]) = lo;
+}
+
 // dW products (activation x gradient) use the hi parts only, and the backward
 // recomputes the activations they take from hi parts: the sums over a step's
 // samples average that rounding (a device check keeps every MLP gradient
@@ -168,7 +194,10 @@ __device__ __forceinline__ f32x4 mfma_w(const bf16x8& a, const bf16x8& bh, const
 // ---- forward -----------------------------------------------------------------
 // A workgroup of 4 wavefronts takes a group of 32 records; wave w owns hidden
 // features [32w, 32w + 32) of both hidden layers (up to three workgroups per
-// CU).  Per group, four phases with a barrier after each:
+// CU).  X and H0 enter their products as (hi, mid, lo) triples, so the
+// pre-activations, and with them the ReLU masks the backward applies, carry
+// fp32 precision; H1 enters the logits as (hi, lo) pairs.  Per group, four
+// phases with a barrier after each:
 //   A  X = [colour features | PE | 1] from the gather and view directions
 //      loaded during the previous group; the features are kept per record
 //   B  H0, then the next group's topology lookups and view-direction loads
@@ -184,9 +213,17 @@ struct RecInfo {
 
 struct alignas(16) SmemFwd {
     uint16_t W2t[16 * 128];  // W2^T [16 logits (3 used)][128 hidden]
-    uint16_t Xh[GS * XM], Xl[GS * XM];
-    uint16_t A1h[GS * SM], A1l[GS * SM];  // H0
-    uint16_t A2h[GS * SM], A2l[GS * SM];  // H1
+    // X is read in phase B only and H1 in phase D only (a barrier between D
+    // and the next group's A), so they share storage
+    union {
+        struct {
+            uint16_t Xh[GS * XM], Xm[GS * XM], Xl[GS * XM];
+        } x;
+        struct {
+            uint16_t A2h[GS * SM], A2l[GS * SM];  // H1
+        } h1;
+    } u;
+    uint16_t A1h[GS * SM], A1m[GS * SM], A1l[GS * SM];  // H0
     RecInfo ri[2][GS];
 };
 
@@ -226,12 +263,6 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
     b2v[2] = W.b2[2];
     for (int e = tid; e < 16 * 128 / 8; e += 256)
         reinterpret_cast<uint4*>(sm.W2t)[e] = reinterpret_cast<const uint4*>(W.w2t)[e];
-    // X columns 40..63 are zero throughout
-    for (int e = tid; e < GS * 24; e += 256) {
-        const int row = e / 24, col = 40 + e % 24;
-        sm.Xh[row * XM + col] = 0;
-        sm.Xl[row * XM + col] = 0;
-    }
 
     const int64_t total = min((int64_t)R.total[0], P.capacity);
     const int64_t ngroups = (total + GS - 1) / GS;
@@ -344,17 +375,20 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
                 v[c] += __shfl_xor(v[c], 4, 64);
             }
             if (gpart == 0) {
-                uint32_t wh[6], wl[6];
+                uint32_t wh[6], wl[6], wt[6];
 #pragma unroll
-                for (int c = 0; c < 6; ++c) split2(v[2 * c], v[2 * c + 1], wh[c], wl[c]);
-                uint32_t* xh = reinterpret_cast<uint32_t*>(&sm.Xh[gs * XM]);
-                uint32_t* xl = reinterpret_cast<uint32_t*>(&sm.Xl[gs * XM]);
+                for (int c = 0; c < 6; ++c) split3(v[2 * c], v[2 * c + 1], wh[c], wl[c], wt[c]);
+                uint32_t* xh = reinterpret_cast<uint32_t*>(&sm.u.x.Xh[gs * XM]);
+                uint32_t* xm = reinterpret_cast<uint32_t*>(&sm.u.x.Xm[gs * XM]);
+                uint32_t* xl = reinterpret_cast<uint32_t*>(&sm.u.x.Xl[gs * XM]);
 #pragma unroll
                 for (int c = 0; c < 6; ++c) {
                     xh[c] = wh[c];
-                    xl[c] = wl[c];
+                    xm[c] = wl[c];
+                    xl[c] = wt[c];
                 }
-                // the features kept for the backward: hi halves 0..11, then lo
+                // the features kept for the backward: hi halves 0..11, then the
+                // second parts (the (hi, lo) pair of the feature)
                 const int64_t qq = base + gs;
                 if (qq < total && (sm.ri[cb][gs].bits & kAliveW)) {
                     rec.feat[3 * qq] = make_uint4(wh[0], wh[1], wh[2], wh[3]);
@@ -387,12 +421,23 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
                 }
                 val[k] = v;
             }
-            uint2 hi, lo;
-            split2(val[0], val[1], hi.x, lo.x);
-            split2(val[2], val[3], hi.y, lo.y);
+            uint2 hi, mi, lo;
+            split3(val[0], val[1], hi.x, mi.x, lo.x);
+            split3(val[2], val[3], hi.y, mi.y, lo.y);
             if (pp < 7) {
-                *reinterpret_cast<uint2*>(&sm.Xh[ps * XM + 12 + 4 * pp]) = hi;
-                *reinterpret_cast<uint2*>(&sm.Xl[ps * XM + 12 + 4 * pp]) = lo;
+                *reinterpret_cast<uint2*>(&sm.u.x.Xh[ps * XM + 12 + 4 * pp]) = hi;
+                *reinterpret_cast<uint2*>(&sm.u.x.Xm[ps * XM + 12 + 4 * pp]) = mi;
+                *reinterpret_cast<uint2*>(&sm.u.x.Xl[ps * XM + 12 + 4 * pp]) = lo;
+            } else {
+                // X columns 40..63 (zero weights) are zeroed for every group: H1
+                // of the previous group shares their storage
+                const uint4 zz = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    *reinterpret_cast<uint4*>(&sm.u.x.Xh[ps * XM + 40 + 8 * k]) = zz;
+                    *reinterpret_cast<uint4*>(&sm.u.x.Xm[ps * XM + 40 + 8 * k]) = zz;
+                    *reinterpret_cast<uint4*>(&sm.u.x.Xl[ps * XM + 40 + 8 * k]) = zz;
+                }
             }
         }
         commit(cb ^ 1);                 // the next group's records
@@ -407,18 +452,17 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
             for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < 2; ++kt) {
-            bf16x8 bh[NT], bl[NT];
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
                 const int xo = (16 * nt + r16) * XM + 32 * kt + 8 * g;
-                bh[nt] = ld_frag(&sm.Xh[xo]);
-                bl[nt] = ld_frag(&sm.Xl[xo]);
+                const bf16x8 bp[3] = {ld_frag(&sm.u.x.Xh[xo]), ld_frag(&sm.u.x.Xm[xo]),
+                                      ld_frag(&sm.u.x.Xl[xo])};
+#pragma unroll
+                for (int part = 0; part < 3; ++part)
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+                        acc[mt][nt] = mfma16x16x32(A0[mt][kt], bp[part], acc[mt][nt]);
             }
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
-                    acc[mt][nt] = mfma_w(A0[mt][kt], bh[nt], bl[nt], acc[mt][nt]);
         }
         uint32_t nib[MT][NT];
 #pragma unroll
@@ -432,7 +476,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
                     h[i] = fmaxf(acc[mt][nt][i] + b0v[mt][i], 0.f);
                     nib[mt][nt] |= (h[i] > 0.f ? 1u : 0u) << i;
                 }
-                st_act<false>(sm.A1h, sm.A1l, nullptr, 16 * nt + r16, f0 + 16 * mt + 4 * g, lane, h);
+                st_act3(sm.A1h, sm.A1m, sm.A1l, 16 * nt + r16, f0 + 16 * mt + 4 * g, h);
             }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
@@ -449,18 +493,17 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
             for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) {
-            bf16x8 bh[NT], bl[NT];
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
                 const int ho = (16 * nt + r16) * SM + 32 * kt + 8 * g;
-                bh[nt] = ld_frag(&sm.A1h[ho]);
-                bl[nt] = ld_frag(&sm.A1l[ho]);
+                const bf16x8 bp[3] = {ld_frag(&sm.A1h[ho]), ld_frag(&sm.A1m[ho]),
+                                      ld_frag(&sm.A1l[ho])};
+#pragma unroll
+                for (int part = 0; part < 3; ++part)
+#pragma unroll
+                    for (int mt = 0; mt < MT; ++mt)
+                        acc[mt][nt] = mfma16x16x32(A1[mt][kt], bp[part], acc[mt][nt]);
             }
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-                for (int nt = 0; nt < NT; ++nt)
-                    acc[mt][nt] = mfma_w(A1[mt][kt], bh[nt], bl[nt], acc[mt][nt]);
         }
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
@@ -473,7 +516,8 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
                     h[i] = fmaxf(acc[mt][nt][i] + b1v[mt][i], 0.f);
                     nib[mt][nt] |= (h[i] > 0.f ? 1u : 0u) << i;
                 }
-                st_act<false>(sm.A2h, sm.A2l, nullptr, 16 * nt + r16, f0 + 16 * mt + 4 * g, lane, h);
+                st_act<false>(sm.u.h1.A2h, sm.u.h1.A2l, nullptr, 16 * nt + r16, f0 + 16 * mt + 4 * g,
+                              lane, h);
             }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
@@ -491,7 +535,8 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
 #pragma unroll
             for (int kt = 0; kt < 4; ++kt)
                 a5 = mfma_w(ld_frag(&sm.W2t[r16 * 128 + 32 * kt + 8 * g]),
-                            ld_frag(&sm.A2h[ho + 32 * kt]), ld_frag(&sm.A2l[ho + 32 * kt]), a5);
+                            ld_frag(&sm.u.h1.A2h[ho + 32 * kt]), ld_frag(&sm.u.h1.A2l[ho + 32 * kt]),
+                            a5);
             if (g == 0) {
                 const int sidx = 16 * nt + r16;
                 const int64_t q = base + sidx;

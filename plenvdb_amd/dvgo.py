@@ -124,6 +124,17 @@ class DirectVoxGO(nn.Module):
                                                        device=self.xyz_min.device),
                                        xyz_min=self.xyz_min, xyz_max=self.xyz_max)
 
+    # -- grid files (dvgo.py:390-396): density.vdb and color.vdb under one prefix --
+    def save_to(self, savedir: str) -> None:
+        for g in (self.density, self.k0):
+            g.save_to(savedir)
+
+    def load_from(self, loaddir: str) -> None:
+        """Each grid takes the file's (pruned) leaf set; the occupancy cache
+        keeps its topology, whose active voxels pruning does not change."""
+        for g in (self.density, self.k0):
+            g.load_from(loaddir)
+
     @torch.no_grad()
     def load_mlp(self, p: dict):
         """Weights given as [in, out] (Linear.weight.T, run.py:98-104)."""

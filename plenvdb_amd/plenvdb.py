@@ -168,8 +168,8 @@ class _BaseVDB:
 
         topo, vals, reso = vdbio.load_grid(path, self.ndim)
         self.topo = topo
-        self.grid = vals
-        self.grad = torch.zeros_like(vals)
+        self.grid = vals.reshape(topo.nslots, self.ndim).contiguous()
+        self.grad = torch.zeros_like(self.grid)
         self.reso = list(reso)
 
 
@@ -285,9 +285,10 @@ class _BaseOpt:
         on load."""
         from . import vdbio
 
+        p = self.params
         for name in ("exp_avg", "exp_avg_sq"):
-            setattr(self, name, vdbio.load_onto(prefix + name + ".vdb", self.params.topo,
-                                                self.params.ndim))
+            v = vdbio.load_onto(prefix + name + ".vdb", p.topo, p.ndim)
+            setattr(self, name, v.reshape(p.topo.nslots, p.ndim).contiguous())
 
     def getinfo(self) -> None:  # plenvdb.h:739-744
         p = self.params

@@ -17,7 +17,9 @@ def _c1_grid(cls, n, seed):
     from plenvdb_amd.plenvdb import ColorVDB, DensityVDB
     from plenvdb_amd.topology import Topology
 
-    mask = scenes.occupancy_mask(scenes.CONFIGS["c1"])
+    mask = scenes.occupancy_mask(scenes.CONFIGS["c1"]).copy()
+    mask[100:] = False  # an active extent below the resolution on two axes
+    mask[:, :, :20] = False
     topo = Topology.from_mask(mask)
     g = (DensityVDB if cls == "density" else ColorVDB)([128] * 3, n, topology=topo)
     rng = np.random.default_rng(seed)
@@ -70,7 +72,7 @@ def test_prune_on_load_drops_background_leaves(tmp_path):
     # leaves 1, 3, 5: inactive, all zero -> background
     order, _ = topo.leaves()
     perm = [int(np.nonzero((o == x).all(1))[0][0]) for x in order]
-    g.grid.copy_(torch.from_numpy(vals[perm].reshape(-1)))
+    g.grid.copy_(torch.from_numpy(vals[perm]).reshape(g.grid.shape))
     path = str(tmp_path / "density.vdb")
     g.save_to(path)
     h = DensityVDB([24, 16, 16], 1)
@@ -104,11 +106,11 @@ def test_optimizer_moments_follow_a_pruned_parameter(tmp_path):
     bg = int(np.nonzero((order == [0, 0, 8]).all(1))[0][0])
     v = rng.normal(size=(3, 512)).astype(np.float32)
     v[bg] = 0
-    g.grid.copy_(torch.from_numpy(v.reshape(-1)))
+    g.grid.copy_(torch.from_numpy(v).reshape(g.grid.shape))
     opt = DensityOpt(g, 0.1, 1e-8, 0.9, 0.99)
     mom = rng.normal(size=(3, 512)).astype(np.float32) ** 2
-    opt.exp_avg.copy_(torch.from_numpy(mom.reshape(-1)))
-    opt.exp_avg_sq.copy_(torch.from_numpy(2 * mom.reshape(-1)))
+    opt.exp_avg.copy_(torch.from_numpy(mom).reshape(opt.exp_avg.shape))
+    opt.exp_avg_sq.copy_(torch.from_numpy(2 * mom).reshape(opt.exp_avg_sq.shape))
     g.save_to(str(tmp_path / "density.vdb"))
     opt.save_to(str(tmp_path / "density_"))
     h = DensityVDB([16, 8, 16], 1)
@@ -121,7 +123,7 @@ def test_optimizer_moments_follow_a_pruned_parameter(tmp_path):
     for j, x in enumerate(ho.tolist()):
         i = int(np.nonzero((order == x).all(1))[0][0])
         np.testing.assert_array_equal(got[j], mom[i])
-    np.testing.assert_array_equal(hopt.exp_avg_sq.cpu().numpy(), 2 * got.reshape(-1))
+    np.testing.assert_array_equal(hopt.exp_avg_sq.cpu().numpy().reshape(-1), 2 * got.reshape(-1))
 
 
 def test_vdbadam_save_load(tmp_path):
