@@ -127,22 +127,27 @@ __device__ __forceinline__ I skip_empty(const StepParams& P, const TopoView& t, 
     return i;
 }
 
-// Runs window(base, n) over the ray's steps in windows of 64, base advancing
-// past empty space (skip_empty), with 32-bit indices when the ray's step count
-// allows.
+// Runs window(base, n) over the ray's steps in windows of 64, with 32-bit
+// indices when the ray's step count allows.  window returns whether it kept a
+// sample; the march enters the ray, and leaves every window that kept none,
+// through skip_empty, so windows inside occupied space pay no skip lookups.
+template <typename I, typename F>
+__device__ __forceinline__ void windows_of(const StepParams& P, const TopoView& t,
+                                           const RaySetup& s, I n, F&& window) {
+    I base = skip_empty(P, t, s, I(0), n);
+    while (base < n) {
+        const bool kept = window(base, n);
+        base += 64;
+        if (!kept) base = skip_empty(P, t, s, base, n);
+    }
+}
 template <typename F>
 __device__ __forceinline__ void for_windows(const StepParams& P, const TopoView& t,
                                             const RaySetup& s, F&& window) {
-    if (s.n_steps < (int64_t(1) << 24)) {
-        const int32_t n = (int32_t)s.n_steps;
-        for (int32_t base = skip_empty(P, t, s, 0, n); base < n;
-             base = skip_empty(P, t, s, base + 64, n))
-            window(base, n);
-    } else {
-        for (int64_t base = skip_empty(P, t, s, int64_t(0), s.n_steps); base < s.n_steps;
-             base = skip_empty(P, t, s, base + 64, s.n_steps))
-            window(base, s.n_steps);
-    }
+    if (s.n_steps < (int64_t(1) << 24))
+        windows_of(P, t, s, (int32_t)s.n_steps, window);
+    else
+        windows_of(P, t, s, s.n_steps, window);
 }
 
 __device__ __forceinline__ int mbcnt(uint64_t m) {
@@ -166,7 +171,9 @@ __global__ void __launch_bounds__(256) k_ray_count(StepParams P, TopoView t, Ray
     int count = 0;
     for_windows(P, t, s, [&](auto base, auto n) {
         const StepPoint sp = step_point(P, t, s, base + lane, n);
-        count += __popcll(__ballot(sp.keep));
+        const uint64_t km = __ballot(sp.keep);
+        count += __popcll(km);
+        return km != 0;
     });
     if (lane == 0) R.count[r] = count;
 }
@@ -251,7 +258,7 @@ __global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, con
     bool stopped = false;
     int written = 0, queued = 0;
     // sample work on queue[0 .. k-1] (k <= 64), in step order
-    auto flush = [&](int k) {
+    auto flush = [&](int k) __attribute__((always_inline)) {
         const bool mine = lane < k;
         float x = 0.f, y = 0.f, z = 0.f, e = 0.f, alpha = 0.f;
         const bool live = mine && !stopped;
@@ -309,7 +316,7 @@ __global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, con
     for_windows(P, t, s, [&](auto base, auto n) {
         const StepPoint sp = step_point(P, t, s, base + lane, n);
         const uint64_t km = __ballot(sp.keep);
-        if (km == 0) return;
+        if (km == 0) return false;
         if (sp.keep) queue[queued + mbcnt(km)] = (int64_t)(base + lane);
         queued += __popcll(km);
         if (queued >= 64) {
@@ -323,6 +330,7 @@ __global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, con
             if (lane + 64 < queued) queue[lane] = rest;
             queued -= 64;
         }
+        return true;
     });
     if (queued > 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
