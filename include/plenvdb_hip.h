@@ -360,6 +360,19 @@ int pvdb_render_build_payload(const pvdb_topology* topo, const int32_t* slot_idx
 int64_t pvdb_render_workspace_bytes(int32_t H, int32_t W, int64_t capacity);
 /* MGRenderer::render_an_image (plenvdb.h:1029-1035): writes out [H*W*3]. */
 int pvdb_render_image(const pvdb_topology* topo, const pvdb_render_desc* desc, void* stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): pvdb_render_image with the frame workspace taken from a fixed
+ * 64 MiB device array inside the library instead of desc->workspace (ignored,
+ * with workspace_bytes).  Allocates nothing.  The frame must fit:
+ * pvdb_render_workspace_bytes(H, W, capacity) <= pvdb_render_noalloc_scratch_bytes(),
+ * else PVDB_EINVAL before any device work.  Every call on one device shares that
+ * device's array (held until the process ends), so calls that overlap on one
+ * device race; pvdb_render_frame_info does not apply to these frames. */
+int pvdb_render_image_noalloc(const pvdb_topology* topo, const pvdb_render_desc* desc,
+                              void* stream);
+/* Bytes of the scratch pvdb_render_image_noalloc renders in (64 MiB). */
+int64_t pvdb_render_noalloc_scratch_bytes(void);
 /* The last frame's record total (synchronous copy to the host; may exceed the
  * capacity) and device pointers of its per-pixel first-march counts, written
  * samples and first record (the reference keeps the counts in n_samples,

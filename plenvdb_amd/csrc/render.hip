@@ -44,6 +44,7 @@
 // weight column and a record's 16-byte halves and the pixel's PE row land on
 // 16-byte boundaries.
 #include <cmath>
+#include <mutex>
 #include <utility>
 #include <vector>
 
@@ -816,21 +817,14 @@ int pvdb_render_frame_info(const pvdb_render_desc* d, uint64_t* total, int32_t**
     return PVDB_OK;
 }
 
-int pvdb_render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void* stream) {
-    if (!topo) {
-        pvdb::set_error("null topology");
-        return PVDB_EHANDLE;
-    }
-    PVDB_CHECK_ARG(d && d->payload && d->weights && d->out && d->workspace, "null pointer");
-    PVDB_CHECK_ARG(d->H > 0 && d->W > 0 && (int64_t)d->H * d->W < (int64_t(1) << 31),
-                   "bad image size");
-    PVDB_CHECK_ARG(d->stepdist > 0.f && std::isfinite(d->stepdist), "stepdist must be positive");
-    PVDB_CHECK_ARG(d->capacity >= 0 && d->capacity < kMaxRecords,
-                   "record capacity must be in [0, 2^31 - 2]");
-    PVDB_CHECK_ARG(reinterpret_cast<uintptr_t>(d->workspace) % 256 == 0,
-                   "workspace must be 256-byte aligned");
-    PVDB_CHECK_ARG(d->workspace_bytes >= pvdb_render_workspace_bytes(d->H, d->W, d->capacity),
-                   "workspace too small");
+}  // extern "C"
+
+namespace {
+
+// The frame on `ws` (a 256-byte aligned workspace of at least
+// pvdb_render_workspace_bytes(H, W, capacity) bytes; the callers check it).
+int render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void* ws,
+                 hipStream_t stream) {
     for (int a = 0; a < 3; ++a) {
         PVDB_CHECK_ARG(d->reso[a] >= 2, "reso must be at least 2");
         PVDB_CHECK_ARG(d->xyz_max[a] > d->xyz_min[a], "empty bounding box");
@@ -844,7 +838,7 @@ int pvdb_render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void
     A.packed = reinterpret_cast<const uint4*>(d->weights);
     A.out = d->out;
     A.counters = reinterpret_cast<unsigned long long*>(d->counters);
-    A.fb = frame_buffers(d->workspace, (int64_t)d->H * d->W, d->capacity);
+    A.fb = frame_buffers(ws, (int64_t)d->H * d->W, d->capacity);
     for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 4; ++c) A.c2w[4 * r + c] = d->c2w[4 * r + c];
     A.fx = d->K[0];
@@ -868,7 +862,7 @@ int pvdb_render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void
     A.thres = d->fast_color_thres;
     A.bg = d->bg;
     A.t_stop = d->t_stop;
-    hipStream_t s = (hipStream_t)stream;
+    hipStream_t s = stream;
     hipLaunchKernelGGL(k_render_reset, dim3(1), dim3(64), 0, s, A.fb.tile_counter, A.fb.total,
                        A.counters);
     PVDB_LAUNCHED("k_render_reset");
@@ -908,6 +902,83 @@ int pvdb_render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void
         PVDB_LAUNCHED("k_render_composite");
     }
     return PVDB_OK;
+}
+
+// Checks shared by both entry points (everything but the workspace).
+int check_frame(const pvdb_topology* topo, const pvdb_render_desc* d) {
+    if (!topo) {
+        pvdb::set_error("null topology");
+        return PVDB_EHANDLE;
+    }
+    PVDB_CHECK_ARG(d && d->payload && d->weights && d->out, "null pointer");
+    PVDB_CHECK_ARG(d->H > 0 && d->W > 0 && (int64_t)d->H * d->W < (int64_t(1) << 31),
+                   "bad image size");
+    PVDB_CHECK_ARG(d->stepdist > 0.f && std::isfinite(d->stepdist), "stepdist must be positive");
+    PVDB_CHECK_ARG(d->capacity >= 0 && d->capacity < kMaxRecords,
+                   "record capacity must be in [0, 2^31 - 2]");
+    return PVDB_OK;
+}
+
+}  // namespace
+
+// Frame workspace of the no-allocation entry point.  The array's name is only
+// a host-side handle: host code uses the device address hipGetSymbolAddress
+// gives for it, kept per device (each device holds its own copy).
+constexpr int64_t kScratchBytes = int64_t(64) << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_of_current_device(void** out) {
+    int dev = 0;
+    PVDB_HIP(hipGetDevice(&dev));
+    PVDB_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        PVDB_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        PVDB_CHECK_ARG(p && reinterpret_cast<uintptr_t>(p) % 256 == 0,
+                       "scratch address not 256-byte aligned");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return PVDB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pvdb_render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void* stream) {
+    const int rc = check_frame(topo, d);
+    if (rc != PVDB_OK) return rc;
+    PVDB_CHECK_ARG(d->workspace, "null workspace");
+    PVDB_CHECK_ARG(reinterpret_cast<uintptr_t>(d->workspace) % 256 == 0,
+                   "workspace must be 256-byte aligned");
+    PVDB_CHECK_ARG(d->workspace_bytes >= pvdb_render_workspace_bytes(d->H, d->W, d->capacity),
+                   "workspace too small");
+    return render_image(topo, d, d->workspace, (hipStream_t)stream);
+}
+
+int64_t pvdb_render_noalloc_scratch_bytes(void) { return kScratchBytes; }
+
+int pvdb_render_image_noalloc(const pvdb_topology* topo, const pvdb_render_desc* d,
+                              void* stream) {
+    const int rc = check_frame(topo, d);
+    if (rc != PVDB_OK) return rc;
+    const int64_t need = pvdb_render_workspace_bytes(d->H, d->W, d->capacity);
+    PVDB_CHECK_ARG(need > 0 && need <= kScratchBytes,
+                   "frame does not fit the 64 MiB scratch of pvdb_render_image_noalloc "
+                   "(pvdb_render_workspace_bytes(H, W, capacity) must be at most "
+                   "pvdb_render_noalloc_scratch_bytes())");
+    void* ws = nullptr;
+    const int rs = scratch_of_current_device(&ws);
+    if (rs != PVDB_OK) return rs;
+    return render_image(topo, d, ws, (hipStream_t)stream);
 }
 
 int pvdb_render_timing(int32_t enable, double* total_ms3) {

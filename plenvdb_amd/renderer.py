@@ -50,6 +50,7 @@ lib.EXTRA_SIGS.update({
     "pvdb_render_build_payload": [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
                                   C.c_int32, C.c_void_p, C.c_void_p],
     "pvdb_render_image": [C.c_void_p, C.POINTER(RenderDesc), C.c_void_p],
+    "pvdb_render_image_noalloc": [C.c_void_p, C.POINTER(RenderDesc), C.c_void_p],
     "pvdb_render_frame_info": [C.POINTER(RenderDesc), C.c_void_p, C.c_void_p, C.c_void_p,
                                C.c_void_p],
     "pvdb_render_timing": [C.c_int32, C.c_void_p],
