@@ -83,62 +83,67 @@ __device__ __forceinline__ StepPoint step_point(const StepParams& P, const TopoV
     return sp;
 }
 
-// Empty-space skip: the first step at or after i whose occupancy voxel can lie
-// in an existing leaf.  While step i's voxel round(p*ms + mb) falls in a
-// 128^3 block without a lower node, or in an 8^3 block without a leaf, every
-// step until the ray leaves that block's rounding box [lo - 0.5, hi + 0.5)
-// is rejected by the occupancy test, so the march jumps to two steps before
-// the exit computed from p_i = s + u*(stepdist*i) (positions come from i, not
-// from a running sum, so a jump changes no sample).  The two steps (a voxel
-// at the configurations' step of half a voxel) cover the rounding of the
-// exit.  Every lane computes the same value.
+// Empty-space skip in two levels.  A coarse pass gives each lane a segment of
+// kSeg consecutive steps (a wave covers 64 * kSeg steps) and looks up the
+// topology's empty-run radius r of the 8^3 cell holding the segment's middle
+// step's occupancy voxel round(p*ms + mb): no leaf lies within r - 1 cells of
+// that cell, so the voxel sits at least 8 (r - 1) voxels inside an empty box
+// on every axis.  Positions are p_i = s + u*(stepdist*i) (from i, not from a
+// running sum), so the segment's steps lie within |g| (kSeg/2 + 1) voxels of
+// the middle one, g the per-step move in voxels; when that stays below
+// 8 (r - 1) - 1 on every axis, every step of the segment falls in a missing
+// leaf and is rejected by the occupancy test: the segment is certified empty.
+// The fine pass then tests the remaining segments' steps, four segments (64
+// steps) per window, in step order.  Without an empty-run map (the two-level
+// layout) no segment is certified and every step is tested.
+constexpr int kSeg = 16;
+
 template <typename I>
-__device__ __forceinline__ I skip_empty(const StepParams& P, const TopoView& t, const RaySetup& s,
-                                        I i, I n) {
-    const float su[3] = {s.ux, s.uy, s.uz}, ss[3] = {s.sx, s.sy, s.sz};
-#pragma unroll 1
-    for (int hop = 0; hop < 16 && i < n; ++hop) {
-        const float dist = P.stepdist * (float)i;
-        int v[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) v[a] = (int)roundf((ss[a] + su[a] * dist) * P.ms[a] + P.mb[a]);
-        int shift;
-        if (lower_of(t, v[0], v[1], v[2]) < 0)
-            shift = 7;
-        else if (leaf_of(t, v[0], v[1], v[2]) < 0)
-            shift = 3;
-        else
-            return i;
-        float dmin = INFINITY;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const float g = su[a] * P.ms[a];
-            if (g == 0.f) continue;
-            const float lo = (float)((v[a] >> shift) << shift);
-            const float bound = g > 0.f ? lo + (float)(1 << shift) - 0.5f : lo - 0.5f;
-            dmin = fminf(dmin, (bound - (ss[a] * P.ms[a] + P.mb[a])) / g);
-        }
-        const float xe = dmin / P.stepdist;
-        if (!(xe < 1.0e9f)) return n;  // no exit before the ray's end
-        const I ie = (I)floorf(xe) - 1;
-        if (ie <= i) return i;
-        i = ie < n ? ie : n;
-    }
-    return i;
+__device__ __forceinline__ bool segment_empty(const StepParams& P, const TopoView& t,
+                                              const RaySetup& s, I mid, float reach) {
+    const float dist = P.stepdist * (float)mid;
+    const int v0 = (int)roundf((s.sx + s.ux * dist) * P.ms[0] + P.mb[0]);
+    const int v1 = (int)roundf((s.sy + s.uy * dist) * P.ms[1] + P.mb[1]);
+    const int v2 = (int)roundf((s.sz + s.uz * dist) * P.ms[2] + P.mb[2]);
+    const int a = (v0 >> 3) - t.lo0, b = (v1 >> 3) - t.lo1, c = (v2 >> 3) - t.lo2;
+    const int oa = a < 0 ? -a : (a >= t.d0 ? a - t.d0 + 1 : 0);
+    const int ob = b < 0 ? -b : (b >= t.d1 ? b - t.d1 + 1 : 0);
+    const int oc = c < 0 ? -c : (c >= t.d2 ? c - t.d2 + 1 : 0);
+    int r = max(oa, max(ob, oc));
+    if (r == 0) r = (int)t.erun[((int64_t)a * t.d1 + b) * t.d2 + c];  // 0: a leaf exists
+    return reach < 8.f * (float)(r - 1) - 1.f;
 }
 
-// Runs window(base, n) over the ray's steps in windows of 64, with 32-bit
-// indices when the ray's step count allows.  window returns whether it kept a
-// sample; the march enters the ray, and leaves every window that kept none,
-// through skip_empty, so windows inside occupied space pay no skip lookups.
+// Runs window(step, n) over the ray's steps, lane l taking step `step` (n for
+// none; steps increase with the lane), skipping certified-empty segments; with
+// 32-bit indices when the ray's step count allows.
 template <typename I, typename F>
 __device__ __forceinline__ void windows_of(const StepParams& P, const TopoView& t,
                                            const RaySetup& s, I n, F&& window) {
-    I base = skip_empty(P, t, s, I(0), n);
-    while (base < n) {
-        const bool kept = window(base, n);
-        base += 64;
-        if (!kept) base = skip_empty(P, t, s, base, n);
+    const int lane = lane_id();
+    if (!t.erun) {
+        for (I base = 0; base < n; base += 64) window(base + lane, n);
+        return;
+    }
+    // voxels a step moves on the worst axis, times the half segment (+1 for rounding)
+    const float g = fmaxf(fmaxf(fabsf(s.ux * P.ms[0]), fabsf(s.uy * P.ms[1])),
+                          fabsf(s.uz * P.ms[2])) * P.stepdist;
+    const float reach = g * (float)(kSeg / 2 + 1);
+    for (I base = 0; base < n; base += I(64 * kSeg)) {
+        const I seg0 = base + (I)(kSeg * lane);
+        const bool live = seg0 < n && !segment_empty(P, t, s, seg0 + kSeg / 2, reach);
+        uint64_t m = __ballot(live);
+        while (m) {  // uniform: four segments per window
+            int pick[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                pick[q] = m ? __ffsll((unsigned long long)m) - 1 : -1;
+                m &= m - 1;
+            }
+            const int sel = pick[lane >> 4];
+            const I step = sel >= 0 ? base + (I)(kSeg * sel + (lane & 15)) : n;
+            window(step < n ? step : n, n);
+        }
     }
 }
 template <typename F>
@@ -169,11 +174,9 @@ __global__ void __launch_bounds__(256) k_ray_count(StepParams P, TopoView t, Ray
     const int lane = lane_id();
     const RaySetup s = ray_setup(P, r);
     int count = 0;
-    for_windows(P, t, s, [&](auto base, auto n) {
-        const StepPoint sp = step_point(P, t, s, base + lane, n);
-        const uint64_t km = __ballot(sp.keep);
-        count += __popcll(km);
-        return km != 0;
+    for_windows(P, t, s, [&](auto step, auto n) {
+        const StepPoint sp = step_point(P, t, s, step, n);
+        count += __popcll(__ballot(sp.keep));
     });
     if (lane == 0) R.count[r] = count;
 }
@@ -313,11 +316,11 @@ __global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, con
             }
         }
     };
-    for_windows(P, t, s, [&](auto base, auto n) {
-        const StepPoint sp = step_point(P, t, s, base + lane, n);
+    for_windows(P, t, s, [&](auto step, auto n) {
+        const StepPoint sp = step_point(P, t, s, step, n);
         const uint64_t km = __ballot(sp.keep);
-        if (km == 0) return false;
-        if (sp.keep) queue[queued + mbcnt(km)] = (int64_t)(base + lane);
+        if (km == 0) return;
+        if (sp.keep) queue[queued + mbcnt(km)] = (int64_t)step;
         queued += __popcll(km);
         if (queued >= 64) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -330,7 +333,6 @@ __global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, con
             if (lane + 64 < queued) queue[lane] = rest;
             queued -= 64;
         }
-        return true;
     });
     if (queued > 0) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -20,6 +20,9 @@
 //     16^3 child table per lower node;
 //   * a dense lower-node table in both layouts (the upper node's child mask),
 //     used by the marchers to skip empty 128^3 blocks;
+//   * in the one-level layout, an empty-run map: per 8^3 cell without a leaf,
+//     the radius (in cells, up to 64) of the empty cube of cells around it,
+//     which lets the marchers step over empty space in one jump;
 //   * per-leaf 512-bit value masks (NanoVDB LeafNode::mValueMask);
 //   * per-leaf origins.
 // Leaf indices and the in-leaf offset (x&7)<<6 | (y&7)<<3 | (z&7) equal the
@@ -34,6 +37,7 @@ struct TopoView {
     const int32_t* origins;      // [nleaf*3]
     const int32_t* lower_table;  // [e0*e1*e2] lower-node (128^3 block) index or -1
     const int32_t* lower_leaves; // two-level: [nlower*4096] leaf index per 8^3 block or -1
+    const uint8_t* erun;         // one-level: [d0*d1*d2] empty-run map (see empty_run), else null
     int32_t lo0, lo1, lo2;       // one-level: leaf-grid origin (voxel >> 3)
     int32_t d0, d1, d2;          // one-level: leaf-grid extent
     int32_t ulo0, ulo1, ulo2;    // lower-node grid origin (voxel >> 7)
@@ -78,6 +82,21 @@ __device__ __forceinline__ int32_t leaf_of(const TopoView& t, int i, int j, int 
 __device__ __forceinline__ int64_t slot_of(const TopoView& t, int i, int j, int k) {
     const int32_t l = leaf_of(t, i, j, k);
     return l < 0 ? int64_t(-1) : (int64_t)l * 512 + voxel_offset(i, j, k);
+}
+
+// For the 8^3 cell L = voxel >> 3 that holds no leaf: r >= 1 such that no
+// leaf lies within r - 1 cells of L (Chebyshev distance).  Outside the leaf
+// grid's box that is L's distance to the box; inside it the empty-run map's
+// entry; 1 without a map (two-level layout).
+__device__ __forceinline__ int empty_run(const TopoView& t, int L0, int L1, int L2) {
+    if (!t.erun) return 1;
+    const int a = L0 - t.lo0, b = L1 - t.lo1, c = L2 - t.lo2;
+    const int oa = a < 0 ? -a : (a >= t.d0 ? a - t.d0 + 1 : 0);
+    const int ob = b < 0 ? -b : (b >= t.d1 ? b - t.d1 + 1 : 0);
+    const int oc = c < 0 ? -c : (c >= t.d2 ? c - t.d2 + 1 : 0);
+    const int out = max(oa, max(ob, oc));
+    if (out > 0) return out;
+    return max(1, (int)t.erun[((int64_t)a * t.d1 + b) * t.d2 + c]);
 }
 
 // NanoVDB isActive for a tree without active tiles: the value-mask bit of an

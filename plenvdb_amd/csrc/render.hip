@@ -1,5 +1,5 @@
 // Merged-VDB renderer: MGRenderer::render_an_image (plenvdb/lib/vdb/plenvdb.h:
-// 1029-1035 -> renderer.cu:370-424) in three kernels and no host round trip.
+// 1029-1035 -> renderer.cu:370-424) in four kernels and no host round trip.
 //
 // The reference renders in two full marches per pixel (first_look_merged
 // counts the kept samples and rewrites t_min / t_max, renderer.cu:222-268;
@@ -8,18 +8,18 @@
 // materialised sample list (cuda_rgbnet, :83-109) and an atomic composite
 // (final_render, :112-119).  Here:
 //
-//   k_render_march  a wavefront takes an 8x8 pixel tile (one pixel per lane,
-//                   tiles handed out by a device counter) and runs both of the
-//                   reference's marches with their own arithmetic: pass 1 with
-//                   trigetDensity's summation order counts the kept samples,
-//                   records the first kept t minus one step and the early-stop
-//                   t; the wave then reserves its tile's records with one
-//                   atomic and scans the counts; pass 2 restarts from the
-//                   rewritten t_min with trigetDensity2's order and writes one
-//                   32-byte record per kept sample (12 colour features in
-//                   fp16, the weight, the pixel), at most pass 1's count of
+//   k_render_count  a wavefront per 8x8 pixel tile, a lane per pixel: the
+//                   first march with trigetDensity's summation order; stores
+//                   the kept count and the rewritten (t_min, t_max) per pixel.
+//                   Light on registers, so it runs at full occupancy.
+//   k_render_fill   the second march from the rewritten t_min with
+//                   trigetDensity2's order: the tile reserves its records with
+//                   one atomic (counts scanned across the wave) and writes one
+//                   32-byte record per kept sample (12 colour features in fp16,
+//                   the weight, the pixel), at most the first march's count of
 //                   them, contiguous per pixel.  So the sample set is the
-//                   reference's, including the pass-2 restart (renderer.cu:263).
+//                   reference's, including the second march's restart
+//                   (renderer.cu:263).
 //   k_render_mlp    persistent waves over 64-record chunks: the RGB MLP on
 //                   v_mfma_f32_16x16x32_f16 (weights in LDS, shared by the
 //                   workgroup's 8 waves) and each record's contribution
@@ -33,7 +33,8 @@
 // and the 12 colour channels of the voxel's index (index 0's values for
 // inactive voxels; slot `nslots` holds index 0's values for corners whose leaf
 // does not exist).  A slot is 32 bytes (fp16: exact for the fp16 data of
-// mergeddata.npz) or 64 bytes (fp32).
+// mergeddata.npz) or 64 bytes (fp32); the densities follow once more as a
+// compact array, which the marches' corner loads read.
 //
 // MLP.  Layer outputs are computed transposed (features x samples) and stay in
 // registers: the D fragment of one layer (lane (g,n): features 16m+4g+i of
@@ -118,13 +119,14 @@ __global__ void k_render_pack(const float* __restrict__ w0, const float* __restr
 template <typename PT>
 __global__ void k_render_payload(const int32_t* __restrict__ slot_idx, int64_t nslots,
                                  const float* __restrict__ den, const float* __restrict__ col,
-                                 int64_t nidx, PT* __restrict__ pay) {
+                                 int64_t nidx, PT* __restrict__ pay, PT* __restrict__ dens) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s > nslots) return;
     int64_t id = s < nslots ? slot_idx[s] : 0;  // slot nslots: index 0 (missing leaves)
     if (id < 0 || id >= nidx) id = 0;           // the host checks the range
     PT* p = pay + s * 16;
     p[0] = (PT)den[id];
+    dens[s] = (PT)den[id];
 #pragma unroll
     for (int c = 0; c < 12; ++c) p[1 + c] = (PT)col[id * 12 + c];
     p[13] = p[14] = p[15] = (PT)0.f;
@@ -132,10 +134,10 @@ __global__ void k_render_payload(const int32_t* __restrict__ slot_idx, int64_t n
 
 // Workspace of one frame (pvdb_render_workspace_bytes), 256-byte aligned parts.
 struct FrameBuffers {
-    uint32_t* tile_counter;          // [1]
     unsigned long long* total;       // [1] records reserved (may exceed the capacity)
     int32_t* pix_start;              // [HW] first record of the pixel (saturated)
     int32_t* pix_count;              // [HW] pass-1 kept count (records reserved)
+    float2* tspan;                   // [HW] pass 2's t range (rewritten t_min, t_max)
     int32_t* pix_written;            // [HW] records pass 2 filled (<= count)
     float* t_last;                   // [HW] transmittance after pass 2
     uint4* pe;                       // [HW][4] PE row: 27 halves, 1, 0 x4
@@ -147,6 +149,7 @@ struct FrameBuffers {
 struct RenderArgs {
     TopoView topo;
     const void* payload;
+    const void* den;  // [nslots + 1] densities alone (the first element of each slot)
     int64_t nslots;
     const uint4* packed;
     float* out;
@@ -155,7 +158,7 @@ struct RenderArgs {
     FrameBuffers fb;
     float c2w[12];
     float fx, cx, fy, cy;
-    float mn[3], ext[3], ws[3];
+    float mn[3], ext[3], ws[3], inv_ws[3];
     int32_t H, W, inverse_y, ntx, ntiles;
     float near_, stepdist, act_shift, interval, thres, bg, t_stop;
 };
@@ -170,10 +173,12 @@ __device__ __forceinline__ float h2f(uint16_t h) {
     return (float)__builtin_bit_cast(_Float16, h);
 }
 
+// Density of a slot from the compact density array (2 or 4 bytes a slot, so
+// the marches' corner loads share cache lines; 2 MB at 1M slots in fp16).
 template <bool F32PAY>
-__device__ __forceinline__ float load_den(const void* pay, int64_t slot) {
-    if (F32PAY) return reinterpret_cast<const float*>(pay)[slot * 16];
-    return h2f(reinterpret_cast<const uint16_t*>(pay)[slot * 16]);
+__device__ __forceinline__ float load_den(const void* den, int64_t slot) {
+    if (F32PAY) return reinterpret_cast<const float*>(den)[slot];
+    return h2f(reinterpret_cast<const uint16_t*>(den)[slot]);
 }
 
 template <bool F32PAY>
@@ -299,6 +304,7 @@ struct BlockCache {
 // renderer.cu:122-167; get_tminmax :50-64).
 struct PixelRay {
     float o[3], d[3], step, tmin, tmax;
+    float inv_d[3];  // 1 / d (INFINITY for d = 0), for the empty-space exits only
 };
 
 __device__ __forceinline__ void pixel_dir(const RenderArgs& A, int px, int py, float* rd) {
@@ -353,6 +359,7 @@ __device__ __forceinline__ PixelRay pixel_ray(const RenderArgs& A, int px, int p
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
         R.d[a] = rd[a] / A.ext[a];
+        R.inv_d[a] = 1.f / R.d[a];
         const float v = R.d[a] == 0.f ? 1e-6f : R.d[a];
         const float ta = (1.f - R.o[a]) / v, tb = (-R.o[a]) / v;
         lo = a == 0 ? fminf(ta, tb) : fmaxf(lo, fminf(ta, tb));
@@ -369,6 +376,52 @@ __device__ __forceinline__ PixelRay pixel_ray(const RenderArgs& A, int px, int p
 // rejected step.  Missing leaves are stepped over with the t += step
 // accumulation alone (see the comment inside), so t stays the reference's
 // running sum and the set of tested steps is unchanged.
+// Takes steps t <- t + s (float, round to nearest) while the next t stays
+// below tsafe, t < tend and iter < max_iter -- the loop
+//     while (t + s < tsafe && t < tend && iter < max_iter) { t += s; ++iter; }
+// with the same result, but runs of steps inside one binade [2^(e-1), 2^e)
+// of t are taken in O(1): there t is an integer multiple T of the binade's
+// ulp u, and t + s rounds to (T + D) u with D = s / u rounded to nearest,
+// the same D for every step, unless s / u lies exactly half way between two
+// integers (round-half-even would then depend on T's parity) or t + s
+// reaches the binade's top.  So k steps give (T + k D) u exactly, with k
+// bounded below the binade's top and below the stopping conditions (counted
+// in double, one step of margin); the loop finishes the rest.
+__device__ __forceinline__ void advance_steps(float& t, float s, float tsafe, float tend,
+                                              int& iter, int max_iter) {
+#pragma unroll 1
+    for (int pass = 0; pass < 4; ++pass) {
+        if (!(t >= 1e-20f) || !(s > 0.f) || iter >= max_iter) break;
+        int e;
+        (void)frexpf(t, &e);                      // t in [2^(e-1), 2^e)
+        const float u = ldexpf(1.f, e - 24);      // ulp of the binade
+        const float q = s / u;                    // exact: a power-of-two scale
+        if (!(q >= 1.f) || !(q < 8388608.f) || q - floorf(q) == 0.5f) break;
+        const double D = (double)rintf(q);
+        const double T = (double)(t / u);         // integer in [2^23, 2^24)
+        // steps j -> j + 1 inside the binade while T + (j + 1) D <= 2^24 - 1
+        const double kb = floor((16777215.0 - T) / D);
+        // the loop's conditions for step j: T + (j + 1) D < tsafe / u, T + j D < tend / u
+        const double ks = ceil(((double)tsafe / (double)u - T) / D) - 1.0;
+        const double ke = ceil(((double)tend / (double)u - T) / D);
+        const double k = fmin(fmin(kb, ks), fmin(ke, (double)(max_iter - iter))) - 1.0;
+        if (!(k >= 4.0)) break;
+        const int ki = (int)k;
+        t = ldexpf((float)(T + (double)ki * D), e - 24);
+        iter += ki;
+        if (kb - 1.0 > k) break;  // stopped by tsafe, tend or the cap: the loop ends it
+        // at the binade's top: plain steps into the next binade, then again
+        for (int z = 0; z < 4 && t + s < tsafe && t < tend && iter < max_iter; ++z) {
+            t += s;
+            ++iter;
+        }
+    }
+    while (t + s < tsafe && t < tend && iter < max_iter) {
+        t += s;
+        ++iter;
+    }
+}
+
 struct Corners {
     int32_t slot[8];  // the host keeps nslots + 1 below 2^31
     float u, v, w;
@@ -387,24 +440,25 @@ __device__ __forceinline__ bool march_step(const RenderArgs& A, const PixelRay& 
     const int vi = (int)rintf(x), vj = (int)rintf(y), vk = (int)rintf(z);
     const int32_t l = bc.get(A.topo, vi, vj, vk);
     if (l < 0) {
-        // Empty space: the rounded position is in a missing leaf, so every step
-        // until the ray leaves that leaf's rounding box [8L - 0.5, 8L + 7.5) is
-        // rejected by this same test.  Take those steps with the accumulation
-        // alone and stop two steps short of the computed exit, which covers its
-        // rounding.
+        // Empty space: the rounded position is in a missing leaf, and by the
+        // empty-run map no leaf lies within r - 1 cells of it, so every step
+        // until the ray leaves the rounding box of that cube of cells,
+        // [8(L - r + 1) - 0.5, 8(L + r) - 0.5), is rejected by this same test.
+        // Those steps are taken with the accumulation alone (in closed form,
+        // advance_steps), stopping two steps short of the computed exit, which
+        // covers its rounding (the exit is formed with reciprocals: a relative
+        // error of a few ulps, far inside the two steps).
         const int L[3] = {vi >> 3, vj >> 3, vk >> 3};
+        const int r = empty_run(A.topo, L[0], L[1], L[2]);
         float te = INFINITY;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             if (R.d[a] == 0.f) continue;
-            const float xb = R.d[a] > 0.f ? 8.f * (float)L[a] + 7.5f : 8.f * (float)L[a] - 0.5f;
-            te = fminf(te, (xb / A.ws[a] - R.o[a]) / R.d[a]);
+            const float xb = R.d[a] > 0.f ? 8.f * (float)(L[a] + r) - 0.5f
+                                          : 8.f * (float)(L[a] - r + 1) - 0.5f;
+            te = fminf(te, (xb * A.inv_ws[a] - R.o[a]) * R.inv_d[a]);
         }
-        const float tsafe = te - 2.f * R.step;
-        while (t + R.step < tsafe && t < tend && iter < max_iter) {
-            t += R.step;
-            ++iter;
-        }
+        advance_steps(t, R.step, te - 2.f * R.step, tend, iter, max_iter);
         return false;
     }
     const int off = voxel_offset(vi, vj, vk);
@@ -413,6 +467,17 @@ __device__ __forceinline__ bool march_step(const RenderArgs& A, const PixelRay& 
     c.u = x - (float)i0;
     c.v = y - (float)j0;
     c.w = z - (float)k0;
+    if (((i0 & 7) != 7) & ((j0 & 7) != 7) & ((k0 & 7) != 7)) {
+        // the cell lies inside one 8^3 block: one lookup, slots by in-leaf offset
+        const int32_t l0 = bc.get(A.topo, i0, j0, k0);
+        const int32_t s0 = l0 * 512 + voxel_offset(i0, j0, k0);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int cx = (kCx >> q) & 1, cy = (kCy >> q) & 1, cz = (kCz >> q) & 1;
+            c.slot[q] = l0 < 0 ? (int32_t)A.nslots : s0 + ((cx << 6) | (cy << 3) | cz);
+        }
+        return true;
+    }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
         const int cx = (kCx >> q) & 1, cy = (kCy >> q) & 1, cz = (kCz >> q) & 1;
@@ -429,158 +494,182 @@ __device__ __forceinline__ int march_cap(const PixelRay& R, float t0, float t1) 
     return (span >= 0.f) ? (int)fminf(span * 1.0625f + 16.f, 16777216.f) : 0;
 }
 
-template <bool F32PAY>
-__global__ void __launch_bounds__(kMarchThreads) k_render_march(RenderArgs A) {
-    const int lane = threadIdx.x & 63;
-    const FrameBuffers& F = A.fb;
-    unsigned long long n_gather = 0, n_written = 0;
-    while (true) {
-        uint32_t tile = 0;
-        if (lane == 0) tile = atomicAdd(F.tile_counter, 1u);
-        tile = __shfl(tile, 0, 64);
-        if (tile >= (uint32_t)A.ntiles) break;  // uniform over the wave
-        const int px = ((int)tile % A.ntx) * 8 + (lane & 7), py = ((int)tile / A.ntx) * 8 + (lane >> 3);
-        const bool valid = px < A.W && py < A.H;
-        const int64_t pix = (int64_t)py * A.W + px;
-        const PixelRay R = pixel_ray(A, px, py);
-        BlockCache bc;
+// The pixel of lane `lane` of tile `tile` (8 x 8 pixels, row-major tiles).
+struct TilePixel {
+    int px, py;
+    bool valid;
+    int64_t pix;
+};
+__device__ __forceinline__ TilePixel tile_pixel(const RenderArgs& A, int tile, int lane) {
+    TilePixel P;
+    P.px = (tile % A.ntx) * 8 + (lane & 7);
+    P.py = (tile / A.ntx) * 8 + (lane >> 3);
+    P.valid = P.px < A.W && P.py < A.H;
+    P.pix = (int64_t)P.py * A.W + P.px;
+    return P;
+}
 
-        // ---- pass 1: first_look_merged (renderer.cu:222-268) ----
+__device__ __forceinline__ void wave_count_add(unsigned long long* dst, unsigned long long v,
+                                               int lane) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if (lane == 0 && v) atomicAdd(dst, v);
+}
+
+// Pass 1, first_look_merged (renderer.cu:222-268): a wave per 8 x 8 tile, a
+// lane per pixel.  Counts the kept samples with trigetDensity's summation
+// order and records the rewritten t range: the first kept t minus one step,
+// and the t of the early stop (or the box exit).
+template <bool F32PAY>
+__global__ void __launch_bounds__(kMarchThreads) k_render_count(RenderArgs A) {
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * (kMarchThreads / 64) + (threadIdx.x >> 6);
+    if (tile >= A.ntiles) return;  // uniform over the wave
+    const FrameBuffers& F = A.fb;
+    const TilePixel tp = tile_pixel(A, tile, lane);
+    unsigned long long n_gather = 0;
+    if (tp.valid) {
+        const PixelRay R = pixel_ray(A, tp.px, tp.py);
+        BlockCache bc;
         float t = R.tmin, T = 1.f, tmin2 = R.tmin, tmax2 = R.tmax;
         int cnt = 0, iter = 0;
         const int cap1 = march_cap(R, R.tmin, R.tmax);
         bool first = false;
-        if (valid) {
-            while (t < R.tmax && iter < cap1) {
-                Corners c;
-                if (!march_step(A, R, t, R.tmax, iter, cap1, bc, c)) continue;
-                // trigetDensity order (:191-220): res += den * (1-u) * (1-v) * (1-w), ...
-                float sig = 0.f;
+        while (t < R.tmax && iter < cap1) {
+            Corners c;
+            if (!march_step(A, R, t, R.tmax, iter, cap1, bc, c)) continue;
+            // trigetDensity order (:191-220): res += den * (1-u) * (1-v) * (1-w), ...
+            float sig = 0.f;
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const float a = ((kCx >> q) & 1) ? c.u : 1.f - c.u;
-                    const float b = ((kCy >> q) & 1) ? c.v : 1.f - c.v;
-                    const float e = ((kCz >> q) & 1) ? c.w : 1.f - c.w;
-                    sig = sig + ((load_den<F32PAY>(A.payload, c.slot[q]) * a) * b) * e;
-                }
-                ++n_gather;
-                const float alpha = 1.f - powf(1.f + expf(sig + A.act_shift), -A.interval);
-                if (alpha <= A.thres) continue;
-                const float wt = T * alpha;
-                T = T * (1.f - alpha);
-                if (wt <= A.thres) continue;
-                ++cnt;
-                if (!first) {
-                    tmin2 = t - R.step;
-                    first = true;
-                }
-                if (T < A.t_stop) {
-                    tmax2 = t;
-                    break;
-                }
+            for (int q = 0; q < 8; ++q) {
+                const float a = ((kCx >> q) & 1) ? c.u : 1.f - c.u;
+                const float b = ((kCy >> q) & 1) ? c.v : 1.f - c.v;
+                const float e = ((kCz >> q) & 1) ? c.w : 1.f - c.w;
+                sig = sig + ((load_den<F32PAY>(A.den, c.slot[q]) * a) * b) * e;
+            }
+            ++n_gather;
+            const float alpha = 1.f - powf(1.f + expf(sig + A.act_shift), -A.interval);
+            if (alpha <= A.thres) continue;
+            const float wt = T * alpha;
+            T = T * (1.f - alpha);
+            if (wt <= A.thres) continue;
+            ++cnt;
+            if (!first) {
+                tmin2 = t - R.step;
+                first = true;
+            }
+            if (T < A.t_stop) {
+                tmax2 = t;
+                break;
             }
         }
+        F.pix_count[tp.pix] = cnt;
+        F.tspan[tp.pix] = make_float2(tmin2, tmax2);
+    }
+    if (A.counters) wave_count_add(A.counters, n_gather, lane);
+}
 
-        // ---- the tile's records: one reservation, counts scanned across the wave ----
-        int incl = cnt;
+// Pass 2, ray_marching_merged (renderer.cu:312-367), from the rewritten t_min
+// to t_max with trigetDensity2's summation order: the tile reserves its records
+// with one atomic (pass 1's counts scanned across the wave) and writes one
+// 32-byte record per kept sample (12 colour features in fp16, the weight, the
+// pixel), at most pass 1's count of them, contiguous per pixel.
+template <bool F32PAY>
+__global__ void __launch_bounds__(kMarchThreads) k_render_fill(RenderArgs A) {
+    const int lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * (kMarchThreads / 64) + (threadIdx.x >> 6);
+    if (tile >= A.ntiles) return;  // uniform over the wave
+    const FrameBuffers& F = A.fb;
+    const TilePixel tp = tile_pixel(A, tile, lane);
+    const int cnt = tp.valid ? F.pix_count[tp.pix] : 0;
+    int incl = cnt;
 #pragma unroll
-        for (int dd = 1; dd < 64; dd <<= 1) {
-            const int o = __shfl_up(incl, dd, 64);
-            if (lane >= dd) incl += o;
-        }
-        const int tile_total = __shfl(incl, 63, 64);
-        unsigned long long base = 0;
-        if (lane == 0 && tile_total > 0) base = atomicAdd(F.total, (unsigned long long)tile_total);
-        base = __shfl(base, 0, 64);
-        const int64_t start = (int64_t)base + (incl - cnt);
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        const int o = __shfl_up(incl, dd, 64);
+        if (lane >= dd) incl += o;
+    }
+    const int tile_total = __shfl(incl, 63, 64);
+    unsigned long long base = 0;
+    if (lane == 0 && tile_total > 0) base = atomicAdd(F.total, (unsigned long long)tile_total);
+    base = __shfl(base, 0, 64);
+    const int64_t start = (int64_t)base + (incl - cnt);
 
-        // ---- pass 2: ray_marching_merged (renderer.cu:312-367) ----
-        int r = 0;
-        T = 1.f;
-        if (valid && cnt > 0) {
-            t = tmin2;
-            iter = 0;
-            const int cap2 = march_cap(R, tmin2, tmax2);
-            while (t < tmax2 && iter < cap2) {
-                Corners c;
-                if (!march_step(A, R, t, tmax2, iter, cap2, bc, c)) continue;
-                float sc[8];
+    int r = 0;
+    float T = 1.f;
+    if (cnt > 0) {
+        const PixelRay R = pixel_ray(A, tp.px, tp.py);
+        const float2 span = F.tspan[tp.pix];
+        const float tmax2 = span.y;
+        BlockCache bc;
+        float t = span.x;
+        int iter = 0;
+        const int cap2 = march_cap(R, span.x, tmax2);
+        while (t < tmax2 && iter < cap2) {
+            Corners c;
+            if (!march_step(A, R, t, tmax2, iter, cap2, bc, c)) continue;
+            float sc[8];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const float a = ((kCx >> q) & 1) ? c.u : 1.f - c.u;
-                    const float b = ((kCy >> q) & 1) ? c.v : 1.f - c.v;
-                    const float e = ((kCz >> q) & 1) ? c.w : 1.f - c.w;
-                    sc[q] = (a * b) * e;
-                }
-                // trigetDensity2 order (:271-300): sum of den * scale, left to right
-                float sig = load_den<F32PAY>(A.payload, c.slot[0]) * sc[0];
+            for (int q = 0; q < 8; ++q) {
+                const float a = ((kCx >> q) & 1) ? c.u : 1.f - c.u;
+                const float b = ((kCy >> q) & 1) ? c.v : 1.f - c.v;
+                const float e = ((kCz >> q) & 1) ? c.w : 1.f - c.w;
+                sc[q] = (a * b) * e;
+            }
+            // trigetDensity2 order (:271-300): sum of den * scale, left to right
+            float sig = load_den<F32PAY>(A.den, c.slot[0]) * sc[0];
 #pragma unroll
-                for (int q = 1; q < 8; ++q) sig = sig + load_den<F32PAY>(A.payload, c.slot[q]) * sc[q];
-                const float alpha = 1.f - powf(1.f + expf(sig + A.act_shift), -A.interval);
-                if (alpha <= A.thres) continue;
-                const float wt = T * alpha;
-                T = T * (1.f - alpha);
-                if (wt <= A.thres) continue;
-                if (r < cnt) {
-                    // trigetColor order (:303-310), fp16 features
-                    float f[12], cq[12];
-                    load_col<F32PAY>(A.payload, c.slot[0], cq);
+            for (int q = 1; q < 8; ++q) sig = sig + load_den<F32PAY>(A.den, c.slot[q]) * sc[q];
+            const float alpha = 1.f - powf(1.f + expf(sig + A.act_shift), -A.interval);
+            if (alpha <= A.thres) continue;
+            const float wt = T * alpha;
+            T = T * (1.f - alpha);
+            if (wt <= A.thres) continue;
+            if (r < cnt) {
+                // trigetColor order (:303-310), fp16 features
+                float f[12], cq[12];
+                load_col<F32PAY>(A.payload, c.slot[0], cq);
 #pragma unroll
-                    for (int ch = 0; ch < 12; ++ch) f[ch] = cq[ch] * sc[0];
+                for (int ch = 0; ch < 12; ++ch) f[ch] = cq[ch] * sc[0];
 #pragma unroll 1
-                    for (int q = 1; q < 8; ++q) {
-                        load_col<F32PAY>(A.payload, c.slot[q], cq);
+                for (int q = 1; q < 8; ++q) {
+                    load_col<F32PAY>(A.payload, c.slot[q], cq);
 #pragma unroll
-                        for (int ch = 0; ch < 12; ++ch) f[ch] = f[ch] + cq[ch] * sc[q];
-                    }
-                    const int64_t q = start + r;
-                    if (q < F.capacity) {
-                        F.rec[2 * q] = make_uint4(pack2(f[0], f[1]), pack2(f[2], f[3]),
-                                                  pack2(f[4], f[5]), pack2(f[6], f[7]));
-                        F.rec[2 * q + 1] = make_uint4(pack2(f[8], f[9]), pack2(f[10], f[11]),
-                                                      __float_as_uint(wt), (uint32_t)pix);
-                    }
+                    for (int ch = 0; ch < 12; ++ch) f[ch] = f[ch] + cq[ch] * sc[q];
                 }
-                ++r;  // pass 2 may keep more than pass 1 counted; those are not written
+                const int64_t q = start + r;
+                if (q < F.capacity) {
+                    F.rec[2 * q] = make_uint4(pack2(f[0], f[1]), pack2(f[2], f[3]),
+                                              pack2(f[4], f[5]), pack2(f[6], f[7]));
+                    F.rec[2 * q + 1] = make_uint4(pack2(f[8], f[9]), pack2(f[10], f[11]),
+                                                  __float_as_uint(wt), (uint32_t)tp.pix);
+                }
             }
-        }
-        const int written = r < cnt ? r : cnt;
-        // reserved records pass 2 did not fill: zero weight, zero features
-        for (int k = written; k < cnt; ++k) {
-            const int64_t q = start + k;
-            if (q < F.capacity) {
-                F.rec[2 * q] = make_uint4(0u, 0u, 0u, 0u);
-                F.rec[2 * q + 1] = make_uint4(0u, 0u, 0u, (uint32_t)pix);
-            }
-        }
-        if (valid) {
-            F.pix_start[pix] = (int32_t)(start < kMaxRecords ? start : kMaxRecords);
-            F.pix_count[pix] = cnt;
-            F.pix_written[pix] = written;
-            F.t_last[pix] = cnt > 0 ? T : 1.f;
-            if (cnt > 0) {
-                uint32_t pe[16];
-                pixel_pe(A, px, py, pe);
-                uint4* row = F.pe + 4 * pix;
-                row[0] = make_uint4(pe[0], pe[1], pe[2], pe[3]);
-                row[1] = make_uint4(pe[4], pe[5], pe[6], pe[7]);
-                row[2] = make_uint4(pe[8], pe[9], pe[10], pe[11]);
-                row[3] = make_uint4(pe[12], pe[13], pe[14], pe[15]);
-            }
-        }
-        n_written += (unsigned long long)written;
-    }
-    if (A.counters) {
-        // wave sums, one atomic per wave
-        for (int off = 32; off > 0; off >>= 1) {
-            n_gather += __shfl_xor(n_gather, off, 64);
-            n_written += __shfl_xor(n_written, off, 64);
-        }
-        if (lane == 0) {
-            atomicAdd(A.counters, n_gather);
-            atomicAdd(A.counters + 1, n_written);
+            ++r;  // pass 2 may keep more than pass 1 counted; those are not written
         }
     }
+    const int written = r < cnt ? r : cnt;
+    // reserved records pass 2 did not fill: zero weight, zero features
+    for (int k = written; k < cnt; ++k) {
+        const int64_t q = start + k;
+        if (q < F.capacity) {
+            F.rec[2 * q] = make_uint4(0u, 0u, 0u, 0u);
+            F.rec[2 * q + 1] = make_uint4(0u, 0u, 0u, (uint32_t)tp.pix);
+        }
+    }
+    if (tp.valid) {
+        F.pix_start[tp.pix] = (int32_t)(start < kMaxRecords ? start : kMaxRecords);
+        F.pix_written[tp.pix] = written;
+        F.t_last[tp.pix] = cnt > 0 ? T : 1.f;
+        if (cnt > 0) {
+            uint32_t pe[16];
+            pixel_pe(A, tp.px, tp.py, pe);
+            uint4* row = F.pe + 4 * tp.pix;
+            row[0] = make_uint4(pe[0], pe[1], pe[2], pe[3]);
+            row[1] = make_uint4(pe[4], pe[5], pe[6], pe[7]);
+            row[2] = make_uint4(pe[8], pe[9], pe[10], pe[11]);
+            row[3] = make_uint4(pe[12], pe[13], pe[14], pe[15]);
+        }
+    }
+    if (A.counters) wave_count_add(A.counters + 1, (unsigned long long)written, lane);
 }
 
 // The MLP over the records, 64 per wave-chunk; each record's contribution
@@ -667,12 +756,8 @@ __global__ void __launch_bounds__(256) k_render_composite(RenderArgs A) {
     A.out[3 * p + 2] = last + a2;
 }
 
-__global__ void k_render_reset(uint32_t* tile_counter, unsigned long long* total,
-                               unsigned long long* counters) {
-    if (threadIdx.x == 0) {
-        *tile_counter = 0u;
-        *total = 0ull;
-    }
+__global__ void k_render_reset(unsigned long long* total, unsigned long long* counters) {
+    if (threadIdx.x == 0) *total = 0ull;
     if (counters && threadIdx.x < 3) counters[threadIdx.x] = 0ull;
 }
 
@@ -706,8 +791,21 @@ struct RenderStage {
     }
 };
 
+// Payload of (nslots + 1) slots: the slot rows (32 or 64 bytes: density and
+// 12 colour channels), then the densities alone (2 or 4 bytes a slot).
+struct PayloadLayout {
+    size_t den, bytes;
+};
+PayloadLayout payload_layout(const TopoView& t, int32_t payload_f32) {
+    const size_t n = (size_t)t.nleaf * 512 + 1, es = payload_f32 ? 4 : 2;
+    PayloadLayout L;
+    L.den = align256(n * 16 * es);
+    L.bytes = align256(L.den + n * es);
+    return L;
+}
+
 struct FrameLayout {
-    size_t tile, total, start, count, written, tlast, pe, rec, contrib, bytes;
+    size_t total, start, count, tspan, written, tlast, pe, rec, contrib, bytes;
 };
 
 FrameLayout frame_layout(int64_t hw, int64_t capacity) {
@@ -718,10 +816,10 @@ FrameLayout frame_layout(int64_t hw, int64_t capacity) {
         o = align256(o + bytes);
         return at;
     };
-    L.tile = take(4);
     L.total = take(8);
     L.start = take(4 * (size_t)hw);
     L.count = take(4 * (size_t)hw);
+    L.tspan = take(8 * (size_t)hw);
     L.written = take(4 * (size_t)hw);
     L.tlast = take(4 * (size_t)hw);
     L.pe = take(64 * (size_t)hw);
@@ -735,10 +833,10 @@ FrameBuffers frame_buffers(void* ws, int64_t hw, int64_t capacity) {
     const FrameLayout L = frame_layout(hw, capacity);
     uint8_t* b = reinterpret_cast<uint8_t*>(ws);
     FrameBuffers F;
-    F.tile_counter = reinterpret_cast<uint32_t*>(b + L.tile);
     F.total = reinterpret_cast<unsigned long long*>(b + L.total);
     F.pix_start = reinterpret_cast<int32_t*>(b + L.start);
     F.pix_count = reinterpret_cast<int32_t*>(b + L.count);
+    F.tspan = reinterpret_cast<float2*>(b + L.tspan);
     F.pix_written = reinterpret_cast<int32_t*>(b + L.written);
     F.t_last = reinterpret_cast<float*>(b + L.tlast);
     F.pe = reinterpret_cast<uint4*>(b + L.pe);
@@ -772,7 +870,7 @@ int pvdb_render_pack_weights(const float* w0, const float* b0, const float* w1, 
 
 int64_t pvdb_render_payload_bytes(const pvdb_topology* topo, int32_t payload_f32) {
     if (!topo) return -1;
-    return (view_of(topo).nleaf * 512 + 1) * (payload_f32 ? 64 : 32);
+    return (int64_t)payload_layout(view_of(topo), payload_f32).bytes;
 }
 
 int pvdb_render_build_payload(const pvdb_topology* topo, const int32_t* slot_idx,
@@ -784,15 +882,18 @@ int pvdb_render_build_payload(const pvdb_topology* topo, const int32_t* slot_idx
     }
     PVDB_CHECK_ARG(slot_idx && den && col && payload, "null pointer");
     PVDB_CHECK_ARG(nidx >= 1, "the data arrays need at least index 0");
-    const int64_t nslots = view_of(topo).nleaf * 512;
+    const TopoView tv = view_of(topo);
+    const int64_t nslots = tv.nleaf * 512;
+    const PayloadLayout L = payload_layout(tv, payload_f32);
+    uint8_t* p = reinterpret_cast<uint8_t*>(payload);
     if (payload_f32)
         hipLaunchKernelGGL(k_render_payload<float>, dim3(pvdb_blocks(nslots + 1, 256)), dim3(256),
                            0, (hipStream_t)stream, slot_idx, nslots, den, col, nidx,
-                           reinterpret_cast<float*>(payload));
+                           reinterpret_cast<float*>(p), reinterpret_cast<float*>(p + L.den));
     else
         hipLaunchKernelGGL(k_render_payload<_Float16>, dim3(pvdb_blocks(nslots + 1, 256)),
                            dim3(256), 0, (hipStream_t)stream, slot_idx, nslots, den, col, nidx,
-                           reinterpret_cast<_Float16*>(payload));
+                           reinterpret_cast<_Float16*>(p), reinterpret_cast<_Float16*>(p + L.den));
     PVDB_LAUNCHED("k_render_payload");
     return PVDB_OK;
 }
@@ -834,6 +935,8 @@ int render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void* ws,
     RenderArgs A;
     A.topo = t;
     A.payload = d->payload;
+    const PayloadLayout PL = payload_layout(t, d->payload_f32);
+    A.den = reinterpret_cast<const uint8_t*>(d->payload) + PL.den;
     A.nslots = t.nleaf * 512;
     A.packed = reinterpret_cast<const uint4*>(d->weights);
     A.out = d->out;
@@ -849,6 +952,7 @@ int render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void* ws,
         A.mn[a] = d->xyz_min[a];
         A.ext[a] = d->xyz_max[a] - d->xyz_min[a];
         A.ws[a] = (float)(d->reso[a] - 1);
+        A.inv_ws[a] = 1.f / A.ws[a];
     }
     A.H = d->H;
     A.W = d->W;
@@ -863,8 +967,7 @@ int render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void* ws,
     A.bg = d->bg;
     A.t_stop = d->t_stop;
     hipStream_t s = stream;
-    hipLaunchKernelGGL(k_render_reset, dim3(1), dim3(64), 0, s, A.fb.tile_counter, A.fb.total,
-                       A.counters);
+    hipLaunchKernelGGL(k_render_reset, dim3(1), dim3(64), 0, s, A.fb.total, A.counters);
     PVDB_LAUNCHED("k_render_reset");
     static int ncu = 0;
     if (ncu == 0) {
@@ -872,17 +975,20 @@ int render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void* ws,
         PVDB_HIP(hipGetDevice(&dev));
         PVDB_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     }
-    // marching: persistent waves, 8 four-wave workgroups per CU, at most one wave per tile
-    const int waves = kMarchThreads / 64;
-    const int want = (A.ntiles + waves - 1) / waves;
-    const int mgrid = want < 8 * ncu ? want : 8 * ncu;
+    // the two marches: a wave per 8 x 8 tile
+    const int mgrid = (A.ntiles + kMarchThreads / 64 - 1) / (kMarchThreads / 64);
     {
         RenderStage st(0, s);
-        if (d->payload_f32)
-            hipLaunchKernelGGL(k_render_march<true>, dim3(mgrid), dim3(kMarchThreads), 0, s, A);
-        else
-            hipLaunchKernelGGL(k_render_march<false>, dim3(mgrid), dim3(kMarchThreads), 0, s, A);
-        PVDB_LAUNCHED("k_render_march");
+        if (d->payload_f32) {
+            hipLaunchKernelGGL(k_render_count<true>, dim3(mgrid), dim3(kMarchThreads), 0, s, A);
+            PVDB_LAUNCHED("k_render_count");
+            hipLaunchKernelGGL(k_render_fill<true>, dim3(mgrid), dim3(kMarchThreads), 0, s, A);
+        } else {
+            hipLaunchKernelGGL(k_render_count<false>, dim3(mgrid), dim3(kMarchThreads), 0, s, A);
+            PVDB_LAUNCHED("k_render_count");
+            hipLaunchKernelGGL(k_render_fill<false>, dim3(mgrid), dim3(kMarchThreads), 0, s, A);
+        }
+        PVDB_LAUNCHED("k_render_fill");
     }
     static bool smem_set = false;
     if (!smem_set) {
@@ -904,6 +1010,16 @@ int render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void* ws,
     return PVDB_OK;
 }
 
+}  // namespace
+
+// Frame workspace of the no-allocation entry point.  The array's name is only
+// a host-side handle: host code uses the device address hipGetSymbolAddress
+// gives for it, kept per device (each device holds its own copy).
+constexpr int64_t kScratchBytes = int64_t(64) << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
 // Checks shared by both entry points (everything but the workspace).
 int check_frame(const pvdb_topology* topo, const pvdb_render_desc* d) {
     if (!topo) {
@@ -918,16 +1034,6 @@ int check_frame(const pvdb_topology* topo, const pvdb_render_desc* d) {
                    "record capacity must be in [0, 2^31 - 2]");
     return PVDB_OK;
 }
-
-}  // namespace
-
-// Frame workspace of the no-allocation entry point.  The array's name is only
-// a host-side handle: host code uses the device address hipGetSymbolAddress
-// gives for it, kept per device (each device holds its own copy).
-constexpr int64_t kScratchBytes = int64_t(64) << 20;
-static __device__ __align__(256) uint8_t scratch[kScratchBytes];
-
-namespace {
 
 constexpr int kMaxDevices = 64;
 std::mutex g_scratch_mu;

@@ -43,11 +43,13 @@ struct pvdb_topology {
     std::vector<int32_t> table;         // one-level leaf table
     std::vector<int32_t> lower_table;   // lower-node table
     std::vector<int32_t> lower_leaves;  // two-level child tables
+    std::vector<uint8_t> erun;          // one-level empty-run map
     int32_t* d_table = nullptr;
     uint64_t* d_masks = nullptr;
     int32_t* d_origins = nullptr;
     int32_t* d_lower = nullptr;
     int32_t* d_lower_leaves = nullptr;
+    uint8_t* d_erun = nullptr;
     int device = 0;
 };
 
@@ -85,6 +87,59 @@ int upload(const std::vector<int32_t>& h, int32_t** d) {
     return e == hipSuccess ? PVDB_OK : pvdb::hip_fail(e, "topology upload");
 }
 
+// Empty-run map over the one-level leaf grid, one byte per 8^3 cell: 0 where
+// a leaf exists, otherwise the smallest r >= 1 such that a leaf lies within
+// Chebyshev distance r of the cell (so every cell within r - 1 is empty), or
+// `cap` when none lies within cap - 1.  Built by cube dilations (each one a
+// pass along x, y and z) until every cell is reached or cap passes were made.
+// Empty for the two-level layout.  The passes are limited so the build stays
+// within about 2^28 cell updates (the map only has to be a lower bound).
+std::vector<uint8_t> empty_run_map(const pvdb_topology* t, int cap) {
+    std::vector<uint8_t> dist;
+    if (t->two_level) return dist;
+    const int n0 = t->dim[0], n1 = t->dim[1], n2 = t->dim[2];
+    const size_t n = (size_t)n0 * n1 * n2;
+    cap = (int)std::max<size_t>(2, std::min<size_t>((size_t)cap, (size_t(1) << 28) / (3 * n + 1)));
+    dist.assign(n, (uint8_t)cap);
+    std::vector<uint8_t> cur(n), tmp(n);
+    size_t left = n;
+    for (size_t c = 0; c < n; ++c) {
+        cur[c] = t->table[c] >= 0;
+        if (cur[c]) {
+            dist[c] = 0;
+            --left;
+        }
+    }
+    auto idx = [&](int a, int b, int c) { return ((size_t)a * n1 + b) * n2 + c; };
+    for (int r = 1; r < cap && left > 0; ++r) {
+        // tmp = cur dilated by one cell along z, then y, then x (back into cur)
+        for (int a = 0; a < n0; ++a)
+            for (int b = 0; b < n1; ++b)
+                for (int c = 0; c < n2; ++c)
+                    tmp[idx(a, b, c)] = cur[idx(a, b, c)] | (c > 0 && cur[idx(a, b, c - 1)]) |
+                                        (c + 1 < n2 && cur[idx(a, b, c + 1)]);
+        std::vector<uint8_t> nb(n);
+        for (int a = 0; a < n0; ++a)
+            for (int b = 0; b < n1; ++b)
+                for (int c = 0; c < n2; ++c)
+                    nb[idx(a, b, c)] = tmp[idx(a, b, c)] | (b > 0 && tmp[idx(a, b - 1, c)]) |
+                                       (b + 1 < n1 && tmp[idx(a, b + 1, c)]);
+        for (int a = 0; a < n0; ++a)
+            for (int b = 0; b < n1; ++b)
+                for (int c = 0; c < n2; ++c) {
+                    const size_t i = idx(a, b, c);
+                    const uint8_t v = nb[i] | (a > 0 && nb[idx(a - 1, b, c)]) |
+                                      (a + 1 < n0 && nb[idx(a + 1, b, c)]);
+                    if (v && !cur[i]) {
+                        dist[i] = (uint8_t)r;
+                        --left;
+                    }
+                    tmp[i] = v;
+                }
+        cur.swap(tmp);
+    }
+    return dist;
+}
 // layout: 0 automatic, 1 one-level (dense leaf table), 2 two-level.
 int finalize(std::vector<LeafRec>& leaves, int layout, pvdb_topology** out) {
     std::sort(leaves.begin(), leaves.end(), nano_less);
@@ -155,6 +210,7 @@ int finalize(std::vector<LeafRec>& leaves, int layout, pvdb_topology** out) {
         }
     }
     t->nactive = nact;
+    t->erun = empty_run_map(t, 64);
     int rc = hipGetDevice(&t->device) == hipSuccess ? PVDB_OK : PVDB_EHIP;
     if (rc == PVDB_OK) rc = upload(t->table, &t->d_table);
     if (rc == PVDB_OK) rc = upload(t->origins, &t->d_origins);
@@ -164,6 +220,12 @@ int finalize(std::vector<LeafRec>& leaves, int layout, pvdb_topology** out) {
         hipError_t e = hipMalloc(&t->d_masks, std::max<size_t>(t->masks.size(), 8) * 8);
         if (e == hipSuccess && !t->masks.empty())
             e = hipMemcpy(t->d_masks, t->masks.data(), t->masks.size() * 8, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = pvdb::hip_fail(e, "topology upload");
+    }
+    if (rc == PVDB_OK && !t->erun.empty()) {
+        hipError_t e = hipMalloc(&t->d_erun, t->erun.size());
+        if (e == hipSuccess)
+            e = hipMemcpy(t->d_erun, t->erun.data(), t->erun.size(), hipMemcpyHostToDevice);
         if (e != hipSuccess) rc = pvdb::hip_fail(e, "topology upload");
     }
     if (rc != PVDB_OK) {
@@ -184,6 +246,7 @@ TopoView view_of(const pvdb_topology* t) {
     v.origins = t->d_origins;
     v.lower_table = t->d_lower;
     v.lower_leaves = t->d_lower_leaves;
+    v.erun = t->d_erun;
     v.lo0 = t->lo[0];
     v.lo1 = t->lo[1];
     v.lo2 = t->lo[2];
@@ -201,6 +264,7 @@ TopoView view_of(const pvdb_topology* t) {
     v.nleaf = t->nleaf;
     return v;
 }
+
 }  // namespace pvdb
 
 extern "C" {
@@ -275,6 +339,7 @@ int pvdb_topo_destroy(pvdb_topology* t) {
     if (t->d_origins) hipFree(t->d_origins);
     if (t->d_lower) hipFree(t->d_lower);
     if (t->d_lower_leaves) hipFree(t->d_lower_leaves);
+    if (t->d_erun) hipFree(t->d_erun);
     if (dev != t->device) hipSetDevice(dev);
     delete t;
     return PVDB_OK;
