@@ -67,6 +67,14 @@ int pvdb_topo_slots(const pvdb_topology* topo, const int32_t* ijk /*host [n][3]*
                     int64_t* slots /*host*/, uint8_t* active /*host, may be NULL*/);
 int pvdb_topo_device_arrays(const pvdb_topology* topo, const int32_t** table,
                             const uint64_t** masks, const int32_t** origins);
+/* No counterpart in the reference (it steps through empty space voxel by
+ * voxel, render_utils_kernel.cu:373-424, renderer.cu:222-367): the empty-run
+ * map the marchers skip with, one byte per 8^3 cell of the leaf grid box
+ * (pvdb_topo_info's leaf_lo3 / leaf_dim3, C order): 0 where a leaf exists,
+ * else r >= 1 with no leaf within Chebyshev distance r - 1 (the smallest such
+ * r, capped at 64).  Copies it to `out` (host, may be NULL) and returns its
+ * size, 0 for the two-level layout, which has none. */
+int64_t pvdb_topo_empty_run(const pvdb_topology* topo, uint8_t* out /*host*/);
 
 /* ---- grid payload kernels ------------------------------------------------
  * Trilinear forward.  order 0: density rule of density_forward_kernel

@@ -425,7 +425,32 @@ __device__ __forceinline__ void advance_steps(float& t, float s, float tsafe, fl
 struct Corners {
     int32_t slot[8];  // the host keeps nslots + 1 below 2^31
     float u, v, w;
+    int i0, j0, k0;   // the cell's lower corner
 };
+
+// Slots of the 8 corners of the cell with lower corner (i0, j0, k0) in the
+// corner order 000,001,011,010,110,111,101,100 (slot nslots, index 0's
+// values, for a corner whose leaf does not exist).
+__device__ __forceinline__ void corner_slots(const RenderArgs& A, int i0, int j0, int k0,
+                                             BlockCache& bc, int32_t* slot) {
+    if (((i0 & 7) != 7) & ((j0 & 7) != 7) & ((k0 & 7) != 7)) {
+        // the cell lies inside one 8^3 block: one lookup, slots by in-leaf offset
+        const int32_t l0 = bc.get(A.topo, i0, j0, k0);
+        const int32_t s0 = l0 * 512 + voxel_offset(i0, j0, k0);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int cx = (kCx >> q) & 1, cy = (kCy >> q) & 1, cz = (kCz >> q) & 1;
+            slot[q] = l0 < 0 ? (int32_t)A.nslots : s0 + ((cx << 6) | (cy << 3) | cz);
+        }
+        return;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int cx = (kCx >> q) & 1, cy = (kCy >> q) & 1, cz = (kCz >> q) & 1;
+        const int32_t lq = bc.get(A.topo, i0 + cx, j0 + cy, k0 + cz);
+        slot[q] = lq < 0 ? (int32_t)A.nslots : lq * 512 + voxel_offset(i0 + cx, j0 + cy, k0 + cz);
+    }
+}
 
 __device__ __forceinline__ bool march_step(const RenderArgs& A, const PixelRay& R, float& t,
                                            float tend, int& iter, int max_iter, BlockCache& bc,
@@ -467,23 +492,10 @@ __device__ __forceinline__ bool march_step(const RenderArgs& A, const PixelRay& 
     c.u = x - (float)i0;
     c.v = y - (float)j0;
     c.w = z - (float)k0;
-    if (((i0 & 7) != 7) & ((j0 & 7) != 7) & ((k0 & 7) != 7)) {
-        // the cell lies inside one 8^3 block: one lookup, slots by in-leaf offset
-        const int32_t l0 = bc.get(A.topo, i0, j0, k0);
-        const int32_t s0 = l0 * 512 + voxel_offset(i0, j0, k0);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int cx = (kCx >> q) & 1, cy = (kCy >> q) & 1, cz = (kCz >> q) & 1;
-            c.slot[q] = l0 < 0 ? (int32_t)A.nslots : s0 + ((cx << 6) | (cy << 3) | cz);
-        }
-        return true;
-    }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int cx = (kCx >> q) & 1, cy = (kCy >> q) & 1, cz = (kCz >> q) & 1;
-        const int32_t lq = bc.get(A.topo, i0 + cx, j0 + cy, k0 + cz);
-        c.slot[q] = lq < 0 ? (int32_t)A.nslots : lq * 512 + voxel_offset(i0 + cx, j0 + cy, k0 + cz);
-    }
+    c.i0 = i0;
+    c.j0 = j0;
+    c.k0 = k0;
+    corner_slots(A, i0, j0, k0, bc, c.slot);
     return true;
 }
 
@@ -624,22 +636,12 @@ __global__ void __launch_bounds__(kMarchThreads) k_render_fill(RenderArgs A) {
             T = T * (1.f - alpha);
             if (wt <= A.thres) continue;
             if (r < cnt) {
-                // trigetColor order (:303-310), fp16 features
-                float f[12], cq[12];
-                load_col<F32PAY>(A.payload, c.slot[0], cq);
-#pragma unroll
-                for (int ch = 0; ch < 12; ++ch) f[ch] = cq[ch] * sc[0];
-#pragma unroll 1
-                for (int q = 1; q < 8; ++q) {
-                    load_col<F32PAY>(A.payload, c.slot[q], cq);
-#pragma unroll
-                    for (int ch = 0; ch < 12; ++ch) f[ch] = f[ch] + cq[ch] * sc[q];
-                }
+                // the sample's cell and fractions; k_render_gather adds its colour
                 const int64_t q = start + r;
                 if (q < F.capacity) {
-                    F.rec[2 * q] = make_uint4(pack2(f[0], f[1]), pack2(f[2], f[3]),
-                                              pack2(f[4], f[5]), pack2(f[6], f[7]));
-                    F.rec[2 * q + 1] = make_uint4(pack2(f[8], f[9]), pack2(f[10], f[11]),
+                    F.rec[2 * q] = make_uint4((uint32_t)c.i0, (uint32_t)c.j0, (uint32_t)c.k0,
+                                              __float_as_uint(c.u));
+                    F.rec[2 * q + 1] = make_uint4(__float_as_uint(c.v), __float_as_uint(c.w),
                                                   __float_as_uint(wt), (uint32_t)tp.pix);
                 }
             }
@@ -647,11 +649,11 @@ __global__ void __launch_bounds__(kMarchThreads) k_render_fill(RenderArgs A) {
         }
     }
     const int written = r < cnt ? r : cnt;
-    // reserved records pass 2 did not fill: zero weight, zero features
+    // reserved records pass 2 did not fill: no cell (i0 = -1), zero weight
     for (int k = written; k < cnt; ++k) {
         const int64_t q = start + k;
         if (q < F.capacity) {
-            F.rec[2 * q] = make_uint4(0u, 0u, 0u, 0u);
+            F.rec[2 * q] = make_uint4(0xffffffffu, 0u, 0u, 0u);
             F.rec[2 * q + 1] = make_uint4(0u, 0u, 0u, (uint32_t)tp.pix);
         }
     }
@@ -670,6 +672,56 @@ __global__ void __launch_bounds__(kMarchThreads) k_render_fill(RenderArgs A) {
         }
     }
     if (A.counters) wave_count_add(A.counters + 1, (unsigned long long)written, lane);
+}
+
+// Colour of each record (trigetColor, renderer.cu:303-310: corner 0's row
+// times its weight, then the others added in corner order, fp32), written over
+// the record's cell as 12 fp16 features: a thread per record, so the colour
+// loads run at full lane occupancy instead of inside the marches' divergent
+// loops.  Records: in (i0, j0, k0, u | v, w, weight, pixel), out (features
+// 0..7 | 8..11, weight, pixel); i0 = -1 marks a reserved record without a
+// sample (zero features).
+template <bool F32PAY>
+__global__ void __launch_bounds__(256) k_render_gather(RenderArgs A) {
+    const FrameBuffers& F = A.fb;
+    const unsigned long long tot = *F.total;
+    const int64_t n = (int64_t)(tot < (unsigned long long)F.capacity ? tot : F.capacity);
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
+         q += (int64_t)gridDim.x * blockDim.x) {
+        const uint4 a = F.rec[2 * q], b = F.rec[2 * q + 1];
+        uint4 o0 = make_uint4(0u, 0u, 0u, 0u), o1 = make_uint4(0u, 0u, b.z, b.w);
+        const int i0 = (int)a.x, j0 = (int)a.y, k0 = (int)a.z;
+        if (i0 >= 0) {
+            const float u = __uint_as_float(a.w), v = __uint_as_float(b.x), w = __uint_as_float(b.y);
+            BlockCache bc;
+            int32_t slot[8];
+            corner_slots(A, i0, j0, k0, bc, slot);
+            float sc[8];
+#pragma unroll
+            for (int c = 0; c < 8; ++c) {
+                const float wa = ((kCx >> c) & 1) ? u : 1.f - u;
+                const float wb = ((kCy >> c) & 1) ? v : 1.f - v;
+                const float we = ((kCz >> c) & 1) ? w : 1.f - w;
+                sc[c] = (wa * wb) * we;
+            }
+            float f[12], cq[12];
+            load_col<F32PAY>(A.payload, slot[0], cq);
+#pragma unroll
+            for (int ch = 0; ch < 12; ++ch) f[ch] = cq[ch] * sc[0];
+#pragma unroll
+            for (int c = 1; c < 8; ++c) {
+                load_col<F32PAY>(A.payload, slot[c], cq);
+#pragma unroll
+                for (int ch = 0; ch < 12; ++ch) f[ch] = f[ch] + cq[ch] * sc[c];
+            }
+            o0 = make_uint4(pack2(f[0], f[1]), pack2(f[2], f[3]), pack2(f[4], f[5]),
+                            pack2(f[6], f[7]));
+            o1.x = pack2(f[8], f[9]);
+            o1.y = pack2(f[10], f[11]);
+        }
+        F.rec[2 * q] = o0;
+        F.rec[2 * q + 1] = o1;
+    }
 }
 
 // The MLP over the records, 64 per wave-chunk; each record's contribution
@@ -983,12 +1035,16 @@ int render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void* ws,
             hipLaunchKernelGGL(k_render_count<true>, dim3(mgrid), dim3(kMarchThreads), 0, s, A);
             PVDB_LAUNCHED("k_render_count");
             hipLaunchKernelGGL(k_render_fill<true>, dim3(mgrid), dim3(kMarchThreads), 0, s, A);
+            PVDB_LAUNCHED("k_render_fill");
+            hipLaunchKernelGGL(k_render_gather<true>, dim3(16 * ncu), dim3(256), 0, s, A);
         } else {
             hipLaunchKernelGGL(k_render_count<false>, dim3(mgrid), dim3(kMarchThreads), 0, s, A);
             PVDB_LAUNCHED("k_render_count");
             hipLaunchKernelGGL(k_render_fill<false>, dim3(mgrid), dim3(kMarchThreads), 0, s, A);
+            PVDB_LAUNCHED("k_render_fill");
+            hipLaunchKernelGGL(k_render_gather<false>, dim3(16 * ncu), dim3(256), 0, s, A);
         }
-        PVDB_LAUNCHED("k_render_fill");
+        PVDB_LAUNCHED("k_render_gather");
     }
     static bool smem_set = false;
     if (!smem_set) {

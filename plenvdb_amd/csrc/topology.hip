@@ -372,6 +372,15 @@ int pvdb_topo_leaves(const pvdb_topology* t, int32_t* origins, uint64_t* masks) 
     return PVDB_OK;
 }
 
+int64_t pvdb_topo_empty_run(const pvdb_topology* t, uint8_t* out) {
+    if (!t) {
+        pvdb::set_error("null topology");
+        return PVDB_EHANDLE;
+    }
+    if (out && !t->erun.empty()) std::memcpy(out, t->erun.data(), t->erun.size());
+    return (int64_t)t->erun.size();
+}
+
 int pvdb_topo_slots(const pvdb_topology* t, const int32_t* ijk, int64_t n, int64_t* slots,
                     uint8_t* active) {
     if (!t) {

@@ -80,6 +80,21 @@ class Topology:
         lib.call("pvdb_topo_leaves", self._h, lib.hptr(o), lib.hptr(m))
         return o, m
 
+    def empty_run(self) -> np.ndarray | None:
+        """The marchers' empty-run map over the leaf grid box (uint8
+        [leaf_dim], 0 where a leaf exists, else the radius r of the empty cube
+        of cells around the cell, capped) or None for the two-level layout."""
+        fn = lib.load().pvdb_topo_empty_run
+        fn.restype = C.c_int64
+        fn.argtypes = [C.c_void_p, C.c_void_p]
+        n = int(fn(self._h, None))
+        if n <= 0:
+            return None
+        out = np.zeros(self.leaf_dim, dtype=np.uint8)
+        assert out.size == n
+        fn(self._h, lib.hptr(out))
+        return out
+
     def slots(self, ijk):
         """Slot index (-1 when the leaf does not exist) and active flag of voxels."""
         q = np.ascontiguousarray(np.asarray(ijk, dtype=np.int32).reshape(-1, 3))

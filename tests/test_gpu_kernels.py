@@ -219,3 +219,27 @@ def test_dense_adam_modes(oracle_mod):
             oracle_mod.dense_adam(rp, g, rm, rv, mode, sz, 1e-8, 0.9, 0.99,
                                   per_lr=pl if mode == 2 else None)
         np.testing.assert_array_equal(tp.cpu().numpy(), rp)
+
+
+@pytest.mark.parametrize("cfg", ["c2", "c4"])
+def test_empty_run_map_is_the_chessboard_distance(cfg):
+    """The marchers' empty-run map equals the chessboard distance from each
+    empty 8^3 cell to the nearest leaf (scipy.ndimage.distance_transform_cdt),
+    capped at 64; 0 on leaves."""
+    from scipy import ndimage
+
+    from plenvdb_amd import scenes
+    from plenvdb_amd.topology import Topology
+
+    topo = Topology.from_mask(scenes.occupancy_mask(scenes.CONFIGS[cfg]))
+    got = topo.empty_run()
+    assert got is not None and list(got.shape) == topo.leaf_dim
+    o, _ = topo.leaves()
+    occ = np.zeros(topo.leaf_dim, dtype=bool)
+    cells = (o >> 3) - np.asarray(topo.leaf_lo)
+    occ[cells[:, 0], cells[:, 1], cells[:, 2]] = True
+    want = np.minimum(ndimage.distance_transform_cdt(~occ, metric="chessboard"), 64)
+    print(f"[result] {cfg}: leaf grid {topo.leaf_dim}, {int(occ.sum())} leaves, "
+          f"max run {int(got.max())}", flush=True)
+    np.testing.assert_array_equal(got, want.astype(np.uint8))
+    assert got.max() >= 2  # the map does skip something here
