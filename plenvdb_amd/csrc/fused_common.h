@@ -149,4 +149,78 @@ struct MlpAdam {
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
+// The dW reduction a launch carries in its first `blocks` workgroups (0: none).
+struct ReduceArgs {
+    const float* part;
+    int nparts;
+    float* dw;
+    MlpAdam adam;
+    int blocks;
+};
+
+// Sum of the per-workgroup dW partials, added into dw without atomics: workgroup
+// `block` owns kReduceVec four-parameter vectors and splits the partials among
+// kReduceGrp lane groups (16-byte loads, eight in flight per lane); the groups'
+// sums are combined through LDS (`red`, kReduceGrp * kReduceVec float4) in a
+// fixed order, so the result is reproducible.  With Adam on (A.p), the same
+// thread then updates the parameters (adam_upd_kernel.cu:8-58, stepmode 0, on
+// the final gradient).  Needs 256 threads.
+constexpr int kReduceVec = 16, kReduceGrp = 16;
+static_assert(kReduceVec * kReduceGrp == 256, "one thread per (vector, group)");
+constexpr int kReduceBlocks = (kPartStride / 4 + kReduceVec - 1) / kReduceVec;
+__device__ __forceinline__ void dw_reduce_block(int block, const float* __restrict__ part,
+                                                int nparts, float* __restrict__ dw,
+                                                const MlpAdam& A, float4* red) {
+    const int lv = threadIdx.x % kReduceVec, grp = threadIdx.x / kReduceVec;
+    const int v = block * kReduceVec + lv;  // float4 index; rows are padded to 16 B
+    const int q = (nparts + kReduceGrp - 1) / kReduceGrp, b0 = grp * q, b1 = min(nparts, b0 + q);
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (4 * v < kMlpParams) {
+        for (int b = b0; b < b1; b += 8) {
+            float4 x[8];
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                x[k] = b + k < b1
+                           ? reinterpret_cast<const float4*>(part + (int64_t)(b + k) * kPartStride)[v]
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                s.x += x[k].x;
+                s.y += x[k].y;
+                s.z += x[k].z;
+                s.w += x[k].w;
+            }
+        }
+    }
+    red[grp * kReduceVec + lv] = s;
+    __syncthreads();
+    if (grp == 0 && 4 * v < kMlpParams) {
+        float4 t = red[lv];
+#pragma unroll
+        for (int g2 = 1; g2 < kReduceGrp; ++g2) {
+            const float4 u = red[g2 * kReduceVec + lv];
+            t.x += u.x;
+            t.y += u.y;
+            t.z += u.z;
+            t.w += u.w;
+        }
+        const float tv[4] = {t.x, t.y, t.z, t.w};
+        const float sz = A.dev_sz ? A.dev_sz[2] : A.sz;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {  // the padding element of the last vector is skipped
+            const int qq = 4 * v + k;
+            if (qq >= kMlpParams) continue;
+            const float gg = dw[qq] + tv[k];
+            dw[qq] = gg;
+            if (A.p) {
+                const float nm = A.b0 * A.m[qq] + (1.f - A.b0) * gg;
+                const float nv = A.b1 * A.v[qq] + ((1.f - A.b1) * gg) * gg;
+                A.m[qq] = nm;
+                A.v[qq] = nv;
+                A.p[qq] = A.p[qq] - (sz * nm) / (sqrtf(nv) + A.eps);
+            }
+        }
+    }
+}
+
 }  // namespace pvdb

@@ -981,67 +981,14 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
     }
 }
 
-// Sum of the per-workgroup dW partials, added into dw without atomics: a
-// workgroup owns kReduceVec four-parameter vectors and splits the partials
-// among kReduceGrp lane groups (16-byte loads, eight in flight per lane); the
-// groups' sums are combined through LDS in a fixed order, so the result is
-// reproducible.  With Adam on, the same thread then updates the parameters
-// (adam_upd_kernel.cu:8-58, stepmode 0, on the final gradient).
-constexpr int kReduceVec = 16, kReduceGrp = 16;
-static_assert(kReduceVec * kReduceGrp == 256, "one thread per (vector, group)");
+// Sum of the per-workgroup dW partials (dw_reduce_block): one workgroup per
+// kReduceVec four-parameter vectors.  The fused step runs these workgroups
+// inside the scatter's launch instead (launch_scatter), where they overlap the
+// scatter's atomics; this launch serves the optimizer-only step.
 __global__ void __launch_bounds__(256) k_dw_reduce(const float* __restrict__ part, int nparts,
                                                    float* __restrict__ dw, MlpAdam A) {
-    __shared__ float4 red[kReduceGrp][kReduceVec];
-    const int lv = threadIdx.x % kReduceVec, grp = threadIdx.x / kReduceVec;
-    const int v = blockIdx.x * kReduceVec + lv;  // float4 index; rows are padded to 16 B
-    const int q = (nparts + kReduceGrp - 1) / kReduceGrp, b0 = grp * q, b1 = min(nparts, b0 + q);
-    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (4 * v < kMlpParams) {
-        for (int b = b0; b < b1; b += 8) {
-            float4 x[8];
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                x[k] = b + k < b1
-                           ? reinterpret_cast<const float4*>(part + (int64_t)(b + k) * kPartStride)[v]
-                           : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                s.x += x[k].x;
-                s.y += x[k].y;
-                s.z += x[k].z;
-                s.w += x[k].w;
-            }
-        }
-    }
-    red[grp][lv] = s;
-    __syncthreads();
-    if (grp == 0 && 4 * v < kMlpParams) {
-        float4 t = red[0][lv];
-#pragma unroll
-        for (int g2 = 1; g2 < kReduceGrp; ++g2) {
-            const float4 u = red[g2][lv];
-            t.x += u.x;
-            t.y += u.y;
-            t.z += u.z;
-            t.w += u.w;
-        }
-        const float tv[4] = {t.x, t.y, t.z, t.w};
-        const float sz = A.dev_sz ? A.dev_sz[2] : A.sz;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {  // the padding element of the last vector is skipped
-            const int q = 4 * v + k;
-            if (q >= kMlpParams) continue;
-            const float gg = dw[q] + tv[k];
-            dw[q] = gg;
-            if (A.p) {
-                const float nm = A.b0 * A.m[q] + (1.f - A.b0) * gg;
-                const float nv = A.b1 * A.v[q] + ((1.f - A.b1) * gg) * gg;
-                A.m[q] = nm;
-                A.v[q] = nv;
-                A.p[q] = A.p[q] - (sz * nm) / (sqrtf(nv) + A.eps);
-            }
-        }
-    }
+    __shared__ float4 red[kReduceGrp * kReduceVec];
+    dw_reduce_block(blockIdx.x, part, nparts, dw, A, red);
 }
 
 
@@ -1070,8 +1017,7 @@ int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const Topo
 
 int launch_dw_reduce(const float* part, int nparts, float* dw, const MlpAdam& adam,
                      hipStream_t s) {
-    hipLaunchKernelGGL(k_dw_reduce, dim3((kPartStride / 4 + kReduceVec - 1) / kReduceVec), dim3(256),
-                       0, s, part, nparts, dw, adam);
+    hipLaunchKernelGGL(k_dw_reduce, dim3(kReduceBlocks), dim3(256), 0, s, part, nparts, dw, adam);
     PVDB_LAUNCHED("k_dw_reduce");
     return PVDB_OK;
 }

@@ -79,12 +79,16 @@ __global__ void __launch_bounds__(256) k_scatter_direct(TopoView topo, RayBuffer
 // kAggRecords (32 or 64, a launch choice) records per chunk.
 constexpr uint32_t kEmptyKey = 0xffffffffu;
 constexpr uint32_t kEnd = 0xffffu;  // end of a pair list
+// The launch's first `red.blocks` workgroups sum the MLP's dW partials
+// instead (dw_reduce_block, independent of the scatter): they overlap the
+// scatter's atomics rather than following them in a launch of their own.
 template <int kAggRecords>
 __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R, Records rec,
                                                      int64_t capacity,
                                                      float* __restrict__ grad_density,
                                                      float* __restrict__ grad_color,
-                                                     uint8_t* __restrict__ leaf_touched) {
+                                                     uint8_t* __restrict__ leaf_touched,
+                                                     ReduceArgs red) {
     constexpr int kAggPairsAll = 8 * kAggRecords;  // (record, corner) pairs of a chunk
     constexpr int kAggSlots = kAggPairsAll;        // every pair fits: probing ends
     constexpr int kAggLog2 = kAggRecords == 64 ? 9 : 8;  // kAggSlots == 1 << kAggLog2
@@ -93,9 +97,16 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
     __shared__ uint32_t key[kAggSlots];
     __shared__ uint32_t head[kAggSlots];          // first pair of the entry's list
     __shared__ uint16_t next[kAggPairsAll];       // next pair of the same entry
-    __shared__ float pv[kAggPairsAll * 13];       // each pair's 13 contributions
+    __shared__ __align__(16) float pv[kAggPairsAll * 13];  // each pair's 13 contributions
     __shared__ uint16_t listed[kAggSlots];        // the entries in use
     __shared__ int nlisted[2];
+    static_assert(kAggPairsAll * 13 >= 4 * kReduceGrp * kReduceVec, "reduction scratch");
+    if ((int)blockIdx.x < red.blocks) {  // uniform over the workgroup
+        dw_reduce_block(blockIdx.x, red.part, red.nparts, red.dw, red.adam,
+                        reinterpret_cast<float4*>(pv));
+        return;
+    }
+    const int wg = (int)blockIdx.x - red.blocks, nwg = (int)gridDim.x - red.blocks;
     const int tid = threadIdx.x;
     for (int e = tid; e < kAggSlots; e += 256) {
         key[e] = kEmptyKey;
@@ -122,9 +133,9 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
             }
         }
     };
-    load(blockIdx.x);
+    load(wg);
     int b = 0;
-    for (int64_t chunk = blockIdx.x; chunk < nchunk; chunk += gridDim.x, b ^= 1) {
+    for (int64_t chunk = wg; chunk < nchunk; chunk += nwg, b ^= 1) {
         // ---- topology of this thread's pairs ----
         int32_t leaf[kAggPairs];
         uint32_t off[kAggPairs];
@@ -179,7 +190,7 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
             next[pair] = (uint16_t)atomicExch(&head[h], (uint32_t)pair);
         }
         __syncthreads();
-        load(chunk + gridDim.x);  // the next chunk's records, in flight during the flush
+        load(chunk + nwg);  // the next chunk's records, in flight during the flush
         // ---- flush: walk each listed entry, add its sums, clear it; reset the
         // other list counter.  16 lanes per entry, so one atomic instruction covers
         // four slots' contiguous 12-float colour rows (fewer cache lines than
@@ -218,10 +229,16 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
 
 namespace pvdb {
 
+int launch_dw_reduce(const float* part, int nparts, float* dw, const MlpAdam& adam,
+                     hipStream_t s);
+
 int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, int64_t capacity,
                    float* grad_density, float* grad_color, uint8_t* leaf_touched, int grid_req,
-                   int chunk_records, hipStream_t s) {
-    if (t.nleaf == 0) return PVDB_OK;
+                   int chunk_records, const ReduceArgs& red_in, hipStream_t s) {
+    if (t.nleaf == 0)
+        return red_in.blocks ? launch_dw_reduce(red_in.part, red_in.nparts, red_in.dw, red_in.adam, s)
+                             : PVDB_OK;
+    const ReduceArgs red = red_in;
     if (t.nleaf < (int64_t(1) << 23) - 1) {  // every slot < 2^32 - 1, the empty key
         PVDB_CHECK_ARG(chunk_records == 32 || chunk_records == 64,
                        "launch[2]: scatter chunk must be 32 or 64 records");
@@ -229,13 +246,14 @@ int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, i
         // each workgroup takes chunks in a grid-stride loop; four (64-record) or
         // eight (32-record) workgroups fit a CU by LDS
         const int64_t want = grid_req > 0 ? grid_req : 1;
-        const unsigned grid = (unsigned)(chunks < want ? (chunks < 1 ? 1 : chunks) : want);
+        const unsigned grid = (unsigned)(chunks < want ? (chunks < 1 ? 1 : chunks) : want) +
+                              (unsigned)red.blocks;
         if (chunk_records == 64)
             hipLaunchKernelGGL(k_scatter_agg<64>, dim3(grid), dim3(256), 0, s, t, R, rec, capacity,
-                               grad_density, grad_color, leaf_touched);
+                               grad_density, grad_color, leaf_touched, red);
         else
             hipLaunchKernelGGL(k_scatter_agg<32>, dim3(grid), dim3(256), 0, s, t, R, rec, capacity,
-                               grad_density, grad_color, leaf_touched);
+                               grad_density, grad_color, leaf_touched, red);
         PVDB_LAUNCHED("k_scatter_agg");
         return PVDB_OK;
     }
@@ -245,7 +263,7 @@ int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, i
     hipLaunchKernelGGL(k_scatter_direct, dim3(grid), dim3(256), 0, s, t, R, rec, capacity,
                        grad_density, grad_color, leaf_touched);
     PVDB_LAUNCHED("k_scatter_direct");
-    return PVDB_OK;
+    return red.blocks ? launch_dw_reduce(red.part, red.nparts, red.dw, red.adam, s) : PVDB_OK;
 }
 
 }  // namespace pvdb

@@ -28,7 +28,7 @@ int launch_dw_reduce(const float* part, int nparts, float* dw, const MlpAdam& ad
                      hipStream_t s);
 int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, int64_t capacity,
                    float* grad_density, float* grad_color, uint8_t* leaf_touched, int grid_req,
-                   int chunk_records, hipStream_t s);
+                   int chunk_records, const ReduceArgs& red, hipStream_t s);
 }  // namespace pvdb
 
 using namespace pvdb;
@@ -133,56 +133,57 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
         for (int k = threadIdx.x; k < 512; k += blockDim.x) {
             const int64_t s = leaf * 512 + k;
             const bool act = (t.masks[leaf * 8 + (k >> 6)] >> (k & 63)) & 1ull;
+            float4* g4 = reinterpret_cast<float4*>(gcol) + s * 3;
             if (act) {
-                const float g = gden[s];
-                if (g != 0.f) adam1(den[s], g, mden[s], vden[s], sz_d, eps, b0, b1);
-                // the voxel's 12 channels as three 16-byte vectors per array
-                float4* g4 = reinterpret_cast<float4*>(gcol) + s * 3;
+                // every operand of an active voxel is loaded at once (one round
+                // trip; a voxel whose gradients are all zero writes nothing back)
                 float4* m4 = reinterpret_cast<float4*>(mcol) + s * 3;
                 float4* v4 = reinterpret_cast<float4*>(vcol) + s * 3;
+                float gd = gden[s], pd = den[s], md = mden[s], vd = vden[s];
                 float gg[12], mm[12], vv[12], pp[12];
-                bool any = false;
 #pragma unroll
                 for (int k4 = 0; k4 < 3; ++k4) {
-                    const float4 q = g4[k4];
+                    const float4 q = g4[k4], qm = m4[k4], qv = v4[k4];
                     gg[4 * k4] = q.x; gg[4 * k4 + 1] = q.y; gg[4 * k4 + 2] = q.z; gg[4 * k4 + 3] = q.w;
+                    mm[4 * k4] = qm.x; mm[4 * k4 + 1] = qm.y; mm[4 * k4 + 2] = qm.z; mm[4 * k4 + 3] = qm.w;
+                    vv[4 * k4] = qv.x; vv[4 * k4 + 1] = qv.y; vv[4 * k4 + 2] = qv.z; vv[4 * k4 + 3] = qv.w;
                 }
-#pragma unroll
-                for (int c = 0; c < 12; ++c) any = any || gg[c] != 0.f;
-                if (any) {
+                if (BF16COL && !cmaster) {
+                    const uint2* c2 = reinterpret_cast<const uint2*>(col) + s * 3;
 #pragma unroll
                     for (int k4 = 0; k4 < 3; ++k4) {
-                        const float4 qm = m4[k4], qv = v4[k4];
-                        mm[4 * k4] = qm.x; mm[4 * k4 + 1] = qm.y; mm[4 * k4 + 2] = qm.z; mm[4 * k4 + 3] = qm.w;
-                        vv[4 * k4] = qv.x; vv[4 * k4 + 1] = qv.y; vv[4 * k4 + 2] = qv.z; vv[4 * k4 + 3] = qv.w;
+                        const uint2 q = c2[k4];
+                        pp[4 * k4] = bf_bits2f((uint16_t)(q.x & 0xffffu));
+                        pp[4 * k4 + 1] = bf_bits2f((uint16_t)(q.x >> 16));
+                        pp[4 * k4 + 2] = bf_bits2f((uint16_t)(q.y & 0xffffu));
+                        pp[4 * k4 + 3] = bf_bits2f((uint16_t)(q.y >> 16));
                     }
-                    if (BF16COL && !cmaster) {
-                        const uint2* c2 = reinterpret_cast<const uint2*>(col) + s * 3;
+                } else {
+                    const float4* c4 =
+                        reinterpret_cast<const float4*>(BF16COL ? cmaster : col) + s * 3;
 #pragma unroll
-                        for (int k4 = 0; k4 < 3; ++k4) {
-                            const uint2 q = c2[k4];
-                            pp[4 * k4] = bf_bits2f((uint16_t)(q.x & 0xffffu));
-                            pp[4 * k4 + 1] = bf_bits2f((uint16_t)(q.x >> 16));
-                            pp[4 * k4 + 2] = bf_bits2f((uint16_t)(q.y & 0xffffu));
-                            pp[4 * k4 + 3] = bf_bits2f((uint16_t)(q.y >> 16));
-                        }
-                    } else {
-                        const float4* c4 =
-                            reinterpret_cast<const float4*>(BF16COL ? cmaster : col) + s * 3;
-#pragma unroll
-                        for (int k4 = 0; k4 < 3; ++k4) {
-                            const float4 q = c4[k4];
-                            pp[4 * k4] = q.x; pp[4 * k4 + 1] = q.y; pp[4 * k4 + 2] = q.z; pp[4 * k4 + 3] = q.w;
-                        }
+                    for (int k4 = 0; k4 < 3; ++k4) {
+                        const float4 q = c4[k4];
+                        pp[4 * k4] = q.x; pp[4 * k4 + 1] = q.y; pp[4 * k4 + 2] = q.z; pp[4 * k4 + 3] = q.w;
                     }
-                    // stepmode 1 skips a Vec3 group only when all three components are zero
+                }
+                if (gd != 0.f) {
+                    adam1(pd, gd, md, vd, sz_d, eps, b0, b1);
+                    den[s] = pd;
+                    mden[s] = md;
+                    vden[s] = vd;
+                }
+                bool any = false;
+                // stepmode 1 skips a Vec3 group only when all three components are zero
 #pragma unroll
-                    for (int grp = 0; grp < 4; ++grp) {
-                        const int c0 = 3 * grp;
-                        if (gg[c0] == 0.f && gg[c0 + 1] == 0.f && gg[c0 + 2] == 0.f) continue;
+                for (int grp = 0; grp < 4; ++grp) {
+                    const int c0 = 3 * grp;
+                    if (gg[c0] == 0.f && gg[c0 + 1] == 0.f && gg[c0 + 2] == 0.f) continue;
+                    any = true;
 #pragma unroll
-                        for (int c = c0; c < c0 + 3; ++c) adam1(pp[c], gg[c], mm[c], vv[c], sz_c, eps, b0, b1);
-                    }
+                    for (int c = c0; c < c0 + 3; ++c) adam1(pp[c], gg[c], mm[c], vv[c], sz_c, eps, b0, b1);
+                }
+                if (any) {
 #pragma unroll
                     for (int k4 = 0; k4 < 3; ++k4) {
                         m4[k4] = make_float4(mm[4 * k4], mm[4 * k4 + 1], mm[4 * k4 + 2], mm[4 * k4 + 3]);
@@ -210,13 +211,10 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
                 }
             }
             gden[s] = 0.f;
-            {
-                float4* g4 = reinterpret_cast<float4*>(gcol) + s * 3;
-                const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-                g4[0] = z;
-                g4[1] = z;
-                g4[2] = z;
-            }
+            const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+            g4[0] = z;
+            g4[1] = z;
+            g4[2] = z;
         }
         __syncthreads();
         if (threadIdx.x == 0) touched[leaf] = 0;
@@ -492,21 +490,19 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
             return rc;
     }
     {
+        // the scatter's launch also carries the dW reduction (its first
+        // workgroups), and the MLP Adam rides on the reduction (each parameter
+        // once its gradient is final)
         StageScope sc(6, s);
-        if ((rc = launch_scatter(t, R, rec, d->capacity, d->grad_density, d->grad_color, touched,
-                                 d->launch[1] > 0 ? d->launch[1] : kScatterGrid,
-                                 d->launch[2] > 0 ? d->launch[2] : kScatterChunk,
-                                 s)))
-            return rc;
-    }
-    {
-        StageScope sc(7, s);
-        // the MLP Adam rides on the reduction (each parameter once its gradient is final)
         MlpAdam adam{nullptr, nullptr, nullptr, nullptr, 0.f, 0.f, 0.f, 0.f};
         if (d->do_adam)
             adam = MlpAdam{d->mlp,           d->mlp_m, d->mlp_v, d->adam_scalars,
                            d->step_size_mlp, d->eps,   d->beta0, d->beta1};
-        if ((rc = launch_dw_reduce(dw_part, grid_bwd, d->mlp_grad, adam, s))) return rc;
+        const ReduceArgs red{dw_part, grid_bwd, d->mlp_grad, adam, kReduceBlocks};
+        if ((rc = launch_scatter(t, R, rec, d->capacity, d->grad_density, d->grad_color, touched,
+                                 d->launch[1] > 0 ? d->launch[1] : kScatterGrid,
+                                 d->launch[2] > 0 ? d->launch[2] : kScatterChunk, red, s)))
+            return rc;
     }
     if (d->ray_out) {
         PVDB_HIP(hipMemcpyAsync(d->ray_out, R.ray_out, 16 * (size_t)d->n_rays,
