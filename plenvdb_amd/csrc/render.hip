@@ -204,82 +204,87 @@ __device__ __forceinline__ void load_col(const void* pay, int64_t slot, float* c
 
 __device__ __forceinline__ f16x8 as_h8(const uint4& v) { return __builtin_bit_cast(f16x8, v); }
 
-// RGB logits of the samples staged in this wave's rows of sX (row = lane),
-// for the 16-sample tiles flagged in tmask; lanes 0..15 write them to sL.
-// Tiles go two at a time, so each weight fragment read from LDS feeds two
-// MFMAs and the two tiles' activations stay in registers.
+// RGB logits of the 64 samples staged in this wave's rows of sX (row =
+// lane); lanes 0..15 write them to sL (the rows of empty tiles hold zeros, so
+// their logits are the bias and are not read).  All four 16-sample tiles go
+// through each layer together, so every weight fragment read from LDS feeds
+// four MFMAs and the tiles' activations stay in registers.
 __device__ __forceinline__ void wave_mlp(const uint4* __restrict__ sW, const float* __restrict__ sB,
                                          const _Float16* __restrict__ sX, float* __restrict__ sL,
-                                         unsigned tmask, int lane) {
+                                         int lane) {
     const int g = lane >> 4, r16 = lane & 15;
     const uint4* W0 = sW;
     const uint4* W1 = sW + kW0Frag / 8;
     const uint4* W2 = W1 + kW1Frag / 8;
-    const float b2x = sB[128], b2y = sB[129], b2z = sB[130];
-    for (int tp = 0; tp < 2; ++tp) {
-        const unsigned pm = (tmask >> (2 * tp)) & 3u;
-        if (!pm) continue;  // uniform over the wave
-        uint32_t h0[2][4][4];  // [tile][k-step][dword]: layer-1 B fragments (fp16 pairs)
-        uint32_t h1[2][4][4];
-        {
-            f16x8 xb[2][2];
+    uint32_t h0[4][4][4];  // [tile][k-step][dword]: layer-1 B fragments (fp16 pairs)
+    {
+        f16x8 xb[4][2];
 #pragma unroll
-            for (int u = 0; u < 2; ++u)
+        for (int u = 0; u < 4; ++u)
 #pragma unroll
-                for (int s = 0; s < 2; ++s)
-                    xb[u][s] = as_h8(*reinterpret_cast<const uint4*>(
-                        sX + (16 * (2 * tp + u) + r16) * XS + 32 * s + 8 * g));
-#pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                const f16x8 a0 = as_h8(W0[(m * 2 + 0) * 64 + lane]);
-                const f16x8 a1 = as_h8(W0[(m * 2 + 1) * 64 + lane]);
-#pragma unroll
-                for (int u = 0; u < 2; ++u) {
-                    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-                    acc = mfma16x16x32h(a0, xb[u][0], acc);
-                    acc = mfma16x16x32h(a1, xb[u][1], acc);
-                    const int s = m >> 1, d = (m & 1) * 2;
-                    h0[u][s][d] = pack2(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f));
-                    h0[u][s][d + 1] = pack2(fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
-                }
-            }
-        }
+            for (int s = 0; s < 2; ++s)
+                xb[u][s] = as_h8(*reinterpret_cast<const uint4*>(sX + (16 * u + r16) * XS +
+                                                                 32 * s + 8 * g));
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
+            const f16x8 a0 = as_h8(W0[(m * 2 + 0) * 64 + lane]);
+            const f16x8 a1 = as_h8(W0[(m * 2 + 1) * 64 + lane]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+                acc = mfma16x16x32h(a0, xb[u][0], acc);
+                acc = mfma16x16x32h(a1, xb[u][1], acc);
+                const int s = m >> 1, d = (m & 1) * 2;
+                h0[u][s][d] = pack2(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f));
+                h0[u][s][d + 1] = pack2(fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
+            }
+        }
+    }
+    // layer 1, and layer 2 as its outputs appear: layer 2's k-step s takes
+    // layer 1's output tiles 2s and 2s + 1, so only one k-step of H1 is live
+    const float b2x = sB[128], b2y = sB[129], b2z = sB[130];
+    f32x4 acc2[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc2[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2) {
+        uint32_t hk[4][4];  // [tile][dword]: layer 2's B fragments of k-step s2
+#pragma unroll
+        for (int mm = 0; mm < 2; ++mm) {
+            const int m = 2 * s2 + mm;
             const float4 bias = *reinterpret_cast<const float4*>(sB + 16 * m + 4 * g);
-            f32x4 acc[2];
-            acc[0] = acc[1] = f32x4{0.f, 0.f, 0.f, 0.f};
+            f32x4 acc[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) acc[u] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const f16x8 a = as_h8(W1[(m * 4 + s) * 64 + lane]);
 #pragma unroll
-                for (int u = 0; u < 2; ++u) {
+                for (int u = 0; u < 4; ++u) {
                     const uint4 bv = make_uint4(h0[u][s][0], h0[u][s][1], h0[u][s][2], h0[u][s][3]);
                     acc[u] = mfma16x16x32h(a, as_h8(bv), acc[u]);
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 2; ++u) {
-                const int s = m >> 1, d = (m & 1) * 2;
-                h1[u][s][d] = pack2(fmaxf(acc[u][0] + bias.x, 0.f), fmaxf(acc[u][1] + bias.y, 0.f));
-                h1[u][s][d + 1] =
+            for (int u = 0; u < 4; ++u) {
+                hk[u][2 * mm] = pack2(fmaxf(acc[u][0] + bias.x, 0.f), fmaxf(acc[u][1] + bias.y, 0.f));
+                hk[u][2 * mm + 1] =
                     pack2(fmaxf(acc[u][2] + bias.z, 0.f), fmaxf(acc[u][3] + bias.w, 0.f));
             }
         }
+        const f16x8 a2 = as_h8(W2[s2 * 64 + lane]);
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int u = 0; u < 4; ++u)
+            acc2[u] = mfma16x16x32h(a2, as_h8(make_uint4(hk[u][0], hk[u][1], hk[u][2], hk[u][3])),
+                                    acc2[u]);
+    }
+    if (g == 0) {
 #pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const uint4 bv = make_uint4(h1[u][s][0], h1[u][s][1], h1[u][s][2], h1[u][s][3]);
-                acc = mfma16x16x32h(as_h8(W2[s * 64 + lane]), as_h8(bv), acc);
-            }
-            if (g == 0 && ((pm >> u) & 1u)) {
-                float* dst = sL + (16 * (2 * tp + u) + r16) * 4;
-                dst[0] = acc[0] + b2x;
-                dst[1] = acc[1] + b2y;
-                dst[2] = acc[2] + b2z;
-            }
+        for (int u = 0; u < 4; ++u) {
+            float* dst = sL + (16 * u + r16) * 4;
+            dst[0] = acc2[u][0] + b2x;
+            dst[1] = acc2[u][1] + b2y;
+            dst[2] = acc2[u][2] + b2z;
         }
     }
 }
@@ -549,6 +554,9 @@ __global__ void __launch_bounds__(kMarchThreads) k_render_count(RenderArgs A) {
         while (t < R.tmax && iter < cap1) {
             Corners c;
             if (!march_step(A, R, t, R.tmax, iter, cap1, bc, c)) continue;
+            float dv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) dv[q] = load_den<F32PAY>(A.den, c.slot[q]);
             // trigetDensity order (:191-220): res += den * (1-u) * (1-v) * (1-w), ...
             float sig = 0.f;
 #pragma unroll
@@ -556,7 +564,7 @@ __global__ void __launch_bounds__(kMarchThreads) k_render_count(RenderArgs A) {
                 const float a = ((kCx >> q) & 1) ? c.u : 1.f - c.u;
                 const float b = ((kCy >> q) & 1) ? c.v : 1.f - c.v;
                 const float e = ((kCz >> q) & 1) ? c.w : 1.f - c.w;
-                sig = sig + ((load_den<F32PAY>(A.den, c.slot[q]) * a) * b) * e;
+                sig = sig + ((dv[q] * a) * b) * e;
             }
             ++n_gather;
             const float alpha = 1.f - powf(1.f + expf(sig + A.act_shift), -A.interval);
@@ -618,6 +626,9 @@ __global__ void __launch_bounds__(kMarchThreads) k_render_fill(RenderArgs A) {
         while (t < tmax2 && iter < cap2) {
             Corners c;
             if (!march_step(A, R, t, tmax2, iter, cap2, bc, c)) continue;
+            float dv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) dv[q] = load_den<F32PAY>(A.den, c.slot[q]);
             float sc[8];
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
@@ -627,9 +638,9 @@ __global__ void __launch_bounds__(kMarchThreads) k_render_fill(RenderArgs A) {
                 sc[q] = (a * b) * e;
             }
             // trigetDensity2 order (:271-300): sum of den * scale, left to right
-            float sig = load_den<F32PAY>(A.den, c.slot[0]) * sc[0];
+            float sig = dv[0] * sc[0];
 #pragma unroll
-            for (int q = 1; q < 8; ++q) sig = sig + load_den<F32PAY>(A.den, c.slot[q]) * sc[q];
+            for (int q = 1; q < 8; ++q) sig = sig + dv[q] * sc[q];
             const float alpha = 1.f - powf(1.f + expf(sig + A.act_shift), -A.interval);
             if (alpha <= A.thres) continue;
             const float wt = T * alpha;
@@ -746,41 +757,72 @@ __global__ void __launch_bounds__(64 * kWaves) k_render_mlp(RenderArgs A) {
     if (A.counters && blockIdx.x == 0 && threadIdx.x == 0) A.counters[2] = tot;
     const int64_t n = (int64_t)(tot < (unsigned long long)F.capacity ? tot : F.capacity);
     const int64_t nchunks = (n + 63) / 64;
-    for (int64_t ch = (int64_t)blockIdx.x * kWaves + wv; ch < nchunks;
-         ch += (int64_t)gridDim.x * kWaves) {
+    const int64_t stride = (int64_t)gridDim.x * kWaves;
+    // Two-stage prefetch: a chunk's records arrive one chunk ahead and its
+    // pixels' PE rows (which need the records) are loaded while the chunk
+    // before it runs the MLP, so neither load's latency sits in front of the
+    // MFMAs.
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    auto load_rec = [&](int64_t ch, uint4& a, uint4& b) {
         const int64_t q = ch * 64 + lane;
-        const bool live = q < n;
-        uint4 a = make_uint4(0u, 0u, 0u, 0u), b = a;
-        if (live) {
+        a = b = z;
+        if (ch < nchunks && q < n) {
             a = F.rec[2 * q];
             b = F.rec[2 * q + 1];
-            const uint4* pr = F.pe + 4 * (int64_t)b.w;
-            row[2] = pr[0];
-            row[3] = pr[1];
-            row[4] = pr[2];
-            row[5] = pr[3];
-        } else {
-            row[2] = row[3] = row[4] = row[5] = make_uint4(0u, 0u, 0u, 0u);
         }
-        row[0] = a;
-        row[1] = make_uint4(b.x, b.y, 0u, 0u);
-        const unsigned long long lm = __ballot(live);
-        unsigned tmask = 0;
-#pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if ((lm >> (16 * k)) & 0xffffull) tmask |= 1u << k;
+    };
+    auto load_pe = [&](int64_t ch, const uint4& b, uint4* pe) {
+        const int64_t q = ch * 64 + lane;
+        pe[0] = pe[1] = pe[2] = pe[3] = z;
+        if (ch < nchunks && q < n) {
+            const uint4* pr = F.pe + 4 * (int64_t)b.w;
+            pe[0] = pr[0];
+            pe[1] = pr[1];
+            pe[2] = pr[2];
+            pe[3] = pr[3];
+        }
+    };
+    int64_t ch = (int64_t)blockIdx.x * kWaves + wv;
+    uint4 ca, cb, cpe[4], na, nb;
+    load_rec(ch, ca, cb);
+    load_pe(ch, cb, cpe);
+    load_rec(ch + stride, na, nb);
+    for (; ch < nchunks; ch += stride) {
+        const int64_t q = ch * 64 + lane;
+        const bool live = q < n;
+        row[0] = ca;
+        row[1] = make_uint4(cb.x, cb.y, 0u, 0u);
+        row[2] = cpe[0];
+        row[3] = cpe[1];
+        row[4] = cpe[2];
+        row[5] = cpe[3];
+        // the next chunk's PE rows and the records of the one after
+        uint4 npe[4], fa, fb;
+        load_pe(ch + stride, nb, npe);
+        load_rec(ch + 2 * stride, fa, fb);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        wave_mlp(sW, sB, sX, sL, tmask, lane);
+        wave_mlp(sW, sB, sX, sL, lane);
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         if (live) {
-            const float w = __uint_as_float(b.z);
+            const float w = __uint_as_float(cb.z);
             const float* lg = sL + lane * 4;
-            F.contrib[q] = make_float4(w / (1.f + expf(-lg[0])), w / (1.f + expf(-lg[1])),
-                                       w / (1.f + expf(-lg[2])), 0.f);
+            // w * sigmoid(raw) with the hardware exp and reciprocal (a few ulps,
+            // far inside the fp16 MLP's own rounding)
+            F.contrib[q] = make_float4(w * __builtin_amdgcn_rcpf(1.f + __expf(-lg[0])),
+                                       w * __builtin_amdgcn_rcpf(1.f + __expf(-lg[1])),
+                                       w * __builtin_amdgcn_rcpf(1.f + __expf(-lg[2])), 0.f);
         }
         __builtin_amdgcn_wave_barrier();  // sL and the rows are rewritten by the next chunk
+        ca = na;
+        cb = nb;
+        cpe[0] = npe[0];
+        cpe[1] = npe[1];
+        cpe[2] = npe[2];
+        cpe[3] = npe[3];
+        na = fa;
+        nb = fb;
     }
 }
 
