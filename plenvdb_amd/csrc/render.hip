@@ -697,8 +697,11 @@ __global__ void __launch_bounds__(256) k_render_gather(RenderArgs A) {
     const FrameBuffers& F = A.fb;
     const unsigned long long tot = *F.total;
     const int64_t n = (int64_t)(tot < (unsigned long long)F.capacity ? tot : F.capacity);
-    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n;
-         q += (int64_t)gridDim.x * blockDim.x) {
+    // one thread per record (the grid covers the capacity): each thread's
+    // chain of dependent loads (record, leaf table, colour rows) is short, and
+    // the waves in flight overlap them
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < n) {
         const uint4 a = F.rec[2 * q], b = F.rec[2 * q + 1];
         uint4 o0 = make_uint4(0u, 0u, 0u, 0u), o1 = make_uint4(0u, 0u, b.z, b.w);
         const int i0 = (int)a.x, j0 = (int)a.y, k0 = (int)a.z;
@@ -1078,13 +1081,15 @@ int render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void* ws,
             PVDB_LAUNCHED("k_render_count");
             hipLaunchKernelGGL(k_render_fill<true>, dim3(mgrid), dim3(kMarchThreads), 0, s, A);
             PVDB_LAUNCHED("k_render_fill");
-            hipLaunchKernelGGL(k_render_gather<true>, dim3(16 * ncu), dim3(256), 0, s, A);
+            hipLaunchKernelGGL(k_render_gather<true>, dim3(pvdb_blocks(A.fb.capacity, 256)),
+                               dim3(256), 0, s, A);
         } else {
             hipLaunchKernelGGL(k_render_count<false>, dim3(mgrid), dim3(kMarchThreads), 0, s, A);
             PVDB_LAUNCHED("k_render_count");
             hipLaunchKernelGGL(k_render_fill<false>, dim3(mgrid), dim3(kMarchThreads), 0, s, A);
             PVDB_LAUNCHED("k_render_fill");
-            hipLaunchKernelGGL(k_render_gather<false>, dim3(16 * ncu), dim3(256), 0, s, A);
+            hipLaunchKernelGGL(k_render_gather<false>, dim3(pvdb_blocks(A.fb.capacity, 256)),
+                               dim3(256), 0, s, A);
         }
         PVDB_LAUNCHED("k_render_gather");
     }
