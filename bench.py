@@ -241,9 +241,9 @@ def bench_batched(args, scene):
 # x1.0); profiles/pmc_traffic_c2_train_c5_render.md holds the runs.
 TRAFFIC_SOURCE = "profiles/pmc_traffic_c2_train_c5_render.md"
 MEASURED_TRAFFIC_MB = {
-    "c2": {"mlp_bwd": 59.67, "scatter": 64.86, "mlp_fwd": 64.29, "ray_march": 14.09,
-           "dw_reduce": 33.00, "composite": 15.24, "adam": 107.12, "ray_count": 2.23},
-    "c5": {"render": 111.83},
+    "c2": {"mlp_bwd": 59.66, "scatter": 97.88, "mlp_fwd": 64.29, "ray_march": 14.53,
+           "composite": 15.28, "adam": 114.86, "ray_count": 2.38},
+    "c5": {"render_mlp": 656.89},
 }
 
 
@@ -267,11 +267,12 @@ def stage_models(r):
         "composite": ("hbm", S * 16 + nw * 32 + N * (12 + 16 + 16 + 16 + 4)),
         # 73,216 FLOP per MLP sample + 6,912 per ray (dW of the PE block)
         "mlp_bwd": ("mfma", nw * MLP_BWD_FLOP + N * MLP_RAY_FLOP),
-        # records + per-record gradients in, distinct voxels' 13 gradients read+written
-        "scatter": ("hbm", S * (16 + 16 + 64) + uw * 13 * 4 * 2),
-        # one dW partial per CU in, MLP gradient read+written; with the MLP
-        # Adam: parameters and both moments read+written
-        "dw_reduce": ("hbm", r["n_cu"] * P * 4 + 2 * P * 4 + 6 * P * 4),
+        # records + per-record gradients in, distinct voxels' 13 gradients
+        # read+written; the same launch reduces the dW partials (one per CU in,
+        # MLP gradient read+written) and applies the MLP Adam (parameters and
+        # both moments read+written)
+        "scatter": ("hbm", S * (16 + 16 + 64) + uw * 13 * 4 * 2
+                    + r["n_cu"] * P * 4 + 2 * P * 4 + 6 * P * 4),
         # B_adam: density p,m,v r/w + g r; colour p,m,v r/w + g r
         "adam": ("hbm", V_u * (28 + 12 * (2 * sc + 20)) + L_t * 512 * 13 * 4),
     }
