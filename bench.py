@@ -207,7 +207,7 @@ def bench_batched(args, scene):
     tr.forward_backward()
     torch.cuda.synchronize()
     rec = tr.records()
-    flags = rec["atwr"][:, 3].view(np.uint32)
+    flags = rec["ab"][:, 1].view(np.uint32)
     alive_w = (flags & np.uint32(1 << 31)) != 0
     gd = tr.grad_density.cpu().numpy()
     gc = tr.grad_color.cpu().numpy()

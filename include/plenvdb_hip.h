@@ -300,11 +300,12 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* desc, void*
 int pvdb_fused_timing(int32_t enable, double* total_ms, int32_t* calls, int32_t n);
 const char* pvdb_fused_stage_name(int32_t i);
 /* Device pointers of the per-sample records and per-ray counts inside the
- * workspace (for inspection and tests).  Per record, float4 each: xyze
- * (index-space position, exp(sigma + shift)), atwr (alpha, T, w, ray id and
- * flags), rgb, adj (dL/dlogit x3, dL/dsigma). */
+ * workspace (for inspection and tests).  Per record: xyze float4 (index-space
+ * position, exp(sigma + shift)), ab float2 (alpha, ray id and flags; the
+ * transmittance and weight are recomputed from the alphas where they are
+ * needed, not stored), rgb float4, adj float4 (dL/dlogit x3, dL/dsigma). */
 int pvdb_fused_records(const pvdb_topology* topo, const pvdb_step_desc* desc, void** xyze,
-                       void** atwr, void** rgb, void** adj, int32_t** count, int32_t** offset,
+                       void** ab, void** rgb, void** adj, int32_t** count, int32_t** offset,
                        int32_t** total);
 
 /* ---- merged-VDB renderer ----------------------------------------------------

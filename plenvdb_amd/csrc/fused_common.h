@@ -55,7 +55,7 @@ struct StepParams {
 // Records: structure of arrays, index = record id.
 struct Records {
     float4* xyze;      // index-space x, y, z; e = exp(sigma + shift)
-    float4* atwr;      // alpha, T, w, ray|flags (as bits)
+    float2* ab;        // alpha, ray|flags (as bits); T and w are recomputed from the alphas
     float4* rgb;       // MLP colour (rgb), written by the forward MLP pass
     float4* adj;       // per-sample adjoints: dL/dlogit (3), dL/dsigma
     uint4* feat;       // [3 per record] the 12 gathered colour features as bf16 hi, then lo

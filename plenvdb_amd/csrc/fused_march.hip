@@ -302,8 +302,7 @@ __global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, con
             if (q >= 0 && q < P.capacity) {
                 uint32_t bits = (uint32_t)r | (alive_a ? kAliveA : 0u) | (alive_w ? kAliveW : 0u);
                 rec.xyze[q] = make_float4(x, y, z, alive_a ? e : 0.f);
-                rec.atwr[q] = make_float4(alive_a ? alpha : 0.f, alive_a ? T : 0.f,
-                                          alive_w ? w : 0.f, __uint_as_float(bits));
+                rec.ab[q] = make_float2(alive_a ? alpha : 0.f, __uint_as_float(bits));
             }
         }
         written += k;
@@ -343,6 +342,27 @@ __global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, con
     if (lane == 0) R.t_last[r] = Tc;
 }
 
+// Transmittance before each record of a 64-record chunk, recomputed from the
+// stored alphas rather than kept per sample: the same inclusive product scan
+// of (1 - alpha) over the alpha-alive lanes, carried from chunk to chunk in Tc,
+// that the forward march ran.  The chunks are the march's 64-sample batches
+// (its record index advances by 64 per batch), and the product for a lane only
+// involves lanes at or before it, so every alive record's T is bit-identical
+// to the march's.
+__device__ __forceinline__ float chunk_transmittance(float alpha, uint32_t bits, int lane,
+                                                     float& Tc) {
+    float incl = (bits & kAliveA) ? 1.f - alpha : 1.f;
+    for (int d = 1; d < 64; d <<= 1) {
+        const float o = __shfl_up(incl, d, 64);
+        if (lane >= d) incl *= o;
+    }
+    float excl = __shfl_up(incl, 1, 64);
+    if (lane == 0) excl = 1.f;
+    const float T = Tc * excl;
+    Tc = __shfl(Tc * incl, 63, 64);
+    return T;
+}
+
 // Per-ray composite: inclusive prefix of w*rgb (kept for the alpha adjoint),
 // ray colour = sum + T_last*bg (dvgo.py:365-370), the loss of run.py:551-574
 // (MSE; background entropy of T_last; per-sample rgb loss with the weights
@@ -357,22 +377,24 @@ __global__ void __launch_bounds__(256) k_ray_composite(StepParams P, RayBuffers 
     const bool rgbper = P.rgbper_scale > 0.f;
     float cr = 0.f, cg = 0.f, cb = 0.f;
     float sq = 0.f;  // this lane's sum of w * |rgb - target|^2 (rgbper term)
+    float Tc = 1.f;
     for (int base = 0; base < cnt; base += 64) {
         const int j = base + lane;
         float vr = 0.f, vg = 0.f, vb = 0.f;
         const int64_t q = off + j;
         const bool valid = j < cnt && q >= 0 && q < P.capacity;
-        if (valid) {
-            const float4 a = rec.atwr[q];
-            if (__float_as_uint(a.w) & kAliveW) {
-                const float4 c = rec.rgb[q];
-                vr = a.z * c.x;
-                vg = a.z * c.y;
-                vb = a.z * c.z;
-                if (rgbper) {
-                    const float e0 = c.x - t0, e1 = c.y - t1, e2 = c.z - t2;
-                    sq += a.z * ((e0 * e0 + e1 * e1) + e2 * e2);
-                }
+        float2 ab = make_float2(0.f, 0.f);
+        if (valid) ab = rec.ab[q];
+        const uint32_t bits = __float_as_uint(ab.y);
+        const float w = chunk_transmittance(ab.x, bits, lane, Tc) * ab.x;
+        if (bits & kAliveW) {
+            const float4 c = rec.rgb[q];
+            vr = w * c.x;
+            vg = w * c.y;
+            vb = w * c.z;
+            if (rgbper) {
+                const float e0 = c.x - t0, e1 = c.y - t1, e2 = c.z - t2;
+                sq += w * ((e0 * e0 + e1 * e1) + e2 * e2);
             }
         }
         for (int d = 1; d < 64; d <<= 1) {
@@ -423,25 +445,25 @@ __global__ void __launch_bounds__(256) k_ray_composite(StepParams P, RayBuffers 
     // (render_utils_kernel.cu:653-676, 506-517), with the exclusive suffix of
     // w*rgb taken as the ray total minus the inclusive prefix
     float pr = 0.f, pg = 0.f, pb = 0.f;
+    Tc = 1.f;
     for (int base = 0; base < cnt; base += 64) {
         const int j = base + lane;
         const int64_t q = off + j;
         const bool valid = j < cnt && q >= 0 && q < P.capacity;
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f), c = a;
+        float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+        float2 ab = make_float2(0.f, 0.f);
         float e = 0.f;
-        uint32_t bits = 0u;
         float vr = 0.f, vg = 0.f, vb = 0.f;
-        if (valid) {
-            a = rec.atwr[q];
-            bits = __float_as_uint(a.w);
-            if (bits & kAliveW) {
-                c = rec.rgb[q];
-                vr = a.z * c.x;
-                vg = a.z * c.y;
-                vb = a.z * c.z;
-            }
-            if (bits & kAliveA) e = rec.xyze[q].w;
+        if (valid) ab = rec.ab[q];
+        const uint32_t bits = __float_as_uint(ab.y);
+        const float T = chunk_transmittance(ab.x, bits, lane, Tc), w = T * ab.x;
+        if (bits & kAliveW) {
+            c = rec.rgb[q];
+            vr = w * c.x;
+            vg = w * c.y;
+            vb = w * c.z;
         }
+        if (bits & kAliveA) e = rec.xyze[q].w;
         for (int d = 1; d < 64; d <<= 1) {  // the same scan as the first pass
             const float o0_ = __shfl_up(vr, d, 64), o1_ = __shfl_up(vg, d, 64),
                         o2_ = __shfl_up(vb, d, 64);
@@ -453,15 +475,15 @@ __global__ void __launch_bounds__(256) k_ray_composite(StepParams P, RayBuffers 
         }
         float gl0 = 0.f, gl1 = 0.f, gl2 = 0.f, gw = 0.f, gs = 0.f;
         if (bits & kAliveW) {
-            gl0 = a.z * (g0 + grp * (c.x - t0)) * c.x * (1.f - c.x);
-            gl1 = a.z * (g1 + grp * (c.y - t1)) * c.y * (1.f - c.y);
-            gl2 = a.z * (g2 + grp * (c.z - t2)) * c.z * (1.f - c.z);
+            gl0 = w * (g0 + grp * (c.x - t0)) * c.x * (1.f - c.x);
+            gl1 = w * (g1 + grp * (c.y - t1)) * c.y * (1.f - c.y);
+            gl2 = w * (g2 + grp * (c.z - t2)) * c.z * (1.f - c.z);
             gw = g0 * c.x + g1 * c.y + g2 * c.z;
         }
         if (bits & kAliveA) {
             const float back = gbg * tl + g0 * (cr - (pr + vr)) + g1 * (cg - (pg + vg)) +
                                g2 * (cb - (pb + vb));
-            const float ga = gw * a.y - back / ((1.f - a.x) + 1e-10f);
+            const float ga = gw * T - back / ((1.f - ab.x) + 1e-10f);
             gs = fminf(e, 1e10f) * powf(1.f + e, -P.interval - 1.f) * P.interval * ga;
         }
         if (valid) rec.adj[q] = make_float4(gl0, gl1, gl2, gs);

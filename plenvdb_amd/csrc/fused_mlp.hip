@@ -268,13 +268,15 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
     const int64_t ngroups = (total + GS - 1) / GS;
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     // staging registers: a group's records (two groups ahead of the one computed)
-    float4 p_xyze = z4, p_atwr = z4;
+    float4 p_xyze = z4;
+    float2 p_ab = make_float2(0.f, 0.f);
     auto stage = [&](int64_t grp) {
         const int64_t q = grp * GS + pr;
-        p_xyze = p_atwr = z4;
+        p_xyze = z4;
+        p_ab = make_float2(0.f, 0.f);
         if (pr >= 0 && grp < ngroups && q < total) {
             p_xyze = rec.xyze[q];
-            p_atwr = rec.atwr[q];
+            p_ab = rec.ab[q];
         }
     };
     auto commit = [&](int b) {
@@ -283,7 +285,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
             ri.x = p_xyze.x;
             ri.y = p_xyze.y;
             ri.z = p_xyze.z;
-            ri.bits = __float_as_uint(p_atwr.w);  // zero for records past the end
+            ri.bits = __float_as_uint(p_ab.y);  // zero for records past the end
             sm.ri[b][pr] = ri;
         }
     };
@@ -675,13 +677,15 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
     const uint4 zu4 = make_uint4(0u, 0u, 0u, 0u);
 
     // register prefetch of a group's records on the staging lanes
-    float4 p_atwr = z4, p_adj = z4, p_vd = z4;
+    float4 p_adj = z4, p_vd = z4;
+    float2 p_ab = make_float2(0.f, 0.f);
     uint4 p_f0 = zu4, p_f1 = zu4, p_r0 = zu4, p_r1 = zu4;
     auto stage_a = [&](int64_t grp) {  // the records (independent loads)
         const int64_t q = grp * GS + pr;
-        p_atwr = p_adj = z4;
+        p_adj = z4;
+        p_ab = make_float2(0.f, 0.f);
         if (pr >= 0 && grp < ngroups && q < total) {
-            p_atwr = rec.atwr[q];
+            p_ab = rec.ab[q];
             p_adj = rec.adj[q];
         }
     };
@@ -690,7 +694,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
         p_vd = z4;
         p_f0 = p_f1 = p_r0 = p_r1 = zu4;
         if (pr >= 0 && grp < ngroups && q < total) {
-            const uint32_t bits = __float_as_uint(p_atwr.w);
+            const uint32_t bits = __float_as_uint(p_ab.y);
             if (bits & kAliveW) {
                 const uint32_t ray = bits & kRayMask;
                 p_vd = make_float4(P.viewdirs[3 * ray], P.viewdirs[3 * ray + 1],
@@ -707,7 +711,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
     // columns 0..11)
     auto park = [&](int b) {
         if (pr >= 0) {
-            sm.bits[b][pr] = __float_as_uint(p_atwr.w);  // zero for records past the end
+            sm.bits[b][pr] = __float_as_uint(p_ab.y);  // zero for records past the end
             sm.gsig[b][pr] = p_adj.w;
             sm.vd[b][pr] = p_vd;
             reinterpret_cast<uint4*>(&sm.relu[b][8 * pr])[0] = p_r0;

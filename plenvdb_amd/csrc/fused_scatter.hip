@@ -35,7 +35,7 @@ __global__ void __launch_bounds__(256) k_scatter_direct(TopoView topo, RayBuffer
          t += (int64_t)gridDim.x * blockDim.x) {
         const int64_t q = t >> 3;
         const int corner = (int)(t & 7);
-        const uint32_t bits = __float_as_uint(rec.atwr[q].w);
+        const uint32_t bits = __float_as_uint(rec.ab[q].y);
         if (!(bits & (kAliveA | kAliveW))) continue;
         const float4 p = rec.xyze[q];
         const Cell cell = make_cell(p.x, p.y, p.z);
@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
             const int64_t q = chunk * kAggRecords + ((tid + 256 * it) >> 3);
             bits[it] = 0u;
             if (chunk < nchunk && q < total) {
-                bits[it] = __float_as_uint(rec.atwr[q].w);
+                bits[it] = __float_as_uint(rec.ab[q].y);
                 pt[it] = rec.xyze[q];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) gr[it][k] = rec.grad[4 * q + k];

@@ -48,7 +48,7 @@ inline bool aligned(const void* p, uintptr_t a) { return (reinterpret_cast<uintp
 
 struct Layout {
     size_t count, offset, total, t_last, ray_out, ray_grad, ray_ctot, ray_loss, loss;
-    size_t xyze, atwr, rgb, adj, feat, relu, grad;
+    size_t xyze, ab, rgb, adj, feat, relu, grad;
     size_t touched, dw_part, w0t, w1t, w2t, w2, w1, w0;
     size_t bytes;
 };
@@ -73,7 +73,7 @@ Layout layout_for(int64_t n_rays, int64_t capacity, int64_t nleaf) {
     L.ray_loss = take(4 * n_rays);
     L.loss = take(4);
     L.xyze = take(16 * capacity);
-    L.atwr = take(16 * capacity);
+    L.ab = take(8 * capacity);
     L.rgb = take(16 * capacity);
     L.adj = take(16 * capacity);
     L.feat = take(48 * capacity);
@@ -414,7 +414,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     R.loss = d->loss ? d->loss : reinterpret_cast<float*>(ws + L.loss);
     Records rec;
     rec.xyze = reinterpret_cast<float4*>(ws + L.xyze);
-    rec.atwr = reinterpret_cast<float4*>(ws + L.atwr);
+    rec.ab = reinterpret_cast<float2*>(ws + L.ab);
     rec.rgb = reinterpret_cast<float4*>(ws + L.rgb);
     rec.adj = reinterpret_cast<float4*>(ws + L.adj);
     rec.feat = reinterpret_cast<uint4*>(ws + L.feat);
@@ -516,7 +516,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
 }
 
 int pvdb_fused_records(const pvdb_topology* topo, const pvdb_step_desc* d, void** xyze,
-                       void** atwr, void** rgb, void** adj, int32_t** count, int32_t** offset,
+                       void** ab, void** rgb, void** adj, int32_t** count, int32_t** offset,
                        int32_t** total) {
     if (!topo) {
         pvdb::set_error("null topology");
@@ -526,7 +526,7 @@ int pvdb_fused_records(const pvdb_topology* topo, const pvdb_step_desc* d, void*
     const Layout L = layout_for(d->n_rays, d->capacity, view_of(topo).nleaf);
     uint8_t* ws = reinterpret_cast<uint8_t*>(d->workspace);
     if (xyze) *xyze = ws + L.xyze;
-    if (atwr) *atwr = ws + L.atwr;
+    if (ab) *ab = ws + L.ab;
     if (rgb) *rgb = ws + L.rgb;
     if (adj) *adj = ws + L.adj;
     if (count) *count = reinterpret_cast<int32_t*>(ws + L.count);

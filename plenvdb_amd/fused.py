@@ -384,8 +384,8 @@ class FusedTrainer:
         total = _read_i32(ptrs[6].value, 1)[0]
         total = min(total, self.capacity)
         out = {"total": total, "count": _read_i32(ptrs[4].value, n), "offset": _read_i32(ptrs[5].value, n)}
-        for k, p in zip(("xyze", "atwr", "rgb", "adj"), ptrs[:4]):
-            out[k] = _read_f32(p.value, 4 * total).reshape(total, 4)
+        for k, p, w in zip(("xyze", "ab", "rgb", "adj"), ptrs[:4], (4, 2, 4, 4)):
+            out[k] = _read_f32(p.value, w * total).reshape(total, w)
         return out
 
 
