@@ -259,19 +259,21 @@ def stage_models(r):
         # gradient zeroed by the same launch
         "ray_count": ("hbm", N * (12 + 12 + 4)
                       + P * 4 + 2 * (128 * 64 + 2 * 128 * 128 + 2 * 16 * 128 + 128 * 32) + P * 4),
-        # rays + offsets in, 32-byte records out, distinct density voxels once
-        "ray_march": ("hbm", N * (12 + 12 + 8) + S * 32 + ua * 4),
+        # rays + offsets in, 24-byte records out (xyze, alpha + flags), distinct
+        # density voxels once
+        "ray_march": ("hbm", N * (12 + 12 + 8) + S * 24 + ua * 4),
         # 36,608 FLOP per MLP sample + 6,912 per ray (PE part of layer 0)
         "mlp_fwd": ("mfma", nw * MLP_FWD_FLOP + N * MLP_RAY_FLOP),
-        # records + rgb in, prefix out; rays in/out
-        "composite": ("hbm", S * 16 + nw * 32 + N * (12 + 16 + 16 + 16 + 4)),
+        # alpha + flags and exp(sigma) in, adjoints out per record; rgb in per
+        # weighted record; rays in/out
+        "composite": ("hbm", S * (8 + 4 + 16) + nw * 16 + N * (12 + 16 + 16 + 16 + 4)),
         # 73,216 FLOP per MLP sample + 6,912 per ray (dW of the PE block)
         "mlp_bwd": ("mfma", nw * MLP_BWD_FLOP + N * MLP_RAY_FLOP),
         # records + per-record gradients in, distinct voxels' 13 gradients
         # read+written; the same launch reduces the dW partials (one per CU in,
         # MLP gradient read+written) and applies the MLP Adam (parameters and
         # both moments read+written)
-        "scatter": ("hbm", S * (16 + 16 + 64) + uw * 13 * 4 * 2
+        "scatter": ("hbm", S * (16 + 8 + 64) + uw * 13 * 4 * 2
                     + r["n_cu"] * P * 4 + 2 * P * 4 + 6 * P * 4),
         # B_adam: density p,m,v r/w + g r; colour p,m,v r/w + g r
         "adam": ("hbm", V_u * (28 + 12 * (2 * sc + 20)) + L_t * 512 * 13 * 4),
