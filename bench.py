@@ -46,7 +46,7 @@ def parse():
     ap.add_argument("--launch", default=None,
                     help="fixed launch shapes of the headline step instead of tuning: "
                          "mlp_fwd_workgroups,scatter_workgroups,scatter_chunk_records")
-    ap.add_argument("--configs", default="c3,c4",
+    ap.add_argument("--configs", default="c1,c3,c4",
                     help="other workloads whose fused step is timed beside the headline one "
                          "(comma list, empty for none)")
     ap.add_argument("--no-plain", action="store_true",
@@ -353,7 +353,9 @@ def bench_config(name: str, steps: int, warmup: int):
     out = {"workload": f"{cfg.name}: {cfg.reso}^3 grid, {cfg.n_spheres} spheres"
                        f"{' + shell' if cfg.shell else ''}, {cfg.n_rays} rays"
                        f"{' (far in [%g, %g])' % (cfg.far_lo, cfg.far_hi) if cfg.far_lo else ''}, "
-                       f"{'bf16' if tr.color_bf16 else 'fp32'} colour, fwd+bwd+Adam",
+                       f"{'bf16' if tr.color_bf16 else 'fp32'} colour"
+                       f"{'' if cfg.use_mlp else ', no MLP (rgb = sigmoid of the %d-ch k0)' % cfg.channels}"
+                       ", fwd+bwd+Adam",
            "ms_per_step": round(ms, 4), "samples_per_step": int(samples),
            "samples_per_s": round(samples / (ms * 1e-3), 1), "launch": tuned["launch"]}
     if samples > tr.capacity:

@@ -208,9 +208,11 @@ int pvdb_alpha2weight_bwd(const float* alpha, const float* weight, const float* 
  * One iteration of plenvdb/run.py:541-598 on the fine-stage model
  * (dvgo.py:296-388 forward with the 12-ch feature grid and the 39-128-128-3
  * rgbnet, MSE loss, backward through Raw2Alpha / Alphas2Weights /
- * QueryVerticalInVDB, MaskedAdam on the MLP and VDBAdam stepmode 1 on both
- * grids) as one fixed launch sequence on `stream`: no host synchronisation, no
- * allocation (all scratch in `workspace`), so it can be captured in a hipGraph.
+ * QueryVerticalInVDB, MaskedAdam on the MLP and VDBAdam on both grids) or on
+ * a model without the MLP (k0_dim, the coarse stage: rgb = sigmoid(k0),
+ * dvgo.py:72-75,344-346) as one fixed launch sequence on `stream`: no host
+ * synchronisation, no allocation (all scratch in `workspace`), so it can be
+ * captured in a hipGraph.
  * Replaces, for that loop, the per-call round trips of DensityVDB/ColorVDB
  * forward/backward (plenvdb.cpp:11-29,46-64) plus render_utils_cuda and
  * adam_upd_cuda.  Payloads: density [slots] fp32, colour [slots][12] fp32 or
@@ -251,7 +253,11 @@ typedef struct pvdb_step_desc {
      * reference's separate VDBAdam.step / MaskedAdam.step after a backward,
      * masked_adam.py:70-80,127-158), over the leaves those calls touched */
     int32_t do_adam;
-    int32_t reserved;
+    /* 0: the MLP model above (12 colour features; mlp* required).  3 or 12: no
+     * MLP, the colour grid holds k0_dim fp32 channels per slot and
+     * rgb = sigmoid(k0) (dvgo.py:72-75,344-346); target then holds k0_dim
+     * values per ray and ray_out k0_dim + 1 (rgb, T_last); mlp* are ignored. */
+    int32_t k0_dim;
     void* workspace; /* dev, pvdb_fused_workspace_bytes() bytes, 256-byte aligned */
     int64_t workspace_bytes;
     /* max records (occupancy-kept samples) per step, in [1, 2^31 - 2].  Records
@@ -260,7 +266,7 @@ typedef struct pvdb_step_desc {
      * the full count saturated at 2^31 - 1, exceeds it.  The sum of the rays'
      * step counts (pvdb_ray_bounds) bounds every step. */
     int64_t capacity;
-    float* ray_out;   /* dev [n][4] rgb + T_last, or NULL */
+    float* ray_out;   /* dev [n][4] rgb + T_last ([n][k0_dim + 1] with k0_dim), or NULL */
     float* loss;      /* dev [1] or NULL */
     /* dev [3] or NULL: stepsz_density, stepsz_color, step_size_mlp read on the
      * device instead of the three fields above, so one captured graph of the
@@ -282,6 +288,16 @@ typedef struct pvdb_step_desc {
      * round-to-nearest bf16 value after each update (NULL: the bf16 payload is
      * updated in place).  Ignored for an fp32 payload. */
     float* color_master;
+    /* dev [slots] per-voxel rate of density stepmode 2 (set_pervoxel_lr,
+     * masked_adam.py:43-46), or NULL */
+    const float* density_lr;
+    /* VDBAdam stepmodes (masked_adam.py:56-68; densityvdb.cu:185-349,
+     * colorvdb.cu:179-351): density 1 skip-zero-grad (the fine stage), 0 every
+     * active voxel, 2 every active voxel with density_lr; colour 1 (a Vec3
+     * group is skipped when its three gradients are zero) or 0.  With both at
+     * 1 the Adam visits only the leaves the backward touched. */
+    int32_t stepmode_density;
+    int32_t stepmode_color;
 } pvdb_step_desc;
 
 /* The default launch shapes (pvdb_step_desc.launch) on the current device. */

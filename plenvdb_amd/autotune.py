@@ -41,7 +41,7 @@ def defaults() -> list[int]:
 def _key(tr) -> str:
     dev = torch.cuda.get_device_properties(torch.cuda.current_device())
     return (f"{dev.name}|{getattr(dev, 'gcnArchName', '')}|rays={tr.n_rays}|cap={tr.capacity}"
-            f"|leaves={tr.topo.nleaf}|bf16={int(tr.color_bf16)}")
+            f"|leaves={tr.topo.nleaf}|bf16={int(tr.color_bf16)}|k0={tr.k0_dim}")
 
 
 def _load_file() -> dict:
@@ -97,7 +97,9 @@ def tune(tr, reps: int = 3, force: bool = False) -> dict:
     best = defaults()
     apply(tr, best)
     timings = {}
-    for idx in (0, 2, 1):  # the chunk length before the scatter's workgroup count
+    # the chunk length before the scatter's workgroup count; a model without the
+    # MLP has no forward-MLP launch to shape
+    for idx in ((2, 1) if tr.k0_dim else (0, 2, 1)):
         t = {}
         for v in CANDIDATES[idx]:
             tr.desc.launch[idx] = v

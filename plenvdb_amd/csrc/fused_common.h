@@ -20,13 +20,23 @@
 //   k_scatter_agg  workgroups over 32- or 64-record chunks: corner contributions
 //                  summed per slot in an LDS hash, device atomics per distinct slot
 //   k_dw_reduce    the dW partials summed in a fixed order (+ the MLP Adam)
-//   k_sparse_adam  VDB Adam over the touched leaves, gradients re-zeroed
+//   k_sparse_adam  VDB Adam over the touched leaves (every leaf in stepmodes 0
+//                  and 2), gradients re-zeroed
+// A model without the MLP (k0_dim 3 or 12: rgb = sigmoid(k0)) runs k_ray_count,
+// k_scan, k_ray_march, then k_direct_composite (the k0 gather, composite, loss
+// and per-sample gradients in one wave-per-ray kernel), k_loss_reduce,
+// k_scatter_agg over its k0 channels and k_sparse_adam.
 #pragma once
 
 #include "mfma_util.h"
 #include "pvdb_common.h"
 
 namespace pvdb {
+
+// Float index of dL/dsigma in a record's gradient row: after the colour
+// gradients (12 features, or the 3 channels of a direct-colour grid).
+template <int NC>
+constexpr int kSigmaAt = NC == 12 ? 12 : 3;
 
 // Per-record flags (bits 30..31 of the packed ray field).
 constexpr uint32_t kAliveA = 1u << 30;  // in the alpha-kept set, before the early stop
@@ -61,7 +71,9 @@ struct Records {
     uint4* feat;       // [3 per record] the 12 gathered colour features as bf16 hi, then lo
     uint32_t* relu;    // [8 per record] forward ReLU masks: word 4l + j = layer l's
                        // hidden features 32j..32j+31, bit f%32
-    float4* grad;      // [4 per record] dL/dfeature (12) and dL/dsigma
+    float4* grad;      // [4 per record] dL/dfeature (12) and dL/dsigma; in the direct-colour
+                       // configurations rgb (NC) between the composite's two passes, then
+                       // dL/dk0 (NC) and dL/dsigma at float kSigmaAt<NC>
 };
 
 struct RayBuffers {
@@ -73,6 +85,7 @@ struct RayBuffers {
     float4* ray_grad;  // [N] dL/drgb (3) + dL/dT_last
     float4* ray_ctot;  // [N] sum of w*rgb (3)
     float* ray_loss;   // [N] the ray's share of the loss (all weighted terms)
+    float* ray_rgb;    // [N][NC + 1] direct-colour configurations: rendered rgb (NC) + T_last
     float* loss;       // [1]
 };
 

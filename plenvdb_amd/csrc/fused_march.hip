@@ -493,6 +493,178 @@ __global__ void __launch_bounds__(256) k_ray_composite(StepParams P, RayBuffers 
     }
 }
 
+// Inclusive prefix sums of v[0..NC-1] over the wavefront's lanes.
+template <int NC>
+__device__ __forceinline__ void wave_prefix(float (&v)[NC], int lane) {
+    for (int d = 1; d < 64; d <<= 1) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const float o = __shfl_up(v[c], d, 64);
+            if (lane >= d) v[c] += o;
+        }
+    }
+}
+
+// Composite of the configurations without the MLP (dvgo.py:72-75,344-346: the
+// colour grid holds k0_dim = NC channels and rgb = sigmoid(k0)).  One wave per
+// ray, as k_ray_composite.  Pass 1 gathers k0 for the weight-kept records (the
+// colour rule of colorvdb.cu:16-26,96-103), keeps rgb in the record's
+// gradient row and sums w*rgb; the ray colour, loss and dL/drgb follow
+// (dvgo.py:365-370, run.py:551-574).  Pass 2 turns each row into the
+// sample's gradients, dL/dk0 = dL/drgb * rgb(1 - rgb) and dL/dsigma through
+// Alphas2Weights and raw2alpha, which the scatter reads: no backward pass
+// over the records sits in between.
+template <int NC>
+__global__ void __launch_bounds__(256) k_direct_composite(StepParams P, TopoView topo,
+                                                          const float* __restrict__ color,
+                                                          RayBuffers R, Records rec) {
+    static_assert(NC == 3 || NC == 12, "direct colour channels");
+    constexpr int kSig = kSigmaAt<NC>;
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= P.n_rays) return;
+    const int lane = lane_id();
+    const int64_t off = R.offset[r];
+    const int cnt = R.count[r];
+    float tg[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) tg[c] = P.target[(int64_t)NC * r + c];
+    const bool rgbper = P.rgbper_scale > 0.f;
+    float ctot[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) ctot[c] = 0.f;
+    float sq = 0.f;  // this lane's sum of w * |rgb - target|^2 (rgbper term)
+    float Tc = 1.f;
+    for (int base = 0; base < cnt; base += 64) {
+        const int j = base + lane;
+        const int64_t q = off + j;
+        const bool valid = j < cnt && q >= 0 && q < P.capacity;
+        float2 ab = make_float2(0.f, 0.f);
+        if (valid) ab = rec.ab[q];
+        const uint32_t bits = __float_as_uint(ab.y);
+        const float w = chunk_transmittance(ab.x, bits, lane, Tc) * ab.x;
+        float v[NC];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[c] = 0.f;
+        if (bits & kAliveW) {
+            const float4 p = rec.xyze[q];
+            LeafCache cache;
+            float rgb[NC + 1];
+            float k0[NC];
+            gather_color<NC>(topo, color, make_cell(p.x, p.y, p.z), cache, k0);
+#pragma unroll
+            for (int c = 0; c < NC; ++c) rgb[c] = 1.f / (1.f + expf(-k0[c]));
+            rgb[NC] = 0.f;
+            float4* row = rec.grad + 4 * q;
+#pragma unroll
+            for (int k = 0; k < (NC + 3) / 4; ++k)
+                row[k] = make_float4(rgb[4 * k], rgb[4 * k + 1], rgb[4 * k + 2],
+                                     4 * k + 3 < NC ? rgb[4 * k + 3] : 0.f);
+            float e2 = 0.f;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                v[c] = w * rgb[c];
+                const float e = rgb[c] - tg[c];
+                e2 += e * e;
+            }
+            if (rgbper) sq += w * e2;
+        }
+        wave_prefix<NC>(v, lane);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) ctot[c] += __shfl(v[c], 63, 64);
+    }
+    // the ray's colour, loss and dL/drgb (every lane holds them)
+    const float tl = R.t_last[r];
+    float g[NC];
+    float loss = 0.f, gsum = 0.f;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const float o = ctot[c] + tl * P.bg, dd = o - tg[c];
+        g[c] = 2.f * dd * P.loss_scale;
+        loss += dd * dd;
+        gsum += g[c];
+    }
+    loss *= P.loss_scale;
+    float gent = 0.f;  // dL/dT_last of the entropy term
+    if (P.ent_scale > 0.f) {
+        const float p = fminf(fmaxf(tl, 1e-6f), 0.999999f);
+        const float lp = logf(p), lq = logf(1.f - p);
+        loss += -(p * lp + (1.f - p) * lq) * P.ent_scale;
+        if (tl >= 1e-6f && tl <= 0.999999f) gent = (lq - lp) * P.ent_scale;
+    }
+    if (rgbper) {
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) sq += __shfl_xor(sq, d, 64);
+        loss += sq * P.rgbper_scale;
+    }
+    const float gbg = P.bg * gsum + gent;      // dL/dT_last
+    const float grp = 2.f * P.rgbper_scale;    // dL/drgb_i = w_i (g + grp (rgb_i - target))
+    if (lane == 0) {
+        float* out = R.ray_rgb + (int64_t)(NC + 1) * r;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) out[c] = ctot[c] + tl * P.bg;
+        out[NC] = tl;
+        R.ray_loss[r] = loss;
+    }
+    float pre[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) pre[c] = 0.f;
+    Tc = 1.f;
+    for (int base = 0; base < cnt; base += 64) {
+        const int j = base + lane;
+        const int64_t q = off + j;
+        const bool valid = j < cnt && q >= 0 && q < P.capacity;
+        float2 ab = make_float2(0.f, 0.f);
+        if (valid) ab = rec.ab[q];
+        const uint32_t bits = __float_as_uint(ab.y);
+        const float T = chunk_transmittance(ab.x, bits, lane, Tc), w = T * ab.x;
+        float rgb[(NC + 3) / 4 * 4];
+#pragma unroll
+        for (int c = 0; c < (NC + 3) / 4 * 4; ++c) rgb[c] = 0.f;
+        float v[NC];
+        float e = 0.f;
+        float4* row = rec.grad + 4 * q;
+        if (bits & kAliveW) {
+#pragma unroll
+            for (int k = 0; k < (NC + 3) / 4; ++k) {
+                const float4 x = row[k];
+                rgb[4 * k] = x.x;
+                rgb[4 * k + 1] = x.y;
+                rgb[4 * k + 2] = x.z;
+                rgb[4 * k + 3] = x.w;
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[c] = w * rgb[c];
+        if (bits & kAliveA) e = rec.xyze[q].w;
+        wave_prefix<NC>(v, lane);
+        float gl[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) gl[c] = 0.f;
+        float gw = 0.f;
+        if (bits & kAliveW) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                gl[c] = w * (g[c] + grp * (rgb[c] - tg[c])) * rgb[c] * (1.f - rgb[c]);
+                gw += g[c] * rgb[c];
+            }
+        }
+        if (bits & kAliveA) {
+            float back = gbg * tl;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) back += g[c] * (ctot[c] - (pre[c] + v[c]));
+            const float ga = gw * T - back / ((1.f - ab.x) + 1e-10f);
+            gl[kSig] = fminf(e, 1e10f) * powf(1.f + e, -P.interval - 1.f) * P.interval * ga;
+        }
+        if (bits & (kAliveA | kAliveW)) {
+#pragma unroll
+            for (int k = 0; k <= kSig / 4; ++k)
+                row[k] = make_float4(gl[4 * k], gl[4 * k + 1], gl[4 * k + 2], gl[4 * k + 3]);
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) pre[c] += __shfl(v[c], 63, 64);
+    }
+}
+
 // Loss = sum of the per-ray terms (one workgroup, a fixed order): 16-byte loads
 // (the per-ray array is 256-byte aligned), wavefront shuffles, then the 16
 // wavefront sums in order.
@@ -525,7 +697,8 @@ namespace pvdb {
 int launch_ray_count(const StepParams& P, const TopoView& t, const RayBuffers& R,
                      const PackArgs& pk, hipStream_t s) {
     const int ray_blocks = (int)pvdb_blocks(P.n_rays, 4);
-    const int grid = ray_blocks + (int)pvdb_blocks(kPackItems, 256);
+    // no pack workgroups without an MLP (pk.prm == nullptr)
+    const int grid = ray_blocks + (pk.prm ? (int)pvdb_blocks(kPackItems, 256) : 0);
     hipLaunchKernelGGL(k_ray_count, dim3(grid), dim3(256), 0, s, P, t, R, pk, ray_blocks);
     PVDB_LAUNCHED("k_ray_count");
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, R, P.n_rays);
@@ -545,6 +718,20 @@ int launch_ray_composite(const StepParams& P, const RayBuffers& R, const Records
                          hipStream_t s) {
     hipLaunchKernelGGL(k_ray_composite, dim3(pvdb_blocks(P.n_rays, 4)), dim3(256), 0, s, P, R, rec);
     PVDB_LAUNCHED("k_ray_composite");
+    hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(1024), 0, s, R, P.n_rays);
+    PVDB_LAUNCHED("k_loss_reduce");
+    return PVDB_OK;
+}
+
+int launch_direct_composite(int nc, const StepParams& P, const TopoView& t, const float* color,
+                            const RayBuffers& R, const Records& rec, hipStream_t s) {
+    PVDB_CHECK_ARG(nc == 3 || nc == 12, "direct colour channels must be 3 or 12");
+    const dim3 grid(pvdb_blocks(P.n_rays, 4));
+    if (nc == 3)
+        hipLaunchKernelGGL(k_direct_composite<3>, grid, dim3(256), 0, s, P, t, color, R, rec);
+    else
+        hipLaunchKernelGGL(k_direct_composite<12>, grid, dim3(256), 0, s, P, t, color, R, rec);
+    PVDB_LAUNCHED("k_direct_composite");
     hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(1024), 0, s, R, P.n_rays);
     PVDB_LAUNCHED("k_loss_reduce");
     return PVDB_OK;

@@ -25,7 +25,9 @@ namespace {
 // Gradients into the grids without binning: one thread per (record, corner), with the
 // "leaf must exist" rule (densityvdb.cu:16-27, colorvdb.cu:28-37); the density
 // term keeps the reference's ((g*a)*b)*c order.  Touched leaves are flagged
-// for the sparse Adam pass.
+// for the sparse Adam pass.  NC colour channels per slot (12 features, or a
+// direct-colour grid's 3).
+template <int NC>
 __global__ void __launch_bounds__(256) k_scatter_direct(TopoView topo, RayBuffers R, Records rec,
                                                  int64_t capacity, float* __restrict__ grad_density,
                                                  float* __restrict__ grad_color,
@@ -48,19 +50,13 @@ __global__ void __launch_bounds__(256) k_scatter_direct(TopoView topo, RayBuffer
         const float d = dz ? cell.w : 1.f - cell.w;
         const int64_t slot = (int64_t)leaf * 512 + voxel_offset(i, j, k);
         leaf_touched[leaf] = 1;
-        const float4* g = rec.grad + 4 * q;
-        if (bits & kAliveA) atomicAdd(grad_density + slot, ((g[3].x * a) * b) * d);
+        const float* g = reinterpret_cast<const float*>(rec.grad + 4 * q);
+        if (bits & kAliveA) atomicAdd(grad_density + slot, ((g[kSigmaAt<NC>] * a) * b) * d);
         if (bits & kAliveW) {
             const float sc = (a * b) * d;
-            float* dst = grad_color + slot * 12;
+            float* dst = grad_color + slot * NC;
 #pragma unroll
-            for (int part = 0; part < 3; ++part) {
-                const float4 v = g[part];
-                atomicAdd(dst + 4 * part + 0, v.x * sc);
-                atomicAdd(dst + 4 * part + 1, v.y * sc);
-                atomicAdd(dst + 4 * part + 2, v.z * sc);
-                atomicAdd(dst + 4 * part + 3, v.w * sc);
-            }
+            for (int c = 0; c < NC; ++c) atomicAdd(dst + c, g[c] * sc);
         }
     }
 }
@@ -76,13 +72,18 @@ __global__ void __launch_bounds__(256) k_scatter_direct(TopoView topo, RayBuffer
 // fourteen: LDS float atomics run at well under one lane per clock.  Every
 // record load is issued before any is used, and a chunk's records are loaded
 // during the previous chunk's flush.
-// kAggRecords (32 or 64, a launch choice) records per chunk.
+// kAggRecords (32 or 64, a launch choice) records per chunk; NC colour channels
+// per slot (12 features, or a direct-colour grid's 3).
 constexpr uint32_t kEmptyKey = 0xffffffffu;
+// component k of a float4 (k a compile-time constant after unrolling)
+__device__ __forceinline__ float lane4(const float4& v, int k) {
+    return k == 0 ? v.x : k == 1 ? v.y : k == 2 ? v.z : v.w;
+}
 constexpr uint32_t kEnd = 0xffffu;  // end of a pair list
 // The launch's first `red.blocks` workgroups sum the MLP's dW partials
 // instead (dw_reduce_block, independent of the scatter): they overlap the
 // scatter's atomics rather than following them in a launch of their own.
-template <int kAggRecords>
+template <int kAggRecords, int NC>
 __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R, Records rec,
                                                      int64_t capacity,
                                                      float* __restrict__ grad_density,
@@ -94,13 +95,18 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
     constexpr int kAggLog2 = kAggRecords == 64 ? 9 : 8;  // kAggSlots == 1 << kAggLog2
     constexpr int kAggPairs = kAggPairsAll / 256;  // pairs per thread
     static_assert(kAggSlots == 1 << kAggLog2 && kAggPairs >= 1, "chunk size");
+    constexpr int kRow = NC + 1;                   // a pair's contributions: colour, density
+    constexpr int kG = NC == 12 ? 4 : 1;           // float4s of a record's gradient row read
+    constexpr int kSig = kSigmaAt<NC>;
+    constexpr int kLanes = NC == 12 ? 16 : 4;      // flush lanes per entry
+    constexpr int kTouchLane = NC == 12 ? 13 : NC; // the lane that flags the leaf
     __shared__ uint32_t key[kAggSlots];
     __shared__ uint32_t head[kAggSlots];          // first pair of the entry's list
     __shared__ uint16_t next[kAggPairsAll];       // next pair of the same entry
-    __shared__ __align__(16) float pv[kAggPairsAll * 13];  // each pair's 13 contributions
+    __shared__ __align__(16) float pv[kAggPairsAll * kRow];  // each pair's contributions
     __shared__ uint16_t listed[kAggSlots];        // the entries in use
     __shared__ int nlisted[2];
-    static_assert(kAggPairsAll * 13 >= 4 * kReduceGrp * kReduceVec, "reduction scratch");
+    static_assert(kAggPairsAll * kRow >= 4 * kReduceGrp * kReduceVec, "reduction scratch");
     if ((int)blockIdx.x < red.blocks) {  // uniform over the workgroup
         dw_reduce_block(blockIdx.x, red.part, red.nparts, red.dw, red.adam,
                         reinterpret_cast<float4*>(pv));
@@ -119,7 +125,7 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
     // this thread's pairs of a chunk: the records are loaded one chunk ahead
     // (issued before the previous chunk's flush)
     uint32_t bits[kAggPairs];
-    float4 pt[kAggPairs], gr[kAggPairs][4];
+    float4 pt[kAggPairs], gr[kAggPairs][kG];
     auto load = [&](int64_t chunk) {
 #pragma unroll
         for (int it = 0; it < kAggPairs; ++it) {
@@ -129,7 +135,7 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
                 bits[it] = __float_as_uint(rec.ab[q].y);
                 pt[it] = rec.xyze[q];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) gr[it][k] = rec.grad[4 * q + k];
+                for (int k = 0; k < kG; ++k) gr[it][k] = rec.grad[4 * q + k];
             }
         }
     };
@@ -164,18 +170,13 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
             if (leaf[it] < 0) continue;
             const int pair = tid + 256 * it;
             const float a = wgt[it][0], bb = wgt[it][1], d = wgt[it][2];
-            float* v = pv + pair * 13;
+            float* v = pv + pair * kRow;
             const float sc = (a * bb) * d;
             const bool w_ = (bits[it] & kAliveW) != 0u;
 #pragma unroll
-            for (int part = 0; part < 3; ++part) {
-                const float4 g = gr[it][part];
-                v[4 * part + 0] = w_ ? g.x * sc : 0.f;
-                v[4 * part + 1] = w_ ? g.y * sc : 0.f;
-                v[4 * part + 2] = w_ ? g.z * sc : 0.f;
-                v[4 * part + 3] = w_ ? g.w * sc : 0.f;
-            }
-            v[12] = (bits[it] & kAliveA) ? ((gr[it][3].x * a) * bb) * d : 0.f;
+            for (int c = 0; c < NC; ++c) v[c] = w_ ? lane4(gr[it][c >> 2], c & 3) * sc : 0.f;
+            v[NC] = (bits[it] & kAliveA) ? ((lane4(gr[it][kSig >> 2], kSig & 3) * a) * bb) * d
+                                         : 0.f;
             const uint32_t s = (uint32_t)leaf[it] * 512u + off[it];
             uint32_t h = (s * 2654435761u) >> (32 - kAggLog2);
             while (true) {
@@ -192,28 +193,25 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
         __syncthreads();
         load(chunk + nwg);  // the next chunk's records, in flight during the flush
         // ---- flush: walk each listed entry, add its sums, clear it; reset the
-        // other list counter.  16 lanes per entry, so one atomic instruction covers
-        // four slots' contiguous 12-float colour rows (fewer cache lines than
-        // narrower groups, measured) ----
+        // other list counter.  16 lanes per entry for 12 channels, so one atomic
+        // instruction covers four slots' contiguous 12-float colour rows (fewer
+        // cache lines than narrower groups, measured); 4 lanes for 3 ----
         const int n = nlisted[b];
         if (tid == 0) nlisted[b ^ 1] = 0;
-        const int c = tid & 15;
-        for (int e = tid >> 4; e < n; e += 16) {
+        const int c = tid & (kLanes - 1);
+        for (int e = tid / kLanes; e < n; e += 256 / kLanes) {
             const int h = listed[e];
             const uint32_t s = key[h];
             float sum = 0.f;
-            for (uint32_t p = head[h]; p != kEnd; p = next[p])  // same walk in all 16 lanes
-                if (c < 13) sum += pv[p * 13 + c];
-            if (c < 13) {
-                if (sum != 0.f) {  // adding zero changes nothing
-                    if (c < 12)
-                        atomicAdd(grad_color + (int64_t)s * 12 + c, sum);
-                    else
-                        atomicAdd(grad_density + s, sum);
-                }
-            } else if (c == 13) {
-                leaf_touched[s >> 9] = 1;
+            for (uint32_t p = head[h]; p != kEnd; p = next[p])  // same walk in the entry's lanes
+                if (c < kRow) sum += pv[p * kRow + c];
+            if (c < kRow && sum != 0.f) {  // adding zero changes nothing
+                if (c < NC)
+                    atomicAdd(grad_color + (int64_t)s * NC + c, sum);
+                else
+                    atomicAdd(grad_density + s, sum);
             }
+            if (c == kTouchLane) leaf_touched[s >> 9] = 1;
         }
         __syncthreads();  // every walk is done: clear the entries
         for (int e = tid; e < n; e += 256) {
@@ -232,9 +230,10 @@ namespace pvdb {
 int launch_dw_reduce(const float* part, int nparts, float* dw, const MlpAdam& adam,
                      hipStream_t s);
 
-int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, int64_t capacity,
-                   float* grad_density, float* grad_color, uint8_t* leaf_touched, int grid_req,
-                   int chunk_records, const ReduceArgs& red_in, hipStream_t s) {
+template <int NC>
+int launch_scatter_nc(const TopoView& t, const RayBuffers& R, const Records& rec, int64_t capacity,
+                      float* grad_density, float* grad_color, uint8_t* leaf_touched, int grid_req,
+                      int chunk_records, const ReduceArgs& red_in, hipStream_t s) {
     if (t.nleaf == 0)
         return red_in.blocks ? launch_dw_reduce(red_in.part, red_in.nparts, red_in.dw, red_in.adam, s)
                              : PVDB_OK;
@@ -249,21 +248,33 @@ int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, i
         const unsigned grid = (unsigned)(chunks < want ? (chunks < 1 ? 1 : chunks) : want) +
                               (unsigned)red.blocks;
         if (chunk_records == 64)
-            hipLaunchKernelGGL(k_scatter_agg<64>, dim3(grid), dim3(256), 0, s, t, R, rec, capacity,
-                               grad_density, grad_color, leaf_touched, red);
+            hipLaunchKernelGGL((k_scatter_agg<64, NC>), dim3(grid), dim3(256), 0, s, t, R, rec,
+                               capacity, grad_density, grad_color, leaf_touched, red);
         else
-            hipLaunchKernelGGL(k_scatter_agg<32>, dim3(grid), dim3(256), 0, s, t, R, rec, capacity,
-                               grad_density, grad_color, leaf_touched, red);
+            hipLaunchKernelGGL((k_scatter_agg<32, NC>), dim3(grid), dim3(256), 0, s, t, R, rec,
+                               capacity, grad_density, grad_color, leaf_touched, red);
         PVDB_LAUNCHED("k_scatter_agg");
         return PVDB_OK;
     }
     // enough threads for a few (record, corner) pairs each at the largest sizes
     const int64_t want = (capacity * 8 + 255) / 256;
     const unsigned grid = (unsigned)(want < 16384 ? (want < 1 ? 1 : want) : 16384);
-    hipLaunchKernelGGL(k_scatter_direct, dim3(grid), dim3(256), 0, s, t, R, rec, capacity,
+    hipLaunchKernelGGL(k_scatter_direct<NC>, dim3(grid), dim3(256), 0, s, t, R, rec, capacity,
                        grad_density, grad_color, leaf_touched);
     PVDB_LAUNCHED("k_scatter_direct");
     return red.blocks ? launch_dw_reduce(red.part, red.nparts, red.dw, red.adam, s) : PVDB_OK;
+}
+
+// nc: colour channels per slot, 12 (the MLP's features) or 3 (a direct-colour grid)
+int launch_scatter(int nc, const TopoView& t, const RayBuffers& R, const Records& rec,
+                   int64_t capacity, float* grad_density, float* grad_color,
+                   uint8_t* leaf_touched, int grid_req, int chunk_records, const ReduceArgs& red,
+                   hipStream_t s) {
+    PVDB_CHECK_ARG(nc == 3 || nc == 12, "colour channels must be 3 or 12");
+    return nc == 12 ? launch_scatter_nc<12>(t, R, rec, capacity, grad_density, grad_color,
+                                            leaf_touched, grid_req, chunk_records, red, s)
+                    : launch_scatter_nc<3>(t, R, rec, capacity, grad_density, grad_color,
+                                           leaf_touched, grid_req, chunk_records, red, s);
 }
 
 }  // namespace pvdb

@@ -21,14 +21,17 @@ int launch_ray_march(const StepParams& P, const TopoView& t, const float* densit
                      const RayBuffers& R, const Records& rec, hipStream_t s);
 int launch_ray_composite(const StepParams& P, const RayBuffers& R, const Records& rec,
                          hipStream_t s);
+int launch_direct_composite(int nc, const StepParams& P, const TopoView& t, const float* color,
+                            const RayBuffers& R, const Records& rec, hipStream_t s);
 int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const TopoView& t,
                const void* color, const MlpPacked& W, const RayBuffers& R, const Records& rec,
                float* dw_partial, hipStream_t s);
 int launch_dw_reduce(const float* part, int nparts, float* dw, const MlpAdam& adam,
                      hipStream_t s);
-int launch_scatter(const TopoView& t, const RayBuffers& R, const Records& rec, int64_t capacity,
-                   float* grad_density, float* grad_color, uint8_t* leaf_touched, int grid_req,
-                   int chunk_records, const ReduceArgs& red, hipStream_t s);
+int launch_scatter(int nc, const TopoView& t, const RayBuffers& R, const Records& rec,
+                   int64_t capacity, float* grad_density, float* grad_color,
+                   uint8_t* leaf_touched, int grid_req, int chunk_records, const ReduceArgs& red,
+                   hipStream_t s);
 }  // namespace pvdb
 
 using namespace pvdb;
@@ -47,7 +50,7 @@ constexpr int64_t kMaxCapacity = 0x7fffffffll;
 inline bool aligned(const void* p, uintptr_t a) { return (reinterpret_cast<uintptr_t>(p) % a) == 0; }
 
 struct Layout {
-    size_t count, offset, total, t_last, ray_out, ray_grad, ray_ctot, ray_loss, loss;
+    size_t count, offset, total, t_last, ray_out, ray_grad, ray_ctot, ray_loss, ray_rgb, loss;
     size_t xyze, ab, rgb, adj, feat, relu, grad;
     size_t touched, dw_part, w0t, w1t, w2t, w2, w1, w0;
     size_t bytes;
@@ -71,6 +74,7 @@ Layout layout_for(int64_t n_rays, int64_t capacity, int64_t nleaf) {
     L.ray_grad = take(16 * n_rays);
     L.ray_ctot = take(16 * n_rays);
     L.ray_loss = take(4 * n_rays);
+    L.ray_rgb = take(4 * 13 * n_rays);
     L.loss = take(4);
     L.xyze = take(16 * capacity);
     L.ab = take(8 * capacity);
@@ -105,15 +109,58 @@ __device__ __forceinline__ void adam1(float& p, float g, float& m, float& v, flo
     p = p - (sz * nm) / (eps + sqrtf(nv));
 }
 
-// Adam with skip-zero-grad (stepmode 1, densityvdb.cu:272-349, colorvdb.cu:300-351) on
-// the leaves the backward touched, then zero their gradients and flags.  One
-// workgroup per touched leaf; untouched leaves have all-zero gradients, which
-// stepmode 1 skips, so the result equals the reference's all-slot pass.
+// A row of NC fp32 values of slot s (16-byte vectors when NC is a multiple of 4).
+template <int NC>
+__device__ __forceinline__ void ld_row(const float* base, int64_t s, float (&v)[NC]) {
+    if constexpr (NC % 4 == 0) {
+        const float4* p = reinterpret_cast<const float4*>(base) + s * (NC / 4);
+#pragma unroll
+        for (int k = 0; k < NC / 4; ++k) {
+            const float4 q = p[k];
+            v[4 * k] = q.x;
+            v[4 * k + 1] = q.y;
+            v[4 * k + 2] = q.z;
+            v[4 * k + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) v[c] = base[s * NC + c];
+    }
+}
+
+template <int NC>
+__device__ __forceinline__ void st_row(float* base, int64_t s, const float (&v)[NC]) {
+    if constexpr (NC % 4 == 0) {
+        float4* p = reinterpret_cast<float4*>(base) + s * (NC / 4);
+#pragma unroll
+        for (int k = 0; k < NC / 4; ++k)
+            p[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
+    } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) base[s * NC + c] = v[c];
+    }
+}
+
+// VDB Adam of the step (densityvdb.cu:185-349, colorvdb.cu:179-351), then the
+// gradients and touched flags re-zeroed.  One workgroup per leaf.
+//   density stepmode 1 (skip-zero-grad) updates a voxel whose gradient is
+//   nonzero; 0 updates every active voxel; 2 every active voxel with its step
+//   scaled by the per-voxel rate, (rate * step) * m / (eps + sqrt(v)).
+//   colour stepmode 1 skips a Vec3 group only when all three components are
+//   zero; 0 updates every active voxel.
+// With both grids in stepmode 1 only the leaves the backward touched are
+// visited: the others have all-zero gradients, which stepmode 1 skips, so the
+// result equals the reference's all-slot pass.  NC colour channels per slot.
 // With a bf16 colour payload and an fp32 master copy (cmaster != nullptr) the
 // update runs on the master and the payload receives its round-to-nearest
 // bf16 value, so updates below half a bf16 step accumulate instead of being
 // lost; without one the bf16 payload is updated in place.
-template <bool BF16COL>
+struct GridAdamModes {
+    int density, color;
+    const float* rate;  // [slots] per-voxel rate of density stepmode 2
+};
+
+template <int NC, bool BF16COL>
 __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __restrict__ touched,
                                                      float* __restrict__ den,
                                                      float* __restrict__ gden,
@@ -122,99 +169,82 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
                                                      float* __restrict__ cmaster,
                                                      float* __restrict__ gcol,
                                                      float* __restrict__ mcol,
-                                                     float* __restrict__ vcol, float sz_d_arg,
-                                                     float sz_c_arg,
+                                                     float* __restrict__ vcol, GridAdamModes md_,
+                                                     float sz_d_arg, float sz_c_arg,
                                                      const float* __restrict__ dev_sz, float eps,
                                                      float b0, float b1) {
+    static_assert(!BF16COL || NC == 12, "bf16 colour payloads hold 12 channels");
     const float sz_d = dev_sz ? dev_sz[0] : sz_d_arg;
     const float sz_c = dev_sz ? dev_sz[1] : sz_c_arg;
+    const bool every = md_.density != 1 || md_.color != 1;
     for (int64_t leaf = blockIdx.x; leaf < t.nleaf; leaf += gridDim.x) {
-        if (!touched[leaf]) continue;  // uniform over the workgroup
+        if (!every && !touched[leaf]) continue;  // uniform over the workgroup
         for (int k = threadIdx.x; k < 512; k += blockDim.x) {
             const int64_t s = leaf * 512 + k;
             const bool act = (t.masks[leaf * 8 + (k >> 6)] >> (k & 63)) & 1ull;
-            float4* g4 = reinterpret_cast<float4*>(gcol) + s * 3;
             if (act) {
                 // every operand of an active voxel is loaded at once (one round
-                // trip; a voxel whose gradients are all zero writes nothing back)
-                float4* m4 = reinterpret_cast<float4*>(mcol) + s * 3;
-                float4* v4 = reinterpret_cast<float4*>(vcol) + s * 3;
+                // trip; a voxel whose gradients are all zero writes nothing back
+                // in stepmode 1)
                 float gd = gden[s], pd = den[s], md = mden[s], vd = vden[s];
-                float gg[12], mm[12], vv[12], pp[12];
+                float gg[NC], mm[NC], vv[NC], pp[NC];
+                ld_row<NC>(gcol, s, gg);
+                ld_row<NC>(mcol, s, mm);
+                ld_row<NC>(vcol, s, vv);
+                if constexpr (BF16COL) {
+                    if (!cmaster) {
+                        const uint2* c2 = reinterpret_cast<const uint2*>(col) + s * 3;
 #pragma unroll
-                for (int k4 = 0; k4 < 3; ++k4) {
-                    const float4 q = g4[k4], qm = m4[k4], qv = v4[k4];
-                    gg[4 * k4] = q.x; gg[4 * k4 + 1] = q.y; gg[4 * k4 + 2] = q.z; gg[4 * k4 + 3] = q.w;
-                    mm[4 * k4] = qm.x; mm[4 * k4 + 1] = qm.y; mm[4 * k4 + 2] = qm.z; mm[4 * k4 + 3] = qm.w;
-                    vv[4 * k4] = qv.x; vv[4 * k4 + 1] = qv.y; vv[4 * k4 + 2] = qv.z; vv[4 * k4 + 3] = qv.w;
-                }
-                if (BF16COL && !cmaster) {
-                    const uint2* c2 = reinterpret_cast<const uint2*>(col) + s * 3;
-#pragma unroll
-                    for (int k4 = 0; k4 < 3; ++k4) {
-                        const uint2 q = c2[k4];
-                        pp[4 * k4] = bf_bits2f((uint16_t)(q.x & 0xffffu));
-                        pp[4 * k4 + 1] = bf_bits2f((uint16_t)(q.x >> 16));
-                        pp[4 * k4 + 2] = bf_bits2f((uint16_t)(q.y & 0xffffu));
-                        pp[4 * k4 + 3] = bf_bits2f((uint16_t)(q.y >> 16));
+                        for (int k4 = 0; k4 < 3; ++k4) {
+                            const uint2 q = c2[k4];
+                            pp[4 * k4] = bf_bits2f((uint16_t)(q.x & 0xffffu));
+                            pp[4 * k4 + 1] = bf_bits2f((uint16_t)(q.x >> 16));
+                            pp[4 * k4 + 2] = bf_bits2f((uint16_t)(q.y & 0xffffu));
+                            pp[4 * k4 + 3] = bf_bits2f((uint16_t)(q.y >> 16));
+                        }
+                    } else {
+                        ld_row<NC>(cmaster, s, pp);
                     }
                 } else {
-                    const float4* c4 =
-                        reinterpret_cast<const float4*>(BF16COL ? cmaster : col) + s * 3;
-#pragma unroll
-                    for (int k4 = 0; k4 < 3; ++k4) {
-                        const float4 q = c4[k4];
-                        pp[4 * k4] = q.x; pp[4 * k4 + 1] = q.y; pp[4 * k4 + 2] = q.z; pp[4 * k4 + 3] = q.w;
-                    }
+                    ld_row<NC>(reinterpret_cast<const float*>(col), s, pp);
                 }
-                if (gd != 0.f) {
-                    adam1(pd, gd, md, vd, sz_d, eps, b0, b1);
+                if (md_.density != 1 || gd != 0.f) {
+                    const float sz = md_.density == 2 ? md_.rate[s] * sz_d : sz_d;
+                    adam1(pd, gd, md, vd, sz, eps, b0, b1);
                     den[s] = pd;
                     mden[s] = md;
                     vden[s] = vd;
                 }
                 bool any = false;
-                // stepmode 1 skips a Vec3 group only when all three components are zero
 #pragma unroll
-                for (int grp = 0; grp < 4; ++grp) {
+                for (int grp = 0; grp < NC / 3; ++grp) {
                     const int c0 = 3 * grp;
-                    if (gg[c0] == 0.f && gg[c0 + 1] == 0.f && gg[c0 + 2] == 0.f) continue;
+                    if (md_.color == 1 && gg[c0] == 0.f && gg[c0 + 1] == 0.f && gg[c0 + 2] == 0.f)
+                        continue;
                     any = true;
 #pragma unroll
                     for (int c = c0; c < c0 + 3; ++c) adam1(pp[c], gg[c], mm[c], vv[c], sz_c, eps, b0, b1);
                 }
                 if (any) {
-#pragma unroll
-                    for (int k4 = 0; k4 < 3; ++k4) {
-                        m4[k4] = make_float4(mm[4 * k4], mm[4 * k4 + 1], mm[4 * k4 + 2], mm[4 * k4 + 3]);
-                        v4[k4] = make_float4(vv[4 * k4], vv[4 * k4 + 1], vv[4 * k4 + 2], vv[4 * k4 + 3]);
-                    }
-                    if (BF16COL) {
+                    st_row<NC>(mcol, s, mm);
+                    st_row<NC>(vcol, s, vv);
+                    if constexpr (BF16COL) {
                         uint2* c2 = reinterpret_cast<uint2*>(col) + s * 3;
 #pragma unroll
                         for (int k4 = 0; k4 < 3; ++k4)
                             c2[k4] = make_uint2((uint32_t)f2bf_bits(pp[4 * k4]) | ((uint32_t)f2bf_bits(pp[4 * k4 + 1]) << 16),
                                                 (uint32_t)f2bf_bits(pp[4 * k4 + 2]) | ((uint32_t)f2bf_bits(pp[4 * k4 + 3]) << 16));
-                        if (cmaster) {
-                            float4* c4 = reinterpret_cast<float4*>(cmaster) + s * 3;
-#pragma unroll
-                            for (int k4 = 0; k4 < 3; ++k4)
-                                c4[k4] = make_float4(pp[4 * k4], pp[4 * k4 + 1], pp[4 * k4 + 2],
-                                                     pp[4 * k4 + 3]);
-                        }
+                        if (cmaster) st_row<NC>(cmaster, s, pp);
                     } else {
-                        float4* c4 = reinterpret_cast<float4*>(col) + s * 3;
-#pragma unroll
-                        for (int k4 = 0; k4 < 3; ++k4)
-                            c4[k4] = make_float4(pp[4 * k4], pp[4 * k4 + 1], pp[4 * k4 + 2], pp[4 * k4 + 3]);
+                        st_row<NC>(reinterpret_cast<float*>(col), s, pp);
                     }
                 }
             }
             gden[s] = 0.f;
-            const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-            g4[0] = z;
-            g4[1] = z;
-            g4[2] = z;
+            float zr[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) zr[c] = 0.f;
+            st_row<NC>(gcol, s, zr);
         }
         __syncthreads();
         if (threadIdx.x == 0) touched[leaf] = 0;
@@ -252,20 +282,23 @@ struct StageScope {
     }
 };
 
-// VDB Adam over the touched leaves (stepmode 1 on both grids).
+// VDB Adam of the grids (over the touched leaves when both run stepmode 1).
 int launch_grid_adam(const pvdb_step_desc* d, const TopoView& t, uint8_t* touched, hipStream_t s) {
     const int ag = (int)std::min<int64_t>(t.nleaf > 0 ? t.nleaf : 1, 4096);
     float* master = d->color_bf16 ? d->color_master : nullptr;
-    if (d->color_bf16)
-        hipLaunchKernelGGL(k_sparse_adam<true>, dim3(ag), dim3(256), 0, s, t, touched, d->density,
-                           d->grad_density, d->density_m, d->density_v, d->color, master,
-                           d->grad_color, d->color_m, d->color_v, d->stepsz_density,
-                           d->stepsz_color, d->adam_scalars, d->eps, d->beta0, d->beta1);
+    const GridAdamModes md{d->stepmode_density, d->stepmode_color, d->density_lr};
+#define PVDB_ADAM(NC, BF)                                                                        \
+    hipLaunchKernelGGL((k_sparse_adam<NC, BF>), dim3(ag), dim3(256), 0, s, t, touched,          \
+                       d->density, d->grad_density, d->density_m, d->density_v, d->color,       \
+                       master, d->grad_color, d->color_m, d->color_v, md, d->stepsz_density,    \
+                       d->stepsz_color, d->adam_scalars, d->eps, d->beta0, d->beta1)
+    if (d->k0_dim == 3)
+        PVDB_ADAM(3, false);
+    else if (d->color_bf16)
+        PVDB_ADAM(12, true);
     else
-        hipLaunchKernelGGL(k_sparse_adam<false>, dim3(ag), dim3(256), 0, s, t, touched,
-                           d->density, d->grad_density, d->density_m, d->density_v, d->color,
-                           master, d->grad_color, d->color_m, d->color_v, d->stepsz_density,
-                           d->stepsz_color, d->adam_scalars, d->eps, d->beta0, d->beta1);
+        PVDB_ADAM(12, false);
+#undef PVDB_ADAM
     PVDB_LAUNCHED("k_sparse_adam");
     return PVDB_OK;
 }
@@ -344,11 +377,19 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
                    "record capacity must be in [1, 2^31 - 2]");
     PVDB_CHECK_ARG(d->rays_o && d->rays_d && d->viewdirs && d->target, "null ray arrays");
     PVDB_CHECK_ARG(d->density && d->grad_density && d->color && d->grad_color, "null grid arrays");
-    PVDB_CHECK_ARG(d->mlp && d->mlp_grad, "null MLP arrays");
+    PVDB_CHECK_ARG(d->k0_dim == 0 || d->k0_dim == 3 || d->k0_dim == 12, "k0_dim must be 0, 3 or 12");
+    const bool direct = d->k0_dim != 0;
+    PVDB_CHECK_ARG(direct || (d->mlp && d->mlp_grad), "null MLP arrays");
+    PVDB_CHECK_ARG(!direct || !d->color_bf16, "a model without the MLP takes an fp32 colour grid");
     PVDB_CHECK_ARG(d->do_adam >= 0 && d->do_adam <= 2, "do_adam must be 0, 1 or 2");
     PVDB_CHECK_ARG(!d->do_adam || (d->density_m && d->density_v && d->color_m && d->color_v &&
-                                   d->mlp_m && d->mlp_v),
+                                   (direct || (d->mlp_m && d->mlp_v))),
                    "Adam needs moment arrays");
+    PVDB_CHECK_ARG(d->stepmode_density >= 0 && d->stepmode_density <= 2 &&
+                       d->stepmode_color >= 0 && d->stepmode_color <= 1,
+                   "stepmode_density must be 0, 1 or 2 and stepmode_color 0 or 1");
+    PVDB_CHECK_ARG(d->stepmode_density != 2 || d->density_lr,
+                   "density stepmode 2 needs density_lr");
     PVDB_CHECK_ARG(d->stepdist > 0.f, "stepdist must be positive");
     PVDB_CHECK_ARG(d->launch[0] >= 0 && d->launch[0] <= 4096 && d->launch[1] >= 0 &&
                        d->launch[1] <= 65536,
@@ -397,7 +438,8 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     P.interval = d->interval;
     P.thres = d->fast_color_thres;
     P.bg = d->bg;
-    P.loss_scale = d->weight_main / (3.f * (float)d->n_rays);
+    const int nc = direct ? d->k0_dim : 3;  // rendered channels
+    P.loss_scale = d->weight_main / ((float)nc * (float)d->n_rays);
     P.ent_scale = d->weight_entropy_last / (float)d->n_rays;
     P.rgbper_scale = d->weight_rgbper / (float)d->n_rays;
     P.capacity = d->capacity;
@@ -411,6 +453,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     R.ray_grad = reinterpret_cast<float4*>(ws + L.ray_grad);
     R.ray_ctot = reinterpret_cast<float4*>(ws + L.ray_ctot);
     R.ray_loss = reinterpret_cast<float*>(ws + L.ray_loss);
+    R.ray_rgb = reinterpret_cast<float*>(ws + L.ray_rgb);
     R.loss = d->loss ? d->loss : reinterpret_cast<float*>(ws + L.loss);
     Records rec;
     rec.xyze = reinterpret_cast<float4*>(ws + L.xyze);
@@ -435,13 +478,13 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     W.w2 = w2;
     W.w1 = w1;
     W.w0 = w0;
-    W.b0 = d->mlp + kB0;
-    W.b1 = d->mlp + kB1;
-    W.b2 = d->mlp + kB2;
+    W.b0 = direct ? nullptr : d->mlp + kB0;
+    W.b1 = direct ? nullptr : d->mlp + kB1;
+    W.b2 = direct ? nullptr : d->mlp + kB2;
 
     int rc;
     if (d->do_adam == 2) {  // the optimizer alone, on the gradients already accumulated
-        {
+        if (!direct) {
             StageScope sc(7, s);
             const MlpAdam adam{d->mlp,           d->mlp_m, d->mlp_v, d->adam_scalars,
                                d->step_size_mlp, d->eps,   d->beta0, d->beta1};
@@ -454,12 +497,42 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
         // the counting march also packs the bf16 MLP operands and zeroes the
         // MLP gradient (accumulated by k_dw_reduce later in the step)
         StageScope sc(1, s);
-        const PackArgs pk{d->mlp, w0t, w1t, w2t, w2, w1, w0, d->mlp_grad};
+        PackArgs pk{d->mlp, w0t, w1t, w2t, w2, w1, w0, d->mlp_grad};
+        if (direct) pk.prm = nullptr;
         if ((rc = launch_ray_count(P, t, R, pk, s))) return rc;
     }
     {
         StageScope sc(2, s);
         if ((rc = launch_ray_march(P, t, d->density, R, rec, s))) return rc;
+    }
+    if (direct) {
+        // no MLP: the composite gathers k0, and its gradient rows go straight
+        // to the scatter
+        {
+            StageScope sc(4, s);
+            if ((rc = launch_direct_composite(d->k0_dim, P, t, reinterpret_cast<const float*>(d->color),
+                                              R, rec, s)))
+                return rc;
+        }
+        {
+            StageScope sc(6, s);
+            const MlpAdam none{nullptr, nullptr, nullptr, nullptr, 0.f, 0.f, 0.f, 0.f};
+            const ReduceArgs red{nullptr, 0, nullptr, none, 0};
+            if ((rc = launch_scatter(d->k0_dim == 3 ? 3 : 12, t, R, rec, d->capacity,
+                                     d->grad_density, d->grad_color, touched,
+                                     d->launch[1] > 0 ? d->launch[1] : kScatterGrid,
+                                     d->launch[2] > 0 ? d->launch[2] : kScatterChunk, red, s)))
+                return rc;
+        }
+        if (d->ray_out) {
+            PVDB_HIP(hipMemcpyAsync(d->ray_out, R.ray_rgb, 4 * (size_t)(nc + 1) * d->n_rays,
+                                    hipMemcpyDeviceToDevice, s));
+        }
+        if (d->do_adam) {
+            StageScope sc(8, s);
+            if ((rc = launch_grid_adam(d, t, touched, s))) return rc;
+        }
+        return PVDB_OK;
     }
     // The sample-side kernels are persistent over the records; their grid is
     // fixed (the record count stays on the device).  The backward holds one
@@ -499,8 +572,8 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
             adam = MlpAdam{d->mlp,           d->mlp_m, d->mlp_v, d->adam_scalars,
                            d->step_size_mlp, d->eps,   d->beta0, d->beta1};
         const ReduceArgs red{dw_part, grid_bwd, d->mlp_grad, adam, kReduceBlocks};
-        if ((rc = launch_scatter(t, R, rec, d->capacity, d->grad_density, d->grad_color, touched,
-                                 d->launch[1] > 0 ? d->launch[1] : kScatterGrid,
+        if ((rc = launch_scatter(12, t, R, rec, d->capacity, d->grad_density, d->grad_color,
+                                 touched, d->launch[1] > 0 ? d->launch[1] : kScatterGrid,
                                  d->launch[2] > 0 ? d->launch[2] : kScatterChunk, red, s)))
             return rc;
     }

@@ -63,4 +63,42 @@ __device__ __forceinline__ float gather_density(const TopoView& t, const float* 
     return acc;
 }
 
+// NC colour channels at an index-space point, colour rule of
+// colorvdb.cu:16-26,96-103: the corner weight (a*b)*c first, then
+// out[c] += val[c] * weight, corners in the order above (a missing leaf adds 0).
+template <int NC>
+__device__ __forceinline__ void gather_color(const TopoView& t, const float* __restrict__ vals,
+                                             const Cell& cell, LeafCache& cache,
+                                             float (&out)[NC]) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) out[c] = 0.f;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int dx = (kCx >> q) & 1, dy = (kCy >> q) & 1, dz = (kCz >> q) & 1;
+        const float a = dx ? cell.u : 1.f - cell.u;
+        const float b = dy ? cell.v : 1.f - cell.v;
+        const float d = dz ? cell.w : 1.f - cell.w;
+        const float sc = (a * b) * d;
+        const int64_t s = cache.slot(t, cell.i0 + dx, cell.j0 + dy, cell.k0 + dz);
+        float v[NC];
+        if constexpr (NC % 4 == 0) {  // 16-byte rows
+            const float4* src = reinterpret_cast<const float4*>(vals) + (s >= 0 ? s : 0) * (NC / 4);
+#pragma unroll
+            for (int k = 0; k < NC / 4; ++k) {
+                const float4 r = s >= 0 ? src[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+                v[4 * k] = r.x;
+                v[4 * k + 1] = r.y;
+                v[4 * k + 2] = r.z;
+                v[4 * k + 3] = r.w;
+            }
+        } else {
+            const float* src = vals + (s >= 0 ? s : 0) * NC;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) v[c] = s >= 0 ? src[c] : 0.f;
+        }
+#pragma unroll
+        for (int c = 0; c < NC; ++c) out[c] = out[c] + v[c] * sc;
+    }
+}
+
 }  // namespace pvdb

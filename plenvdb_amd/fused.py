@@ -36,13 +36,14 @@ class StepDesc(C.Structure):
         ("mlp", C.c_void_p), ("mlp_grad", C.c_void_p), ("mlp_m", C.c_void_p), ("mlp_v", C.c_void_p),
         ("stepsz_density", C.c_float), ("stepsz_color", C.c_float), ("step_size_mlp", C.c_float),
         ("eps", C.c_float), ("beta0", C.c_float), ("beta1", C.c_float),
-        ("do_adam", C.c_int32), ("reserved", C.c_int32),
+        ("do_adam", C.c_int32), ("k0_dim", C.c_int32),
         ("workspace", C.c_void_p), ("workspace_bytes", C.c_int64), ("capacity", C.c_int64),
         ("ray_out", C.c_void_p), ("loss", C.c_void_p), ("adam_scalars", C.c_void_p),
         ("launch", C.c_int32 * 4),
         ("weight_main", C.c_float), ("weight_entropy_last", C.c_float),
         ("weight_rgbper", C.c_float), ("reserved2", C.c_int32),
         ("color_master", C.c_void_p),
+        ("density_lr", C.c_void_p), ("stepmode_density", C.c_int32), ("stepmode_color", C.c_int32),
     ]
 
 
@@ -90,23 +91,40 @@ def _f32(x) -> float:
 class FusedTrainer:
     """One training iteration per `step()` on the fused kernels.
 
-    scene: scenes.make_scene(...) output (must use the MLP).  color_bf16
-    defaults to the configuration's choice.  weight_main, weight_entropy_last
-    and weight_rgbper are the loss weights of run.py:551-574 (the fine stage of
-    configs/default.py:73-74 uses 1, 0.001 and 0.01; the defaults here are MSE only).
+    scene: scenes.make_scene(...) output.  A configuration with the MLP runs
+    the fine-stage model (12 colour features, rgbnet); one without it runs
+    rgb = sigmoid(k0) over its colour grid's channels (3, the coarse stage's
+    k0, or 12; dvgo.py:72-75,344-346).  color_bf16 defaults to the
+    configuration's choice.  weight_main, weight_entropy_last and
+    weight_rgbper are the loss weights of run.py:551-574 (the fine stage of
+    configs/default.py:73-74 uses 1, 0.001 and 0.01, the coarse stage
+    configs/default.py:52-55 uses 1, 0.01 and 0.1; the defaults here are MSE
+    only).  stepmode_density / stepmode_color are VDBAdam's stepmodes
+    (masked_adam.py:56-68): 1 is skip_zero_grad (the fine stage, the
+    default), 0 updates every active voxel, density 2 applies the per-voxel
+    rate of `set_pervoxel_lr` (the coarse stage: density 2, colour 0).
     """
 
     def __init__(self, scene: dict, topo: Topology | None = None, color_bf16: bool | None = None,
                  lr_grid=0.1, lr_mlp=1e-3, lrate_decay=20, capacity: int | None = None,
                  weight_main: float = 1.0, weight_entropy_last: float = 0.0,
-                 weight_rgbper: float = 0.0, color_master: bool = True, device="cuda"):
+                 weight_rgbper: float = 0.0, color_master: bool = True,
+                 stepmode_density: int = 1, stepmode_color: int = 1, device="cuda"):
         cfg = scene["cfg"]
-        if not cfg.use_mlp:
-            raise ValueError("the fused path runs the MLP configurations")
+        self.k0_dim = 0 if cfg.use_mlp else int(cfg.channels)
+        if self.k0_dim not in (0, 3, 12):
+            raise ValueError("a model without the MLP takes 3 or 12 colour channels")
+        if stepmode_density not in (0, 1, 2) or stepmode_color not in (0, 1):
+            raise ValueError("stepmode_density must be 0, 1 or 2 and stepmode_color 0 or 1")
+        self.stepmodes = (int(stepmode_density), int(stepmode_color))
         self.scene = scene
         dev = torch.device(device)
         self.topo = topo if topo is not None else Topology.from_mask(scene["mask"])
         self.color_bf16 = cfg.color_bf16 if color_bf16 is None else bool(color_bf16)
+        if self.k0_dim and self.color_bf16:
+            raise ValueError("a model without the MLP takes an fp32 colour grid")
+        nc = self.k0_dim or 12  # colour channels per slot
+        self.n_out = self.k0_dim or 3  # rendered channels
         ns = self.topo.nslots
         den, col = scenes.grid_values(cfg, ns)
         self.density = torch.from_numpy(den).to(dev)
@@ -115,15 +133,19 @@ class FusedTrainer:
         # the bf16 payload's fp32 master, which the colour Adam updates
         self.color_master = colt if (self.color_bf16 and color_master) else None
         self.grad_density = torch.zeros(ns, device=dev)
-        self.grad_color = torch.zeros(ns, 12, device=dev)
+        self.grad_color = torch.zeros(ns, nc, device=dev)
         self.m_density = torch.zeros(ns, device=dev)
         self.v_density = torch.zeros(ns, device=dev)
-        self.m_color = torch.zeros(ns, 12, device=dev)
-        self.v_color = torch.zeros(ns, 12, device=dev)
-        self.mlp = torch.from_numpy(pack_mlp(scene["mlp"])).to(dev)
-        self.mlp_grad = torch.zeros_like(self.mlp)
-        self.mlp_m = torch.zeros_like(self.mlp)
-        self.mlp_v = torch.zeros_like(self.mlp)
+        self.m_color = torch.zeros(ns, nc, device=dev)
+        self.v_color = torch.zeros(ns, nc, device=dev)
+        self.density_lr = None  # per-voxel rate of density stepmode 2
+        if self.k0_dim:
+            self.mlp = self.mlp_grad = self.mlp_m = self.mlp_v = None
+        else:
+            self.mlp = torch.from_numpy(pack_mlp(scene["mlp"])).to(dev)
+            self.mlp_grad = torch.zeros_like(self.mlp)
+            self.mlp_m = torch.zeros_like(self.mlp)
+            self.mlp_v = torch.zeros_like(self.mlp)
         self.rays_o = torch.from_numpy(scene["rays_o"]).to(dev).contiguous()
         self.rays_d = torch.from_numpy(scene["rays_d"]).to(dev).contiguous()
         self.viewdirs = torch.from_numpy(scene["viewdirs"]).to(dev).contiguous()
@@ -137,7 +159,7 @@ class FusedTrainer:
         self.capacity = int(capacity)
         self.ws_bytes = _workspace_bytes(self.n_rays, self.capacity, self.topo.nleaf)
         self.workspace = torch.empty(self.ws_bytes, dtype=torch.uint8, device=dev)
-        self.ray_out = torch.zeros(self.n_rays, 4, device=dev)
+        self.ray_out = torch.zeros(self.n_rays, self.n_out + 1, device=dev)
         self.loss = torch.zeros(1, device=dev)
         self.lr_grid = float(lr_grid)
         self.lr_mlp = float(lr_mlp)
@@ -185,8 +207,12 @@ class FusedTrainer:
         d.density_m, d.density_v = self.m_density.data_ptr(), self.v_density.data_ptr()
         d.color, d.grad_color = self.color.data_ptr(), self.grad_color.data_ptr()
         d.color_m, d.color_v = self.m_color.data_ptr(), self.v_color.data_ptr()
-        d.mlp, d.mlp_grad = self.mlp.data_ptr(), self.mlp_grad.data_ptr()
-        d.mlp_m, d.mlp_v = self.mlp_m.data_ptr(), self.mlp_v.data_ptr()
+        if self.mlp is not None:
+            d.mlp, d.mlp_grad = self.mlp.data_ptr(), self.mlp_grad.data_ptr()
+            d.mlp_m, d.mlp_v = self.mlp_m.data_ptr(), self.mlp_v.data_ptr()
+        d.k0_dim = self.k0_dim
+        d.stepmode_density, d.stepmode_color = self.stepmodes
+        d.density_lr = None if self.density_lr is None else self.density_lr.data_ptr()
         d.eps, d.beta0, d.beta1 = 1e-8, 0.9, 0.99
         d.workspace = self.workspace.data_ptr()
         d.workspace_bytes = self.ws_bytes
@@ -273,6 +299,25 @@ class FusedTrainer:
             self.capture_graph()
         return out
 
+    def set_pervoxel_lr(self, count) -> None:
+        """Per-voxel rate of density stepmode 2 from a dense [X, Y, Z] view-count
+        grid: count / count.max() (masked_adam.py:43-46), copied onto the slots
+        (plenvdb.h:713-719); the step then runs density stepmode 2."""
+        c = torch.as_tensor(count, device=self.density.device).float()
+        reso = [int(r) for r in self.scene["reso"]]
+        if c.numel() != reso[0] * reso[1] * reso[2]:
+            raise ValueError("set_pervoxel_lr: size does not match the resolution")
+        c = (c / c.max()).reshape(-1).contiguous()
+        self.density_lr = torch.zeros(self.topo.nslots, device=self.density.device)
+        lib.call("pvdb_copy_from_dense", self.topo.handle, lib.ptr(self.density_lr), 1, lib.ptr(c),
+                 *reso, lib.stream())
+        torch.cuda.current_stream().synchronize()
+        self.stepmodes = (2, self.stepmodes[1])
+        self.desc.density_lr = self.density_lr.data_ptr()
+        self.desc.stepmode_density = 2
+        if self.graph is not None:
+            self.capture_graph()
+
     # -- checkpoints: the plain path's file set (PlainTrainer.save_checkpoint) --
     def _rgbnet(self):
         """The MLP as DirectVoxGO's rgbnet module (its state_dict layout)."""
@@ -303,6 +348,11 @@ class FusedTrainer:
             vdbio.save_grid(prefix + name, t, val, reso)
         sc = {"lr": self.lr_grid, "eps": 1e-8, "beta0": 0.9, "beta1": 0.99, "step": self.step_count}
         torch.save({k + n: v for k in ("d", "c") for n, v in sc.items()}, prefix + "vdbopt.tar")
+        if self.mlp is None:
+            torch.save({"global_step": int(self.step_count if global_step is None else global_step),
+                        "rgbnet_state_dict": None, "optimizer_state_dict": None},
+                       prefix + "last.tar")
+            return
         net, lin = self._rgbnet()
         pm = unpack_mlp(self.mlp_m.detach().cpu().numpy())
         pv = unpack_mlp(self.mlp_v.detach().cpu().numpy())
@@ -326,10 +376,11 @@ class FusedTrainer:
         by leaf origin.  Returns the saved global step."""
         from . import vdbio
 
+        nc = self.k0_dim or 12
         topo, den, _ = vdbio.load_grid(prefix + "density.vdb", 1)
-        ctopo, col, _ = vdbio.load_grid(prefix + "color.vdb", 12)
+        ctopo, col, _ = vdbio.load_grid(prefix + "color.vdb", nc)
         if ctopo.nleaf != topo.nleaf:  # a colour leaf pruned alone: keep the density's set
-            col = vdbio.load_onto(prefix + "color.vdb", topo, 12)
+            col = vdbio.load_onto(prefix + "color.vdb", topo, nc)
         if topo.nleaf != self.topo.nleaf:
             raise ValueError("the checkpoint's leaf set differs from this trainer's topology")
         self.density.copy_(den)
@@ -343,6 +394,11 @@ class FusedTrainer:
         self.lr_grid = _f32(st["dlr"])
         self.step_count = int(st["dstep"])
         ck = torch.load(prefix + "last.tar", weights_only=True)
+        if self.mlp is None:
+            self.grad_density.zero_()
+            self.grad_color.zero_()
+            lib.call("pvdb_fused_init", self.topo.handle, C.byref(self.desc), lib.stream())
+            return int(ck["global_step"])
         net, lin = self._rgbnet()
         net.load_state_dict(ck["rgbnet_state_dict"])
         vec = lambda ts: np.concatenate([  # noqa: E731
@@ -363,7 +419,8 @@ class FusedTrainer:
     def zero_grad(self):
         self.grad_density.zero_()
         self.grad_color.zero_()
-        self.mlp_grad.zero_()
+        if self.mlp_grad is not None:
+            self.mlp_grad.zero_()
 
     def overflowed(self) -> bool:
         """Whether the last step had more records than the capacity (the step

@@ -23,7 +23,7 @@ Configurations
 from __future__ import annotations
 
 import math
-from dataclasses import dataclass, field
+from dataclasses import dataclass, field, replace
 
 import numpy as np
 
@@ -187,12 +187,17 @@ def intrinsics(H: int = 800, W: int = 800, focal: float = 1111.0):
     return np.array([[focal, 0, 0.5 * W], [0, focal, 0.5 * H], [0, 0, 1]], dtype=np.float32)
 
 
-def make_scene(name: str, n_rays: int | None = None) -> dict:
+def make_scene(name: str, n_rays: int | None = None, channels: int | None = None) -> dict:
     """Everything a training step needs, as host numpy arrays.  Grid values are
     given per slot, so the caller needs the topology's slot order: the leaves
     come from `occupancy_mask`, sorted into NanoVDB order by the topology
-    builder."""
+    builder.  `channels` replaces the colour channel count of a configuration
+    without the MLP (3 gives the coarse stage's k0 grid, dvgo.py:72-75)."""
     cfg = CONFIGS[name]
+    if channels is not None:
+        if cfg.use_mlp:
+            raise ValueError("the MLP configurations take 12 colour features")
+        cfg = replace(cfg, channels=int(channels))
     mask = occupancy_mask(cfg)
     o, d, vd, far = rays(cfg, n_rays)
     sc = {

@@ -121,6 +121,12 @@ def test_fused_step_rejects_capacity_and_misaligned_buffers():
         (dict(color=base + 8), "16-byte"),
         (dict(color=base + 4, color_bf16=1), "8-byte"),
         (dict(workspace=base + 64), "256-byte"),
+        (dict(k0_dim=5), "k0_dim"),
+        (dict(k0_dim=3, color_bf16=1), "fp32 colour grid"),
+        (dict(mlp=None), "null MLP"),
+        (dict(stepmode_density=3), "stepmode"),
+        (dict(stepmode_color=2), "stepmode"),
+        (dict(stepmode_density=2), "density_lr"),
     ]
     for kw, msg in cases:
         with pytest.raises(RuntimeError, match=msg):
