@@ -566,19 +566,19 @@ constexpr int kBw2Frags = 8 * 64, kBw0Frags = 4 * 64;
 // One workgroup of 8 wavefronts per CU (two per SIMD); wave w owns hidden
 // features [16w, 16w + 16) of both hidden layers, except in the g_H0 product
 // (phase 8), where it owns features [32(w/2), +32) of samples [16(w%2), +16).
-// Per group of 32 records the
-// phases are (a barrier after each):
-//   3  H0^T = W0^T X^T + b0 under the forward's ReLU mask m0    -> A1h, H0f
-//   4  H1^T = W1^T H0^T + b1 under the forward's mask m1         -> F2
+// Per group of 32 records three stages, a barrier after each, each pairing
+// products that do not depend on one another:
+//   A  H0^T = W0^T X^T + b0 under the forward's ReLU mask m0    -> A1h, H0f
+//      g_H1^T = W2 g_L^T masked by m1                           -> G1h, G1l, G1f
+//   B  H1^T = W1^T H0^T + b1 under the forward's mask m1         -> F2
+//      g_H0^T = W1 g_H1^T masked by m0                          -> A2h, A2l, F3
+//      dW1 += H0^T g_H1; the next group's records are parked
+//   C  dW2 += H1^T g_L; dW0 += X^T g_H0; waves 0-1: g_X^T = W0[0:12] g_H0^T
+//      over the whole hidden layer, written from registers as the records'
+//      feature gradients; waves 2-5: the next group's positional encoding
 // H0 and H1 are recomputed for the dW operands only (from hi parts, one MFMA
 // per product): the masks come from the forward pass (per-record words), so the
 // backward applies exactly the ReLU pattern the forward applied.
-//   7  g_H1^T = W2 g_L^T masked by m1 -> A1, G1f; dW2 += H1^T g_L; the next
-//      group's records are staged (park)
-//   8  g_H0^T = W1 g_H1^T masked by m0 -> A2, F2; dW1 += H0^T g_H1; the next
-//      group's positional encoding
-//   9  dW0 += X^T g_H0; waves 0-1: g_X^T = W0[0:12] g_H0^T over the whole hidden
-//      layer, written from registers as the records' feature gradients
 // X (features, encoding, bias), the per-sample adjoints and the flags are
 // double-buffered, so the next group's inputs are staged while this one runs.
 constexpr int kBwdWaves = 8;
@@ -588,12 +588,14 @@ struct alignas(16) SmemBwd {
     uint32_t relu[2][GS * 8];  // the forward's ReLU mask words of each record
     uint16_t Xh[2][GS * XM];
     uint16_t Xf[2][XF_ROWS * FM];
-    uint16_t A1h[GS * SM], A1l[GS * SM];
-    uint16_t A2h[GS * SM], A2l[GS * SM];
+    uint16_t A1h[GS * SM];               // H0 (hi), sample-major
+    uint16_t G1h[GS * SM], G1l[GS * SM]; // g_H1, sample-major
+    uint16_t A2h[GS * SM], A2l[GS * SM]; // g_H0, sample-major
     uint16_t GLh[2][GS * GM], GLl[2][GS * GM];
-    uint16_t H0f[128 * FM];
-    uint16_t F2[128 * FM];
-    uint16_t G1f[128 * FM];
+    uint16_t H0f[128 * FM];              // H0, feature-major
+    uint16_t F2[128 * FM];               // H1, feature-major
+    uint16_t G1f[128 * FM];              // g_H1, feature-major
+    uint16_t F3[128 * FM];               // g_H0, feature-major
     uint16_t GLf[2][16 * FM];
     float gsig[2][GS];
     float4 vd[2][GS];
@@ -786,7 +788,8 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
 
     for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {  // uniform bound
         const int64_t base = grp * GS, next = grp + gridDim.x;
-        // ---- 3. H0^T slice = W0^T X^T + b0 under the forward's mask -> A1h, H0f ----
+        // ---- A. H0^T slice = W0^T X^T + b0 under the forward's mask -> A1h, H0f;
+        //         g_H1^T slice = W2[slice] g_L^T under the mask -> G1h, G1l, G1f ----
         uint32_t m0 = 0u, m1 = 0u;  // the forward's ReLU masks: bit nt * 4 + i
         {
             const int sh = ((wv & 1) << 4) + 4 * g;  // features fr..fr+3 in word wv/2
@@ -808,6 +811,23 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
                 acc[nt] = mfma16x16x32(A0[kt], ld_frag(&sm.Xh[cb][xo]), acc[nt]);
             }
         }
+        {
+            const s16x4 a = __builtin_bit_cast(s16x4, sm.Bw2[wv * 64 + lane]);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                f32x4 c4 = {0.f, 0.f, 0.f, 0.f};
+                const int go = (16 * nt + r16) * GM + 4 * g;
+                c4 = mfma16x16x16(a, ld_frag4(&sm.GLh[cb][go]), c4);
+                c4 = mfma16x16x16(a, ld_frag4(&sm.GLl[cb][go]), c4);
+                float gv[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    gv[i] = (m1 >> (nt * 4 + i)) & 1u ? c4[i] : 0.f;
+                    db1[i] += gv[i];
+                }
+                st_act<true>(sm.G1h, sm.G1l, sm.G1f, 16 * nt + r16, fr, lane, gv);
+            }
+        }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
             float h[4];
@@ -815,8 +835,49 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
             for (int i = 0; i < 4; ++i) h[i] = (m0 >> (nt * 4 + i)) & 1u ? acc[nt][i] + b0v[i] : 0.f;
             st_hi_fm(sm.A1h, sm.H0f, 16 * nt + r16, fr, lane, h);
         }
+        stage_b(next);  // the next group's records were loaded during the last group
         __syncthreads();
-        // ---- 4. H1^T slice = W1^T H0^T + b1 under the forward's mask -> F2 (dW2 only) ----
+        // ---- B. H1^T slice = W1^T H0^T + b1 under the mask -> F2 (dW2 only);
+        //         g_H0^T tile = W1[rows] g_H1^T under the mask -> A2h, A2l, F3;
+        //         dW1 += H0^T g_H1; the next group's records are parked ----
+        // (g_H0 in 32 features x 16 samples per wave: half the g_H1 reads of 16 x 32)
+        f32x4 acc2[2];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) acc2[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kt = 0; kt < 4; ++kt) {
+            const int go = (16 * sh + r16) * SM + 32 * kt + 8 * g;
+            const bf16x8 bh = ld_frag(&sm.G1h[go]), bl = ld_frag(&sm.G1l[go]);
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) acc2[mt] = mfma_w(Bw1r[mt][kt], bh, bl, acc2[mt]);
+        }
+        {
+            // dW1[in][out] += sum over samples H0[s][in] g_H1[s][out]; this wave's
+            // block: inputs 64 (wv & 1) .. +63, outputs 32 (wv >> 1) .. +31 (fewer
+            // LDS reads than a full column of 128 inputs x 16 outputs)
+            const int i0 = 64 * (wv & 1), o0 = 32 * (wv >> 1);
+            const bf16x8 b0 = ld_frag(&sm.G1f[(o0 + r16) * FM + 8 * g]);
+            const bf16x8 b1 = ld_frag(&sm.G1f[(o0 + 16 + r16) * FM + 8 * g]);
+#pragma unroll
+            for (int mi = 0; mi < 4; ++mi) {
+                const bf16x8 a = ld_frag(&sm.H0f[(i0 + 16 * mi + r16) * FM + 8 * g]);
+                dW1[2 * mi] = mfma16x16x32(a, b0, dW1[2 * mi]);
+                dW1[2 * mi + 1] = mfma16x16x32(a, b1, dW1[2 * mi + 1]);
+            }
+        }
+        {
+            // g_H0 masked by the layer-0 word of features 32 fb .. +31
+            const uint32_t mw = sm.relu[cb][8 * (16 * sh + r16) + fb];
+#pragma unroll
+            for (int mt = 0; mt < 2; ++mt) {
+                float gv[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    gv[i] = (mw >> (16 * mt + 4 * g + i)) & 1u ? acc2[mt][i] : 0.f;
+                st_act<true>(sm.A2h, sm.A2l, sm.F3, 16 * sh + r16, 32 * fb + 16 * mt + 4 * g, lane,
+                             gv);
+            }
+        }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -834,73 +895,15 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
             for (int i = 0; i < 4; ++i) h[i] = (m1 >> (nt * 4 + i)) & 1u ? acc[nt][i] + b1v[i] : 0.f;
             st_fm(sm.F2, 16 * nt + r16, fr, lane, h);
         }
-        stage_b(next);  // the next group's records were loaded during the last group
+        park(cb ^ 1);  // the next group's stage-B loads (issued at stage A) have landed
         __syncthreads();
-        // ---- 7. g_H1^T slice = W2[slice] g_L^T, masked -> A1 (and G1f); dW2; park ----
+        // ---- C. dW2 += H1^T g_L; dW0 columns of this wave; g_X^T (waves 0-1) ->
+        //         record gradients; the next group's encoding (waves 2-5) ----
+        // dW2[in = f0 + row][out] += sum over the group's samples H1[s][in] gL[s][out]
+        dW2 = mfma16x16x32(ld_frag(&sm.F2[(f0 + r16) * FM + 8 * g]),
+                           ld_frag(&sm.GLf[cb][r16 * FM + 8 * g]), dW2);
         {
-            const s16x4 a = __builtin_bit_cast(s16x4, sm.Bw2[wv * 64 + lane]);
-#pragma unroll
-            for (int nt = 0; nt < NT; ++nt) {
-                f32x4 c4 = {0.f, 0.f, 0.f, 0.f};
-                const int go = (16 * nt + r16) * GM + 4 * g;
-                c4 = mfma16x16x16(a, ld_frag4(&sm.GLh[cb][go]), c4);
-                c4 = mfma16x16x16(a, ld_frag4(&sm.GLl[cb][go]), c4);
-                float gv[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    gv[i] = (m1 >> (nt * 4 + i)) & 1u ? c4[i] : 0.f;
-                    db1[i] += gv[i];
-                }
-                st_act<true>(sm.A1h, sm.A1l, sm.G1f, 16 * nt + r16, fr, lane, gv);
-            }
-            // dW2[in = f0 + row][out] += sum over the group's samples H1[s][in] gL[s][out]
-            dW2 = mfma16x16x32(ld_frag(&sm.F2[(f0 + r16) * FM + 8 * g]),
-                               ld_frag(&sm.GLf[cb][r16 * FM + 8 * g]), dW2);
-        }
-        __syncthreads();
-        // ---- 8. g_H0^T tile = W1[rows] g_H1^T, masked -> A2 (and F2); dW1; encode ----
-        // (32 features x 16 samples per wave: half the g_H1 reads of 16 x 32)
-#pragma unroll
-        for (int mt = 0; mt < 2; ++mt) acc[mt] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int kt = 0; kt < 4; ++kt) {
-            const int go = (16 * sh + r16) * SM + 32 * kt + 8 * g;
-            const bf16x8 bh = ld_frag(&sm.A1h[go]), bl = ld_frag(&sm.A1l[go]);
-#pragma unroll
-            for (int mt = 0; mt < 2; ++mt) acc[mt] = mfma_w(Bw1r[mt][kt], bh, bl, acc[mt]);
-        }
-        {
-            // dW1[in][out] += sum over samples H0[s][in] g_H1[s][out]; this wave's
-            // block: inputs 64 (wv & 1) .. +63, outputs 32 (wv >> 1) .. +31 (fewer
-            // LDS reads than a full column of 128 inputs x 16 outputs)
-            const int i0 = 64 * (wv & 1), o0 = 32 * (wv >> 1);
-            const bf16x8 b0 = ld_frag(&sm.G1f[(o0 + r16) * FM + 8 * g]);
-            const bf16x8 b1 = ld_frag(&sm.G1f[(o0 + 16 + r16) * FM + 8 * g]);
-#pragma unroll
-            for (int mi = 0; mi < 4; ++mi) {
-                const bf16x8 a = ld_frag(&sm.H0f[(i0 + 16 * mi + r16) * FM + 8 * g]);
-                dW1[2 * mi] = mfma16x16x32(a, b0, dW1[2 * mi]);
-                dW1[2 * mi + 1] = mfma16x16x32(a, b1, dW1[2 * mi + 1]);
-            }
-        }
-        // A2 is free and F2 was last read in step 7 (before its barrier): both take
-        // g_H0, masked by the layer-0 word of features 32 fb .. +31
-        {
-            const uint32_t mw = sm.relu[cb][8 * (16 * sh + r16) + fb];
-#pragma unroll
-            for (int mt = 0; mt < 2; ++mt) {
-                float gv[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-                    gv[i] = (mw >> (16 * mt + 4 * g + i)) & 1u ? acc[mt][i] : 0.f;
-                st_act<true>(sm.A2h, sm.A2l, sm.F2, 16 * sh + r16, 32 * fb + 16 * mt + 4 * g, lane, gv);
-            }
-        }
-        park(cb ^ 1);  // the next group's stage-B loads (issued after step 4) have landed
-        __syncthreads();
-        // ---- 9. dW0 columns of this wave; g_X^T (waves 0-1) -> record gradients ----
-        {
-            const bf16x8 bh = ld_frag(&sm.F2[(f0 + r16) * FM + 8 * g]);
+            const bf16x8 bh = ld_frag(&sm.F3[(f0 + r16) * FM + 8 * g]);
 #pragma unroll
             for (int mi = 0; mi < 3; ++mi)
                 dW0[mi] = mfma16x16x32(ld_frag(&sm.Xf[cb][(16 * mi + r16) * FM + 8 * g]), bh, dW0[mi]);
