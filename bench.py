@@ -45,7 +45,8 @@ def parse():
                     help="built-in launch shapes instead of the autotuner's choice")
     ap.add_argument("--launch", default=None,
                     help="fixed launch shapes of the headline step instead of tuning: "
-                         "mlp_fwd_workgroups,scatter_workgroups,scatter_chunk_records")
+                         "mlp_fwd_workgroups,scatter_workgroups,scatter_chunk_records"
+                         "[,ray_waves]")
     ap.add_argument("--configs", default="c1,c3,c4",
                     help="other workloads whose fused step is timed beside the headline one "
                          "(comma list, empty for none)")
@@ -158,9 +159,9 @@ def bench_batched(args, scene):
         from plenvdb_amd import autotune
 
         v = [int(x) for x in args.launch.split(",")]
-        if len(v) != 3:
-            raise SystemExit("--launch takes three integers")
-        launch = [v[0], v[1], v[2], 0]
+        if len(v) not in (3, 4):
+            raise SystemExit("--launch takes three or four integers")
+        launch = v + [0] if len(v) == 3 else v
         autotune.apply(tr, launch)
         tuned = {"launch": {autotune.NAMES[i]: launch[i] for i in autotune.NAMES},
                  "source": "given"}

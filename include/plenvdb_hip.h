@@ -274,8 +274,9 @@ typedef struct pvdb_step_desc {
     const float* adam_scalars;
     /* launch shapes, 0 = built-in default (results do not depend on them):
      * [0] forward-MLP workgroups, [1] scatter workgroups, [2] scatter records
-     * per chunk (32 or 64), [3] unused; pvdb_fused_launch_defaults() fills the
-     * defaults for this device */
+     * per chunk (32 or 64), [3] rays per workgroup of the wave-per-ray kernels
+     * (marches and composite: 1, 2, 4 or 8); pvdb_fused_launch_defaults()
+     * fills the defaults for this device */
     int32_t launch[4];
     /* loss weights of plenvdb/run.py:551-574 (cfg_train.weight_*): the MSE
      * term (1.0 in every configuration), the background entropy of T_last

@@ -1,14 +1,16 @@
 """Launch-shape autotuner of the fused training step.
 
-Three launch choices of `pvdb_fused_step` are free (`pvdb_step_desc.launch`):
-the forward MLP's workgroup count, the scatter's workgroup count and the
-scatter's records per chunk (32 or 64).  None changes the computed values
-beyond the order of the scatter's float atomics.  `tune` picks them per device
-and workload by coordinate descent: each candidate is timed on the stage it
-affects (the step's own HIP-event stage timers, several forward+backward passes
-per candidate) with the other choices held at their current best.  Choices are
-cached per (device, rays, capacity, leaves, colour type) for the process and,
-when PVDB_TUNE_CACHE names a JSON file, across processes.
+Four launch choices of `pvdb_fused_step` are free (`pvdb_step_desc.launch`):
+the forward MLP's workgroup count, the scatter's workgroup count, the
+scatter's records per chunk (32 or 64) and the rays (wavefronts) per workgroup
+of the wave-per-ray kernels (the two marches and the composite).  None changes
+the computed values beyond the order of the scatter's float atomics.  `tune`
+picks them per device and workload by coordinate descent: each candidate is
+timed on the stages it affects (the step's own HIP-event stage timers, several
+forward+backward passes per candidate) with the other choices held at their
+current best.  Choices are cached per (device, rays, capacity, leaves, colour
+type, model) for the process and, when PVDB_TUNE_CACHE names a JSON file,
+across processes.
 """
 from __future__ import annotations
 
@@ -24,9 +26,12 @@ CANDIDATES = {
     0: (256, 512, 768, 1024),         # forward-MLP workgroups
     2: (32, 64),                      # scatter records per chunk
     1: (512, 768, 1024, 1536, 2048),  # scatter workgroups
+    3: (1, 2, 4, 8),                  # rays per workgroup of the wave-per-ray kernels
 }
-STAGE = {0: "mlp_fwd", 1: "scatter", 2: "scatter"}
-NAMES = {0: "mlp_fwd_workgroups", 1: "scatter_workgroups", 2: "scatter_chunk_records"}
+STAGE = {0: ("mlp_fwd",), 1: ("scatter",), 2: ("scatter",),
+         3: ("ray_count", "ray_march", "composite")}
+NAMES = {0: "mlp_fwd_workgroups", 1: "scatter_workgroups", 2: "scatter_chunk_records",
+         3: "ray_waves"}
 
 _cache: dict[str, list[int]] = {}
 
@@ -62,7 +67,8 @@ def _save_file(entries: dict) -> None:
         json.dump(data, f, indent=1, sort_keys=True)
 
 
-def _stage_ms(tr, stage: str, reps: int) -> float:
+def _stage_ms(tr, stages: tuple, reps: int) -> float:
+    """Mean time per step of the named stages (summed)."""
     from .fused import stage_timing
 
     tr.forward_backward()  # warm-up with this choice
@@ -71,8 +77,8 @@ def _stage_ms(tr, stage: str, reps: int) -> float:
     for _ in range(reps):
         tr.forward_backward()
     torch.cuda.synchronize()
-    tot, calls = stage_timing(0)[stage]
-    return tot / max(calls, 1)
+    st = stage_timing(0)
+    return sum(st[s][0] / max(st[s][1], 1) for s in stages)
 
 
 def apply(tr, launch: list[int]) -> None:
@@ -99,7 +105,7 @@ def tune(tr, reps: int = 3, force: bool = False) -> dict:
     timings = {}
     # the chunk length before the scatter's workgroup count; a model without the
     # MLP has no forward-MLP launch to shape
-    for idx in ((2, 1) if tr.k0_dim else (0, 2, 1)):
+    for idx in ((2, 1, 3) if tr.k0_dim else (0, 2, 1, 3)):
         t = {}
         for v in CANDIDATES[idx]:
             tr.desc.launch[idx] = v

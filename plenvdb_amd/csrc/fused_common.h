@@ -60,7 +60,12 @@ struct StepParams {
     float ent_scale;           // weight_entropy_last / N: entropy of T_last, mean over rays
     float rgbper_scale;        // weight_rgbper / N: sum of w * |rgb_i - target|^2 over samples
     int64_t capacity;          // record capacity
+    int32_t ray_waves;         // rays (wavefronts) per workgroup of the wave-per-ray kernels
 };
+
+// Rays per workgroup of the wave-per-ray kernels (k_ray_count, k_ray_march and
+// the composites) at most; the launch chooses 1, 2, 4 or 8.
+constexpr int kMaxRayWaves = 8;
 
 // Records: structure of arrays, index = record id.
 struct Records {

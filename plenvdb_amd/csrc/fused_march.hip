@@ -162,14 +162,15 @@ __device__ __forceinline__ int mbcnt(uint64_t m) {
 
 // Counting march: occupancy-kept samples per ray.  The workgroups past the
 // rays' pack the step's MLP operands (independent work, one launch fewer).
-__global__ void __launch_bounds__(256) k_ray_count(StepParams P, TopoView t, RayBuffers R,
-                                                   PackArgs pk, int ray_blocks) {
+__global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_count(StepParams P, TopoView t,
+                                                                 RayBuffers R, PackArgs pk,
+                                                                 int ray_blocks) {
     if ((int)blockIdx.x >= ray_blocks) {  // uniform per workgroup
-        const int item = ((int)blockIdx.x - ray_blocks) * 256 + (int)threadIdx.x;
+        const int item = ((int)blockIdx.x - ray_blocks) * (int)blockDim.x + (int)threadIdx.x;
         if (item < kPackItems) pack_mlp_item(item, pk);
         return;
     }
-    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (r >= P.n_rays) return;
     const int lane = lane_id();
     const RaySetup s = ray_setup(P, r);
@@ -247,13 +248,14 @@ __global__ void __launch_bounds__(1024) k_scan(RayBuffers R, int n) {
 // in LDS (in step order) and the sample work -- density gather, raw2alpha,
 // the transmittance scan, records -- runs on 64 queued samples at a time, so
 // lanes are not spent on rejected steps.
-__global__ void __launch_bounds__(256) k_ray_march(StepParams P, TopoView t, const float* density,
-                                                   RayBuffers R, Records rec) {
-    __shared__ int64_t queue_all[4][128];
-    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+__global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_march(StepParams P, TopoView t,
+                                                                 const float* density,
+                                                                 RayBuffers R, Records rec) {
+    extern __shared__ int64_t queue_all[];  // 128 entries per wave (dynamic: waves x 1 KB)
+    const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (r >= P.n_rays) return;  // uniform over the wave
     const int lane = lane_id();
-    int64_t* queue = queue_all[threadIdx.x >> 6];
+    int64_t* queue = queue_all + 128 * (threadIdx.x >> 6);
     const RaySetup s = ray_setup(P, r);
     const int64_t off = R.offset[r];
     const int cnt = R.count[r];  // the counting march's total bounds this ray's records
@@ -367,8 +369,9 @@ __device__ __forceinline__ float chunk_transmittance(float alpha, uint32_t bits,
 // ray colour = sum + T_last*bg (dvgo.py:365-370), the loss of run.py:551-574
 // (MSE; background entropy of T_last; per-sample rgb loss with the weights
 // detached) and its gradient.
-__global__ void __launch_bounds__(256) k_ray_composite(StepParams P, RayBuffers R, Records rec) {
-    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+__global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_composite(StepParams P, RayBuffers R,
+                                                                     Records rec) {
+    const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (r >= P.n_rays) return;
     const int lane = lane_id();
     const int64_t off = R.offset[r];
@@ -515,12 +518,12 @@ __device__ __forceinline__ void wave_prefix(float (&v)[NC], int lane) {
 // Alphas2Weights and raw2alpha, which the scatter reads: no backward pass
 // over the records sits in between.
 template <int NC>
-__global__ void __launch_bounds__(256) k_direct_composite(StepParams P, TopoView topo,
+__global__ void __launch_bounds__(kMaxRayWaves * 64) k_direct_composite(StepParams P, TopoView topo,
                                                           const float* __restrict__ color,
                                                           RayBuffers R, Records rec) {
     static_assert(NC == 3 || NC == 12, "direct colour channels");
     constexpr int kSig = kSigmaAt<NC>;
-    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (r >= P.n_rays) return;
     const int lane = lane_id();
     const int64_t off = R.offset[r];
@@ -696,10 +699,11 @@ namespace pvdb {
 
 int launch_ray_count(const StepParams& P, const TopoView& t, const RayBuffers& R,
                      const PackArgs& pk, hipStream_t s) {
-    const int ray_blocks = (int)pvdb_blocks(P.n_rays, 4);
+    const int w = P.ray_waves;
+    const int ray_blocks = (int)pvdb_blocks(P.n_rays, w);
     // no pack workgroups without an MLP (pk.prm == nullptr)
-    const int grid = ray_blocks + (pk.prm ? (int)pvdb_blocks(kPackItems, 256) : 0);
-    hipLaunchKernelGGL(k_ray_count, dim3(grid), dim3(256), 0, s, P, t, R, pk, ray_blocks);
+    const int grid = ray_blocks + (pk.prm ? (int)pvdb_blocks(kPackItems, 64 * w) : 0);
+    hipLaunchKernelGGL(k_ray_count, dim3(grid), dim3(64 * w), 0, s, P, t, R, pk, ray_blocks);
     PVDB_LAUNCHED("k_ray_count");
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, R, P.n_rays);
     PVDB_LAUNCHED("k_scan");
@@ -708,7 +712,9 @@ int launch_ray_count(const StepParams& P, const TopoView& t, const RayBuffers& R
 
 int launch_ray_march(const StepParams& P, const TopoView& t, const float* density,
                      const RayBuffers& R, const Records& rec, hipStream_t s) {
-    hipLaunchKernelGGL(k_ray_march, dim3(pvdb_blocks(P.n_rays, 4)), dim3(256), 0, s, P, t, density,
+    const int w = P.ray_waves;
+    hipLaunchKernelGGL(k_ray_march, dim3(pvdb_blocks(P.n_rays, w)), dim3(64 * w),
+                       (unsigned)(w * 128 * sizeof(int64_t)), s, P, t, density,
                        R, rec);
     PVDB_LAUNCHED("k_ray_march");
     return PVDB_OK;
@@ -716,7 +722,9 @@ int launch_ray_march(const StepParams& P, const TopoView& t, const float* densit
 
 int launch_ray_composite(const StepParams& P, const RayBuffers& R, const Records& rec,
                          hipStream_t s) {
-    hipLaunchKernelGGL(k_ray_composite, dim3(pvdb_blocks(P.n_rays, 4)), dim3(256), 0, s, P, R, rec);
+    const int w = P.ray_waves;
+    hipLaunchKernelGGL(k_ray_composite, dim3(pvdb_blocks(P.n_rays, w)), dim3(64 * w), 0, s, P, R,
+                       rec);
     PVDB_LAUNCHED("k_ray_composite");
     hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(1024), 0, s, R, P.n_rays);
     PVDB_LAUNCHED("k_loss_reduce");
@@ -726,11 +734,12 @@ int launch_ray_composite(const StepParams& P, const RayBuffers& R, const Records
 int launch_direct_composite(int nc, const StepParams& P, const TopoView& t, const float* color,
                             const RayBuffers& R, const Records& rec, hipStream_t s) {
     PVDB_CHECK_ARG(nc == 3 || nc == 12, "direct colour channels must be 3 or 12");
-    const dim3 grid(pvdb_blocks(P.n_rays, 4));
+    const int w = P.ray_waves;
+    const dim3 grid(pvdb_blocks(P.n_rays, w));
     if (nc == 3)
-        hipLaunchKernelGGL(k_direct_composite<3>, grid, dim3(256), 0, s, P, t, color, R, rec);
+        hipLaunchKernelGGL(k_direct_composite<3>, grid, dim3(64 * w), 0, s, P, t, color, R, rec);
     else
-        hipLaunchKernelGGL(k_direct_composite<12>, grid, dim3(256), 0, s, P, t, color, R, rec);
+        hipLaunchKernelGGL(k_direct_composite<12>, grid, dim3(64 * w), 0, s, P, t, color, R, rec);
     PVDB_LAUNCHED("k_direct_composite");
     hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(1024), 0, s, R, P.n_rays);
     PVDB_LAUNCHED("k_loss_reduce");

@@ -42,6 +42,7 @@ constexpr int kMaxMlpGrid = 512;   // workgroups of the MLP kernels at most (dW 
 constexpr int kFwdGrid = 512;      // default forward-MLP workgroups
 constexpr int kScatterChunk = 64;  // default scatter records per chunk
 constexpr int kScatterGrid = 1024; // default scatter workgroups
+constexpr int kRayWaves = 4;       // default rays per workgroup of the wave-per-ray kernels
 // Records are indexed through int32 offsets (saturating at INT32_MAX), so a
 // capacity below that bound keeps every stored offset of a dropped record
 // outside the record arrays.
@@ -312,7 +313,7 @@ int pvdb_fused_launch_defaults(int32_t* launch4) {
     launch4[0] = kFwdGrid;
     launch4[1] = kScatterGrid;
     launch4[2] = kScatterChunk;
-    launch4[3] = 0;
+    launch4[3] = kRayWaves;
     return PVDB_OK;
 }
 
@@ -396,6 +397,9 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
                    "launch[0] must be in [0, 4096] and launch[1] in [0, 65536]");
     PVDB_CHECK_ARG(d->launch[2] == 0 || d->launch[2] == 32 || d->launch[2] == 64,
                    "launch[2]: scatter chunk must be 32 or 64 records");
+    PVDB_CHECK_ARG(d->launch[3] == 0 || d->launch[3] == 1 || d->launch[3] == 2 ||
+                       d->launch[3] == 4 || d->launch[3] == kMaxRayWaves,
+                   "launch[3]: rays per workgroup must be 1, 2, 4 or 8");
     PVDB_CHECK_ARG(d->weight_main >= 0.f && d->weight_entropy_last >= 0.f &&
                        d->weight_rgbper >= 0.f && d->weight_main < INFINITY &&
                        d->weight_entropy_last < INFINITY && d->weight_rgbper < INFINITY,
@@ -443,6 +447,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     P.ent_scale = d->weight_entropy_last / (float)d->n_rays;
     P.rgbper_scale = d->weight_rgbper / (float)d->n_rays;
     P.capacity = d->capacity;
+    P.ray_waves = d->launch[3] > 0 ? d->launch[3] : kRayWaves;
 
     RayBuffers R;
     R.count = reinterpret_cast<int32_t*>(ws + L.count);

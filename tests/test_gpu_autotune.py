@@ -27,7 +27,7 @@ def test_autotune_choices_and_invariance():
     print("[result]", out["launch"], flush=True)
     for i, name in autotune.NAMES.items():
         assert out["launch"][name] in autotune.CANDIDATES[i]
-    assert list(tr.desc.launch)[:3] == [out["launch"][autotune.NAMES[i]] for i in range(3)]
+    assert list(tr.desc.launch) == [out["launch"][autotune.NAMES[i]] for i in range(4)]
     # every candidate of each choice gives the same values
     for idx, cands in autotune.CANDIDATES.items():
         for v in cands:
@@ -58,6 +58,10 @@ def test_launch_shapes_are_checked():
     with pytest.raises(RuntimeError, match="launch"):
         tr.forward_backward()
     tr.desc.launch[0] = 0
+    tr.desc.launch[3] = 3
+    with pytest.raises(RuntimeError, match="rays per workgroup"):
+        tr.forward_backward()
+    tr.desc.launch[3] = 0
     tr.forward_backward()
     torch.cuda.synchronize()
     assert np.isfinite(float(tr.loss.item()))
