@@ -196,13 +196,15 @@ __device__ __forceinline__ void st_act3(uint16_t* sh, uint16_t* sm_, uint16_t* s
 // features [32w, 32w + 32) of both hidden layers (up to three workgroups per
 // CU).  X and H0 enter their products as (hi, mid, lo) triples, so the
 // pre-activations, and with them the ReLU masks the backward applies, carry
-// fp32 precision; H1 enters the logits as (hi, lo) pairs.  Per group, four
+// fp32 precision; H1 enters the logits as (hi, lo) pairs.  Per group, three
 // phases with a barrier after each:
-//   A  X = [colour features | PE | 1] from the gather and view directions
-//      loaded during the previous group; the features are kept per record
+//   A  rgb of the previous group from its partial logits; X = [colour
+//      features | PE | 1] from the gather and view directions loaded during
+//      the previous group; the features are kept per record
 //   B  H0, then the next group's topology lookups and view-direction loads
-//   C  H1, then the next group's colour loads
-//   D  logits -> rgb
+//   C  H1, then each wave's partial logits over its own 32 hidden features
+//      (it reads back only the H1 it wrote: no workgroup barrier between),
+//      then the next group's colour loads
 // The records are staged two groups ahead (flags and positions double-buffered
 // in LDS); W0^T and W1^T slices stay in registers, W2^T in LDS.  The ReLU
 // masks of both hidden layers are stored per record for the backward.
@@ -225,6 +227,8 @@ struct alignas(16) SmemFwd {
     } u;
     uint16_t A1h[GS * SM], A1m[GS * SM], A1l[GS * SM];  // H0
     RecInfo ri[2][GS];
+    float4 plog[4][GS];  // each wave's partial logits (0..2) over its 32 hidden features
+    uint32_t pbits[GS];  // the flags of the group whose partial logits these are
 };
 
 __device__ __forceinline__ float sigmoidf_(float x) { return 1.f / (1.f + expf(-x)); }
@@ -345,10 +349,30 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
     lookup(0);
     fetch();
     int cb = 0;  // buffer of the current group's records
+    // rgb of a group from the four waves' partial logits (summed in wave
+    // order), written by the threads of its 32 samples
+    auto finish_rgb = [&](int64_t pbase) {
+        if (tid < GS) {
+            const int64_t q = pbase + tid;
+            if (q < total) {
+                const float4 l0 = sm.plog[0][tid], l1 = sm.plog[1][tid], l2 = sm.plog[2][tid],
+                             l3 = sm.plog[3][tid];
+                const bool alive = (sm.pbits[tid] & kAliveW) != 0u;
+                rec.rgb[q] = alive ? make_float4(sigmoidf_(((l0.x + l1.x) + l2.x) + l3.x + b2v[0]),
+                                                 sigmoidf_(((l0.y + l1.y) + l2.y) + l3.y + b2v[1]),
+                                                 sigmoidf_(((l0.z + l1.z) + l2.z) + l3.z + b2v[2]),
+                                                 0.f)
+                                   : z4;
+            }
+        }
+    };
+    int64_t pbase = -1;  // the group whose partial logits wait in plog (-1: none)
 
     for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {  // uniform bound
         const int64_t base = grp * GS;
-        // ---- A. X = [colour features (12) | PE (27) | 1 | 0...]; stage the records ----
+        // ---- A. the previous group's rgb; X = [colour features (12) | PE (27) | 1 | 0...];
+        //         stage the records ----
+        if (pbase >= 0) finish_rgb(pbase);
         {
             // trilinear colour: corner gpart's row times its weight, summed over
             // the record's 8 threads (zero where the record or corner is absent)
@@ -527,33 +551,28 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
             const int64_t q = base + 16 * nt + r16;
             if (g == 0 && q < total) rec.relu[8 * q + 4 + wv] = w;
         }
-        fetch();
-        __syncthreads();
-        // ---- D. logits (wave nt takes tile nt): L^T = W2^T H1^T + b2, rgb = sigmoid ----
-        if (wv < NT) {
-            const int nt = wv;
-            f32x4 a5 = {0.f, 0.f, 0.f, 0.f};
-            const int ho = (16 * nt + r16) * SM + 8 * g;
+        // partial logits over this wave's hidden features f0 .. f0 + 31 (one
+        // k-step of L^T = W2^T H1^T), read back from the H1 this wave just wrote
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        {
+            const bf16x8 w2 = ld_frag(&sm.W2t[r16 * 128 + f0 + 8 * g]);
 #pragma unroll
-            for (int kt = 0; kt < 4; ++kt)
-                a5 = mfma_w(ld_frag(&sm.W2t[r16 * 128 + 32 * kt + 8 * g]),
-                            ld_frag(&sm.u.h1.A2h[ho + 32 * kt]), ld_frag(&sm.u.h1.A2l[ho + 32 * kt]),
-                            a5);
-            if (g == 0) {
-                const int sidx = 16 * nt + r16;
-                const int64_t q = base + sidx;
-                if (q < total) {
-                    const bool alive = (sm.ri[cb][sidx].bits & kAliveW) != 0u;
-                    rec.rgb[q] = alive ? make_float4(sigmoidf_(a5[0] + b2v[0]),
-                                                     sigmoidf_(a5[1] + b2v[1]),
-                                                     sigmoidf_(a5[2] + b2v[2]), 0.f)
-                                       : z4;
-                }
+            for (int nt = 0; nt < NT; ++nt) {
+                const int ho = (16 * nt + r16) * SM + f0 + 8 * g;
+                f32x4 a5 = {0.f, 0.f, 0.f, 0.f};
+                a5 = mfma_w(w2, ld_frag(&sm.u.h1.A2h[ho]), ld_frag(&sm.u.h1.A2l[ho]), a5);
+                if (g == 0) sm.plog[wv][16 * nt + r16] = make_float4(a5[0], a5[1], a5[2], 0.f);
             }
         }
+        if (tid < GS) sm.pbits[tid] = sm.ri[cb][tid].bits;
+        pbase = base;
+        fetch();
         __syncthreads();
         cb ^= 1;
     }
+    if (pbase >= 0) finish_rgb(pbase);  // the loop ended on a barrier
 }
 
 // Backward weights as MFMA A fragments in LDS, [tile][k-step][lane] (each
