@@ -73,7 +73,9 @@ struct Records {
     float2* ab;        // alpha, ray|flags (as bits); T and w are recomputed from the alphas
     float4* rgb;       // MLP colour (rgb), written by the forward MLP pass
     float4* adj;       // per-sample adjoints: dL/dlogit (3), dL/dsigma
-    uint4* feat;       // [3 per record] the 12 gathered colour features as bf16 hi, then lo
+    uint4* xrow;       // [5 per record] the backward's MLP input row, X columns 0..39 as bf16
+                       // hi parts: the 12 gathered colour features, the 27 view-direction
+                       // encoding inputs and the bias input (a zero row unless weight-kept)
     uint32_t* relu;    // [8 per record] forward ReLU masks: word 4l + j = layer l's
                        // hidden features 32j..32j+31, bit f%32
     float4* grad;      // [4 per record] dL/dfeature (12) and dL/dsigma; in the direct-colour

@@ -413,13 +413,12 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
                     xm[c] = wl[c];
                     xl[c] = wt[c];
                 }
-                // the features kept for the backward: hi halves 0..11, then the
-                // second parts (the (hi, lo) pair of the feature)
+                // the backward's input row, features part (X columns 0..11, hi)
                 const int64_t qq = base + gs;
-                if (qq < total && (sm.ri[cb][gs].bits & kAliveW)) {
-                    rec.feat[3 * qq] = make_uint4(wh[0], wh[1], wh[2], wh[3]);
-                    rec.feat[3 * qq + 1] = make_uint4(wh[4], wh[5], wl[0], wl[1]);
-                    rec.feat[3 * qq + 2] = make_uint4(wl[2], wl[3], wl[4], wl[5]);
+                if (qq < total) {
+                    uint32_t* xr = reinterpret_cast<uint32_t*>(rec.xrow + 5 * qq);
+                    *reinterpret_cast<uint4*>(xr) = make_uint4(wh[0], wh[1], wh[2], wh[3]);
+                    *reinterpret_cast<uint2*>(xr + 4) = make_uint2(wh[4], wh[5]);
                 }
             }
         }
@@ -454,6 +453,10 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
                 *reinterpret_cast<uint2*>(&sm.u.x.Xh[ps * XM + 12 + 4 * pp]) = hi;
                 *reinterpret_cast<uint2*>(&sm.u.x.Xm[ps * XM + 12 + 4 * pp]) = mi;
                 *reinterpret_cast<uint2*>(&sm.u.x.Xl[ps * XM + 12 + 4 * pp]) = lo;
+                // the backward's input row, encoding part (X columns 12 + 4pp .. +3)
+                const int64_t qq = base + ps;
+                if (qq < total)
+                    reinterpret_cast<uint2*>(rec.xrow + 5 * qq)[3 + pp] = hi;
             } else {
                 // X columns 40..63 (zero weights) are zeroed for every group: H1
                 // of the previous group shares their storage
@@ -585,8 +588,8 @@ constexpr int kBw2Frags = 8 * 64, kBw0Frags = 4 * 64;
 // One workgroup of 8 wavefronts per CU (two per SIMD); wave w owns hidden
 // features [16w, 16w + 16) of both hidden layers, except in the g_H0 product
 // (phase 8), where it owns features [32(w/2), +32) of samples [16(w%2), +16).
-// Per group of 32 records three stages, a barrier after each, each pairing
-// products that do not depend on one another:
+// Per group of 32 records three stages of work, each pairing products that
+// do not depend on one another:
 //   A  H0^T = W0^T X^T + b0 under the forward's ReLU mask m0    -> A1h, H0f
 //      g_H1^T = W2 g_L^T masked by m1                           -> G1h, G1l, G1f
 //   B  H1^T = W1^T H0^T + b1 under the forward's mask m1         -> F2
@@ -594,7 +597,11 @@ constexpr int kBw2Frags = 8 * 64, kBw0Frags = 4 * 64;
 //      dW1 += H0^T g_H1; the next group's records are parked
 //   C  dW2 += H1^T g_L; dW0 += X^T g_H0; waves 0-1: g_X^T = W0[0:12] g_H0^T
 //      over the whole hidden layer, written from registers as the records'
-//      feature gradients; waves 2-5: the next group's positional encoding
+//      feature gradients
+// run as two barrier-separated steps: group k's A with group k-1's C, then
+// group k's B.  The input rows X come whole from the forward pass (features,
+// encoding, bias input), so group k+1's A needs nothing computed after group
+// k's B.
 // H0 and H1 are recomputed for the dW operands only (from hi parts, one MFMA
 // per product): the masks come from the forward pass (per-record words), so the
 // backward applies exactly the ReLU pattern the forward applied.
@@ -617,7 +624,6 @@ struct alignas(16) SmemBwd {
     uint16_t F3[128 * FM];               // g_H0, feature-major
     uint16_t GLf[2][16 * FM];
     float gsig[2][GS];
-    float4 vd[2][GS];
     uint32_t bits[2][GS];
 };
 
@@ -697,44 +703,38 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
     const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
     const uint4 zu4 = make_uint4(0u, 0u, 0u, 0u);
 
-    // register prefetch of a group's records on the staging lanes
-    float4 p_adj = z4, p_vd = z4;
-    float2 p_ab = make_float2(0.f, 0.f);
-    uint4 p_f0 = zu4, p_f1 = zu4, p_r0 = zu4, p_r1 = zu4;
-    auto stage_a = [&](int64_t grp) {  // the records (independent loads)
+    // register prefetch of a group's records: the staging lanes (4 per wave, a
+    // record each) take its flags, adjoints and ReLU masks; the row lanes (20
+    // per wave) one 16-byte fifth of its input row each.  Every address
+    // follows from the record index alone, so the loads of the next group are
+    // issued at the start of the current one and land during its products.
+    float4 p_adj = z4;
+    uint32_t p_bits = 0u;
+    uint4 p_r0 = zu4, p_r1 = zu4, p_x = zu4;
+    const int xe = (lane >= 32 && lane < 52) ? wv * 20 + (lane - 32) : -1;
+    const int xr = xe / 5, xj = xe % 5;  // row lane: record, 8-column part
+    auto stage = [&](int64_t grp) {
         const int64_t q = grp * GS + pr;
         p_adj = z4;
-        p_ab = make_float2(0.f, 0.f);
+        p_bits = 0u;
+        p_r0 = p_r1 = zu4;
         if (pr >= 0 && grp < ngroups && q < total) {
-            p_ab = rec.ab[q];
+            p_bits = __float_as_uint(rec.ab[q].y);
             p_adj = rec.adj[q];
+            p_r0 = reinterpret_cast<const uint4*>(rec.relu)[2 * q];  // ReLU masks
+            p_r1 = reinterpret_cast<const uint4*>(rec.relu)[2 * q + 1];
         }
-    };
-    auto stage_b = [&](int64_t grp) {  // view direction and kept features (need the flags)
-        const int64_t q = grp * GS + pr;
-        p_vd = z4;
-        p_f0 = p_f1 = p_r0 = p_r1 = zu4;
-        if (pr >= 0 && grp < ngroups && q < total) {
-            const uint32_t bits = __float_as_uint(p_ab.y);
-            if (bits & kAliveW) {
-                const uint32_t ray = bits & kRayMask;
-                p_vd = make_float4(P.viewdirs[3 * ray], P.viewdirs[3 * ray + 1],
-                                   P.viewdirs[3 * ray + 2], 0.f);
-                p_f0 = rec.feat[3 * q];  // hi halves 0..7
-                p_f1 = rec.feat[3 * q + 1];  // hi halves 8..11 (.x, .y)
-                p_r0 = reinterpret_cast<const uint4*>(rec.relu)[2 * q];  // ReLU masks
-                p_r1 = reinterpret_cast<const uint4*>(rec.relu)[2 * q + 1];
-            }
-        }
+        const int64_t qx = grp * GS + xr;
+        p_x = zu4;
+        if (xe >= 0 && grp < ngroups && qx < total) p_x = rec.xrow[5 * qx + xj];
     };
     // staged records -> buffer b: flags, adjoints (dL/dlogit as (hi, lo) pairs,
-    // dL/dsigma), view direction, ReLU masks, the kept features' hi parts (X
-    // columns 0..11)
+    // dL/dsigma), ReLU masks, and the input rows (X columns 0..39, sample-major
+    // and feature-major)
     auto park = [&](int b) {
         if (pr >= 0) {
-            sm.bits[b][pr] = __float_as_uint(p_ab.y);  // zero for records past the end
+            sm.bits[b][pr] = p_bits;  // zero for records past the end
             sm.gsig[b][pr] = p_adj.w;
-            sm.vd[b][pr] = p_vd;
             reinterpret_cast<uint4*>(&sm.relu[b][8 * pr])[0] = p_r0;
             reinterpret_cast<uint4*>(&sm.relu[b][8 * pr])[1] = p_r1;
             const float gl[3] = {p_adj.x, p_adj.y, p_adj.z};
@@ -749,66 +749,68 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
                 sm.GLl[b][pr * GM + o] = l;
                 sm.GLf[b][o * FM + pr] = h;
             }
-            const uint32_t w6[6] = {p_f0.x, p_f0.y, p_f0.z, p_f0.w, p_f1.x, p_f1.y};
-            uint32_t* xh = reinterpret_cast<uint32_t*>(&sm.Xh[b][pr * XM]);
+        }
+        if (xe >= 0) {
+            *reinterpret_cast<uint4*>(&sm.Xh[b][xr * XM + 8 * xj]) = p_x;  // 16-byte aligned
+            const uint32_t w4[4] = {p_x.x, p_x.y, p_x.z, p_x.w};
 #pragma unroll
-            for (int c = 0; c < 6; ++c) {  // hi halves of features 2c, 2c + 1
-                xh[c] = w6[c];
-                sm.Xf[b][(2 * c) * FM + pr] = (uint16_t)(w6[c] & 0xffffu);
-                sm.Xf[b][(2 * c + 1) * FM + pr] = (uint16_t)(w6[c] >> 16);
+            for (int c = 0; c < 4; ++c) {
+                sm.Xf[b][(8 * xj + 2 * c) * FM + xr] = (uint16_t)(w4[c] & 0xffffu);
+                sm.Xf[b][(8 * xj + 2 * c + 1) * FM + xr] = (uint16_t)(w4[c] >> 16);
             }
         }
     };
-    // positional encoding (dvgo.py:354-358) and bias column of buffer b's group:
-    // thread (sample ps, part pp < 7) writes X columns 12 + 4pp .. +3; hardware
-    // sin/cos (error about 1e-6 for |arg| <= 8, below what the (hi, lo) pair keeps)
-    auto encode = [&](int b) {  // waves 2..5
-        const int te = tid - 128;
-        const int ps = te & (GS - 1), pp = te >> 5;
-        const bool alive = (sm.bits[b][ps] & kAliveW) != 0u;
-        const float4 v4 = sm.vd[b][ps];  // zero unless alive
-        const float vd[3] = {v4.x, v4.y, v4.z};
-        float val[4];
+    // stage C of a group whose g_H0 and H1 sit in A2h/A2l, F3 and F2 (its
+    // inputs in buffer b, its records from pbase): dW2, dW0 and the records'
+    // feature gradients
+    auto finish = [&](int b, int64_t pbase) {
+        // dW2[in = f0 + row][out] += sum over the group's samples H1[s][in] gL[s][out]
+        dW2 = mfma16x16x32(ld_frag(&sm.F2[(f0 + r16) * FM + 8 * g]),
+                           ld_frag(&sm.GLf[b][r16 * FM + 8 * g]), dW2);
+        {
+            const bf16x8 bh = ld_frag(&sm.F3[(f0 + r16) * FM + 8 * g]);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            const int p = pp * 4 + k;  // 0..27: PE 0..26, then the bias column
-            float v = 0.f;
-            if (alive) {
-                if (p < 3) {
-                    v = vd[p];
-                } else if (p < 27) {
-                    const int idx = p - 3, axis = (idx % 12) / 4, fq = idx % 4;
-                    const float arg = vd[axis] * (float)(1 << fq);
-                    v = idx < 12 ? __sinf(arg) : __cosf(arg);
-                } else {
-                    v = 1.f;
-                }
-            }
-            val[k] = v;
+            for (int mi = 0; mi < 3; ++mi)
+                dW0[mi] = mfma16x16x32(ld_frag(&sm.Xf[b][(16 * mi + r16) * FM + 8 * g]), bh, dW0[mi]);
         }
-        uint2 hi, lo;
-        split2(val[0], val[1], hi.x, lo.x);
-        split2(val[2], val[3], hi.y, lo.y);
-        const uint2 t = quad_transpose(hi, lane & 3);  // the quad: 4 samples, one part
-        if (pp < 7) {
-            *reinterpret_cast<uint2*>(&sm.Xh[b][ps * XM + 12 + 4 * pp]) = hi;
-            *reinterpret_cast<uint2*>(&sm.Xf[b][(12 + 4 * pp + (lane & 3)) * FM + (ps & ~3)]) = t;
+        if (wv < NT) {
+            const int nt = wv;
+            f32x4 gx = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kt = 0; kt < 4; ++kt) {
+                const int go = (16 * nt + r16) * SM + 32 * kt + 8 * g;
+                gx = mfma_w(as_bf8(sm.Bw0[kt * 64 + lane]), ld_frag(&sm.A2h[go]),
+                            ld_frag(&sm.A2l[go]), gx);
+            }
+            // lane (g, r16): features 4g..4g+3 of sample 16 nt + r16 = part g of its
+            // record's gradient; part 3 carries dL/dsigma
+            const int sidx = 16 * nt + r16;
+            const int64_t q = pbase + sidx;
+            if (q < total) {
+                const uint32_t bits = sm.bits[b][sidx];
+                float4 v = z4;
+                if (g < 3) {
+                    if (bits & kAliveW) v = make_float4(gx[0], gx[1], gx[2], gx[3]);
+                } else {
+                    v.x = sm.gsig[b][sidx];
+                }
+                rec.grad[4 * q + g] = v;
+            }
         }
     };
 
-    stage_a(blockIdx.x);
-    stage_b(blockIdx.x);
+    stage(blockIdx.x);
     park(0);
     __syncthreads();
-    if (wv >= 2 && wv < 6) encode(0);
-    __syncthreads();
-    stage_a(blockIdx.x + gridDim.x);
-    int cb = 0;  // buffer of the current group
+    int cb = 0;          // buffer of the current group
+    int64_t pbase = -1;  // the previous group, whose stage C is pending (-1: none)
 
     for (int64_t grp = blockIdx.x; grp < ngroups; grp += gridDim.x) {  // uniform bound
         const int64_t base = grp * GS, next = grp + gridDim.x;
-        // ---- A. H0^T slice = W0^T X^T + b0 under the forward's mask -> A1h, H0f;
-        //         g_H1^T slice = W2[slice] g_L^T under the mask -> G1h, G1l, G1f ----
+        stage(next);  // in flight during this group's products, parked at its end
+        // ---- 1. this group's stage A: H0^T slice = W0^T X^T + b0 under the
+        //         forward's mask -> A1h, H0f; g_H1^T slice = W2[slice] g_L^T under
+        //         the mask -> G1h, G1l, G1f.  The previous group's stage C ----
         uint32_t m0 = 0u, m1 = 0u;  // the forward's ReLU masks: bit nt * 4 + i
         {
             const int sh = ((wv & 1) << 4) + 4 * g;  // features fr..fr+3 in word wv/2
@@ -854,9 +856,9 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
             for (int i = 0; i < 4; ++i) h[i] = (m0 >> (nt * 4 + i)) & 1u ? acc[nt][i] + b0v[i] : 0.f;
             st_hi_fm(sm.A1h, sm.H0f, 16 * nt + r16, fr, lane, h);
         }
-        stage_b(next);  // the next group's records were loaded during the last group
+        if (pbase >= 0) finish(cb ^ 1, pbase);
         __syncthreads();
-        // ---- B. H1^T slice = W1^T H0^T + b1 under the mask -> F2 (dW2 only);
+        // ---- 2. B. H1^T slice = W1^T H0^T + b1 under the mask -> F2 (dW2 only);
         //         g_H0^T tile = W1[rows] g_H1^T under the mask -> A2h, A2l, F3;
         //         dW1 += H0^T g_H1; the next group's records are parked ----
         // (g_H0 in 32 features x 16 samples per wave: half the g_H1 reads of 16 x 32)
@@ -914,49 +916,12 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
             for (int i = 0; i < 4; ++i) h[i] = (m1 >> (nt * 4 + i)) & 1u ? acc[nt][i] + b1v[i] : 0.f;
             st_fm(sm.F2, 16 * nt + r16, fr, lane, h);
         }
-        park(cb ^ 1);  // the next group's stage-B loads (issued at stage A) have landed
+        park(cb ^ 1);  // the next group's records, loaded at this group's start
         __syncthreads();
-        // ---- C. dW2 += H1^T g_L; dW0 columns of this wave; g_X^T (waves 0-1) ->
-        //         record gradients; the next group's encoding (waves 2-5) ----
-        // dW2[in = f0 + row][out] += sum over the group's samples H1[s][in] gL[s][out]
-        dW2 = mfma16x16x32(ld_frag(&sm.F2[(f0 + r16) * FM + 8 * g]),
-                           ld_frag(&sm.GLf[cb][r16 * FM + 8 * g]), dW2);
-        {
-            const bf16x8 bh = ld_frag(&sm.F3[(f0 + r16) * FM + 8 * g]);
-#pragma unroll
-            for (int mi = 0; mi < 3; ++mi)
-                dW0[mi] = mfma16x16x32(ld_frag(&sm.Xf[cb][(16 * mi + r16) * FM + 8 * g]), bh, dW0[mi]);
-        }
-        if (wv < NT) {
-            const int nt = wv;
-            f32x4 gx = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-            for (int kt = 0; kt < 4; ++kt) {
-                const int go = (16 * nt + r16) * SM + 32 * kt + 8 * g;
-                gx = mfma_w(as_bf8(sm.Bw0[kt * 64 + lane]), ld_frag(&sm.A2h[go]),
-                            ld_frag(&sm.A2l[go]), gx);
-            }
-            // lane (g, r16): features 4g..4g+3 of sample 16 nt + r16 = part g of its
-            // record's gradient; part 3 carries dL/dsigma
-            const int sidx = 16 * nt + r16;
-            const int64_t q = base + sidx;
-            if (q < total) {
-                const uint32_t bits = sm.bits[cb][sidx];
-                float4 v = z4;
-                if (g < 3) {
-                    if (bits & kAliveW) v = make_float4(gx[0], gx[1], gx[2], gx[3]);
-                } else {
-                    v.x = sm.gsig[cb][sidx];
-                }
-                rec.grad[4 * q + g] = v;
-            }
-        } else if (wv < 6) {
-            encode(cb ^ 1);
-        }
-        stage_a(next + gridDim.x);  // the records of the group after next
-        __syncthreads();
+        pbase = base;
         cb ^= 1;
     }
+    if (pbase >= 0) finish(cb ^ 1, pbase);  // the loop ended on a barrier
 
     // ---- flush this workgroup's dW partial ----
     float* out = dw_partial + (int64_t)blockIdx.x * kPartStride;

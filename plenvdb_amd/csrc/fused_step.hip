@@ -52,7 +52,7 @@ inline bool aligned(const void* p, uintptr_t a) { return (reinterpret_cast<uintp
 
 struct Layout {
     size_t count, offset, total, t_last, ray_out, ray_grad, ray_ctot, ray_loss, ray_rgb, loss;
-    size_t xyze, ab, rgb, adj, feat, relu, grad;
+    size_t xyze, ab, rgb, adj, xrow, relu, grad;
     size_t touched, dw_part, w0t, w1t, w2t, w2, w1, w0;
     size_t bytes;
 };
@@ -81,7 +81,7 @@ Layout layout_for(int64_t n_rays, int64_t capacity, int64_t nleaf) {
     L.ab = take(8 * capacity);
     L.rgb = take(16 * capacity);
     L.adj = take(16 * capacity);
-    L.feat = take(48 * capacity);
+    L.xrow = take(80 * capacity);
     L.relu = take(32 * capacity);
     L.grad = take(64 * capacity);
     L.touched = take(nleaf > 0 ? nleaf : 1);
@@ -465,7 +465,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     rec.ab = reinterpret_cast<float2*>(ws + L.ab);
     rec.rgb = reinterpret_cast<float4*>(ws + L.rgb);
     rec.adj = reinterpret_cast<float4*>(ws + L.adj);
-    rec.feat = reinterpret_cast<uint4*>(ws + L.feat);
+    rec.xrow = reinterpret_cast<uint4*>(ws + L.xrow);
     rec.relu = reinterpret_cast<uint32_t*>(ws + L.relu);
     rec.grad = reinterpret_cast<float4*>(ws + L.grad);
     uint8_t* touched = ws + L.touched;
