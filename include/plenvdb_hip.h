@@ -283,7 +283,12 @@ typedef struct pvdb_step_desc {
      * (clamped to [1e-6, 1 - 1e-6], mean over rays) and the per-sample rgb
      * loss sum(w.detach() * |rgb_i - target|^2) / N; 0 switches a term off */
     float weight_main, weight_entropy_last, weight_rgbper;
-    int32_t reserved2;
+    /* 0: grad_color is [slots][C] and grad_density [slots].  Otherwise both are
+     * rows grad_stride floats apart (C <= grad_stride, a multiple of 4 for
+     * C = 12), and grad_density may point inside grad_color's rows (e.g.
+     * grad_color + C): a voxel's gradients then share one cache line, and the
+     * step's atomics for it are one memory request */
+    int32_t grad_stride;
     /* dev [slots][12] fp32 or NULL: with a bf16 colour payload, the fp32
      * master the colour Adam updates; the payload then receives its
      * round-to-nearest bf16 value after each update (NULL: the bf16 payload is

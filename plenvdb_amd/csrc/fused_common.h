@@ -63,6 +63,18 @@ struct StepParams {
     int32_t ray_waves;         // rays (wavefronts) per workgroup of the wave-per-ray kernels
 };
 
+// The grid gradients the scatter adds into and the sparse Adam consumes:
+// voxel s's density gradient at density[s * ds], its colour gradient row at
+// color[s * cs .. + NC).  Separate arrays have ds = 1 and cs = NC; rows of one
+// buffer (ds = cs = the row length, the density inside the row) put a voxel's
+// gradients in one cache line, so the scatter's atomics for a voxel are one
+// memory request instead of two or three.
+struct GradRows {
+    float* density;
+    float* color;
+    int32_t ds, cs;
+};
+
 // Rays per workgroup of the wave-per-ray kernels (k_ray_count, k_ray_march and
 // the composites) at most; the launch chooses 1, 2, 4 or 8.
 constexpr int kMaxRayWaves = 8;

@@ -29,9 +29,8 @@ namespace {
 // direct-colour grid's 3).
 template <int NC>
 __global__ void __launch_bounds__(256) k_scatter_direct(TopoView topo, RayBuffers R, Records rec,
-                                                 int64_t capacity, float* __restrict__ grad_density,
-                                                 float* __restrict__ grad_color,
-                                                 uint8_t* __restrict__ leaf_touched) {
+                                                        int64_t capacity, GradRows G,
+                                                        uint8_t* __restrict__ leaf_touched) {
     const int64_t total = min((int64_t)R.total[0], capacity);
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total * 8;
          t += (int64_t)gridDim.x * blockDim.x) {
@@ -51,10 +50,10 @@ __global__ void __launch_bounds__(256) k_scatter_direct(TopoView topo, RayBuffer
         const int64_t slot = (int64_t)leaf * 512 + voxel_offset(i, j, k);
         leaf_touched[leaf] = 1;
         const float* g = reinterpret_cast<const float*>(rec.grad + 4 * q);
-        if (bits & kAliveA) atomicAdd(grad_density + slot, ((g[kSigmaAt<NC>] * a) * b) * d);
+        if (bits & kAliveA) atomicAdd(G.density + slot * G.ds, ((g[kSigmaAt<NC>] * a) * b) * d);
         if (bits & kAliveW) {
             const float sc = (a * b) * d;
-            float* dst = grad_color + slot * NC;
+            float* dst = G.color + slot * G.cs;
 #pragma unroll
             for (int c = 0; c < NC; ++c) atomicAdd(dst + c, g[c] * sc);
         }
@@ -85,9 +84,7 @@ constexpr uint32_t kEnd = 0xffffu;  // end of a pair list
 // scatter's atomics rather than following them in a launch of their own.
 template <int kAggRecords, int NC>
 __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R, Records rec,
-                                                     int64_t capacity,
-                                                     float* __restrict__ grad_density,
-                                                     float* __restrict__ grad_color,
+                                                     int64_t capacity, GradRows G,
                                                      uint8_t* __restrict__ leaf_touched,
                                                      ReduceArgs red) {
     constexpr int kAggPairsAll = 8 * kAggRecords;  // (record, corner) pairs of a chunk
@@ -207,9 +204,9 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
                 if (c < kRow) sum += pv[p * kRow + c];
             if (c < kRow && sum != 0.f) {  // adding zero changes nothing
                 if (c < NC)
-                    atomicAdd(grad_color + (int64_t)s * NC + c, sum);
+                    atomicAdd(G.color + (int64_t)s * G.cs + c, sum);
                 else
-                    atomicAdd(grad_density + s, sum);
+                    atomicAdd(G.density + (int64_t)s * G.ds, sum);
             }
             if (c == kTouchLane) leaf_touched[s >> 9] = 1;
         }
@@ -232,7 +229,7 @@ int launch_dw_reduce(const float* part, int nparts, float* dw, const MlpAdam& ad
 
 template <int NC>
 int launch_scatter_nc(const TopoView& t, const RayBuffers& R, const Records& rec, int64_t capacity,
-                      float* grad_density, float* grad_color, uint8_t* leaf_touched, int grid_req,
+                      const GradRows& G, uint8_t* leaf_touched, int grid_req,
                       int chunk_records, const ReduceArgs& red_in, hipStream_t s) {
     if (t.nleaf == 0)
         return red_in.blocks ? launch_dw_reduce(red_in.part, red_in.nparts, red_in.dw, red_in.adam, s)
@@ -249,32 +246,31 @@ int launch_scatter_nc(const TopoView& t, const RayBuffers& R, const Records& rec
                               (unsigned)red.blocks;
         if (chunk_records == 64)
             hipLaunchKernelGGL((k_scatter_agg<64, NC>), dim3(grid), dim3(256), 0, s, t, R, rec,
-                               capacity, grad_density, grad_color, leaf_touched, red);
+                               capacity, G, leaf_touched, red);
         else
             hipLaunchKernelGGL((k_scatter_agg<32, NC>), dim3(grid), dim3(256), 0, s, t, R, rec,
-                               capacity, grad_density, grad_color, leaf_touched, red);
+                               capacity, G, leaf_touched, red);
         PVDB_LAUNCHED("k_scatter_agg");
         return PVDB_OK;
     }
     // enough threads for a few (record, corner) pairs each at the largest sizes
     const int64_t want = (capacity * 8 + 255) / 256;
     const unsigned grid = (unsigned)(want < 16384 ? (want < 1 ? 1 : want) : 16384);
-    hipLaunchKernelGGL(k_scatter_direct<NC>, dim3(grid), dim3(256), 0, s, t, R, rec, capacity,
-                       grad_density, grad_color, leaf_touched);
+    hipLaunchKernelGGL(k_scatter_direct<NC>, dim3(grid), dim3(256), 0, s, t, R, rec, capacity, G,
+                       leaf_touched);
     PVDB_LAUNCHED("k_scatter_direct");
     return red.blocks ? launch_dw_reduce(red.part, red.nparts, red.dw, red.adam, s) : PVDB_OK;
 }
 
 // nc: colour channels per slot, 12 (the MLP's features) or 3 (a direct-colour grid)
 int launch_scatter(int nc, const TopoView& t, const RayBuffers& R, const Records& rec,
-                   int64_t capacity, float* grad_density, float* grad_color,
-                   uint8_t* leaf_touched, int grid_req, int chunk_records, const ReduceArgs& red,
-                   hipStream_t s) {
+                   int64_t capacity, const GradRows& G, uint8_t* leaf_touched, int grid_req,
+                   int chunk_records, const ReduceArgs& red, hipStream_t s) {
     PVDB_CHECK_ARG(nc == 3 || nc == 12, "colour channels must be 3 or 12");
-    return nc == 12 ? launch_scatter_nc<12>(t, R, rec, capacity, grad_density, grad_color,
-                                            leaf_touched, grid_req, chunk_records, red, s)
-                    : launch_scatter_nc<3>(t, R, rec, capacity, grad_density, grad_color,
-                                           leaf_touched, grid_req, chunk_records, red, s);
+    return nc == 12 ? launch_scatter_nc<12>(t, R, rec, capacity, G, leaf_touched, grid_req,
+                                            chunk_records, red, s)
+                    : launch_scatter_nc<3>(t, R, rec, capacity, G, leaf_touched, grid_req,
+                                           chunk_records, red, s);
 }
 
 }  // namespace pvdb

@@ -29,9 +29,8 @@ int launch_mlp(bool bwd, bool bf16col, int grid, const StepParams& P, const Topo
 int launch_dw_reduce(const float* part, int nparts, float* dw, const MlpAdam& adam,
                      hipStream_t s);
 int launch_scatter(int nc, const TopoView& t, const RayBuffers& R, const Records& rec,
-                   int64_t capacity, float* grad_density, float* grad_color,
-                   uint8_t* leaf_touched, int grid_req, int chunk_records, const ReduceArgs& red,
-                   hipStream_t s);
+                   int64_t capacity, const GradRows& G, uint8_t* leaf_touched, int grid_req,
+                   int chunk_records, const ReduceArgs& red, hipStream_t s);
 }  // namespace pvdb
 
 using namespace pvdb;
@@ -110,11 +109,13 @@ __device__ __forceinline__ void adam1(float& p, float g, float& m, float& v, flo
     p = p - (sz * nm) / (eps + sqrtf(nv));
 }
 
-// A row of NC fp32 values of slot s (16-byte vectors when NC is a multiple of 4).
+// A row of NC fp32 values of slot s, rows `stride` floats apart (16-byte
+// vectors when NC is a multiple of 4; the stride is then one too).
 template <int NC>
-__device__ __forceinline__ void ld_row(const float* base, int64_t s, float (&v)[NC]) {
+__device__ __forceinline__ void ld_row(const float* base, int64_t s, float (&v)[NC],
+                                       int64_t stride = NC) {
     if constexpr (NC % 4 == 0) {
-        const float4* p = reinterpret_cast<const float4*>(base) + s * (NC / 4);
+        const float4* p = reinterpret_cast<const float4*>(base + s * stride);
 #pragma unroll
         for (int k = 0; k < NC / 4; ++k) {
             const float4 q = p[k];
@@ -125,20 +126,21 @@ __device__ __forceinline__ void ld_row(const float* base, int64_t s, float (&v)[
         }
     } else {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) v[c] = base[s * NC + c];
+        for (int c = 0; c < NC; ++c) v[c] = base[s * stride + c];
     }
 }
 
 template <int NC>
-__device__ __forceinline__ void st_row(float* base, int64_t s, const float (&v)[NC]) {
+__device__ __forceinline__ void st_row(float* base, int64_t s, const float (&v)[NC],
+                                       int64_t stride = NC) {
     if constexpr (NC % 4 == 0) {
-        float4* p = reinterpret_cast<float4*>(base) + s * (NC / 4);
+        float4* p = reinterpret_cast<float4*>(base + s * stride);
 #pragma unroll
         for (int k = 0; k < NC / 4; ++k)
             p[k] = make_float4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]);
     } else {
 #pragma unroll
-        for (int c = 0; c < NC; ++c) base[s * NC + c] = v[c];
+        for (int c = 0; c < NC; ++c) base[s * stride + c] = v[c];
     }
 }
 
@@ -163,12 +165,10 @@ struct GridAdamModes {
 
 template <int NC, bool BF16COL>
 __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __restrict__ touched,
-                                                     float* __restrict__ den,
-                                                     float* __restrict__ gden,
+                                                     float* __restrict__ den, GradRows G,
                                                      float* __restrict__ mden,
                                                      float* __restrict__ vden, void* col,
                                                      float* __restrict__ cmaster,
-                                                     float* __restrict__ gcol,
                                                      float* __restrict__ mcol,
                                                      float* __restrict__ vcol, GridAdamModes md_,
                                                      float sz_d_arg, float sz_c_arg,
@@ -187,9 +187,9 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
                 // every operand of an active voxel is loaded at once (one round
                 // trip; a voxel whose gradients are all zero writes nothing back
                 // in stepmode 1)
-                float gd = gden[s], pd = den[s], md = mden[s], vd = vden[s];
+                float gd = G.density[s * G.ds], pd = den[s], md = mden[s], vd = vden[s];
                 float gg[NC], mm[NC], vv[NC], pp[NC];
-                ld_row<NC>(gcol, s, gg);
+                ld_row<NC>(G.color, s, gg, G.cs);
                 ld_row<NC>(mcol, s, mm);
                 ld_row<NC>(vcol, s, vv);
                 if constexpr (BF16COL) {
@@ -241,11 +241,11 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
                     }
                 }
             }
-            gden[s] = 0.f;
+            G.density[s * G.ds] = 0.f;
             float zr[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) zr[c] = 0.f;
-            st_row<NC>(gcol, s, zr);
+            st_row<NC>(G.color, s, zr, G.cs);
         }
         __syncthreads();
         if (threadIdx.x == 0) touched[leaf] = 0;
@@ -283,16 +283,30 @@ struct StageScope {
     }
 };
 
+// Colour channels per slot of the step's model (12 features, or k0_dim).
+inline int slot_channels(const pvdb_step_desc* d) { return d->k0_dim == 3 ? 3 : 12; }
+
+// The gradient arrays as rows: grad_stride 0 means separate arrays.
+inline GradRows grad_rows_of(const pvdb_step_desc* d) {
+    GradRows G;
+    G.density = d->grad_density;
+    G.color = d->grad_color;
+    G.ds = d->grad_stride > 0 ? d->grad_stride : 1;
+    G.cs = d->grad_stride > 0 ? d->grad_stride : slot_channels(d);
+    return G;
+}
+
 // VDB Adam of the grids (over the touched leaves when both run stepmode 1).
 int launch_grid_adam(const pvdb_step_desc* d, const TopoView& t, uint8_t* touched, hipStream_t s) {
     const int ag = (int)std::min<int64_t>(t.nleaf > 0 ? t.nleaf : 1, 4096);
     float* master = d->color_bf16 ? d->color_master : nullptr;
     const GridAdamModes md{d->stepmode_density, d->stepmode_color, d->density_lr};
+    const GradRows G = grad_rows_of(d);
 #define PVDB_ADAM(NC, BF)                                                                        \
     hipLaunchKernelGGL((k_sparse_adam<NC, BF>), dim3(ag), dim3(256), 0, s, t, touched,          \
-                       d->density, d->grad_density, d->density_m, d->density_v, d->color,       \
-                       master, d->grad_color, d->color_m, d->color_v, md, d->stepsz_density,    \
-                       d->stepsz_color, d->adam_scalars, d->eps, d->beta0, d->beta1)
+                       d->density, G, d->density_m, d->density_v, d->color, master, d->color_m, \
+                       d->color_v, md, d->stepsz_density, d->stepsz_color, d->adam_scalars,     \
+                       d->eps, d->beta0, d->beta1)
     if (d->k0_dim == 3)
         PVDB_ADAM(3, false);
     else if (d->color_bf16)
@@ -391,6 +405,11 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
                    "stepmode_density must be 0, 1 or 2 and stepmode_color 0 or 1");
     PVDB_CHECK_ARG(d->stepmode_density != 2 || d->density_lr,
                    "density stepmode 2 needs density_lr");
+    PVDB_CHECK_ARG(d->grad_stride == 0 ||
+                       (d->grad_stride >= slot_channels(d) &&
+                        (slot_channels(d) % 4 != 0 || d->grad_stride % 4 == 0)),
+                   "grad_stride must be 0, or at least the colour channels (a multiple of 4 "
+                   "for 12 channels)");
     PVDB_CHECK_ARG(d->stepdist > 0.f, "stepdist must be positive");
     PVDB_CHECK_ARG(d->launch[0] >= 0 && d->launch[0] <= 4096 && d->launch[1] >= 0 &&
                        d->launch[1] <= 65536,
@@ -523,9 +542,8 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
             StageScope sc(6, s);
             const MlpAdam none{nullptr, nullptr, nullptr, nullptr, 0.f, 0.f, 0.f, 0.f};
             const ReduceArgs red{nullptr, 0, nullptr, none, 0};
-            if ((rc = launch_scatter(d->k0_dim == 3 ? 3 : 12, t, R, rec, d->capacity,
-                                     d->grad_density, d->grad_color, touched,
-                                     d->launch[1] > 0 ? d->launch[1] : kScatterGrid,
+            if ((rc = launch_scatter(slot_channels(d), t, R, rec, d->capacity, grad_rows_of(d),
+                                     touched, d->launch[1] > 0 ? d->launch[1] : kScatterGrid,
                                      d->launch[2] > 0 ? d->launch[2] : kScatterChunk, red, s)))
                 return rc;
         }
@@ -577,8 +595,8 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
             adam = MlpAdam{d->mlp,           d->mlp_m, d->mlp_v, d->adam_scalars,
                            d->step_size_mlp, d->eps,   d->beta0, d->beta1};
         const ReduceArgs red{dw_part, grid_bwd, d->mlp_grad, adam, kReduceBlocks};
-        if ((rc = launch_scatter(12, t, R, rec, d->capacity, d->grad_density, d->grad_color,
-                                 touched, d->launch[1] > 0 ? d->launch[1] : kScatterGrid,
+        if ((rc = launch_scatter(12, t, R, rec, d->capacity, grad_rows_of(d), touched,
+                                 d->launch[1] > 0 ? d->launch[1] : kScatterGrid,
                                  d->launch[2] > 0 ? d->launch[2] : kScatterChunk, red, s)))
             return rc;
     }

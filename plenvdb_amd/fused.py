@@ -41,7 +41,7 @@ class StepDesc(C.Structure):
         ("ray_out", C.c_void_p), ("loss", C.c_void_p), ("adam_scalars", C.c_void_p),
         ("launch", C.c_int32 * 4),
         ("weight_main", C.c_float), ("weight_entropy_last", C.c_float),
-        ("weight_rgbper", C.c_float), ("reserved2", C.c_int32),
+        ("weight_rgbper", C.c_float), ("grad_stride", C.c_int32),
         ("color_master", C.c_void_p),
         ("density_lr", C.c_void_p), ("stepmode_density", C.c_int32), ("stepmode_color", C.c_int32),
     ]
@@ -132,8 +132,12 @@ class FusedTrainer:
         self.color = colt.to(torch.bfloat16).contiguous() if self.color_bf16 else colt
         # the bf16 payload's fp32 master, which the colour Adam updates
         self.color_master = colt if (self.color_bf16 and color_master) else None
-        self.grad_density = torch.zeros(ns, device=dev)
-        self.grad_color = torch.zeros(ns, nc, device=dev)
+        # the gradients as one row per slot (colour, then density, padded to 16
+        # bytes): a voxel's gradients share a cache line for the step's atomics
+        self.grad_row = 16 if nc == 12 else 4
+        self.grad_rows = torch.zeros(ns, self.grad_row, device=dev)
+        self.grad_color = self.grad_rows[:, :nc]
+        self.grad_density = self.grad_rows[:, nc]
         self.m_density = torch.zeros(ns, device=dev)
         self.v_density = torch.zeros(ns, device=dev)
         self.m_color = torch.zeros(ns, nc, device=dev)
@@ -204,6 +208,7 @@ class FusedTrainer:
         d.fast_color_thres = float(sc["fast_color_thres"])
         d.bg = float(sc["bg"])
         d.density, d.grad_density = self.density.data_ptr(), self.grad_density.data_ptr()
+        d.grad_stride = self.grad_row
         d.density_m, d.density_v = self.m_density.data_ptr(), self.v_density.data_ptr()
         d.color, d.grad_color = self.color.data_ptr(), self.grad_color.data_ptr()
         d.color_m, d.color_v = self.m_color.data_ptr(), self.v_color.data_ptr()
@@ -417,8 +422,7 @@ class FusedTrainer:
         return int(ck["global_step"])
 
     def zero_grad(self):
-        self.grad_density.zero_()
-        self.grad_color.zero_()
+        self.grad_rows.zero_()
         if self.mlp_grad is not None:
             self.mlp_grad.zero_()
 

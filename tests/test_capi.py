@@ -127,6 +127,9 @@ def test_fused_step_rejects_capacity_and_misaligned_buffers():
         (dict(stepmode_density=3), "stepmode"),
         (dict(stepmode_color=2), "stepmode"),
         (dict(stepmode_density=2), "density_lr"),
+        (dict(grad_stride=8), "grad_stride"),
+        (dict(grad_stride=14), "grad_stride"),
+        (dict(k0_dim=3, grad_stride=2), "grad_stride"),
     ]
     for kw, msg in cases:
         with pytest.raises(RuntimeError, match=msg):

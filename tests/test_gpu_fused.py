@@ -246,3 +246,48 @@ def test_fused_edge_rays_vs_fp64_oracle(oracle_mod):
     print(f"[result] rgb {err:.2e} grad density {e_den:.2e} colour {e_col:.2e} mlp {e_mlp:.2e}",
           flush=True)
     assert err <= 1e-3 and e_den <= 1e-3 and e_col <= 1e-3 and e_mlp <= 1e-3
+
+
+@pytest.mark.parametrize("name,channels", [("c2", None), ("c1", 3)])
+def test_fused_gradient_layouts_agree(name, channels):
+    """The step's gradients as rows of one buffer (colour, density, padding:
+    FusedTrainer's layout) equal those in separate [slots][C] and [slots]
+    arrays (grad_stride 0), up to the order of the scatter's float atomics;
+    the Adam step that follows is the same."""
+    from plenvdb_amd import scenes
+    from plenvdb_amd.fused import FusedTrainer
+
+    print("[case] gradient layouts", name, channels, flush=True)
+    sc = scenes.make_scene(name, n_rays=1024, channels=channels)
+    tr = FusedTrainer(sc)
+    tr.forward_backward()
+    torch.cuda.synchronize()
+    rows = (tr.grad_density.clone(), tr.grad_color.clone())
+    tr.zero_grad()
+    gd = torch.zeros_like(rows[0])
+    gc = torch.zeros_like(rows[1])
+    tr.desc.grad_density, tr.desc.grad_color, tr.desc.grad_stride = gd.data_ptr(), gc.data_ptr(), 0
+    tr.forward_backward()
+    torch.cuda.synchronize()
+    assert float(tr.grad_rows.abs().max()) == 0.0  # nothing went to the rows
+    for a, b in ((gd, rows[0]), (gc, rows[1])):
+        scale = max(float(b.abs().max()), 1e-30)
+        assert float(b.abs().max()) > 0
+        assert float((a - b).abs().max()) <= 1e-6 * scale
+    # Adam reads either layout: fed the same gradient values, the two leave
+    # the same grids, bit for bit (the twin's own backward sets its touched
+    # leaves, then its rows take the first run's values)
+    twin = FusedTrainer(sc)
+    twin.forward_backward()
+    torch.cuda.synchronize()
+    twin.grad_density.copy_(rows[0])
+    twin.grad_color.copy_(rows[1])
+    gd.copy_(rows[0])
+    gc.copy_(rows[1])
+    tr.adam_only()
+    twin.adam_only()
+    torch.cuda.synchronize()
+    assert float(gd.abs().max()) == 0.0 and float(gc.abs().max()) == 0.0
+    assert float(twin.grad_rows.abs().max()) == 0.0
+    assert torch.equal(tr.density, twin.density) and torch.equal(tr.color, twin.color)
+    assert torch.equal(tr.m_color, twin.m_color) and torch.equal(tr.v_density, twin.v_density)
