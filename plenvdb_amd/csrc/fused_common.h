@@ -73,6 +73,8 @@ struct GradRows {
     float* density;
     float* color;
     int32_t ds, cs;
+    bool whole;  // rows of 16 (4) floats with the density right after the 12 (3)
+                 // colour values: a row is moved as whole 16-byte vectors
 };
 
 // Rays per workgroup of the wave-per-ray kernels (k_ray_count, k_ray_march and

@@ -187,9 +187,26 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
                 // every operand of an active voxel is loaded at once (one round
                 // trip; a voxel whose gradients are all zero writes nothing back
                 // in stepmode 1)
-                float gd = G.density[s * G.ds], pd = den[s], md = mden[s], vd = vden[s];
+                float gd, pd = den[s], md = mden[s], vd = vden[s];
                 float gg[NC], mm[NC], vv[NC], pp[NC];
-                ld_row<NC>(G.color, s, gg, G.cs);
+                if (G.whole) {  // the row's vectors: colour, then the density
+                    float r[NC + 1 + (NC == 12 ? 3 : 0)];
+                    const float4* p = reinterpret_cast<const float4*>(G.color + s * G.cs);
+#pragma unroll
+                    for (int k = 0; k < (NC + 1 + 3) / 4; ++k) {
+                        const float4 q = p[k];
+                        r[4 * k] = q.x;
+                        r[4 * k + 1] = q.y;
+                        r[4 * k + 2] = q.z;
+                        r[4 * k + 3] = q.w;
+                    }
+#pragma unroll
+                    for (int c = 0; c < NC; ++c) gg[c] = r[c];
+                    gd = r[NC];
+                } else {
+                    gd = G.density[s * G.ds];
+                    ld_row<NC>(G.color, s, gg, G.cs);
+                }
                 ld_row<NC>(mcol, s, mm);
                 ld_row<NC>(vcol, s, vv);
                 if constexpr (BF16COL) {
@@ -241,11 +258,17 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
                     }
                 }
             }
-            G.density[s * G.ds] = 0.f;
-            float zr[NC];
+            if (G.whole) {  // whole-line stores
+                float4* p = reinterpret_cast<float4*>(G.color + s * G.cs);
 #pragma unroll
-            for (int c = 0; c < NC; ++c) zr[c] = 0.f;
-            st_row<NC>(G.color, s, zr, G.cs);
+                for (int k = 0; k < (NC + 1 + 3) / 4; ++k) p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            } else {
+                G.density[s * G.ds] = 0.f;
+                float zr[NC];
+#pragma unroll
+                for (int c = 0; c < NC; ++c) zr[c] = 0.f;
+                st_row<NC>(G.color, s, zr, G.cs);
+            }
         }
         __syncthreads();
         if (threadIdx.x == 0) touched[leaf] = 0;
@@ -293,6 +316,9 @@ inline GradRows grad_rows_of(const pvdb_step_desc* d) {
     G.color = d->grad_color;
     G.ds = d->grad_stride > 0 ? d->grad_stride : 1;
     G.cs = d->grad_stride > 0 ? d->grad_stride : slot_channels(d);
+    const int nc = slot_channels(d);
+    G.whole = d->grad_stride == (nc == 12 ? 16 : 4) && d->grad_density == d->grad_color + nc &&
+              aligned(d->grad_color, 16);
     return G;
 }
 
