@@ -178,98 +178,117 @@ __global__ void __launch_bounds__(256) k_sparse_adam(TopoView t, uint8_t* __rest
     const float sz_d = dev_sz ? dev_sz[0] : sz_d_arg;
     const float sz_c = dev_sz ? dev_sz[1] : sz_c_arg;
     const bool every = md_.density != 1 || md_.color != 1;
-    for (int64_t leaf = blockIdx.x; leaf < t.nleaf; leaf += gridDim.x) {
-        if (!every && !touched[leaf]) continue;  // uniform over the workgroup
-        for (int k = threadIdx.x; k < 512; k += blockDim.x) {
-            const int64_t s = leaf * 512 + k;
-            const bool act = (t.masks[leaf * 8 + (k >> 6)] >> (k & 63)) & 1ull;
-            if (act) {
-                // every operand of an active voxel is loaded at once (one round
-                // trip; a voxel whose gradients are all zero writes nothing back
-                // in stepmode 1)
-                float gd, pd = den[s], md = mden[s], vd = vden[s];
-                float gg[NC], mm[NC], vv[NC], pp[NC];
-                if (G.whole) {  // the row's vectors: colour, then the density
-                    float r[NC + 1 + (NC == 12 ? 3 : 0)];
-                    const float4* p = reinterpret_cast<const float4*>(G.color + s * G.cs);
+    // a voxel's operands, loaded for both of a thread's voxels before either
+    // is updated (two round trips in flight per thread)
+    struct Vox {
+        float gd, pd, md, vd;
+        float gg[NC], mm[NC], vv[NC], pp[NC];
+    };
+    auto load = [&](int64_t s, Vox& v) {
+        v.pd = den[s];
+        v.md = mden[s];
+        v.vd = vden[s];
+        if (G.whole) {  // the row's vectors: colour, then the density
+            float r[NC + 1 + (NC == 12 ? 3 : 0)];
+            const float4* p = reinterpret_cast<const float4*>(G.color + s * G.cs);
 #pragma unroll
-                    for (int k = 0; k < (NC + 1 + 3) / 4; ++k) {
-                        const float4 q = p[k];
-                        r[4 * k] = q.x;
-                        r[4 * k + 1] = q.y;
-                        r[4 * k + 2] = q.z;
-                        r[4 * k + 3] = q.w;
-                    }
-#pragma unroll
-                    for (int c = 0; c < NC; ++c) gg[c] = r[c];
-                    gd = r[NC];
-                } else {
-                    gd = G.density[s * G.ds];
-                    ld_row<NC>(G.color, s, gg, G.cs);
-                }
-                ld_row<NC>(mcol, s, mm);
-                ld_row<NC>(vcol, s, vv);
-                if constexpr (BF16COL) {
-                    if (!cmaster) {
-                        const uint2* c2 = reinterpret_cast<const uint2*>(col) + s * 3;
-#pragma unroll
-                        for (int k4 = 0; k4 < 3; ++k4) {
-                            const uint2 q = c2[k4];
-                            pp[4 * k4] = bf_bits2f((uint16_t)(q.x & 0xffffu));
-                            pp[4 * k4 + 1] = bf_bits2f((uint16_t)(q.x >> 16));
-                            pp[4 * k4 + 2] = bf_bits2f((uint16_t)(q.y & 0xffffu));
-                            pp[4 * k4 + 3] = bf_bits2f((uint16_t)(q.y >> 16));
-                        }
-                    } else {
-                        ld_row<NC>(cmaster, s, pp);
-                    }
-                } else {
-                    ld_row<NC>(reinterpret_cast<const float*>(col), s, pp);
-                }
-                if (md_.density != 1 || gd != 0.f) {
-                    const float sz = md_.density == 2 ? md_.rate[s] * sz_d : sz_d;
-                    adam1(pd, gd, md, vd, sz, eps, b0, b1);
-                    den[s] = pd;
-                    mden[s] = md;
-                    vden[s] = vd;
-                }
-                bool any = false;
-#pragma unroll
-                for (int grp = 0; grp < NC / 3; ++grp) {
-                    const int c0 = 3 * grp;
-                    if (md_.color == 1 && gg[c0] == 0.f && gg[c0 + 1] == 0.f && gg[c0 + 2] == 0.f)
-                        continue;
-                    any = true;
-#pragma unroll
-                    for (int c = c0; c < c0 + 3; ++c) adam1(pp[c], gg[c], mm[c], vv[c], sz_c, eps, b0, b1);
-                }
-                if (any) {
-                    st_row<NC>(mcol, s, mm);
-                    st_row<NC>(vcol, s, vv);
-                    if constexpr (BF16COL) {
-                        uint2* c2 = reinterpret_cast<uint2*>(col) + s * 3;
-#pragma unroll
-                        for (int k4 = 0; k4 < 3; ++k4)
-                            c2[k4] = make_uint2((uint32_t)f2bf_bits(pp[4 * k4]) | ((uint32_t)f2bf_bits(pp[4 * k4 + 1]) << 16),
-                                                (uint32_t)f2bf_bits(pp[4 * k4 + 2]) | ((uint32_t)f2bf_bits(pp[4 * k4 + 3]) << 16));
-                        if (cmaster) st_row<NC>(cmaster, s, pp);
-                    } else {
-                        st_row<NC>(reinterpret_cast<float*>(col), s, pp);
-                    }
-                }
+            for (int kv = 0; kv < (NC + 1 + 3) / 4; ++kv) {
+                const float4 q = p[kv];
+                r[4 * kv] = q.x;
+                r[4 * kv + 1] = q.y;
+                r[4 * kv + 2] = q.z;
+                r[4 * kv + 3] = q.w;
             }
-            if (G.whole) {  // whole-line stores
-                float4* p = reinterpret_cast<float4*>(G.color + s * G.cs);
 #pragma unroll
-                for (int k = 0; k < (NC + 1 + 3) / 4; ++k) p[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int c = 0; c < NC; ++c) v.gg[c] = r[c];
+            v.gd = r[NC];
+        } else {
+            v.gd = G.density[s * G.ds];
+            ld_row<NC>(G.color, s, v.gg, G.cs);
+        }
+        ld_row<NC>(mcol, s, v.mm);
+        ld_row<NC>(vcol, s, v.vv);
+        if constexpr (BF16COL) {
+            if (!cmaster) {
+                const uint2* c2 = reinterpret_cast<const uint2*>(col) + s * 3;
+#pragma unroll
+                for (int k4 = 0; k4 < 3; ++k4) {
+                    const uint2 q = c2[k4];
+                    v.pp[4 * k4] = bf_bits2f((uint16_t)(q.x & 0xffffu));
+                    v.pp[4 * k4 + 1] = bf_bits2f((uint16_t)(q.x >> 16));
+                    v.pp[4 * k4 + 2] = bf_bits2f((uint16_t)(q.y & 0xffffu));
+                    v.pp[4 * k4 + 3] = bf_bits2f((uint16_t)(q.y >> 16));
+                }
             } else {
-                G.density[s * G.ds] = 0.f;
-                float zr[NC];
+                ld_row<NC>(cmaster, s, v.pp);
+            }
+        } else {
+            ld_row<NC>(reinterpret_cast<const float*>(col), s, v.pp);
+        }
+    };
+    // the update (a voxel whose gradients are all zero writes nothing back in
+    // stepmode 1)
+    auto update = [&](int64_t s, Vox& v) {
+        if (md_.density != 1 || v.gd != 0.f) {
+            const float sz = md_.density == 2 ? md_.rate[s] * sz_d : sz_d;
+            adam1(v.pd, v.gd, v.md, v.vd, sz, eps, b0, b1);
+            den[s] = v.pd;
+            mden[s] = v.md;
+            vden[s] = v.vd;
+        }
+        bool any = false;
 #pragma unroll
-                for (int c = 0; c < NC; ++c) zr[c] = 0.f;
-                st_row<NC>(G.color, s, zr, G.cs);
+        for (int grp = 0; grp < NC / 3; ++grp) {
+            const int c0 = 3 * grp;
+            if (md_.color == 1 && v.gg[c0] == 0.f && v.gg[c0 + 1] == 0.f && v.gg[c0 + 2] == 0.f)
+                continue;
+            any = true;
+#pragma unroll
+            for (int c = c0; c < c0 + 3; ++c)
+                adam1(v.pp[c], v.gg[c], v.mm[c], v.vv[c], sz_c, eps, b0, b1);
+        }
+        if (any) {
+            st_row<NC>(mcol, s, v.mm);
+            st_row<NC>(vcol, s, v.vv);
+            if constexpr (BF16COL) {
+                uint2* c2 = reinterpret_cast<uint2*>(col) + s * 3;
+#pragma unroll
+                for (int k4 = 0; k4 < 3; ++k4)
+                    c2[k4] = make_uint2((uint32_t)f2bf_bits(v.pp[4 * k4]) | ((uint32_t)f2bf_bits(v.pp[4 * k4 + 1]) << 16),
+                                        (uint32_t)f2bf_bits(v.pp[4 * k4 + 2]) | ((uint32_t)f2bf_bits(v.pp[4 * k4 + 3]) << 16));
+                if (cmaster) st_row<NC>(cmaster, s, v.pp);
+            } else {
+                st_row<NC>(reinterpret_cast<float*>(col), s, v.pp);
             }
         }
+    };
+    auto zero = [&](int64_t s) {
+        if (G.whole) {  // whole-line stores
+            float4* p = reinterpret_cast<float4*>(G.color + s * G.cs);
+#pragma unroll
+            for (int kv = 0; kv < (NC + 1 + 3) / 4; ++kv) p[kv] = make_float4(0.f, 0.f, 0.f, 0.f);
+        } else {
+            G.density[s * G.ds] = 0.f;
+            float zr[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) zr[c] = 0.f;
+            st_row<NC>(G.color, s, zr, G.cs);
+        }
+    };
+    for (int64_t leaf = blockIdx.x; leaf < t.nleaf; leaf += gridDim.x) {
+        if (!every && !touched[leaf]) continue;  // uniform over the workgroup
+        // voxels k and k + 256 of the leaf (a 256-thread workgroup)
+        const int k = threadIdx.x;
+        const int64_t s0 = leaf * 512 + k, s1 = s0 + 256;
+        const bool a0 = (t.masks[leaf * 8 + (k >> 6)] >> (k & 63)) & 1ull;
+        const bool a1 = (t.masks[leaf * 8 + 4 + (k >> 6)] >> (k & 63)) & 1ull;
+        Vox v0, v1;
+        if (a0) load(s0, v0);
+        if (a1) load(s1, v1);
+        if (a0) update(s0, v0);
+        if (a1) update(s1, v1);
+        zero(s0);
+        zero(s1);
         __syncthreads();
         if (threadIdx.x == 0) touched[leaf] = 0;
     }
