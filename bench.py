@@ -242,8 +242,8 @@ def bench_batched(args, scene):
 # x1.0); profiles/pmc_traffic_c2_train_c5_render.md holds the runs.
 TRAFFIC_SOURCE = "profiles/pmc_traffic_c2_train_c5_render.md"
 MEASURED_TRAFFIC_MB = {
-    "c2": {"mlp_bwd": 59.66, "scatter": 97.88, "mlp_fwd": 64.29, "ray_march": 14.53,
-           "composite": 15.28, "adam": 114.86, "ray_count": 2.38},
+    "c2": {"mlp_bwd": 63.73, "scatter": 85.94, "mlp_fwd": 69.57, "ray_march": 14.14,
+           "composite": 15.73, "adam": 122.02, "ray_count": 3.15},
     "c5": {"render_mlp": 656.89},
 }
 
