@@ -108,7 +108,13 @@ struct RayBuffers {
     float* ray_loss;   // [N] the ray's share of the loss (all weighted terms)
     float* ray_rgb;    // [N][NC + 1] direct-colour configurations: rendered rgb (NC) + T_last
     float* loss;       // [1]
+    uint16_t* steps;   // [N][kStepCache] the counting march's kept step indices, in step order
 };
+
+// Kept step indices the counting march stores per ray (16-bit: rays of fewer
+// than 65536 steps).  A ray with more kept steps than this, or more steps, is
+// marched again by the forward march.
+constexpr int kStepCache = 256;
 
 // Packed bf16 MLP weights for MFMA (row-major, zero padded):
 //   w0t [128][64]  W0^T (in 0..38 used)       w1t [128][128] W1^T

@@ -160,8 +160,10 @@ __device__ __forceinline__ int mbcnt(uint64_t m) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-// Counting march: occupancy-kept samples per ray.  The workgroups past the
-// rays' pack the step's MLP operands (independent work, one launch fewer).
+// Counting march: occupancy-kept samples per ray, and the first kStepCache
+// kept step indices of each ray of fewer than 65536 steps, for the forward
+// march.  The workgroups past the rays' pack the step's MLP operands
+// (independent work, one launch fewer).
 __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_count(StepParams P, TopoView t,
                                                                  RayBuffers R, PackArgs pk,
                                                                  int ray_blocks) {
@@ -175,9 +177,16 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_count(StepParams P, T
     const int lane = lane_id();
     const RaySetup s = ray_setup(P, r);
     int count = 0;
+    uint16_t* cache = R.steps + (int64_t)r * kStepCache;
+    const bool cached = s.n_steps < 65536;  // uniform over the wave
     for_windows(P, t, s, [&](auto step, auto n) {
         const StepPoint sp = step_point(P, t, s, step, n);
-        count += __popcll(__ballot(sp.keep));
+        const uint64_t km = __ballot(sp.keep);
+        if (cached && sp.keep) {
+            const int at = count + mbcnt(km);
+            if (at < kStepCache) cache[at] = (uint16_t)step;
+        }
+        count += __popcll(km);
     });
     if (lane == 0) R.count[r] = count;
 }
@@ -243,11 +252,12 @@ __global__ void __launch_bounds__(1024) k_scan(RayBuffers R, int n) {
     if (tid == 0) R.total[0] = (int)(carry < kSat ? carry : kSat);
 }
 
-// Forward march: one record per occupancy-kept sample.  The occupancy test
-// runs over windows of 64 consecutive steps; the kept step indices are queued
-// in LDS (in step order) and the sample work -- density gather, raw2alpha,
-// the transmittance scan, records -- runs on 64 queued samples at a time, so
-// lanes are not spent on rejected steps.
+// Forward march: one record per occupancy-kept sample.  The sample work --
+// density gather, raw2alpha, the transmittance scan, records -- runs on 64
+// kept steps at a time, in step order, so lanes are not spent on rejected
+// steps.  The kept step indices come from the counting march's per-ray store
+// when it holds them all; otherwise the occupancy test runs again over windows
+// of 64 consecutive steps and the kept indices are queued in LDS.
 __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_march(StepParams P, TopoView t,
                                                                  const float* density,
                                                                  RayBuffers R, Records rec) {
@@ -262,13 +272,13 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_march(StepParams P, T
     float Tc = 1.f;  // transmittance carried from batch to batch (FP32, as the reference's T_cum)
     bool stopped = false;
     int written = 0, queued = 0;
-    // sample work on queue[0 .. k-1] (k <= 64), in step order
-    auto flush = [&](int k) __attribute__((always_inline)) {
+    // sample work on the next k (<= 64) kept steps, lane l holding step i of
+    // the l-th, in step order
+    auto flush = [&](int k, int64_t i) __attribute__((always_inline)) {
         const bool mine = lane < k;
         float x = 0.f, y = 0.f, z = 0.f, e = 0.f, alpha = 0.f;
         const bool live = mine && !stopped;
         if (live) {
-            const int64_t i = queue[lane];
             const float dist = P.stepdist * (float)i;
             const float px = s.sx + s.ux * dist, py = s.sy + s.uy * dist, pz = s.sz + s.uz * dist;
             x = ((px - P.mn[0]) / P.len[0]) * P.rm1[0];
@@ -317,29 +327,38 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_march(StepParams P, T
             }
         }
     };
-    for_windows(P, t, s, [&](auto step, auto n) {
-        const StepPoint sp = step_point(P, t, s, step, n);
-        const uint64_t km = __ballot(sp.keep);
-        if (km == 0) return;
-        if (sp.keep) queue[queued + mbcnt(km)] = (int64_t)step;
-        queued += __popcll(km);
-        if (queued >= 64) {
+    if (cnt <= kStepCache && s.n_steps < 65536) {
+        // the counting march stored every kept step of this ray (uniform branch)
+        const uint16_t* cache = R.steps + (int64_t)r * kStepCache;
+        for (int b = 0; b < cnt; b += 64) {
+            const int k = min(64, cnt - b);
+            flush(k, lane < k ? (int64_t)cache[b + lane] : int64_t(0));
+        }
+    } else {
+        for_windows(P, t, s, [&](auto step, auto n) {
+            const StepPoint sp = step_point(P, t, s, step, n);
+            const uint64_t km = __ballot(sp.keep);
+            if (km == 0) return;
+            if (sp.keep) queue[queued + mbcnt(km)] = (int64_t)step;
+            queued += __popcll(km);
+            if (queued >= 64) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                flush(64, queue[lane]);
+                // the overflow (fewer than 64 entries) moves to the queue's front
+                const int64_t rest = lane + 64 < queued ? queue[lane + 64] : 0;
+                __builtin_amdgcn_wave_barrier();
+                if (lane + 64 < queued) queue[lane] = rest;
+                queued -= 64;
+            }
+        });
+        if (queued > 0) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            flush(64);
-            // the overflow (fewer than 64 entries) moves to the queue's front
-            const int64_t rest = lane + 64 < queued ? queue[lane + 64] : 0;
-            __builtin_amdgcn_wave_barrier();
-            if (lane + 64 < queued) queue[lane] = rest;
-            queued -= 64;
+            flush(queued, queue[lane]);
         }
-    });
-    if (queued > 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        flush(queued);
     }
     if (lane == 0) R.t_last[r] = Tc;
 }

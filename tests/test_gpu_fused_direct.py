@@ -70,6 +70,45 @@ def test_direct_fused_vs_fp64_oracle(oracle_mod, n_rays, channels, weights):
     assert e_den <= 1e-5 and e_col <= 1e-5
 
 
+def test_direct_long_rays_vs_fp64_oracle(oracle_mod):
+    """Rays with more occupancy-kept steps than the counting march stores per
+    ray (256 step indices), which the forward march marches again, beside rays
+    that read the stored indices, in one launch: a fully active c1 grid (3
+    channels) with each ray's far bound drawn between 40% and all of its path
+    through the box, so the kept counts spread from ~100 to ~390 steps.  Exact per-ray counts
+    and the FP32 bars of the direct path against the FP64 oracle."""
+    from plenvdb_amd import scenes
+    from plenvdb_amd.fused import FusedTrainer
+
+    print("[case] fused without MLP, long rays: c1/256, 3 channels, fully active grid",
+          flush=True)
+    n = 256
+    sc = scenes.make_scene("c1", n_rays=n, channels=3)
+    sc["mask"] = np.ones_like(sc["mask"])
+    stepdist = float(np.float32(sc["stepsize"]) * np.float32(sc["voxel_size"]))
+    t0, t1, _ = oracle_mod.ray_bounds(sc["rays_o"], sc["rays_d"], sc["xyz_min"], sc["xyz_max"],
+                                      sc["near"], 1e9, stepdist)
+    u = np.random.default_rng(3).uniform(0.4, 1.0, n)
+    sc["far_per_ray"] = (np.asarray(t0) + u * (np.asarray(t1) - np.asarray(t0))).astype(np.float32)
+    tr = FusedTrainer(sc)
+    tr.forward_backward()
+    torch.cuda.synchronize()
+    ref = direct_oracle(oracle_mod, tr, sc)
+    counts = tr.records()["count"]
+    np.testing.assert_array_equal(counts, np.bincount(ref["ray_id_mask"], minlength=n))
+    n_long = int((counts > 256).sum())
+    out = tr.ray_out.cpu().numpy()
+    err_rgb = float(np.abs(out[:, :3] - ref["rgb_marched"]).max())
+    err_loss = abs(float(tr.loss.item()) - ref["loss"]) / abs(ref["loss"])
+    e_den = rel_l2(tr.grad_density.cpu().numpy(), ref["grad_density"])
+    e_col = rel_l2(tr.grad_color.cpu().numpy(), ref["grad_color"])
+    print(f"[result] {n_long} of {n} rays keep more than 256 steps (max {int(counts.max())}); "
+          f"rgb {err_rgb:.2e} loss {err_loss:.2e} grad density {e_den:.2e} colour {e_col:.2e}",
+          flush=True)
+    assert 30 <= n_long <= n - 30
+    assert err_rgb <= 1e-5 and err_loss <= 1e-5 and e_den <= 1e-5 and e_col <= 1e-5
+
+
 @pytest.mark.parametrize("channels,modes", [(12, (1, 1)), (3, (0, 0)), (3, (2, 0))])
 def test_direct_adam_bit_exact_in_each_stepmode(oracle_mod, channels, modes):
     from plenvdb_amd import scenes
