@@ -266,7 +266,9 @@ typedef struct pvdb_step_desc {
      * the full count saturated at 2^31 - 1, exceeds it.  The sum of the rays'
      * step counts (pvdb_ray_bounds) bounds every step. */
     int64_t capacity;
-    float* ray_out;   /* dev [n][4] rgb + T_last ([n][k0_dim + 1] with k0_dim), or NULL */
+    /* dev [n][4] rgb + T_last, 16-byte aligned ([n][k0_dim + 1] with k0_dim), or
+     * NULL; written in place by the step */
+    float* ray_out;
     float* loss;      /* dev [1] or NULL */
     /* dev [3] or NULL: stepsz_density, stepsz_color, step_size_mlp read on the
      * device instead of the three fields above, so one captured graph of the

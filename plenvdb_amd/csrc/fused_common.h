@@ -189,14 +189,45 @@ struct MlpAdam {
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
-// The dW reduction a launch carries in its first `blocks` workgroups (0: none).
+// The dW reduction a launch carries in its first `blocks` workgroups (0: none),
+// and the loss sum it carries in the workgroup after them (when ray_loss is set).
 struct ReduceArgs {
     const float* part;
     int nparts;
     float* dw;
     MlpAdam adam;
     int blocks;
+    const float* ray_loss;  // [n_rays] per-ray loss terms, or null
+    float* loss;            // [1] their sum
+    int n_rays;
 };
+
+// loss[0] = sum of v[0..n), by one wavefront in a fixed order (lane l adds
+// v[4l..4l+3], v[4(l+64)..], ... in turn, then a butterfly over the lanes), so
+// the sum does not depend on the launch that carries it.  v is 16-byte aligned.
+__device__ __forceinline__ void loss_sum_wave(const float* __restrict__ v, int n,
+                                              float* __restrict__ loss) {
+    const int lane = lane_id();
+    const int n4 = n >> 2;
+    const float4* v4 = reinterpret_cast<const float4*>(v);
+    float s = 0.f;
+    int i = lane;
+    for (; i + 192 < n4; i += 256) {  // four 16-byte loads in flight per lane
+        const float4 a = v4[i], b = v4[i + 64], c = v4[i + 128], d = v4[i + 192];
+        s += ((a.x + a.y) + a.z) + a.w;
+        s += ((b.x + b.y) + b.z) + b.w;
+        s += ((c.x + c.y) + c.z) + c.w;
+        s += ((d.x + d.y) + d.z) + d.w;
+    }
+    for (; i < n4; i += 64) {
+        const float4 a = v4[i];
+        s += ((a.x + a.y) + a.z) + a.w;
+    }
+    if (lane < (n & 3)) s += v[4 * n4 + lane];
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
+    if (lane == 0) loss[0] = s;
+}
 
 // Sum of the per-workgroup dW partials, added into dw without atomics: workgroup
 // `block` owns kReduceVec four-parameter vectors and splits the partials among

@@ -687,31 +687,6 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_direct_composite(StepPara
     }
 }
 
-// Loss = sum of the per-ray terms (one workgroup, a fixed order): 16-byte loads
-// (the per-ray array is 256-byte aligned), wavefront shuffles, then the 16
-// wavefront sums in order.
-__global__ void __launch_bounds__(1024) k_loss_reduce(RayBuffers R, int n) {
-    __shared__ float wsum[16];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int n4 = n >> 2;
-    float s = 0.f;
-    for (int i = tid; i < n4; i += 1024) {
-        const float4 v = reinterpret_cast<const float4*>(R.ray_loss)[i];
-        s += ((v.x + v.y) + v.z) + v.w;
-    }
-    if (tid < (n & 3)) s += R.ray_loss[4 * n4 + tid];
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
-    if (lane == 0) wsum[wv] = s;
-    __syncthreads();
-    if (tid == 0) {
-        float t = 0.f;
-#pragma unroll
-        for (int w = 0; w < 16; ++w) t += wsum[w];
-        R.loss[0] = t;
-    }
-}
-
 }  // namespace
 
 namespace pvdb {
@@ -745,8 +720,6 @@ int launch_ray_composite(const StepParams& P, const RayBuffers& R, const Records
     hipLaunchKernelGGL(k_ray_composite, dim3(pvdb_blocks(P.n_rays, w)), dim3(64 * w), 0, s, P, R,
                        rec);
     PVDB_LAUNCHED("k_ray_composite");
-    hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(1024), 0, s, R, P.n_rays);
-    PVDB_LAUNCHED("k_loss_reduce");
     return PVDB_OK;
 }
 
@@ -760,8 +733,6 @@ int launch_direct_composite(int nc, const StepParams& P, const TopoView& t, cons
     else
         hipLaunchKernelGGL(k_direct_composite<12>, grid, dim3(64 * w), 0, s, P, t, color, R, rec);
     PVDB_LAUNCHED("k_direct_composite");
-    hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(1024), 0, s, R, P.n_rays);
-    PVDB_LAUNCHED("k_loss_reduce");
     return PVDB_OK;
 }
 

@@ -80,8 +80,9 @@ __device__ __forceinline__ float lane4(const float4& v, int k) {
 }
 constexpr uint32_t kEnd = 0xffffu;  // end of a pair list
 // The launch's first `red.blocks` workgroups sum the MLP's dW partials
-// instead (dw_reduce_block, independent of the scatter): they overlap the
-// scatter's atomics rather than following them in a launch of their own.
+// instead (dw_reduce_block, independent of the scatter), and the next one the
+// loss when red.ray_loss is set: they overlap the scatter's atomics rather than
+// following them in launches of their own.
 template <int kAggRecords, int NC>
 __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R, Records rec,
                                                      int64_t capacity, GradRows G,
@@ -109,7 +110,12 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
                         reinterpret_cast<float4*>(pv));
         return;
     }
-    const int wg = (int)blockIdx.x - red.blocks, nwg = (int)gridDim.x - red.blocks;
+    const int lead = red.blocks + (red.ray_loss ? 1 : 0);  // workgroups before the scatter's
+    if ((int)blockIdx.x < lead) {  // the loss workgroup
+        if (threadIdx.x < 64) loss_sum_wave(red.ray_loss, red.n_rays, red.loss);
+        return;
+    }
+    const int wg = (int)blockIdx.x - lead, nwg = (int)gridDim.x - lead;
     const int tid = threadIdx.x;
     for (int e = tid; e < kAggSlots; e += 256) {
         key[e] = kEmptyKey;
@@ -220,6 +226,12 @@ __global__ void __launch_bounds__(256) k_scatter_agg(TopoView topo, RayBuffers R
     }
 }
 
+// Loss = sum of the per-ray terms, for the launches that carry no loss
+// workgroup (one wavefront, the order of loss_sum_wave).
+__global__ void __launch_bounds__(64) k_loss_reduce(const float* ray_loss, int n, float* loss) {
+    loss_sum_wave(ray_loss, n, loss);
+}
+
 }  // namespace
 
 namespace pvdb {
@@ -227,13 +239,22 @@ namespace pvdb {
 int launch_dw_reduce(const float* part, int nparts, float* dw, const MlpAdam& adam,
                      hipStream_t s);
 
+// The reductions riding on a scatter launch, as launches of their own (the
+// scatter paths that do not carry them).
+int launch_reductions(const ReduceArgs& red, hipStream_t s) {
+    if (red.ray_loss) {
+        hipLaunchKernelGGL(k_loss_reduce, dim3(1), dim3(64), 0, s, red.ray_loss, red.n_rays,
+                           red.loss);
+        PVDB_LAUNCHED("k_loss_reduce");
+    }
+    return red.blocks ? launch_dw_reduce(red.part, red.nparts, red.dw, red.adam, s) : PVDB_OK;
+}
+
 template <int NC>
 int launch_scatter_nc(const TopoView& t, const RayBuffers& R, const Records& rec, int64_t capacity,
                       const GradRows& G, uint8_t* leaf_touched, int grid_req,
                       int chunk_records, const ReduceArgs& red_in, hipStream_t s) {
-    if (t.nleaf == 0)
-        return red_in.blocks ? launch_dw_reduce(red_in.part, red_in.nparts, red_in.dw, red_in.adam, s)
-                             : PVDB_OK;
+    if (t.nleaf == 0) return launch_reductions(red_in, s);
     const ReduceArgs red = red_in;
     if (t.nleaf < (int64_t(1) << 23) - 1) {  // every slot < 2^32 - 1, the empty key
         PVDB_CHECK_ARG(chunk_records == 32 || chunk_records == 64,
@@ -243,7 +264,7 @@ int launch_scatter_nc(const TopoView& t, const RayBuffers& R, const Records& rec
         // eight (32-record) workgroups fit a CU by LDS
         const int64_t want = grid_req > 0 ? grid_req : 1;
         const unsigned grid = (unsigned)(chunks < want ? (chunks < 1 ? 1 : chunks) : want) +
-                              (unsigned)red.blocks;
+                              (unsigned)red.blocks + (red.ray_loss ? 1u : 0u);
         if (chunk_records == 64)
             hipLaunchKernelGGL((k_scatter_agg<64, NC>), dim3(grid), dim3(256), 0, s, t, R, rec,
                                capacity, G, leaf_touched, red);
@@ -259,7 +280,7 @@ int launch_scatter_nc(const TopoView& t, const RayBuffers& R, const Records& rec
     hipLaunchKernelGGL(k_scatter_direct<NC>, dim3(grid), dim3(256), 0, s, t, R, rec, capacity, G,
                        leaf_touched);
     PVDB_LAUNCHED("k_scatter_direct");
-    return red.blocks ? launch_dw_reduce(red.part, red.nparts, red.dw, red.adam, s) : PVDB_OK;
+    return launch_reductions(red, s);
 }
 
 // nc: colour channels per slot, 12 (the MLP's features) or 3 (a direct-colour grid)

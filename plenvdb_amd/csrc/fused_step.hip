@@ -477,6 +477,9 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
                        aligned(d->color_master, 16),
                    "colour arrays must be 16-byte aligned (8-byte for a bf16 payload)");
     PVDB_CHECK_ARG(aligned(d->workspace, 256), "workspace must be 256-byte aligned");
+    // the composite writes ray_out in place, a 16-byte vector per ray with the MLP
+    PVDB_CHECK_ARG(aligned(d->ray_out, direct ? 4 : 16),
+                   "ray_out must be 16-byte aligned (4-byte without the MLP)");
     const TopoView t = view_of(topo);
     const Layout L = layout_for(d->n_rays, d->capacity, t.nleaf);
     PVDB_CHECK_ARG(d->workspace && d->workspace_bytes >= (int64_t)L.bytes, "workspace too small");
@@ -519,11 +522,13 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     R.offset = reinterpret_cast<int32_t*>(ws + L.offset);
     R.total = reinterpret_cast<int32_t*>(ws + L.total);
     R.t_last = reinterpret_cast<float*>(ws + L.t_last);
-    R.ray_out = reinterpret_cast<float4*>(ws + L.ray_out);
+    // the composite writes the caller's per-ray output when there is one
+    R.ray_out = d->ray_out && !direct ? reinterpret_cast<float4*>(d->ray_out)
+                                      : reinterpret_cast<float4*>(ws + L.ray_out);
     R.ray_grad = reinterpret_cast<float4*>(ws + L.ray_grad);
     R.ray_ctot = reinterpret_cast<float4*>(ws + L.ray_ctot);
     R.ray_loss = reinterpret_cast<float*>(ws + L.ray_loss);
-    R.ray_rgb = reinterpret_cast<float*>(ws + L.ray_rgb);
+    R.ray_rgb = d->ray_out && direct ? d->ray_out : reinterpret_cast<float*>(ws + L.ray_rgb);
     R.loss = d->loss ? d->loss : reinterpret_cast<float*>(ws + L.loss);
     R.steps = reinterpret_cast<uint16_t*>(ws + L.steps);
     Records rec;
@@ -588,15 +593,11 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
         {
             StageScope sc(6, s);
             const MlpAdam none{nullptr, nullptr, nullptr, nullptr, 0.f, 0.f, 0.f, 0.f};
-            const ReduceArgs red{nullptr, 0, nullptr, none, 0};
+            const ReduceArgs red{nullptr, 0, nullptr, none, 0, R.ray_loss, R.loss, d->n_rays};
             if ((rc = launch_scatter(slot_channels(d), t, R, rec, d->capacity, grad_rows_of(d),
                                      touched, d->launch[1] > 0 ? d->launch[1] : kScatterGrid,
                                      d->launch[2] > 0 ? d->launch[2] : kScatterChunk, red, s)))
                 return rc;
-        }
-        if (d->ray_out) {
-            PVDB_HIP(hipMemcpyAsync(d->ray_out, R.ray_rgb, 4 * (size_t)(nc + 1) * d->n_rays,
-                                    hipMemcpyDeviceToDevice, s));
         }
         if (d->do_adam) {
             StageScope sc(8, s);
@@ -641,15 +642,12 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
         if (d->do_adam)
             adam = MlpAdam{d->mlp,           d->mlp_m, d->mlp_v, d->adam_scalars,
                            d->step_size_mlp, d->eps,   d->beta0, d->beta1};
-        const ReduceArgs red{dw_part, grid_bwd, d->mlp_grad, adam, kReduceBlocks};
+        const ReduceArgs red{dw_part, grid_bwd, d->mlp_grad, adam, kReduceBlocks,
+                             R.ray_loss, R.loss, d->n_rays};
         if ((rc = launch_scatter(12, t, R, rec, d->capacity, grad_rows_of(d), touched,
                                  d->launch[1] > 0 ? d->launch[1] : kScatterGrid,
                                  d->launch[2] > 0 ? d->launch[2] : kScatterChunk, red, s)))
             return rc;
-    }
-    if (d->ray_out) {
-        PVDB_HIP(hipMemcpyAsync(d->ray_out, R.ray_out, 16 * (size_t)d->n_rays,
-                                hipMemcpyDeviceToDevice, s));
     }
     if (d->do_adam) {
         StageScope sc(8, s);

@@ -121,6 +121,8 @@ def test_fused_step_rejects_capacity_and_misaligned_buffers():
         (dict(color=base + 8), "16-byte"),
         (dict(color=base + 4, color_bf16=1), "8-byte"),
         (dict(workspace=base + 64), "256-byte"),
+        (dict(ray_out=base + 8), "ray_out"),
+        (dict(ray_out=base + 2, k0_dim=3), "ray_out"),
         (dict(k0_dim=5), "k0_dim"),
         (dict(k0_dim=3, color_bf16=1), "fp32 colour grid"),
         (dict(mlp=None), "null MLP"),
