@@ -189,6 +189,58 @@ struct MlpAdam {
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
 
+// Wavefront scans on the DPP datapath, for a fully active wavefront: four row
+// shifts (1, 2, 4, 8 lanes within each 16-lane row), then lane 15's value
+// broadcast into rows 1 and 3 and lane 31's into rows 2 and 3.  A lane whose
+// source lies outside its row reads the identity.  Each step is one VALU
+// operation, where a shuffle is an LDS permute round trip.
+template <int kCtrl, int kRows = 0xf>
+__device__ __forceinline__ float dpp_mov(float x, float ident) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+        __builtin_bit_cast(int, ident), __builtin_bit_cast(int, x), kCtrl, kRows, 0xf, false));
+}
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114,
+              kDppRowShr8 = 0x118, kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143,
+              kDppWaveShr1 = 0x138;
+
+// inclusive product over lanes 0..l
+__device__ __forceinline__ float wave_incl_prod(float x) {
+    x *= dpp_mov<kDppRowShr1>(x, 1.f);
+    x *= dpp_mov<kDppRowShr2>(x, 1.f);
+    x *= dpp_mov<kDppRowShr4>(x, 1.f);
+    x *= dpp_mov<kDppRowShr8>(x, 1.f);
+    x *= dpp_mov<kDppRowBcast15, 0xa>(x, 1.f);
+    x *= dpp_mov<kDppRowBcast31, 0xc>(x, 1.f);
+    return x;
+}
+
+// inclusive sums over lanes 0..l of N values at once (their steps interleave)
+template <int N>
+__device__ __forceinline__ void wave_incl_sum(float (&x)[N]) {
+#pragma unroll
+    for (int c = 0; c < N; ++c) x[c] += dpp_mov<kDppRowShr1>(x[c], 0.f);
+#pragma unroll
+    for (int c = 0; c < N; ++c) x[c] += dpp_mov<kDppRowShr2>(x[c], 0.f);
+#pragma unroll
+    for (int c = 0; c < N; ++c) x[c] += dpp_mov<kDppRowShr4>(x[c], 0.f);
+#pragma unroll
+    for (int c = 0; c < N; ++c) x[c] += dpp_mov<kDppRowShr8>(x[c], 0.f);
+#pragma unroll
+    for (int c = 0; c < N; ++c) x[c] += dpp_mov<kDppRowBcast15, 0xa>(x[c], 0.f);
+#pragma unroll
+    for (int c = 0; c < N; ++c) x[c] += dpp_mov<kDppRowBcast31, 0xc>(x[c], 0.f);
+}
+
+// lane l - 1's value (ident on lane 0)
+__device__ __forceinline__ float wave_shr1(float x, float ident) {
+    return dpp_mov<kDppWaveShr1>(x, ident);
+}
+
+// lane `src`'s value (src uniform over the wavefront)
+__device__ __forceinline__ float wave_read(float x, int src) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), src));
+}
+
 // The dW reduction a launch carries in its first `blocks` workgroups (0: none),
 // and the loss sum it carries in the workgroup after them (when ray_loss is set).
 struct ReduceArgs {

@@ -294,14 +294,8 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_march(StepParams P, T
         // inclusive product of (1 - alpha) over alpha-kept lanes, in step order
         // (a log-depth scan: each T is a product of the same factors as the
         // reference's running product, grouped differently)
-        const float f = akeep ? 1.f - alpha : 1.f;
-        float incl = f;
-        for (int d = 1; d < 64; d <<= 1) {
-            const float o = __shfl_up(incl, d, 64);
-            if (lane >= d) incl *= o;
-        }
-        float excl = __shfl_up(incl, 1, 64);
-        if (lane == 0) excl = 1.f;
+        const float incl = wave_incl_prod(akeep ? 1.f - alpha : 1.f);
+        const float excl = wave_shr1(incl, 1.f);
         const float T = Tc * excl, Ta = Tc * incl;
         const uint64_t stopm = __ballot(akeep && Ta < 1e-3f);
         const int stop_lane = stopm ? __ffsll((unsigned long long)stopm) - 1 : 64;
@@ -320,10 +314,10 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_march(StepParams P, T
         written += k;
         if (!stopped) {
             if (stop_lane < 64) {
-                Tc = __shfl(Ta, stop_lane, 64);
+                Tc = wave_read(Ta, stop_lane);
                 stopped = true;
             } else {
-                Tc = __shfl(Ta, 63, 64);
+                Tc = wave_read(Ta, 63);
             }
         }
     };
@@ -372,15 +366,9 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_march(StepParams P, T
 // to the march's.
 __device__ __forceinline__ float chunk_transmittance(float alpha, uint32_t bits, int lane,
                                                      float& Tc) {
-    float incl = (bits & kAliveA) ? 1.f - alpha : 1.f;
-    for (int d = 1; d < 64; d <<= 1) {
-        const float o = __shfl_up(incl, d, 64);
-        if (lane >= d) incl *= o;
-    }
-    float excl = __shfl_up(incl, 1, 64);
-    if (lane == 0) excl = 1.f;
-    const float T = Tc * excl;
-    Tc = __shfl(Tc * incl, 63, 64);
+    const float incl = wave_incl_prod((bits & kAliveA) ? 1.f - alpha : 1.f);
+    const float T = Tc * wave_shr1(incl, 1.f);
+    Tc = wave_read(Tc * incl, 63);
     return T;
 }
 
@@ -419,18 +407,11 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_composite(StepParams 
                 sq += w * ((e0 * e0 + e1 * e1) + e2 * e2);
             }
         }
-        for (int d = 1; d < 64; d <<= 1) {
-            const float o0 = __shfl_up(vr, d, 64), o1 = __shfl_up(vg, d, 64),
-                        o2 = __shfl_up(vb, d, 64);
-            if (lane >= d) {
-                vr += o0;
-                vg += o1;
-                vb += o2;
-            }
-        }
-        cr += __shfl(vr, 63, 64);
-        cg += __shfl(vg, 63, 64);
-        cb += __shfl(vb, 63, 64);
+        float v3[3] = {vr, vg, vb};
+        wave_incl_sum(v3);
+        cr += wave_read(v3[0], 63);
+        cg += wave_read(v3[1], 63);
+        cb += wave_read(v3[2], 63);
     }
     // the ray's colour, loss and dL/drgb (every lane holds them)
     const float tl = R.t_last[r];
@@ -486,14 +467,12 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_composite(StepParams 
             vb = w * c.z;
         }
         if (bits & kAliveA) e = rec.xyze[q].w;
-        for (int d = 1; d < 64; d <<= 1) {  // the same scan as the first pass
-            const float o0_ = __shfl_up(vr, d, 64), o1_ = __shfl_up(vg, d, 64),
-                        o2_ = __shfl_up(vb, d, 64);
-            if (lane >= d) {
-                vr += o0_;
-                vg += o1_;
-                vb += o2_;
-            }
+        {  // the same scan as the first pass
+            float v3[3] = {vr, vg, vb};
+            wave_incl_sum(v3);
+            vr = v3[0];
+            vg = v3[1];
+            vb = v3[2];
         }
         float gl0 = 0.f, gl1 = 0.f, gl2 = 0.f, gw = 0.f, gs = 0.f;
         if (bits & kAliveW) {
@@ -509,21 +488,9 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_composite(StepParams 
             gs = fminf(e, 1e10f) * powf(1.f + e, -P.interval - 1.f) * P.interval * ga;
         }
         if (valid) rec.adj[q] = make_float4(gl0, gl1, gl2, gs);
-        pr += __shfl(vr, 63, 64);
-        pg += __shfl(vg, 63, 64);
-        pb += __shfl(vb, 63, 64);
-    }
-}
-
-// Inclusive prefix sums of v[0..NC-1] over the wavefront's lanes.
-template <int NC>
-__device__ __forceinline__ void wave_prefix(float (&v)[NC], int lane) {
-    for (int d = 1; d < 64; d <<= 1) {
-#pragma unroll
-        for (int c = 0; c < NC; ++c) {
-            const float o = __shfl_up(v[c], d, 64);
-            if (lane >= d) v[c] += o;
-        }
+        pr += wave_read(vr, 63);
+        pg += wave_read(vg, 63);
+        pb += wave_read(vb, 63);
     }
 }
 
@@ -590,9 +557,9 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_direct_composite(StepPara
             }
             if (rgbper) sq += w * e2;
         }
-        wave_prefix<NC>(v, lane);
+        wave_incl_sum(v);
 #pragma unroll
-        for (int c = 0; c < NC; ++c) ctot[c] += __shfl(v[c], 63, 64);
+        for (int c = 0; c < NC; ++c) ctot[c] += wave_read(v[c], 63);
     }
     // the ray's colour, loss and dL/drgb (every lane holds them)
     const float tl = R.t_last[r];
@@ -658,7 +625,7 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_direct_composite(StepPara
 #pragma unroll
         for (int c = 0; c < NC; ++c) v[c] = w * rgb[c];
         if (bits & kAliveA) e = rec.xyze[q].w;
-        wave_prefix<NC>(v, lane);
+        wave_incl_sum(v);
         float gl[16];
 #pragma unroll
         for (int c = 0; c < 16; ++c) gl[c] = 0.f;
@@ -683,7 +650,7 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_direct_composite(StepPara
                 row[k] = make_float4(gl[4 * k], gl[4 * k + 1], gl[4 * k + 2], gl[4 * k + 3]);
         }
 #pragma unroll
-        for (int c = 0; c < NC; ++c) pre[c] += __shfl(v[c], 63, 64);
+        for (int c = 0; c < NC; ++c) pre[c] += wave_read(v[c], 63);
     }
 }
 
