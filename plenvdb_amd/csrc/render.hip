@@ -24,8 +24,9 @@
 //                   v_mfma_f32_16x16x32_f16 (weights in LDS, shared by the
 //                   workgroup's 8 waves) and each record's contribution
 //                   w / (1 + exp(-raw)) (final_render).
-//   k_render_composite  per pixel: T_last * bg plus its records' contributions
-//                   in sample order (a fixed order, so frames are reproducible).
+//   k_render_composite  per pixel: T_last * bg plus its records' contributions,
+//                   eight lanes per pixel reading its run of records together
+//                   (a fixed order, so frames are reproducible).
 //
 // Data layout.  The merged index grid (vdb_compression.py:19-59: index 1 +
 // rank of an active voxel, 0 elsewhere) is a topology whose value masks are
@@ -830,27 +831,43 @@ __global__ void __launch_bounds__(64 * kWaves) k_render_mlp(RenderArgs A) {
 }
 
 // Pixel colour: T_last * bg + the pixel's contributions in sample order.
+constexpr int kCompLanes = 8;  // lanes per pixel in k_render_composite
+
 __global__ void __launch_bounds__(256) k_render_composite(RenderArgs A) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (int64_t)A.H * A.W) return;
+    // lane j of a pixel's eight sums the pixel's records j, j + 8, ... (the
+    // eight read 128 contiguous bytes per step), then the eight add up their
+    // sums; every lane takes part in the exchange (no early exit)
+    const int64_t p = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kCompLanes;
+    const int j = (int)(threadIdx.x & (kCompLanes - 1));
+    const bool on = p < (int64_t)A.H * A.W;  // uniform over a pixel's lanes
     const FrameBuffers& F = A.fb;
-    const unsigned long long tot = *F.total;
-    const int64_t n = (int64_t)(tot < (unsigned long long)F.capacity ? tot : F.capacity);
-    const int64_t s = F.pix_start[p];
-    const int wn = F.pix_written[p];
     float a0 = 0.f, a1 = 0.f, a2 = 0.f;
-    for (int k = 0; k < wn; ++k) {
-        const int64_t q = s + k;
-        if (q >= n) break;
-        const float4 c = F.contrib[q];
-        a0 = a0 + c.x;
-        a1 = a1 + c.y;
-        a2 = a2 + c.z;
+    if (on) {
+        const unsigned long long tot = *F.total;
+        const int64_t n = (int64_t)(tot < (unsigned long long)F.capacity ? tot : F.capacity);
+        const int64_t s = F.pix_start[p];
+        const int wn = F.pix_written[p];
+        for (int k = j; k < wn; k += kCompLanes) {
+            const int64_t q = s + k;
+            if (q >= n) break;
+            const float4 c = F.contrib[q];
+            a0 = a0 + c.x;
+            a1 = a1 + c.y;
+            a2 = a2 + c.z;
+        }
     }
-    const float last = F.t_last[p] * A.bg;
-    A.out[3 * p] = last + a0;
-    A.out[3 * p + 1] = last + a1;
-    A.out[3 * p + 2] = last + a2;
+#pragma unroll
+    for (int d = 1; d < kCompLanes; d <<= 1) {
+        a0 += __shfl_xor(a0, d, 64);
+        a1 += __shfl_xor(a1, d, 64);
+        a2 += __shfl_xor(a2, d, 64);
+    }
+    if (on && j == 0) {
+        const float last = F.t_last[p] * A.bg;
+        A.out[3 * p] = last + a0;
+        A.out[3 * p + 1] = last + a1;
+        A.out[3 * p + 2] = last + a2;
+    }
 }
 
 __global__ void k_render_reset(unsigned long long* total, unsigned long long* counters) {
@@ -1106,8 +1123,9 @@ int render_image(const pvdb_topology* topo, const pvdb_render_desc* d, void* ws,
     }
     {
         RenderStage st(2, s);
-        hipLaunchKernelGGL(k_render_composite, dim3(pvdb_blocks((int64_t)A.H * A.W, 256)),
-                           dim3(256), 0, s, A);
+        hipLaunchKernelGGL(k_render_composite,
+                           dim3(pvdb_blocks((int64_t)A.H * A.W * kCompLanes, 256)), dim3(256), 0,
+                           s, A);
         PVDB_LAUNCHED("k_render_composite");
     }
     return PVDB_OK;
