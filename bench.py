@@ -242,8 +242,8 @@ def bench_batched(args, scene):
 # x1.0); profiles/pmc_traffic_c2_train_c5_render.md holds the runs.
 TRAFFIC_SOURCE = "profiles/pmc_traffic_c2_train_c5_render.md"
 MEASURED_TRAFFIC_MB = {
-    "c2": {"mlp_bwd": 63.73, "scatter": 85.94, "mlp_fwd": 69.57, "ray_march": 14.14,
-           "composite": 15.73, "adam": 122.02, "ray_count": 3.15},
+    "c2": {"mlp_bwd": 63.72, "scatter": 85.97, "mlp_fwd": 69.6, "ray_march": 14.43,
+           "composite": 15.7, "adam": 122.0, "ray_count": 3.64},
     "c5": {"render_mlp": 656.89},
 }
 
@@ -256,13 +256,13 @@ def stage_models(r):
     ua, uw, L_t, V_u = r["uniq_all"], r["uniq_w"], r["l_t"], r["v_u"]
     P = 22019
     return {
-        # rays in, counts out; the MLP parameters packed to bf16 and the MLP
-        # gradient zeroed by the same launch
-        "ray_count": ("hbm", N * (12 + 12 + 4)
+        # rays in, counts and the kept steps' 2-byte indices out; the MLP
+        # parameters packed to bf16 and the MLP gradient zeroed by the same launch
+        "ray_count": ("hbm", N * (12 + 12 + 4) + S * 2
                       + P * 4 + 2 * (128 * 64 + 2 * 128 * 128 + 2 * 16 * 128 + 128 * 32) + P * 4),
-        # rays + offsets in, 24-byte records out (xyze, alpha + flags), distinct
-        # density voxels once
-        "ray_march": ("hbm", N * (12 + 12 + 8) + S * 24 + ua * 4),
+        # rays + offsets and the kept step indices in, 24-byte records out
+        # (xyze, alpha + flags), distinct density voxels once
+        "ray_march": ("hbm", N * (12 + 12 + 8) + S * (2 + 24) + ua * 4),
         # 36,608 FLOP per MLP sample + 6,912 per ray (PE part of layer 0)
         "mlp_fwd": ("mfma", nw * MLP_FWD_FLOP + N * MLP_RAY_FLOP),
         # alpha + flags and exp(sigma) in, adjoints out per record; rgb in per
@@ -273,9 +273,9 @@ def stage_models(r):
         # records + per-record gradients in, distinct voxels' 13 gradients
         # read+written; the same launch reduces the dW partials (one per CU in,
         # MLP gradient read+written) and applies the MLP Adam (parameters and
-        # both moments read+written)
+        # both moments read+written), and sums the per-ray loss terms
         "scatter": ("hbm", S * (16 + 8 + 64) + uw * 13 * 4 * 2
-                    + r["n_cu"] * P * 4 + 2 * P * 4 + 6 * P * 4),
+                    + r["n_cu"] * P * 4 + 2 * P * 4 + 6 * P * 4 + N * 4),
         # B_adam: density p,m,v r/w + g r; colour p,m,v r/w + g r
         "adam": ("hbm", V_u * (28 + 12 * (2 * sc + 20)) + L_t * 512 * 13 * 4),
     }
