@@ -400,26 +400,24 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
                 v[c] += __shfl_xor(v[c], 2, 64);
                 v[c] += __shfl_xor(v[c], 4, 64);
             }
-            if (gpart == 0) {
-                uint32_t wh[6], wl[6], wt[6];
+            // the butterfly leaves the same twelve sums (bit for bit: each step
+            // adds the same two operands on every lane) in all eight lanes of
+            // the record, so lane gpart < 6 splits and stores channel pair gpart
+            if (gpart < 6) {
+                float a = v[0], b = v[1];
 #pragma unroll
-                for (int c = 0; c < 6; ++c) split3(v[2 * c], v[2 * c + 1], wh[c], wl[c], wt[c]);
-                uint32_t* xh = reinterpret_cast<uint32_t*>(&sm.u.x.Xh[gs * XM]);
-                uint32_t* xm = reinterpret_cast<uint32_t*>(&sm.u.x.Xm[gs * XM]);
-                uint32_t* xl = reinterpret_cast<uint32_t*>(&sm.u.x.Xl[gs * XM]);
-#pragma unroll
-                for (int c = 0; c < 6; ++c) {
-                    xh[c] = wh[c];
-                    xm[c] = wl[c];
-                    xl[c] = wt[c];
+                for (int c = 1; c < 6; ++c) {
+                    a = gpart == c ? v[2 * c] : a;
+                    b = gpart == c ? v[2 * c + 1] : b;
                 }
+                uint32_t wh, wm, wl;
+                split3(a, b, wh, wm, wl);
+                reinterpret_cast<uint32_t*>(&sm.u.x.Xh[gs * XM])[gpart] = wh;
+                reinterpret_cast<uint32_t*>(&sm.u.x.Xm[gs * XM])[gpart] = wm;
+                reinterpret_cast<uint32_t*>(&sm.u.x.Xl[gs * XM])[gpart] = wl;
                 // the backward's input row, features part (X columns 0..11, hi)
                 const int64_t qq = base + gs;
-                if (qq < total) {
-                    uint32_t* xr = reinterpret_cast<uint32_t*>(rec.xrow + 5 * qq);
-                    *reinterpret_cast<uint4*>(xr) = make_uint4(wh[0], wh[1], wh[2], wh[3]);
-                    *reinterpret_cast<uint2*>(xr + 4) = make_uint2(wh[4], wh[5]);
-                }
+                if (qq < total) reinterpret_cast<uint32_t*>(rec.xrow + 5 * qq)[gpart] = wh;
             }
         }
         {
