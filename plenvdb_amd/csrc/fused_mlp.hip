@@ -472,11 +472,13 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
         __syncthreads();
         // ---- B. H0^T slice = relu(W0^T X^T + b0) -> A1; next group's lookups ----
         // k-step outer: each B fragment is read once and the (mt, nt) chains interleave
+        // the accumulators start at the bias (the MFMA adds it, no separate pass)
         f32x4 acc[MT][NT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int nt = 0; nt < NT; ++nt)
+                acc[mt][nt] = f32x4{b0v[mt][0], b0v[mt][1], b0v[mt][2], b0v[mt][3]};
 #pragma unroll
         for (int kt = 0; kt < 2; ++kt) {
 #pragma unroll
@@ -500,8 +502,15 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
                 nib[mt][nt] = 0u;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    h[i] = fmaxf(acc[mt][nt][i] + b0v[mt][i], 0.f);
-                    nib[mt][nt] |= (h[i] > 0.f ? 1u : 0u) << i;
+                    // relu as an integer select on one integer compare, which
+                    // also gives the mask bit: a float is > 0 exactly when its
+                    // bits are > 0 as an int (a float max would first
+                    // canonicalize the MFMA output, one more instruction per
+                    // value; +NaN passes through, as torch's relu passes NaN)
+                    const uint32_t hb = __float_as_uint(acc[mt][nt][i]);
+                    const bool on = (int32_t)hb > 0;
+                    h[i] = __uint_as_float(on ? hb : 0u);
+                    nib[mt][nt] |= (on ? 1u : 0u) << i;
                 }
                 st_act3(sm.A1h, sm.A1m, sm.A1l, 16 * nt + r16, f0 + 16 * mt + 4 * g, h);
             }
@@ -517,7 +526,8 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int nt = 0; nt < NT; ++nt)
+                acc[mt][nt] = f32x4{b1v[mt][0], b1v[mt][1], b1v[mt][2], b1v[mt][3]};
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) {
 #pragma unroll
@@ -540,8 +550,15 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
                 nib[mt][nt] = 0u;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    h[i] = fmaxf(acc[mt][nt][i] + b1v[mt][i], 0.f);
-                    nib[mt][nt] |= (h[i] > 0.f ? 1u : 0u) << i;
+                    // relu as an integer select on one integer compare, which
+                    // also gives the mask bit: a float is > 0 exactly when its
+                    // bits are > 0 as an int (a float max would first
+                    // canonicalize the MFMA output, one more instruction per
+                    // value; +NaN passes through, as torch's relu passes NaN)
+                    const uint32_t hb = __float_as_uint(acc[mt][nt][i]);
+                    const bool on = (int32_t)hb > 0;
+                    h[i] = __uint_as_float(on ? hb : 0u);
+                    nib[mt][nt] |= (on ? 1u : 0u) << i;
                 }
                 st_act<false>(sm.u.h1.A2h, sm.u.h1.A2l, nullptr, 16 * nt + r16, f0 + 16 * mt + 4 * g,
                               lane, h);
@@ -819,9 +836,9 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
                 m1 |= ((mw[4 + (wv >> 1)] >> sh) & 0xFu) << (nt * 4);
             }
         }
-        f32x4 acc[NT];
+        f32x4 acc[NT];  // starting at the bias, as in the forward
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{b0v[0], b0v[1], b0v[2], b0v[3]};
 #pragma unroll
         for (int kt = 0; kt < 2; ++kt) {
 #pragma unroll
@@ -851,7 +868,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
         for (int nt = 0; nt < NT; ++nt) {
             float h[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) h[i] = (m0 >> (nt * 4 + i)) & 1u ? acc[nt][i] + b0v[i] : 0.f;
+            for (int i = 0; i < 4; ++i) h[i] = (m0 >> (nt * 4 + i)) & 1u ? acc[nt][i] : 0.f;
             st_hi_fm(sm.A1h, sm.H0f, 16 * nt + r16, fr, lane, h);
         }
         if (pbase >= 0) finish(cb ^ 1, pbase);
@@ -898,7 +915,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
             }
         }
 #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4{b1v[0], b1v[1], b1v[2], b1v[3]};
 #pragma unroll
         for (int kt = 0; kt < 4; ++kt) {
 #pragma unroll
@@ -911,7 +928,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
         for (int nt = 0; nt < NT; ++nt) {
             float h[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) h[i] = (m1 >> (nt * 4 + i)) & 1u ? acc[nt][i] + b1v[i] : 0.f;
+            for (int i = 0; i < 4; ++i) h[i] = (m1 >> (nt * 4 + i)) & 1u ? acc[nt][i] : 0.f;
             st_fm(sm.F2, 16 * nt + r16, fr, lane, h);
         }
         park(cb ^ 1);  // the next group's records, loaded at this group's start
