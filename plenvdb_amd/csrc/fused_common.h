@@ -109,6 +109,9 @@ struct RayBuffers {
     float* ray_rgb;    // [N][NC + 1] direct-colour configurations: rendered rgb (NC) + T_last
     float* loss;       // [1]
     uint16_t* steps;   // [N][kStepCache] the counting march's kept step indices, in step order
+    uint16_t* pe;      // [N][3][32] with the MLP: the ray's MLP input columns 12..39 (view
+                       // direction, its sin/cos encoding, the bias column) as bf16 hi,
+                       // mid and lo rows, written by the counting march; null without
 };
 
 // Kept step indices the counting march stores per ray (16-bit: rays of fewer
@@ -188,6 +191,35 @@ struct MlpAdam {
 };
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+
+// Two floats -> two bf16 (round to nearest even) in one dword, element 0 in
+// the low half (v_cvt_pk_bf16_f32).
+__device__ __forceinline__ uint32_t pk_bf(float a, float b) {
+    typedef __bf16 bf2_t __attribute__((ext_vector_type(2)));
+    typedef float f2_t __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f2_t){a, b}, bf2_t));
+}
+__device__ __forceinline__ float bf_lo(uint32_t p) { return __uint_as_float(p << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t p) { return __uint_as_float(p & 0xffff0000u); }
+
+// x = hi + lo with hi = bf16(x) and lo = bf16(x - hi): the pair carries about
+// 16 significant bits, so products of a bf16 weight with (hi, lo) keep the
+// activation's precision.  Two values per call, packed as above.
+__device__ __forceinline__ void split2(float a, float b, uint32_t& h, uint32_t& l) {
+    h = pk_bf(a, b);
+    l = pk_bf(a - bf_lo(h), b - bf_hi(h));
+}
+// x = hi + mid + lo, each bf16: about 24 significant bits, as many as fp32
+// holds (x - hi and mid's remainder are exact in fp32).  The forward pass feeds
+// its products with triples: with pairs, about one pre-activation in a million
+// rounds to the other side of zero at the benchmarked sizes, and each such
+// ReLU flip moves its sample's feature gradient by a whole term.
+__device__ __forceinline__ void split3(float a, float b, uint32_t& h, uint32_t& m, uint32_t& l) {
+    h = pk_bf(a, b);
+    const float ra = a - bf_lo(h), rb = b - bf_hi(h);
+    m = pk_bf(ra, rb);
+    l = pk_bf(ra - bf_lo(m), rb - bf_hi(m));
+}
 
 // Wavefront scans on the DPP datapath, for a fully active wavefront: four row
 // shifts (1, 2, 4, 8 lanes within each 16-lane row), then lane 15's value

@@ -175,6 +175,26 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_count(StepParams P, T
     const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (r >= P.n_rays) return;
     const int lane = lane_id();
+    if (R.pe && lane < 28) {
+        // the MLP's input columns 12..39 for this ray's samples (dvgo.py:354-358):
+        // the view direction, sin then cos of it at 1, 2, 4, 8 (hardware sin/cos,
+        // error about 1e-6 for |arg| <= 8), and the bias column; one column per lane
+        const float* vd = P.viewdirs + 3 * (int64_t)r;
+        float v = 1.f;
+        if (lane < 3) {
+            v = vd[lane];
+        } else if (lane < 27) {
+            const int idx = lane - 3, axis = (idx % 12) / 4, fq = idx % 4;
+            const float arg = vd[axis] * (float)(1 << fq);
+            v = idx < 12 ? __sinf(arg) : __cosf(arg);
+        }
+        uint32_t h, m, l;
+        split3(v, 0.f, h, m, l);
+        uint16_t* row = R.pe + 96 * (int64_t)r;
+        row[lane] = (uint16_t)h;
+        row[32 + lane] = (uint16_t)m;
+        row[64 + lane] = (uint16_t)l;
+    }
     const RaySetup s = ray_setup(P, r);
     int count = 0;
     uint16_t* cache = R.steps + (int64_t)r * kStepCache;

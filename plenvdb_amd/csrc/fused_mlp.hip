@@ -62,34 +62,6 @@ __device__ __forceinline__ s16x4 ld_frag4(const uint16_t* p) {
 
 __device__ __forceinline__ bf16x8 as_bf8(const uint4& v) { return __builtin_bit_cast(bf16x8, v); }
 
-// Two floats -> two bf16 (round to nearest even) in one dword, element 0 in
-// the low half (v_cvt_pk_bf16_f32).
-__device__ __forceinline__ uint32_t pk_bf(float a, float b) {
-    typedef __bf16 bf2_t __attribute__((ext_vector_type(2)));
-    typedef float f2_t __attribute__((ext_vector_type(2)));
-    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f2_t){a, b}, bf2_t));
-}
-__device__ __forceinline__ float bf_lo(uint32_t p) { return __uint_as_float(p << 16); }
-__device__ __forceinline__ float bf_hi(uint32_t p) { return __uint_as_float(p & 0xffff0000u); }
-
-// x = hi + lo with hi = bf16(x) and lo = bf16(x - hi): the pair carries about
-// 16 significant bits, so products of a bf16 weight with (hi, lo) keep the
-// activation's precision.  Two values per call, packed as above.
-__device__ __forceinline__ void split2(float a, float b, uint32_t& h, uint32_t& l) {
-    h = pk_bf(a, b);
-    l = pk_bf(a - bf_lo(h), b - bf_hi(h));
-}
-// x = hi + mid + lo, each bf16: about 24 significant bits, as many as fp32
-// holds (x - hi and mid's remainder are exact in fp32).  The forward pass feeds
-// its products with triples: with pairs, about one pre-activation in a million
-// rounds to the other side of zero at the benchmarked sizes, and each such
-// ReLU flip moves its sample's feature gradient by a whole term.
-__device__ __forceinline__ void split3(float a, float b, uint32_t& h, uint32_t& m, uint32_t& l) {
-    h = pk_bf(a, b);
-    const float ra = a - bf_lo(h), rb = b - bf_hi(h);
-    m = pk_bf(ra, rb);
-    l = pk_bf(ra - bf_lo(m), rb - bf_hi(m));
-}
 __device__ __forceinline__ void split_bf(float x, uint16_t& h, uint16_t& l) {
     uint32_t hh, ll;
     split2(x, 0.f, hh, ll);
@@ -297,7 +269,7 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
     int64_t c_slot = -1;
     float c_sc = 0.f;
     Raw c_raw[3];
-    float vdp[3] = {0.f, 0.f, 0.f};  // view direction of sample ps
+    uint2 pe_h = make_uint2(0u, 0u), pe_m = pe_h, pe_l = pe_h;  // sample ps's X columns 12 + 4pp ..
     auto lookup = [&](int b) {  // topology and view direction (after the records' barrier)
         const RecInfo ri = sm.ri[b][gs];
         c_slot = -1;
@@ -311,13 +283,15 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
             c_sc = (a * bb) * d;
             c_slot = slot_of(topo, cell.i0 + dx, cell.j0 + dy, cell.k0 + dz);
         }
+        // the encoding columns of sample ps from its ray's row (the counting
+        // march wrote them), zero for a sample that is not weight-kept
         const uint32_t pb = sm.ri[b][ps].bits;
-        vdp[0] = vdp[1] = vdp[2] = 0.f;
-        if (pb & kAliveW) {
-            const uint32_t ray = pb & kRayMask;
-            vdp[0] = P.viewdirs[3 * ray];
-            vdp[1] = P.viewdirs[3 * ray + 1];
-            vdp[2] = P.viewdirs[3 * ray + 2];
+        pe_h = pe_m = pe_l = make_uint2(0u, 0u);
+        if ((pb & kAliveW) && pp < 7) {
+            const uint2* row = reinterpret_cast<const uint2*>(R.pe + 96 * (int64_t)(pb & kRayMask));
+            pe_h = row[pp];
+            pe_m = row[8 + pp];
+            pe_l = row[16 + pp];
         }
     };
     auto fetch = [&]() {  // the colour row of the looked-up corner
@@ -422,31 +396,9 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
         }
         {
             // positional encoding (dvgo.py:354-358) and the bias column: thread
-            // (sample ps, part pp < 7) writes X columns 12 + 4pp .. +3; hardware
-            // sin/cos (error about 1e-6 for |arg| <= 8, below what the (hi, lo) pair
-            // keeps), as in the backward
-            const bool alive = (sm.ri[cb][ps].bits & kAliveW) != 0u;
-            float val[4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                const int p = pp * 4 + k;  // 0..27: PE 0..26, then the bias column
-                float v = 0.f;
-                if (alive) {
-                    if (p < 3) {
-                        v = vdp[p];
-                    } else if (p < 27) {
-                        const int idx = p - 3, axis = (idx % 12) / 4, fq = idx % 4;
-                        const float arg = vdp[axis] * (float)(1 << fq);
-                        v = idx < 12 ? __sinf(arg) : __cosf(arg);
-                    } else {
-                        v = 1.f;
-                    }
-                }
-                val[k] = v;
-            }
-            uint2 hi, mi, lo;
-            split3(val[0], val[1], hi.x, mi.x, lo.x);
-            split3(val[2], val[3], hi.y, mi.y, lo.y);
+            // (sample ps, part pp < 7) writes X columns 12 + 4pp .. +3, as (hi,
+            // mid, lo) bf16 from its ray's encoding row
+            const uint2 hi = pe_h, mi = pe_m, lo = pe_l;
             if (pp < 7) {
                 *reinterpret_cast<uint2*>(&sm.u.x.Xh[ps * XM + 12 + 4 * pp]) = hi;
                 *reinterpret_cast<uint2*>(&sm.u.x.Xm[ps * XM + 12 + 4 * pp]) = mi;
