@@ -242,8 +242,8 @@ def bench_batched(args, scene):
 # x1.0); profiles/pmc_traffic_c2_train_c5_render.md holds the runs.
 TRAFFIC_SOURCE = "profiles/pmc_traffic_c2_train_c5_render.md"
 MEASURED_TRAFFIC_MB = {
-    "c2": {"mlp_bwd": 63.72, "scatter": 85.97, "mlp_fwd": 69.6, "ray_march": 14.43,
-           "composite": 15.7, "adam": 122.0, "ray_count": 3.64},
+    "c2": {"mlp_bwd": 63.72, "scatter": 85.97, "mlp_fwd": 71.69, "ray_march": 14.44,
+           "composite": 15.7, "adam": 122.02, "ray_count": 5.97},
     "c5": {"render_mlp": 656.89},
 }
 
@@ -256,9 +256,10 @@ def stage_models(r):
     ua, uw, L_t, V_u = r["uniq_all"], r["uniq_w"], r["l_t"], r["v_u"]
     P = 22019
     return {
-        # rays in, counts and the kept steps' 2-byte indices out; the MLP
-        # parameters packed to bf16 and the MLP gradient zeroed by the same launch
-        "ray_count": ("hbm", N * (12 + 12 + 4) + S * 2
+        # rays in, counts, the kept steps' 2-byte indices and the rays' 168-byte
+        # encoding rows out; the MLP parameters packed to bf16 and the MLP
+        # gradient zeroed by the same launch
+        "ray_count": ("hbm", N * (12 + 12 + 4 + 12 + 168) + S * 2
                       + P * 4 + 2 * (128 * 64 + 2 * 128 * 128 + 2 * 16 * 128 + 128 * 32) + P * 4),
         # rays + offsets and the kept step indices in, 24-byte records out
         # (xyze, alpha + flags), distinct density voxels once
