@@ -175,25 +175,13 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_count(StepParams P, T
     const int r = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (r >= P.n_rays) return;
     const int lane = lane_id();
-    if (R.pe && lane < 28) {
-        // the MLP's input columns 12..39 for this ray's samples (dvgo.py:354-358):
-        // the view direction, sin then cos of it at 1, 2, 4, 8 (hardware sin/cos,
-        // error about 1e-6 for |arg| <= 8), and the bias column; one column per lane
-        const float* vd = P.viewdirs + 3 * (int64_t)r;
-        float v = 1.f;
-        if (lane < 3) {
-            v = vd[lane];
-        } else if (lane < 27) {
-            const int idx = lane - 3, axis = (idx % 12) / 4, fq = idx % 4;
-            const float arg = vd[axis] * (float)(1 << fq);
-            v = idx < 12 ? __sinf(arg) : __cosf(arg);
-        }
-        uint32_t h, m, l;
-        split3(v, 0.f, h, m, l);
-        uint16_t* row = R.pe + 96 * (int64_t)r;
-        row[lane] = (uint16_t)h;
-        row[32 + lane] = (uint16_t)m;
-        row[64 + lane] = (uint16_t)l;
+    // the view direction, loaded now and encoded after the march (its loads
+    // then add no latency to the wave)
+    float vd0 = 0.f, vd1 = 0.f, vd2 = 0.f;
+    if (R.pe) {
+        vd0 = P.viewdirs[3 * (int64_t)r];
+        vd1 = P.viewdirs[3 * (int64_t)r + 1];
+        vd2 = P.viewdirs[3 * (int64_t)r + 2];
     }
     const RaySetup s = ray_setup(P, r);
     int count = 0;
@@ -209,6 +197,25 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_count(StepParams P, T
         count += __popcll(km);
     });
     if (lane == 0) R.count[r] = count;
+    if (R.pe && lane < 28) {
+        // the MLP's input columns 12..39 for this ray's samples (dvgo.py:354-358):
+        // the view direction, sin then cos of it at 1, 2, 4, 8 (hardware sin/cos,
+        // error about 1e-6 for |arg| <= 8), and the bias column; one column per lane
+        float v = 1.f;
+        if (lane < 3) {
+            v = lane == 0 ? vd0 : (lane == 1 ? vd1 : vd2);
+        } else if (lane < 27) {
+            const int idx = lane - 3, axis = (idx % 12) / 4, fq = idx % 4;
+            const float arg = (axis == 0 ? vd0 : (axis == 1 ? vd1 : vd2)) * (float)(1 << fq);
+            v = idx < 12 ? __sinf(arg) : __cosf(arg);
+        }
+        uint32_t h, m, l;
+        split3(v, 0.f, h, m, l);
+        uint16_t* row = R.pe + 96 * (int64_t)r;
+        row[lane] = (uint16_t)h;
+        row[32 + lane] = (uint16_t)m;
+        row[64 + lane] = (uint16_t)l;
+    }
 }
 
 // Exclusive scan of the per-ray counts by one workgroup of 1024 threads, in
