@@ -109,6 +109,8 @@ struct RayBuffers {
     float* ray_rgb;    // [N][NC + 1] direct-colour configurations: rendered rgb (NC) + T_last
     float* loss;       // [1]
     uint16_t* steps;   // [N][kStepCache] the counting march's kept step indices, in step order
+    float4* setup;     // [N][2] the counting march's ray set-up for the forward march:
+                       // (o + d t_min, .), (d / |d|, .), the step count in the .w pair
     uint16_t* pe;      // [N][3][32] with the MLP: the ray's MLP input columns 12..39 (view
                        // direction, its sin/cos encoding, the bias column) as bf16 hi,
                        // mid and lo rows, written by the counting march; null without

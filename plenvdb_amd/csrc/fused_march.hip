@@ -184,6 +184,12 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_count(StepParams P, T
         vd2 = P.viewdirs[3 * (int64_t)r + 2];
     }
     const RaySetup s = ray_setup(P, r);
+    if (lane < 2) {  // the set-up, for the forward march (its divisions run once)
+        const uint64_t ns = (uint64_t)s.n_steps;
+        R.setup[2 * (int64_t)r + lane] =
+            lane == 0 ? make_float4(s.sx, s.sy, s.sz, __uint_as_float((uint32_t)ns))
+                      : make_float4(s.ux, s.uy, s.uz, __uint_as_float((uint32_t)(ns >> 32)));
+    }
     int count = 0;
     uint16_t* cache = R.steps + (int64_t)r * kStepCache;
     const bool cached = s.n_steps < 65536;  // uniform over the wave
@@ -293,7 +299,18 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_march(StepParams P, T
     if (r >= P.n_rays) return;  // uniform over the wave
     const int lane = lane_id();
     int64_t* queue = queue_all + 128 * (threadIdx.x >> 6);
-    const RaySetup s = ray_setup(P, r);
+    RaySetup s;  // the counting march's set-up (positions and step count only)
+    {
+        const float4 a = R.setup[2 * (int64_t)r], b = R.setup[2 * (int64_t)r + 1];
+        s.sx = a.x;
+        s.sy = a.y;
+        s.sz = a.z;
+        s.ux = b.x;
+        s.uy = b.y;
+        s.uz = b.z;
+        s.n_steps = (int64_t)(((uint64_t)__float_as_uint(b.w) << 32) | __float_as_uint(a.w));
+        s.t_min = s.t_max = 0.f;  // unused by the forward march
+    }
     const int64_t off = R.offset[r];
     const int cnt = R.count[r];  // the counting march's total bounds this ray's records
     float Tc = 1.f;  // transmittance carried from batch to batch (FP32, as the reference's T_cum)

@@ -51,7 +51,7 @@ inline bool aligned(const void* p, uintptr_t a) { return (reinterpret_cast<uintp
 
 struct Layout {
     size_t count, offset, total, t_last, ray_out, ray_grad, ray_ctot, ray_loss, ray_rgb, loss, steps,
-        pe;
+        pe, setup;
     size_t xyze, ab, rgb, adj, xrow, relu, grad;
     size_t touched, dw_part, w0t, w1t, w2t, w2, w1, w0;
     size_t bytes;
@@ -79,6 +79,7 @@ Layout layout_for(int64_t n_rays, int64_t capacity, int64_t nleaf) {
     L.loss = take(4);
     L.steps = take(2 * (size_t)kStepCache * n_rays);
     L.pe = take(2 * 96 * (size_t)n_rays);
+    L.setup = take(32 * (size_t)n_rays);
     L.xyze = take(16 * capacity);
     L.ab = take(8 * capacity);
     L.rgb = take(16 * capacity);
@@ -534,6 +535,7 @@ int pvdb_fused_step(const pvdb_topology* topo, const pvdb_step_desc* d, void* st
     R.loss = d->loss ? d->loss : reinterpret_cast<float*>(ws + L.loss);
     R.steps = reinterpret_cast<uint16_t*>(ws + L.steps);
     R.pe = direct ? nullptr : reinterpret_cast<uint16_t*>(ws + L.pe);
+    R.setup = reinterpret_cast<float4*>(ws + L.setup);
     Records rec;
     rec.xyze = reinterpret_cast<float4*>(ws + L.xyze);
     rec.ab = reinterpret_cast<float2*>(ws + L.ab);
