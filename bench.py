@@ -256,14 +256,14 @@ def stage_models(r):
     ua, uw, L_t, V_u = r["uniq_all"], r["uniq_w"], r["l_t"], r["v_u"]
     P = 22019
     return {
-        # rays in, counts, the kept steps' 2-byte indices and the rays' 168-byte
-        # encoding rows out; the MLP parameters packed to bf16 and the MLP
-        # gradient zeroed by the same launch
-        "ray_count": ("hbm", N * (12 + 12 + 4 + 12 + 168) + S * 2
+        # rays in, counts, the kept steps' 2-byte indices, the rays' 168-byte
+        # encoding rows and 32-byte set-ups out; the MLP parameters packed to
+        # bf16 and the MLP gradient zeroed by the same launch
+        "ray_count": ("hbm", N * (12 + 12 + 4 + 12 + 168 + 32) + S * 2
                       + P * 4 + 2 * (128 * 64 + 2 * 128 * 128 + 2 * 16 * 128 + 128 * 32) + P * 4),
-        # rays + offsets and the kept step indices in, 24-byte records out
-        # (xyze, alpha + flags), distinct density voxels once
-        "ray_march": ("hbm", N * (12 + 12 + 8) + S * (2 + 24) + ua * 4),
+        # ray set-ups + offsets and the kept step indices in, 24-byte records
+        # out (xyze, alpha + flags), distinct density voxels once
+        "ray_march": ("hbm", N * (32 + 8) + S * (2 + 24) + ua * 4),
         # 36,608 FLOP per MLP sample + 6,912 per ray (PE part of layer 0)
         "mlp_fwd": ("mfma", nw * MLP_FWD_FLOP + N * MLP_RAY_FLOP),
         # alpha + flags and exp(sigma) in, adjoints out per record; rgb in per
