@@ -225,33 +225,40 @@ __global__ void __launch_bounds__(kMaxRayWaves * 64) k_ray_count(StepParams P, T
 }
 
 // Exclusive scan of the per-ray counts by one workgroup of 1024 threads, in
-// chunks of 8192: each thread takes 8 consecutive counts (two 16-byte loads),
+// chunks of 1024 * kScanPer: each thread takes kScanPer consecutive counts
+// (16-byte loads, all issued before any is used),
 // wavefronts scan their thread sums with shuffles, one LDS pass combines the
 // 16 wavefronts, and a carry runs from chunk to chunk.  The sums run in 64 bits
 // and every stored offset (and the total) saturates at INT32_MAX: a batch with
 // more samples than that keeps offsets >= the record capacity (which the step
 // caps below INT32_MAX), so its excess records are dropped, never wrapped.
+constexpr int kScanPer = 16;  // counts per thread and chunk (65,536 rays in four chunks)
+
 __global__ void __launch_bounds__(1024) k_scan(RayBuffers R, int n) {
     __shared__ long long wsum[16];
     __shared__ long long carry_s;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const long long kSat = 0x7fffffffll;
     long long carry = 0;
-    for (int c0 = 0; c0 < n; c0 += 8192) {
-        const int b = c0 + 8 * tid;
-        int v[8];
-        if (b + 8 <= n) {  // 32-byte aligned (the workspace arrays are 256-byte aligned)
-            const int4 a0 = *reinterpret_cast<const int4*>(R.count + b);
-            const int4 a1 = *reinterpret_cast<const int4*>(R.count + b + 4);
-            v[0] = a0.x; v[1] = a0.y; v[2] = a0.z; v[3] = a0.w;
-            v[4] = a1.x; v[5] = a1.y; v[6] = a1.z; v[7] = a1.w;
+    for (int c0 = 0; c0 < n; c0 += 1024 * kScanPer) {
+        const int b = c0 + kScanPer * tid;
+        int v[kScanPer];
+        if (b + kScanPer <= n) {  // 64-byte aligned (the workspace arrays are 256-byte aligned)
+#pragma unroll
+            for (int k = 0; k < kScanPer / 4; ++k) {
+                const int4 a = *reinterpret_cast<const int4*>(R.count + b + 4 * k);
+                v[4 * k] = a.x;
+                v[4 * k + 1] = a.y;
+                v[4 * k + 2] = a.z;
+                v[4 * k + 3] = a.w;
+            }
         } else {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) v[k] = b + k < n ? R.count[b + k] : 0;
+            for (int k = 0; k < kScanPer; ++k) v[k] = b + k < n ? R.count[b + k] : 0;
         }
         long long sum = 0;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) sum += v[k];
+        for (int k = 0; k < kScanPer; ++k) sum += v[k];
         long long inc = sum;  // inclusive scan of the thread sums within the wavefront
 #pragma unroll
         for (int d = 1; d < 64; d <<= 1) {
@@ -264,18 +271,20 @@ __global__ void __launch_bounds__(1024) k_scan(RayBuffers R, int n) {
         for (int w = 0; w < wv; ++w) before += wsum[w];
         if (tid == 1023) carry_s = before + inc;
         long long run = before + inc - sum;
-        int off[8];
+        int off[kScanPer];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) {
+        for (int k = 0; k < kScanPer; ++k) {
             off[k] = (int)(run < kSat ? run : kSat);
             run += v[k];
         }
-        if (b + 8 <= n) {
-            *reinterpret_cast<int4*>(R.offset + b) = make_int4(off[0], off[1], off[2], off[3]);
-            *reinterpret_cast<int4*>(R.offset + b + 4) = make_int4(off[4], off[5], off[6], off[7]);
+        if (b + kScanPer <= n) {
+#pragma unroll
+            for (int k = 0; k < kScanPer / 4; ++k)
+                *reinterpret_cast<int4*>(R.offset + b + 4 * k) =
+                    make_int4(off[4 * k], off[4 * k + 1], off[4 * k + 2], off[4 * k + 3]);
         } else {
 #pragma unroll
-            for (int k = 0; k < 8; ++k)
+            for (int k = 0; k < kScanPer; ++k)
                 if (b + k < n) R.offset[b + k] = off[k];
         }
         __syncthreads();

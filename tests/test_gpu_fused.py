@@ -132,10 +132,11 @@ def test_fused_step_applies_adam(oracle_mod):
     assert np.isfinite(float(tr.loss.item()))
 
 
-@pytest.mark.parametrize("n_rays", [8192, 20001])
+@pytest.mark.parametrize("n_rays", [8192, 20001, 40000])
 def test_fused_record_offsets_are_the_exclusive_scan(n_rays):
     """Per-ray record offsets are the exclusive prefix sum of the per-ray counts
-    (integer work: exact), across the scan's 8192-ray chunks and a ragged tail."""
+    (integer work: exact), within one of the scan's 16,384-ray chunks, across
+    two and three of them, and with ragged tails."""
     from plenvdb_amd import scenes
     from plenvdb_amd.fused import FusedTrainer
 
