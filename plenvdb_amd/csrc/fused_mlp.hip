@@ -163,6 +163,14 @@ __device__ __forceinline__ void st_act3(uint16_t* sh, uint16_t* sm_, uint16_t* s
 // samples average that rounding (a device check keeps every MLP gradient
 // within 6e-4 of FP64), so the gradients' lo parts stay sample-major.
 
+// x where bit k of m is set, else +0: the bit sign-extended by one v_bfe_i32
+// and ANDed with x's bits (two instructions; a compare and a select take
+// three, with the bit extraction)
+__device__ __forceinline__ float keep_if(uint32_t m, int k, float x) {
+    const uint32_t all = (uint32_t)__builtin_amdgcn_sbfe((int)m, k, 1);
+    return __uint_as_float(__float_as_uint(x) & all);
+}
+
 // ---- forward -----------------------------------------------------------------
 // A workgroup of 4 wavefronts takes a group of 32 records; wave w owns hidden
 // features [32w, 32w + 32) of both hidden layers (up to three workgroups per
@@ -810,7 +818,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
                 float gv[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    gv[i] = (m1 >> (nt * 4 + i)) & 1u ? c4[i] : 0.f;
+                    gv[i] = keep_if(m1, nt * 4 + i, c4[i]);
                     db1[i] += gv[i];
                 }
                 st_act<true>(sm.G1h, sm.G1l, sm.G1f, 16 * nt + r16, fr, lane, gv);
@@ -820,7 +828,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
         for (int nt = 0; nt < NT; ++nt) {
             float h[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) h[i] = (m0 >> (nt * 4 + i)) & 1u ? acc[nt][i] : 0.f;
+            for (int i = 0; i < 4; ++i) h[i] = keep_if(m0, nt * 4 + i, acc[nt][i]);
             st_hi_fm(sm.A1h, sm.H0f, 16 * nt + r16, fr, lane, h);
         }
         if (pbase >= 0) finish(cb ^ 1, pbase);
@@ -861,7 +869,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
                 float gv[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    gv[i] = (mw >> (16 * mt + 4 * g + i)) & 1u ? acc2[mt][i] : 0.f;
+                    gv[i] = keep_if(mw, 16 * mt + 4 * g + i, acc2[mt][i]);
                 st_act<true>(sm.A2h, sm.A2l, sm.F3, 16 * sh + r16, 32 * fb + 16 * mt + 4 * g, lane,
                              gv);
             }
@@ -880,7 +888,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
         for (int nt = 0; nt < NT; ++nt) {
             float h[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) h[i] = (m1 >> (nt * 4 + i)) & 1u ? acc[nt][i] : 0.f;
+            for (int i = 0; i < 4; ++i) h[i] = keep_if(m1, nt * 4 + i, acc[nt][i]);
             st_fm(sm.F2, 16 * nt + r16, fr, lane, h);
         }
         park(cb ^ 1);  // the next group's records, loaded at this group's start
