@@ -69,9 +69,11 @@ __device__ __forceinline__ void split_bf(float x, uint16_t& h, uint16_t& l) {
     l = (uint16_t)ll;
 }
 
+// Quad permutation, all lanes active: every lane reads a valid source, so the
+// move needs no old value (update_dpp's would be materialised with a v_mov).
 template <int CTRL>
-__device__ __forceinline__ uint32_t dpp(uint32_t v) {  // quad permutation, all lanes active
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
 }
 
 // A lane holds four consecutive features (rows) of one sample; the four lanes of
