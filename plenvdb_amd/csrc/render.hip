@@ -164,10 +164,20 @@ struct RenderArgs {
     float near_, stepdist, act_shift, interval, thres, bg, t_stop;
 };
 
+// Two floats -> two fp16 (round to nearest even) in one dword, a in the low
+// half: one v_cvt_pk_f16_f32.
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ uint32_t pack2(float a, float b) {
-    const _Float16 ha = (_Float16)a, hb = (_Float16)b;
-    return (uint32_t)__builtin_bit_cast(uint16_t, ha) |
-           ((uint32_t)__builtin_bit_cast(uint16_t, hb) << 16);
+    typedef float f2_t __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((f2_t){a, b}, half2_t));
+}
+// relu of two floats as packed fp16: converted first, then one v_pk_max_f16
+// (rounding to nearest is monotonic and keeps 0, so this equals converting
+// the relu'd values, up to the sign of a zero)
+__device__ __forceinline__ uint32_t relu_pack2(float a, float b) {
+    const half2_t z = {(_Float16)0.f, (_Float16)0.f};
+    const half2_t h = __builtin_bit_cast(half2_t, pack2(a, b));
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(h, z));
 }
 
 __device__ __forceinline__ float h2f(uint16_t h) {
@@ -236,8 +246,8 @@ __device__ __forceinline__ void wave_mlp(const uint4* __restrict__ sW, const flo
                 acc = mfma16x16x32h(a0, xb[u][0], acc);
                 acc = mfma16x16x32h(a1, xb[u][1], acc);
                 const int s = m >> 1, d = (m & 1) * 2;
-                h0[u][s][d] = pack2(fmaxf(acc[0], 0.f), fmaxf(acc[1], 0.f));
-                h0[u][s][d + 1] = pack2(fmaxf(acc[2], 0.f), fmaxf(acc[3], 0.f));
+                h0[u][s][d] = relu_pack2(acc[0], acc[1]);
+                h0[u][s][d + 1] = relu_pack2(acc[2], acc[3]);
             }
         }
     }
@@ -253,10 +263,11 @@ __device__ __forceinline__ void wave_mlp(const uint4* __restrict__ sW, const flo
 #pragma unroll
         for (int mm = 0; mm < 2; ++mm) {
             const int m = 2 * s2 + mm;
+            // the accumulators start at the bias (the MFMA adds it)
             const float4 bias = *reinterpret_cast<const float4*>(sB + 16 * m + 4 * g);
             f32x4 acc[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) acc[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int u = 0; u < 4; ++u) acc[u] = f32x4{bias.x, bias.y, bias.z, bias.w};
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
                 const f16x8 a = as_h8(W1[(m * 4 + s) * 64 + lane]);
@@ -268,9 +279,8 @@ __device__ __forceinline__ void wave_mlp(const uint4* __restrict__ sW, const flo
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                hk[u][2 * mm] = pack2(fmaxf(acc[u][0] + bias.x, 0.f), fmaxf(acc[u][1] + bias.y, 0.f));
-                hk[u][2 * mm + 1] =
-                    pack2(fmaxf(acc[u][2] + bias.z, 0.f), fmaxf(acc[u][3] + bias.w, 0.f));
+                hk[u][2 * mm] = relu_pack2(acc[u][0], acc[u][1]);
+                hk[u][2 * mm + 1] = relu_pack2(acc[u][2], acc[u][3]);
             }
         }
         const f16x8 a2 = as_h8(W2[s2 * 64 + lane]);
