@@ -244,7 +244,7 @@ TRAFFIC_SOURCE = "profiles/pmc_traffic_c2_train_c5_render.md"
 MEASURED_TRAFFIC_MB = {
     "c2": {"mlp_bwd": 63.71, "scatter": 85.97, "mlp_fwd": 71.7, "ray_march": 13.9,
            "composite": 15.69, "adam": 122.02, "ray_count": 6.23},
-    "c5": {"render_mlp": 656.89},
+    "c5": {"render_mlp": 656.47},
 }
 
 
