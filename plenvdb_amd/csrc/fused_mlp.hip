@@ -378,12 +378,17 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
                     v[4 * k + 3] = c_raw[k].w * c_sc;
                 }
             }
+            // butterfly over the record's eight lanes: the exchanges with lanes
+            // l^1 and l^2 as quad permutes on the DPP path (folded into the
+            // adds), the one with l^4 as a shuffle
 #pragma unroll
-            for (int c = 0; c < 12; ++c) {
-                v[c] += __shfl_xor(v[c], 1, 64);
-                v[c] += __shfl_xor(v[c], 2, 64);
-                v[c] += __shfl_xor(v[c], 4, 64);
-            }
+            for (int c = 0; c < 12; ++c)
+                v[c] += __uint_as_float(dpp<0xB1>(__float_as_uint(v[c])));  // quad_perm [1,0,3,2]
+#pragma unroll
+            for (int c = 0; c < 12; ++c)
+                v[c] += __uint_as_float(dpp<0x4E>(__float_as_uint(v[c])));  // quad_perm [2,3,0,1]
+#pragma unroll
+            for (int c = 0; c < 12; ++c) v[c] += __shfl_xor(v[c], 4, 64);
             // the butterfly leaves the same twelve sums (bit for bit: each step
             // adds the same two operands on every lane) in all eight lanes of
             // the record, so lane gpart < 6 splits and stores channel pair gpart
