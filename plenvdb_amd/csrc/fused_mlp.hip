@@ -69,8 +69,9 @@ __device__ __forceinline__ void split_bf(float x, uint16_t& h, uint16_t& l) {
     l = (uint16_t)ll;
 }
 
-// Quad permutation, all lanes active: every lane reads a valid source, so the
-// move needs no old value (update_dpp's would be materialised with a v_mov).
+// DPP permutation within rows (quad permutes, the half-row mirror), all lanes
+// active: every lane reads a valid source, so the move needs no old value
+// (update_dpp's would be materialised with a v_mov).
 template <int CTRL>
 __device__ __forceinline__ uint32_t dpp(uint32_t v) {
     return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
@@ -378,9 +379,10 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
                     v[4 * k + 3] = c_raw[k].w * c_sc;
                 }
             }
-            // butterfly over the record's eight lanes: the exchanges with lanes
-            // l^1 and l^2 as quad permutes on the DPP path (folded into the
-            // adds), the one with l^4 as a shuffle
+            // butterfly over the record's eight lanes on the DPP path (folded
+            // into the adds): lanes l^1 and l^2 by quad permutes, then the
+            // other quad of the eight by the half-row mirror (lane j of the
+            // eight reads lane 7 - j, which holds the other quad's sum)
 #pragma unroll
             for (int c = 0; c < 12; ++c)
                 v[c] += __uint_as_float(dpp<0xB1>(__float_as_uint(v[c])));  // quad_perm [1,0,3,2]
@@ -388,7 +390,8 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
             for (int c = 0; c < 12; ++c)
                 v[c] += __uint_as_float(dpp<0x4E>(__float_as_uint(v[c])));  // quad_perm [2,3,0,1]
 #pragma unroll
-            for (int c = 0; c < 12; ++c) v[c] += __shfl_xor(v[c], 4, 64);
+            for (int c = 0; c < 12; ++c)
+                v[c] += __uint_as_float(dpp<0x141>(__float_as_uint(v[c])));  // row_half_mirror
             // the butterfly leaves the same twelve sums (bit for bit: each step
             // adds the same two operands on every lane) in all eight lanes of
             // the record, so lane gpart < 6 splits and stores channel pair gpart
