@@ -91,7 +91,8 @@ struct Records {
                        // hi parts: the 12 gathered colour features, the 27 view-direction
                        // encoding inputs and the bias input (a zero row unless weight-kept)
     uint32_t* relu;    // [8 per record] forward ReLU masks: word 4l + j = layer l's
-                       // hidden features 32j..32j+31, bit f%32
+                       // hidden features 32j + 16m + 4g + i (m, i < 2, 4; g < 4) at
+                       // bit 8g + 4m + i
     float4* grad;      // [4 per record] dL/dfeature (12) and dL/dsigma; in the direct-colour
                        // configurations rgb (NC) between the composite's two passes, then
                        // dL/dk0 (NC) and dL/dsigma at float kSigmaAt<NC>

@@ -132,14 +132,12 @@ __device__ __forceinline__ void st_fm(uint16_t* fm, int col, int fr, int lane, c
     *reinterpret_cast<uint2*>(&fm[(fr + q) * FM + (col & ~3)]) = quad_transpose(hi, q);
 }
 
-// A wave's 32-bit slice of one sample's ReLU mask from the D fragments of its
-// two 16-feature tiles: lane (g, n) holds bit i of nib[mt] for feature
-// 16 mt + 4g + i; the four lanes of column n are OR-ed together.
-__device__ __forceinline__ uint32_t relu_word(uint32_t nib0, uint32_t nib1, int g) {
-    uint32_t w = (nib0 << (4 * g)) | (nib1 << (16 + 4 * g));
-    w |= (uint32_t)__shfl_xor((int)w, 16, 64);
-    w |= (uint32_t)__shfl_xor((int)w, 32, 64);
-    return w;
+// A wave's 32-bit slice of one sample's ReLU mask is written a byte per lane:
+// lane (g, n) holds bit i of nib[mt] for feature 16 mt + 4g + i and stores
+// byte g of sample n's word, nib[0] | nib[1] << 4, so the bit of feature
+// 16 mt + 4g + i is bit 8g + 4mt + i (no exchange between the lanes).
+__device__ __forceinline__ uint8_t relu_byte(uint32_t nib0, uint32_t nib1) {
+    return (uint8_t)(nib0 | (nib1 << 4));
 }
 
 // a (bf16 weight) times a split activation
@@ -486,9 +484,9 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
             }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
-            const uint32_t w = relu_word(nib[0][nt], nib[1][nt], g);
             const int64_t q = base + 16 * nt + r16;
-            if (g == 0 && q < total) rec.relu[8 * q + wv] = w;
+            if (q < total)
+                reinterpret_cast<uint8_t*>(rec.relu + 8 * q + wv)[g] = relu_byte(nib[0][nt], nib[1][nt]);
         }
         lookup(cb ^ 1);
         __syncthreads();
@@ -535,9 +533,10 @@ __global__ void __launch_bounds__(256, 3) k_mlp_fwd(StepParams P, TopoView topo,
             }
 #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
-            const uint32_t w = relu_word(nib[0][nt], nib[1][nt], g);
             const int64_t q = base + 16 * nt + r16;
-            if (g == 0 && q < total) rec.relu[8 * q + 4 + wv] = w;
+            if (q < total)
+                reinterpret_cast<uint8_t*>(rec.relu + 8 * q + 4 + wv)[g] =
+                    relu_byte(nib[0][nt], nib[1][nt]);
         }
         // partial logits over this wave's hidden features f0 .. f0 + 31 (one
         // k-step of L^T = W2^T H1^T), read back from the H1 this wave just wrote
@@ -798,7 +797,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
         //         the mask -> G1h, G1l, G1f.  The previous group's stage C ----
         uint32_t m0 = 0u, m1 = 0u;  // the forward's ReLU masks: bit nt * 4 + i
         {
-            const int sh = ((wv & 1) << 4) + 4 * g;  // features fr..fr+3 in word wv/2
+            const int sh = 8 * g + 4 * (wv & 1);  // features fr..fr+3 in word wv/2
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
                 const uint32_t* mw = &sm.relu[cb][8 * (16 * nt + r16)];
@@ -879,7 +878,7 @@ __global__ void __launch_bounds__(64 * kBwdWaves) k_mlp_bwd(StepParams P, MlpPac
                 float gv[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
-                    gv[i] = keep_if(mw, 16 * mt + 4 * g + i, acc2[mt][i]);
+                    gv[i] = keep_if(mw, 8 * g + 4 * mt + i, acc2[mt][i]);
                 st_act<true>(sm.A2h, sm.A2l, sm.F3, 16 * sh + r16, 32 * fb + 16 * mt + 4 * g, lane,
                              gv);
             }
